@@ -1,0 +1,311 @@
+"""TEST INFRASTRUCTURE ONLY: CPU restatement of QuAcK's restricted G0W0/evGW.
+
+This module is the checker for the HIP path.  Only tests/, the smoke check
+and the CPU-baseline leg of bench.py import it; the product library never
+does.  Every function follows one QuAcK routine call-for-call (file:line in
+each docstring; paths relative to QuAcK's src/), in FP64, with QuAcK's index
+conventions:
+
+* ERI arrays are the flat physicist column-major arrays QuAcK holds:
+  ``G(p,q,r,s) = <pq|rs> = (pr|qs)`` at ``p + n q + n^2 r + n^3 s``;
+  here they are viewed as ``G = eri.reshape((n,)*4, order='F')``.
+* particle-hole index ``ia = (a-nO) + (i-nC)*nn`` (0-based), a fastest
+  (LR/phRLR_A.f90:151).
+* ``XpY[m, ia]``: row = RPA mode, column = ph pair (LR/phRLR.f90:77-78).
+
+dsyev('V','U') of utils/wrap_lapack.f90:99-136 is called through SciPy's
+LAPACK binding with the same driver and triangle; dsysvx of
+wrap_lapack.f90:560-598 likewise.
+
+Pinning: tests/test_oracle.py checks each function against the compiled
+QuAcK routines (oracle/_ref), the FeatherBench / Rtest_ref.dat goldens
+(He/6-31G, H2O/cc-pVDZ) and closed forms (nS = 1 He, RPA supermatrix
+eigenvalues, the (X+Y)^T(X-Y) = 1 normalisation, brute-force einsums).
+"""
+import numpy as np
+from scipy.linalg import lapack
+
+FLOW = 500.0          # SRG flow parameter, GW/RG0W0.f90:98, GW/evRGW.f90:88
+QP_MAXIT = 64         # GW/RGW_QP_graph.f90:31
+QP_THRESH = 1e-6      # GW/RGW_QP_graph.f90:32
+HA_TO_EV = 27.211386245988  # include/parameters.h:16
+
+
+def G4(eri, n):
+    return np.asarray(eri).reshape((n, n, n, n), order="F")
+
+
+def diagonalize_matrix(A):
+    """utils/wrap_lapack.f90:99-136: dsyev('V','U'), ascending eigenvalues."""
+    w, v, info = lapack.dsyev(np.asfortranarray(A), compute_v=1, lower=0)
+    if info != 0:
+        raise RuntimeError("dsyev info=%d" % info)
+    return w, v
+
+
+def aotomo_eri_rhf(c, eri_ao):
+    """AOtoMO/AOtoMO_ERI_RHF.f90:31-51: four dgemm('T','N') quarter transforms,
+    each contracting the leading AO index and appending the MO index."""
+    nBas, nOrb = c.shape
+    a = np.asarray(eri_ao).reshape(nBas, -1, order="F")         # (i1, i2 i3 i4)
+    a = (a.T @ c)                                               # (i2 i3 i4, p)
+    a = a.reshape(nBas, -1, order="F").T @ c                    # (i3 i4 p, q)
+    a = a.reshape(nBas, -1, order="F").T @ c                    # (i4 p q, r)
+    a = a.reshape(nBas, -1, order="F").T @ c                    # (p q r, s)
+    return a.reshape(-1, order="F")
+
+
+def ph_dims(nOrb, nO, nC=0, nR=0):
+    nV = nOrb - nO
+    return nC, nO, nV, nR, (nO - nC) * (nV - nR)
+
+
+def phRLR_A(e, eri, nO, lam=1.0, dRPA=True, nC=0, nR=0):
+    """LR/phRLR_A.f90:140-186 (singlet): A(ia,jb) = 2 lam <bi|ja> - (1-dRPA) lam <bj|ai>
+    + delta_ij delta_ab (e_a - e_i)."""
+    n = len(e)
+    G = G4(eri, n)
+    occ, vir = slice(nC, nO), slice(nO, n - nR)
+    nS = (nO - nC) * (n - nR - nO)
+    A = 2.0 * lam * G[vir, occ, occ, vir].transpose(1, 3, 2, 0).reshape(nS, nS)
+    if not dRPA:
+        A -= lam * G[vir, occ, vir, occ].transpose(3, 2, 1, 0).reshape(nS, nS)
+    dE = (e[None, vir] - e[occ, None]).reshape(-1)
+    A[np.diag_indices(nS)] += dE
+    return A
+
+
+def phRLR_B(eri, n, nO, lam=1.0, dRPA=True, nC=0, nR=0):
+    """LR/phRLR_B.f90:271-312 (singlet): B(ia,jb) = 2 lam <bi|ja> - (1-dRPA) lam <bj|ia>."""
+    G = G4(eri, n)
+    occ, vir = slice(nC, nO), slice(nO, n - nR)
+    nS = (nO - nC) * (n - nR - nO)
+    B = 2.0 * lam * G[vir, occ, occ, vir].transpose(1, 3, 2, 0).reshape(nS, nS)
+    if not dRPA:
+        B -= lam * G[vir, occ, occ, vir].transpose(2, 3, 1, 0).reshape(nS, nS)
+    return B
+
+
+def ADAt(A, D):
+    """utils/utils.f90:634-678: B = A diag(D) A^T."""
+    return A @ (D[:, None] * A.T)
+
+
+def phRLR(TDA, A, B):
+    """LR/phRLR.f90:36-95: returns EcRPA, Om, XpY, XmY."""
+    nS = A.shape[0]
+    if TDA:
+        Om, V = diagonalize_matrix(A)
+        XpY = V.T.copy()
+        XmY = XpY.copy()
+    else:
+        ApB = A + B
+        AmB = A - B
+        Om, U = diagonalize_matrix(AmB)
+        AmBSq = ADAt(U, np.sqrt(Om))
+        AmBIv = ADAt(U, 1.0 / np.sqrt(Om))
+        tmp = ApB @ AmBSq
+        Z = AmBSq @ tmp
+        Om, Z = diagonalize_matrix(Z)
+        Om = np.sqrt(np.abs(Om))
+        XpY = (Z.T @ AmBSq) * (1.0 / np.sqrt(Om))[:, None]
+        XmY = (Z.T @ AmBIv) * np.sqrt(Om)[:, None]
+    EcRPA = 0.5 * (np.sum(Om) - np.trace(A))
+    return EcRPA, Om, XpY, XmY
+
+
+def excitation_density(eri, XpY, n, nO, nC=0, nR=0):
+    """GW/RGW_excitation_density.f90:26-50: tmp(p,q,jb) = <pj|qb>;
+    rho(p,q,m) = sum_jb tmp(p,q,jb) XpY(m,jb).  Returned as rho[p, q, m]."""
+    G = G4(eri, n)
+    tmp = G[:, nC:nO, :, nO:n - nR].transpose(0, 2, 1, 3)           # [p, q, j, b]
+    nS = XpY.shape[0]
+    tmp = tmp.reshape(n, n, nS)                                      # jb = j*nn + b
+    return np.einsum("pqk,mk->pqm", tmp, XpY, optimize=True)
+
+
+def self_energy_diag(e, Om, rho, nO, eta, nC=0, nR=0):
+    """GW/RGW_self_energy_diag.f90:115-191: SigC(p), Z(p) = 1/(1-dSig), EcGM."""
+    n = len(e)
+    occ, vir = slice(nC, nO), slice(nO, n - nR)
+    act = slice(nC, n - nR)
+    Sig = np.zeros(n)
+    dS = np.zeros(n)
+    r2 = rho ** 2
+    eps = e[act, None, None] - e[None, occ, None] + Om[None, None, :]
+    num = 2.0 * r2[act, occ, :]
+    Sig[act] += np.sum(num * eps / (eps ** 2 + eta ** 2), axis=(1, 2))
+    dS[act] -= np.sum(num * (eps ** 2 - eta ** 2) / (eps ** 2 + eta ** 2) ** 2, axis=(1, 2))
+    eps = e[act, None, None] - e[None, vir, None] - Om[None, None, :]
+    num = 2.0 * r2[act, vir, :]
+    Sig[act] += np.sum(num * eps / (eps ** 2 + eta ** 2), axis=(1, 2))
+    dS[act] -= np.sum(num * (eps ** 2 - eta ** 2) / (eps ** 2 + eta ** 2) ** 2, axis=(1, 2))
+    eps = e[vir, None, None] - e[None, occ, None] + Om[None, None, :]
+    num = 4.0 * r2[vir, occ, :]
+    EcGM = -np.sum(num * eps / (eps ** 2 + eta ** 2))
+    Z = 1.0 / (1.0 - dS)
+    return Sig, Z, EcGM
+
+
+def srg_self_energy_diag(e, Om, rho, nO, flow=FLOW, nC=0, nR=0):
+    """GW/RGW_SRG_self_energy_diag.f90:64-160."""
+    n = len(e)
+    occ, vir = slice(nC, nO), slice(nO, n - nR)
+    act = slice(nC, n - nR)
+    s = flow
+    Sig = np.zeros(n)
+    dS = np.zeros(n)
+    r2 = rho ** 2
+    for blk, sgn in ((occ, 1.0), (vir, -1.0)):
+        D = e[act, None, None] - e[None, blk, None] + sgn * Om[None, None, :]
+        reg = 1.0 - np.exp(-2.0 * s * D * D)
+        Sig[act] += np.sum(2.0 * r2[act, blk, :] * reg / D, axis=(1, 2))
+        dS[act] -= np.sum(2.0 * r2[act, blk, :] * reg / D ** 2, axis=(1, 2))
+    D = e[vir, None, None] - e[None, occ, None] + Om[None, None, :]
+    EcGM = -np.sum(4.0 * r2[vir, occ, :] * (1.0 - np.exp(-2.0 * s * D * D)) / D)
+    return Sig, 1.0 / (1.0 - dS), EcGM
+
+
+def re_sigc_dsigc(p, w, e, Om, rho, nO, eta, srg=False, flow=FLOW, nC=0, nR=0):
+    """GW/RGW_Re_SigC.f90:127-147 and GW/RGW_Re_dSigC.f90:179-199 (SRG:
+    GW/RGW_SRG_Re_SigC.f90, GW/RGW_SRG_Re_dSigC.f90) for one orbital p."""
+    n = len(e)
+    occ, vir = slice(nC, nO), slice(nO, n - nR)
+    sig = 0.0
+    dsig = 0.0
+    for blk, sgn in ((occ, 1.0), (vir, -1.0)):
+        eps = w - e[blk, None] + sgn * Om[None, :]
+        num = 2.0 * rho[p, blk, :] ** 2
+        if srg:
+            reg = 1.0 - np.exp(-2.0 * flow * eps * eps)
+            sig += np.sum(num * reg / eps)
+            dsig -= np.sum(num * reg / eps ** 2)
+        else:
+            sig += np.sum(num * eps / (eps ** 2 + eta ** 2))
+            dsig -= np.sum(num * (eps ** 2 - eta ** 2) / (eps ** 2 + eta ** 2) ** 2)
+    return sig, dsig
+
+
+def qp_graph(eHF, Om, rho, eGWlin, eOld, Z, nO, eta, srg=False, nC=0, nR=0):
+    """GW/RGW_QP_graph.f90:50-92: per-orbital Newton on w - eHF(p) - Sig(w) = 0.
+    Start at eGWlin(p); Sig built from eOld; |f| > 1e-6 tested before each
+    update; at nIt == 64 eGW(p) = eGWlin(p) and Z(p) keeps its input value,
+    otherwise Z(p) = df of the last step.  Returns eGW, Z, iterations."""
+    n = len(eHF)
+    eGW = np.zeros(n)
+    Z = np.array(Z, dtype=np.float64)
+    its = np.zeros(n, dtype=np.int64)
+    for p in range(nC, n - nR):
+        w = eGWlin[p]
+        nIt = 0
+        f = 1.0
+        df = 0.0
+        while abs(f) > QP_THRESH and nIt < QP_MAXIT:
+            nIt += 1
+            sig, dsig = re_sigc_dsigc(p, w, eOld, Om, rho, nO, eta, srg, nC=nC, nR=nR)
+            f = w - eHF[p] - sig
+            df = 1.0 / (1.0 - dsig)
+            w = w - df * f
+        its[p] = nIt
+        if nIt == QP_MAXIT:
+            eGW[p] = eGWlin[p]
+        else:
+            eGW[p] = w
+            Z[p] = df
+    return eGW, Z, its
+
+
+def linear_solve(A, b):
+    """utils/wrap_lapack.f90:560-598: dsysvx('N','U'); returns x, rcond."""
+    res = lapack.dsysvx(np.asfortranarray(A), b.reshape(-1, 1), lower=0)
+    # SciPy returns (a_s, udut, ipiv, b_s, x, rcond, ferr, berr, info)
+    x, rcond = res[4], res[5]
+    return x.reshape(-1), float(rcond)
+
+
+class DIIS:
+    """utils/DIIS_extrapolation.f90:1-61 with utils/utils.f90:506-536 (prepend)."""
+
+    def __init__(self, n, max_diis):
+        self.err = np.zeros((n, max_diis))
+        self.e = np.zeros((n, max_diis))
+        self.n_diis = 0
+        self.max_diis = max_diis
+
+    def __call__(self, err_in, e_inout):
+        self.n_diis = min(self.n_diis + 1, self.max_diis)
+        m = self.n_diis
+        self.err[:, 1:m] = self.err[:, 0:m - 1].copy()
+        self.err[:, 0] = err_in
+        self.e[:, 1:m] = self.e[:, 0:m - 1].copy()
+        self.e[:, 0] = e_inout
+        A = np.zeros((m + 1, m + 1))
+        A[:m, :m] = self.err[:, :m].T @ self.err[:, :m]
+        A[:m, m] = -1.0
+        A[m, :m] = -1.0
+        b = np.zeros(m + 1)
+        b[m] = -1.0
+        w, rcond = linear_solve(A, b)
+        if rcond > 1e-13:
+            e_inout = self.e[:, :m] @ w[:m]
+        return e_inout, rcond
+
+
+def screening(e, eri, nO, TDA_W=False):
+    n = len(e)
+    A = phRLR_A(e, eri, nO)
+    B = np.zeros_like(A) if TDA_W else phRLR_B(eri, n, nO)
+    return phRLR(TDA_W, A, B)
+
+
+def rg0w0(eri_mo, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False):
+    """GW/RG0W0.f90:88-233 (linDM, BSE, ACFDT and ppBSE branches off)."""
+    n = len(eHF)
+    EcRPA, Om, XpY, XmY = screening(eHF, eri_mo, nO, TDA_W)
+    rho = excitation_density(eri_mo, XpY, n, nO)
+    if doSRG:
+        SigC, Z, EcGM = srg_self_energy_diag(eHF, Om, rho, nO)
+    else:
+        SigC, Z, EcGM = self_energy_diag(eHF, Om, rho, nO, eta)
+    eGWlin = eHF + Z * SigC
+    its = None
+    if linearize:
+        eGW = eGWlin.copy()
+    else:
+        eGW, Z, its = qp_graph(eHF, Om, rho, eGWlin, eHF, Z, nO, eta, doSRG)
+    EcRPA2, _, _, _ = screening(eGW, eri_mo, nO, TDA_W)
+    return {"eGW": eGW, "SigC": SigC, "Z": Z, "Om": Om, "EcRPA": EcRPA2, "EcGM": EcGM,
+            "eGWlin": eGWlin, "rho": rho, "XpY": XpY, "XmY": XmY, "EcRPA0": EcRPA, "qp_iters": its}
+
+
+def evrgw(eri_mo, eHF, nO, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, linearize=False, TDA_W=False,
+          doSRG=False):
+    """GW/evRGW.f90:104-200.  status 1 where QuAcK prints 'Convergence failed'."""
+    n = len(eHF)
+    eGW = eHF.copy()
+    eOld = eGW.copy()
+    Z = np.ones(n)
+    diis = DIIS(n, max(max_diis, 1))
+    nSCF = 0
+    Conv = 1.0
+    hist = []
+    while Conv > thresh and nSCF <= maxSCF:
+        EcRPA, Om, XpY, XmY = screening(eGW, eri_mo, nO, TDA_W)
+        rho = excitation_density(eri_mo, XpY, n, nO)
+        if doSRG:
+            SigC, Z, EcGM = srg_self_energy_diag(eGW, Om, rho, nO)
+        else:
+            SigC, Z, EcGM = self_energy_diag(eGW, Om, rho, nO, eta)
+        if linearize:
+            eGW = eHF + SigC
+        else:
+            eGW, Z, _ = qp_graph(eHF, Om, rho, eOld, eOld, Z, nO, eta, doSRG)
+        Conv = np.max(np.abs(eGW - eOld))
+        if max_diis > 1:
+            eGW, _ = diis(eGW - eOld, eGW)
+        eOld = eGW.copy()
+        hist.append(eGW.copy())
+        nSCF += 1
+    status = 1 if nSCF == maxSCF + 1 else 0
+    return {"eGW": eGW, "SigC": SigC, "Z": Z, "Om": Om, "EcRPA": EcRPA, "EcGM": EcGM, "nSCF": nSCF,
+            "status": status, "Conv": Conv, "history": hist}

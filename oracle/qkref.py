@@ -1,0 +1,128 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes view of oracle/_ref/libqkref.so.
+
+libqkref.so is QuAcK's own restricted G0W0/evGW Fortran (GW/, LR/, utils/,
+AOtoMO/ sources compiled unchanged by oracle/Makefile) plus the bind(C)
+entry points of oracle/ref_driver.f90.  Only tests, the smoke check and the
+CPU-baseline leg of bench.py use it, and only as the checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", "libqkref.so")
+_lib = None
+
+_I = ctypes.c_int
+_D = ctypes.c_double
+_P = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+
+
+def available():
+    return os.path.exists(LIB)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise RuntimeError("oracle/_ref/libqkref.so is not built (make -C oracle)")
+        lib = ctypes.CDLL(LIB)
+        lib.qkref_aotomo.argtypes = [_I, _I, _P, _P, _P]
+        lib.qkref_phrlr.argtypes = [_I] * 7 + [_P, _P, _P, _P, _P, _P]
+        lib.qkref_excitation_density.argtypes = [_I] * 5 + [_P, _P, _P]
+        lib.qkref_self_energy_diag.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
+        lib.qkref_qp_graph.argtypes = [_I, _D] + [_I] * 6 + [_P] * 7
+        lib.qkref_rg0w0.argtypes = [_I, _I, _I, _D] + [_I] * 6 + [_D, _D] + [_P] * 8
+        lib.qkref_evrgw.argtypes = [_I, _I, _I, _I, _D, _I, _D] + [_I] * 6 + [_D, _D] + [_P] * 7 + [_IP]
+        lib.qkref_evrgw.restype = _I
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.flags.c_contiguous)
+    return a.ctypes.data_as(_P)
+
+
+def _f(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+def _dims(nOrb, nO, nC=0, nR=0):
+    nV = nOrb - nO
+    nS = (nO - nC) * (nV - nR)
+    return nC, nO, nV, nR, nS
+
+
+def aotomo(c, eri_ao):
+    nBas, nOrb = c.shape
+    c = _f(c)
+    out = np.zeros(nOrb ** 4)
+    load().qkref_aotomo(nBas, nOrb, _p(c), _p(np.ascontiguousarray(eri_ao)), _p(out))
+    return out
+
+
+def phrlr(e, eri_mo, nO, tda=False, nC=0, nR=0):
+    nOrb = len(e)
+    nC, nO, nV, nR, nS = _dims(nOrb, nO, nC, nR)
+    Ec = np.zeros(1)
+    Om = np.zeros(nS)
+    XpY = np.zeros((nS, nS), order="F")
+    XmY = np.zeros((nS, nS), order="F")
+    load().qkref_phrlr(int(tda), nOrb, nC, nO, nV, nR, nS, _p(_f(e)), _p(np.ascontiguousarray(eri_mo)),
+                       _p(Ec), _p(Om), _p(XpY), _p(XmY))
+    return {"EcRPA": Ec[0], "Om": Om, "XpY": XpY, "XmY": XmY}
+
+
+def excitation_density(eri_mo, XpY, nOrb, nO, nC=0, nR=0):
+    nC, nO, nV, nR, nS = _dims(nOrb, nO, nC, nR)
+    rho = np.zeros((nOrb, nOrb, nS), order="F")
+    load().qkref_excitation_density(nOrb, nC, nO, nR, nS, _p(np.ascontiguousarray(eri_mo)), _p(_f(XpY)), _p(rho))
+    return rho
+
+
+def self_energy_diag(e, Om, rho, nO, eta=0.0, srg=False, nC=0, nR=0):
+    nOrb = len(e)
+    nC, nO, nV, nR, nS = _dims(nOrb, nO, nC, nR)
+    Ec = np.zeros(1)
+    Sig = np.zeros(nOrb)
+    Z = np.zeros(nOrb)
+    load().qkref_self_energy_diag(int(srg), eta, nOrb, nC, nO, nV, nR, nS, _p(_f(e)), _p(_f(Om)), _p(_f(rho)),
+                                  _p(Ec), _p(Sig), _p(Z))
+    return {"EcGM": Ec[0], "SigC": Sig, "Z": Z}
+
+
+def qp_graph(eHF, Om, rho, eGWlin, eOld, Z, nO, eta=0.0, srg=False, nC=0, nR=0):
+    nOrb = len(eHF)
+    nC, nO, nV, nR, nS = _dims(nOrb, nO, nC, nR)
+    eGW = np.zeros(nOrb)
+    Zio = np.array(Z, dtype=np.float64)
+    load().qkref_qp_graph(int(srg), eta, nOrb, nC, nO, nV, nR, nS, _p(_f(eHF)), _p(_f(Om)), _p(_f(rho)),
+                          _p(_f(eGWlin)), _p(_f(eOld)), _p(eGW), _p(Zio))
+    return {"eGW": eGW, "Z": Zio}
+
+
+def rg0w0(eri_mo, eHF, nO, eta=0.0, linearize=False, tda_w=False, srg=False, ENuc=0.0, ERHF=0.0):
+    nOrb = len(eHF)
+    nC, nO, nV, nR, nS = _dims(nOrb, nO)
+    eGW, Sig, Z = np.zeros(nOrb), np.zeros(nOrb), np.zeros(nOrb)
+    Om, Ec, Egm = np.zeros(nS), np.zeros(1), np.zeros(1)
+    load().qkref_rg0w0(int(tda_w), int(linearize), int(srg), eta, nOrb, nC, nO, nV, nR, nS, ENuc, ERHF,
+                       _p(np.ascontiguousarray(eri_mo)), _p(_f(eHF)), _p(eGW), _p(Sig), _p(Z), _p(Om),
+                       _p(Ec), _p(Egm))
+    return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": Ec[0], "EcGM": Egm[0]}
+
+
+def evrgw(eri_mo, eHF, nO, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, linearize=False, tda_w=False,
+          srg=False, ENuc=0.0, ERHF=0.0):
+    nOrb = len(eHF)
+    nC, nO, nV, nR, nS = _dims(nOrb, nO)
+    eGW, Sig, Z = np.zeros(nOrb), np.zeros(nOrb), np.zeros(nOrb)
+    Ec, Egm = np.zeros(1), np.zeros(1)
+    nscf = ctypes.c_int(0)
+    st = load().qkref_evrgw(int(tda_w), int(linearize), int(srg), maxSCF, thresh, max_diis, eta, nOrb, nC, nO, nV,
+                            nR, nS, ENuc, ERHF, _p(np.ascontiguousarray(eri_mo)), _p(_f(eHF)), _p(eGW), _p(Sig),
+                            _p(Z), _p(Ec), _p(Egm), ctypes.byref(nscf))
+    return {"eGW": eGW, "SigC": Sig, "Z": Z, "EcRPA": Ec[0], "EcGM": Egm[0], "nSCF": nscf.value, "status": st}

@@ -1,0 +1,184 @@
+! TEST INFRASTRUCTURE ONLY -- never part of the product path.
+!
+! bind(C) entry points over QuAcK's own restricted G0W0/evGW routines, which
+! oracle/Makefile compiles unchanged from the QuAcK sources (flang + MKL).
+! The product library never links this; tests, __graft_entry__.smoke() and the
+! CPU-baseline leg of bench.py load it through ctypes as the checker.
+!
+! qkref_rg0w0 replays the screening / self-energy / QP sequence of
+! GW/RG0W0.f90:88-233 (no linDM, BSE, ACFDT or ppBSE branch) and
+! qkref_evrgw the loop of GW/evRGW.f90:104-200, returning a status instead of
+! `stop` on non-convergence.  All arrays are Fortran column-major, exactly as
+! QuAcK holds them.
+
+subroutine qkref_aotomo(nBas, nOrb, c, ERI_AO, ERI_MO) bind(C, name='qkref_aotomo')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: nBas, nOrb
+  real(c_double), intent(in) :: c(nBas, nOrb), ERI_AO(nBas, nBas, nBas, nBas)
+  real(c_double), intent(out) :: ERI_MO(nOrb, nOrb, nOrb, nOrb)
+  call AOtoMO_ERI_RHF(nBas, nOrb, c, ERI_AO, ERI_MO)
+end subroutine
+
+subroutine qkref_phrlr(tda, nOrb, nC, nO, nV, nR, nS, e, ERI, EcRPA, Om, XpY, XmY) bind(C, name='qkref_phrlr')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: tda, nOrb, nC, nO, nV, nR, nS
+  real(c_double), intent(in) :: e(nOrb), ERI(nOrb, nOrb, nOrb, nOrb)
+  real(c_double), intent(out) :: EcRPA, Om(nS), XpY(nS, nS), XmY(nS, nS)
+  double precision, allocatable :: Aph(:,:), Bph(:,:)
+  logical :: TDA_W
+  TDA_W = tda /= 0
+  allocate(Aph(nS, nS), Bph(nS, nS))
+  Bph = 0d0
+  call phRLR_A(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, e, ERI, Aph)
+  if(.not. TDA_W) call phRLR_B(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, ERI, Bph)
+  call phRLR(TDA_W, nS, Aph, Bph, EcRPA, Om, XpY, XmY)
+  deallocate(Aph, Bph)
+end subroutine
+
+subroutine qkref_excitation_density(nOrb, nC, nO, nR, nS, ERI, XpY, rho) bind(C, name='qkref_excitation_density')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: nOrb, nC, nO, nR, nS
+  real(c_double), intent(in) :: ERI(nOrb, nOrb, nOrb, nOrb), XpY(nS, nS)
+  real(c_double), intent(out) :: rho(nOrb, nOrb, nS)
+  call RGW_excitation_density(nOrb, nC, nO, nR, nS, ERI, XpY, rho)
+end subroutine
+
+subroutine qkref_self_energy_diag(srg, eta, nOrb, nC, nO, nV, nR, nS, e, Om, rho, EcGM, SigC, Z) &
+    bind(C, name='qkref_self_energy_diag')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: srg, nOrb, nC, nO, nV, nR, nS
+  real(c_double), value :: eta
+  real(c_double), intent(in) :: e(nOrb), Om(nS), rho(nOrb, nOrb, nS)
+  real(c_double), intent(out) :: EcGM, SigC(nOrb), Z(nOrb)
+  if(srg /= 0) then
+    call RGW_SRG_self_energy_diag(500d0, nOrb, nOrb, nC, nO, nV, nR, nS, e, Om, rho, EcGM, SigC, Z)
+  else
+    call RGW_self_energy_diag(eta, nOrb, nOrb, nC, nO, nV, nR, nS, e, Om, rho, EcGM, SigC, Z)
+  end if
+end subroutine
+
+subroutine qkref_qp_graph(srg, eta, nOrb, nC, nO, nV, nR, nS, eHF, Om, rho, eGWlin, eOld, eGW, Z) &
+    bind(C, name='qkref_qp_graph')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: srg, nOrb, nC, nO, nV, nR, nS
+  real(c_double), value :: eta
+  real(c_double), intent(in) :: eHF(nOrb), Om(nS), rho(nOrb, nOrb, nS), eGWlin(nOrb), eOld(nOrb)
+  real(c_double), intent(out) :: eGW(nOrb)
+  real(c_double), intent(inout) :: Z(nOrb)
+  call RGW_QP_graph(srg /= 0, eta, 500d0, nOrb, nC, nO, nV, nR, nS, eHF, Om, rho, eGWlin, eOld, eGW, Z)
+end subroutine
+
+! G0W0: GW/RG0W0.f90:88-233.  Om0 is the screening at eHF; EcRPA comes from
+! the second RPA at eGW (RG0W0.f90:222-225).
+subroutine qkref_rg0w0(tda, lin, srg, eta, nOrb, nC, nO, nV, nR, nS, ENuc, ERHF, ERI, eHF, &
+                       eGW, SigC, Z, Om0, EcRPA, EcGM) bind(C, name='qkref_rg0w0')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: tda, lin, srg, nOrb, nC, nO, nV, nR, nS
+  real(c_double), value :: eta, ENuc, ERHF
+  real(c_double), intent(in) :: ERI(nOrb, nOrb, nOrb, nOrb), eHF(nOrb)
+  real(c_double), intent(out) :: eGW(nOrb), SigC(nOrb), Z(nOrb), Om0(nS), EcRPA, EcGM
+  double precision, allocatable :: Aph(:,:), Bph(:,:), Om(:), XpY(:,:), XmY(:,:), rho(:,:,:), eGWlin(:)
+  logical :: TDA_W, doSRG
+
+  TDA_W = tda /= 0
+  doSRG = srg /= 0
+  allocate(Aph(nS,nS), Bph(nS,nS), Om(nS), XpY(nS,nS), XmY(nS,nS), rho(nOrb,nOrb,nS), eGWlin(nOrb))
+  Bph = 0d0
+
+                 call phRLR_A(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, eHF, ERI, Aph)
+  if(.not.TDA_W) call phRLR_B(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, ERI, Bph)
+  call phRLR(TDA_W, nS, Aph, Bph, EcRPA, Om, XpY, XmY)
+  Om0(:) = Om(:)
+
+  call RGW_excitation_density(nOrb, nC, nO, nR, nS, ERI, XpY, rho)
+
+  if(doSRG) then
+    call RGW_SRG_self_energy_diag(500d0, nOrb, nOrb, nC, nO, nV, nR, nS, eHF, Om, rho, EcGM, SigC, Z)
+  else
+    call RGW_self_energy_diag(eta, nOrb, nOrb, nC, nO, nV, nR, nS, eHF, Om, rho, EcGM, SigC, Z)
+  end if
+
+  eGWlin(:) = eHF(:) + Z(:)*SigC(:)
+  if(lin /= 0) then
+    eGW(:) = eGWlin(:)
+  else
+    call RGW_QP_graph(doSRG, eta, 500d0, nOrb, nC, nO, nV, nR, nS, eHF, Om, rho, eGWlin, eHF, eGW, Z)
+  end if
+
+                 call phRLR_A(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, eGW, ERI, Aph)
+  if(.not.TDA_W) call phRLR_B(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, ERI, Bph)
+  call phRLR(TDA_W, nS, Aph, Bph, EcRPA, Om, XpY, XmY)
+
+  call print_RG0W0(nOrb, nC, nO, nV, nR, eHF, ENuc, ERHF, SigC, Z, eGW, EcRPA, EcGM)
+  deallocate(Aph, Bph, Om, XpY, XmY, rho, eGWlin)
+end subroutine
+
+! evGW: GW/evRGW.f90:104-200.  Returns 0 on convergence, 1 when nSCF reaches
+! maxSCF+1 (where QuAcK prints "Convergence failed" and stops).
+function qkref_evrgw(tda, lin, srg, maxSCF, thresh, max_diis, eta, nOrb, nC, nO, nV, nR, nS, ENuc, ERHF, ERI, eHF, &
+                     eGW, SigC, Z, EcRPA, EcGM, nSCF_out) result(status) bind(C, name='qkref_evrgw')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: tda, lin, srg, maxSCF, max_diis, nOrb, nC, nO, nV, nR, nS
+  real(c_double), value :: thresh, eta, ENuc, ERHF
+  real(c_double), intent(in) :: ERI(nOrb, nOrb, nOrb, nOrb), eHF(nOrb)
+  real(c_double), intent(out) :: eGW(nOrb), SigC(nOrb), Z(nOrb), EcRPA, EcGM
+  integer(c_int), intent(out) :: nSCF_out
+  integer(c_int) :: status
+  double precision, allocatable :: Aph(:,:), Bph(:,:), Om(:), XpY(:,:), XmY(:,:), rho(:,:,:), eOld(:)
+  double precision, allocatable :: error_diis(:,:), e_diis(:,:)
+  logical :: TDA_W, doSRG
+  integer :: nSCF, n_diis
+  double precision :: Conv, rcond
+
+  TDA_W = tda /= 0
+  doSRG = srg /= 0
+  allocate(Aph(nS,nS), Bph(nS,nS), Om(nS), XpY(nS,nS), XmY(nS,nS), rho(nOrb,nOrb,nS), eOld(nOrb), &
+           error_diis(nOrb,max(max_diis,1)), e_diis(nOrb,max(max_diis,1)))
+  Bph = 0d0
+  nSCF = 0
+  n_diis = 0
+  Conv = 1d0
+  e_diis(:,:) = 0d0
+  error_diis(:,:) = 0d0
+  eGW(:) = eHF(:)
+  eOld(:) = eGW(:)
+  Z(:) = 1d0
+  rcond = 0d0
+
+  do while(Conv > thresh .and. nSCF <= maxSCF)
+                   call phRLR_A(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, eGW, ERI, Aph)
+    if(.not.TDA_W) call phRLR_B(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, ERI, Bph)
+    call phRLR(TDA_W, nS, Aph, Bph, EcRPA, Om, XpY, XmY)
+    call RGW_excitation_density(nOrb, nC, nO, nR, nS, ERI, XpY, rho)
+    if(doSRG) then
+      call RGW_SRG_self_energy_diag(500d0, nOrb, nOrb, nC, nO, nV, nR, nS, eGW, Om, rho, EcGM, SigC, Z)
+    else
+      call RGW_self_energy_diag(eta, nOrb, nOrb, nC, nO, nV, nR, nS, eGW, Om, rho, EcGM, SigC, Z)
+    end if
+    if(lin /= 0) then
+      eGW(:) = eHF(:) + SigC(:)
+    else
+      call RGW_QP_graph(doSRG, eta, 500d0, nOrb, nC, nO, nV, nR, nS, eHF, Om, rho, eOld, eOld, eGW, Z)
+    end if
+    Conv = maxval(abs(eGW - eOld))
+    call print_evRGW(nOrb, nC, nO, nV, nR, nSCF, Conv, eHF, ENuc, ERHF, SigC, Z, eGW, EcRPA, EcGM)
+    if(max_diis > 1) then
+      n_diis = min(n_diis+1, max_diis)
+      call DIIS_extrapolation(rcond, nOrb, nOrb, n_diis, error_diis, e_diis, eGW-eOld, eGW)
+    end if
+    eOld(:) = eGW(:)
+    nSCF = nSCF + 1
+  end do
+
+  nSCF_out = nSCF
+  status = 0
+  if(nSCF == maxSCF+1) status = 1
+  deallocate(Aph, Bph, Om, XpY, XmY, rho, eOld, error_diis, e_diis)
+end function
