@@ -1,0 +1,117 @@
+/*
+ * quack_amd: C-ABI of the MI355X (gfx950) restricted G0W0/evGW engine.
+ *
+ * Conventions (identical to QuAcK's Fortran, so a bind(C) interface passes
+ * its arrays straight through):
+ *   - all arrays are FP64, Fortran column-major, caller-owned host memory
+ *     unless the name ends in _device;
+ *   - ERI(p,q,r,s) = <pq|rs> = (pr|qs), physicist order
+ *     (utils/read_2e_integrals.f90:55-63);
+ *   - particle-hole index ia = (a-nO) + (i-nC-1)*(nBas-nR-nO) with 1-based
+ *     i, a (LR/phRLR_A.f90:151), a fastest;
+ *   - XpY(m, ia): row = RPA mode m, column = ph pair (LR/phRLR.f90:77-78);
+ *   - rho(p, q, m) (GW/RGW_excitation_density.f90:24).
+ * Every entry point returns QK_OK (0) or one of the QK_E_* codes below; a
+ * message for the last failure is kept per handle (qk_gw_last_error) and per
+ * thread for the handle-less entry points (qk_last_error).
+ * Calls on one handle are not re-entrant; every call is synchronous at
+ * return.  Call from outside OpenMP parallel regions.
+ */
+#ifndef QUACK_AMD_H
+#define QUACK_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    QK_OK = 0,
+    QK_E_ARG = 1,          /* invalid size, flag or pointer */
+    QK_E_HIP = 2,          /* HIP runtime or device error */
+    QK_E_EIG = 3,          /* eigensolver failure (rocSOLVER info != 0) */
+    QK_E_NOCONV = 4,       /* evGW reached maxSCF+1 cycles (GW/evRGW.f90:190-199) */
+    QK_E_UNSUPPORTED = 5,  /* option not implemented on the GPU path */
+    QK_E_ALLOC = 6         /* device memory exhausted */
+};
+
+typedef struct qk_gw_ctx *qk_gw_handle;
+
+/* ---- handle-based path: device-resident state across evGW cycles ------- */
+
+/* Sizes as in GW/RGW.f90:44-50.  nS must equal (nO-nC)*(nOrb-nR-nO).
+ * Replaces the allocation block of GW/RG0W0.f90:109-110 / evRGW.f90:99-100. */
+int qk_gw_create(qk_gw_handle *h, int nBas, int nOrb, int nC, int nO, int nV, int nR, int nS);
+int qk_gw_destroy(qk_gw_handle h);
+const char *qk_gw_last_error(qk_gw_handle h);
+
+/* AO->MO transformation of the block (pq|jb) that the GW path needs
+ * (j active occupied, b active virtual), on the device.  Replaces the call
+ * AOtoMO_ERI_RHF(nBas,nOrb,c,ERI_AO,ERI_MO) of QuAcK/RQuAcK.f90:459 for this
+ * path (AOtoMO/AOtoMO_ERI_RHF.f90:1-55).  ERI_AO: nBas^4, c: nBas x nOrb. */
+int qk_gw_set_eri_ao(qk_gw_handle h, const double *ERI_AO, const double *c);
+/* Same with ERI_AO and c already in device memory (the handle keeps no
+ * reference to them after return). */
+int qk_gw_set_eri_ao_device(qk_gw_handle h, const double *ERI_AO_device, const double *c_device);
+/* MO integrals as QuAcK passes them to RG0W0/evRGW (GW/RGW.f90:85-86,102-103):
+ * ERI_MO is nOrb^4; only the (pq|jb) block is copied to the device. */
+int qk_gw_set_eri_mo(qk_gw_handle h, const double *ERI_MO);
+
+/* G0W0@RHF: GW/RG0W0.f90:88-233 (screening at eHF, excitation densities,
+ * diagonal self-energy, linearised or graphical QP solve, second RPA at eGW
+ * for EcRPA).  Om receives the screening energies at eHF (nS); any output
+ * pointer except eGW may be NULL.  linDM/BSE/ACFDT/ppBSE are not on this
+ * path: callers keep QuAcK's CPU routines for them. */
+int qk_rg0w0(qk_gw_handle h, const double *eHF, double eta, int linearize, int TDA_W, int doSRG, double *eGW,
+             double *SigC, double *Z, double *Om, double *EcRPA, double *EcGM);
+
+/* evGW@RHF: GW/evRGW.f90:104-200.  eGW_start (may be NULL) replaces the
+ * eGW = eHF start of evRGW.f90:110; NULL reproduces QuAcK.  Returns
+ * QK_E_NOCONV (outputs filled with the last cycle) where QuAcK stops with
+ * "Convergence failed". */
+int qk_evrgw(qk_gw_handle h, const double *eHF, const double *eGW_start, int maxSCF, double thresh, int max_diis,
+             double eta, int linearize, int TDA_W, int doSRG, double *eGW, double *SigC, double *Z, double *Om,
+             double *EcRPA, double *EcGM, int *nSCF);
+
+/* Device time (ms, HIP events on the handle's stream) of the stages of the
+ * last qk_rg0w0 / qk_evrgw call: [0] AO->MO block (last set_eri_*),
+ * [1] RPA matrix build, [2] eigensolve, [3] excitation densities,
+ * [4] self-energy + QP solve, [5] total of the GW call.  n <= 6. */
+int qk_gw_stage_times(qk_gw_handle h, double *ms, int n);
+
+/* ---- per-routine entry points (host in, host out) ---------------------- */
+
+/* AOtoMO/AOtoMO_ERI_RHF.f90:1-55: full ERI_MO(nOrb^4). */
+int qk_aotomo_eri_rhf(int nBas, int nOrb, const double *c, const double *ERI_AO, double *ERI_MO);
+
+/* LR/phRLR.f90:1-97 for general Aph, Bph (nS x nS): EcRPA, Om (ascending),
+ * XpY, XmY.  TDA != 0 uses Aph only (phRLR.f90:38-43). */
+int qk_phrlr(int TDA, int nS, const double *Aph, const double *Bph, double *EcRPA, double *Om, double *XpY,
+             double *XmY);
+
+/* LR/phRLR_A.f90:43-89 and LR/phRLR_B.f90:34-75, singlet (ispin = 1). */
+int qk_phrlr_a(int dRPA, int nBas, int nC, int nO, int nV, int nR, int nS, double lambda, const double *e,
+               const double *ERI, double *Aph);
+int qk_phrlr_b(int dRPA, int nBas, int nC, int nO, int nV, int nR, int nS, double lambda, const double *ERI,
+               double *Bph);
+
+/* GW/RGW_excitation_density.f90:1-80: rho(nOrb,nOrb,nS) from ERI_MO(nOrb^4). */
+int qk_rgw_excitation_density(int nOrb, int nC, int nO, int nR, int nS, const double *ERI, const double *XpY,
+                              double *rho);
+
+/* GW/RGW_self_energy_diag.f90:1-113 (doSRG = 0) or
+ * GW/RGW_SRG_self_energy_diag.f90:1-160 with flow 500 (doSRG != 0). */
+int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double *e,
+                            const double *Om, const double *rho, double *EcGM, double *SigC, double *Z);
+
+/* GW/RGW_QP_graph.f90:1-97: Newton solve of the QP equation; Z is in/out as
+ * in QuAcK (unchanged for an orbital whose solve hits maxIt = 64). */
+int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO, int nV, int nR, int nS,
+                    const double *eHF, const double *Om, const double *rho, const double *eGWlin, const double *eOld,
+                    double *eGW, double *Z);
+
+const char *qk_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

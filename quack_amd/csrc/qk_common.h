@@ -1,0 +1,43 @@
+// Shared declarations of the HIP G0W0/evGW library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/quack_amd.h"
+
+namespace qk {
+
+// Thrown inside the library, turned into a status code at the C-ABI edge.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define QK_HIP(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            throw ::qk::Error{QK_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_) + " at " + \
+                                            __FILE__ + ":" + std::to_string(__LINE__)};               \
+    } while (0)
+
+#define QK_CHECK(cond, code, msg)                              \
+    do {                                                       \
+        if (!(cond)) throw ::qk::Error{(code), std::string(msg)}; \
+    } while (0)
+
+// Batched addressing: batch z = z0 + zdiv * z1; operand offset = z0*s0 + z1*s1.
+struct Batch {
+    long long sa0 = 0, sa1 = 0, sb0 = 0, sb1 = 0, sc0 = 0, sc1 = 0;
+    int zdiv = 1;
+    int count = 1;
+};
+
+// C(MxN) = alpha * op(A)(MxK) * op(B)(KxN) + beta * C, column-major, FP64 MFMA.
+void dgemm(hipStream_t s, bool transA, bool transB, int M, int N, int K, double alpha, const double* A,
+           long long lda, const double* B, long long ldb, double beta, double* C, long long ldc,
+           const Batch& b = Batch());
+
+}  // namespace qk

@@ -1,0 +1,891 @@
+// C-ABI of the restricted G0W0/evGW engine: device-resident handle plus the
+// per-routine entry points declared in include/quack_amd.h.
+//
+// Control flow follows GW/RG0W0.f90:88-233 and GW/evRGW.f90:104-200.  The
+// screening uses that, with dRPA coupling (isp_W = 1, dRPA_W = .true.,
+// GW/RG0W0.f90:88-89), A and B share the element 2(ia|jb)
+// (LR/phRLR_A.f90:160, LR/phRLR_B.f90:290), so A-B = diag(e_a-e_i) and
+// (A-B)^1/2 (A+B) (A-B)^1/2 = D^2 + 4 D^1/2 K D^1/2: one symmetric eigensolve
+// instead of the two of LR/phRLR.f90:54,67, and XpY/XmY follow from its
+// eigenvectors by diagonal scalings (LR/phRLR.f90:77-81).
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "qk_kernels.h"
+
+namespace qk {
+namespace {
+
+constexpr int QP_MAXIT = 64;       // GW/RGW_QP_graph.f90:31
+constexpr double QP_THRESH = 1e-6; // GW/RGW_QP_graph.f90:32
+constexpr double SRG_FLOW = 500.0; // GW/RG0W0.f90:98
+
+thread_local std::string g_last_error;
+
+template <class T>
+T* dalloc(size_t count) {
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{QK_E_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
+                                    hipGetErrorString(e)};
+    }
+    return static_cast<T*>(p);
+}
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void rb_check(rocblas_status st, const char* what) {
+    if (st != rocblas_status_success) throw Error{QK_E_HIP, std::string(what) + ": rocblas status " + std::to_string((int)st)};
+}
+
+// rocSOLVER dsyevd on the stream bound to rb; eigenvalues ascending in w.
+void syevd(hipStream_t s, rocblas_handle rb, bool vectors, int n, double* A, double* w, double* work, int* dinfo) {
+    rb_check(rocsolver_dsyevd(rb, vectors ? rocblas_evect_original : rocblas_evect_none, rocblas_fill_upper, n, A, n,
+                              w, work, dinfo),
+             "rocsolver_dsyevd");
+    int info = 0;
+    QK_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+    QK_HIP(hipStreamSynchronize(s));
+    QK_CHECK(info == 0, QK_E_EIG, "rocsolver_dsyevd failed, info = " + std::to_string(info));
+}
+
+// Tiny symmetric solve for DIIS: LU with partial pivoting; rcond is the exact
+// 1-norm reciprocal condition number (what dsysvx estimates,
+// utils/wrap_lapack.f90:560-598) from the explicit inverse of the (<= 33)x(<= 33) matrix.
+double small_solve(int n, std::vector<double> A, std::vector<double> b, std::vector<double>& x) {
+    std::vector<double> Ainv(n * n, 0.0), LU = A;
+    std::vector<int> piv(n);
+    double anorm = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += std::fabs(A[i + n * j]);
+        anorm = std::max(anorm, s);
+    }
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(LU[i + n * k]) > std::fabs(LU[p + n * k])) p = i;
+        piv[k] = p;
+        if (LU[p + n * k] == 0.0) { x.assign(n, 0.0); return 0.0; }
+        if (p != k)
+            for (int j = 0; j < n; ++j) std::swap(LU[k + n * j], LU[p + n * j]);
+        for (int i = k + 1; i < n; ++i) {
+            LU[i + n * k] /= LU[k + n * k];
+            for (int j = k + 1; j < n; ++j) LU[i + n * j] -= LU[i + n * k] * LU[k + n * j];
+        }
+    }
+    auto lusolve = [&](std::vector<double>& r) {
+        for (int k = 0; k < n; ++k) std::swap(r[k], r[piv[k]]);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) r[i] -= LU[i + n * j] * r[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) r[i] -= LU[i + n * j] * r[j];
+            r[i] /= LU[i + n * i];
+        }
+    };
+    double inorm = 0.0;
+    for (int j = 0; j < n; ++j) {
+        std::vector<double> ej(n, 0.0);
+        ej[j] = 1.0;
+        lusolve(ej);
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += std::fabs(ej[i]);
+        inorm = std::max(inorm, s);
+    }
+    x = b;
+    lusolve(x);
+    if (!(anorm > 0.0) || !(inorm > 0.0) || !std::isfinite(inorm)) return 0.0;
+    return 1.0 / (anorm * inorm);
+}
+
+}  // namespace
+
+// utils/DIIS_extrapolation.f90:1-61 (history prepend utils/utils.f90:506-536).
+struct Diis {
+    int n, max_diis, n_diis = 0;
+    std::vector<double> err, e;  // column-major (n, max_diis)
+    Diis(int n_, int m_) : n(n_), max_diis(m_), err((size_t)n_ * m_, 0.0), e((size_t)n_ * m_, 0.0) {}
+    void step(const double* err_in, double* e_inout) {
+        n_diis = std::min(n_diis + 1, max_diis);
+        int m = n_diis;
+        for (int j = m - 2; j >= 0; --j) {
+            std::memcpy(&err[(size_t)n * (j + 1)], &err[(size_t)n * j], sizeof(double) * n);
+            std::memcpy(&e[(size_t)n * (j + 1)], &e[(size_t)n * j], sizeof(double) * n);
+        }
+        std::memcpy(&err[0], err_in, sizeof(double) * n);
+        std::memcpy(&e[0], e_inout, sizeof(double) * n);
+        int N = m + 1;
+        std::vector<double> A((size_t)N * N, 0.0), b(N, 0.0), w;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < n; ++k) s += err[(size_t)n * i + k] * err[(size_t)n * j + k];
+                A[i + N * j] = s;
+            }
+        for (int i = 0; i < m; ++i) { A[i + N * m] = -1.0; A[m + N * i] = -1.0; }
+        b[m] = -1.0;
+        double rcond = small_solve(N, A, b, w);
+        if (rcond > 1e-13) {
+            for (int k = 0; k < n; ++k) {
+                double s = 0.0;
+                for (int j = 0; j < m; ++j) s += w[j] * e[(size_t)n * j + k];
+                e_inout[k] = s;
+            }
+        }
+    }
+};
+
+}  // namespace qk
+
+struct qk_gw_ctx {
+    int nBas = 0, nOrb = 0, nC = 0, nO = 0, nV = 0, nR = 0, nS = 0, o = 0, v = 0;
+    hipStream_t s = nullptr;
+    rocblas_handle rb = nullptr;
+    double *M = nullptr, *K = nullptr, *Cm = nullptr, *rhoT = nullptr;
+    double *e = nullptr, *e0 = nullptr, *eG = nullptr, *dE = nullptr, *sd = nullptr, *Om = nullptr, *Om0 = nullptr;
+    double *Sig = nullptr, *Z = nullptr, *eLin = nullptr, *eGW = nullptr, *w = nullptr, *f = nullptr, *df = nullptr;
+    double *part = nullptr, *scal = nullptr, *work = nullptr;
+    int *nit = nullptr, *done = nullptr, *info = nullptr, *active = nullptr;
+    int nchunk = 1;
+    bool have_eri = false;
+    double t_ao = 0.0;
+    double t_stage[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::string err;
+};
+
+namespace qk {
+namespace {
+
+using Ctx = qk_gw_ctx;
+
+// Event-timed stage on the handle's stream (accumulates into t_stage[k]).
+struct Stage {
+    Ctx* c;
+    int k;
+    Stage(Ctx* c_, int k_) : c(c_), k(k_) { QK_HIP(hipEventRecord(c->ev[0], c->s)); }
+    ~Stage() {
+        // never throws: a failed timing read leaves the stage time unchanged
+        float ms = 0.f;
+        if (hipEventRecord(c->ev[1], c->s) == hipSuccess && hipEventSynchronize(c->ev[1]) == hipSuccess &&
+            hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess)
+            c->t_stage[k] += ms;
+    }
+};
+
+void check_sizes(int nBas, int nOrb, int nC, int nO, int nV, int nR, int nS) {
+    QK_CHECK(nBas > 0 && nOrb > 0 && nOrb <= nBas, QK_E_ARG, "need 0 < nOrb <= nBas");
+    QK_CHECK(nC >= 0 && nR >= 0 && nO > nC && nO < nOrb - nR, QK_E_ARG, "need nC < nO < nOrb - nR");
+    QK_CHECK(nV == nOrb - nO, QK_E_ARG, "need nV = nOrb - nO");
+    QK_CHECK((long long)nS == (long long)(nO - nC) * (nV - nR), QK_E_ARG, "need nS = (nO-nC)*(nV-nR)");
+    QK_CHECK((long long)nS * nS < (1LL << 40), QK_E_ARG, "nS too large");
+}
+
+// RPA screening at energies e (device): Om, and W = scaled eigenvectors in Cm.
+void screening(Ctx* c, const double* e, int tda, bool vectors) {
+    {
+        Stage st(c, 0);
+        rpa_build(c->s, e, c->K, c->dE, c->sd, c->Cm, c->scal, c->nC, c->nO, c->v, c->nS, tda);
+    }
+    {
+        Stage st(c, 1);
+        syevd(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->info);
+    }
+    {
+        Stage st(c, 0);
+        rpa_finalize(c->s, c->Om, c->Cm, c->sd, c->scal, c->nS, tda, vectors);
+    }
+}
+
+// rhoT[m + nS (p + n q)] = sum_jb W[jb, m] M[pq, jb]  (rho^T = W^T M^T)
+void excitation(Ctx* c) {
+    Stage st(c, 2);
+    long long n2 = (long long)c->nOrb * c->nOrb;
+    QK_CHECK(n2 < (1LL << 31), QK_E_ARG, "nOrb^2 too large");
+    dgemm(c->s, true, true, c->nS, (int)n2, c->nS, 1.0, c->Cm, c->nS, c->M, n2, 0.0, c->rhoT, c->nS);
+}
+
+void sigma_static(Ctx* c, const double* e, double eta, int srg) {
+    Stage st(c, 3);
+    sigma_partial(c->s, c->rhoT, e, e, c->Om, nullptr, eta, srg, SRG_FLOW, c->nOrb, c->nC, c->nO, c->nR, c->nS,
+                  c->nchunk, c->part);
+    sigma_finalize(c->s, c->part, c->nchunk, c->nOrb, c->nC, c->nO, c->nR, c->Sig, c->Z, c->scal);
+}
+
+// GW/RGW_QP_graph.f90:50-92 for all active orbitals at once: e0 = eHF, eG =
+// energies in G, start = eGWlin; Z in/out.  Result in c->eGW.
+void qp_graph(Ctx* c, const double* e0, const double* eG, const double* start, double eta, int srg, double flow) {
+    Stage st(c, 3);
+    int n = c->nOrb;
+    qp_init(c->s, start, c->w, c->f, c->nit, c->done, n, c->nC, c->nR);
+    int h_active = 1;
+    for (int it = 0; it < QP_MAXIT && h_active > 0; ++it) {
+        sigma_partial(c->s, c->rhoT, c->w, eG, c->Om, c->done, eta, srg, flow, n, c->nC, c->nO, c->nR, c->nS,
+                      c->nchunk, c->part);
+        qp_update(c->s, c->part, c->nchunk, e0, c->w, c->f, c->df, c->nit, c->done, n, QP_THRESH, QP_MAXIT, c->active);
+        QK_HIP(hipMemcpyAsync(&h_active, c->active, sizeof(int), hipMemcpyDeviceToHost, c->s));
+        QK_HIP(hipStreamSynchronize(c->s));
+    }
+    qp_finish(c->s, c->w, c->df, c->nit, start, c->eGW, c->Z, n, c->nC, c->nR, QP_MAXIT);
+}
+
+void h2d(Ctx* c, double* dst, const double* src, size_t count) {
+    QK_HIP(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyHostToDevice, c->s));
+}
+void d2h(Ctx* c, double* dst, const double* src, size_t count) {
+    if (dst) QK_HIP(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, c->s));
+}
+double d2h_scalar(Ctx* c, const double* src) {
+    double v = 0.0;
+    QK_HIP(hipMemcpyAsync(&v, src, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    QK_HIP(hipStreamSynchronize(c->s));
+    return v;
+}
+
+// (pq|jb) from AO integrals on the device: four quarter transforms (occupied
+// index first: 2N^4 o + 6 N^3 o v flops instead of 8 N^5 for the full ERI_MO).
+void aotomo_block(hipStream_t s, const double* f, const double* cc, double* M, int N, int n, int nC, int nO, int nR) {
+    int o = nO - nC, v = n - nR - nO;
+    long long N2 = (long long)N * N, N3 = N2 * N;
+    QK_CHECK(N3 < (1LL << 31), QK_E_ARG, "nBas^3 too large");
+    const double* Cocc = cc + (long long)N * nC;
+    const double* Cvir = cc + (long long)N * nO;
+    double* T1 = dalloc<double>((size_t)N3 * o);
+    double* T2 = nullptr;
+    try {
+        // T1[mu, lam, nu, j] = sum_sig (mu nu|lam sig) C[sig, j]
+        dgemm(s, false, false, (int)N3, o, N, 1.0, f, N3, Cocc, N, 0.0, T1, N3);
+        // T2[mu, nu, b, j] = sum_lam T1[mu, lam, nu, j] C[lam, b]; batch (nu, j)
+        T2 = dalloc<double>((size_t)N2 * v * o);
+        Batch b2;
+        b2.zdiv = N; b2.count = N * o;
+        b2.sa0 = N2; b2.sa1 = N3;
+        b2.sc0 = N; b2.sc1 = N2 * v;
+        dgemm(s, false, false, N, v, N, 1.0, T1, N, Cvir, N, 0.0, T2, N2, b2);
+        QK_HIP(hipStreamSynchronize(s));
+        dfree(T1);
+        // T3[p, nu, b, j] = sum_mu C[mu, p] T2[mu, nu, b, j]
+        T1 = dalloc<double>((size_t)n * N * v * o);
+        long long cols = (long long)N * v * o;
+        QK_CHECK(cols < (1LL << 31), QK_E_ARG, "block too large");
+        dgemm(s, true, false, n, (int)cols, N, 1.0, cc, N, T2, N, 0.0, T1, n);
+        QK_HIP(hipStreamSynchronize(s));
+        dfree(T2);
+        // M[p, q, b, j] = sum_nu T3[p, nu, b, j] C[nu, q]; batch over (b + v j)
+        Batch b4;
+        b4.zdiv = v * o; b4.count = v * o;
+        b4.sa0 = (long long)n * N;
+        b4.sc0 = (long long)n * n;
+        dgemm(s, false, false, n, n, N, 1.0, T1, n, cc, N, 0.0, M, n, b4);
+        QK_HIP(hipStreamSynchronize(s));
+        dfree(T1);
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        dfree(T1);
+        dfree(T2);
+        throw;
+    }
+}
+
+void after_eri(Ctx* c) {
+    extract_k(c->s, c->M, c->K, c->nOrb, c->nC, c->nO, c->v, c->nS);
+    QK_HIP(hipStreamSynchronize(c->s));
+    c->have_eri = true;
+}
+
+// gather X[p + n j' + n o q + n o n b'] = ERI_MO(p, nC+j', q, nO+b') on the host
+void gather_mo_block(const double* ERI, double* X, int n, int nC, int nO, int nR) {
+    int o = nO - nC, v = n - nR - nO;
+    size_t n1 = n, n2 = n1 * n, n3 = n2 * n;
+    for (int b = 0; b < v; ++b)
+        for (int q = 0; q < n; ++q)
+            std::memcpy(X + n1 * o * (q + n1 * b), ERI + n1 * nC + n2 * q + n3 * (nO + b), sizeof(double) * n1 * o);
+}
+
+void upload_mo_block(hipStream_t s, const double* ERI_MO, double* M, int n, int nC, int nO, int nR) {
+    int o = nO - nC, v = n - nR - nO;
+    size_t cnt = (size_t)n * n * o * v;
+    double* hX = nullptr;
+    QK_HIP(hipHostMalloc((void**)&hX, cnt * sizeof(double), hipHostMallocDefault));
+    double* dX = nullptr;
+    try {
+        gather_mo_block(ERI_MO, hX, n, nC, nO, nR);
+        dX = dalloc<double>(cnt);
+        QK_HIP(hipMemcpyAsync(dX, hX, cnt * sizeof(double), hipMemcpyHostToDevice, s));
+        permute_block(s, dX, M, n, o, v);
+        QK_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        dfree(dX);
+        (void)hipHostFree(hX);
+        throw;
+    }
+    dfree(dX);
+    (void)hipHostFree(hX);
+}
+
+// Shared stream and rocBLAS handle of the handle-less entry points.
+struct Scratch {
+    hipStream_t s = nullptr;
+    rocblas_handle rb = nullptr;
+    Scratch() {
+        QK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        rb_check(rocblas_create_handle(&rb), "rocblas_create_handle");
+        rb_check(rocblas_set_stream(rb, s), "rocblas_set_stream");
+    }
+};
+Scratch& scratch() {
+    static Scratch* sc = new Scratch();  // lives until process exit
+    return *sc;
+}
+
+template <class F>
+int guarded(std::string* slot, F&& fn) {
+    try {
+        fn();
+        return QK_OK;
+    } catch (const Error& e) {
+        if (slot) *slot = e.msg;
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (slot) *slot = "host allocation failed";
+        g_last_error = "host allocation failed";
+        return QK_E_ALLOC;
+    } catch (...) {
+        if (slot) *slot = "unknown error";
+        g_last_error = "unknown error";
+        return QK_E_HIP;
+    }
+}
+
+void release(Ctx* c) {
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    double* ds[] = {c->M, c->K, c->Cm, c->rhoT, c->e, c->e0, c->eG, c->dE, c->sd, c->Om, c->Om0, c->Sig, c->Z,
+                    c->eLin, c->eGW, c->w, c->f, c->df, c->part, c->scal, c->work};
+    for (double* p : ds)
+        if (p) (void)hipFree(p);
+    int* is[] = {c->nit, c->done, c->info, c->active};
+    for (int* p : is)
+        if (p) (void)hipFree(p);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->rb) (void)rocblas_destroy_handle(c->rb);
+    if (c->s) (void)hipStreamDestroy(c->s);
+}
+
+}  // namespace
+}  // namespace qk
+
+using namespace qk;
+
+extern "C" {
+
+const char* qk_last_error(void) { return g_last_error.c_str(); }
+
+const char* qk_gw_last_error(qk_gw_handle h) { return h ? h->err.c_str() : g_last_error.c_str(); }
+
+int qk_gw_create(qk_gw_handle* h, int nBas, int nOrb, int nC, int nO, int nV, int nR, int nS) {
+    if (!h) return QK_E_ARG;
+    *h = nullptr;
+    Ctx* c = new (std::nothrow) Ctx();
+    if (!c) return QK_E_ALLOC;
+    int rc = guarded(nullptr, [&] {
+        check_sizes(nBas, nOrb, nC, nO, nV, nR, nS);
+        c->nBas = nBas; c->nOrb = nOrb; c->nC = nC; c->nO = nO; c->nV = nV; c->nR = nR; c->nS = nS;
+        c->o = nO - nC; c->v = nOrb - nR - nO;
+        QK_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+        rb_check(rocblas_create_handle(&c->rb), "rocblas_create_handle");
+        rb_check(rocblas_set_stream(c->rb, c->s), "rocblas_set_stream");
+        QK_HIP(hipEventCreate(&c->ev[0]));
+        QK_HIP(hipEventCreate(&c->ev[1]));
+        size_t n = nOrb, S = nS;
+        c->M = dalloc<double>(n * n * S);
+        c->K = dalloc<double>(S * S);
+        c->Cm = dalloc<double>(S * S);
+        c->rhoT = dalloc<double>(n * n * S);
+        for (double** p : {&c->e, &c->e0, &c->eG, &c->Sig, &c->Z, &c->eLin, &c->eGW, &c->w, &c->f, &c->df})
+            *p = dalloc<double>(n);
+        for (double** p : {&c->dE, &c->sd, &c->Om, &c->Om0, &c->work}) *p = dalloc<double>(S);
+        c->nchunk = sigma_chunks(nS, nOrb - nR - nC);
+        c->part = dalloc<double>(3 * n * c->nchunk);
+        c->scal = dalloc<double>(8);
+        c->nit = dalloc<int>(n);
+        c->done = dalloc<int>(n);
+        c->info = dalloc<int>(1);
+        c->active = dalloc<int>(1);
+    });
+    if (rc != QK_OK) {
+        release(c);
+        delete c;
+        return rc;
+    }
+    *h = c;
+    return QK_OK;
+}
+
+int qk_gw_destroy(qk_gw_handle h) {
+    if (!h) return QK_E_ARG;
+    release(h);
+    delete h;
+    return QK_OK;
+}
+
+int qk_gw_set_eri_ao_device(qk_gw_handle c, const double* f, const double* cc) {
+    if (!c || !f || !cc) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        QK_HIP(hipEventRecord(c->ev[0], c->s));
+        aotomo_block(c->s, f, cc, c->M, c->nBas, c->nOrb, c->nC, c->nO, c->nR);
+        QK_HIP(hipEventRecord(c->ev[1], c->s));
+        QK_HIP(hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        QK_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        c->t_ao = ms;
+        after_eri(c);
+    });
+}
+
+int qk_gw_set_eri_ao(qk_gw_handle c, const double* ERI_AO, const double* cc) {
+    if (!c || !ERI_AO || !cc) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        size_t N = c->nBas;
+        double *df = nullptr, *dc = nullptr;
+        try {
+            df = dalloc<double>(N * N * N * N);
+            dc = dalloc<double>(N * c->nOrb);
+            h2d(c, df, ERI_AO, N * N * N * N);
+            h2d(c, dc, cc, N * c->nOrb);
+            QK_HIP(hipEventRecord(c->ev[0], c->s));
+            aotomo_block(c->s, df, dc, c->M, c->nBas, c->nOrb, c->nC, c->nO, c->nR);
+            QK_HIP(hipEventRecord(c->ev[1], c->s));
+            QK_HIP(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            QK_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            c->t_ao = ms;
+            after_eri(c);
+        } catch (...) {
+            (void)hipStreamSynchronize(c->s);
+            dfree(df);
+            dfree(dc);
+            throw;
+        }
+        dfree(df);
+        dfree(dc);
+    });
+}
+
+int qk_gw_set_eri_mo(qk_gw_handle c, const double* ERI_MO) {
+    if (!c || !ERI_MO) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        upload_mo_block(c->s, ERI_MO, c->M, c->nOrb, c->nC, c->nO, c->nR);
+        c->t_ao = 0.0;
+        after_eri(c);
+    });
+}
+
+int qk_gw_stage_times(qk_gw_handle c, double* ms, int n) {
+    if (!c || !ms || n < 0 || n > 6) return QK_E_ARG;
+    double all[6] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4]};
+    for (int i = 0; i < n; ++i) ms[i] = all[i];
+    return QK_OK;
+}
+
+int qk_rg0w0(qk_gw_handle c, const double* eHF, double eta, int linearize, int TDA_W, int doSRG, double* eGW,
+             double* SigC, double* Z, double* Om, double* EcRPA, double* EcGM) {
+    if (!c || !eHF || !eGW) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        QK_CHECK(c->have_eri, QK_E_ARG, "no integrals: call qk_gw_set_eri_* first");
+        for (double& t : c->t_stage) t = 0.0;
+        auto t0 = std::chrono::steady_clock::now();
+        int n = c->nOrb;
+        h2d(c, c->e0, eHF, n);
+        // screening at eHF (GW/RG0W0.f90:116-119) and spectral weights (:127)
+        screening(c, c->e0, TDA_W, true);
+        QK_HIP(hipMemcpyAsync(c->Om0, c->Om, sizeof(double) * c->nS, hipMemcpyDeviceToDevice, c->s));
+        excitation(c);
+        // self-energy (:169-177) and QP equation (:188-208)
+        sigma_static(c, c->e0, eta, doSRG);
+        double ecgm = d2h_scalar(c, c->scal + 2);
+        lin_qp(c->s, c->e0, c->Z, c->Sig, c->eLin, n, 1);
+        d2h(c, SigC, c->Sig, n);
+        if (linearize) {
+            QK_HIP(hipMemcpyAsync(c->eGW, c->eLin, sizeof(double) * n, hipMemcpyDeviceToDevice, c->s));
+        } else {
+            qp_graph(c, c->e0, c->e0, c->eLin, eta, doSRG, SRG_FLOW);
+        }
+        // RPA correlation energy at eGW (:222-225): eigenvalues only
+        screening(c, c->eGW, TDA_W, false);
+        double ecrpa = d2h_scalar(c, c->scal + 1);
+        d2h(c, eGW, c->eGW, n);
+        d2h(c, Z, c->Z, n);
+        d2h(c, Om, c->Om0, c->nS);
+        QK_HIP(hipStreamSynchronize(c->s));
+        if (EcRPA) *EcRPA = ecrpa;
+        if (EcGM) *EcGM = ecgm;
+        c->t_stage[4] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int qk_evrgw(qk_gw_handle c, const double* eHF, const double* eGW_start, int maxSCF, double thresh, int max_diis,
+             double eta, int linearize, int TDA_W, int doSRG, double* eGW, double* SigC, double* Z, double* Om,
+             double* EcRPA, double* EcGM, int* nSCF_out) {
+    if (!c || !eHF || !eGW) return QK_E_ARG;
+    int conv_status = QK_OK;
+    int rc = guarded(&c->err, [&] {
+        QK_CHECK(c->have_eri, QK_E_ARG, "no integrals: call qk_gw_set_eri_* first");
+        QK_CHECK(maxSCF >= 0, QK_E_ARG, "maxSCF < 0");
+        for (double& t : c->t_stage) t = 0.0;
+        auto t0 = std::chrono::steady_clock::now();
+        int n = c->nOrb;
+        std::vector<double> hGW(eGW_start ? eGW_start : eHF, (eGW_start ? eGW_start : eHF) + n), hOld = hGW,
+            herr(n);
+        Diis diis(n, std::max(max_diis, 1));
+        h2d(c, c->e0, eHF, n);
+        int nSCF = 0;
+        double Conv = 1.0, ecrpa = 0.0, ecgm = 0.0;
+        // GW/evRGW.f90:119-183
+        while (Conv > thresh && nSCF <= maxSCF) {
+            h2d(c, c->e, hGW.data(), n);  // eGW == eOld here
+            screening(c, c->e, TDA_W, true);
+            ecrpa = d2h_scalar(c, c->scal + 1);
+            excitation(c);
+            sigma_static(c, c->e, eta, doSRG);
+            ecgm = d2h_scalar(c, c->scal + 2);
+            if (linearize) {
+                lin_qp(c->s, c->e0, c->Z, c->Sig, c->eGW, n, 0);
+            } else {
+                qp_graph(c, c->e0, c->e, c->e, eta, doSRG, SRG_FLOW);
+            }
+            d2h(c, hGW.data(), c->eGW, n);
+            QK_HIP(hipStreamSynchronize(c->s));
+            Conv = 0.0;
+            for (int p = 0; p < n; ++p) {
+                herr[p] = hGW[p] - hOld[p];
+                Conv = std::max(Conv, std::fabs(herr[p]));
+            }
+            if (max_diis > 1) diis.step(herr.data(), hGW.data());
+            hOld = hGW;
+            ++nSCF;
+        }
+        std::memcpy(eGW, hGW.data(), sizeof(double) * n);
+        d2h(c, SigC, c->Sig, n);
+        d2h(c, Z, c->Z, n);
+        d2h(c, Om, c->Om, c->nS);
+        QK_HIP(hipStreamSynchronize(c->s));
+        if (EcRPA) *EcRPA = ecrpa;
+        if (EcGM) *EcGM = ecgm;
+        if (nSCF_out) *nSCF_out = nSCF;
+        if (nSCF == maxSCF + 1) {
+            conv_status = QK_E_NOCONV;
+            c->err = "evGW: convergence failed after " + std::to_string(nSCF) + " cycles";
+        }
+        c->t_stage[4] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+    return rc != QK_OK ? rc : conv_status;
+}
+
+// ---- per-routine entry points ----------------------------------------------
+
+int qk_aotomo_eri_rhf(int nBas, int nOrb, const double* c, const double* ERI_AO, double* ERI_MO) {
+    if (nBas <= 0 || nOrb <= 0 || nOrb > nBas || !c || !ERI_AO || !ERI_MO) return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        hipStream_t s = scratch().s;
+        size_t N = nBas, n = nOrb;
+        QK_CHECK(N * N * N < (1ULL << 31), QK_E_ARG, "nBas too large for the full transform");
+        double *a = nullptr, *b = nullptr, *dc = nullptr;
+        try {
+            a = dalloc<double>(N * N * N * N);
+            b = dalloc<double>(N * N * N * n);
+            dc = dalloc<double>(N * n);
+            QK_HIP(hipMemcpyAsync(a, ERI_AO, sizeof(double) * N * N * N * N, hipMemcpyHostToDevice, s));
+            QK_HIP(hipMemcpyAsync(dc, c, sizeof(double) * N * n, hipMemcpyHostToDevice, s));
+            // AOtoMO/AOtoMO_ERI_RHF.f90:232-252: dgemm('T','N', rest, nOrb, nBas) four times
+            long long rest = N * N * N;
+            dgemm(s, true, false, (int)rest, (int)n, (int)N, 1.0, a, N, dc, N, 0.0, b, rest);
+            rest = N * N * n;
+            dgemm(s, true, false, (int)rest, (int)n, (int)N, 1.0, b, N, dc, N, 0.0, a, rest);
+            rest = N * n * n;
+            dgemm(s, true, false, (int)rest, (int)n, (int)N, 1.0, a, N, dc, N, 0.0, b, rest);
+            rest = n * n * n;
+            dgemm(s, true, false, (int)rest, (int)n, (int)N, 1.0, b, N, dc, N, 0.0, a, rest);
+            QK_HIP(hipMemcpyAsync(ERI_MO, a, sizeof(double) * n * n * n * n, hipMemcpyDeviceToHost, s));
+            QK_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            dfree(a); dfree(b); dfree(dc);
+            throw;
+        }
+        dfree(a); dfree(b); dfree(dc);
+    });
+}
+
+int qk_phrlr_a(int dRPA, int nBas, int nC, int nO, int nV, int nR, int nS, double lambda, const double* e,
+               const double* ERI, double* Aph) {
+    if (!e || !ERI || !Aph) return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        check_sizes(nBas, nBas, nC, nO, nV, nR, nS);
+        hipStream_t s = scratch().s;
+        size_t n = nBas;
+        double *G = nullptr, *de = nullptr, *A = nullptr;
+        try {
+            G = dalloc<double>(n * n * n * n);
+            de = dalloc<double>(n);
+            A = dalloc<double>((size_t)nS * nS);
+            QK_HIP(hipMemcpyAsync(G, ERI, sizeof(double) * n * n * n * n, hipMemcpyHostToDevice, s));
+            QK_HIP(hipMemcpyAsync(de, e, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            phrlr_ab(s, G, de, A, (int)n, nC, nO, nBas - nR - nO, nS, lambda, dRPA, 0);
+            QK_HIP(hipMemcpyAsync(Aph, A, sizeof(double) * nS * nS, hipMemcpyDeviceToHost, s));
+            QK_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            dfree(G); dfree(de); dfree(A);
+            throw;
+        }
+        dfree(G); dfree(de); dfree(A);
+    });
+}
+
+int qk_phrlr_b(int dRPA, int nBas, int nC, int nO, int nV, int nR, int nS, double lambda, const double* ERI,
+               double* Bph) {
+    if (!ERI || !Bph) return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        check_sizes(nBas, nBas, nC, nO, nV, nR, nS);
+        hipStream_t s = scratch().s;
+        size_t n = nBas;
+        double *G = nullptr, *B = nullptr;
+        try {
+            G = dalloc<double>(n * n * n * n);
+            B = dalloc<double>((size_t)nS * nS);
+            QK_HIP(hipMemcpyAsync(G, ERI, sizeof(double) * n * n * n * n, hipMemcpyHostToDevice, s));
+            phrlr_ab(s, G, nullptr, B, (int)n, nC, nO, nBas - nR - nO, nS, lambda, dRPA, 1);
+            QK_HIP(hipMemcpyAsync(Bph, B, sizeof(double) * nS * nS, hipMemcpyDeviceToHost, s));
+            QK_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            dfree(G); dfree(B);
+            throw;
+        }
+        dfree(G); dfree(B);
+    });
+}
+
+int qk_phrlr(int TDA, int nS, const double* Aph, const double* Bph, double* EcRPA, double* Om, double* XpY,
+             double* XmY) {
+    if (nS <= 0 || !Aph || (!TDA && !Bph) || !Om || !XpY || !XmY) return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        hipStream_t s = scratch().s;
+        rocblas_handle rb = scratch().rb;
+        size_t S2 = (size_t)nS * nS;
+        std::vector<double*> bufs;
+        auto get = [&](size_t cnt) { bufs.push_back(dalloc<double>(cnt)); return bufs.back(); };
+        try {
+            double* A = get(S2);
+            double* U = get(S2);
+            double* w = get(nS);
+            double* work = get(nS);
+            double* sc = get(4);
+            int* info = dalloc<int>(1);
+            bufs.push_back(reinterpret_cast<double*>(info));
+            QK_HIP(hipMemcpyAsync(A, Aph, sizeof(double) * S2, hipMemcpyHostToDevice, s));
+            trace(s, A, nS, sc);
+            if (TDA) {
+                // LR/phRLR.f90:40-43
+                QK_HIP(hipMemcpyAsync(U, A, sizeof(double) * S2, hipMemcpyDeviceToDevice, s));
+                syevd(s, rb, true, nS, U, w, work, info);
+                double* X = get(S2);
+                transpose(s, U, X, nS, nS, nS, nS);
+                rpa_finalize(s, w, nullptr, nullptr, sc, nS, 1, false);
+                QK_HIP(hipMemcpyAsync(XpY, X, sizeof(double) * S2, hipMemcpyDeviceToHost, s));
+                QK_HIP(hipMemcpyAsync(XmY, X, sizeof(double) * S2, hipMemcpyDeviceToHost, s));
+            } else {
+                // LR/phRLR.f90:47-81
+                double* B = get(S2);
+                double* ApB = get(S2);
+                double* Sq = get(S2);
+                double* Iv = get(S2);
+                double* T = get(S2);
+                QK_HIP(hipMemcpyAsync(B, Bph, sizeof(double) * S2, hipMemcpyHostToDevice, s));
+                apb_amb(s, A, B, ApB, U, (long long)S2);
+                syevd(s, rb, true, nS, U, w, work, info);
+                colscale_pow(s, U, w, T, nS, 0.5);
+                dgemm(s, false, true, nS, nS, nS, 1.0, T, nS, U, nS, 0.0, Sq, nS);   // ADAt(U, sqrt(w))
+                colscale_pow(s, U, w, T, nS, -0.5);
+                dgemm(s, false, true, nS, nS, nS, 1.0, T, nS, U, nS, 0.0, Iv, nS);   // ADAt(U, 1/sqrt(w))
+                dgemm(s, false, false, nS, nS, nS, 1.0, ApB, nS, Sq, nS, 0.0, T, nS);  // tmp = ApB AmBSq
+                dgemm(s, false, false, nS, nS, nS, 1.0, Sq, nS, T, nS, 0.0, U, nS);    // Z = AmBSq tmp
+                syevd(s, rb, true, nS, U, w, work, info);
+                rpa_finalize(s, w, nullptr, nullptr, sc, nS, 0, false);                 // Om = sqrt|w|
+                dgemm(s, true, false, nS, nS, nS, 1.0, U, nS, Sq, nS, 0.0, T, nS);     // XpY = Z^T AmBSq
+                rowscale_pow(s, T, w, nS, -0.5);
+                QK_HIP(hipMemcpyAsync(XpY, T, sizeof(double) * S2, hipMemcpyDeviceToHost, s));
+                dgemm(s, true, false, nS, nS, nS, 1.0, U, nS, Iv, nS, 0.0, ApB, nS);   // XmY = Z^T AmBIv
+                rowscale_pow(s, ApB, w, nS, 0.5);
+                QK_HIP(hipMemcpyAsync(XmY, ApB, sizeof(double) * S2, hipMemcpyDeviceToHost, s));
+            }
+            QK_HIP(hipMemcpyAsync(Om, w, sizeof(double) * nS, hipMemcpyDeviceToHost, s));
+            double ec = 0.0;
+            QK_HIP(hipMemcpyAsync(&ec, sc + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+            QK_HIP(hipStreamSynchronize(s));
+            if (EcRPA) *EcRPA = ec;
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            for (double* p : bufs) (void)hipFree(p);
+            throw;
+        }
+        for (double* p : bufs) (void)hipFree(p);
+    });
+}
+
+int qk_rgw_excitation_density(int nOrb, int nC, int nO, int nR, int nS, const double* ERI, const double* XpY,
+                              double* rho) {
+    if (!ERI || !XpY || !rho) return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        check_sizes(nOrb, nOrb, nC, nO, nOrb - nO, nR, nS);
+        hipStream_t s = scratch().s;
+        size_t n = nOrb, n2 = n * n;
+        double *M = nullptr, *X = nullptr, *R = nullptr;
+        try {
+            M = dalloc<double>(n2 * nS);
+            X = dalloc<double>((size_t)nS * nS);
+            R = dalloc<double>(n2 * nS);
+            upload_mo_block(s, ERI, M, nOrb, nC, nO, nR);
+            QK_HIP(hipMemcpyAsync(X, XpY, sizeof(double) * nS * nS, hipMemcpyHostToDevice, s));
+            // GW/RGW_excitation_density.f90:48-50: dgemm('N','T', nOrb^2, nS, nS)
+            dgemm(s, false, true, (int)n2, nS, nS, 1.0, M, n2, X, nS, 0.0, R, n2);
+            QK_HIP(hipMemcpyAsync(rho, R, sizeof(double) * n2 * nS, hipMemcpyDeviceToHost, s));
+            QK_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            dfree(M); dfree(X); dfree(R);
+            throw;
+        }
+        dfree(M); dfree(X); dfree(R);
+    });
+}
+
+namespace {
+// upload rho(p,q,m) (QuAcK layout) into rhoT[m + nS (p + n q)]
+double* upload_rho(hipStream_t s, const double* rho, int n, int nS) {
+    size_t n2 = (size_t)n * n;
+    double* tmp = dalloc<double>(n2 * nS);
+    double* out = nullptr;
+    try {
+        out = dalloc<double>(n2 * nS);
+        QK_HIP(hipMemcpyAsync(tmp, rho, sizeof(double) * n2 * nS, hipMemcpyHostToDevice, s));
+        transpose(s, tmp, out, (int)n2, nS, n2, nS);
+        QK_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        dfree(tmp); dfree(out);
+        throw;
+    }
+    dfree(tmp);
+    return out;
+}
+}  // namespace
+
+int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double* e,
+                            const double* Om, const double* rho, double* EcGM, double* SigC, double* Z) {
+    if (!e || !Om || !rho || !SigC || !Z) return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        check_sizes(nOrb, nOrb, nC, nO, nV, nR, nS);
+        hipStream_t s = scratch().s;
+        int n = nOrb;
+        int nch = sigma_chunks(nS, n - nR - nC);
+        std::vector<double*> bufs;
+        auto get = [&](size_t cnt) { bufs.push_back(dalloc<double>(cnt)); return bufs.back(); };
+        try {
+            double* rT = upload_rho(s, rho, n, nS);
+            bufs.push_back(rT);
+            double* de = get(n);
+            double* dO = get(nS);
+            double* part = get((size_t)3 * n * nch);
+            double* dS = get(n);
+            double* dZ = get(n);
+            double* sc = get(4);
+            QK_HIP(hipMemcpyAsync(de, e, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
+            sigma_partial(s, rT, de, de, dO, nullptr, eta, doSRG, SRG_FLOW, n, nC, nO, nR, nS, nch, part);
+            sigma_finalize(s, part, nch, n, nC, nO, nR, dS, dZ, sc);
+            QK_HIP(hipMemcpyAsync(SigC, dS, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            QK_HIP(hipMemcpyAsync(Z, dZ, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            double gm = 0.0;
+            QK_HIP(hipMemcpyAsync(&gm, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+            QK_HIP(hipStreamSynchronize(s));
+            if (EcGM) *EcGM = gm;
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            for (double* p : bufs) (void)hipFree(p);
+            throw;
+        }
+        for (double* p : bufs) (void)hipFree(p);
+    });
+}
+
+int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO, int nV, int nR, int nS,
+                    const double* eHF, const double* Om, const double* rho, const double* eGWlin, const double* eOld,
+                    double* eGW, double* Z) {
+    if (!eHF || !Om || !rho || !eGWlin || !eOld || !eGW || !Z) return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        check_sizes(nOrb, nOrb, nC, nO, nV, nR, nS);
+        hipStream_t s = scratch().s;
+        int n = nOrb;
+        int nch = sigma_chunks(nS, n - nR - nC);
+        std::vector<void*> bufs;
+        auto get = [&](size_t cnt) { double* p = dalloc<double>(cnt); bufs.push_back(p); return p; };
+        auto geti = [&](size_t cnt) { int* p = dalloc<int>(cnt); bufs.push_back(p); return p; };
+        try {
+            double* rT = upload_rho(s, rho, n, nS);
+            bufs.push_back(rT);
+            double* d0 = get(n);
+            double* dG = get(n);
+            double* dL = get(n);
+            double* dO = get(nS);
+            double* w = get(n);
+            double* f = get(n);
+            double* df = get(n);
+            double* dE = get(n);
+            double* dZ = get(n);
+            double* part = get((size_t)3 * n * nch);
+            int* nit = geti(n);
+            int* done = geti(n);
+            int* act = geti(1);
+            QK_HIP(hipMemcpyAsync(d0, eHF, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            QK_HIP(hipMemcpyAsync(dG, eOld, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            QK_HIP(hipMemcpyAsync(dL, eGWlin, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
+            QK_HIP(hipMemcpyAsync(dZ, Z, sizeof(double) * n, hipMemcpyHostToDevice, s));
+            qp_init(s, dL, w, f, nit, done, n, nC, nR);
+            int h_active = 1;
+            for (int it = 0; it < QP_MAXIT && h_active > 0; ++it) {
+                sigma_partial(s, rT, w, dG, dO, done, eta, doSRG, flow, n, nC, nO, nR, nS, nch, part);
+                qp_update(s, part, nch, d0, w, f, df, nit, done, n, QP_THRESH, QP_MAXIT, act);
+                QK_HIP(hipMemcpyAsync(&h_active, act, sizeof(int), hipMemcpyDeviceToHost, s));
+                QK_HIP(hipStreamSynchronize(s));
+            }
+            qp_finish(s, w, df, nit, dL, dE, dZ, n, nC, nR, QP_MAXIT);
+            QK_HIP(hipMemcpyAsync(eGW, dE, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            QK_HIP(hipMemcpyAsync(Z, dZ, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            QK_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            for (void* p : bufs) (void)hipFree(p);
+            throw;
+        }
+        for (void* p : bufs) (void)hipFree(p);
+    });
+}
+
+}  // extern "C"

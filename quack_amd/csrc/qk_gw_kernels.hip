@@ -1,0 +1,432 @@
+// HBM-streaming FP64 kernels of the restricted GW path (gfx950).
+//
+// Index conventions on the device (all column-major, 0-based):
+//   M[p + n q + n^2 jb]        (pq|jb), the only ERI block the path needs
+//   K[ia + nS jb]              (ia|jb), ia = a' + v*i', jb = b' + v*j'
+//   Cm[ia + nS jb]             RPA matrix, then its eigenvectors (syevd)
+//   rhoT[m + nS (p + n q)]     excitation densities rho(p,q,m): contiguous in m
+//   where n = nOrb, o = nO - nC, v = nOrb - nR - nO, nS = o*v, i = nC + i',
+//   a = nO + a'.
+#include "qk_kernels.h"
+
+namespace qk {
+namespace {
+
+constexpr int RB = 256;  // reduction block
+
+__device__ inline double block_sum(double v, double* sh) {
+    // wavefront reduction (64 lanes) then across the waves of the block
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) r += sh[k];
+    return r;  // valid in thread 0
+}
+
+// ---- ERI block extraction -------------------------------------------------
+
+// X[p + n j' + n o q + n o n b'] = ERI(p, nC+j', q, nO+b')  ->  M[p + n q + n^2 (b' + v j')]
+__global__ void k_permute_block(const double* __restrict__ X, double* __restrict__ M, int n, int o, int v) {
+    long long total = (long long)n * n * o * v;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        long long p = t % n, r = t / n;
+        long long q = r % n; r /= n;
+        long long jb = r;  // b' + v j'
+        long long b = jb % v, j = jb / v;
+        M[t] = X[p + n * (j + o * (q + (long long)n * b))];
+    }
+}
+
+// K[ia + nS jb] = M[(nC+i') + n (nO+a') + n^2 jb]
+__global__ void k_extract_k(const double* __restrict__ M, double* __restrict__ K, int n, int nC, int nO, int v, int nS) {
+    long long total = (long long)nS * nS;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        long long ia = t % nS, jb = t / nS;
+        long long a = ia % v, i = ia / v;
+        K[t] = M[(nC + i) + (long long)n * (nO + a) + (long long)n * n * jb];
+    }
+}
+
+// ---- RPA matrix ---------------------------------------------------------
+
+// dE[ia] = e_a - e_i ; sd[ia] = sqrt(dE)
+__global__ void k_delta(const double* __restrict__ e, double* __restrict__ dE, double* __restrict__ sd, int nC, int nO,
+                        int v, int nS) {
+    int ia = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ia < nS) {
+        int a = ia % v, i = ia / v;
+        double d = e[nO + a] - e[nC + i];
+        dE[ia] = d;
+        sd[ia] = sqrt(d);
+    }
+}
+
+// TDA: Cm = A = diag(dE) + 2K ; RPA: Cm = (A-B)^1/2 (A+B) (A-B)^1/2 = diag(dE^2) + 4 sd K sd
+__global__ void k_build_c(const double* __restrict__ K, const double* __restrict__ dE, const double* __restrict__ sd,
+                          double* __restrict__ Cm, int nS, int tda) {
+    long long total = (long long)nS * nS;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int ia = (int)(t % nS), jb = (int)(t / nS);
+        double k = K[t];
+        double c;
+        if (tda) {
+            c = 2.0 * k;
+            if (ia == jb) c += dE[ia];
+        } else {
+            c = 4.0 * sd[ia] * k * sd[jb];
+            if (ia == jb) c += dE[ia] * dE[ia];
+        }
+        Cm[t] = c;
+    }
+}
+
+// out[0] = Tr(A) = sum_ia dE[ia] + 2 K[ia,ia]
+__global__ void k_trace_a(const double* __restrict__ K, const double* __restrict__ dE, int nS, double* out) {
+    __shared__ double sh[RB / 64];
+    double s = 0.0;
+    for (int ia = threadIdx.x; ia < nS; ia += blockDim.x) s += dE[ia] + 2.0 * K[(long long)ia * nS + ia];
+    double r = block_sum(s, sh);
+    if (threadIdx.x == 0) out[0] = r;
+}
+
+// Om = sqrt|lambda| (RPA) or lambda (TDA); out[1] = EcRPA = (sum Om - TrA)/2 with TrA = out[0]
+__global__ void k_finalize_om(double* __restrict__ Om, int nS, int tda, double* out) {
+    __shared__ double sh[RB / 64];
+    double s = 0.0;
+    for (int m = threadIdx.x; m < nS; m += blockDim.x) {
+        double w = Om[m];
+        if (!tda) w = sqrt(fabs(w));
+        Om[m] = w;
+        s += w;
+    }
+    double r = block_sum(s, sh);
+    if (threadIdx.x == 0) out[1] = 0.5 * (r - out[0]);
+}
+
+// W[jb + nS m] = Z[jb + nS m] * sd[jb] / sqrt(Om[m])   (so that rho = M W)
+__global__ void k_scale_w(double* __restrict__ W, const double* __restrict__ sd, const double* __restrict__ Om, int nS) {
+    long long total = (long long)nS * nS;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int jb = (int)(t % nS), m = (int)(t / nS);
+        W[t] = W[t] * sd[jb] / sqrt(Om[m]);
+    }
+}
+
+// ---- self-energy --------------------------------------------------------
+
+// Per (p, chunk of m): partial sums over q and m of
+//   occupied q:  eps = w_p - e_q + Om_m ;  virtual q: eps = w_p - e_q - Om_m
+//   sig += 2 rho^2 eps/(eps^2+eta^2) ; dsig -= 2 rho^2 (eps^2-eta^2)/(eps^2+eta^2)^2
+// (SRG: sig += 2 rho^2 (1-exp(-2 s eps^2))/eps, dsig -= 2 rho^2 (1-exp(-2 s eps^2))/eps^2).
+// part[3*(p*nchunk + c) + {0,1,2}] = {sig_occ, sig_vir, dsig}.
+// p runs over the active orbitals p = nC + blockIdx.y; skip[p] != 0 exits.
+__global__ __launch_bounds__(RB) void k_sigma_partial(const double* __restrict__ rhoT, const double* __restrict__ w,
+                                                      const double* __restrict__ e, const double* __restrict__ Om,
+                                                      const int* __restrict__ skip, double eta, int srg, double flow,
+                                                      int n, int nC, int nO, int nR, int nS, int chunk,
+                                                      double* __restrict__ part) {
+    const int p = nC + blockIdx.y;
+    const int c = blockIdx.x;
+    const int nchunk = gridDim.x;
+    if (skip && skip[p]) return;
+    const double wp = w[p];
+    const int m0 = c * chunk, m1 = min(nS, m0 + chunk);
+    const double eta2 = eta * eta;
+    double so = 0.0, sv = 0.0, ds = 0.0;
+    for (int q = nC; q < n - nR; ++q) {
+        const double sgn = q < nO ? 1.0 : -1.0;
+        const double wq = wp - e[q];
+        const double* rq = rhoT + (long long)nS * (p + (long long)n * q);  // rho(p, q, :)
+        double s_acc = 0.0, d_acc = 0.0;
+        for (int m = m0 + threadIdx.x; m < m1; m += RB) {
+            double r = rq[m];
+            double num = 2.0 * r * r;
+            double eps = wq + sgn * Om[m];
+            double e2 = eps * eps;
+            if (!srg) {
+                double den = 1.0 / (e2 + eta2);
+                s_acc += num * eps * den;
+                d_acc -= num * (e2 - eta2) * den * den;
+            } else {
+                double reg = num * (1.0 - exp(-2.0 * flow * e2)) / eps;
+                s_acc += reg;
+                d_acc -= reg / eps;
+            }
+        }
+        if (q < nO) so += s_acc; else sv += s_acc;
+        ds += d_acc;
+    }
+    __shared__ double sh[RB / 64];
+    double r0 = block_sum(so, sh);
+    double r1 = block_sum(sv, sh);
+    double r2 = block_sum(ds, sh);
+    if (threadIdx.x == 0) {
+        double* o = part + 3LL * ((long long)p * nchunk + c);
+        o[0] = r0; o[1] = r1; o[2] = r2;
+    }
+}
+
+// Static self-energy (RGW_self_energy_diag): Sig, Z = 1/(1-dSig) for active p;
+// zero outside (GW/RGW_self_energy_diag.f90:115-116,191); EcGM = -2 sum_{p virtual} sig_occ(p)
+// (the same eps = e_a - e_i + Om_m as GW/RGW_self_energy_diag.f90:179-181).
+__global__ void k_sigma_finalize(const double* __restrict__ part, int nchunk, int n, int nC, int nO, int nR,
+                                 double* __restrict__ Sig, double* __restrict__ Z, double* out) {
+    __shared__ double sh[RB / 64];
+    double gm = 0.0;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        double s = 0.0, d = 0.0, so = 0.0;
+        if (p >= nC && p < n - nR) {
+            for (int c = 0; c < nchunk; ++c) {
+                const double* o = part + 3LL * ((long long)p * nchunk + c);
+                so += o[0];
+                s += o[0] + o[1];
+                d += o[2];
+            }
+            if (p >= nO) gm += so;
+        }
+        Sig[p] = s;
+        Z[p] = 1.0 / (1.0 - d);
+    }
+    double r = block_sum(gm, sh);
+    if (threadIdx.x == 0) out[2] = -2.0 * r;
+}
+
+// eGWlin = e0 + Z*Sig (GW/RG0W0.f90:188) ; or e0 + Sig (evRGW.f90:147, zflag = 0)
+__global__ void k_lin(const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) out[p] = e0[p] + (zflag ? Z[p] * Sig[p] : Sig[p]);
+}
+
+// ---- QP Newton (GW/RGW_QP_graph.f90:50-92) --------------------------------
+
+__global__ void k_qp_init(const double* __restrict__ eGWlin, double* __restrict__ w, double* __restrict__ f,
+                          int* __restrict__ nit, int* __restrict__ done, int n, int nC, int nR) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) {
+        w[p] = eGWlin[p];
+        f[p] = 1.0;
+        nit[p] = 0;
+        done[p] = (p < nC || p >= n - nR) ? 1 : 0;
+    }
+}
+
+// One Newton step for every active orbital; active[0] = number still active.
+__global__ void k_qp_update(const double* __restrict__ part, int nchunk, const double* __restrict__ eHF,
+                            double* __restrict__ w, double* __restrict__ f, double* __restrict__ df,
+                            int* __restrict__ nit, int* __restrict__ done, int n, double thresh, int maxit,
+                            int* __restrict__ active) {
+    __shared__ double sh[RB / 64];
+    double cnt = 0.0;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        if (done[p]) continue;
+        double s = 0.0, d = 0.0;
+        for (int c = 0; c < nchunk; ++c) {
+            const double* o = part + 3LL * ((long long)p * nchunk + c);
+            s += o[0] + o[1];
+            d += o[2];
+        }
+        int it = nit[p] + 1;
+        double wp = w[p];
+        double fp = wp - eHF[p] - s;
+        double dfp = 1.0 / (1.0 - d);
+        w[p] = wp - dfp * fp;
+        f[p] = fp;
+        df[p] = dfp;
+        nit[p] = it;
+        if (!(fabs(fp) > thresh && it < maxit)) done[p] = 1;
+        else cnt += 1.0;
+    }
+    double r = block_sum(cnt, sh);
+    if (threadIdx.x == 0) active[0] = (int)r;
+}
+
+// eGW(p) = w, Z(p) = df unless nIt == maxIt (then eGW = eGWlin, Z kept)
+__global__ void k_qp_finish(const double* __restrict__ w, const double* __restrict__ df, const int* __restrict__ nit,
+                            const double* __restrict__ eGWlin, double* __restrict__ eGW, double* __restrict__ Z, int n,
+                            int nC, int nR, int maxit) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (p < nC || p >= n - nR) {
+        eGW[p] = 0.0;  // QuAcK leaves frozen orbitals of eGW unset; they are never read
+        return;
+    }
+    if (nit[p] == maxit) {
+        eGW[p] = eGWlin[p];
+    } else {
+        eGW[p] = w[p];
+        Z[p] = df[p];
+    }
+}
+
+// ---- layout helpers -------------------------------------------------------
+
+// out[j + ld_out*i] = in[i + ld_in*j], tiled through LDS (rows x cols of `in`)
+__global__ void k_transpose(const double* __restrict__ in, double* __restrict__ out, int rows, int cols,
+                            long long ld_in, long long ld_out, long long bs_in, long long bs_out) {
+    __shared__ double t[32][33];
+    in += blockIdx.z * bs_in;
+    out += blockIdx.z * bs_out;
+    int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        int i = i0 + threadIdx.x, j = j0 + k;
+        if (i < rows && j < cols) t[k][threadIdx.x] = in[i + ld_in * j];
+    }
+    __syncthreads();
+    for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+        int j = j0 + threadIdx.x, i = i0 + k;
+        if (i < rows && j < cols) out[j + ld_out * i] = t[threadIdx.x][k];
+    }
+}
+
+// general phRLR helpers
+__global__ void k_axpby2(const double* A, const double* B, double* ApB, double* AmB, long long total) {
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        ApB[t] = A[t] + B[t];
+        AmB[t] = A[t] - B[t];
+    }
+}
+// U[:, k] *= d[k]^pw  (pw = +1/2 or -1/2), written to out
+__global__ void k_colscale_pow(const double* U, const double* d, double* out, int n, double pw) {
+    long long total = (long long)n * n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int k = (int)(t / n);
+        out[t] = U[t] * (pw > 0 ? sqrt(d[k]) : 1.0 / sqrt(d[k]));
+    }
+}
+// X[m + n ia] *= s[m]^pw with s = Om
+__global__ void k_rowscale_pow(double* X, const double* s, int n, double pw) {
+    long long total = (long long)n * n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int m = (int)(t % n);
+        X[t] *= (pw > 0 ? sqrt(s[m]) : 1.0 / sqrt(s[m]));
+    }
+}
+__global__ void k_trace(const double* A, int n, double* out) {
+    __shared__ double sh[RB / 64];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += A[(long long)i * n + i];
+    double r = block_sum(s, sh);
+    if (threadIdx.x == 0) out[0] = r;
+}
+
+// phRLR_A / phRLR_B (singlet) from a full ERI(n^4):
+// A(ia,jb) = 2 lam ERI(b,i,j,a) - (1-dRPA) lam ERI(b,j,a,i) + delta (e_a - e_i)
+// B(ia,jb) = 2 lam ERI(b,i,j,a) - (1-dRPA) lam ERI(b,j,i,a)
+__global__ void k_phrlr_ab(const double* __restrict__ G, const double* __restrict__ e, double* __restrict__ out, int n,
+                           int nC, int nO, int v, int nS, double lam, int dRPA, int isB) {
+    long long total = (long long)nS * nS;
+    long long n1 = n, n2 = n1 * n, n3 = n2 * n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        long long ia = t % nS, jb = t / nS;
+        long long a = nO + ia % v, i = nC + ia / v, b = nO + jb % v, j = nC + jb / v;
+        double x = 2.0 * lam * G[b + n1 * i + n2 * j + n3 * a];
+        if (!dRPA) x -= lam * (isB ? G[b + n1 * j + n2 * i + n3 * a] : G[b + n1 * j + n2 * a + n3 * i]);
+        if (!isB && ia == jb) x += e[a] - e[i];
+        out[t] = x;
+    }
+}
+
+inline int grid_for(long long total, int bs = 256) {
+    long long g = (total + bs - 1) / bs;
+    return (int)std::min<long long>(g, 65536);
+}
+
+}  // namespace
+
+// ---- host wrappers --------------------------------------------------------
+
+void permute_block(hipStream_t s, const double* X, double* M, int n, int o, int v) {
+    hipLaunchKernelGGL(k_permute_block, dim3(grid_for((long long)n * n * o * v)), dim3(256), 0, s, X, M, n, o, v);
+    QK_HIP(hipGetLastError());
+}
+void extract_k(hipStream_t s, const double* M, double* K, int n, int nC, int nO, int v, int nS) {
+    hipLaunchKernelGGL(k_extract_k, dim3(grid_for((long long)nS * nS)), dim3(256), 0, s, M, K, n, nC, nO, v, nS);
+    QK_HIP(hipGetLastError());
+}
+void rpa_build(hipStream_t s, const double* e, const double* K, double* dE, double* sd, double* Cm, double* scal,
+               int nC, int nO, int v, int nS, int tda) {
+    hipLaunchKernelGGL(k_delta, dim3((nS + 255) / 256), dim3(256), 0, s, e, dE, sd, nC, nO, v, nS);
+    hipLaunchKernelGGL(k_build_c, dim3(grid_for((long long)nS * nS)), dim3(256), 0, s, K, dE, sd, Cm, nS, tda);
+    hipLaunchKernelGGL(k_trace_a, dim3(1), dim3(RB), 0, s, K, dE, nS, scal);
+    QK_HIP(hipGetLastError());
+}
+void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w) {
+    hipLaunchKernelGGL(k_finalize_om, dim3(1), dim3(RB), 0, s, Om, nS, tda, scal);
+    if (scale_w && !tda)
+        hipLaunchKernelGGL(k_scale_w, dim3(grid_for((long long)nS * nS)), dim3(256), 0, s, W, sd, Om, nS);
+    QK_HIP(hipGetLastError());
+}
+int sigma_chunks(int nS, int nact) {
+    // aim for >= ~2048 blocks in total, each with at least 4 m per thread
+    int want = std::max(1, 2048 / std::max(1, nact));
+    int maxc = std::max(1, (nS + 4 * RB - 1) / (4 * RB));
+    return std::min(want, maxc);
+}
+void sigma_partial(hipStream_t s, const double* rhoT, const double* w, const double* e, const double* Om,
+                   const int* skip, double eta, int srg, double flow, int n, int nC, int nO, int nR, int nS,
+                   int nchunk, double* part) {
+    int nact = n - nR - nC;
+    int chunk = (nS + nchunk - 1) / nchunk;
+    hipLaunchKernelGGL(k_sigma_partial, dim3(nchunk, nact), dim3(RB), 0, s, rhoT, w, e, Om, skip, eta, srg, flow, n,
+                       nC, nO, nR, nS, chunk, part);
+    QK_HIP(hipGetLastError());
+}
+void sigma_finalize(hipStream_t s, const double* part, int nchunk, int n, int nC, int nO, int nR, double* Sig,
+                    double* Z, double* scal) {
+    hipLaunchKernelGGL(k_sigma_finalize, dim3(1), dim3(RB), 0, s, part, nchunk, n, nC, nO, nR, Sig, Z, scal);
+    QK_HIP(hipGetLastError());
+}
+void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag) {
+    hipLaunchKernelGGL(k_lin, dim3((n + 255) / 256), dim3(256), 0, s, e0, Z, Sig, out, n, zflag);
+    QK_HIP(hipGetLastError());
+}
+void qp_init(hipStream_t s, const double* eGWlin, double* w, double* f, int* nit, int* done, int n, int nC, int nR) {
+    hipLaunchKernelGGL(k_qp_init, dim3((n + 255) / 256), dim3(256), 0, s, eGWlin, w, f, nit, done, n, nC, nR);
+    QK_HIP(hipGetLastError());
+}
+void qp_update(hipStream_t s, const double* part, int nchunk, const double* eHF, double* w, double* f, double* df,
+               int* nit, int* done, int n, double thresh, int maxit, int* active) {
+    hipLaunchKernelGGL(k_qp_update, dim3(1), dim3(RB), 0, s, part, nchunk, eHF, w, f, df, nit, done, n, thresh, maxit,
+                       active);
+    QK_HIP(hipGetLastError());
+}
+void qp_finish(hipStream_t s, const double* w, const double* df, const int* nit, const double* eGWlin, double* eGW,
+               double* Z, int n, int nC, int nR, int maxit) {
+    hipLaunchKernelGGL(k_qp_finish, dim3((n + 255) / 256), dim3(256), 0, s, w, df, nit, eGWlin, eGW, Z, n, nC, nR,
+                       maxit);
+    QK_HIP(hipGetLastError());
+}
+void transpose(hipStream_t s, const double* in, double* out, int rows, int cols, long long ld_in, long long ld_out,
+               int batch, long long bs_in, long long bs_out) {
+    dim3 g((rows + 31) / 32, (cols + 31) / 32, batch);
+    hipLaunchKernelGGL(k_transpose, g, dim3(32, 8), 0, s, in, out, rows, cols, ld_in, ld_out, bs_in, bs_out);
+    QK_HIP(hipGetLastError());
+}
+void apb_amb(hipStream_t s, const double* A, const double* B, double* ApB, double* AmB, long long total) {
+    hipLaunchKernelGGL(k_axpby2, dim3(grid_for(total)), dim3(256), 0, s, A, B, ApB, AmB, total);
+    QK_HIP(hipGetLastError());
+}
+void colscale_pow(hipStream_t s, const double* U, const double* d, double* out, int n, double pw) {
+    hipLaunchKernelGGL(k_colscale_pow, dim3(grid_for((long long)n * n)), dim3(256), 0, s, U, d, out, n, pw);
+    QK_HIP(hipGetLastError());
+}
+void rowscale_pow(hipStream_t s, double* X, const double* sv, int n, double pw) {
+    hipLaunchKernelGGL(k_rowscale_pow, dim3(grid_for((long long)n * n)), dim3(256), 0, s, X, sv, n, pw);
+    QK_HIP(hipGetLastError());
+}
+void trace(hipStream_t s, const double* A, int n, double* out) {
+    hipLaunchKernelGGL(k_trace, dim3(1), dim3(RB), 0, s, A, n, out);
+    QK_HIP(hipGetLastError());
+}
+void phrlr_ab(hipStream_t s, const double* G, const double* e, double* out, int n, int nC, int nO, int v, int nS,
+              double lam, int dRPA, int isB) {
+    hipLaunchKernelGGL(k_phrlr_ab, dim3(grid_for((long long)nS * nS)), dim3(256), 0, s, G, e, out, n, nC, nO, v, nS,
+                       lam, dRPA, isB);
+    QK_HIP(hipGetLastError());
+}
+
+}  // namespace qk

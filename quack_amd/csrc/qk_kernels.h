@@ -1,0 +1,36 @@
+// Host-side launchers of the GW-path kernels (qk_gw_kernels.hip).
+#pragma once
+#include <algorithm>
+
+#include "qk_common.h"
+
+namespace qk {
+
+void permute_block(hipStream_t s, const double* X, double* M, int n, int o, int v);
+void extract_k(hipStream_t s, const double* M, double* K, int n, int nC, int nO, int v, int nS);
+// scal[0] = Tr(A), scal[1] = EcRPA, scal[2] = EcGM
+void rpa_build(hipStream_t s, const double* e, const double* K, double* dE, double* sd, double* Cm, double* scal,
+               int nC, int nO, int v, int nS, int tda);
+void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w);
+int sigma_chunks(int nS, int nact);
+void sigma_partial(hipStream_t s, const double* rhoT, const double* w, const double* e, const double* Om,
+                   const int* skip, double eta, int srg, double flow, int n, int nC, int nO, int nR, int nS,
+                   int nchunk, double* part);
+void sigma_finalize(hipStream_t s, const double* part, int nchunk, int n, int nC, int nO, int nR, double* Sig,
+                    double* Z, double* scal);
+void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag);
+void qp_init(hipStream_t s, const double* eGWlin, double* w, double* f, int* nit, int* done, int n, int nC, int nR);
+void qp_update(hipStream_t s, const double* part, int nchunk, const double* eHF, double* w, double* f, double* df,
+               int* nit, int* done, int n, double thresh, int maxit, int* active);
+void qp_finish(hipStream_t s, const double* w, const double* df, const int* nit, const double* eGWlin, double* eGW,
+               double* Z, int n, int nC, int nR, int maxit);
+void transpose(hipStream_t s, const double* in, double* out, int rows, int cols, long long ld_in, long long ld_out,
+               int batch = 1, long long bs_in = 0, long long bs_out = 0);
+void apb_amb(hipStream_t s, const double* A, const double* B, double* ApB, double* AmB, long long total);
+void colscale_pow(hipStream_t s, const double* U, const double* d, double* out, int n, double pw);
+void rowscale_pow(hipStream_t s, double* X, const double* sv, int n, double pw);
+void trace(hipStream_t s, const double* A, int n, double* out);
+void phrlr_ab(hipStream_t s, const double* G, const double* e, double* out, int n, int nC, int nO, int v, int nS,
+              double lam, int dRPA, int isB);
+
+}  // namespace qk

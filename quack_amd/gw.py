@@ -1,0 +1,266 @@
+"""Python host mirror of QuAcK's restricted GW entry points over the HIP C-ABI.
+
+The functions keep the names, argument meaning and output of QuAcK's Fortran
+routines (GW/RG0W0.f90, GW/evRGW.f90, LR/phRLR*.f90, GW/RGW_*.f90,
+AOtoMO/AOtoMO_ERI_RHF.f90) and call libquack_amd.so, the gfx950 build of
+quack_amd/csrc.  There is no CPU fallback: a missing library or a device
+error raises.
+
+Arrays are numpy FP64.  4-index integrals are the flat column-major arrays
+QuAcK holds (``G(p,q,r,s) = <pq|rs>`` at ``p + n q + n^2 r + n^3 s``); 2-D
+arrays are returned in Fortran order with QuAcK's index meaning
+(``XpY[m, ia]``, ``rho[p, q, m]``).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libquack_amd.so")
+
+QK_OK, QK_E_ARG, QK_E_HIP, QK_E_EIG, QK_E_NOCONV, QK_E_UNSUPPORTED, QK_E_ALLOC = range(7)
+_NAMES = {1: "QK_E_ARG", 2: "QK_E_HIP", 3: "QK_E_EIG", 4: "QK_E_NOCONV", 5: "QK_E_UNSUPPORTED", 6: "QK_E_ALLOC"}
+
+EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_ao", "qk_gw_set_eri_ao_device",
+           "qk_gw_set_eri_mo", "qk_rg0w0", "qk_evrgw", "qk_gw_stage_times", "qk_aotomo_eri_rhf", "qk_phrlr",
+           "qk_phrlr_a", "qk_phrlr_b", "qk_rgw_excitation_density", "qk_rgw_self_energy_diag", "qk_rgw_qp_graph",
+           "qk_last_error"]
+
+
+class QKError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (_NAMES.get(code, str(code)), msg))
+        self.code = code
+
+
+class NotConverged(QKError):
+    """evGW reached maxSCF+1 cycles (QuAcK prints 'Convergence failed' and stops)."""
+
+    def __init__(self, code, msg, result):
+        super().__init__(code, msg)
+        self.result = result
+
+
+_I, _D = ctypes.c_int, ctypes.c_double
+_P = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_H = ctypes.c_void_p
+_lib = None
+
+
+def library_path():
+    return LIB_PATH
+
+
+def load():
+    """Load libquack_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libquack_amd.so is not built; run __graft_entry__.build() (no CPU fallback exists)")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.qk_gw_create.argtypes = [ctypes.POINTER(_H)] + [_I] * 7
+    lib.qk_gw_destroy.argtypes = [_H]
+    lib.qk_gw_last_error.argtypes = [_H]
+    lib.qk_gw_last_error.restype = ctypes.c_char_p
+    lib.qk_last_error.restype = ctypes.c_char_p
+    lib.qk_gw_set_eri_ao.argtypes = [_H, _P, _P]
+    lib.qk_gw_set_eri_ao_device.argtypes = [_H, _H, _H]
+    lib.qk_gw_set_eri_mo.argtypes = [_H, _P]
+    lib.qk_rg0w0.argtypes = [_H, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
+    lib.qk_evrgw.argtypes = [_H, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
+    lib.qk_gw_stage_times.argtypes = [_H, _P, _I]
+    lib.qk_aotomo_eri_rhf.argtypes = [_I, _I, _P, _P, _P]
+    lib.qk_phrlr.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P]
+    lib.qk_phrlr_a.argtypes = [_I] * 7 + [_D, _P, _P, _P]
+    lib.qk_phrlr_b.argtypes = [_I] * 7 + [_D, _P, _P]
+    lib.qk_rgw_excitation_density.argtypes = [_I] * 5 + [_P, _P, _P]
+    lib.qk_rgw_self_energy_diag.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
+    lib.qk_rgw_qp_graph.argtypes = [_I, _D, _D] + [_I] * 6 + [_P] * 7
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_P)
+
+
+def _f64(a, order="F"):
+    return np.require(np.asarray(a, dtype=np.float64), requirements=[order, "A"])
+
+
+def _check(rc, handle=None):
+    if rc != QK_OK:
+        lib = load()
+        msg = (lib.qk_gw_last_error(handle) if handle else lib.qk_last_error()) or b""
+        raise QKError(rc, msg.decode(errors="replace"))
+
+
+def dims(nOrb, nO, nC=0, nR=0):
+    nV = nOrb - nO
+    return nC, nO, nV, nR, (nO - nC) * (nV - nR)
+
+
+class GWEngine:
+    """Device-resident G0W0/evGW state for one molecule (one qk_gw_handle)."""
+
+    def __init__(self, nBas, nOrb, nO, nC=0, nR=0):
+        lib = load()
+        self.nBas, self.nOrb = nBas, nOrb
+        self.nC, self.nO, self.nV, self.nR, self.nS = dims(nOrb, nO, nC, nR)
+        h = _H()
+        _check(lib.qk_gw_create(ctypes.byref(h), nBas, nOrb, nC, nO, self.nV, nR, self.nS))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().qk_gw_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_eri_ao(self, eri_ao, c):
+        eri_ao = np.ascontiguousarray(eri_ao, dtype=np.float64).reshape(-1)
+        assert eri_ao.size == self.nBas ** 4
+        c = _f64(c)
+        assert c.shape == (self.nBas, self.nOrb)
+        _check(load().qk_gw_set_eri_ao(self.h, _p(eri_ao), _p(c)), self.h)
+
+    def set_eri_ao_device(self, eri_ptr, c_ptr):
+        """Device pointers (e.g. torch tensor.data_ptr()) of ERI_AO and c (column-major)."""
+        _check(load().qk_gw_set_eri_ao_device(self.h, _H(eri_ptr), _H(c_ptr)), self.h)
+
+    def set_eri_mo(self, eri_mo):
+        eri_mo = np.ascontiguousarray(eri_mo, dtype=np.float64).reshape(-1)
+        assert eri_mo.size == self.nOrb ** 4
+        _check(load().qk_gw_set_eri_mo(self.h, _p(eri_mo)), self.h)
+
+    def g0w0(self, eHF, eta=0.0, linearize=False, TDA_W=False, doSRG=False):
+        n, nS = self.nOrb, self.nS
+        eHF = _f64(eHF)
+        eGW, Sig, Z, Om = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(nS)
+        Ec, Egm = _D(0.0), _D(0.0)
+        rc = load().qk_rg0w0(self.h, _p(eHF), eta, int(linearize), int(TDA_W), int(doSRG), _p(eGW), _p(Sig), _p(Z),
+                             _p(Om), ctypes.byref(Ec), ctypes.byref(Egm))
+        _check(rc, self.h)
+        return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": Ec.value, "EcGM": Egm.value}
+
+    def evgw(self, eHF, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, linearize=False, TDA_W=False, doSRG=False,
+             eGW_start=None):
+        n, nS = self.nOrb, self.nS
+        eHF = _f64(eHF)
+        start = None if eGW_start is None else _f64(eGW_start)
+        eGW, Sig, Z, Om = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(nS)
+        Ec, Egm, ns = _D(0.0), _D(0.0), _I(0)
+        rc = load().qk_evrgw(self.h, _p(eHF), _p(start), maxSCF, thresh, max_diis, eta, int(linearize), int(TDA_W),
+                             int(doSRG), _p(eGW), _p(Sig), _p(Z), _p(Om), ctypes.byref(Ec), ctypes.byref(Egm),
+                             ctypes.byref(ns))
+        res = {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": Ec.value, "EcGM": Egm.value, "nSCF": ns.value}
+        if rc == QK_E_NOCONV:
+            raise NotConverged(rc, load().qk_gw_last_error(self.h).decode(), res)
+        _check(rc, self.h)
+        return res
+
+    def stage_times(self):
+        ms = np.zeros(6)
+        _check(load().qk_gw_stage_times(self.h, _p(ms), 6), self.h)
+        return dict(zip(["aotomo", "rpa_build", "eigensolve", "excitation", "sigma_qp", "total"], ms))
+
+
+# ---- QuAcK-named entry points -------------------------------------------------
+
+def RG0W0(ERI, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False, nC=0, nR=0):
+    """GW/RG0W0.f90 on the GPU from MO integrals ERI(nOrb^4) and eHF."""
+    n = len(eHF)
+    with GWEngine(n, n, nO, nC, nR) as g:
+        g.set_eri_mo(ERI)
+        return g.g0w0(eHF, eta, linearize, TDA_W, doSRG)
+
+
+def evRGW(ERI, eHF, nO, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, linearize=False, TDA_W=False, doSRG=False,
+          nC=0, nR=0):
+    """GW/evRGW.f90 on the GPU from MO integrals ERI(nOrb^4) and eHF."""
+    n = len(eHF)
+    with GWEngine(n, n, nO, nC, nR) as g:
+        g.set_eri_mo(ERI)
+        return g.evgw(eHF, maxSCF, thresh, max_diis, eta, linearize, TDA_W, doSRG)
+
+
+def AOtoMO_ERI_RHF(c, ERI_AO):
+    c = _f64(c)
+    nBas, nOrb = c.shape
+    eri_ao = np.ascontiguousarray(ERI_AO, dtype=np.float64).reshape(-1)
+    out = np.zeros(nOrb ** 4)
+    _check(load().qk_aotomo_eri_rhf(nBas, nOrb, _p(c), _p(eri_ao), _p(out)))
+    return out
+
+
+def phRLR_A(e, ERI, nO, lam=1.0, dRPA=True, nC=0, nR=0):
+    n = len(e)
+    nC, nO, nV, nR, nS = dims(n, nO, nC, nR)
+    A = np.zeros((nS, nS), order="F")
+    eri = np.ascontiguousarray(ERI, dtype=np.float64).reshape(-1)
+    _check(load().qk_phrlr_a(int(dRPA), n, nC, nO, nV, nR, nS, lam, _p(_f64(e)), _p(eri), _p(A)))
+    return A
+
+
+def phRLR_B(ERI, n, nO, lam=1.0, dRPA=True, nC=0, nR=0):
+    nC, nO, nV, nR, nS = dims(n, nO, nC, nR)
+    B = np.zeros((nS, nS), order="F")
+    eri = np.ascontiguousarray(ERI, dtype=np.float64).reshape(-1)
+    _check(load().qk_phrlr_b(int(dRPA), n, nC, nO, nV, nR, nS, lam, _p(eri), _p(B)))
+    return B
+
+
+def phRLR(TDA, A, B):
+    """LR/phRLR.f90: returns EcRPA, Om, XpY, XmY."""
+    A = _f64(A)
+    nS = A.shape[0]
+    B = A if (TDA or B is None) else _f64(B)
+    Om = np.zeros(nS)
+    XpY = np.zeros((nS, nS), order="F")
+    XmY = np.zeros((nS, nS), order="F")
+    Ec = _D(0.0)
+    _check(load().qk_phrlr(int(TDA), nS, _p(A), _p(B), ctypes.byref(Ec), _p(Om), _p(XpY), _p(XmY)))
+    return Ec.value, Om, XpY, XmY
+
+
+def RGW_excitation_density(ERI, XpY, nOrb, nO, nC=0, nR=0):
+    nC, nO, nV, nR, nS = dims(nOrb, nO, nC, nR)
+    rho = np.zeros((nOrb, nOrb, nS), order="F")
+    eri = np.ascontiguousarray(ERI, dtype=np.float64).reshape(-1)
+    _check(load().qk_rgw_excitation_density(nOrb, nC, nO, nR, nS, _p(eri), _p(_f64(XpY)), _p(rho)))
+    return rho
+
+
+def RGW_self_energy_diag(e, Om, rho, nO, eta=0.0, doSRG=False, nC=0, nR=0):
+    n = len(e)
+    nC, nO, nV, nR, nS = dims(n, nO, nC, nR)
+    Sig, Z = np.zeros(n), np.zeros(n)
+    Ec = _D(0.0)
+    _check(load().qk_rgw_self_energy_diag(int(doSRG), eta, n, nC, nO, nV, nR, nS, _p(_f64(e)), _p(_f64(Om)),
+                                          _p(_f64(rho)), ctypes.byref(Ec), _p(Sig), _p(Z)))
+    return Sig, Z, Ec.value
+
+
+def RGW_QP_graph(eHF, Om, rho, eGWlin, eOld, Z, nO, eta=0.0, doSRG=False, flow=500.0, nC=0, nR=0):
+    n = len(eHF)
+    nC, nO, nV, nR, nS = dims(n, nO, nC, nR)
+    eGW = np.zeros(n)
+    Zio = np.array(Z, dtype=np.float64)
+    _check(load().qk_rgw_qp_graph(int(doSRG), eta, flow, n, nC, nO, nV, nR, nS, _p(_f64(eHF)), _p(_f64(Om)),
+                                  _p(_f64(rho)), _p(_f64(eGWlin)), _p(_f64(eOld)), _p(eGW), _p(Zio)))
+    return eGW, Zio

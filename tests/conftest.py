@@ -1,0 +1,34 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and libquack_amd.so")
+
+
+@pytest.fixture(scope="session")
+def systems():
+    """Small closed-shell systems with integrals and RHF from the producer."""
+    from quack_amd import intgen as ig
+    out = {}
+    for name, mol in (("He", ig.he_631g()), ("H2O", ig.h2o_feather()), ("H10", ig.h_chain(10, seed=0))):
+        out[name] = ig.system(mol)
+    return out
+
+
+@pytest.fixture(scope="session")
+def mo_systems(systems):
+    """Same systems with MO integrals (oracle restatement of AOtoMO_ERI_RHF)."""
+    from oracle import gw_oracle as go
+    out = {}
+    for k, s in systems.items():
+        d = dict(s)
+        d["eri_mo"] = go.aotomo_eri_rhf(s["c"], s["eri"])
+        out[k] = d
+    return out

@@ -1,0 +1,78 @@
+"""Writes the committed fixtures of tests/golden/ (run in a tree with the QuAcK sources).
+
+* quack_goldens.json: the scalar results QuAcK's own test files hold for this
+  path (He/6-31G and H2O/cc-pVDZ; tests/feather_bench.py:5-54,58-101 and
+  test/Rtest_ref.dat:51-62), plus the options they were produced with.
+* h10_ref.npz: outputs of QuAcK's own routines (oracle/_ref/libqkref.so,
+  compiled from the QuAcK sources by oracle/Makefile) for the H10 chain
+  workload (1.40 bohr + U(-0.10, 0.10), numpy default_rng(0), cc-pVDZ,
+  N = 50): the sign- and rotation-invariant quantities only (eHF, Om, eGW,
+  Z, SigC, EcRPA, EcGM, nSCF).
+
+    python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import qkref  # noqa: E402
+from quack_amd import intgen as ig  # noqa: E402
+
+GOLDENS = {
+    "source": {
+        "feather_bench": "tests/feather_bench.py:13-39 (He), :68-94 (H2O)",
+        "Rtest_ref": "test/Rtest_ref.dat:1-6,51-62 (He)",
+    },
+    "gw_options": {"maxSCF": 256, "thresh": 1e-5, "lin": False, "eta": 0.0, "TDA_W": False, "reg": False},
+    "He": {
+        "basis": "6-31g",
+        "feather": {"RHF energy": -2.855160426154444, "RHF HOMO energy": -0.914126628640145,
+                    "RHF LUMO energy": 1.399859335255765,
+                    "G0W0 correlation energy": -0.019314094399372, "G0W0 HOMO energy": -0.87053388021722,
+                    "G0W0 LUMO energy": 1.377171287041735, "evGW correlation energy": -0.019335511771337,
+                    "evGW HOMO energy": -0.868460640984803, "evGW LUMO energy": 1.376287581502582},
+        "Rtest_ref": {"RHF energy": -2.855160426884076, "RHF HOMO energy": -0.914126628614305,
+                      "RHF LUMO energy": 1.399859335225087,
+                      "G0W0 correlation energy": -0.019314094399756, "G0W0 HOMO energy": -0.870533880190454,
+                      "G0W0 LUMO energy": 1.377171287010956, "evGW correlation energy": -0.019335511771724,
+                      "evGW HOMO energy": -0.868460640957913, "evGW LUMO energy": 1.376287581471769},
+    },
+    "H2O": {
+        "basis": "cc-pvdz",
+        "geometry_angstrom": [["O", 0.0, 0.0, 0.0], ["H", 0.7571, 0.0, 0.5861], ["H", -0.7571, 0.0, 0.5861]],
+        "feather": {"RHF energy": -85.21935817501823, "RHF HOMO energy": -0.493132793449897,
+                    "RHF LUMO energy": 0.185534869842355,
+                    "G0W0 correlation energy": -0.23853664665404, "G0W0 HOMO energy": -0.446828623007469,
+                    "G0W0 LUMO energy": 0.173026609033024, "evGW correlation energy": -0.239414217281308,
+                    "evGW HOMO energy": -0.443076613314424, "evGW LUMO energy": 0.172691758111392},
+    },
+}
+
+
+def main():
+    with open(os.path.join(HERE, "quack_goldens.json"), "w") as f:
+        json.dump(GOLDENS, f, indent=1)
+    s = ig.system(ig.h_chain(10, seed=0))
+    nO = s["nO"]
+    mo = qkref.aotomo(s["c"], s["eri"])
+    out = {"eHF": s["eHF"], "ERHF": s["ERHF"], "nO": nO}
+    for tag, kw in (("g0w0", {}), ("g0w0_lin", {"linearize": True}), ("g0w0_srg", {"srg": True}),
+                    ("g0w0_tda", {"tda_w": True})):
+        r = qkref.rg0w0(mo, s["eHF"], nO, **kw)
+        for k in ("eGW", "SigC", "Z", "Om", "EcRPA", "EcGM"):
+            out[tag + "_" + k] = r[k]
+    for tag, kw in (("evgw", {"max_diis": 5, "thresh": 1e-6}), ("evgw_nodiis", {"max_diis": 1, "thresh": 1e-6}),
+                    ("evgw_lin", {"linearize": True, "max_diis": 5, "thresh": 1e-6, "maxSCF": 1})):
+        r = qkref.evrgw(mo, s["eHF"], nO, **kw)
+        for k in ("eGW", "SigC", "Z", "EcRPA", "EcGM", "nSCF", "status"):
+            out[tag + "_" + k] = r[k]
+    np.savez(os.path.join(HERE, "h10_ref.npz"), **{k: np.asarray(v) for k, v in out.items()})
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle and QuAcK's outputs.
+
+Tolerances (FP64 form): eGW within 1e-8 Ha of QuAcK's routines on identical
+integrals (the stated bar is 1e-6 Ha), RPA excitation energies Om within
+1e-8 Ha, everything else within 1e-9 relative unless stated.  Eigenvector
+signs are free, so only sign-invariant quantities are compared unless the
+sign is fixed first.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import gw_oracle as go
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+H10 = np.load(os.path.join(HERE, "golden", "h10_ref.npz"))
+GOLD = json.load(open(os.path.join(HERE, "golden", "quack_goldens.json")))
+
+
+@pytest.fixture(scope="module")
+def gw():
+    from quack_amd import gw as m
+    m.load()
+    return m
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name", ["H2O", "H10"])
+def test_aotomo_full(gw, systems, mo_systems, name):
+    say("aotomo", name)
+    s = systems[name]
+    out = gw.AOtoMO_ERI_RHF(s["c"], s["eri"])
+    ref = mo_systems[name]["eri_mo"]
+    assert np.abs(out - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("name", ["H2O", "H10"])
+def test_phrlr_ab_and_general(gw, mo_systems, name):
+    say("phrlr", name)
+    s = mo_systems[name]
+    e, nO, n = s["eHF"], s["nO"], len(s["eHF"])
+    for dRPA in (True, False):
+        A = gw.phRLR_A(e, s["eri_mo"], nO, dRPA=dRPA)
+        B = gw.phRLR_B(s["eri_mo"], n, nO, dRPA=dRPA)
+        assert np.abs(A - go.phRLR_A(e, s["eri_mo"], nO, dRPA=dRPA)).max() < 1e-14
+        assert np.abs(B - go.phRLR_B(s["eri_mo"], n, nO, dRPA=dRPA)).max() < 1e-14
+    A = go.phRLR_A(e, s["eri_mo"], nO)
+    B = go.phRLR_B(s["eri_mo"], n, nO)
+    for tda in (False, True):
+        Ec, Om, XpY, XmY = gw.phRLR(tda, A, B)
+        Ec0, Om0, XpY0, XmY0 = go.phRLR(tda, A, B)
+        assert np.abs(Om - Om0).max() < 1e-10
+        assert abs(Ec - Ec0) < 1e-10
+        nS = len(Om)
+        assert np.abs(XpY @ XmY.T - np.eye(nS)).max() < 1e-9
+        # per-mode sign: align on the largest component
+        k = np.argmax(np.abs(XpY0), axis=1)
+        sgn = np.sign(XpY[np.arange(nS), k] * XpY0[np.arange(nS), k])
+        gap = np.min(np.diff(Om0)) if nS > 1 else 1.0
+        if gap > 1e-6:
+            assert np.abs(sgn[:, None] * XpY - XpY0).max() < 1e-7
+            assert np.abs(sgn[:, None] * XmY - XmY0).max() < 1e-7
+
+
+@pytest.mark.parametrize("name", ["He", "H2O", "H10"])
+def test_excitation_density_sigma_qp(gw, mo_systems, name):
+    say("rho/sigma/qp", name)
+    s = mo_systems[name]
+    e, nO, n = s["eHF"], s["nO"], len(s["eHF"])
+    Ec, Om, XpY, XmY = go.screening(e, s["eri_mo"], nO)
+    rho = gw.RGW_excitation_density(s["eri_mo"], XpY, n, nO)
+    rho0 = go.excitation_density(s["eri_mo"], XpY, n, nO)
+    assert np.abs(rho - rho0).max() < 1e-12
+    for srg in (False, True):
+        eta = 0.0
+        Sig, Z, Egm = gw.RGW_self_energy_diag(e, Om, rho0, nO, eta=eta, doSRG=srg)
+        Sig0, Z0, Egm0 = (go.srg_self_energy_diag(e, Om, rho0, nO) if srg
+                          else go.self_energy_diag(e, Om, rho0, nO, eta))
+        assert np.abs(Sig - Sig0).max() < 1e-12
+        assert np.abs(Z - Z0).max() < 1e-10
+        assert abs(Egm - Egm0) < 1e-12
+        lin = e + Z0 * Sig0
+        eg, zz = gw.RGW_QP_graph(e, Om, rho0, lin, e, Z0, nO, eta=eta, doSRG=srg)
+        eg0, zz0, its = go.qp_graph(e, Om, rho0, lin, e, Z0, nO, eta, srg)
+        assert np.abs(eg - eg0).max() < 1e-10
+        assert np.abs(zz - zz0).max() < 1e-9
+
+
+def test_self_energy_eta_positive(gw, mo_systems):
+    say("sigma eta>0")
+    s = mo_systems["H2O"]
+    e, nO = s["eHF"], s["nO"]
+    Ec, Om, XpY, XmY = go.screening(e, s["eri_mo"], nO)
+    rho0 = go.excitation_density(s["eri_mo"], XpY, len(e), nO)
+    Sig, Z, Egm = gw.RGW_self_energy_diag(e, Om, rho0, nO, eta=0.05)
+    Sig0, Z0, Egm0 = go.self_energy_diag(e, Om, rho0, nO, 0.05)
+    assert np.abs(Sig - Sig0).max() < 1e-12 and np.abs(Z - Z0).max() < 1e-10 and abs(Egm - Egm0) < 1e-12
+
+
+@pytest.mark.parametrize("via", ["ao", "mo"])
+def test_g0w0_h10_vs_quack(gw, systems, mo_systems, via):
+    say("g0w0 H10", via)
+    s = systems["H10"]
+    nO, n = s["nO"], s["nOrb"]
+    with gw.GWEngine(s["nBas"], n, nO) as g:
+        if via == "ao":
+            g.set_eri_ao(s["eri"], s["c"])
+        else:
+            g.set_eri_mo(mo_systems["H10"]["eri_mo"])
+        for tag, kw in (("g0w0", {}), ("g0w0_lin", {"linearize": True}), ("g0w0_srg", {"doSRG": True}),
+                        ("g0w0_tda", {"TDA_W": True})):
+            r = g.g0w0(s["eHF"], **kw)
+            assert np.abs(r["eGW"] - H10[tag + "_eGW"]).max() < 1e-8, tag
+            assert np.abs(r["Om"] - H10[tag + "_Om"]).max() < 1e-8, tag
+            assert np.abs(r["SigC"] - H10[tag + "_SigC"]).max() < 1e-9, tag
+            assert np.abs(r["Z"] - H10[tag + "_Z"]).max() < 1e-8, tag
+            assert abs(r["EcRPA"] - H10[tag + "_EcRPA"]) < 1e-9, tag
+            assert abs(r["EcGM"] - H10[tag + "_EcGM"]) < 1e-9, tag
+
+
+def test_evgw_h10_vs_quack(gw, systems):
+    say("evgw H10")
+    s = systems["H10"]
+    nO, n = s["nO"], s["nOrb"]
+    with gw.GWEngine(s["nBas"], n, nO) as g:
+        g.set_eri_ao(s["eri"], s["c"])
+        for tag, kw in (("evgw", {"max_diis": 5, "thresh": 1e-6}), ("evgw_nodiis", {"max_diis": 1, "thresh": 1e-6})):
+            r = g.evgw(s["eHF"], **kw)
+            assert r["nSCF"] == int(H10[tag + "_nSCF"]), tag
+            assert np.abs(r["eGW"] - H10[tag + "_eGW"]).max() < 1e-8, tag
+            assert abs(r["EcRPA"] - H10[tag + "_EcRPA"]) < 1e-8, tag
+            assert abs(r["EcGM"] - H10[tag + "_EcGM"]) < 1e-8, tag
+        with pytest.raises(gw.NotConverged) as ei:
+            g.evgw(s["eHF"], linearize=True, max_diis=5, thresh=1e-6, maxSCF=1)
+        r = ei.value.result
+        assert r["nSCF"] == int(H10["evgw_lin_nSCF"]) == 2
+        assert np.abs(r["eGW"] - H10["evgw_lin_eGW"]).max() < 1e-7
+
+
+@pytest.mark.parametrize("name", ["He", "H2O"])
+def test_quack_goldens_on_gpu(gw, systems, name):
+    """QuAcK's own test values (tests/feather_bench.py) through the GPU path."""
+    say("goldens", name)
+    s = systems[name]
+    nO = s["nO"]
+    g = GOLD[name]["feather"]
+    with gw.GWEngine(s["nBas"], s["nOrb"], nO) as eng:
+        eng.set_eri_ao(s["eri"], s["c"])
+        r = eng.g0w0(s["eHF"])
+        assert abs(r["EcRPA"] - g["G0W0 correlation energy"]) < 1e-7 * abs(g["G0W0 correlation energy"])
+        assert abs(r["eGW"][nO - 1] - g["G0W0 HOMO energy"]) < 1e-7 * abs(g["G0W0 HOMO energy"])
+        assert abs(r["eGW"][nO] - g["G0W0 LUMO energy"]) < 1e-7 * abs(g["G0W0 LUMO energy"])
+        r = eng.evgw(s["eHF"], maxSCF=256, thresh=1e-5, max_diis=1)
+        assert abs(r["EcRPA"] - g["evGW correlation energy"]) < 1e-7 * abs(g["evGW correlation energy"])
+        assert abs(r["eGW"][nO - 1] - g["evGW HOMO energy"]) < 1e-7 * abs(g["evGW HOMO energy"])
+        assert abs(r["eGW"][nO] - g["evGW LUMO energy"]) < 1e-7 * abs(g["evGW LUMO energy"])
+
+
+def test_argument_errors(gw):
+    say("argument errors")
+    import ctypes
+    assert gw.load().qk_gw_create(None, 10, 10, 0, 3, 7, 0, 21) == gw.QK_E_ARG  # null handle pointer
+    h = ctypes.c_void_p()
+    assert gw.load().qk_gw_create(ctypes.byref(h), 10, 10, 0, 3, 7, 0, 22) == gw.QK_E_ARG  # nS != 21
+    assert gw.load().qk_gw_create(ctypes.byref(h), 10, 10, 0, 0, 10, 0, 0) == gw.QK_E_ARG  # no occupied
+    with gw.GWEngine(10, 10, 3) as g:
+        with pytest.raises(gw.QKError):
+            g.g0w0(np.zeros(10))  # no integrals yet
