@@ -10,9 +10,9 @@
 // consumed, so global latency overlaps the MFMA chain.
 //
 // MFMA operand roles are swapped (srcA = B^T fragment, srcB = A^T fragment) so
-// that each accumulator lane holds 4 results of ONE column of C at rows
-// lane%16: a wavefront's store then writes 16 consecutive doubles of C per
-// column, i.e. coalesced 128-byte segments of the column-major output.
+// that lane l of an accumulator holds row l%16 of C (in 4 columns): a
+// wavefront's store then writes 16 consecutive doubles of each column, i.e.
+// 128-byte segments of the column-major output.
 #include "qk_common.h"
 
 namespace qk {
@@ -123,7 +123,9 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void dgemm_mfma(
         }
     }
 
-    // acc[a][b][r] at lane: row m0+wm+a*16+l16, column n0+wn+b*16+4*lk+r
+    // v_mfma_f64_16x16x4f64 leaves D(i', j') in lane j' + 16 (i' % 4), register i' / 4.
+    // With srcA = B^T and srcB = A^T, D(i', j') = C(row j', col i'): acc[a][b][r]
+    // at this lane is C(m0+wm+a*16+l16, n0+wn+b*16+lk+4r).
 #pragma unroll
     for (int a = 0; a < BLK_M; ++a) {
         int gi = m0 + wm + a * 16 + l16;
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void dgemm_mfma(
         for (int b = 0; b < BLK_N; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int gj = n0 + wn + b * 16 + 4 * lk + r;
+                int gj = n0 + wn + b * 16 + lk + 4 * r;
                 if (gj < N) {
                     double* c = C + gi + ldc * (long long)gj;
                     double v = alpha * acc[a][b][r];
