@@ -28,7 +28,12 @@ def load():
     if _lib is None:
         if not available():
             raise RuntimeError("oracle/_ref/libqkref.so is not built (make -C oracle)")
-        lib = ctypes.CDLL(LIB)
+        # MKL behind this library gives wrong dsyev/dsysvx results in a process
+        # that has imported torch (torch carries its own MKL); callers that use
+        # torch run the reference in a child process (bench.py does).
+        if "torch" in __import__("sys").modules:
+            raise RuntimeError("load oracle/_ref in a process without torch (MKL clash)")
+        lib = ctypes.CDLL(LIB, mode=os.RTLD_NOW | os.RTLD_LOCAL | os.RTLD_DEEPBIND)
         lib.qkref_aotomo.argtypes = [_I, _I, _P, _P, _P]
         lib.qkref_phrlr.argtypes = [_I] * 7 + [_P, _P, _P, _P, _P, _P]
         lib.qkref_excitation_density.argtypes = [_I] * 5 + [_P, _P, _P]

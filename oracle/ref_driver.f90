@@ -34,6 +34,7 @@ subroutine qkref_phrlr(tda, nOrb, nC, nO, nV, nR, nS, e, ERI, EcRPA, Om, XpY, Xm
   call phRLR_A(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, e, ERI, Aph)
   if(.not. TDA_W) call phRLR_B(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, ERI, Bph)
   call phRLR(TDA_W, nS, Aph, Bph, EcRPA, Om, XpY, XmY)
+  flush(6)
   deallocate(Aph, Bph)
 end subroutine
 
@@ -71,6 +72,7 @@ subroutine qkref_qp_graph(srg, eta, nOrb, nC, nO, nV, nR, nS, eHF, Om, rho, eGWl
   real(c_double), intent(out) :: eGW(nOrb)
   real(c_double), intent(inout) :: Z(nOrb)
   call RGW_QP_graph(srg /= 0, eta, 500d0, nOrb, nC, nO, nV, nR, nS, eHF, Om, rho, eGWlin, eOld, eGW, Z)
+  flush(6)
 end subroutine
 
 ! G0W0: GW/RG0W0.f90:88-233.  Om0 is the screening at eHF; EcRPA comes from
@@ -116,6 +118,7 @@ subroutine qkref_rg0w0(tda, lin, srg, eta, nOrb, nC, nO, nV, nR, nS, ENuc, ERHF,
   call phRLR(TDA_W, nS, Aph, Bph, EcRPA, Om, XpY, XmY)
 
   call print_RG0W0(nOrb, nC, nO, nV, nR, eHF, ENuc, ERHF, SigC, Z, eGW, EcRPA, EcGM)
+  flush(6)
   deallocate(Aph, Bph, Om, XpY, XmY, rho, eGWlin)
 end subroutine
 
@@ -177,6 +180,7 @@ function qkref_evrgw(tda, lin, srg, maxSCF, thresh, max_diis, eta, nOrb, nC, nO,
     nSCF = nSCF + 1
   end do
 
+  flush(6)
   nSCF_out = nSCF
   status = 0
   if(nSCF == maxSCF+1) status = 1

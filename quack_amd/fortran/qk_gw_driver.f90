@@ -1,0 +1,114 @@
+! qk_gw_driver: runs the GPU G0W0/evGW drop-ins from QuAcK-format input files.
+!
+!   qk_gw_driver <working_dir>
+!
+! Reads, from <working_dir>:
+!   input/methods    the "# G0W0 evGW qsGW scGW" line and the "# Rtest ..." line
+!                    (src/QuAcK/read_methods.f90:191-199)
+!   input/options    the "# GW:" line, 14 fields (src/QuAcK/read_options.f90:240-266)
+!   input/molecule   "# nAt nEla nElb nCore nRyd" (PyDuck.py:137-145)
+!   int/ENuc.dat     nuclear repulsion (PyDuck.py:150-153)
+!   int/MO.dat       nOrb, ERHF and eHF(1:nOrb): the RHF reference QuAcK computes
+!                    in-process before RGW (src/QuAcK/RQuAcK.f90:252)
+!   int/ERI_MO.bin   ERI_MO(nOrb,nOrb,nOrb,nOrb), raw FP64 stream, the array
+!                    AOtoMO_ERI_RHF hands to RGW (src/QuAcK/RQuAcK.f90:459,684)
+! and writes test/Rtest.dat with QuAcK's dump format when Rtest is set.
+
+program qk_gw_driver
+
+  implicit none
+  character(len=512) :: wd, line
+  logical :: doG0W0, doevGW, doqsGW, doscGW, dotest, dotU, dotG
+  integer :: maxSCF, max_diis, nfreqs, nAt, nEla, nElb, nC, nR, nO, nV, nS, nOrb, ios
+  double precision :: thresh, eta, shift, ENuc, ERHF
+  character(len=1) :: a1, a2, a3, a4, a5, a6, a7, a8
+  logical :: lin, TDA_W, reg
+  double precision, allocatable :: eHF(:), ERI(:,:,:,:), dip(:,:,:), eGW(:)
+
+  call get_command_argument(1, wd)
+  if(len_trim(wd) == 0) wd = '.'
+
+  ! methods
+  doG0W0 = .false.; doevGW = .false.; dotest = .false.
+  open(unit=11, file=trim(wd)//'/input/methods', status='old', action='read')
+  do
+    read(11,'(A)',iostat=ios) line
+    if(ios /= 0) exit
+    if(index(line, '# G0W0') == 1) then
+      read(11,*) a1, a2, a3, a4
+      doG0W0 = a1 == 'T'; doevGW = a2 == 'T'; doqsGW = a3 == 'T'; doscGW = a4 == 'T'
+    else if(index(line, '# Rtest') == 1) then
+      read(11,*) a1, a2, a3
+      dotest = a1 == 'T'; dotU = a2 == 'T'; dotG = a3 == 'T'
+    end if
+  end do
+  close(11)
+
+  ! options (defaults of src/QuAcK/read_options.f90:242-254)
+  maxSCF = 64; thresh = 1d-5; max_diis = 1; lin = .false.; eta = 0d0; TDA_W = .false.; reg = .false.
+  open(unit=12, file=trim(wd)//'/input/options', status='old', action='read')
+  do
+    read(12,'(A)',iostat=ios) line
+    if(ios /= 0) exit
+    if(index(line, '# GW:') == 1) then
+      read(12,*) maxSCF, thresh, max_diis, a1, eta, a2, a3, nfreqs, shift, a4, a5, a6, a7, a8
+      lin = a1 == 'T'; TDA_W = a2 == 'T'; reg = a3 == 'T'
+    end if
+  end do
+  close(12)
+
+  ! molecule
+  open(unit=13, file=trim(wd)//'/input/molecule', status='old', action='read')
+  read(13,*)
+  read(13,*) nAt, nEla, nElb, nC, nR
+  close(13)
+  if(nEla /= nElb) then
+    write(*,*) 'qk_gw_driver: restricted (closed-shell) molecules only'
+    error stop 2
+  end if
+  nO = nEla
+
+  open(unit=14, file=trim(wd)//'/int/ENuc.dat', status='old', action='read')
+  read(14,*) ENuc
+  close(14)
+
+  open(unit=15, file=trim(wd)//'/int/MO.dat', status='old', action='read')
+  read(15,*) nOrb
+  read(15,*) ERHF
+  allocate(eHF(nOrb))
+  read(15,*) eHF
+  close(15)
+
+  nV = nOrb - nO
+  nS = (nO - nC)*(nV - nR)
+  allocate(ERI(nOrb,nOrb,nOrb,nOrb), dip(nOrb,nOrb,3), eGW(nOrb))
+  dip = 0d0
+  open(unit=16, file=trim(wd)//'/int/ERI_MO.bin', status='old', action='read', form='unformatted', access='stream')
+  read(16) ERI
+  close(16)
+
+  if(dotest) open(unit=1231597, file=trim(wd)//'/test/Rtest.dat', status='replace')
+
+  if(doG0W0) call RG0W0_hip(dotest, .false., .false., .false., .false., .false., TDA_W, .false., .false., .false., &
+                            .false., .true., .false., lin, eta, reg, .false., nOrb, nOrb, nC, nO, nV, nR, nS, ENuc, &
+                            ERHF, ERI, dip, eHF, eGW)
+  if(doevGW) call evRGW_hip(dotest, maxSCF, thresh, max_diis, .false., .false., .false., .false., .false., TDA_W, &
+                            .false., .false., .false., .false., .true., .false., lin, eta, reg, nOrb, nOrb, nC, nO, &
+                            nV, nR, nS, ENuc, ERHF, ERI, dip, eHF)
+
+  if(dotest) close(1231597)
+
+end program
+
+! QuAcK's test-dump format (src/test/dump_test_value.f90:19-20): the
+! description on one line, the value as F20.15 on the next.
+subroutine dump_test_value(branch, description, val)
+  implicit none
+  character(len=1), intent(in) :: branch
+  character(len=*), intent(in) :: description
+  double precision, intent(in) :: val
+  if(branch == 'R') then
+    write(1231597,*) trim(description)
+    write(1231597,'(F20.15)') val
+  end if
+end subroutine

@@ -1,0 +1,97 @@
+! ISO_C_BINDING interface to libquack_amd.so (include/quack_amd.h).
+!
+! Same style as QuAcK's own bind(C) module for its GPU library
+! (src/GPU/cu_quack_module.f90:11-36): value integers, c_double arrays passed
+! by reference, an opaque handle as type(c_ptr).  All arrays keep QuAcK's
+! Fortran layout, so no transposes are needed on either side.
+
+module quack_amd_gw
+
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: QK_OK = 0
+  integer(c_int), parameter :: QK_E_ARG = 1
+  integer(c_int), parameter :: QK_E_HIP = 2
+  integer(c_int), parameter :: QK_E_EIG = 3
+  integer(c_int), parameter :: QK_E_NOCONV = 4
+  integer(c_int), parameter :: QK_E_UNSUPPORTED = 5
+  integer(c_int), parameter :: QK_E_ALLOC = 6
+
+  interface
+
+    integer(c_int) function qk_gw_create(h, nBas, nOrb, nC, nO, nV, nR, nS) bind(C, name='qk_gw_create')
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: h
+      integer(c_int), value :: nBas, nOrb, nC, nO, nV, nR, nS
+    end function
+
+    integer(c_int) function qk_gw_destroy(h) bind(C, name='qk_gw_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    type(c_ptr) function qk_gw_last_error(h) bind(C, name='qk_gw_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    integer(c_int) function qk_gw_set_eri_ao(h, ERI_AO, c) bind(C, name='qk_gw_set_eri_ao')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: ERI_AO(*), c(*)
+    end function
+
+    integer(c_int) function qk_gw_set_eri_mo(h, ERI_MO) bind(C, name='qk_gw_set_eri_mo')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: ERI_MO(*)
+    end function
+
+    integer(c_int) function qk_rg0w0(h, eHF, eta, linearize, TDA_W, doSRG, eGW, SigC, Z, Om, EcRPA, EcGM) &
+        bind(C, name='qk_rg0w0')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: eHF(*)
+      real(c_double), value :: eta
+      integer(c_int), value :: linearize, TDA_W, doSRG
+      real(c_double), intent(out) :: eGW(*), SigC(*), Z(*), Om(*), EcRPA, EcGM
+    end function
+
+    integer(c_int) function qk_evrgw(h, eHF, eGW_start, maxSCF, thresh, max_diis, eta, linearize, TDA_W, doSRG, &
+                                     eGW, SigC, Z, Om, EcRPA, EcGM, nSCF) bind(C, name='qk_evrgw')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: eHF(*)
+      type(c_ptr), value :: eGW_start
+      integer(c_int), value :: maxSCF, max_diis, linearize, TDA_W, doSRG
+      real(c_double), value :: thresh, eta
+      real(c_double), intent(out) :: eGW(*), SigC(*), Z(*), Om(*), EcRPA, EcGM
+      integer(c_int), intent(out) :: nSCF
+    end function
+
+  end interface
+
+contains
+
+  ! Stop with a nonzero code and the library's message (QuAcK's CUDA helpers
+  ! exit(0) on errors, src/cuda/src/utils.cu:10-16; here the status survives).
+  subroutine qk_check(rc, h, where)
+    integer(c_int), intent(in) :: rc
+    type(c_ptr), intent(in) :: h
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: i
+    if(rc == QK_OK) return
+    text = ''
+    call c_f_pointer(qk_gw_last_error(h), msg, [512])
+    do i = 1, 512
+      if(msg(i) == c_null_char) exit
+      text(i:i) = msg(i)
+    end do
+    write(*,'(A,A,A,I0,A,A)') ' quack_amd: ', where, ' failed with status ', rc, ': ', trim(text)
+    error stop 70
+  end subroutine
+
+end module

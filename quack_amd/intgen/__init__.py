@@ -178,18 +178,18 @@ def rhf(S, T, V, eri, nO, maxit=200, conv=1e-10, jk=jk_numpy, ndiis=8):
     e, Cp = np.linalg.eigh(X.T @ H @ X)
     c = X @ Cp
     fs, errs = [], []
-    E_old = 0.0
     for it in range(maxit):
         Co = c[:, :nO]
         D = Co @ Co.T
         J, K = jk(eri, D)
         F = H + 2.0 * J - K
         E = float(np.sum(D * (H + F)))
-        err = F @ D @ S - S @ D @ F
+        # commutator in the orthogonalised basis (zero at convergence even
+        # when canonical orthogonalisation dropped near-dependent functions)
+        err = X.T @ (F @ D @ S - S @ D @ F) @ X
         errmax = np.abs(err).max()
-        if errmax < conv and abs(E - E_old) < 1e-12:
+        if errmax < conv:
             break
-        E_old = E
         fs.append(F)
         errs.append(err)
         fs, errs = fs[-ndiis:], errs[-ndiis:]
@@ -224,3 +224,26 @@ def system(mol, jk=jk_numpy):
     scf = rhf(S, T, V, eri, mol.nO, jk=jk)
     return {"mol": mol, "S": S, "T": T, "V": V, "eri": eri, "ENuc": mol.enuc(), "nBas": mol.nBas,
             "nO": mol.nO, **scf}
+
+
+def jk_torch(eri_t, chunk=32):
+    """Fock-build closure over a device tensor holding the flat eri array.
+
+    Used to prepare RHF references for large workloads (N = 250: 31 GB) on the
+    GPU; torch is only plumbing here, the GW path never calls it."""
+    import torch
+
+    def jk(eri_unused, D):
+        n = D.shape[0]
+        X = eri_t.view(n, n, n, n)                      # X[d, c, b, a] = (ac|bd)
+        Dt = torch.from_numpy(np.ascontiguousarray(D)).to(eri_t.device)
+        K = (Dt.T.reshape(1, -1) @ X.view(n * n, n * n)).view(n, n).T
+        J = torch.zeros(n, n, dtype=torch.float64, device=eri_t.device)
+        for d0 in range(0, n, chunk):
+            d1 = min(n, d0 + chunk)
+            # sum_b D[b, d] X[d, c, b, a] for d in the chunk -> (c, a)
+            v = Dt[:, d0:d1].T.reshape(d1 - d0, 1, 1, n)
+            J += torch.matmul(v, X[d0:d1]).sum(dim=0).squeeze(1)
+        return J.T.cpu().numpy(), K.cpu().numpy()
+
+    return jk

@@ -1,0 +1,109 @@
+"""QuAcK-format input files for the Fortran drop-in driver (quack_amd/fortran/qk_gw_driver).
+
+Writes the files QuAcK reads (input/methods, input/options, input/molecule,
+input/hpc_flags, int/ENuc.dat, int/nBas.dat, int/ERI.bin with the layouts of
+PyDuck.py:137-153,329-345 and input/*.default) plus the two arrays QuAcK
+computes in-process before RGW (int/MO.dat: nOrb, ERHF, eHF; int/ERI_MO.bin).
+"""
+import os
+
+import numpy as np
+
+METHODS = """# RHF UHF GHF ROHF HFB cRHF cUHF eRHF scGHF
+  T   F   F   F    F   F    F    F    F
+# MP2 MP3
+  F   F
+# CCD pCCD DCD CCSD CCSD(T)
+  F   F    F   F    F
+# drCCD rCCD crCCD lCCD
+  F     F    F     F
+# CIS CIS(D) CID CISD FCI
+  F   F      F   F    F
+# phRPA phRPAx crRPA ppRPA BRPA
+  F     F      F     F     F
+# OORPA
+  F
+# G0F2 evGF2 qsGF2 G0F3 evGF3 psdG0F3 scGF2
+  F    F     F     F    F     F     F
+# G0W0 evGW qsGW scGW
+  {g0w0}    {evgw}    F    F
+# G0T0pp evGTpp qsGTpp ufG0T0pp
+  F      F        F       F
+# G0T0eh evGTeh qsGTeh
+  F      F      F
+# evParquet qsParquet
+  F         F
+# IPEA-ADC2 IPEA-ADC3 SOSEX 2SOSEX G3W2 psdG3W2
+  F         F         F     F      F    F
+# ADC-GW ADC-2SOSEX ADC(3)-G3W2 ADC(3x)-G3W2 ADC-G3W2
+  F      F          F           F            F
+# Rtest Utest Gtest
+  {test}     F     F
+"""
+
+OPTIONS = """# HF: maxSCF thresh    DIIS guess mix shift stab search aordm readFCIDUMP MOM WriteMOs
+      256    0.0000001   5    1   0.0  0.0   F     F      F       F        F    F
+# MP: reg
+      F
+# CC: maxSCF thresh  DIIS
+      64     0.00001 5
+# spin: TDA singlet triplet
+         F     T      T
+# RPA/MP2: CVS-Alpha CVS-Beta  FC-Alpha FC-Beta
+               0         0          0      0
+# OORPA:  maxIter thresh  dRPA state diagHess
+          256     0.00001 T    0     T
+# GF: maxSCF thresh  DIIS lin eta renorm reg linDM restart_scGF2 verbose_scGF2
+      256    0.00001 5    F   0.0 0      F   F     F         F
+# GW: maxSCF thresh  DIIS lin eta TDA_W reg nfreqs shift linDM restart_scGW verbose_scGW adjust_mu_scGW change_sign_XoB
+      {maxscf}    {thresh} {diis}    {lin}   {eta} {tda}     {reg}   1      0.0   F     F            F            F              F
+# GT: maxSCF thresh  DIIS lin eta TDA_T reg linDM
+      256    0.00001 5    F   0.0 F     F   F
+# ACFDT: AC Kx  XBS
+         F  F   F
+# BSE: phBSE phBSE2 ppBSE dBSE dTDA
+       F     F      F     F    F
+"""
+
+
+def tf(b):
+    return "T" if b else "F"
+
+
+def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=1e-5, max_diis=5, lin=False,
+                 eta=0.0, TDA_W=False, reg=False, eri_mo=None):
+    """sysd: a dict from quack_amd.intgen.system(); eri_mo: MO integrals (nOrb^4, flat)."""
+    for d in ("input", "int", "test"):
+        os.makedirs(os.path.join(wd, d), exist_ok=True)
+    mol = sysd["mol"]
+    with open(os.path.join(wd, "input", "methods"), "w") as f:
+        f.write(METHODS.format(g0w0=tf(g0w0), evgw=tf(evgw), test=tf(test)))
+    with open(os.path.join(wd, "input", "options"), "w") as f:
+        f.write(OPTIONS.format(maxscf=maxSCF, thresh="%.10g" % thresh, diis=max_diis, lin=tf(lin), eta="%.10g" % eta,
+                               tda=tf(TDA_W), reg=tf(reg)))
+    with open(os.path.join(wd, "input", "hpc_flags"), "w") as f:
+        f.write("# if True (T), switch to HPC mode\n  F\n# if True (T), use GPU\n  T\n")
+    with open(os.path.join(wd, "input", "molecule"), "w") as f:
+        f.write("# nAt nEla nElb nCore nRyd\n%d %d %d 0 0\n# Znuc x  y  z\n" % (len(mol.symbols), mol.nO, mol.nO))
+        for s, r in zip(mol.symbols, mol.xyz):
+            f.write("%s %.17g %.17g %.17g\n" % (s, r[0], r[1], r[2]))
+    with open(os.path.join(wd, "int", "ENuc.dat"), "w") as f:
+        f.write("%.17g \n" % float(sysd["ENuc"]))
+    with open(os.path.join(wd, "int", "nBas.dat"), "w") as f:
+        f.write(" %d " % sysd["nBas"])
+    np.ascontiguousarray(sysd["eri"], dtype=np.float64).tofile(os.path.join(wd, "int", "ERI.bin"))
+    with open(os.path.join(wd, "int", "MO.dat"), "w") as f:
+        f.write("%d\n%.17g\n" % (sysd["nOrb"], sysd["ERHF"]))
+        for e in sysd["eHF"]:
+            f.write("%.17g\n" % e)
+    if eri_mo is not None:
+        np.ascontiguousarray(eri_mo, dtype=np.float64).tofile(os.path.join(wd, "int", "ERI_MO.bin"))
+
+
+def read_test_dump(path):
+    """Parse QuAcK's test dump (description line, value line), src/test/dump_test_value.f90:19-20."""
+    out = {}
+    lines = [ln.strip() for ln in open(path) if ln.strip()]
+    for k in range(0, len(lines) - 1, 2):
+        out[lines[k]] = float(lines[k + 1])
+    return out
