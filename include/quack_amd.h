@@ -72,6 +72,14 @@ int qk_evrgw(qk_gw_handle h, const double *eHF, const double *eGW_start, int max
              double eta, int linearize, int TDA_W, int doSRG, double *eGW, double *SigC, double *Z, double *Om,
              double *EcRPA, double *EcGM, int *nSCF);
 
+/* Precision of the excitation-density contraction rho = sum_jb (pq|jb) XpY(m,jb)
+ * (GW/RGW_excitation_density.f90:48-50): 0 = FP64 MFMA (default, QuAcK's
+ * arithmetic); 1 = split precision: each FP64 operand as three BF16 slices,
+ * six slice products on BF16 matrix cores, FP32 accumulation over 256-term
+ * chunks, FP64 recombination.  The RPA eigensolve and the QP Newton solve stay
+ * FP64 in both modes.  No counterpart in QuAcK (which is FP64 throughout). */
+int qk_gw_set_precision(qk_gw_handle h, int mode);
+
 /* Device time (ms, HIP events on the handle's stream) of the stages of the
  * last qk_rg0w0 / qk_evrgw call: [0] AO->MO block (last set_eri_*),
  * [1] RPA matrix build, [2] eigensolve, [3] excitation densities,

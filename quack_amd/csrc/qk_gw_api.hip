@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "qk_kernels.h"
+#include "qk_split.h"
 
 namespace qk {
 namespace {
@@ -160,6 +161,9 @@ struct qk_gw_ctx {
     int *nit = nullptr, *done = nullptr, *info = nullptr, *active = nullptr;
     int nchunk = 1;
     bool have_eri = false;
+    int precision = 0;        // 0: FP64 MFMA, 1: BF16x3 split with FP64 recombination
+    qk::Split3 Msplit, Wsplit;  // M^T and W slices (mixed precision)
+    bool msplit_valid = false;
     double t_ao = 0.0;
     double t_stage[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -214,7 +218,18 @@ void excitation(Ctx* c) {
     Stage st(c, 2);
     long long n2 = (long long)c->nOrb * c->nOrb;
     QK_CHECK(n2 < (1LL << 31), QK_E_ARG, "nOrb^2 too large");
-    dgemm(c->s, true, true, c->nS, (int)n2, c->nS, 1.0, c->Cm, c->nS, c->M, n2, 0.0, c->rhoT, c->nS);
+    if (c->precision == 0) {
+        dgemm(c->s, true, true, c->nS, (int)n2, c->nS, 1.0, c->Cm, c->nS, c->M, n2, 0.0, c->rhoT, c->nS);
+        return;
+    }
+    // rho^T(m, pq) = sum_jb W(jb, m) M(pq, jb): A rows m (k = jb contiguous in W),
+    // B rows pq (M^T, split once per molecule)
+    if (!c->msplit_valid) {
+        split3(c->s, c->M, n2, false, (int)n2, c->nS, c->Msplit);
+        c->msplit_valid = true;
+    }
+    split3(c->s, c->Cm, c->nS, true, c->nS, c->nS, c->Wsplit);
+    gemm_bf16x3(c->s, c->nS, (int)n2, c->Wsplit, c->Msplit, c->rhoT, c->nS);
 }
 
 void sigma_static(Ctx* c, const double* e, double eta, int srg) {
@@ -299,7 +314,17 @@ void aotomo_block(hipStream_t s, const double* f, const double* cc, double* M, i
     }
 }
 
+void alloc_split(Split3& sp, int rows, int K) {
+    sp.rows = rows;
+    sp.kpad = pad32(K);
+    for (auto& p : sp.s) p = dalloc<__bf16>((size_t)rows * sp.kpad);
+}
+void free_split(Split3& sp) {
+    for (auto& p : sp.s) dfree(p);
+}
+
 void after_eri(Ctx* c) {
+    c->msplit_valid = false;
     extract_k(c->s, c->M, c->K, c->nOrb, c->nC, c->nO, c->v, c->nS);
     QK_HIP(hipStreamSynchronize(c->s));
     c->have_eri = true;
@@ -380,6 +405,8 @@ void release(Ctx* c) {
     int* is[] = {c->nit, c->done, c->info, c->active};
     for (int* p : is)
         if (p) (void)hipFree(p);
+    free_split(c->Msplit);
+    free_split(c->Wsplit);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->rb) (void)rocblas_destroy_handle(c->rb);
@@ -492,6 +519,18 @@ int qk_gw_set_eri_mo(qk_gw_handle c, const double* ERI_MO) {
         upload_mo_block(c->s, ERI_MO, c->M, c->nOrb, c->nC, c->nO, c->nR);
         c->t_ao = 0.0;
         after_eri(c);
+    });
+}
+
+int qk_gw_set_precision(qk_gw_handle c, int mode) {
+    if (!c || mode < 0 || mode > 1) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        if (mode == 1 && !c->Wsplit.s[0]) {
+            alloc_split(c->Wsplit, c->nS, c->nS);
+            alloc_split(c->Msplit, c->nOrb * c->nOrb, c->nS);
+            c->msplit_valid = false;
+        }
+        c->precision = mode;
     });
 }
 

@@ -1,0 +1,25 @@
+// Split-precision operands: three BF16 slices of an FP64 matrix, row-major with
+// k contiguous and k padded to a multiple of 32 (qk_gemm_bf16x3.hip).
+#pragma once
+#include <algorithm>
+
+#include "qk_common.h"
+
+namespace qk {
+
+struct Split3 {
+    __bf16* s[3] = {nullptr, nullptr, nullptr};
+    int rows = 0;
+    int kpad = 0;
+};
+
+inline int pad32(int k) { return (k + 31) / 32 * 32; }
+
+// out.s[t][r * kpad + k] = slice t of src(r, k);
+// src(r, k) = src[k + ld*r] if k_contiguous else src[r + ld*k]; k >= K gives 0.
+void split3(hipStream_t s, const double* src, long long ld, bool k_contiguous, int rows, int K, Split3& out);
+
+// C[i + ldc*j] = sum_k A(i, k) B(j, k), FP64 result from BF16 slice products.
+void gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc);
+
+}  // namespace qk

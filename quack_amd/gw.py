@@ -24,7 +24,7 @@ _NAMES = {1: "QK_E_ARG", 2: "QK_E_HIP", 3: "QK_E_EIG", 4: "QK_E_NOCONV", 5: "QK_
 EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_ao", "qk_gw_set_eri_ao_device",
            "qk_gw_set_eri_mo", "qk_rg0w0", "qk_evrgw", "qk_gw_stage_times", "qk_aotomo_eri_rhf", "qk_phrlr",
            "qk_phrlr_a", "qk_phrlr_b", "qk_rgw_excitation_density", "qk_rgw_self_energy_diag", "qk_rgw_qp_graph",
-           "qk_last_error"]
+           "qk_last_error", "qk_gw_set_precision"]
 
 
 class QKError(RuntimeError):
@@ -71,6 +71,7 @@ def load():
     lib.qk_rg0w0.argtypes = [_H, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_evrgw.argtypes = [_H, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
     lib.qk_gw_stage_times.argtypes = [_H, _P, _I]
+    lib.qk_gw_set_precision.argtypes = [_H, _I]
     lib.qk_aotomo_eri_rhf.argtypes = [_I, _I, _P, _P, _P]
     lib.qk_phrlr.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_phrlr_a.argtypes = [_I] * 7 + [_D, _P, _P, _P]
@@ -131,6 +132,10 @@ class GWEngine:
             self.close()
         except Exception:
             pass
+
+    def set_precision(self, mode):
+        """0: FP64 (QuAcK's arithmetic); 1: split-BF16 excitation densities ("mixed")."""
+        _check(load().qk_gw_set_precision(self.h, {"fp64": 0, "mixed": 1}.get(mode, mode)), self.h)
 
     def set_eri_ao(self, eri_ao, c):
         eri_ao = np.ascontiguousarray(eri_ao, dtype=np.float64).reshape(-1)
