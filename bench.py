@@ -5,9 +5,13 @@ Workload (default): the seeded 2D H50 lattice (5 x 10 grid, 1.6 bohr spacing,
 functions; QuAcK's canonical orthogonalisation (threshold 1e-6) keeps
 nOrb = 249, so o = 25, v = 224, nS = 5600.  One step is one molecule: the
 AO->MO transformation of the (pq|jb) block from the AO integrals resident
-in HBM, then evGW@RHF to 1e-6 Ha (eta 0, graphical QP, DIIS off: QuAcK's
-stored evGW results use max_diis = 1, and DIIS 5 needs ~214 cycles on this
-lattice instead of 24), i.e. the
+in HBM, then SRG-regularised evGW@RHF (QuAcK's `reg` option, flow 500) to
+1e-6 Ha with the graphical QP solve and DIIS off (QuAcK's stored evGW results
+use max_diis = 1).  The regularisation is what makes a 1 meV comparison
+meaningful here: with eta = 0 and no regularisation the graphical QP solution
+of this lattice is chaotic in FP64 itself (a 1e-12 Ha perturbation of eHF moves
+QP energies by up to 1.5 eV), while with SRG the same perturbation moves them
+by < 1e-7 meV.  That is the
 "AO to MO transformation" + "evGW" stages QuAcK times.  Inputs (AO ERIs,
 MO coefficients, eHF) are produced before the timed region by the integral
 producer and an RHF whose Fock builds run through torch on the device.
@@ -27,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6       # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0      # GB/s
+PEAK_BF16 = 2500.0     # TFLOP/s, MI355X dense BF16 matrix
 
 
 def log(*a):
@@ -56,7 +61,7 @@ def build_inputs(nx, ny, seed):
 CPU_SAMPLE = r"""
 import json, os, sys, time
 sys.path.insert(0, sys.argv[1])
-method, max_diis = sys.argv[2], int(sys.argv[3])
+method, max_diis, reg = sys.argv[2], int(sys.argv[3]), bool(int(sys.argv[4]))
 from oracle import qkref
 from quack_amd import intgen as ig
 s = ig.system(ig.h_chain(10, seed=0))
@@ -64,8 +69,8 @@ nO = s["nO"]
 def one():
     mo = qkref.aotomo(s["c"], s["eri"])
     if method == "evgw":
-        return qkref.evrgw(mo, s["eHF"], nO, maxSCF=256, thresh=1e-6, max_diis=max_diis)
-    return qkref.rg0w0(mo, s["eHF"], nO)
+        return qkref.evrgw(mo, s["eHF"], nO, maxSCF=256, thresh=1e-6, max_diis=max_diis, srg=reg)
+    return qkref.rg0w0(mo, s["eHF"], nO, srg=reg)
 sys.stdout.flush()
 saved = os.dup(1); null = os.open(os.devnull, os.O_WRONLY); os.dup2(null, 1)
 r = one()
@@ -78,7 +83,7 @@ print(json.dumps({"t": t, "reps": reps, "eGW": list(r["eGW"]), "EcRPA": float(r[
 """
 
 
-def cpu_baseline(method, max_diis):
+def cpu_baseline(method, max_diis, reg):
     """QuAcK's own routines (oracle/_ref: GW/, LR/, AOtoMO/ compiled from
     source with flang + MKL) on the host cores, on a bounded sample.
 
@@ -99,7 +104,7 @@ def cpu_baseline(method, max_diis):
     from quack_amd import intgen as ig
     if not qkref.available():
         return None
-    p = subprocess.run([sys.executable, "-c", CPU_SAMPLE, ROOT, method, str(max_diis)], capture_output=True,
+    p = subprocess.run([sys.executable, "-c", CPU_SAMPLE, ROOT, method, str(max_diis), str(int(reg))], capture_output=True,
                        text=True, timeout=300)
     if p.returncode != 0:
         raise RuntimeError("cpu sample failed: " + p.stderr[-2000:])
@@ -110,8 +115,8 @@ def cpu_baseline(method, max_diis):
         def gpu_one():
             g.set_eri_ao(s["eri"], s["c"])
             if method == "evgw":
-                return g.evgw(s["eHF"], maxSCF=256, thresh=1e-6, max_diis=max_diis)
-            return g.g0w0(s["eHF"])
+                return g.evgw(s["eHF"], maxSCF=256, thresh=1e-6, max_diis=max_diis, doSRG=reg)
+            return g.g0w0(s["eHF"], doSRG=reg)
         r = gpu_one()
         reps_g, t0 = 0, time.time()
         while time.time() - t0 < 2.0:
@@ -123,7 +128,7 @@ def cpu_baseline(method, max_diis):
         (method, ref["t"], t_gpu, err))
     threads = os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
     return {"value": 1.0 / ref["t"], "unit": "molecules/s", "cores": int(threads), "kind": "reference",
-            "sample": ("%d x one H10-chain molecule (cc-pVDZ, N=50, nS=225): QuAcK's AOtoMO_ERI_RHF + %s "
+            "sample": ("%d x one H10-chain molecule (cc-pVDZ, N=50, nS=225), same options: QuAcK's AOtoMO_ERI_RHF + %s "
                        "(phRLR_A/B, phRLR, RGW_excitation_density, RGW_self_energy_diag, RGW_QP_graph%s), "
                        "host ERIs; the N=250 lattice itself does not fit a bounded CPU sample" %
                        (ref["reps"], "evRGW loop" if method == "evgw" else "RG0W0 sequence",
@@ -142,6 +147,9 @@ def main():
     ap.add_argument("--ny", type=int, default=10)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--max-diis", type=int, default=1)
+    ap.add_argument("--form", choices=["fp64", "mixed"], default="mixed",
+                    help="fp64: QuAcK's arithmetic; mixed: split-BF16 excitation densities")
+    ap.add_argument("--no-reg", action="store_true", help="plain eta=0 evGW instead of SRG-regularised")
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -159,13 +167,25 @@ def main():
     eng = gw.GWEngine(nBas, nOrb, nO)
     nS = eng.nS
 
+    reg = not args.no_reg
+
     def step():
         eng.set_eri_ao_device(eri_t.data_ptr(), c_t.data_ptr())
         if args.method == "evgw":
-            return eng.evgw(scf["eHF"], maxSCF=256, thresh=1e-6, max_diis=args.max_diis)
-        return eng.g0w0(scf["eHF"])
+            return eng.evgw(scf["eHF"], maxSCF=256, thresh=1e-6, max_diis=args.max_diis, doSRG=reg)
+        return eng.g0w0(scf["eHF"], doSRG=reg)
 
-    for w in range(args.warmup):
+    # FP64 form of the same molecule: the accuracy reference and the baseline rate
+    eng.set_precision("fp64")
+    step()
+    torch.cuda.synchronize()
+    t = time.time()
+    ref = step()
+    t_fp64 = time.time() - t
+    log("fp64 form: %.3f s/molecule%s" % (t_fp64, (" nSCF=%d" % ref["nSCF"]) if "nSCF" in ref else ""))
+    eng.set_precision(args.form)
+
+    for w in range(args.warmup if args.form != "fp64" else 0):
         t = time.time()
         r = step()
         log("warmup %d: %.3f s%s" % (w, time.time() - t, (" nSCF=%d" % r["nSCF"]) if "nSCF" in r else ""))
@@ -184,11 +204,19 @@ def main():
     t1 = time.time()
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = args.steps / (t1 - t0)
+    mev = 1e-3 / 27.211386245988
+    dE = np.abs(r["eGW"] - ref["eGW"])
+    window = np.abs(scf["eHF"] - 0.5 * (scf["eHF"][nO - 1] + scf["eHF"][nO])) < 1.0
 
-    # dominant hand-written kernel: excitation densities rho^T = W^T M^T
-    # (FP64 MFMA GEMM), one launch per screening cycle: 2 nOrb^2 nS^2 flops
-    flops = 2.0 * nOrb * nOrb * nS * nS
+    # dominant hand-written kernel: excitation densities rho^T = W^T M^T, one
+    # launch per screening cycle.  FP64 form: 2 nOrb^2 nS^2 flop on FP64 MFMA.
+    # Mixed form: the split algorithm issues six BF16 slice products, i.e.
+    # 6 x 2 nOrb^2 nS^2 flop on BF16 MFMA (plus the per-cycle split of W).
     gemm_ms = stage["excitation"] / ncycles
+    if args.form == "fp64":
+        flops, peak, kname = 2.0 * nOrb * nOrb * nS * nS, PEAK_FP64, "dgemm_mfma (excitation densities rho = M W)"
+    else:
+        flops, peak, kname = 12.0 * nOrb * nOrb * nS * nS, PEAK_BF16, "k_gemm_bf16x3 (excitation densities, 6 BF16 slice products)"
     achieved = flops / (gemm_ms * 1e-3) / 1e12
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_excitation_gemm.json")
@@ -200,7 +228,7 @@ def main():
     cpu = None
     if args.cpu_baseline == "auto":
         try:
-            cpu = cpu_baseline(args.method, args.max_diis)
+            cpu = cpu_baseline(args.method, args.max_diis, reg)
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline failed: %r" % (e,))
     out = {
@@ -214,14 +242,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.form == "fp64" else "bf16x3-slices/f32-acc/f64 (contractions), f64 (eigensolve, QP)",
         "data": "synthetic: seeded H50 lattice, integrals from the project's own Gaussian-integral producer",
-        "config": {"workload": "H50 5x10 lattice cc-pVDZ %s, FP64 form" % args.method, "nBas": nBas, "nOrb": nOrb,
+        "config": {"workload": "H50 5x10 lattice cc-pVDZ %s%s, %s form" % ("SRG-" if reg else "", args.method, args.form),
+                   "nBas": nBas, "nOrb": nOrb,
                    "nO": nO, "nS": nS, "method": args.method, "thresh": 1e-6 if args.method == "evgw" else None,
-                   "max_diis": args.max_diis, "eta": 0.0, "cycles_per_molecule": ncycles / args.steps},
+                   "max_diis": args.max_diis, "eta": 0.0, "srg_flow": 500.0 if reg else None,
+                   "cycles_per_molecule": ncycles / args.steps, "form": args.form},
+        "fp64_form": {"molecules_per_s": 1.0 / t_fp64, "cycles": ref.get("nSCF"),
+                      "speedup_of_form": value * t_fp64},
+        "accuracy_vs_fp64": {"max_abs_dE_meV_all": float(dE.max() / mev),
+                             "orbitals_over_1meV": int((dE > mev).sum()),
+                             "max_abs_dE_meV_within_1Ha_of_gap": float(dE[window].max() / mev),
+                             "HOMO_dE_meV": float(dE[nO - 1] / mev), "LUMO_dE_meV": float(dE[nO] / mev)},
         "stages_ms_per_molecule": {k: v / args.steps for k, v in stage.items()},
-        "roofline": {"kernel": "dgemm_mfma (excitation densities rho = M W)", "bound": "mfma",
-                     "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+        "roofline": {"kernel": kname, "bound": "mfma",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": gemm_ms},
         "cpu_baseline": cpu,
         "result": {"HOMO": float(r["eGW"][nO - 1]), "LUMO": float(r["eGW"][nO]), "EcRPA": r["EcRPA"],
