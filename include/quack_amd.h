@@ -72,6 +72,17 @@ int qk_evrgw(qk_gw_handle h, const double *eHF, const double *eGW_start, int max
              double eta, int linearize, int TDA_W, int doSRG, double *eGW, double *SigC, double *Z, double *Om,
              double *EcRPA, double *EcGM, int *nSCF);
 
+/* Screening at energies e (phRLR at e + excitation densities, as one cycle of
+ * GW/evRGW.f90:123-130), kept in the handle for qk_gw_sigma_at.  Used to build
+ * the learned warm start's features; no separate counterpart in QuAcK. */
+int qk_gw_screen(qk_gw_handle h, const double *e, int TDA_W);
+/* With the last screening (qk_gw_screen, or the last cycle of qk_evrgw):
+ * Sig(p) = Sigma_c(w_p) and dSig(p) = dSigma_c/dw(w_p)
+ * for every active orbital, G built from eG (GW/RGW_Re_SigC.f90,
+ * GW/RGW_Re_dSigC.f90 and the SRG variants; zero for frozen orbitals). */
+int qk_gw_sigma_at(qk_gw_handle h, const double *eG, const double *w, double eta, int doSRG, double *Sig,
+                   double *dSig);
+
 /* Precision of the excitation-density contraction rho = sum_jb (pq|jb) XpY(m,jb)
  * (GW/RGW_excitation_density.f90:48-50): 0 = FP64 MFMA (default, QuAcK's
  * arithmetic); 1 = split precision: each FP64 operand as three BF16 slices,
@@ -118,6 +129,27 @@ int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO
                     double *eGW, double *Z);
 
 const char *qk_last_error(void);
+
+/* ---- learned warm start: per-orbital MLP (no counterpart in QuAcK) ------ */
+
+/* nin -> width (SiLU) x depth hidden layers -> 1; FP32 weights, He-uniform
+ * initialisation from `seed`, zero biases.  width <= 256, multiple of 16. */
+typedef struct qk_mlp_ctx *qk_mlp_handle;
+int qk_mlp_create(qk_mlp_handle *h, int nin, int width, int depth, unsigned long long seed);
+int qk_mlp_destroy(qk_mlp_handle h);
+int qk_mlp_nparams(qk_mlp_handle h, int *n);
+/* flat FP32 parameters, layer by layer: W_l (out x in, row-major), b_l */
+int qk_mlp_get_params(qk_mlp_handle h, float *out);
+int qk_mlp_set_params(qk_mlp_handle h, const float *in);
+/* Y[n] from X[n x nin] (row-major); bf16 != 0 runs the fused BF16 matrix-core
+ * kernel (FP32 accumulation), else FP32 kernels. */
+int qk_mlp_forward(qk_mlp_handle h, const float *X, int n, float *Y, int bf16);
+/* MSE loss mean((Y - T)^2) and its gradient with respect to the parameters. */
+int qk_mlp_loss_grad(qk_mlp_handle h, const float *X, const float *T, int n, float *loss, float *grad);
+/* Adam (beta 0.9/0.999, eps 1e-8) on shuffled mini-batches, on the device. */
+int qk_mlp_train(qk_mlp_handle h, const float *X, const float *T, int n, int epochs, int batch, float lr,
+                 unsigned long long seed, float *loss_per_epoch);
+const char *qk_mlp_last_error(qk_mlp_handle h);
 
 #ifdef __cplusplus
 }

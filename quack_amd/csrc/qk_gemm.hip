@@ -23,9 +23,9 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int KT = 16;
 
 template <int WAVES_M, int WAVES_N, int BLK_M, int BLK_N, bool TA, bool TB>
-__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void dgemm_mfma(
-    int M, int N, int K, double alpha, const double* __restrict__ A, long long lda, const double* __restrict__ B,
-    long long ldb, double beta, double* __restrict__ C, long long ldc, Batch bt, int tiles_m) {
+__device__ __forceinline__ void dgemm_tile(int tile, int M, int N, int K, double alpha, const double* __restrict__ A,
+                                           long long lda, const double* __restrict__ B, long long ldb, double beta,
+                                           double* __restrict__ C, long long ldc, int tiles_m) {
     constexpr int NT = 64 * WAVES_M * WAVES_N;
     constexpr int BM = WAVES_M * BLK_M * 16;
     constexpr int BN = WAVES_N * BLK_N * 16;
@@ -38,13 +38,6 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void dgemm_mfma(
     __shared__ double As[KT * PADM];
     __shared__ double Bs[KT * PADN];
 
-    const int z = blockIdx.z;
-    const long long z0 = z % bt.zdiv, z1 = z / bt.zdiv;
-    A += z0 * bt.sa0 + z1 * bt.sa1;
-    B += z0 * bt.sb0 + z1 * bt.sb1;
-    C += z0 * bt.sc0 + z1 * bt.sc1;
-
-    const int tile = blockIdx.x;
     const int m0 = (tile % tiles_m) * BM;
     const int n0 = (tile / tiles_m) * BN;
     const int tid = threadIdx.x;
@@ -145,6 +138,36 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void dgemm_mfma(
     }
 }
 
+template <int WAVES_M, int WAVES_N, int BLK_M, int BLK_N, bool TA, bool TB>
+__global__ __launch_bounds__(64 * WAVES_M * WAVES_N, 2) void dgemm_mfma(
+    int M, int N, int K, double alpha, const double* __restrict__ A, long long lda, const double* __restrict__ B,
+    long long ldb, double beta, double* __restrict__ C, long long ldc, Batch bt, int tiles_m) {
+    const int z = blockIdx.z;
+    const long long z0 = z % bt.zdiv, z1 = z / bt.zdiv;
+    dgemm_tile<WAVES_M, WAVES_N, BLK_M, BLK_N, TA, TB>(blockIdx.x, M, N, K, alpha, A + z0 * bt.sa0 + z1 * bt.sa1, lda,
+                                                       B + z0 * bt.sb0 + z1 * bt.sb1, ldb, beta,
+                                                       C + z0 * bt.sc0 + z1 * bt.sc1, ldc, tiles_m);
+}
+
+// Grouped form: problem g owns tiles [tile0[g], tile0[g+1]); every workgroup
+// finds its problem by binary search, so problems of any shape share one launch.
+template <int WAVES_M, int WAVES_N, int BLK_M, int BLK_N, bool TA, bool TB>
+__global__ __launch_bounds__(64 * WAVES_M * WAVES_N, 2) void dgemm_mfma_grouped(const GemmProblem* __restrict__ pr,
+                                                                                 const int* __restrict__ tile0,
+                                                                                 int ngroup) {
+    int t = blockIdx.x;
+    int lo = 0, hi = ngroup - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (tile0[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const GemmProblem& q = pr[lo];
+    constexpr int BM = WAVES_M * BLK_M * 16;
+    int tiles_m = (q.M + BM - 1) / BM;
+    dgemm_tile<WAVES_M, WAVES_N, BLK_M, BLK_N, TA, TB>(t - tile0[lo], q.M, q.N, q.K, q.alpha, q.A, q.lda, q.B, q.ldb,
+                                                       q.beta, q.C, q.ldc, tiles_m);
+}
+
 template <int WM, int WN, int BMK, int BNK, bool TA, bool TB>
 void launch(hipStream_t s, int M, int N, int K, double alpha, const double* A, long long lda, const double* B,
             long long ldb, double beta, double* C, long long ldc, const Batch& bt) {
@@ -169,7 +192,24 @@ void dispatch(hipStream_t s, int M, int N, int K, double alpha, const double* A,
         launch<2, 2, 4, 4, TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);  // 128 x 128
 }
 
+template <int WM, int WN, int BMK, int BNK, bool TA, bool TB>
+void launch_grouped(hipStream_t s, const GemmProblem* dprob, const int* dtile0, int ngroup, int ntiles) {
+    hipLaunchKernelGGL((dgemm_mfma_grouped<WM, WN, BMK, BNK, TA, TB>), dim3(ntiles), dim3(64 * WM * WN), 0, s, dprob,
+                       dtile0, ngroup);
+    QK_HIP(hipGetLastError());
+}
+
 }  // namespace
+
+int dgemm_grouped_tile_rows(int cfg) { return cfg == 0 ? 64 : 128; }
+
+void dgemm_grouped_tt(hipStream_t s, int cfg, const GemmProblem* dprob, const int* dtile0, int ngroup, int ntiles) {
+    if (ntiles <= 0) return;
+    if (cfg == 0)
+        launch_grouped<2, 2, 2, 2, true, true>(s, dprob, dtile0, ngroup, ntiles);
+    else
+        launch_grouped<2, 2, 4, 4, true, true>(s, dprob, dtile0, ngroup, ntiles);
+}
 
 void dgemm(hipStream_t s, bool transA, bool transB, int M, int N, int K, double alpha, const double* A, long long lda,
            const double* B, long long ldb, double beta, double* C, long long ldc, const Batch& bt) {

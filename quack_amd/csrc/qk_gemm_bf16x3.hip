@@ -72,90 +72,122 @@ __global__ void k_split3_rc(const double* __restrict__ src, long long ld, int ro
     }
 }
 
-template <int WAVES_M, int WAVES_N, int WM, int WN>
-__global__ __launch_bounds__(64 * WAVES_M * WAVES_N) void k_gemm_bf16x3(
-    int M, int N, int kpad, const __bf16* __restrict__ A0, const __bf16* __restrict__ A1,
-    const __bf16* __restrict__ A2, const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
-    const __bf16* __restrict__ B2, double* __restrict__ C, long long ldc, int tiles_m) {
-    constexpr int BM = WAVES_M * WM * 16, BN = WAVES_N * WN * 16;
+// Workgroup tile BM x BN = 128 x 64 (4 wavefronts as 2 x 2, each 64 x 32 =
+// 4 x 2 MFMA tiles).  Per 32-wide k-slab the workgroup stages the three
+// slices of its A rows and B rows in LDS (36 KB); the next slab is fetched into
+// registers while the current one feeds the matrix cores.
+constexpr int BM = 128, BN = 64, NTH = 256;
+constexpr int LROW = 40;  // bf16 per LDS row: 32 k + 8 pad (80 B) against bank conflicts
+static_assert(BM == 128 && BN == 64 && NTH == 256, "slab staging assumes a 128 x 64 tile and 256 threads");
+
+__global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, const __bf16* __restrict__ A0,
+                                                     const __bf16* __restrict__ A1, const __bf16* __restrict__ A2,
+                                                     const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
+                                                     const __bf16* __restrict__ B2, double* __restrict__ C,
+                                                     long long ldc, int tiles_m) {
+    __shared__ __attribute__((aligned(16))) __bf16 As[3][BM][LROW];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[3][BN][LROW];
     const int tile = blockIdx.x;
     const int m0 = (tile % tiles_m) * BM, n0 = (tile / tiles_m) * BN;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm0 = m0 + (wave % WAVES_M) * WM * 16;
-    const int wn0 = n0 + (wave / WAVES_M) * WN * 16;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
     const int l16 = lane & 15, lk = (lane >> 4) * 8;
 
-    // row pointers (clamped to row 0 when outside; their results are never stored)
-    const __bf16 *pa[WM][3], *pb[WN][3];
-#pragma unroll
-    for (int a = 0; a < WM; ++a) {
-        int i = wm0 + a * 16 + l16;
-        long long off = (long long)(i < M ? i : 0) * kpad + lk;
-        pa[a][0] = A0 + off; pa[a][1] = A1 + off; pa[a][2] = A2 + off;
-    }
-#pragma unroll
-    for (int b = 0; b < WN; ++b) {
-        int j = wn0 + b * 16 + l16;
-        long long off = (long long)(j < N ? j : 0) * kpad + lk;
-        pb[b][0] = B0 + off; pb[b][1] = B1 + off; pb[b][2] = B2 + off;
-    }
+    // chunk c of the slab: A part c < ACH -> slice c / (BM*4), row (c/4) % BM, k-chunk c % 4
+    // slab chunks: thread tid owns row tid/4 (+64) and 16-byte k-chunk tid%4 of
+    // each A slice (6 chunks) and row tid/4 of each B slice (3 chunks)
+    const int rr = tid >> 2, kc8 = (tid & 3) * 8;
+    const int ia0 = m0 + rr, ia1 = m0 + rr + 64, jb0 = n0 + rr;
+    const long long oa0 = (long long)(ia0 < M ? ia0 : 0) * kpad + kc8;
+    const long long oa1 = (long long)(ia1 < M ? ia1 : 0) * kpad + kc8;
+    const long long ob0 = (long long)(jb0 < N ? jb0 : 0) * kpad + kc8;
+    const bool va0 = ia0 < M, va1 = ia1 < M, vb0 = jb0 < N;
+    uint4 pa00, pa01, pa10, pa11, pa20, pa21, pb0, pb1, pb2;
+    const uint4 zero4 = {0u, 0u, 0u, 0u};
+#define QK_FETCH(k0)                                                                     \
+    do {                                                                                 \
+        pa00 = va0 ? *reinterpret_cast<const uint4*>(A0 + oa0 + (k0)) : zero4;           \
+        pa01 = va1 ? *reinterpret_cast<const uint4*>(A0 + oa1 + (k0)) : zero4;           \
+        pa10 = va0 ? *reinterpret_cast<const uint4*>(A1 + oa0 + (k0)) : zero4;           \
+        pa11 = va1 ? *reinterpret_cast<const uint4*>(A1 + oa1 + (k0)) : zero4;           \
+        pa20 = va0 ? *reinterpret_cast<const uint4*>(A2 + oa0 + (k0)) : zero4;           \
+        pa21 = va1 ? *reinterpret_cast<const uint4*>(A2 + oa1 + (k0)) : zero4;           \
+        pb0 = vb0 ? *reinterpret_cast<const uint4*>(B0 + ob0 + (k0)) : zero4;            \
+        pb1 = vb0 ? *reinterpret_cast<const uint4*>(B1 + ob0 + (k0)) : zero4;            \
+        pb2 = vb0 ? *reinterpret_cast<const uint4*>(B2 + ob0 + (k0)) : zero4;            \
+    } while (0)
 
-    d4 acc64[WM][WN];
+    d4 acc64[4][2];
+    f4 acc[4][2];
 #pragma unroll
-    for (int a = 0; a < WM; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < WN; ++b) acc64[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < 2; ++b) {
+            acc64[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+            acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+        }
 
-    for (int kc = 0; kc < kpad; kc += KC) {
-        f4 acc[WM][WN];
+    QK_FETCH(0);
+    for (int k0 = 0; k0 < kpad; k0 += KS) {
+        __syncthreads();
+        *reinterpret_cast<uint4*>(&As[0][rr][kc8]) = pa00;
+        *reinterpret_cast<uint4*>(&As[0][rr + 64][kc8]) = pa01;
+        *reinterpret_cast<uint4*>(&As[1][rr][kc8]) = pa10;
+        *reinterpret_cast<uint4*>(&As[1][rr + 64][kc8]) = pa11;
+        *reinterpret_cast<uint4*>(&As[2][rr][kc8]) = pa20;
+        *reinterpret_cast<uint4*>(&As[2][rr + 64][kc8]) = pa21;
+        *reinterpret_cast<uint4*>(&Bs[0][rr][kc8]) = pb0;
+        *reinterpret_cast<uint4*>(&Bs[1][rr][kc8]) = pb1;
+        *reinterpret_cast<uint4*>(&Bs[2][rr][kc8]) = pb2;
+        __syncthreads();
+        if (k0 + KS < kpad) QK_FETCH(k0 + KS);
+        bf16x8 fb[2][3];
 #pragma unroll
-        for (int a = 0; a < WM; ++a)
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int b = 0; b < WN; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-        const int kend = min(kpad, kc + KC);
-        for (int k = kc; k < kend; k += KS) {
-            bf16x8 fa[WM][3], fb[WN][3];
+            for (int sl = 0; sl < 3; ++sl)
+                fb[b][sl] = *reinterpret_cast<const bf16x8*>(&Bs[sl][wn + b * 16 + l16][lk]);
 #pragma unroll
-            for (int a = 0; a < WM; ++a)
+        for (int a = 0; a < 4; ++a) {
+            bf16x8 fa[3];
 #pragma unroll
-                for (int s = 0; s < 3; ++s) fa[a][s] = *reinterpret_cast<const bf16x8*>(pa[a][s] + k);
+            for (int sl = 0; sl < 3; ++sl) fa[sl] = *reinterpret_cast<const bf16x8*>(&As[sl][wm + a * 16 + l16][lk]);
 #pragma unroll
-            for (int b = 0; b < WN; ++b)
+            for (int b = 0; b < 2; ++b) {
+                f4 c = acc[a][b];
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][2], fa[0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[1], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[2], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[1], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[0], c, 0, 0, 0);
+                acc[a][b] = c;
+            }
+        }
+        // FP64 recombination every KC k-values
+        if ((k0 + KS) % KC == 0 || k0 + KS >= kpad) {
 #pragma unroll
-                for (int s = 0; s < 3; ++s) fb[b][s] = *reinterpret_cast<const bf16x8*>(pb[b][s] + k);
+            for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int a = 0; a < WM; ++a)
+                for (int b = 0; b < 2; ++b) {
 #pragma unroll
-                for (int b = 0; b < WN; ++b) {
-                    // smallest terms first
-                    f4 c = acc[a][b];
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][2], fa[a][0], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[a][1], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[a][2], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[a][0], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[a][1], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[a][0], c, 0, 0, 0);
-                    acc[a][b] = c;
+                    for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc[a][b][r];
+                    acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
                 }
         }
-#pragma unroll
-        for (int a = 0; a < WM; ++a)
-#pragma unroll
-            for (int b = 0; b < WN; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc[a][b][r];
     }
 
-    // acc64[a][b][r] = C(row wm0 + a*16 + l16, col wn0 + b*16 + 4*(lane/16) + r)
+#undef QK_FETCH
+    // acc64[a][b][r] = C(row m0+wm+a*16+l16, col n0+wn+b*16+4*(lane/16)+r)
 #pragma unroll
-    for (int a = 0; a < WM; ++a) {
-        int i = wm0 + a * 16 + l16;
+    for (int a = 0; a < 4; ++a) {
+        int i = m0 + wm + a * 16 + l16;
         if (i >= M) continue;
 #pragma unroll
-        for (int b = 0; b < WN; ++b)
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int j = wn0 + b * 16 + 4 * (lane >> 4) + r;
+                int j = n0 + wn + b * 16 + 4 * (lane >> 4) + r;
                 if (j < N) C[i + ldc * (long long)j] = acc64[a][b][r];
             }
     }
@@ -181,12 +213,10 @@ void split3(hipStream_t s, const double* src, long long ld, bool k_contiguous, i
 
 void gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc) {
     QK_CHECK(A.kpad == B.kpad && A.rows == M && B.rows == N, QK_E_ARG, "gemm_bf16x3: operand shapes");
-    constexpr int WAVES_M = 2, WAVES_N = 2, WM = 2, WN = 4;
-    constexpr int BM = WAVES_M * WM * 16, BN = WAVES_N * WN * 16;
     int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
     QK_CHECK((long long)tm * tn < (1LL << 31), QK_E_ARG, "gemm_bf16x3: grid too large");
-    hipLaunchKernelGGL((k_gemm_bf16x3<WAVES_M, WAVES_N, WM, WN>), dim3(tm * tn), dim3(64 * WAVES_M * WAVES_N), 0, s,
-                       M, N, A.kpad, A.s[0], A.s[1], A.s[2], B.s[0], B.s[1], B.s[2], C, ldc, tm);
+    hipLaunchKernelGGL(k_gemm_bf16x3, dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1], A.s[2], B.s[0],
+                       B.s[1], B.s[2], C, ldc, tm);
     QK_HIP(hipGetLastError());
 }
 

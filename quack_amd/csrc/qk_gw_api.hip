@@ -161,6 +161,7 @@ struct qk_gw_ctx {
     int *nit = nullptr, *done = nullptr, *info = nullptr, *active = nullptr;
     int nchunk = 1;
     bool have_eri = false;
+    bool have_screen = false;
     int precision = 0;        // 0: FP64 MFMA, 1: BF16x3 split with FP64 recombination
     qk::Split3 Msplit, Wsplit;  // M^T and W slices (mixed precision)
     bool msplit_valid = false;
@@ -325,6 +326,7 @@ void free_split(Split3& sp) {
 
 void after_eri(Ctx* c) {
     c->msplit_valid = false;
+    c->have_screen = false;
     extract_k(c->s, c->M, c->K, c->nOrb, c->nC, c->nO, c->v, c->nS);
     QK_HIP(hipStreamSynchronize(c->s));
     c->have_eri = true;
@@ -522,6 +524,36 @@ int qk_gw_set_eri_mo(qk_gw_handle c, const double* ERI_MO) {
     });
 }
 
+int qk_gw_screen(qk_gw_handle c, const double* e, int TDA_W) {
+    if (!c || !e) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        QK_CHECK(c->have_eri, QK_E_ARG, "no integrals: call qk_gw_set_eri_* first");
+        for (double& t : c->t_stage) t = 0.0;
+        h2d(c, c->e, e, c->nOrb);
+        screening(c, c->e, TDA_W, true);
+        excitation(c);
+        QK_HIP(hipStreamSynchronize(c->s));
+        c->have_screen = true;
+    });
+}
+
+int qk_gw_sigma_at(qk_gw_handle c, const double* eG, const double* w, double eta, int doSRG, double* Sig,
+                   double* dSig) {
+    if (!c || !eG || !w || !Sig || !dSig) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        QK_CHECK(c->have_screen, QK_E_ARG, "no screening: call qk_gw_screen first");
+        int n = c->nOrb;
+        h2d(c, c->eG, eG, n);
+        h2d(c, c->w, w, n);
+        sigma_partial(c->s, c->rhoT, c->w, c->eG, c->Om, nullptr, eta, doSRG, SRG_FLOW, n, c->nC, c->nO, c->nR,
+                      c->nS, c->nchunk, c->part);
+        sigma_out(c->s, c->part, c->nchunk, n, c->nC, c->nR, c->f, c->df);
+        d2h(c, Sig, c->f, n);
+        d2h(c, dSig, c->df, n);
+        QK_HIP(hipStreamSynchronize(c->s));
+    });
+}
+
 int qk_gw_set_precision(qk_gw_handle c, int mode) {
     if (!c || mode < 0 || mode > 1) return QK_E_ARG;
     return guarded(&c->err, [&] {
@@ -554,6 +586,7 @@ int qk_rg0w0(qk_gw_handle c, const double* eHF, double eta, int linearize, int T
         screening(c, c->e0, TDA_W, true);
         QK_HIP(hipMemcpyAsync(c->Om0, c->Om, sizeof(double) * c->nS, hipMemcpyDeviceToDevice, c->s));
         excitation(c);
+        c->have_screen = true;
         // self-energy (:169-177) and QP equation (:188-208)
         sigma_static(c, c->e0, eta, doSRG);
         double ecgm = d2h_scalar(c, c->scal + 2);
@@ -566,6 +599,7 @@ int qk_rg0w0(qk_gw_handle c, const double* eHF, double eta, int linearize, int T
         }
         // RPA correlation energy at eGW (:222-225): eigenvalues only
         screening(c, c->eGW, TDA_W, false);
+        c->have_screen = false;  // the eigenvectors are gone
         double ecrpa = d2h_scalar(c, c->scal + 1);
         d2h(c, eGW, c->eGW, n);
         d2h(c, Z, c->Z, n);
@@ -600,6 +634,7 @@ int qk_evrgw(qk_gw_handle c, const double* eHF, const double* eGW_start, int max
             screening(c, c->e, TDA_W, true);
             ecrpa = d2h_scalar(c, c->scal + 1);
             excitation(c);
+            c->have_screen = true;
             sigma_static(c, c->e, eta, doSRG);
             ecgm = d2h_scalar(c, c->scal + 2);
             if (linearize) {

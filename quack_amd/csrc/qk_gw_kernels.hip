@@ -195,6 +195,22 @@ __global__ void k_sigma_finalize(const double* __restrict__ part, int nchunk, in
     if (threadIdx.x == 0) out[2] = -2.0 * r;
 }
 
+// Sig(p), dSig(p) for every active p from the partial sums (zero elsewhere)
+__global__ void k_sigma_out(const double* __restrict__ part, int nchunk, int n, int nC, int nR, double* __restrict__ Sig,
+                            double* __restrict__ dSig) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double s = 0.0, d = 0.0;
+    if (p >= nC && p < n - nR)
+        for (int c = 0; c < nchunk; ++c) {
+            const double* o = part + 3LL * ((long long)p * nchunk + c);
+            s += o[0] + o[1];
+            d += o[2];
+        }
+    Sig[p] = s;
+    dSig[p] = d;
+}
+
 // eGWlin = e0 + Z*Sig (GW/RG0W0.f90:188) ; or e0 + Sig (evRGW.f90:147, zflag = 0)
 __global__ void k_lin(const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -378,6 +394,10 @@ void sigma_partial(hipStream_t s, const double* rhoT, const double* w, const dou
 void sigma_finalize(hipStream_t s, const double* part, int nchunk, int n, int nC, int nO, int nR, double* Sig,
                     double* Z, double* scal) {
     hipLaunchKernelGGL(k_sigma_finalize, dim3(1), dim3(RB), 0, s, part, nchunk, n, nC, nO, nR, Sig, Z, scal);
+    QK_HIP(hipGetLastError());
+}
+void sigma_out(hipStream_t s, const double* part, int nchunk, int n, int nC, int nR, double* Sig, double* dSig) {
+    hipLaunchKernelGGL(k_sigma_out, dim3((n + 255) / 256), dim3(256), 0, s, part, nchunk, n, nC, nR, Sig, dSig);
     QK_HIP(hipGetLastError());
 }
 void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag) {

@@ -18,6 +18,7 @@ void sigma_partial(hipStream_t s, const double* rhoT, const double* w, const dou
                    int nchunk, double* part);
 void sigma_finalize(hipStream_t s, const double* part, int nchunk, int n, int nC, int nO, int nR, double* Sig,
                     double* Z, double* scal);
+void sigma_out(hipStream_t s, const double* part, int nchunk, int n, int nC, int nR, double* Sig, double* dSig);
 void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag);
 void qp_init(hipStream_t s, const double* eGWlin, double* w, double* f, int* nit, int* done, int n, int nC, int nR);
 void qp_update(hipStream_t s, const double* part, int nchunk, const double* eHF, double* w, double* f, double* df,

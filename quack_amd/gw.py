@@ -24,7 +24,9 @@ _NAMES = {1: "QK_E_ARG", 2: "QK_E_HIP", 3: "QK_E_EIG", 4: "QK_E_NOCONV", 5: "QK_
 EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_ao", "qk_gw_set_eri_ao_device",
            "qk_gw_set_eri_mo", "qk_rg0w0", "qk_evrgw", "qk_gw_stage_times", "qk_aotomo_eri_rhf", "qk_phrlr",
            "qk_phrlr_a", "qk_phrlr_b", "qk_rgw_excitation_density", "qk_rgw_self_energy_diag", "qk_rgw_qp_graph",
-           "qk_last_error", "qk_gw_set_precision"]
+           "qk_last_error", "qk_gw_set_precision", "qk_mlp_create", "qk_mlp_destroy", "qk_mlp_nparams",
+           "qk_mlp_get_params", "qk_mlp_set_params", "qk_mlp_forward", "qk_mlp_loss_grad", "qk_mlp_train",
+           "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at"]
 
 
 class QKError(RuntimeError):
@@ -72,6 +74,19 @@ def load():
     lib.qk_evrgw.argtypes = [_H, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
     lib.qk_gw_stage_times.argtypes = [_H, _P, _I]
     lib.qk_gw_set_precision.argtypes = [_H, _I]
+    lib.qk_gw_screen.argtypes = [_H, _P, _I]
+    lib.qk_gw_sigma_at.argtypes = [_H, _P, _P, _D, _I, _P, _P]
+    _F = ctypes.POINTER(ctypes.c_float)
+    lib.qk_mlp_create.argtypes = [ctypes.POINTER(_H), _I, _I, _I, ctypes.c_ulonglong]
+    lib.qk_mlp_destroy.argtypes = [_H]
+    lib.qk_mlp_nparams.argtypes = [_H, _IP]
+    lib.qk_mlp_get_params.argtypes = [_H, _F]
+    lib.qk_mlp_set_params.argtypes = [_H, _F]
+    lib.qk_mlp_forward.argtypes = [_H, _F, _I, _F, _I]
+    lib.qk_mlp_loss_grad.argtypes = [_H, _F, _F, _I, _F, _F]
+    lib.qk_mlp_train.argtypes = [_H, _F, _F, _I, _I, _I, ctypes.c_float, ctypes.c_ulonglong, _F]
+    lib.qk_mlp_last_error.argtypes = [_H]
+    lib.qk_mlp_last_error.restype = ctypes.c_char_p
     lib.qk_aotomo_eri_rhf.argtypes = [_I, _I, _P, _P, _P]
     lib.qk_phrlr.argtypes = [_I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_phrlr_a.argtypes = [_I] * 7 + [_D, _P, _P, _P]
@@ -179,6 +194,15 @@ class GWEngine:
         _check(rc, self.h)
         return res
 
+    def screen(self, e, TDA_W=False):
+        _check(load().qk_gw_screen(self.h, _p(_f64(e)), int(TDA_W)), self.h)
+
+    def sigma_at(self, eG, w, eta=0.0, doSRG=False):
+        n = self.nOrb
+        Sig, dSig = np.zeros(n), np.zeros(n)
+        _check(load().qk_gw_sigma_at(self.h, _p(_f64(eG)), _p(_f64(w)), eta, int(doSRG), _p(Sig), _p(dSig)), self.h)
+        return Sig, dSig
+
     def stage_times(self):
         ms = np.zeros(6)
         _check(load().qk_gw_stage_times(self.h, _p(ms), 6), self.h)
@@ -269,3 +293,70 @@ def RGW_QP_graph(eHF, Om, rho, eGWlin, eOld, Z, nO, eta=0.0, doSRG=False, flow=5
     _check(load().qk_rgw_qp_graph(int(doSRG), eta, flow, n, nC, nO, nV, nR, nS, _p(_f64(eHF)), _p(_f64(Om)),
                                   _p(_f64(rho)), _p(_f64(eGWlin)), _p(_f64(eOld)), _p(eGW), _p(Zio)))
     return eGW, Zio
+
+
+# ---- per-orbital MLP (learned warm start) ---------------------------------
+
+class MLP:
+    """Device MLP over the qk_mlp_* C-ABI (FP32 weights; FP32 or BF16 inference)."""
+
+    def __init__(self, nin=14, width=128, depth=3, seed=0):
+        lib = load()
+        h = _H()
+        _check(lib.qk_mlp_create(ctypes.byref(h), nin, width, depth, seed))
+        self.h, self.nin, self.width, self.depth = h, nin, width, depth
+        n = _I(0)
+        lib.qk_mlp_nparams(h, ctypes.byref(n))
+        self.nparams = n.value
+
+    def _chk(self, rc):
+        if rc != QK_OK:
+            raise QKError(rc, load().qk_mlp_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().qk_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _f(a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def get_params(self):
+        out = np.zeros(self.nparams, np.float32)
+        self._chk(load().qk_mlp_get_params(self.h, self._f(out)[1]))
+        return out
+
+    def set_params(self, p):
+        a, ptr = self._f(p)
+        assert a.size == self.nparams
+        self._chk(load().qk_mlp_set_params(self.h, ptr))
+
+    def forward(self, X, bf16=False):
+        X, px = self._f(X)
+        n = X.shape[0]
+        Y = np.zeros(n, np.float32)
+        self._chk(load().qk_mlp_forward(self.h, px, n, self._f(Y)[1], int(bf16)))
+        return Y
+
+    def loss_grad(self, X, T):
+        X, px = self._f(X)
+        T, pt = self._f(T)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.nparams, np.float32)
+        self._chk(load().qk_mlp_loss_grad(self.h, px, pt, X.shape[0], self._f(loss)[1], self._f(g)[1]))
+        return float(loss[0]), g
+
+    def train(self, X, T, epochs=100, batch=1024, lr=1e-3, seed=0):
+        X, px = self._f(X)
+        T, pt = self._f(T)
+        hist = np.zeros(max(epochs, 1), np.float32)
+        self._chk(load().qk_mlp_train(self.h, px, pt, X.shape[0], epochs, batch, lr, seed, self._f(hist)[1]))
+        return hist[:epochs]

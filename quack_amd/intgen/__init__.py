@@ -214,7 +214,11 @@ def rhf(S, T, V, eri, nO, maxit=200, conv=1e-10, jk=jk_numpy, ndiis=8):
     # final orbitals from the converged Fock matrix
     e, Cp = np.linalg.eigh(X.T @ F @ X)
     c = X @ Cp
-    return {"eHF": e, "c": np.asfortranarray(c), "ERHF": E, "nOrb": X.shape[1], "niter": it}
+    # diagonal exchange self-energy in the MO basis, -(c^T K c)_pp
+    Co = c[:, :nO]
+    J, K = jk(eri, Co @ Co.T)
+    Sx = -np.einsum("mp,mn,np->p", c, K, c)
+    return {"eHF": e, "c": np.asfortranarray(c), "ERHF": E, "nOrb": X.shape[1], "niter": it, "Sx": Sx}
 
 
 def system(mol, jk=jk_numpy):
