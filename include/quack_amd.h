@@ -91,6 +91,23 @@ int qk_gw_sigma_at(qk_gw_handle h, const double *eG, const double *w, double eta
  * FP64 in both modes.  No counterpart in QuAcK (which is FP64 throughout). */
 int qk_gw_set_precision(qk_gw_handle h, int mode);
 
+/* Eigensolver of the RPA problem inside qk_evrgw, qk_gw_screen and the first
+ * RPA of qk_rg0w0 (LR/phRLR.f90:77, diagonalize_matrix = dsyev):
+ *   mode 0 (default): rocSOLVER dsyevd for every RPA, as QuAcK re-diagonalises
+ *           every cycle;
+ *   mode 1: when no orbital energy moved by more than de_max (Ha) since the
+ *           handle's last eigenvectors were computed (same TDA_W), refine those
+ *           eigenvectors by Newton steps for X^T X = I, X^T C X diagonal (FP64
+ *           matrix-core GEMMs, quadratic convergence, stopped when the largest
+ *           correction is below 1e-8); dsyevd otherwise, or when a correction
+ *           exceeds 0.25 or six steps do not converge.
+ * Both modes return FP64 eigenpairs of the same matrix.  No counterpart in
+ * QuAcK.  Mode 1 keeps four nS x nS FP64 work arrays on the device. */
+int qk_gw_set_eigensolver(qk_gw_handle h, int mode, double de_max);
+/* counts[0..n-1], n <= 4: dsyevd calls, refined solves, Newton steps, and
+ * refinements abandoned for dsyevd, since the handle was created. */
+int qk_gw_eig_stats(qk_gw_handle h, long long *counts, int n);
+
 /* Device time (ms, HIP events on the handle's stream) of the stages of the
  * last qk_rg0w0 / qk_evrgw call: [0] AO->MO block (last set_eri_*),
  * [1] RPA matrix build, [2] eigensolve, [3] excitation densities,

@@ -165,6 +165,18 @@ struct qk_gw_ctx {
     int precision = 0;        // 0: FP64 MFMA, 1: BF16x3 split with FP64 recombination
     qk::Split3 Msplit, Wsplit;  // M^T and W slices (mixed precision)
     bool msplit_valid = false;
+    // eigensolver: 0 = rocSOLVER dsyevd for every RPA (QuAcK's scheme); 1 = refine
+    // the previous eigenvectors by Newton steps when the energies moved by at
+    // most eig_demax since they were computed, dsyevd otherwise or on failure
+    int eig_mode = 0;
+    double eig_demax = 5e-3;
+    double *Xv = nullptr, *Ym = nullptr, *Sm = nullptr, *Gm = nullptr, *lam = nullptr, *eig_dbuf = nullptr;
+    int *perm = nullptr, *eig_ibuf = nullptr;
+    long long* eig_lbuf = nullptr;
+    bool have_vec = false;
+    int tda_vec = 0;
+    std::vector<double> e_vec;
+    long long eig_count[4] = {0, 0, 0, 0};  // dsyevd calls, refined solves, Newton steps, fallbacks
     double t_ao = 0.0;
     double t_stage[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -198,15 +210,336 @@ void check_sizes(int nBas, int nOrb, int nC, int nO, int nV, int nR, int nS) {
     QK_CHECK((long long)nS * nS < (1LL << 40), QK_E_ARG, "nS too large");
 }
 
-// RPA screening at energies e (device): Om, and W = scaled eigenvectors in Cm.
-void screening(Ctx* c, const double* e, int tda, bool vectors) {
+void h2d(Ctx* c, double* dst, const double* src, size_t count) {
+    QK_HIP(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyHostToDevice, c->s));
+}
+void d2h(Ctx* c, double* dst, const double* src, size_t count) {
+    if (dst) QK_HIP(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, c->s));
+}
+double d2h_scalar(Ctx* c, const double* src) {
+    double v = 0.0;
+    QK_HIP(hipMemcpyAsync(&v, src, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    QK_HIP(hipStreamSynchronize(c->s));
+    return v;
+}
+
+constexpr int EIG_MAXIT = 8;       // Newton steps before falling back to dsyevd
+constexpr double EIG_BIG = 0.25;   // |E_ij| above this: outside the convergence region
+constexpr double EIG_TOL = 1e-8;   // last step: the next correction would be O(1e-16)
+constexpr double EIG_DEG = 1e-9;   // eigenvalues closer than this (relative) form a cluster
+constexpr int EIG_KMAX = 32;       // largest cluster handled
+
+// Cyclic Jacobi eigen-decomposition of a small symmetric matrix A (k x k,
+// column-major, destroyed): V = eigenvectors (columns), w = eigenvalues,
+// ascending.  Rotation angles as in Golub & Van Loan, Matrix Computations,
+// sec. 8.5 (symmetric Schur decomposition).
+void jacobi_eig(int k, std::vector<double>& A, std::vector<double>& V, std::vector<double>& w) {
+    V.assign((size_t)k * k, 0.0);
+    for (int i = 0; i < k; ++i) V[i + (size_t)k * i] = 1.0;
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int q = 0; q < k; ++q)
+            for (int p = 0; p < k; ++p) {
+                double a = A[p + (size_t)k * q] * A[p + (size_t)k * q];
+                tot += a;
+                if (p != q) off += a;
+            }
+        if (off <= 1e-34 * tot) break;
+        for (int p = 0; p < k - 1; ++p)
+            for (int q = p + 1; q < k; ++q) {
+                double apq = A[p + (size_t)k * q];
+                if (apq == 0.0) continue;
+                double tau = (A[q + (size_t)k * q] - A[p + (size_t)k * p]) / (2.0 * apq);
+                double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                double cs = 1.0 / std::sqrt(1.0 + t * t), sn = t * cs;
+                for (int r = 0; r < k; ++r) {  // A <- A J
+                    double x = A[r + (size_t)k * p], y = A[r + (size_t)k * q];
+                    A[r + (size_t)k * p] = cs * x - sn * y;
+                    A[r + (size_t)k * q] = sn * x + cs * y;
+                }
+                for (int r = 0; r < k; ++r) {  // A <- J^T A
+                    double x = A[p + (size_t)k * r], y = A[q + (size_t)k * r];
+                    A[p + (size_t)k * r] = cs * x - sn * y;
+                    A[q + (size_t)k * r] = sn * x + cs * y;
+                }
+                for (int r = 0; r < k; ++r) {  // V <- V J
+                    double x = V[r + (size_t)k * p], y = V[r + (size_t)k * q];
+                    V[r + (size_t)k * p] = cs * x - sn * y;
+                    V[r + (size_t)k * q] = sn * x + cs * y;
+                }
+            }
+    }
+    std::vector<int> o(k);
+    for (int i = 0; i < k; ++i) o[i] = i;
+    std::sort(o.begin(), o.end(), [&](int a, int b) { return A[a + (size_t)k * a] < A[b + (size_t)k * b]; });
+    std::vector<double> V2((size_t)k * k);
+    w.resize(k);
+    for (int j = 0; j < k; ++j) {
+        w[j] = A[o[j] + (size_t)k * o[j]];
+        for (int r = 0; r < k; ++r) V2[r + (size_t)k * j] = V[r + (size_t)k * o[j]];
+    }
+    V.swap(V2);
+}
+
+// Clusters of the refinement: labels (merge only) and their member lists.
+struct EigClusters {
+    std::vector<int> lab, ptr, idx, scl, loc;
+    std::vector<long long> boff;
+    int ncl = 0, m = 0;
+    long long nblk = 0;
+    explicit EigClusters(int n) : lab(n) {
+        for (int i = 0; i < n; ++i) lab[i] = i;
+    }
+    void merge(int a, int b) {
+        int la = lab[a], lb = lab[b];
+        if (la == lb) return;
+        int keep = std::min(la, lb), drop = std::max(la, lb);
+        for (int& l : lab)
+            if (l == drop) l = keep;
+    }
+    // members of clusters of size >= 2; false if one exceeds EIG_KMAX
+    bool build() {
+        int n = (int)lab.size();
+        std::vector<int> cnt(n, 0);
+        for (int l : lab) ++cnt[l];
+        ptr.assign(1, 0);
+        idx.clear(); scl.clear(); boff.clear();
+        std::vector<int> first(n, -1);
+        ncl = 0;
+        for (int l = 0; l < n; ++l)
+            if (cnt[l] >= 2) {
+                if (cnt[l] > EIG_KMAX) return false;
+                first[l] = ncl++;
+            }
+        std::vector<std::vector<int>> mem(ncl);
+        for (int i = 0; i < n; ++i)
+            if (first[lab[i]] >= 0) mem[first[lab[i]]].push_back(i);
+        nblk = 0;
+        for (int c = 0; c < ncl; ++c) {
+            boff.push_back(nblk);
+            nblk += (long long)mem[c].size() * mem[c].size();
+            for (int i : mem[c]) {
+                idx.push_back(i);
+                scl.push_back(c);
+            }
+            ptr.push_back((int)idx.size());
+        }
+        m = (int)idx.size();
+        loc.resize(m);
+        for (int s = 0; s < m; ++s) loc[s] = s;
+        return true;
+    }
+};
+
+// Upload the member lists; ib: ptr | idx | scl | loc | lab, lb: boff.
+void upload_clusters(Ctx* c, const EigClusters& cl) {
+    int n = c->nS;
+    int* ib = c->eig_ibuf;
+    QK_HIP(hipMemcpyAsync(ib, cl.ptr.data(), sizeof(int) * cl.ptr.size(), hipMemcpyHostToDevice, c->s));
+    if (cl.m > 0) {
+        QK_HIP(hipMemcpyAsync(ib + (n + 1), cl.idx.data(), sizeof(int) * cl.m, hipMemcpyHostToDevice, c->s));
+        QK_HIP(hipMemcpyAsync(ib + (2 * n + 1), cl.scl.data(), sizeof(int) * cl.m, hipMemcpyHostToDevice, c->s));
+        QK_HIP(hipMemcpyAsync(ib + (3 * n + 1), cl.loc.data(), sizeof(int) * cl.m, hipMemcpyHostToDevice, c->s));
+        QK_HIP(hipMemcpyAsync(c->eig_lbuf, cl.boff.data(), sizeof(long long) * cl.ncl, hipMemcpyHostToDevice, c->s));
+    }
+    QK_HIP(hipMemcpyAsync(ib + (4 * n + 1), cl.lab.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->s));
+}
+
+// Rotation inside each cluster that diagonalises the cluster block of S in the
+// metric G: T = S + (R S + S R)/2 (R = I - G, first order), T = Q diag(w) Q^T;
+// with orth, W = (I + R/2) Q (orthonormalising too).  Blocks from the device
+// arrays bS, bG (packed, cl.boff); W packed the same way into Wh.
+void cluster_rotations(const EigClusters& cl, const std::vector<double>& bS, const std::vector<double>& bG, bool orth,
+                       std::vector<double>& Wh, std::vector<double>* wout) {
+    Wh.assign(cl.nblk, 0.0);
+    std::vector<double> T, Q, w;
+    for (int c = 0; c < cl.ncl; ++c) {
+        int k = cl.ptr[c + 1] - cl.ptr[c];
+        const double* S = bS.data() + cl.boff[c];
+        const double* G = bG.data() + cl.boff[c];
+        T.assign((size_t)k * k, 0.0);
+        for (int q = 0; q < k; ++q)
+            for (int p = 0; p < k; ++p) {
+                double t = S[p + k * q];
+                for (int u = 0; u < k; ++u) {
+                    double rpu = (p == u ? 1.0 : 0.0) - G[p + k * u];
+                    double ruq = (u == q ? 1.0 : 0.0) - G[u + k * q];
+                    t += 0.5 * (rpu * S[u + k * q] + S[p + k * u] * ruq);
+                }
+                T[p + (size_t)k * q] = t;
+            }
+        for (int q = 0; q < k; ++q)  // symmetrise against rounding
+            for (int p = 0; p < q; ++p) {
+                double a = 0.5 * (T[p + (size_t)k * q] + T[q + (size_t)k * p]);
+                T[p + (size_t)k * q] = T[q + (size_t)k * p] = a;
+            }
+        jacobi_eig(k, T, Q, w);
+        double* W = Wh.data() + cl.boff[c];
+        for (int q = 0; q < k; ++q)
+            for (int p = 0; p < k; ++p) {
+                double v = Q[p + (size_t)k * q];
+                if (orth)
+                    for (int u = 0; u < k; ++u) v += 0.5 * ((p == u ? 1.0 : 0.0) - G[p + k * u]) * Q[u + (size_t)k * q];
+                W[p + k * q] = v;
+            }
+        if (wout)
+            for (int r = 0; r < k; ++r) (*wout)[cl.idx[cl.ptr[c] + r]] = w[r];
+    }
+}
+
+// Eigen-decomposition of C (in Cm, intact on failure) from the eigenvectors Xv
+// of a nearby matrix.  Quadratically convergent Newton iteration for
+// X^T X = I, X^T C X diagonal: Y = C X, S = X^T Y, G = X^T X, X <- X (I + E).
+// Exactly or nearly degenerate eigenvalues (relative gap <= 1e-9, or a
+// correction above 0.25 between them) form clusters whose vectors are rotated
+// to diagonalise their block before each step and once more at the end.
+// On success Cm = eigenvectors and Om = eigenvalues, ascending (as dsyevd).
+bool refine_eig(Ctx* c) {
+    const int n = c->nS;
+    hipStream_t s = c->s;
+    double *X = c->Xv, *Y = c->Ym;
+    std::vector<double> lam(n), colmax(n), bS, bG, Wh;
+    std::vector<int> colarg(n), order(n), pos(n);
+    EigClusters cl(n);
+    int* d_ptr = c->eig_ibuf;
+    int* d_idx = d_ptr + (n + 1);
+    int* d_scl = d_ptr + (2 * n + 1);
+    int* d_loc = d_ptr + (3 * n + 1);
+    int* d_lab = d_ptr + (4 * n + 1);
+    int* d_arg = d_ptr + (5 * n + 1);
+    double* d_Q = c->eig_dbuf;
+    double* d_bS = d_Q + (size_t)EIG_KMAX * n;
+    double* d_bG = d_bS + (size_t)EIG_KMAX * n;
+    double* d_cmax = d_bG + (size_t)EIG_KMAX * n;
+
+    auto fetch_lam = [&] {
+        eig_lambda(s, c->Sm, c->Gm, c->lam, n);
+        QK_HIP(hipMemcpyAsync(lam.data(), c->lam, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return lam[a] < lam[b]; });
+        for (int k = 0; k < n; ++k) pos[order[k]] = k;
+    };
+    auto fetch_blocks = [&](const double* A, const double* B, long long lda, const int* dix) {
+        gather_blocks(s, A, B, lda, d_ptr, dix, c->eig_lbuf, cl.ncl, d_bS, d_bG);
+        bS.resize(cl.nblk);
+        bG.resize(cl.nblk);
+        QK_HIP(hipMemcpyAsync(bS.data(), d_bS, sizeof(double) * cl.nblk, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipMemcpyAsync(bG.data(), d_bG, sizeof(double) * cl.nblk, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipStreamSynchronize(s));
+    };
+
+    bool ok = false;
+    for (int it = 0; it < EIG_MAXIT && !ok; ++it) {
+        dgemm(s, false, false, n, n, n, 1.0, c->Cm, n, X, n, 0.0, Y, n);
+        dgemm(s, true, false, n, n, n, 1.0, X, n, Y, n, 0.0, c->Sm, n);
+        dgemm(s, true, false, n, n, n, 1.0, X, n, X, n, 0.0, c->Gm, n);
+        ++c->eig_count[2];
+        fetch_lam();
+        double lmax = 0.0;
+        for (double l : lam) lmax = std::max(lmax, std::fabs(l));
+        if (!(lmax < 1e300)) break;
+        for (int k = 0; k + 1 < n; ++k)
+            if (lam[order[k + 1]] - lam[order[k]] <= EIG_DEG * lmax) cl.merge(order[k], order[k + 1]);
+        bool settled = false;
+        for (int rep = 0; rep < 12 && !settled; ++rep) {
+            if (!cl.build()) {
+                c->Xv = X;
+                c->Ym = Y;
+                return false;
+            }
+            upload_clusters(c, cl);
+            if (cl.ncl > 0) {
+                fetch_blocks(c->Sm, c->Gm, n, d_idx);
+                cluster_rotations(cl, bS, bG, false, Wh, nullptr);
+                QK_HIP(hipMemcpyAsync(d_Q, Wh.data(), sizeof(double) * cl.nblk, hipMemcpyHostToDevice, s));
+                for (double* A : {c->Sm, c->Gm}) {
+                    mix_cols(s, A, n, n, d_idx, A, n, d_idx, d_ptr, d_scl, d_Q, c->eig_lbuf, cl.m, Y);
+                    mix_rows(s, A, n, d_idx, d_ptr, d_scl, d_Q, c->eig_lbuf, cl.m, Y);
+                }
+                mix_cols(s, X, n, n, d_idx, X, n, d_idx, d_ptr, d_scl, d_Q, c->eig_lbuf, cl.m, Y);
+                fetch_lam();
+            }
+            eig_check(s, c->Sm, c->Gm, c->lam, d_lab, n, d_cmax, d_arg);
+            QK_HIP(hipMemcpyAsync(colmax.data(), d_cmax, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+            QK_HIP(hipMemcpyAsync(colarg.data(), d_arg, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+            QK_HIP(hipStreamSynchronize(s));
+            settled = true;
+            for (int j = 0; j < n; ++j)
+                if (!(colmax[j] <= EIG_BIG)) {
+                    settled = false;
+                    int a = std::min(pos[j], pos[colarg[j]]), b = std::max(pos[j], pos[colarg[j]]);
+                    for (int k = a; k < b; ++k) cl.merge(order[k], order[k + 1]);
+                }
+        }
+        if (!settled) break;
+        double emax = 0.0;
+        for (double v : colmax) emax = std::max(emax, v);
+        eig_update(s, c->Sm, c->Gm, c->lam, d_lab, n);
+        dgemm(s, false, false, n, n, n, 1.0, X, n, c->Sm, n, 0.0, Y, n);
+        std::swap(X, Y);
+        ok = emax < EIG_TOL;
+    }
+    c->Xv = X;
+    c->Ym = Y;
+    if (!ok) return false;
+    // final Rayleigh-Ritz inside the clusters with the converged vectors
+    if (!cl.build()) return false;
+    if (cl.ncl > 0) {
+        upload_clusters(c, cl);
+        const int m = cl.m;
+        gather_cols(s, X, n, d_idx, m, c->Sm);                                  // Xc (n x m)
+        dgemm(s, false, false, n, m, n, 1.0, c->Cm, n, c->Sm, n, 0.0, c->Gm, n);  // C Xc
+        dgemm(s, true, false, m, m, n, 1.0, c->Sm, n, c->Gm, n, 0.0, Y, m);      // Xc^T C Xc
+        dgemm(s, true, false, m, m, n, 1.0, c->Sm, n, c->Sm, n, 0.0, c->Gm, m);  // Xc^T Xc
+        fetch_blocks(Y, c->Gm, m, d_loc);
+        cluster_rotations(cl, bS, bG, true, Wh, &lam);
+        QK_HIP(hipMemcpyAsync(d_Q, Wh.data(), sizeof(double) * cl.nblk, hipMemcpyHostToDevice, s));
+        mix_cols(s, c->Sm, n, n, d_loc, X, n, d_idx, d_ptr, d_scl, d_Q, c->eig_lbuf, m, Y);
+    }
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return lam[a] < lam[b]; });
+    std::vector<double> sorted(n);
+    for (int i = 0; i < n; ++i) sorted[i] = lam[perm[i]];
+    QK_HIP(hipMemcpyAsync(c->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(c->Om, sorted.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    permute_cols(s, X, c->perm, c->Cm, n);
+    QK_HIP(hipStreamSynchronize(s));  // host vectors leave scope
+    return true;
+}
+
+// RPA screening at energies e (device; eh = the same energies on the host).
+// Cm = eigenvectors (scaled to W when vectors), Om = excitation energies.
+void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors) {
     {
         Stage st(c, 0);
         rpa_build(c->s, e, c->K, c->dE, c->sd, c->Cm, c->scal, c->nC, c->nO, c->v, c->nS, tda);
     }
     {
         Stage st(c, 1);
-        syevd(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->info);
+        bool use_refine = c->eig_mode == 1 && vectors && c->have_vec && c->tda_vec == tda;
+        if (use_refine) {
+            double d = 0.0;
+            for (int p = 0; p < c->nOrb; ++p) d = std::max(d, std::fabs(eh[p] - c->e_vec[p]));
+            use_refine = d <= c->eig_demax;
+        }
+        bool done = false;
+        if (use_refine) {
+            done = refine_eig(c);
+            ++c->eig_count[done ? 1 : 3];
+        }
+        if (!done) {
+            syevd(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->info);
+            ++c->eig_count[0];
+            if (c->eig_mode == 1 && vectors)
+                QK_HIP(hipMemcpyAsync(c->Xv, c->Cm, sizeof(double) * c->nS * c->nS, hipMemcpyDeviceToDevice, c->s));
+        }
+        if (c->eig_mode == 1 && vectors) {
+            c->have_vec = true;
+            c->tda_vec = tda;
+            c->e_vec.assign(eh, eh + c->nOrb);
+        }
     }
     {
         Stage st(c, 0);
@@ -257,18 +590,6 @@ void qp_graph(Ctx* c, const double* e0, const double* eG, const double* start, d
     qp_finish(c->s, c->w, c->df, c->nit, start, c->eGW, c->Z, n, c->nC, c->nR, QP_MAXIT);
 }
 
-void h2d(Ctx* c, double* dst, const double* src, size_t count) {
-    QK_HIP(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyHostToDevice, c->s));
-}
-void d2h(Ctx* c, double* dst, const double* src, size_t count) {
-    if (dst) QK_HIP(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, c->s));
-}
-double d2h_scalar(Ctx* c, const double* src) {
-    double v = 0.0;
-    QK_HIP(hipMemcpyAsync(&v, src, sizeof(double), hipMemcpyDeviceToHost, c->s));
-    QK_HIP(hipStreamSynchronize(c->s));
-    return v;
-}
 
 // (pq|jb) from AO integrals on the device: four quarter transforms (occupied
 // index first: 2N^4 o + 6 N^3 o v flops instead of 8 N^5 for the full ERI_MO).
@@ -327,6 +648,7 @@ void free_split(Split3& sp) {
 void after_eri(Ctx* c) {
     c->msplit_valid = false;
     c->have_screen = false;
+    c->have_vec = false;
     extract_k(c->s, c->M, c->K, c->nOrb, c->nC, c->nO, c->v, c->nS);
     QK_HIP(hipStreamSynchronize(c->s));
     c->have_eri = true;
@@ -407,6 +729,11 @@ void release(Ctx* c) {
     int* is[] = {c->nit, c->done, c->info, c->active};
     for (int* p : is)
         if (p) (void)hipFree(p);
+    for (double* p : {c->Xv, c->Ym, c->Sm, c->Gm, c->lam, c->eig_dbuf})
+        if (p) (void)hipFree(p);
+    for (int* p : {c->perm, c->eig_ibuf})
+        if (p) (void)hipFree(p);
+    if (c->eig_lbuf) (void)hipFree(c->eig_lbuf);
     free_split(c->Msplit);
     free_split(c->Wsplit);
     for (auto& e : c->ev)
@@ -530,7 +857,7 @@ int qk_gw_screen(qk_gw_handle c, const double* e, int TDA_W) {
         QK_CHECK(c->have_eri, QK_E_ARG, "no integrals: call qk_gw_set_eri_* first");
         for (double& t : c->t_stage) t = 0.0;
         h2d(c, c->e, e, c->nOrb);
-        screening(c, c->e, TDA_W, true);
+        screening(c, c->e, e, TDA_W, true);
         excitation(c);
         QK_HIP(hipStreamSynchronize(c->s));
         c->have_screen = true;
@@ -566,6 +893,33 @@ int qk_gw_set_precision(qk_gw_handle c, int mode) {
     });
 }
 
+int qk_gw_set_eigensolver(qk_gw_handle c, int mode, double de_max) {
+    if (!c || mode < 0 || mode > 1 || !(de_max >= 0.0)) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        if (mode == 1 && !c->Xv) {
+            size_t S = c->nS;
+            c->Xv = dalloc<double>(S * S);
+            c->Ym = dalloc<double>(S * S);
+            c->Sm = dalloc<double>(S * S);
+            c->Gm = dalloc<double>(S * S);
+            c->lam = dalloc<double>(S);
+            c->eig_dbuf = dalloc<double>((3 * EIG_KMAX + 1) * S);
+            c->eig_ibuf = dalloc<int>(6 * S + 1);
+            c->eig_lbuf = dalloc<long long>(S);
+            c->perm = dalloc<int>(S);
+        }
+        c->eig_mode = mode;
+        c->eig_demax = de_max;
+        c->have_vec = false;
+    });
+}
+
+int qk_gw_eig_stats(qk_gw_handle c, long long* counts, int n) {
+    if (!c || !counts || n < 0 || n > 4) return QK_E_ARG;
+    for (int i = 0; i < n; ++i) counts[i] = c->eig_count[i];
+    return QK_OK;
+}
+
 int qk_gw_stage_times(qk_gw_handle c, double* ms, int n) {
     if (!c || !ms || n < 0 || n > 6) return QK_E_ARG;
     double all[6] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4]};
@@ -583,7 +937,7 @@ int qk_rg0w0(qk_gw_handle c, const double* eHF, double eta, int linearize, int T
         int n = c->nOrb;
         h2d(c, c->e0, eHF, n);
         // screening at eHF (GW/RG0W0.f90:116-119) and spectral weights (:127)
-        screening(c, c->e0, TDA_W, true);
+        screening(c, c->e0, eHF, TDA_W, true);
         QK_HIP(hipMemcpyAsync(c->Om0, c->Om, sizeof(double) * c->nS, hipMemcpyDeviceToDevice, c->s));
         excitation(c);
         c->have_screen = true;
@@ -598,7 +952,10 @@ int qk_rg0w0(qk_gw_handle c, const double* eHF, double eta, int linearize, int T
             qp_graph(c, c->e0, c->e0, c->eLin, eta, doSRG, SRG_FLOW);
         }
         // RPA correlation energy at eGW (:222-225): eigenvalues only
-        screening(c, c->eGW, TDA_W, false);
+        std::vector<double> hgw(n);
+        d2h(c, hgw.data(), c->eGW, n);
+        QK_HIP(hipStreamSynchronize(c->s));
+        screening(c, c->eGW, hgw.data(), TDA_W, false);
         c->have_screen = false;  // the eigenvectors are gone
         double ecrpa = d2h_scalar(c, c->scal + 1);
         d2h(c, eGW, c->eGW, n);
@@ -631,7 +988,7 @@ int qk_evrgw(qk_gw_handle c, const double* eHF, const double* eGW_start, int max
         // GW/evRGW.f90:119-183
         while (Conv > thresh && nSCF <= maxSCF) {
             h2d(c, c->e, hGW.data(), n);  // eGW == eOld here
-            screening(c, c->e, TDA_W, true);
+            screening(c, c->e, hGW.data(), TDA_W, true);
             ecrpa = d2h_scalar(c, c->scal + 1);
             excitation(c);
             c->have_screen = true;

@@ -346,6 +346,166 @@ __global__ void k_phrlr_ab(const double* __restrict__ G, const double* __restric
     }
 }
 
+// ---- eigenvector refinement ----------------------------------------------
+// For symmetric C and near-eigenvectors X (columns): S = X^T C X, G = X^T X.
+
+// Rayleigh quotients lam_i = S_ii / G_ii
+__global__ void k_eig_lambda(const double* __restrict__ S, const double* __restrict__ G, double* __restrict__ lam,
+                             int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lam[i] = S[(long long)i * n + i] / G[(long long)i * n + i];
+}
+
+// One Newton step of the eigenvector equations (Ogita & Aishima, Japan J.
+// Indust. Appl. Math. 35 (2018) 1007-1035), with R = I - G and clusters of
+// (nearly) equal eigenvalues (lab[i] = cluster label; inside a cluster the
+// vectors have been rotated so that the cluster block of S is diagonal):
+//   E_ii = R_ii / 2;  E_ij = R_ij / 2 (same cluster);
+//   E_ij = (S_ij + lam_j R_ij) / (lam_j - lam_i) otherwise.
+// S is overwritten with F = I + E (so that X <- X F).
+__global__ void k_eig_update(double* __restrict__ S, const double* __restrict__ G, const double* __restrict__ lam,
+                             const int* __restrict__ lab, int n) {
+    long long total = (long long)n * n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(t % n), j = (int)(t / n);
+        double g = G[t], f;
+        if (i == j) {
+            f = 1.0 + 0.5 * (1.0 - g);
+        } else if (lab[i] == lab[j]) {
+            f = -0.5 * g;
+        } else {
+            double lj = lam[j];
+            f = (S[t] - lj * g) / (lj - lam[i]);
+            if (!(fabs(f) <= 1e300)) f = 0.0;  // excluded by the check pass before any update
+        }
+        S[t] = f;
+    }
+}
+
+// Check pass, one block per column j: colmax[j] = max |E_ij| over i outside
+// j's cluster (non-finite -> 1e300), colarg[j] = that i.
+__global__ __launch_bounds__(RB) void k_eig_check(const double* __restrict__ S, const double* __restrict__ G,
+                                                  const double* __restrict__ lam, const int* __restrict__ lab, int n,
+                                                  double* __restrict__ colmax, int* __restrict__ colarg) {
+    __shared__ double shv[RB];
+    __shared__ int shi[RB];
+    const int j = blockIdx.x;
+    const double lj = lam[j];
+    const int labj = lab[j];
+    const double* Sj = S + (long long)j * n;
+    const double* Gj = G + (long long)j * n;
+    double best = 0.0;
+    int arg = j;
+    for (int i = threadIdx.x; i < n; i += RB) {
+        if (i == j || lab[i] == labj) continue;
+        double a = fabs((Sj[i] - lj * Gj[i]) / (lj - lam[i]));
+        if (!(a <= 1e300)) a = 1e300;
+        if (a > best) {
+            best = a;
+            arg = i;
+        }
+    }
+    shv[threadIdx.x] = best;
+    shi[threadIdx.x] = arg;
+    __syncthreads();
+    for (int off = RB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off && shv[threadIdx.x + off] > shv[threadIdx.x]) {
+            shv[threadIdx.x] = shv[threadIdx.x + off];
+            shi[threadIdx.x] = shi[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        colmax[j] = shv[0];
+        colarg[j] = shi[0];
+    }
+}
+
+// Diagonal blocks of clusters: for cluster c (members idx[ptr[c] .. ptr[c+1]),
+// size k), outA[boff[c] + r + k q] = A[idx[r] + lda idx[q]], same for B.
+__global__ void k_gather_blocks(const double* __restrict__ A, const double* __restrict__ B, long long lda,
+                                const int* __restrict__ ptr, const int* __restrict__ idx,
+                                const long long* __restrict__ boff, double* __restrict__ outA,
+                                double* __restrict__ outB) {
+    const int c = blockIdx.x;
+    const int p0 = ptr[c], k = ptr[c + 1] - p0;
+    for (int t = threadIdx.x; t < k * k; t += blockDim.x) {
+        int r = t % k, q = t / k;
+        long long src = idx[p0 + r] + lda * idx[p0 + q];
+        outA[boff[c] + t] = A[src];
+        outB[boff[c] + t] = B[src];
+    }
+}
+
+// Column mixing inside clusters: member slot s (cluster c = scl[s], local
+// column r = s - ptr[c], size k) gets
+//   tmp[i + nrows s] = sum_q A[i + lda cidx[ptr[c] + q]] Q[boff[c] + q + k r].
+__global__ void k_mix_cols(const double* __restrict__ A, long long lda, int nrows, const int* __restrict__ cidx,
+                           const int* __restrict__ ptr, const int* __restrict__ scl, const double* __restrict__ Q,
+                           const long long* __restrict__ boff, int m, double* __restrict__ tmp) {
+    long long total = (long long)nrows * m;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(t % nrows), sl = (int)(t / nrows);
+        int c = scl[sl], p0 = ptr[c], k = ptr[c + 1] - p0, r = sl - p0;
+        const double* q = Q + boff[c] + (long long)k * r;
+        double acc = 0.0;
+        for (int u = 0; u < k; ++u) acc += A[i + lda * cidx[p0 + u]] * q[u];
+        tmp[t] = acc;
+    }
+}
+// A[i + lda idx[s]] = tmp[i + nrows s]
+__global__ void k_scatter_cols(double* __restrict__ A, long long lda, int nrows, const int* __restrict__ idx, int m,
+                               const double* __restrict__ tmp) {
+    long long total = (long long)nrows * m;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(t % nrows), sl = (int)(t / nrows);
+        A[i + lda * idx[sl]] = tmp[t];
+    }
+}
+// Row mixing: tmp[s + m j] = sum_q Q[boff[c] + q + k r] A[idx[ptr[c] + q] + n j]
+__global__ void k_mix_rows(const double* __restrict__ A, int n, const int* __restrict__ idx,
+                           const int* __restrict__ ptr, const int* __restrict__ scl, const double* __restrict__ Q,
+                           const long long* __restrict__ boff, int m, double* __restrict__ tmp) {
+    long long total = (long long)n * m;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int sl = (int)(t % m), j = (int)(t / m);
+        int c = scl[sl], p0 = ptr[c], k = ptr[c + 1] - p0, r = sl - p0;
+        const double* q = Q + boff[c] + (long long)k * r;
+        const double* Aj = A + (long long)n * j;
+        double acc = 0.0;
+        for (int u = 0; u < k; ++u) acc += q[u] * Aj[idx[p0 + u]];
+        tmp[t] = acc;
+    }
+}
+// A[idx[s] + n j] = tmp[s + m j]
+__global__ void k_scatter_rows(double* __restrict__ A, int n, const int* __restrict__ idx, int m,
+                               const double* __restrict__ tmp) {
+    long long total = (long long)n * m;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int sl = (int)(t % m), j = (int)(t / m);
+        A[idx[sl] + (long long)n * j] = tmp[t];
+    }
+}
+// out[i + n s] = X[i + n idx[s]]
+__global__ void k_gather_cols(const double* __restrict__ X, int n, const int* __restrict__ idx, int m,
+                              double* __restrict__ out) {
+    long long total = (long long)n * m;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(t % n), sl = (int)(t / n);
+        out[t] = X[i + (long long)n * idx[sl]];
+    }
+}
+
+// out[:, k] = X[:, perm[k]]
+__global__ void k_permute_cols(const double* __restrict__ X, const int* __restrict__ perm, double* __restrict__ out,
+                               int n) {
+    long long total = (long long)n * n;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        int r = (int)(t % n), k = (int)(t / n);
+        out[t] = X[r + (long long)n * perm[k]];
+    }
+}
+
 inline int grid_for(long long total, int bs = 256) {
     long long g = (total + bs - 1) / bs;
     return (int)std::min<long long>(g, 65536);
@@ -446,6 +606,52 @@ void phrlr_ab(hipStream_t s, const double* G, const double* e, double* out, int 
               double lam, int dRPA, int isB) {
     hipLaunchKernelGGL(k_phrlr_ab, dim3(grid_for((long long)nS * nS)), dim3(256), 0, s, G, e, out, n, nC, nO, v, nS,
                        lam, dRPA, isB);
+    QK_HIP(hipGetLastError());
+}
+
+void eig_lambda(hipStream_t s, const double* S, const double* G, double* lam, int n) {
+    hipLaunchKernelGGL(k_eig_lambda, dim3((n + 255) / 256), dim3(256), 0, s, S, G, lam, n);
+    QK_HIP(hipGetLastError());
+}
+void eig_update(hipStream_t s, double* S, const double* G, const double* lam, const int* lab, int n) {
+    hipLaunchKernelGGL(k_eig_update, dim3(grid_for((long long)n * n)), dim3(256), 0, s, S, G, lam, lab, n);
+    QK_HIP(hipGetLastError());
+}
+void eig_check(hipStream_t s, const double* S, const double* G, const double* lam, const int* lab, int n,
+               double* colmax, int* colarg) {
+    hipLaunchKernelGGL(k_eig_check, dim3(n), dim3(RB), 0, s, S, G, lam, lab, n, colmax, colarg);
+    QK_HIP(hipGetLastError());
+}
+void gather_blocks(hipStream_t s, const double* A, const double* B, long long lda, const int* ptr, const int* idx,
+                   const long long* boff, int ncl, double* outA, double* outB) {
+    if (ncl <= 0) return;
+    hipLaunchKernelGGL(k_gather_blocks, dim3(ncl), dim3(256), 0, s, A, B, lda, ptr, idx, boff, outA, outB);
+    QK_HIP(hipGetLastError());
+}
+void mix_cols(hipStream_t s, const double* A, long long lda, int nrows, const int* cidx, double* D, long long ldd,
+              const int* idx, const int* ptr, const int* scl, const double* Q, const long long* boff, int m,
+              double* tmp) {
+    if (m <= 0) return;
+    int g = grid_for((long long)nrows * m);
+    hipLaunchKernelGGL(k_mix_cols, dim3(g), dim3(256), 0, s, A, lda, nrows, cidx, ptr, scl, Q, boff, m, tmp);
+    hipLaunchKernelGGL(k_scatter_cols, dim3(g), dim3(256), 0, s, D, ldd, nrows, idx, m, tmp);
+    QK_HIP(hipGetLastError());
+}
+void mix_rows(hipStream_t s, double* A, int n, const int* idx, const int* ptr, const int* scl, const double* Q,
+              const long long* boff, int m, double* tmp) {
+    if (m <= 0) return;
+    int g = grid_for((long long)n * m);
+    hipLaunchKernelGGL(k_mix_rows, dim3(g), dim3(256), 0, s, A, n, idx, ptr, scl, Q, boff, m, tmp);
+    hipLaunchKernelGGL(k_scatter_rows, dim3(g), dim3(256), 0, s, A, n, idx, m, tmp);
+    QK_HIP(hipGetLastError());
+}
+void gather_cols(hipStream_t s, const double* X, int n, const int* idx, int m, double* out) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_gather_cols, dim3(grid_for((long long)n * m)), dim3(256), 0, s, X, n, idx, m, out);
+    QK_HIP(hipGetLastError());
+}
+void permute_cols(hipStream_t s, const double* X, const int* perm, double* out, int n) {
+    hipLaunchKernelGGL(k_permute_cols, dim3(grid_for((long long)n * n)), dim3(256), 0, s, X, perm, out, n);
     QK_HIP(hipGetLastError());
 }
 

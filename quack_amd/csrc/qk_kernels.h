@@ -34,4 +34,24 @@ void trace(hipStream_t s, const double* A, int n, double* out);
 void phrlr_ab(hipStream_t s, const double* G, const double* e, double* out, int n, int nC, int nO, int v, int nS,
               double lam, int dRPA, int isB);
 
+// Eigenvector refinement (clusters: lab[i] = cluster label of vector i;
+// members of cluster c are idx[ptr[c] .. ptr[c+1]), scl[slot] = cluster of a
+// member slot, boff[c] = offset of its k x k block in packed Q / block arrays).
+void eig_lambda(hipStream_t s, const double* S, const double* G, double* lam, int n);
+void eig_update(hipStream_t s, double* S, const double* G, const double* lam, const int* lab, int n);
+void eig_check(hipStream_t s, const double* S, const double* G, const double* lam, const int* lab, int n,
+               double* colmax, int* colarg);
+void gather_blocks(hipStream_t s, const double* A, const double* B, long long lda, const int* ptr, const int* idx,
+                   const long long* boff, int ncl, double* outA, double* outB);
+// D[:, idx[slot]] <- sum_q A[:, cidx[ptr[c]+q]] Q_c[q, r] (A read in full before
+// D is written, so D = A is allowed); tmp: nrows*m
+void mix_cols(hipStream_t s, const double* A, long long lda, int nrows, const int* cidx, double* D, long long ldd,
+              const int* idx, const int* ptr, const int* scl, const double* Q, const long long* boff, int m,
+              double* tmp);
+// A[idx[slot], :] <- sum_q Q_c[q, r] A'[idx[ptr[c]+q], :]; tmp: n*m
+void mix_rows(hipStream_t s, double* A, int n, const int* idx, const int* ptr, const int* scl, const double* Q,
+              const long long* boff, int m, double* tmp);
+void gather_cols(hipStream_t s, const double* X, int n, const int* idx, int m, double* out);
+void permute_cols(hipStream_t s, const double* X, const int* perm, double* out, int n);
+
 }  // namespace qk

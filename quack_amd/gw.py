@@ -26,7 +26,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_phrlr_a", "qk_phrlr_b", "qk_rgw_excitation_density", "qk_rgw_self_energy_diag", "qk_rgw_qp_graph",
            "qk_last_error", "qk_gw_set_precision", "qk_mlp_create", "qk_mlp_destroy", "qk_mlp_nparams",
            "qk_mlp_get_params", "qk_mlp_set_params", "qk_mlp_forward", "qk_mlp_loss_grad", "qk_mlp_train",
-           "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at"]
+           "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats"]
 
 
 class QKError(RuntimeError):
@@ -76,6 +76,8 @@ def load():
     lib.qk_gw_set_precision.argtypes = [_H, _I]
     lib.qk_gw_screen.argtypes = [_H, _P, _I]
     lib.qk_gw_sigma_at.argtypes = [_H, _P, _P, _D, _I, _P, _P]
+    lib.qk_gw_set_eigensolver.argtypes = [_H, _I, _D]
+    lib.qk_gw_eig_stats.argtypes = [_H, ctypes.POINTER(ctypes.c_longlong), _I]
     _F = ctypes.POINTER(ctypes.c_float)
     lib.qk_mlp_create.argtypes = [ctypes.POINTER(_H), _I, _I, _I, ctypes.c_ulonglong]
     lib.qk_mlp_destroy.argtypes = [_H]
@@ -151,6 +153,17 @@ class GWEngine:
     def set_precision(self, mode):
         """0: FP64 (QuAcK's arithmetic); 1: split-BF16 excitation densities ("mixed")."""
         _check(load().qk_gw_set_precision(self.h, {"fp64": 0, "mixed": 1}.get(mode, mode)), self.h)
+
+    def set_eigensolver(self, mode, de_max=5e-3):
+        """"syevd" (0): dsyevd for every RPA, QuAcK's scheme; "refine" (1):
+        Newton refinement of the previous eigenvectors when no orbital energy
+        moved by more than de_max Ha, dsyevd otherwise."""
+        _check(load().qk_gw_set_eigensolver(self.h, {"syevd": 0, "refine": 1}.get(mode, mode), de_max), self.h)
+
+    def eig_stats(self):
+        c = (ctypes.c_longlong * 4)()
+        _check(load().qk_gw_eig_stats(self.h, c, 4), self.h)
+        return {"syevd": c[0], "refined": c[1], "newton_steps": c[2], "fallbacks": c[3]}
 
     def set_eri_ao(self, eri_ao, c):
         eri_ao = np.ascontiguousarray(eri_ao, dtype=np.float64).reshape(-1)

