@@ -16,7 +16,8 @@ by < 1e-7 meV.  That is the
 MO coefficients, eHF) are produced before the timed region by the integral
 producer and an RHF whose Fock builds run through torch on the device.
 
-    python bench.py [--steps K] [--warmup W] [--method evgw|g0w0] [--cpu-baseline auto|off]
+    python bench.py [--steps K] [--warmup W] [--method evgw|g0w0] [--form fp64|mixed]
+                    [--eig syevd|refine] [--concurrent C] [--cpu-baseline auto|off]
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -149,13 +150,28 @@ def main():
     ap.add_argument("--max-diis", type=int, default=1)
     ap.add_argument("--form", choices=["fp64", "mixed"], default="mixed",
                     help="fp64: QuAcK's arithmetic; mixed: split-BF16 excitation densities")
+    ap.add_argument("--eig", choices=["syevd", "refine"], default="refine",
+                    help="syevd: dsyevd every cycle (QuAcK); refine: Newton refinement of the previous eigenvectors")
+    ap.add_argument("--concurrent", type=int, default=2,
+                    help="molecules in flight at once (one engine, stream and host thread each); a step is one "
+                         "such batch")
+    ap.add_argument("--learned", choices=["auto", "off"], default="auto",
+                    help="also run the learned warm start once and report it beside the headline")
     ap.add_argument("--no-reg", action="store_true", help="plain eta=0 evGW instead of SRG-regularised")
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
+    ap.add_argument("--workload", choices=["h50", "df600"], default="h50",
+                    help="h50: the headline N=250 lattice; df600: the precision-form measurement on the synthetic "
+                         "density-fitted system (N=600, o=60, naux=1800), FP64 against split precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the GW engine is single-GPU in this build (molecules shard across GPUs "
               "with no exchange; see the sharding notes).  Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "df600":
+        return run_df600(args)
+    K = max(1, args.concurrent)
+
+    from concurrent.futures import ThreadPoolExecutor
 
     import numpy as np
     import torch
@@ -164,67 +180,73 @@ def main():
     gw.load()
     mol, scf, eri, eri_t, c_t = build_inputs(args.nx, args.ny, args.seed)
     nBas, nOrb, nO = mol.nBas, scf["nOrb"], mol.nO
-    eng = gw.GWEngine(nBas, nOrb, nO)
+    engs = [gw.GWEngine(nBas, nOrb, nO) for _ in range(K)]
+    eng = engs[0]
     nS = eng.nS
-
     reg = not args.no_reg
 
-    def step():
-        eng.set_eri_ao_device(eri_t.data_ptr(), c_t.data_ptr())
+    def step_on(g):
+        g.set_eri_ao_device(eri_t.data_ptr(), c_t.data_ptr())
         if args.method == "evgw":
-            return eng.evgw(scf["eHF"], maxSCF=256, thresh=1e-6, max_diis=args.max_diis, doSRG=reg)
-        return eng.g0w0(scf["eHF"], doSRG=reg)
+            return g.evgw(scf["eHF"], maxSCF=256, thresh=1e-6, max_diis=args.max_diis, doSRG=reg)
+        return g.g0w0(scf["eHF"], doSRG=reg)
 
-    # FP64 form of the same molecule: the accuracy reference and the baseline rate
+    # plain FP64 form (dsyevd every cycle, one molecule at a time): the accuracy
+    # reference and the baseline rate
     eng.set_precision("fp64")
-    step()
+    eng.set_eigensolver("syevd")
+    step_on(eng)
     torch.cuda.synchronize()
     t = time.time()
-    ref = step()
+    ref = step_on(eng)
     t_fp64 = time.time() - t
-    log("fp64 form: %.3f s/molecule%s" % (t_fp64, (" nSCF=%d" % ref["nSCF"]) if "nSCF" in ref else ""))
-    eng.set_precision(args.form)
+    log("plain fp64 form: %.3f s/molecule%s" % (t_fp64, (" nSCF=%d" % ref["nSCF"]) if "nSCF" in ref else ""))
 
-    for w in range(args.warmup if args.form != "fp64" else 0):
-        t = time.time()
-        r = step()
-        log("warmup %d: %.3f s%s" % (w, time.time() - t, (" nSCF=%d" % r["nSCF"]) if "nSCF" in r else ""))
+    for g in engs:
+        g.set_precision(args.form)
+        g.set_eigensolver(args.eig, 0.05)
+    # one molecule alone: per-stage device times and the dominant kernel's launch time
+    t = time.time()
+    r1 = step_on(eng)
+    t_single = time.time() - t
+    st1 = eng.stage_times()
+    log("%s/%s form, one molecule alone: %.3f s, stages %s, eig %s" %
+        (args.form, args.eig, t_single, {k: round(v, 1) for k, v in st1.items()}, eng.eig_stats()))
+    pool = ThreadPoolExecutor(K)
+    for w in range(max(0, args.warmup - 1)):
+        list(pool.map(step_on, engs))
+        log("warmup %d done" % (w + 1))
     torch.cuda.synchronize()
-    stage = {k: 0.0 for k in ("aotomo", "rpa_build", "eigensolve", "excitation", "sigma_qp")}
-    ncycles = 0
     t0 = time.time()
     for k in range(args.steps):
-        r = step()
-        st = eng.stage_times()
-        for key in stage:
-            stage[key] += st[key]
-        ncycles += r.get("nSCF", 1)
-        log("step %d done%s" % (k, (" nSCF=%d" % r["nSCF"]) if "nSCF" in r else ""))
+        rs = list(pool.map(step_on, engs))
+        log("step %d done (%d molecules)" % (k, K))
     torch.cuda.synchronize()
     t1 = time.time()
+    pool.shutdown()
+    r = rs[0]
     ms_step = (t1 - t0) * 1e3 / args.steps
-    value = args.steps / (t1 - t0)
+    value = K * args.steps / (t1 - t0)
     mev = 1e-3 / 27.211386245988
-    dE = np.abs(r["eGW"] - ref["eGW"])
+    dE = np.max([np.abs(x["eGW"] - ref["eGW"]) for x in rs], axis=0)
     window = np.abs(scf["eHF"] - 0.5 * (scf["eHF"][nO - 1] + scf["eHF"][nO])) < 1.0
 
-    # dominant hand-written kernel: excitation densities rho^T = W^T M^T, one
-    # launch per screening cycle.  FP64 form: 2 nOrb^2 nS^2 flop on FP64 MFMA.
-    # Mixed form: the split algorithm issues six BF16 slice products, i.e.
-    # 6 x 2 nOrb^2 nS^2 flop on BF16 MFMA (plus the per-cycle split of W).
-    gemm_ms = stage["excitation"] / ncycles
+    # dominant hand-written kernel: the excitation-density GEMM rho^T = W^T M^T,
+    # one launch per screening cycle, timed alone with HIP events on the
+    # engine's stream (molecule run alone).  FP64 form: 2 nOrb^2 nS^2 flop on
+    # FP64 MFMA; mixed form: six BF16 slice products, 6 x 2 nOrb^2 nS^2 flop on
+    # BF16 MFMA.
+    gemm_ms = st1["excitation_gemm"] / max(1.0, st1["excitation_gemm_launches"])
     if args.form == "fp64":
         flops, peak, kname = 2.0 * nOrb * nOrb * nS * nS, PEAK_FP64, "dgemm_mfma (excitation densities rho = M W)"
     else:
         flops, peak, kname = 12.0 * nOrb * nOrb * nS * nS, PEAK_BF16, "k_gemm_bf16x3 (excitation densities, 6 BF16 slice products)"
     achieved = flops / (gemm_ms * 1e-3) / 1e12
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_excitation_gemm.json")
-    if os.path.exists(tf):
-        try:
-            traffic = json.load(open(tf)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic = read_traffic(args.form)
+
+    learned = None
+    if args.learned == "auto" and args.method == "evgw" and reg:
+        learned = learned_form(eng, scf, nO, ref, args)
     cpu = None
     if args.cpu_baseline == "auto":
         try:
@@ -244,18 +266,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.form == "fp64" else "bf16x3-slices/f32-acc/f64 (contractions), f64 (eigensolve, QP)",
         "data": "synthetic: seeded H50 lattice, integrals from the project's own Gaussian-integral producer",
-        "config": {"workload": "H50 5x10 lattice cc-pVDZ %s%s, %s form" % ("SRG-" if reg else "", args.method, args.form),
-                   "nBas": nBas, "nOrb": nOrb,
-                   "nO": nO, "nS": nS, "method": args.method, "thresh": 1e-6 if args.method == "evgw" else None,
-                   "max_diis": args.max_diis, "eta": 0.0, "srg_flow": 500.0 if reg else None,
-                   "cycles_per_molecule": ncycles / args.steps, "form": args.form},
-        "fp64_form": {"molecules_per_s": 1.0 / t_fp64, "cycles": ref.get("nSCF"),
-                      "speedup_of_form": value * t_fp64},
+        "config": {"workload": "H50 5x10 lattice cc-pVDZ %s%s, %s form, %s eigensolver, %d molecules in flight" %
+                               ("SRG-" if reg else "", args.method, args.form, args.eig, K),
+                   "nBas": nBas, "nOrb": nOrb, "nO": nO, "nS": nS, "method": args.method,
+                   "thresh": 1e-6 if args.method == "evgw" else None, "max_diis": args.max_diis, "eta": 0.0,
+                   "srg_flow": 500.0 if reg else None, "cycles_per_molecule": r.get("nSCF", 1), "form": args.form,
+                   "eigensolver": args.eig, "molecules_in_flight": K, "step": "%d molecules" % K},
+        "plain_fp64_form": {"molecules_per_s": 1.0 / t_fp64, "cycles": ref.get("nSCF"),
+                            "speedup_of_this_form": value * t_fp64},
+        "this_form_one_molecule_alone": {"s": t_single, "speedup_vs_plain_fp64": t_fp64 / t_single,
+                                         "stages_ms": st1, "eig_stats": eng.eig_stats()},
         "accuracy_vs_fp64": {"max_abs_dE_meV_all": float(dE.max() / mev),
                              "orbitals_over_1meV": int((dE > mev).sum()),
                              "max_abs_dE_meV_within_1Ha_of_gap": float(dE[window].max() / mev),
                              "HOMO_dE_meV": float(dE[nO - 1] / mev), "LUMO_dE_meV": float(dE[nO] / mev)},
-        "stages_ms_per_molecule": {k: v / args.steps for k, v in stage.items()},
+        "learned_form": learned,
         "roofline": {"kernel": kname, "bound": "mfma",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": gemm_ms},
@@ -264,7 +289,103 @@ def main():
                    "nSCF": r.get("nSCF")},
     }
     print(json.dumps(out), flush=True)
-    eng.close()
+    for g in engs:
+        g.close()
+
+
+def run_df600(args):
+    """Precision form on the synthetic density-fitted system (quack_amd/dfgen.py:
+    N = 600, o = 60, v = 540, nS = 32400, naux = 1800, decay 40, max (ia|jb) =
+    0.1 Ha, seed 0, factors generated on the device): SRG-G0W0 in the FP64 form
+    and in the split-precision form from the same factors.  The split form
+    forms (ia|jb) = Bov^T Bov, T = Bov W and rho = T^T B from BF16 slices; the
+    eigensolve and the QP solve are FP64 in both."""
+    import numpy as np
+    import torch
+
+    from quack_amd import dfgen, gw
+    n, nO, naux = 600, 60, 1800
+    t = time.time()
+    e, Bt, scale = dfgen.synthetic_df(n, nO, naux, seed=args.seed, device="cuda")
+    torch.cuda.synchronize()
+    log("df600 inputs: B (%d x %d x %d) on the device in %.1f s, scale %.3e" % (n, n, naux, time.time() - t, scale))
+    g = gw.GWEngine(n, n, nO)
+    nS = g.nS
+    res = {}
+    for form in ("fp64", "mixed"):
+        g.set_precision(form)
+        t = time.time()
+        g.set_eri_df_device(Bt.data_ptr(), naux)
+        t_k = time.time() - t
+        t = time.time()
+        r = g.g0w0(e, doSRG=True)
+        dt = time.time() - t
+        st = g.stage_times()
+        res[form] = (r, dt, st, t_k)
+        log("df600 %s: (ia|jb) %.2f s, G0W0 %.1f s, stages %s" % (form, t_k, dt, {k: round(v, 1) for k, v in st.items()}))
+    mev = 1e-3 / 27.211386245988
+    d = np.abs(res["mixed"][0]["eGW"] - res["fp64"][0]["eGW"])
+    dom = np.abs(res["mixed"][0]["Om"] - res["fp64"][0]["Om"])
+    flops_rho = 2.0 * nS * n * n * naux
+    forms = {}
+    for form, peak, mult in (("fp64", PEAK_FP64, 1.0), ("mixed", PEAK_BF16, 6.0)):
+        r, dt, st, t_k = res[form]
+        ms = st["excitation_gemm"] / max(1.0, st["excitation_gemm_launches"])
+        ach = mult * flops_rho / (ms * 1e-3) / 1e12
+        forms[form] = {"g0w0_s": dt, "ia_jb_s": t_k, "stages_ms": st, "rho_gemm_ms": ms,
+                       "rho_gemm_roofline": {"achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                                             "flops_per_launch": mult * flops_rho}}
+    out = {"metric": "precision form: split-precision vs FP64 contractions on the synthetic DF system",
+           "value": res["fp64"][2]["excitation_gemm"] / max(1e-9, res["mixed"][2]["excitation_gemm"]),
+           "unit": "x (rho GEMM speed-up, mixed over fp64)", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f64 vs bf16x3-slices/f32-acc/f64", "data": "synthetic: seeded DF factors (dfgen)",
+           "config": {"workload": "synthetic DF N=600 o=60 naux=1800 SRG-G0W0", "nOrb": n, "nO": nO, "nS": nS,
+                      "naux": naux, "decay": 40.0, "kmax_Ha": 0.1, "scale": scale, "seed": args.seed},
+           "forms": forms,
+           "accuracy_mixed_vs_fp64": {"max_abs_dE_meV_all": float(d.max() / mev), "orbitals_over_1meV": int((d > mev).sum()),
+                                      "max_abs_dOm_Ha": float(dom.max()),
+                                      "HOMO_dE_meV": float(d[nO - 1] / mev), "LUMO_dE_meV": float(d[nO] / mev)}}
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
+def read_traffic(form):
+    """HBM bytes per launch of the excitation GEMM from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    f = os.path.join(ROOT, "profiles", "pmc_excitation_gemm_h50_lattice.md")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("| %s |" % form):
+            try:
+                return float(line.split("|")[-2])
+            except ValueError:
+                return None
+    return None
+
+
+def learned_form(eng, scf, nO, ref, args):
+    """Learned warm start (quack_amd/learned.py) on the same molecule, one run."""
+    import numpy as np
+
+    from quack_amd import learned
+    if not os.path.exists(learned.DEFAULT_MODEL):
+        return None
+    ws = learned.WarmStart(learned.DEFAULT_MODEL)
+    sysd = {"eHF": scf["eHF"], "Sx": scf["Sx"], "nO": nO}
+    eng.set_eigensolver(args.eig, 0.05)
+    t = time.time()
+    r = ws.evgw(eng, sysd, doSRG=True, thresh=1e-6)
+    dt = time.time() - t
+    mev = 1e-3 / 27.211386245988
+    dE = np.abs(r["eGW"] - ref["eGW"])
+    window = np.abs(scf["eHF"] - 0.5 * (scf["eHF"][nO - 1] + scf["eHF"][nO])) < 1.0
+    out = {"s_per_molecule": dt, "cycles": int(r["nSCF"]) + 1, "plain_cycles": ref.get("nSCF"),
+           "max_abs_dE_meV_all": float(dE.max() / mev), "orbitals_over_1meV": int((dE > mev).sum()),
+           "max_abs_dE_meV_within_1Ha_of_gap": float(dE[window].max() / mev),
+           "HOMO_dE_meV": float(dE[nO - 1] / mev), "LUMO_dE_meV": float(dE[nO] / mev)}
+    log("learned form: %s" % out)
+    return out
 
 
 if __name__ == "__main__":

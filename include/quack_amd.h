@@ -56,6 +56,15 @@ int qk_gw_set_eri_ao_device(qk_gw_handle h, const double *ERI_AO_device, const d
  * ERI_MO is nOrb^4; only the (pq|jb) block is copied to the device. */
 int qk_gw_set_eri_mo(qk_gw_handle h, const double *ERI_MO);
 
+/* Density-fitted MO integrals (pq|rs) = sum_P B(P,p,q) B(P,r,s) (chemists'
+ * notation; ERI(p,q,r,s) = <pq|rs> = sum_P B(P,p,r) B(P,q,s)), B[P + naux*(p +
+ * nOrb*q)], symmetric in p, q; requires nBas = nOrb.  The (ia|jb) block is
+ * formed at once in the handle's current precision; the excitation densities
+ * then run as rho = B^T (Bov W) and the four-index block is never formed.
+ * The path of QuAcK's density-fitted callers; QuAcK's RGW itself takes ERI_MO. */
+int qk_gw_set_eri_df(qk_gw_handle h, const double *B, int naux);
+int qk_gw_set_eri_df_device(qk_gw_handle h, const double *B_device, int naux);
+
 /* G0W0@RHF: GW/RG0W0.f90:88-233 (screening at eHF, excitation densities,
  * diagonal self-energy, linearised or graphical QP solve, second RPA at eGW
  * for EcRPA).  Om receives the screening energies at eHF (nS); any output
@@ -104,14 +113,17 @@ int qk_gw_set_precision(qk_gw_handle h, int mode);
  * Both modes return FP64 eigenpairs of the same matrix.  No counterpart in
  * QuAcK.  Mode 1 keeps four nS x nS FP64 work arrays on the device. */
 int qk_gw_set_eigensolver(qk_gw_handle h, int mode, double de_max);
-/* counts[0..n-1], n <= 4: dsyevd calls, refined solves, Newton steps, and
- * refinements abandoned for dsyevd, since the handle was created. */
+/* counts[0..n-1], n <= 8, since the handle was created: dsyevd calls, refined
+ * solves, Newton steps, refinements abandoned for dsyevd; of those, abandoned
+ * because a cluster of close eigenvalues outgrew 32, because merging clusters
+ * did not settle, because the steps did not converge; the largest cluster met. */
 int qk_gw_eig_stats(qk_gw_handle h, long long *counts, int n);
 
 /* Device time (ms, HIP events on the handle's stream) of the stages of the
  * last qk_rg0w0 / qk_evrgw call: [0] AO->MO block (last set_eri_*),
  * [1] RPA matrix build, [2] eigensolve, [3] excitation densities,
- * [4] self-energy + QP solve, [5] total of the GW call.  n <= 6. */
+ * [4] self-energy + QP solve, [5] total of the GW call (host clock),
+ * [6] the excitation-density GEMM launches alone, [7] their number.  n <= 8. */
 int qk_gw_stage_times(qk_gw_handle h, double *ms, int n);
 
 /* ---- per-routine entry points (host in, host out) ---------------------- */

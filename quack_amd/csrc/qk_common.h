@@ -29,10 +29,13 @@ struct Error {
     } while (0)
 
 // Batched addressing: batch z = z0 + zdiv * z1; operand offset = z0*s0 + z1*s1.
+// upper != 0: only output tiles that meet the upper triangle (row <= column)
+// are computed (for a symmetric product; the rest of C is left as it was).
 struct Batch {
     long long sa0 = 0, sa1 = 0, sb0 = 0, sb1 = 0, sc0 = 0, sc1 = 0;
     int zdiv = 1;
     int count = 1;
+    int upper = 0;
 };
 
 // One problem of a grouped GEMM (device-resident descriptor).

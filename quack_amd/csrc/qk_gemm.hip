@@ -142,6 +142,10 @@ template <int WAVES_M, int WAVES_N, int BLK_M, int BLK_N, bool TA, bool TB>
 __global__ __launch_bounds__(64 * WAVES_M * WAVES_N, 2) void dgemm_mfma(
     int M, int N, int K, double alpha, const double* __restrict__ A, long long lda, const double* __restrict__ B,
     long long ldb, double beta, double* __restrict__ C, long long ldc, Batch bt, int tiles_m) {
+    if (bt.upper) {
+        constexpr int BM = WAVES_M * BLK_M * 16, BN = WAVES_N * BLK_N * 16;
+        if ((int)(blockIdx.x % tiles_m) * BM >= (int)(blockIdx.x / tiles_m) * BN + BN) return;
+    }
     const int z = blockIdx.z;
     const long long z0 = z % bt.zdiv, z1 = z / bt.zdiv;
     dgemm_tile<WAVES_M, WAVES_N, BLK_M, BLK_N, TA, TB>(blockIdx.x, M, N, K, alpha, A + z0 * bt.sa0 + z1 * bt.sa1, lda,

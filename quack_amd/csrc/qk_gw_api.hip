@@ -165,6 +165,13 @@ struct qk_gw_ctx {
     int precision = 0;        // 0: FP64 MFMA, 1: BF16x3 split with FP64 recombination
     qk::Split3 Msplit, Wsplit;  // M^T and W slices (mixed precision)
     bool msplit_valid = false;
+    // density-fitted integrals (pq|rs) = sum_P B(P,pq) B(P,rs): B[P + naux (p + nOrb q)],
+    // Bov[P + naux ia] its (i, a) block, T = Bov W (naux x nS)
+    bool use_df = false;
+    int naux = 0;
+    double *Bdf = nullptr, *Bov = nullptr, *Tdf = nullptr;
+    qk::Split3 Bsplit, BovTsplit, Tsplit;  // B (rows pq), Bov^T (rows P, k = jb), T (rows m)
+    bool bsplit_valid = false;
     // eigensolver: 0 = rocSOLVER dsyevd for every RPA (QuAcK's scheme); 1 = refine
     // the previous eigenvectors by Newton steps when the energies moved by at
     // most eig_demax since they were computed, dsyevd otherwise or on failure
@@ -176,10 +183,16 @@ struct qk_gw_ctx {
     bool have_vec = false;
     int tda_vec = 0;
     std::vector<double> e_vec;
-    long long eig_count[4] = {0, 0, 0, 0};  // dsyevd calls, refined solves, Newton steps, fallbacks
+    // dsyevd calls, refined solves, Newton steps, fallbacks; fallbacks because a
+    // cluster outgrew EIG_KMAX, because merging did not settle, because the
+    // steps did not converge; largest cluster met
+    long long eig_count[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double t_ao = 0.0;
-    double t_stage[5] = {0, 0, 0, 0, 0};
+    // [0..4] RPA build, eigensolve, excitation densities, self-energy + QP, total;
+    // [5] excitation-density GEMM launches alone, [6] their number
+    double t_stage[7] = {0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev2[2] = {nullptr, nullptr};
     std::string err;
 };
 
@@ -192,12 +205,15 @@ using Ctx = qk_gw_ctx;
 struct Stage {
     Ctx* c;
     int k;
-    Stage(Ctx* c_, int k_) : c(c_), k(k_) { QK_HIP(hipEventRecord(c->ev[0], c->s)); }
+    hipEvent_t* ev;
+    Stage(Ctx* c_, int k_, bool inner = false) : c(c_), k(k_), ev(inner ? c_->ev2 : c_->ev) {
+        QK_HIP(hipEventRecord(ev[0], c->s));
+    }
     ~Stage() {
         // never throws: a failed timing read leaves the stage time unchanged
         float ms = 0.f;
-        if (hipEventRecord(c->ev[1], c->s) == hipSuccess && hipEventSynchronize(c->ev[1]) == hipSuccess &&
-            hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess)
+        if (hipEventRecord(ev[1], c->s) == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess &&
+            hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess)
             c->t_stage[k] += ms;
     }
 };
@@ -221,6 +237,15 @@ double d2h_scalar(Ctx* c, const double* src) {
     QK_HIP(hipMemcpyAsync(&v, src, sizeof(double), hipMemcpyDeviceToHost, c->s));
     QK_HIP(hipStreamSynchronize(c->s));
     return v;
+}
+
+void alloc_split(Split3& sp, int rows, int K) {
+    sp.rows = rows;
+    sp.kpad = pad32(K);
+    for (auto& p : sp.s) p = dalloc<__bf16>((size_t)rows * sp.kpad);
+}
+void free_split(Split3& sp) {
+    for (auto& p : sp.s) dfree(p);
 }
 
 constexpr int EIG_MAXIT = 8;       // Newton steps before falling back to dsyevd
@@ -285,7 +310,7 @@ void jacobi_eig(int k, std::vector<double>& A, std::vector<double>& V, std::vect
 struct EigClusters {
     std::vector<int> lab, ptr, idx, scl, loc;
     std::vector<long long> boff;
-    int ncl = 0, m = 0;
+    int ncl = 0, m = 0, biggest = 1;
     long long nblk = 0;
     explicit EigClusters(int n) : lab(n) {
         for (int i = 0; i < n; ++i) lab[i] = i;
@@ -302,6 +327,7 @@ struct EigClusters {
         int n = (int)lab.size();
         std::vector<int> cnt(n, 0);
         for (int l : lab) ++cnt[l];
+        biggest = *std::max_element(cnt.begin(), cnt.end());
         ptr.assign(1, 0);
         idx.clear(); scl.clear(); boff.clear();
         std::vector<int> first(n, -1);
@@ -401,6 +427,7 @@ bool refine_eig(Ctx* c) {
     std::vector<double> lam(n), colmax(n), bS, bG, Wh;
     std::vector<int> colarg(n), order(n), pos(n);
     EigClusters cl(n);
+    const long long before_fail = c->eig_count[4] + c->eig_count[5];
     int* d_ptr = c->eig_ibuf;
     int* d_idx = d_ptr + (n + 1);
     int* d_scl = d_ptr + (2 * n + 1);
@@ -431,9 +458,13 @@ bool refine_eig(Ctx* c) {
 
     bool ok = false;
     for (int it = 0; it < EIG_MAXIT && !ok; ++it) {
+        Batch up;
+        up.upper = 1;  // S and G are symmetric: upper tiles, then mirror
         dgemm(s, false, false, n, n, n, 1.0, c->Cm, n, X, n, 0.0, Y, n);
-        dgemm(s, true, false, n, n, n, 1.0, X, n, Y, n, 0.0, c->Sm, n);
-        dgemm(s, true, false, n, n, n, 1.0, X, n, X, n, 0.0, c->Gm, n);
+        dgemm(s, true, false, n, n, n, 1.0, X, n, Y, n, 0.0, c->Sm, n, up);
+        dgemm(s, true, false, n, n, n, 1.0, X, n, X, n, 0.0, c->Gm, n, up);
+        sym_lower(s, c->Sm, n);
+        sym_lower(s, c->Gm, n);
         ++c->eig_count[2];
         fetch_lam();
         double lmax = 0.0;
@@ -444,10 +475,13 @@ bool refine_eig(Ctx* c) {
         bool settled = false;
         for (int rep = 0; rep < 12 && !settled; ++rep) {
             if (!cl.build()) {
+                ++c->eig_count[4];
+                c->eig_count[7] = std::max<long long>(c->eig_count[7], cl.biggest);
                 c->Xv = X;
                 c->Ym = Y;
                 return false;
             }
+            c->eig_count[7] = std::max<long long>(c->eig_count[7], cl.biggest);
             upload_clusters(c, cl);
             if (cl.ncl > 0) {
                 fetch_blocks(c->Sm, c->Gm, n, d_idx);
@@ -472,7 +506,10 @@ bool refine_eig(Ctx* c) {
                     for (int k = a; k < b; ++k) cl.merge(order[k], order[k + 1]);
                 }
         }
-        if (!settled) break;
+        if (!settled) {
+            ++c->eig_count[5];
+            break;
+        }
         double emax = 0.0;
         for (double v : colmax) emax = std::max(emax, v);
         eig_update(s, c->Sm, c->Gm, c->lam, d_lab, n);
@@ -482,7 +519,10 @@ bool refine_eig(Ctx* c) {
     }
     c->Xv = X;
     c->Ym = Y;
-    if (!ok) return false;
+    if (!ok) {
+        if (c->eig_count[4] + c->eig_count[5] == before_fail) ++c->eig_count[6];
+        return false;
+    }
     // final Rayleigh-Ritz inside the clusters with the converged vectors
     if (!cl.build()) return false;
     if (cl.ncl > 0) {
@@ -548,11 +588,24 @@ void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors)
 }
 
 // rhoT[m + nS (p + n q)] = sum_jb W[jb, m] M[pq, jb]  (rho^T = W^T M^T)
+void excitation_df(Ctx* c);
+
 void excitation(Ctx* c) {
     Stage st(c, 2);
     long long n2 = (long long)c->nOrb * c->nOrb;
     QK_CHECK(n2 < (1LL << 31), QK_E_ARG, "nOrb^2 too large");
+    c->t_stage[6] += 1.0;
+    if (c->use_df) {
+        excitation_df(c);
+        return;
+    }
+    if (c->precision == 1 && !c->Msplit.s[0]) {
+        alloc_split(c->Wsplit, c->nS, c->nS);
+        alloc_split(c->Msplit, (int)n2, c->nS);
+        c->msplit_valid = false;
+    }
     if (c->precision == 0) {
+        Stage g(c, 5, true);
         dgemm(c->s, true, true, c->nS, (int)n2, c->nS, 1.0, c->Cm, c->nS, c->M, n2, 0.0, c->rhoT, c->nS);
         return;
     }
@@ -563,7 +616,39 @@ void excitation(Ctx* c) {
         c->msplit_valid = true;
     }
     split3(c->s, c->Cm, c->nS, true, c->nS, c->nS, c->Wsplit);
+    Stage g(c, 5, true);
     gemm_bf16x3(c->s, c->nS, (int)n2, c->Wsplit, c->Msplit, c->rhoT, c->nS);
+}
+
+// Density-fitted excitation densities: T(P, m) = sum_jb Bov(P, jb) W(jb, m),
+// rho^T(m, pq) = sum_P T(P, m) B(P, pq).  The second product (2 nS nOrb^2 naux
+// flop) is the one timed as the excitation GEMM.
+void excitation_df(Ctx* c) {
+    const int nS = c->nS, naux = c->naux;
+    const long long n2 = (long long)c->nOrb * c->nOrb;
+    if (c->precision == 0) {
+        dgemm(c->s, false, false, naux, nS, nS, 1.0, c->Bov, naux, c->Cm, nS, 0.0, c->Tdf, naux);
+        Stage g(c, 5, true);
+        dgemm(c->s, true, false, nS, (int)n2, naux, 1.0, c->Tdf, naux, c->Bdf, naux, 0.0, c->rhoT, nS);
+        return;
+    }
+    if (!c->Wsplit.s[0]) alloc_split(c->Wsplit, nS, nS);
+    if (!c->Tsplit.s[0]) {
+        alloc_split(c->Tsplit, nS, naux);
+        alloc_split(c->BovTsplit, naux, nS);
+        alloc_split(c->Bsplit, (int)n2, naux);
+        c->bsplit_valid = false;
+    }
+    if (!c->bsplit_valid) {
+        split3(c->s, c->Bov, naux, false, naux, nS, c->BovTsplit);
+        split3(c->s, c->Bdf, naux, true, (int)n2, naux, c->Bsplit);
+        c->bsplit_valid = true;
+    }
+    split3(c->s, c->Cm, nS, true, nS, nS, c->Wsplit);
+    gemm_bf16x3(c->s, naux, nS, c->BovTsplit, c->Wsplit, c->Tdf, naux);
+    split3(c->s, c->Tdf, naux, true, nS, naux, c->Tsplit);
+    Stage g(c, 5, true);
+    gemm_bf16x3(c->s, nS, (int)n2, c->Tsplit, c->Bsplit, c->rhoT, nS);
 }
 
 void sigma_static(Ctx* c, const double* e, double eta, int srg) {
@@ -636,13 +721,11 @@ void aotomo_block(hipStream_t s, const double* f, const double* cc, double* M, i
     }
 }
 
-void alloc_split(Split3& sp, int rows, int K) {
-    sp.rows = rows;
-    sp.kpad = pad32(K);
-    for (auto& p : sp.s) p = dalloc<__bf16>((size_t)rows * sp.kpad);
-}
-void free_split(Split3& sp) {
-    for (auto& p : sp.s) dfree(p);
+
+// The four-index block M is allocated on first use (a density-fitted handle never needs it).
+void ensure_m(Ctx* c) {
+    if (!c->M) c->M = dalloc<double>((size_t)c->nOrb * c->nOrb * c->nS);
+    c->use_df = false;
 }
 
 void after_eri(Ctx* c) {
@@ -736,7 +819,14 @@ void release(Ctx* c) {
     if (c->eig_lbuf) (void)hipFree(c->eig_lbuf);
     free_split(c->Msplit);
     free_split(c->Wsplit);
+    free_split(c->Bsplit);
+    free_split(c->BovTsplit);
+    free_split(c->Tsplit);
+    for (double* p : {c->Bdf, c->Bov, c->Tdf})
+        if (p) (void)hipFree(p);
     for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->ev2)
         if (e) (void)hipEventDestroy(e);
     if (c->rb) (void)rocblas_destroy_handle(c->rb);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -765,10 +855,8 @@ int qk_gw_create(qk_gw_handle* h, int nBas, int nOrb, int nC, int nO, int nV, in
         QK_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
         rb_check(rocblas_create_handle(&c->rb), "rocblas_create_handle");
         rb_check(rocblas_set_stream(c->rb, c->s), "rocblas_set_stream");
-        QK_HIP(hipEventCreate(&c->ev[0]));
-        QK_HIP(hipEventCreate(&c->ev[1]));
+        for (auto* e : {&c->ev[0], &c->ev[1], &c->ev2[0], &c->ev2[1]}) QK_HIP(hipEventCreate(e));
         size_t n = nOrb, S = nS;
-        c->M = dalloc<double>(n * n * S);
         c->K = dalloc<double>(S * S);
         c->Cm = dalloc<double>(S * S);
         c->rhoT = dalloc<double>(n * n * S);
@@ -802,6 +890,7 @@ int qk_gw_destroy(qk_gw_handle h) {
 int qk_gw_set_eri_ao_device(qk_gw_handle c, const double* f, const double* cc) {
     if (!c || !f || !cc) return QK_E_ARG;
     return guarded(&c->err, [&] {
+        ensure_m(c);
         QK_HIP(hipEventRecord(c->ev[0], c->s));
         aotomo_block(c->s, f, cc, c->M, c->nBas, c->nOrb, c->nC, c->nO, c->nR);
         QK_HIP(hipEventRecord(c->ev[1], c->s));
@@ -819,6 +908,7 @@ int qk_gw_set_eri_ao(qk_gw_handle c, const double* ERI_AO, const double* cc) {
         size_t N = c->nBas;
         double *df = nullptr, *dc = nullptr;
         try {
+            ensure_m(c);
             df = dalloc<double>(N * N * N * N);
             dc = dalloc<double>(N * c->nOrb);
             h2d(c, df, ERI_AO, N * N * N * N);
@@ -845,10 +935,73 @@ int qk_gw_set_eri_ao(qk_gw_handle c, const double* ERI_AO, const double* cc) {
 int qk_gw_set_eri_mo(qk_gw_handle c, const double* ERI_MO) {
     if (!c || !ERI_MO) return QK_E_ARG;
     return guarded(&c->err, [&] {
+        ensure_m(c);
         upload_mo_block(c->s, ERI_MO, c->M, c->nOrb, c->nC, c->nO, c->nR);
         c->t_ao = 0.0;
         after_eri(c);
     });
+}
+
+int qk_gw_set_eri_df_device(qk_gw_handle c, const double* B, int naux) {
+    if (!c || !B || naux <= 0) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        QK_CHECK(c->nBas == c->nOrb, QK_E_ARG, "density-fitted integrals are in the MO basis: need nBas = nOrb");
+        const long long n2 = (long long)c->nOrb * c->nOrb;
+        QK_CHECK(n2 < (1LL << 31) && (long long)naux * c->nS < (1LL << 40), QK_E_ARG, "sizes too large");
+        if (c->naux != naux) {
+            dfree(c->Bdf); dfree(c->Bov); dfree(c->Tdf);
+            c->Bdf = c->Bov = c->Tdf = nullptr;
+            free_split(c->Bsplit); free_split(c->BovTsplit); free_split(c->Tsplit);
+            c->naux = naux;
+        }
+        if (!c->Bdf) {
+            c->Bdf = dalloc<double>((size_t)naux * n2);
+            c->Bov = dalloc<double>((size_t)naux * c->nS);
+            c->Tdf = dalloc<double>((size_t)naux * c->nS);
+        }
+        QK_HIP(hipMemcpyAsync(c->Bdf, B, sizeof(double) * naux * n2, hipMemcpyDeviceToDevice, c->s));
+        QK_HIP(hipEventRecord(c->ev[0], c->s));
+        gather_bov(c->s, c->Bdf, c->Bov, naux, c->nOrb, c->nC, c->nO, c->v, c->nS);
+        // K(ia, jb) = (ia|jb) = sum_P Bov(P, ia) Bov(P, jb), in the handle's precision
+        if (c->precision == 0) {
+            Batch up;
+            up.upper = 1;
+            dgemm(c->s, true, false, c->nS, c->nS, naux, 1.0, c->Bov, naux, c->Bov, naux, 0.0, c->K, c->nS, up);
+            sym_lower(c->s, c->K, c->nS);
+        } else {
+            Split3 sp;
+            alloc_split(sp, c->nS, naux);
+            split3(c->s, c->Bov, naux, true, c->nS, naux, sp);
+            gemm_bf16x3(c->s, c->nS, c->nS, sp, sp, c->K, c->nS);
+            QK_HIP(hipStreamSynchronize(c->s));
+            free_split(sp);
+        }
+        QK_HIP(hipEventRecord(c->ev[1], c->s));
+        QK_HIP(hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        QK_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        c->t_ao = ms;
+        c->use_df = true;
+        c->bsplit_valid = false;
+        c->have_screen = false;
+        c->have_vec = false;
+        c->have_eri = true;
+    });
+}
+
+int qk_gw_set_eri_df(qk_gw_handle c, const double* B, int naux) {
+    if (!c || !B || naux <= 0) return QK_E_ARG;
+    double* d = nullptr;
+    int rc = guarded(&c->err, [&] {
+        size_t cnt = (size_t)naux * c->nOrb * c->nOrb;
+        d = dalloc<double>(cnt);
+        QK_HIP(hipMemcpyAsync(d, B, sizeof(double) * cnt, hipMemcpyHostToDevice, c->s));
+        QK_HIP(hipStreamSynchronize(c->s));
+    });
+    if (rc == QK_OK) rc = qk_gw_set_eri_df_device(c, d, naux);
+    if (c) (void)hipStreamSynchronize(c->s);
+    dfree(d);
+    return rc;
 }
 
 int qk_gw_screen(qk_gw_handle c, const double* e, int TDA_W) {
@@ -884,12 +1037,7 @@ int qk_gw_sigma_at(qk_gw_handle c, const double* eG, const double* w, double eta
 int qk_gw_set_precision(qk_gw_handle c, int mode) {
     if (!c || mode < 0 || mode > 1) return QK_E_ARG;
     return guarded(&c->err, [&] {
-        if (mode == 1 && !c->Wsplit.s[0]) {
-            alloc_split(c->Wsplit, c->nS, c->nS);
-            alloc_split(c->Msplit, c->nOrb * c->nOrb, c->nS);
-            c->msplit_valid = false;
-        }
-        c->precision = mode;
+        c->precision = mode;  // slices are allocated on first use
     });
 }
 
@@ -915,14 +1063,15 @@ int qk_gw_set_eigensolver(qk_gw_handle c, int mode, double de_max) {
 }
 
 int qk_gw_eig_stats(qk_gw_handle c, long long* counts, int n) {
-    if (!c || !counts || n < 0 || n > 4) return QK_E_ARG;
+    if (!c || !counts || n < 0 || n > 8) return QK_E_ARG;
     for (int i = 0; i < n; ++i) counts[i] = c->eig_count[i];
     return QK_OK;
 }
 
 int qk_gw_stage_times(qk_gw_handle c, double* ms, int n) {
-    if (!c || !ms || n < 0 || n > 6) return QK_E_ARG;
-    double all[6] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4]};
+    if (!c || !ms || n < 0 || n > 8) return QK_E_ARG;
+    double all[8] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4],
+                     c->t_stage[5], c->t_stage[6]};
     for (int i = 0; i < n; ++i) ms[i] = all[i];
     return QK_OK;
 }

@@ -124,37 +124,58 @@ __global__ void k_scale_w(double* __restrict__ W, const double* __restrict__ sd,
 // (SRG: sig += 2 rho^2 (1-exp(-2 s eps^2))/eps, dsig -= 2 rho^2 (1-exp(-2 s eps^2))/eps^2).
 // part[3*(p*nchunk + c) + {0,1,2}] = {sig_occ, sig_vir, dsig}.
 // p runs over the active orbitals p = nC + blockIdx.y; skip[p] != 0 exits.
+// Each thread owns SIG_MPT modes m (chunk = RB * SIG_MPT), keeps their Om in
+// registers and streams rho(p, q, m) for all q with SIG_MPT independent loads.
+// For 2 s eps^2 > 38, exp(-2 s eps^2) < 2^-54 and 1 - exp(...) rounds to 1.0,
+// so the exponential is skipped there with bit-identical results.
+constexpr int SIG_MPT = 4;
 __global__ __launch_bounds__(RB) void k_sigma_partial(const double* __restrict__ rhoT, const double* __restrict__ w,
                                                       const double* __restrict__ e, const double* __restrict__ Om,
                                                       const int* __restrict__ skip, double eta, int srg, double flow,
-                                                      int n, int nC, int nO, int nR, int nS, int chunk,
+                                                      int n, int nC, int nO, int nR, int nS,
                                                       double* __restrict__ part) {
     const int p = nC + blockIdx.y;
     const int c = blockIdx.x;
     const int nchunk = gridDim.x;
     if (skip && skip[p]) return;
     const double wp = w[p];
-    const int m0 = c * chunk, m1 = min(nS, m0 + chunk);
     const double eta2 = eta * eta;
+    const int mb = c * (RB * SIG_MPT) + threadIdx.x;
+    double om[SIG_MPT];
+#pragma unroll
+    for (int k = 0; k < SIG_MPT; ++k) {
+        int m = mb + k * RB;
+        om[k] = m < nS ? Om[m] : 0.0;
+    }
     double so = 0.0, sv = 0.0, ds = 0.0;
     for (int q = nC; q < n - nR; ++q) {
         const double sgn = q < nO ? 1.0 : -1.0;
         const double wq = wp - e[q];
         const double* rq = rhoT + (long long)nS * (p + (long long)n * q);  // rho(p, q, :)
+        double r[SIG_MPT];
+#pragma unroll
+        for (int k = 0; k < SIG_MPT; ++k) {
+            int m = mb + k * RB;
+            r[k] = m < nS ? rq[m] : 0.0;
+        }
         double s_acc = 0.0, d_acc = 0.0;
-        for (int m = m0 + threadIdx.x; m < m1; m += RB) {
-            double r = rq[m];
-            double num = 2.0 * r * r;
-            double eps = wq + sgn * Om[m];
+#pragma unroll
+        for (int k = 0; k < SIG_MPT; ++k) {
+            if (mb + k * RB >= nS) break;  // padding modes contribute nothing (eps may be 0 there)
+            double num = 2.0 * r[k] * r[k];
+            double eps = wq + sgn * om[k];
             double e2 = eps * eps;
             if (!srg) {
                 double den = 1.0 / (e2 + eta2);
                 s_acc += num * eps * den;
                 d_acc -= num * (e2 - eta2) * den * den;
             } else {
-                double reg = num * (1.0 - exp(-2.0 * flow * e2)) / eps;
+                double x = 2.0 * flow * e2;
+                double g = x > 38.0 ? 1.0 : 1.0 - exp(-x);
+                double inv = 1.0 / eps;
+                double reg = num * g * inv;
                 s_acc += reg;
-                d_acc -= reg / eps;
+                d_acc -= reg * inv;
             }
         }
         if (q < nO) so += s_acc; else sv += s_acc;
@@ -496,6 +517,36 @@ __global__ void k_gather_cols(const double* __restrict__ X, int n, const int* __
     }
 }
 
+// A[i + n j] = A[j + n i] for i > j (lower triangle from the upper), 32 x 32
+// tiles through LDS; grid (tiles, tiles), only tiles with bi >= bj work.
+__global__ void k_sym_lower(double* __restrict__ A, int n) {
+    __shared__ double t[32][33];
+    const int bi = blockIdx.x, bj = blockIdx.y;
+    if (bi < bj) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+    // source tile: rows bj*32.., columns bi*32.. (upper part)
+    for (int r = ty; r < 32; r += 8) {
+        int row = bj * 32 + tx, col = bi * 32 + r;
+        t[r][tx] = (row < n && col < n) ? A[row + (long long)n * col] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        int row = bi * 32 + tx, col = bj * 32 + r;  // destination (row, col) = source (col, row)
+        if (row < n && col < n && row > col) A[row + (long long)n * col] = t[tx][r];
+    }
+}
+
+// Bov[P + naux ia] = B[P + naux (i + n a)], i = nC + ia / v, a = nO + ia % v
+__global__ void k_gather_bov(const double* __restrict__ B, double* __restrict__ Bov, int naux, int n, int nC, int nO,
+                             int v, int nS) {
+    long long total = (long long)naux * nS;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+        long long P = t % naux, ia = t / naux;
+        long long i = nC + ia / v, a = nO + ia % v;
+        Bov[t] = B[P + naux * (i + (long long)n * a)];
+    }
+}
+
 // out[:, k] = X[:, perm[k]]
 __global__ void k_permute_cols(const double* __restrict__ X, const int* __restrict__ perm, double* __restrict__ out,
                                int n) {
@@ -537,18 +588,16 @@ void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double
     QK_HIP(hipGetLastError());
 }
 int sigma_chunks(int nS, int nact) {
-    // aim for >= ~2048 blocks in total, each with at least 4 m per thread
-    int want = std::max(1, 2048 / std::max(1, nact));
-    int maxc = std::max(1, (nS + 4 * RB - 1) / (4 * RB));
-    return std::min(want, maxc);
+    (void)nact;
+    return (nS + RB * SIG_MPT - 1) / (RB * SIG_MPT);
 }
 void sigma_partial(hipStream_t s, const double* rhoT, const double* w, const double* e, const double* Om,
                    const int* skip, double eta, int srg, double flow, int n, int nC, int nO, int nR, int nS,
                    int nchunk, double* part) {
     int nact = n - nR - nC;
-    int chunk = (nS + nchunk - 1) / nchunk;
+    QK_CHECK((long long)nchunk * RB * SIG_MPT >= nS, QK_E_ARG, "sigma_partial: too few chunks");
     hipLaunchKernelGGL(k_sigma_partial, dim3(nchunk, nact), dim3(RB), 0, s, rhoT, w, e, Om, skip, eta, srg, flow, n,
-                       nC, nO, nR, nS, chunk, part);
+                       nC, nO, nR, nS, part);
     QK_HIP(hipGetLastError());
 }
 void sigma_finalize(hipStream_t s, const double* part, int nchunk, int n, int nC, int nO, int nR, double* Sig,
@@ -648,6 +697,16 @@ void mix_rows(hipStream_t s, double* A, int n, const int* idx, const int* ptr, c
 void gather_cols(hipStream_t s, const double* X, int n, const int* idx, int m, double* out) {
     if (m <= 0) return;
     hipLaunchKernelGGL(k_gather_cols, dim3(grid_for((long long)n * m)), dim3(256), 0, s, X, n, idx, m, out);
+    QK_HIP(hipGetLastError());
+}
+void gather_bov(hipStream_t s, const double* B, double* Bov, int naux, int n, int nC, int nO, int v, int nS) {
+    hipLaunchKernelGGL(k_gather_bov, dim3(grid_for((long long)naux * nS)), dim3(256), 0, s, B, Bov, naux, n, nC, nO, v,
+                       nS);
+    QK_HIP(hipGetLastError());
+}
+void sym_lower(hipStream_t s, double* A, int n) {
+    int nt = (n + 31) / 32;
+    hipLaunchKernelGGL(k_sym_lower, dim3(nt, nt), dim3(256), 0, s, A, n);
     QK_HIP(hipGetLastError());
 }
 void permute_cols(hipStream_t s, const double* X, const int* perm, double* out, int n) {

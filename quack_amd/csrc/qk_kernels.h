@@ -52,6 +52,10 @@ void mix_cols(hipStream_t s, const double* A, long long lda, int nrows, const in
 void mix_rows(hipStream_t s, double* A, int n, const int* idx, const int* ptr, const int* scl, const double* Q,
               const long long* boff, int m, double* tmp);
 void gather_cols(hipStream_t s, const double* X, int n, const int* idx, int m, double* out);
+// (i, a) block of density-fitting factors: Bov[P + naux ia] = B[P + naux (i + n a)]
+void gather_bov(hipStream_t s, const double* B, double* Bov, int naux, int n, int nC, int nO, int v, int nS);
+// A symmetric from its upper triangle: A[i, j] = A[j, i] for i > j
+void sym_lower(hipStream_t s, double* A, int n);
 void permute_cols(hipStream_t s, const double* X, const int* perm, double* out, int n);
 
 }  // namespace qk

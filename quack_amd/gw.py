@@ -26,7 +26,8 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_phrlr_a", "qk_phrlr_b", "qk_rgw_excitation_density", "qk_rgw_self_energy_diag", "qk_rgw_qp_graph",
            "qk_last_error", "qk_gw_set_precision", "qk_mlp_create", "qk_mlp_destroy", "qk_mlp_nparams",
            "qk_mlp_get_params", "qk_mlp_set_params", "qk_mlp_forward", "qk_mlp_loss_grad", "qk_mlp_train",
-           "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats"]
+           "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats",
+           "qk_gw_set_eri_df", "qk_gw_set_eri_df_device"]
 
 
 class QKError(RuntimeError):
@@ -77,6 +78,8 @@ def load():
     lib.qk_gw_screen.argtypes = [_H, _P, _I]
     lib.qk_gw_sigma_at.argtypes = [_H, _P, _P, _D, _I, _P, _P]
     lib.qk_gw_set_eigensolver.argtypes = [_H, _I, _D]
+    lib.qk_gw_set_eri_df.argtypes = [_H, _P, _I]
+    lib.qk_gw_set_eri_df_device.argtypes = [_H, _H, _I]
     lib.qk_gw_eig_stats.argtypes = [_H, ctypes.POINTER(ctypes.c_longlong), _I]
     _F = ctypes.POINTER(ctypes.c_float)
     lib.qk_mlp_create.argtypes = [ctypes.POINTER(_H), _I, _I, _I, ctypes.c_ulonglong]
@@ -161,9 +164,23 @@ class GWEngine:
         _check(load().qk_gw_set_eigensolver(self.h, {"syevd": 0, "refine": 1}.get(mode, mode), de_max), self.h)
 
     def eig_stats(self):
-        c = (ctypes.c_longlong * 4)()
-        _check(load().qk_gw_eig_stats(self.h, c, 4), self.h)
-        return {"syevd": c[0], "refined": c[1], "newton_steps": c[2], "fallbacks": c[3]}
+        c = (ctypes.c_longlong * 8)()
+        _check(load().qk_gw_eig_stats(self.h, c, 8), self.h)
+        return {"syevd": c[0], "refined": c[1], "newton_steps": c[2], "fallbacks": c[3], "fail_cluster_size": c[4],
+                "fail_unsettled": c[5], "fail_noconv": c[6], "largest_cluster": c[7]}
+
+    def set_eri_df(self, B):
+        """Density-fitted MO integrals: B of shape (naux, nOrb, nOrb) (any float64
+        array; stored with P fastest), (pq|rs) = sum_P B[P,p,q] B[P,r,s]."""
+        B = np.asarray(B, dtype=np.float64)
+        naux = B.shape[0]
+        assert B.shape == (naux, self.nOrb, self.nOrb) and self.nBas == self.nOrb
+        flat = np.ascontiguousarray(np.transpose(B, (2, 1, 0))).reshape(-1)  # P fastest, then p, then q
+        _check(load().qk_gw_set_eri_df(self.h, flat.ctypes.data_as(_P), naux), self.h)
+
+    def set_eri_df_device(self, ptr, naux):
+        """B already on the device as B[P + naux*(p + nOrb*q)] (a data_ptr())."""
+        _check(load().qk_gw_set_eri_df_device(self.h, ptr, naux), self.h)
 
     def set_eri_ao(self, eri_ao, c):
         eri_ao = np.ascontiguousarray(eri_ao, dtype=np.float64).reshape(-1)
@@ -217,9 +234,10 @@ class GWEngine:
         return Sig, dSig
 
     def stage_times(self):
-        ms = np.zeros(6)
-        _check(load().qk_gw_stage_times(self.h, _p(ms), 6), self.h)
-        return dict(zip(["aotomo", "rpa_build", "eigensolve", "excitation", "sigma_qp", "total"], ms))
+        ms = np.zeros(8)
+        _check(load().qk_gw_stage_times(self.h, _p(ms), 8), self.h)
+        return dict(zip(["aotomo", "rpa_build", "eigensolve", "excitation", "sigma_qp", "total", "excitation_gemm",
+                         "excitation_gemm_launches"], ms))
 
 
 # ---- QuAcK-named entry points -------------------------------------------------

@@ -12,6 +12,20 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and libquack_amd.so")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_hip_runtime_first():
+    """torch ships its own HIP runtime and math libraries; in a process that
+    uses both torch device memory and libquack_amd.so, torch's must be loaded
+    first, so that the library binds to the same runtime (loading the library
+    first leaves torch without a device).  A no-op without a GPU."""
+    try:
+        import torch
+        torch.cuda.is_available()
+    except Exception:
+        pass
+    yield
+
+
 @pytest.fixture(scope="session")
 def systems():
     """Small closed-shell systems with integrals and RHF from the producer."""

@@ -228,3 +228,31 @@ def test_diis_matches_reference_algebra():
         assert rcond > 1e-13
         assert np.abs(out - np.array(hist_e).T @ w).max() < 1e-10
         assert abs(w.sum() - 1.0) < 1e-12
+
+
+def test_dfgen_recipe():
+    """Synthetic DF factors: symmetric in p, q; Gram-matrix scaling puts the
+    largest (ia|jb) at kmax; the physicist ERI built from them has the 8-fold
+    symmetry and <pq|rs> = (pr|qs)."""
+    from quack_amd import dfgen
+    e, B, scale = dfgen.synthetic_df(10, 3, 15, seed=1, kmax=0.1)
+    assert np.allclose(B, np.transpose(B, (0, 2, 1)))
+    K = np.einsum("Pia,Pjb->iajb", B[:, :3, 3:], B[:, :3, 3:]).reshape(21, 21)
+    assert abs(np.abs(K).max() - 0.1) < 1e-14 and np.all(np.linalg.eigvalsh(K) > -1e-14)
+    G = dfgen.eri_mo_from_df(B)
+    chem = np.einsum("Ppq,Prs->pqrs", B, B)
+    assert np.allclose(G, np.transpose(chem, (0, 2, 1, 3)))
+    assert np.allclose(G, np.transpose(G, (1, 0, 3, 2))) and np.allclose(G, np.transpose(G, (2, 3, 0, 1)))
+    assert np.all(np.diff(e[:3]) > 0) and np.all(np.diff(e[3:]) > 0) and e[2] < -0.3 and e[3] > 0.05
+
+
+def test_dfgen_torch_layout():
+    """The torch generator's [q, p, P] tensor is symmetric in p, q and scaled the same way."""
+    import torch
+
+    from quack_amd import dfgen
+    e, Bt, scale = dfgen.synthetic_df(12, 4, 9, seed=2, device="cpu")
+    B = Bt.numpy()
+    assert B.shape == (12, 12, 9) and np.allclose(B, np.transpose(B, (1, 0, 2)))
+    ov = B[4:, :4, :]
+    assert abs((ov * ov).sum(axis=2).max() - 0.1) < 1e-14
