@@ -159,6 +159,29 @@ int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO
 
 const char *qk_last_error(void);
 
+/* ---- batched G0W0: many molecules of one size in one handle ------------- */
+
+/* nmol molecules with the same (nBas, nOrb, nC, nO, nV, nR, nS) on one stream:
+ * each stage runs for all of them before the next (one strided-batched
+ * rocSOLVER eigensolve per RPA, one batched excitation-density GEMM in the
+ * FP64 form, one host synchronisation per QP Newton sweep for the batch).
+ * GW/RG0W0.f90:88-233 per molecule; no counterpart in QuAcK, which runs one
+ * molecule per process.  Callers with uneven sizes group molecules by size. */
+typedef struct qk_gwb_ctx *qk_gwb_handle;
+int qk_gwb_create(qk_gwb_handle *h, int nmol, int nBas, int nOrb, int nC, int nO, int nV, int nR, int nS);
+int qk_gwb_destroy(qk_gwb_handle h);
+const char *qk_gwb_last_error(qk_gwb_handle h);
+int qk_gwb_set_precision(qk_gwb_handle h, int mode); /* as qk_gw_set_precision */
+/* molecule m: ERI_AO + m*stride_eri (nBas^4), c + m*stride_c (nBas x nOrb), device memory */
+int qk_gwb_set_eri_ao_device(qk_gwb_handle h, const double *ERI_AO_device, long long stride_eri,
+                             const double *c_device, long long stride_c);
+int qk_gwb_set_eri_mo(qk_gwb_handle h, int mol, const double *ERI_MO); /* host nOrb^4, one molecule */
+/* eHF, eGW, SigC, Z: nOrb x nmol; Om: nS x nmol; EcRPA, EcGM: nmol (outputs may be NULL except eGW) */
+int qk_gwb_g0w0(qk_gwb_handle h, const double *eHF, double eta, int linearize, int TDA_W, int doSRG, double *eGW,
+                double *SigC, double *Z, double *Om, double *EcRPA, double *EcGM);
+/* [0] AO->MO, [1] eigensolves, [2] excitation densities, [3] self-energy + QP (ms), n <= 4 */
+int qk_gwb_stage_times(qk_gwb_handle h, double *ms, int n);
+
 /* ---- learned warm start: per-orbital MLP (no counterpart in QuAcK) ------ */
 
 /* nin -> width (SiLU) x depth hidden layers -> 1; FP32 weights, He-uniform

@@ -193,6 +193,7 @@ struct qk_gw_ctx {
     double t_stage[7] = {0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t ev2[2] = {nullptr, nullptr};
+    bool shared = false;  // stream, rocBLAS handle and the batch slots belong to a qk_gwb handle
     std::string err;
 };
 
@@ -805,6 +806,10 @@ int guarded(std::string* slot, F&& fn) {
 
 void release(Ctx* c) {
     if (c->s) (void)hipStreamSynchronize(c->s);
+    if (c->shared) {  // slots of the batch's contiguous arrays
+        c->M = c->Cm = c->rhoT = c->Om = c->work = nullptr;
+        c->info = c->active = nullptr;
+    }
     double* ds[] = {c->M, c->K, c->Cm, c->rhoT, c->e, c->e0, c->eG, c->dE, c->sd, c->Om, c->Om0, c->Sig, c->Z,
                     c->eLin, c->eGW, c->w, c->f, c->df, c->part, c->scal, c->work};
     for (double* p : ds)
@@ -828,8 +833,10 @@ void release(Ctx* c) {
         if (e) (void)hipEventDestroy(e);
     for (auto& e : c->ev2)
         if (e) (void)hipEventDestroy(e);
-    if (c->rb) (void)rocblas_destroy_handle(c->rb);
-    if (c->s) (void)hipStreamDestroy(c->s);
+    if (!c->shared) {
+        if (c->rb) (void)rocblas_destroy_handle(c->rb);
+        if (c->s) (void)hipStreamDestroy(c->s);
+    }
 }
 
 }  // namespace
@@ -1466,6 +1473,311 @@ int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO
         }
         for (void* p : bufs) (void)hipFree(p);
     });
+}
+
+
+}  // extern "C"
+
+// ---- batched G0W0: many molecules of one size, stage by stage -----------------
+
+struct qk_gwb_ctx {
+    int nmol = 0;
+    hipStream_t s = nullptr;
+    rocblas_handle rb = nullptr;
+    std::vector<qk_gw_ctx*> mol;
+    // contiguous per-molecule arrays (uniform strides for the batched calls)
+    double *M = nullptr, *Cm = nullptr, *rhoT = nullptr, *Om = nullptr, *E = nullptr;
+    int *info = nullptr, *active = nullptr;
+    int precision = 0;
+    bool have_eri = false;
+    double t_stage[4] = {0, 0, 0, 0};  // AO->MO, eigensolves, excitation densities, self-energy + QP
+    std::string err;
+};
+
+namespace qk {
+namespace {
+
+using BCtx = qk_gwb_ctx;
+
+void release_batch(BCtx* b) {
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    for (Ctx* c : b->mol) {
+        release(c);
+        delete c;
+    }
+    b->mol.clear();
+    for (double* p : {b->M, b->Cm, b->rhoT, b->Om, b->E})
+        if (p) (void)hipFree(p);
+    for (int* p : {b->info, b->active})
+        if (p) (void)hipFree(p);
+    if (b->rb) (void)rocblas_destroy_handle(b->rb);
+    if (b->s) (void)hipStreamDestroy(b->s);
+}
+
+void syevd_batched(BCtx* b, bool vectors) {
+    const int nS = b->mol[0]->nS;
+    const long long S2 = (long long)nS * nS;
+    rb_check(rocsolver_dsyevd_strided_batched(b->rb, vectors ? rocblas_evect_original : rocblas_evect_none,
+                                              rocblas_fill_upper, nS, b->Cm, nS, S2, b->Om, nS, b->E, nS, b->info,
+                                              b->nmol),
+             "rocsolver_dsyevd_strided_batched");
+    std::vector<int> info(b->nmol);
+    QK_HIP(hipMemcpyAsync(info.data(), b->info, sizeof(int) * b->nmol, hipMemcpyDeviceToHost, b->s));
+    QK_HIP(hipStreamSynchronize(b->s));
+    for (int m = 0; m < b->nmol; ++m)
+        QK_CHECK(info[m] == 0, QK_E_EIG, "rocsolver_dsyevd_strided_batched failed for molecule " + std::to_string(m));
+}
+
+// Event-timed region on the batch stream
+struct BStage {
+    BCtx* b;
+    int k;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BStage(BCtx* b_, int k_) : b(b_), k(k_) {
+        QK_HIP(hipEventCreate(&e0));
+        QK_HIP(hipEventCreate(&e1));
+        QK_HIP(hipEventRecord(e0, b->s));
+    }
+    ~BStage() {
+        float ms = 0.f;
+        if (hipEventRecord(e1, b->s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+            hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+            b->t_stage[k] += ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+};
+
+// screening of every molecule at its energies e (device, per-molecule ctx
+// array): RPA build, one batched eigensolve, finalisation (W scaling, EcRPA)
+void screening_batch(BCtx* b, double* Ctx::*e, int tda, bool vectors) {
+    for (Ctx* c : b->mol)
+        rpa_build(c->s, c->*e, c->K, c->dE, c->sd, c->Cm, c->scal, c->nC, c->nO, c->v, c->nS, tda);
+    {
+        BStage st(b, 1);
+        syevd_batched(b, vectors);
+    }
+    for (Ctx* c : b->mol) rpa_finalize(c->s, c->Om, c->Cm, c->sd, c->scal, c->nS, tda, vectors);
+}
+
+void excitation_batch(BCtx* b) {
+    BStage st(b, 2);
+    Ctx* c0 = b->mol[0];
+    const long long n2 = (long long)c0->nOrb * c0->nOrb, nS = c0->nS;
+    if (b->precision == 0) {
+        Batch bt;
+        bt.count = b->nmol;
+        bt.zdiv = b->nmol;
+        bt.sa0 = nS * nS;
+        bt.sb0 = n2 * nS;
+        bt.sc0 = n2 * nS;
+        dgemm(b->s, true, true, (int)nS, (int)n2, (int)nS, 1.0, b->Cm, nS, b->M, n2, 0.0, b->rhoT, nS, bt);
+        return;
+    }
+    for (Ctx* c : b->mol) {
+        if (!c->Msplit.s[0]) {
+            alloc_split(c->Wsplit, c->nS, c->nS);
+            alloc_split(c->Msplit, (int)n2, c->nS);
+            c->msplit_valid = false;
+        }
+        if (!c->msplit_valid) {
+            split3(c->s, c->M, n2, false, (int)n2, c->nS, c->Msplit);
+            c->msplit_valid = true;
+        }
+        split3(c->s, c->Cm, c->nS, true, c->nS, c->nS, c->Wsplit);
+        gemm_bf16x3(c->s, c->nS, (int)n2, c->Wsplit, c->Msplit, c->rhoT, c->nS);
+    }
+}
+
+// GW/RGW_QP_graph.f90:50-92 for every active orbital of every molecule, one
+// host synchronisation per Newton sweep for the whole batch
+void qp_graph_batch(BCtx* b, double eta, int srg) {
+    const int nm = b->nmol;
+    for (Ctx* c : b->mol) qp_init(c->s, c->eLin, c->w, c->f, c->nit, c->done, c->nOrb, c->nC, c->nR);
+    std::vector<int> act(nm, 1);
+    for (int it = 0; it < QP_MAXIT; ++it) {
+        bool any = false;
+        for (int m = 0; m < nm; ++m) {
+            if (!act[m]) continue;
+            Ctx* c = b->mol[m];
+            sigma_partial(c->s, c->rhoT, c->w, c->e0, c->Om, c->done, eta, srg, SRG_FLOW, c->nOrb, c->nC, c->nO,
+                          c->nR, c->nS, c->nchunk, c->part);
+            qp_update(c->s, c->part, c->nchunk, c->e0, c->w, c->f, c->df, c->nit, c->done, c->nOrb, QP_THRESH,
+                      QP_MAXIT, c->active);
+            any = true;
+        }
+        if (!any) break;
+        QK_HIP(hipMemcpyAsync(act.data(), b->active, sizeof(int) * nm, hipMemcpyDeviceToHost, b->s));
+        QK_HIP(hipStreamSynchronize(b->s));
+        if (std::all_of(act.begin(), act.end(), [](int a) { return a == 0; })) break;
+    }
+    for (Ctx* c : b->mol)
+        qp_finish(c->s, c->w, c->df, c->nit, c->eLin, c->eGW, c->Z, c->nOrb, c->nC, c->nR, QP_MAXIT);
+}
+
+}  // namespace
+}  // namespace qk
+
+extern "C" {
+
+int qk_gwb_create(qk_gwb_handle* h, int nmol, int nBas, int nOrb, int nC, int nO, int nV, int nR, int nS) {
+    if (!h || nmol <= 0) return QK_E_ARG;
+    *h = nullptr;
+    BCtx* b = new (std::nothrow) BCtx();
+    if (!b) return QK_E_ALLOC;
+    int rc = guarded(&b->err, [&] {
+        check_sizes(nBas, nOrb, nC, nO, nV, nR, nS);
+        b->nmol = nmol;
+        QK_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+        rb_check(rocblas_create_handle(&b->rb), "rocblas_create_handle");
+        rb_check(rocblas_set_stream(b->rb, b->s), "rocblas_set_stream");
+        const size_t n = nOrb, S = nS, nm = nmol;
+        b->M = dalloc<double>(nm * n * n * S);
+        b->Cm = dalloc<double>(nm * S * S);
+        b->rhoT = dalloc<double>(nm * n * n * S);
+        b->Om = dalloc<double>(nm * S);
+        b->E = dalloc<double>(nm * S);
+        b->info = dalloc<int>(nm);
+        b->active = dalloc<int>(nm);
+        for (int m = 0; m < nmol; ++m) {
+            Ctx* c = new Ctx();
+            b->mol.push_back(c);
+            c->shared = true;
+            c->s = b->s;
+            c->rb = b->rb;
+            c->nBas = nBas; c->nOrb = nOrb; c->nC = nC; c->nO = nO; c->nV = nV; c->nR = nR; c->nS = nS;
+            c->o = nO - nC; c->v = nOrb - nR - nO;
+            c->M = b->M + m * n * n * S;
+            c->Cm = b->Cm + m * S * S;
+            c->rhoT = b->rhoT + m * n * n * S;
+            c->Om = b->Om + m * S;
+            c->work = b->E + m * S;
+            c->info = b->info + m;
+            c->active = b->active + m;
+            c->K = dalloc<double>(S * S);
+            for (double** p : {&c->e, &c->e0, &c->eG, &c->Sig, &c->Z, &c->eLin, &c->eGW, &c->w, &c->f, &c->df})
+                *p = dalloc<double>(n);
+            for (double** p : {&c->dE, &c->sd, &c->Om0}) *p = dalloc<double>(S);
+            c->nchunk = sigma_chunks(nS, nOrb - nR - nC);
+            c->part = dalloc<double>(3 * n * c->nchunk);
+            c->scal = dalloc<double>(8);
+            c->nit = dalloc<int>(n);
+            c->done = dalloc<int>(n);
+        }
+    });
+    if (rc != QK_OK) {
+        g_last_error = b->err;
+        release_batch(b);
+        delete b;
+        return rc;
+    }
+    *h = b;
+    return QK_OK;
+}
+
+int qk_gwb_destroy(qk_gwb_handle h) {
+    if (!h) return QK_E_ARG;
+    release_batch(h);
+    delete h;
+    return QK_OK;
+}
+
+const char* qk_gwb_last_error(qk_gwb_handle h) { return h ? h->err.c_str() : g_last_error.c_str(); }
+
+int qk_gwb_set_precision(qk_gwb_handle h, int mode) {
+    if (!h || mode < 0 || mode > 1) return QK_E_ARG;
+    h->precision = mode;
+    for (Ctx* c : h->mol) c->precision = mode;
+    return QK_OK;
+}
+
+int qk_gwb_set_eri_ao_device(qk_gwb_handle h, const double* ERI_AO, long long stride_eri, const double* c,
+                             long long stride_c) {
+    if (!h || !ERI_AO || !c) return QK_E_ARG;
+    return guarded(&h->err, [&] {
+        BStage st(h, 0);
+        for (int m = 0; m < h->nmol; ++m) {
+            Ctx* x = h->mol[m];
+            aotomo_block(h->s, ERI_AO + m * stride_eri, c + m * stride_c, x->M, x->nBas, x->nOrb, x->nC, x->nO,
+                         x->nR);
+            extract_k(h->s, x->M, x->K, x->nOrb, x->nC, x->nO, x->v, x->nS);
+            x->msplit_valid = false;
+            x->have_eri = true;
+        }
+        QK_HIP(hipStreamSynchronize(h->s));
+        h->have_eri = true;
+    });
+}
+
+int qk_gwb_set_eri_mo(qk_gwb_handle h, int mol, const double* ERI_MO) {
+    if (!h || !ERI_MO || mol < 0 || mol >= h->nmol) return QK_E_ARG;
+    return guarded(&h->err, [&] {
+        Ctx* x = h->mol[mol];
+        upload_mo_block(h->s, ERI_MO, x->M, x->nOrb, x->nC, x->nO, x->nR);
+        extract_k(h->s, x->M, x->K, x->nOrb, x->nC, x->nO, x->v, x->nS);
+        QK_HIP(hipStreamSynchronize(h->s));
+        x->msplit_valid = false;
+        x->have_eri = true;
+        h->have_eri = std::all_of(h->mol.begin(), h->mol.end(), [](Ctx* c) { return c->have_eri; });
+    });
+}
+
+int qk_gwb_g0w0(qk_gwb_handle h, const double* eHF, double eta, int linearize, int TDA_W, int doSRG, double* eGW,
+                double* SigC, double* Z, double* Om, double* EcRPA, double* EcGM) {
+    if (!h || !eHF || !eGW) return QK_E_ARG;
+    return guarded(&h->err, [&] {
+        QK_CHECK(h->have_eri, QK_E_ARG, "no integrals for some molecule: call qk_gwb_set_eri_* first");
+        for (double& t : h->t_stage) t = 0.0;
+        const int n = h->mol[0]->nOrb, nS = h->mol[0]->nS, nm = h->nmol;
+        for (int m = 0; m < nm; ++m) h2d(h->mol[m], h->mol[m]->e0, eHF + (size_t)m * n, n);
+        // GW/RG0W0.f90:116-127: screening at eHF, one batched eigensolve
+        screening_batch(h, &Ctx::e0, TDA_W, true);
+        for (Ctx* c : h->mol)
+            QK_HIP(hipMemcpyAsync(c->Om0, c->Om, sizeof(double) * nS, hipMemcpyDeviceToDevice, h->s));
+        excitation_batch(h);
+        {
+            BStage st(h, 3);
+            for (Ctx* c : h->mol) {
+                sigma_partial(c->s, c->rhoT, c->e0, c->e0, c->Om, nullptr, eta, doSRG, SRG_FLOW, n, c->nC, c->nO,
+                              c->nR, nS, c->nchunk, c->part);
+                sigma_finalize(c->s, c->part, c->nchunk, n, c->nC, c->nO, c->nR, c->Sig, c->Z, c->scal);
+                lin_qp(c->s, c->e0, c->Z, c->Sig, c->eLin, n, 1);
+            }
+            if (linearize) {
+                for (Ctx* c : h->mol)
+                    QK_HIP(hipMemcpyAsync(c->eGW, c->eLin, sizeof(double) * n, hipMemcpyDeviceToDevice, h->s));
+            } else {
+                qp_graph_batch(h, eta, doSRG);
+            }
+        }
+        std::vector<std::vector<double>> sig(nm, std::vector<double>(n)), sc1(nm, std::vector<double>(8));
+        for (int m = 0; m < nm; ++m) {
+            d2h(h->mol[m], sig[m].data(), h->mol[m]->Sig, n);
+            d2h(h->mol[m], sc1[m].data(), h->mol[m]->scal, 8);  // [2] = EcGM
+        }
+        // RPA correlation energy at eGW (GW/RG0W0.f90:222-225): eigenvalues only
+        screening_batch(h, &Ctx::eGW, TDA_W, false);
+        std::vector<double> sc2((size_t)nm * 8);
+        for (int m = 0; m < nm; ++m) {
+            Ctx* c = h->mol[m];
+            d2h(c, sc2.data() + (size_t)m * 8, c->scal, 8);
+            d2h(c, eGW + (size_t)m * n, c->eGW, n);
+            if (Z) d2h(c, Z + (size_t)m * n, c->Z, n);
+            if (Om) d2h(c, Om + (size_t)m * nS, c->Om0, nS);
+        }
+        QK_HIP(hipStreamSynchronize(h->s));
+        for (int m = 0; m < nm; ++m) {
+            if (SigC) std::memcpy(SigC + (size_t)m * n, sig[m].data(), sizeof(double) * n);
+            if (EcRPA) EcRPA[m] = sc2[(size_t)m * 8 + 1];
+            if (EcGM) EcGM[m] = sc1[m][2];
+        }
+    });
+}
+
+int qk_gwb_stage_times(qk_gwb_handle h, double* ms, int n) {
+    if (!h || !ms || n < 0 || n > 4) return QK_E_ARG;
+    for (int i = 0; i < n; ++i) ms[i] = h->t_stage[i];
+    return QK_OK;
 }
 
 }  // extern "C"

@@ -27,7 +27,9 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_last_error", "qk_gw_set_precision", "qk_mlp_create", "qk_mlp_destroy", "qk_mlp_nparams",
            "qk_mlp_get_params", "qk_mlp_set_params", "qk_mlp_forward", "qk_mlp_loss_grad", "qk_mlp_train",
            "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats",
-           "qk_gw_set_eri_df", "qk_gw_set_eri_df_device"]
+           "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
+           "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
+           "qk_gwb_stage_times"]
 
 
 class QKError(RuntimeError):
@@ -79,6 +81,16 @@ def load():
     lib.qk_gw_sigma_at.argtypes = [_H, _P, _P, _D, _I, _P, _P]
     lib.qk_gw_set_eigensolver.argtypes = [_H, _I, _D]
     lib.qk_gw_set_eri_df.argtypes = [_H, _P, _I]
+    _LL = ctypes.c_longlong
+    lib.qk_gwb_create.argtypes = [ctypes.POINTER(_H)] + [_I] * 8
+    lib.qk_gwb_destroy.argtypes = [_H]
+    lib.qk_gwb_last_error.argtypes = [_H]
+    lib.qk_gwb_last_error.restype = ctypes.c_char_p
+    lib.qk_gwb_set_precision.argtypes = [_H, _I]
+    lib.qk_gwb_set_eri_ao_device.argtypes = [_H, _H, _LL, _H, _LL]
+    lib.qk_gwb_set_eri_mo.argtypes = [_H, _I, _P]
+    lib.qk_gwb_g0w0.argtypes = [_H, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
+    lib.qk_gwb_stage_times.argtypes = [_H, _P, _I]
     lib.qk_gw_set_eri_df_device.argtypes = [_H, _H, _I]
     lib.qk_gw_eig_stats.argtypes = [_H, ctypes.POINTER(ctypes.c_longlong), _I]
     _F = ctypes.POINTER(ctypes.c_float)
@@ -238,6 +250,71 @@ class GWEngine:
         _check(load().qk_gw_stage_times(self.h, _p(ms), 8), self.h)
         return dict(zip(["aotomo", "rpa_build", "eigensolve", "excitation", "sigma_qp", "total", "excitation_gemm",
                          "excitation_gemm_launches"], ms))
+
+
+class GWBatch:
+    """nmol molecules of one size (qk_gwb_*): G0W0 for all of them stage by
+    stage on one stream.  Uneven batches: group by size (quack_amd/batch.py)."""
+
+    def __init__(self, nmol, nBas, nOrb, nO, nC=0, nR=0):
+        self.nmol, self.nBas, self.nOrb, self.nO = nmol, nBas, nOrb, nO
+        nV = nOrb - nO
+        self.nS = (nO - nC) * (nV - nR)
+        h = _H()
+        rc = load().qk_gwb_create(ctypes.byref(h), nmol, nBas, nOrb, nC, nO, nV, nR, self.nS)
+        if rc != 0:
+            raise QKError(rc, load().qk_gwb_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise QKError(rc, load().qk_gwb_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().qk_gwb_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, mode):
+        self._check(load().qk_gwb_set_precision(self.h, {"fp64": 0, "mixed": 1}.get(mode, mode)))
+
+    def set_eri_ao_device(self, eri_ptr, stride_eri, c_ptr, stride_c):
+        """Device pointers: molecule m's AO ERIs at eri_ptr + 8*m*stride_eri bytes
+        (nBas^4, physicist order), its MO coefficients at c_ptr + 8*m*stride_c
+        (nBas x nOrb, column-major)."""
+        self._check(load().qk_gwb_set_eri_ao_device(self.h, eri_ptr, stride_eri, c_ptr, stride_c))
+
+    def set_eri_mo(self, m, eri_mo):
+        eri_mo = np.asfortranarray(eri_mo, dtype=np.float64)
+        self._check(load().qk_gwb_set_eri_mo(self.h, m, eri_mo.ctypes.data_as(_P)))
+
+    def g0w0(self, eHF, eta=0.0, linearize=False, TDA_W=False, doSRG=False):
+        n, nS, nm = self.nOrb, self.nS, self.nmol
+        eHF = np.asfortranarray(np.asarray(eHF, dtype=np.float64).reshape(nm, n).T)
+        eGW, Sig, Z = (np.zeros((n, nm), order="F") for _ in range(3))
+        Om = np.zeros((nS, nm), order="F")
+        Ec, Egm = np.zeros(nm), np.zeros(nm)
+        self._check(load().qk_gwb_g0w0(self.h, _p(eHF), eta, int(linearize), int(TDA_W), int(doSRG), _p(eGW),
+                                       _p(Sig), _p(Z), _p(Om), _p(Ec), _p(Egm)))
+        return {"eGW": eGW.T.copy(), "SigC": Sig.T.copy(), "Z": Z.T.copy(), "Om": Om.T.copy(), "EcRPA": Ec,
+                "EcGM": Egm}
+
+    def stage_times(self):
+        ms = np.zeros(4)
+        self._check(load().qk_gwb_stage_times(self.h, _p(ms), 4))
+        return dict(zip(["aotomo", "eigensolve", "excitation", "sigma_qp"], ms))
 
 
 # ---- QuAcK-named entry points -------------------------------------------------

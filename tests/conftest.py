@@ -12,18 +12,19 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and libquack_amd.so")
 
 
-@pytest.fixture(scope="session", autouse=True)
-def _torch_hip_runtime_first():
+def pytest_collection_finish(session):
     """torch ships its own HIP runtime and math libraries; in a process that
     uses both torch device memory and libquack_amd.so, torch's must be loaded
     first, so that the library binds to the same runtime (loading the library
-    first leaves torch without a device).  A no-op without a GPU."""
-    try:
-        import torch
-        torch.cuda.is_available()
-    except Exception:
-        pass
-    yield
+    first leaves torch without a device).  Done only when GPU tests were
+    selected: the CPU tests load the compiled reference, which must not share
+    a process with torch."""
+    if any(item.get_closest_marker("gpu") for item in session.items):
+        try:
+            import torch
+            torch.cuda.is_available()
+        except Exception:
+            pass
 
 
 @pytest.fixture(scope="session")
