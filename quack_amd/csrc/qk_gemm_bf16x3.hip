@@ -72,57 +72,79 @@ __global__ void k_split3_rc(const double* __restrict__ src, long long ld, int ro
     }
 }
 
-// Workgroup tile BM x BN = 128 x 64 (4 wavefronts as 2 x 2, each 64 x 32 =
-// 4 x 2 MFMA tiles).  Per 32-wide k-slab the workgroup stages the three
-// slices of its A rows and B rows in LDS (36 KB); the next slab is fetched into
+// Workgroup tile BM x BN = 128 x 128, 4 wavefronts as 2 x 2, each 64 x 64 =
+// 4 x 4 MFMA tiles.  Per 32-wide k-slab the workgroup stages the three slices
+// of its A rows and B rows in LDS (60 KB); the next slab is fetched into
 // registers while the current one feeds the matrix cores.
-constexpr int BM = 128, BN = 64, NTH = 256;
+//
+// Tile order.  Workgroups are dispatched round-robin over the 8 XCDs, each
+// with its own L2; workgroup p runs logical tile L = x q + min(x, r) + p / 8
+// (x = p % 8, T = 8 q + r tiles), so every XCD works through a contiguous
+// range of logical tiles.  Logical tiles run in groups of GM m-tiles by all
+// n-tiles, m fastest, so the ~32 workgroups an XCD holds at once share about
+// 8 A row-blocks and 4 B row-blocks: each slab fetched from HBM feeds several
+// workgroups from L2.
+constexpr int BM = 128, BN = 128, NTH = 256, GM = 8, NXCD = 8;
 constexpr int LROW = 40;  // bf16 per LDS row: 32 k + 8 pad (80 B) against bank conflicts
-static_assert(BM == 128 && BN == 64 && NTH == 256, "slab staging assumes a 128 x 64 tile and 256 threads");
+static_assert(BM == 128 && BN == 128 && NTH == 256, "slab staging assumes a 128 x 128 tile and 256 threads");
+inline int grid_for(long long total) { return (int)std::min<long long>((total + 255) / 256, 65536); }
+
+__device__ __forceinline__ void tile_of(int pid, int tm, int tn, int& mb, int& nb) {
+    const int T = tm * tn, q = T / NXCD, r = T % NXCD;
+    const int x = pid % NXCD, slot = pid / NXCD;
+    const int L = x * q + min(x, r) + slot;
+    const int per = GM * tn, g = L / per, first = g * GM;
+    const int gsm = min(tm - first, GM), w = L % per;
+    mb = first + w % gsm;
+    nb = w / gsm;
+}
 
 __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, const __bf16* __restrict__ A0,
                                                      const __bf16* __restrict__ A1, const __bf16* __restrict__ A2,
                                                      const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
                                                      const __bf16* __restrict__ B2, double* __restrict__ C,
-                                                     long long ldc, int tiles_m) {
+                                                     long long ldc, int tiles_m, int tiles_n) {
     __shared__ __attribute__((aligned(16))) __bf16 As[3][BM][LROW];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[3][BN][LROW];
-    const int tile = blockIdx.x;
-    const int m0 = (tile % tiles_m) * BM, n0 = (tile / tiles_m) * BN;
+    int mb, nb;
+    tile_of(blockIdx.x, tiles_m, tiles_n, mb, nb);
+    const int m0 = mb * BM, n0 = nb * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
     const int l16 = lane & 15, lk = (lane >> 4) * 8;
 
-    // chunk c of the slab: A part c < ACH -> slice c / (BM*4), row (c/4) % BM, k-chunk c % 4
-    // slab chunks: thread tid owns row tid/4 (+64) and 16-byte k-chunk tid%4 of
-    // each A slice (6 chunks) and row tid/4 of each B slice (3 chunks)
+    // slab staging: thread tid owns rows tid/4 and tid/4 + 64 and the 16-byte
+    // k-chunk tid%4 of every slice of A and of B (12 chunks)
     const int rr = tid >> 2, kc8 = (tid & 3) * 8;
-    const int ia0 = m0 + rr, ia1 = m0 + rr + 64, jb0 = n0 + rr;
+    const int ia0 = m0 + rr, ia1 = m0 + rr + 64, jb0 = n0 + rr, jb1 = n0 + rr + 64;
     const long long oa0 = (long long)(ia0 < M ? ia0 : 0) * kpad + kc8;
     const long long oa1 = (long long)(ia1 < M ? ia1 : 0) * kpad + kc8;
     const long long ob0 = (long long)(jb0 < N ? jb0 : 0) * kpad + kc8;
-    const bool va0 = ia0 < M, va1 = ia1 < M, vb0 = jb0 < N;
-    uint4 pa00, pa01, pa10, pa11, pa20, pa21, pb0, pb1, pb2;
-    const uint4 zero4 = {0u, 0u, 0u, 0u};
-#define QK_FETCH(k0)                                                                     \
-    do {                                                                                 \
-        pa00 = va0 ? *reinterpret_cast<const uint4*>(A0 + oa0 + (k0)) : zero4;           \
-        pa01 = va1 ? *reinterpret_cast<const uint4*>(A0 + oa1 + (k0)) : zero4;           \
-        pa10 = va0 ? *reinterpret_cast<const uint4*>(A1 + oa0 + (k0)) : zero4;           \
-        pa11 = va1 ? *reinterpret_cast<const uint4*>(A1 + oa1 + (k0)) : zero4;           \
-        pa20 = va0 ? *reinterpret_cast<const uint4*>(A2 + oa0 + (k0)) : zero4;           \
-        pa21 = va1 ? *reinterpret_cast<const uint4*>(A2 + oa1 + (k0)) : zero4;           \
-        pb0 = vb0 ? *reinterpret_cast<const uint4*>(B0 + ob0 + (k0)) : zero4;            \
-        pb1 = vb0 ? *reinterpret_cast<const uint4*>(B1 + ob0 + (k0)) : zero4;            \
-        pb2 = vb0 ? *reinterpret_cast<const uint4*>(B2 + ob0 + (k0)) : zero4;            \
+    const long long ob1 = (long long)(jb1 < N ? jb1 : 0) * kpad + kc8;
+    // rows past M / N read row 0 (in bounds) and are zeroed by a mask: no branches
+    const unsigned ma0 = ia0 < M ? ~0u : 0u, ma1 = ia1 < M ? ~0u : 0u;
+    const unsigned mb0 = jb0 < N ? ~0u : 0u, mb1 = jb1 < N ? ~0u : 0u;
+    // one register set holds the next slab while the current one is in LDS;
+    // the row mask is applied when a slab is written to LDS, not at the load,
+    // so the loads stay in flight during the current slab's products
+    uint4 pa00, pa01, pa10, pa11, pa20, pa21, pb00, pb01, pb10, pb11, pb20, pb21;
+#define QK_LD(dst, base, off) dst = *reinterpret_cast<const uint4*>((base) + (off))
+#define QK_FETCH(k0)                                                                                        \
+    do {                                                                                                    \
+        QK_LD(pa00, A0, oa0 + (k0)); QK_LD(pa01, A0, oa1 + (k0)); QK_LD(pa10, A1, oa0 + (k0));             \
+        QK_LD(pa11, A1, oa1 + (k0)); QK_LD(pa20, A2, oa0 + (k0)); QK_LD(pa21, A2, oa1 + (k0));             \
+        QK_LD(pb00, B0, ob0 + (k0)); QK_LD(pb01, B0, ob1 + (k0)); QK_LD(pb10, B1, ob0 + (k0));             \
+        QK_LD(pb11, B1, ob1 + (k0)); QK_LD(pb20, B2, ob0 + (k0)); QK_LD(pb21, B2, ob1 + (k0));             \
     } while (0)
+#define QK_ST(arr, row, v, msk) \
+    *reinterpret_cast<uint4*>(&arr[row][kc8]) = uint4{v.x & (msk), v.y & (msk), v.z & (msk), v.w & (msk)}
 
-    d4 acc64[4][2];
-    f4 acc[4][2];
+    d4 acc64[4][4];
+    f4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < 4; ++b) {
             acc64[a][b] = d4{0.0, 0.0, 0.0, 0.0};
             acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
         }
@@ -130,20 +152,17 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
     QK_FETCH(0);
     for (int k0 = 0; k0 < kpad; k0 += KS) {
         __syncthreads();
-        *reinterpret_cast<uint4*>(&As[0][rr][kc8]) = pa00;
-        *reinterpret_cast<uint4*>(&As[0][rr + 64][kc8]) = pa01;
-        *reinterpret_cast<uint4*>(&As[1][rr][kc8]) = pa10;
-        *reinterpret_cast<uint4*>(&As[1][rr + 64][kc8]) = pa11;
-        *reinterpret_cast<uint4*>(&As[2][rr][kc8]) = pa20;
-        *reinterpret_cast<uint4*>(&As[2][rr + 64][kc8]) = pa21;
-        *reinterpret_cast<uint4*>(&Bs[0][rr][kc8]) = pb0;
-        *reinterpret_cast<uint4*>(&Bs[1][rr][kc8]) = pb1;
-        *reinterpret_cast<uint4*>(&Bs[2][rr][kc8]) = pb2;
+        QK_ST(As[0], rr, pa00, ma0); QK_ST(As[0], rr + 64, pa01, ma1);
+        QK_ST(As[1], rr, pa10, ma0); QK_ST(As[1], rr + 64, pa11, ma1);
+        QK_ST(As[2], rr, pa20, ma0); QK_ST(As[2], rr + 64, pa21, ma1);
+        QK_ST(Bs[0], rr, pb00, mb0); QK_ST(Bs[0], rr + 64, pb01, mb1);
+        QK_ST(Bs[1], rr, pb10, mb0); QK_ST(Bs[1], rr + 64, pb11, mb1);
+        QK_ST(Bs[2], rr, pb20, mb0); QK_ST(Bs[2], rr + 64, pb21, mb1);
         __syncthreads();
         if (k0 + KS < kpad) QK_FETCH(k0 + KS);
-        bf16x8 fb[2][3];
+        bf16x8 fb[4][3];
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int sl = 0; sl < 3; ++sl)
                 fb[b][sl] = *reinterpret_cast<const bf16x8*>(&Bs[sl][wn + b * 16 + l16][lk]);
@@ -152,24 +171,27 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
             bf16x8 fa[3];
 #pragma unroll
             for (int sl = 0; sl < 3; ++sl) fa[sl] = *reinterpret_cast<const bf16x8*>(&As[sl][wm + a * 16 + l16][lk]);
+            // six slice products, smallest weight first; the four b tiles are
+            // independent, so consecutive matrix-core ops never wait on each other
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                f4 c = acc[a][b];
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][2], fa[0], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[1], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[2], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[0], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[1], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[0], c, 0, 0, 0);
-                acc[a][b] = c;
-            }
+            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][2], fa[0], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[1], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[2], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[0], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[1], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[0], acc[a][b], 0, 0, 0);
         }
         // FP64 recombination every KC k-values
         if ((k0 + KS) % KC == 0 || k0 + KS >= kpad) {
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
+                for (int b = 0; b < 4; ++b) {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc[a][b][r];
                     acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
@@ -177,14 +199,16 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
         }
     }
 
+#undef QK_ST
 #undef QK_FETCH
+#undef QK_LD
     // acc64[a][b][r] = C(row m0+wm+a*16+l16, col n0+wn+b*16+4*(lane/16)+r)
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         int i = m0 + wm + a * 16 + l16;
         if (i >= M) continue;
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 int j = n0 + wn + b * 16 + 4 * (lane >> 4) + r;
@@ -192,8 +216,6 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
             }
     }
 }
-
-inline int grid_for(long long total) { return (int)std::min<long long>((total + 255) / 256, 65536); }
 
 }  // namespace
 
@@ -216,7 +238,7 @@ void gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, 
     int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
     QK_CHECK((long long)tm * tn < (1LL << 31), QK_E_ARG, "gemm_bf16x3: grid too large");
     hipLaunchKernelGGL(k_gemm_bf16x3, dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1], A.s[2], B.s[0],
-                       B.s[1], B.s[2], C, ldc, tm);
+                       B.s[1], B.s[2], C, ldc, tm, tn);
     QK_HIP(hipGetLastError());
 }
 
