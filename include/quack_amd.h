@@ -123,7 +123,9 @@ int qk_gw_eig_stats(qk_gw_handle h, long long *counts, int n);
  * last qk_rg0w0 / qk_evrgw call: [0] AO->MO block (last set_eri_*),
  * [1] RPA matrix build, [2] eigensolve, [3] excitation densities,
  * [4] self-energy + QP solve, [5] total of the GW call (host clock),
- * [6] the excitation-density GEMM launches alone, [7] their number.  n <= 8. */
+ * [6] the excitation-density GEMM launches alone, [7] their number;
+ * [8], [9] the same two summed over every call since the handle was created.
+ * n <= 10. */
 int qk_gw_stage_times(qk_gw_handle h, double *ms, int n);
 
 /* ---- per-routine entry points (host in, host out) ---------------------- */

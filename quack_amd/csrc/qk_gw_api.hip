@@ -193,6 +193,7 @@ struct qk_gw_ctx {
     double t_stage[7] = {0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t ev2[2] = {nullptr, nullptr};
+    double t_gemm_total = 0.0, n_gemm_total = 0.0;  // excitation GEMM ms and launches since creation
     bool shared = false;  // stream, rocBLAS handle and the batch slots belong to a qk_gwb handle
     std::string err;
 };
@@ -215,7 +216,10 @@ struct Stage {
         float ms = 0.f;
         if (hipEventRecord(ev[1], c->s) == hipSuccess && hipEventSynchronize(ev[1]) == hipSuccess &&
             hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess)
+        {
             c->t_stage[k] += ms;
+            if (k == 5) c->t_gemm_total += ms;
+        }
     }
 };
 
@@ -596,6 +600,7 @@ void excitation(Ctx* c) {
     long long n2 = (long long)c->nOrb * c->nOrb;
     QK_CHECK(n2 < (1LL << 31), QK_E_ARG, "nOrb^2 too large");
     c->t_stage[6] += 1.0;
+    c->n_gemm_total += 1.0;
     if (c->use_df) {
         excitation_df(c);
         return;
@@ -1076,9 +1081,9 @@ int qk_gw_eig_stats(qk_gw_handle c, long long* counts, int n) {
 }
 
 int qk_gw_stage_times(qk_gw_handle c, double* ms, int n) {
-    if (!c || !ms || n < 0 || n > 8) return QK_E_ARG;
-    double all[8] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4],
-                     c->t_stage[5], c->t_stage[6]};
+    if (!c || !ms || n < 0 || n > 10) return QK_E_ARG;
+    double all[10] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4],
+                      c->t_stage[5], c->t_stage[6], c->t_gemm_total, c->n_gemm_total};
     for (int i = 0; i < n; ++i) ms[i] = all[i];
     return QK_OK;
 }
