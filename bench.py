@@ -3,21 +3,27 @@
 Workload (default): the seeded 2D H50 lattice (5 x 10 grid, 1.6 bohr spacing,
 +-0.15 bohr in-plane jitter, numpy default_rng(0)), cc-pVDZ: nBas = 250 basis
 functions; QuAcK's canonical orthogonalisation (threshold 1e-6) keeps
-nOrb = 249, so o = 25, v = 224, nS = 5600.  One step is one molecule: the
-AO->MO transformation of the (pq|jb) block from the AO integrals resident
-in HBM, then SRG-regularised evGW@RHF (QuAcK's `reg` option, flow 500) to
-1e-6 Ha with the graphical QP solve and DIIS off (QuAcK's stored evGW results
-use max_diis = 1).  The regularisation is what makes a 1 meV comparison
-meaningful here: with eta = 0 and no regularisation the graphical QP solution
-of this lattice is chaotic in FP64 itself (a 1e-12 Ha perturbation of eHF moves
-QP energies by up to 1.5 eV), while with SRG the same perturbation moves them
-by < 1e-7 meV.  That is the
-"AO to MO transformation" + "evGW" stages QuAcK times.  Inputs (AO ERIs,
-MO coefficients, eHF) are produced before the timed region by the integral
-producer and an RHF whose Fock builds run through torch on the device.
+nOrb = 249, so o = 25, v = 224, nS = 5600.  Every molecule is the AO->MO
+transformation of the (pq|jb) block from the AO integrals resident in HBM, then
+SRG-regularised evGW@RHF (QuAcK's `reg` option, flow 500) to 1e-6 Ha with the
+graphical QP solve and DIIS off (QuAcK's stored evGW results use max_diis = 1).
+The regularisation is what makes a 1 meV comparison meaningful: with eta = 0
+and no regularisation the graphical QP solution of this lattice is chaotic in
+FP64 itself (a 1e-12 Ha perturbation of eHF moves QP energies by up to 1.5 eV),
+while with SRG the same perturbation moves them by < 1e-7 meV.
 
-    python bench.py [--steps K] [--warmup W] [--method evgw|g0w0] [--form fp64|mixed]
-                    [--eig syevd|refine] [--concurrent C] [--cpu-baseline auto|off]
+Forms: the plain FP64 form (QuAcK's arithmetic, dsyevd every cycle, one
+molecule at a time) is measured first as the accuracy reference and the
+baseline rate; the timed form (default) runs the excitation densities in split
+precision, refines the previous cycle's eigenvectors where it can, and keeps
+two molecules in flight (one engine, stream and host thread each).  A step is
+one such batch of molecules.  Inputs (AO ERIs, MO coefficients, eHF) are
+produced before the timed region by the integral producer and an RHF whose Fock
+builds run through torch on the device.
+
+    python bench.py [--steps K] [--warmup W] [--form fp64|mixed] [--eig syevd|refine]
+                    [--concurrent C] [--method evgw|g0w0] [--cpu-baseline auto|off]
+                    [--learned auto|off] [--workload h50|df600|batch]
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
