@@ -212,13 +212,14 @@ def main():
     ref = step_on(eng)
     t_fp64 = time.time() - t
     log("plain fp64 form: %.3f s/molecule%s" % (t_fp64, (" nSCF=%d" % ref["nSCF"]) if "nSCF" in ref else ""))
-    st0 = eng.stage_times()  # the plain-form launches, left out of an fp64-form average
-    t_ref_gemm, n_ref_gemm = st0["excitation_gemm_all_calls"], st0["excitation_gemm_launches_all_calls"]
+    st0 = eng.stage_times()  # the plain-form launches, part of the fp64 kernel's population only
 
     for g in engs:
         g.set_precision(args.form)
         g.set_eigensolver(args.eig, 0.05)
-    # one molecule alone: per-stage device times and the dominant kernel's launch time
+    # first call of every GEMM shape in this form: the tile autotuner's trials
+    step_on(eng)
+    # one molecule alone: per-stage device times and the dominant kernel's solo launch time
     t = time.time()
     r1 = step_on(eng)
     t_single = time.time() - t
@@ -230,12 +231,16 @@ def main():
         list(pool.map(step_on, engs))
         log("warmup %d done" % (w + 1))
     torch.cuda.synchronize()
+    c0 = [g.stage_times() for g in engs]
+    log("timed region starts, monotonic ns %d" % time.monotonic_ns())
     t0 = time.time()
     for k in range(args.steps):
         rs = list(pool.map(step_on, engs))
         log("step %d done (%d molecules)" % (k, K))
     torch.cuda.synchronize()
     t1 = time.time()
+    log("timed region ends, monotonic ns %d" % time.monotonic_ns())
+    c1 = [g.stage_times() for g in engs]
     pool.shutdown()
     r = rs[0]
     ms_step = (t1 - t0) * 1e3 / args.steps
@@ -246,21 +251,25 @@ def main():
 
     # dominant hand-written kernel: the excitation-density GEMM rho^T = W^T M^T,
     # one launch per screening cycle, timed with HIP events on each engine's own
-    # stream; the average is over every launch of this form in the process (the
-    # molecule alone, the warm-up and timed steps with their concurrent
-    # molecules, the learned run), the same population a rocprofv3 kernel
-    # summary of this command averages over.  FP64 form: 2 nOrb^2 nS^2 flop on
-    # FP64 MFMA; mixed form: six BF16 slice products, 6 x 2 nOrb^2 nS^2 flop on
-    # BF16 MFMA.
+    # stream over the timed region (launches of concurrent molecules overlap, so
+    # each takes longer than alone; the solo figure is reported beside it).
+    # FP64 form: 2 nOrb^2 nS^2 flop on FP64 MFMA; mixed form: six BF16 slice
+    # products, 6 x 2 nOrb^2 nS^2 flop on BF16 MFMA.
+    gemm_t = sum(b["excitation_gemm_all_calls"] - a["excitation_gemm_all_calls"] for a, b in zip(c0, c1))
+    gemm_n = sum(b["excitation_gemm_launches_all_calls"] - a["excitation_gemm_launches_all_calls"] for a, b in zip(c0, c1))
+    gemm_ms = gemm_t / max(1.0, gemm_n)
+    solo_ms = st1["excitation_gemm"] / max(1.0, st1["excitation_gemm_launches"])
     learned = None
     if args.learned == "auto" and args.method == "evgw" and reg:
         learned = learned_form(eng, scf, nO, ref, args)
+    # every launch of this form's kernel in the process (tuning trials, warm-up,
+    # solo, timed and learned runs): the population of a rocprofv3 kernel
+    # summary of this command
     tot = [g.stage_times() for g in engs]
-    # (the plain-FP64 reference runs launched the FP64 kernel: they belong to the
-    # average only when this form's kernel is that one)
     sub = args.form != "fp64"
-    gemm_ms = (sum(x["excitation_gemm_all_calls"] for x in tot) - (t_ref_gemm if sub else 0.0)) / \
-        max(1.0, sum(x["excitation_gemm_launches_all_calls"] for x in tot) - (n_ref_gemm if sub else 0.0))
+    all_t = sum(x["excitation_gemm_all_calls"] for x in tot) - (st0["excitation_gemm_all_calls"] if sub else 0.0)
+    all_n = sum(x["excitation_gemm_launches_all_calls"] for x in tot) - \
+        (st0["excitation_gemm_launches_all_calls"] if sub else 0.0)
     if args.form == "fp64":
         flops, peak, kname = 2.0 * nOrb * nOrb * nS * nS, PEAK_FP64, "dgemm_mfma (excitation densities rho = M W)"
     else:
@@ -304,7 +313,11 @@ def main():
         "learned_form": learned,
         "roofline": {"kernel": kname, "bound": "mfma",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": gemm_ms},
+                     "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": gemm_ms,
+                     "launches_timed": int(gemm_n), "concurrent_molecules": K,
+                     "all_launches": {"n": int(all_n), "avg_launch_ms": all_t / max(1.0, all_n)},
+                     "solo": {"avg_launch_ms": solo_ms, "achieved": flops / (solo_ms * 1e-3) / 1e12,
+                              "frac": flops / (solo_ms * 1e-3) / 1e12 / peak}},
         "cpu_baseline": cpu,
         "result": {"HOMO": float(r["eGW"][nO - 1]), "LUMO": float(r["eGW"][nO]), "EcRPA": r["EcRPA"],
                    "nSCF": r.get("nSCF")},

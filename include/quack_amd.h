@@ -123,7 +123,8 @@ int qk_gw_eig_stats(qk_gw_handle h, long long *counts, int n);
  * last qk_rg0w0 / qk_evrgw call: [0] AO->MO block (last set_eri_*),
  * [1] RPA matrix build, [2] eigensolve, [3] excitation densities,
  * [4] self-energy + QP solve, [5] total of the GW call (host clock),
- * [6] the excitation-density GEMM launches alone, [7] their number;
+ * [6] the excitation-density GEMM launches alone (the product that writes rho),
+ * [7] their number (tuning trials of the autotuner included);
  * [8], [9] the same two summed over every call since the handle was created.
  * n <= 10. */
 int qk_gw_stage_times(qk_gw_handle h, double *ms, int n);
@@ -160,6 +161,14 @@ int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO
                     double *eGW, double *Z);
 
 const char *qk_last_error(void);
+
+/* Tile autotuner of the library's GEMMs (process-wide; on by default): the first
+ * call of each large GEMM shape times every tile configuration (FP64) or tile
+ * grouping (split precision) once on the caller's stream and keeps the fastest.
+ * All candidates give identical results.  No counterpart in QuAcK.
+ * qk_autotune_report writes one line per tuned shape (QK_E_ARG if len is too small). */
+int qk_set_autotune(int on);
+int qk_autotune_report(char *buf, int len);
 
 /* ---- batched G0W0: many molecules of one size in one handle ------------- */
 

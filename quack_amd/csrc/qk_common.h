@@ -38,27 +38,15 @@ struct Batch {
     int upper = 0;
 };
 
-// One problem of a grouped GEMM (device-resident descriptor).
-struct GemmProblem {
-    int M, N, K;
-    double alpha, beta;
-    const double* A;
-    long long lda;
-    const double* B;
-    long long ldb;
-    double* C;
-    long long ldc;
-};
-
-// Grouped C_g = op(A_g) op(B_g) with op = transpose for both operands
-// (C_g = A_g^T B_g^T), all problems in one launch.  cfg 0: 64x64 tiles, cfg 1:
-// 128x128 tiles; tile0[g] is the first tile of problem g (prefix sums of
-// ceil(M/T) * ceil(N/T)), ntiles the total.
-int dgemm_grouped_tile_rows(int cfg);
-void dgemm_grouped_tt(hipStream_t s, int cfg, const GemmProblem* dprob, const int* dtile0, int ngroup, int ntiles);
+// Process-wide GEMM tile autotuner (on by default): see qk_gemm.hip.
+void set_autotune(bool on);
+bool autotune_on();
+std::string autotune_report();
+std::string autotune_report_bf16();
 
 // C(MxN) = alpha * op(A)(MxK) * op(B)(KxN) + beta * C, column-major, FP64 MFMA.
-void dgemm(hipStream_t s, bool transA, bool transB, int M, int N, int K, double alpha, const double* A,
+// Returns the number of kernel launches it issued (more than one while tuning).
+int dgemm(hipStream_t s, bool transA, bool transB, int M, int N, int K, double alpha, const double* A,
            long long lda, const double* B, long long ldb, double beta, double* C, long long ldc,
            const Batch& b = Batch());
 

@@ -13,6 +13,11 @@
 // that lane l of an accumulator holds row l%16 of C (in 4 columns): a
 // wavefront's store then writes 16 consecutive doubles of each column, i.e.
 // 128-byte segments of the column-major output.
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "qk_common.h"
 
 namespace qk {
@@ -153,25 +158,6 @@ __global__ __launch_bounds__(64 * WAVES_M * WAVES_N, 2) void dgemm_mfma(
                                                        C + z0 * bt.sc0 + z1 * bt.sc1, ldc, tiles_m);
 }
 
-// Grouped form: problem g owns tiles [tile0[g], tile0[g+1]); every workgroup
-// finds its problem by binary search, so problems of any shape share one launch.
-template <int WAVES_M, int WAVES_N, int BLK_M, int BLK_N, bool TA, bool TB>
-__global__ __launch_bounds__(64 * WAVES_M * WAVES_N, 2) void dgemm_mfma_grouped(const GemmProblem* __restrict__ pr,
-                                                                                 const int* __restrict__ tile0,
-                                                                                 int ngroup) {
-    int t = blockIdx.x;
-    int lo = 0, hi = ngroup - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (tile0[mid] <= t) lo = mid; else hi = mid - 1;
-    }
-    const GemmProblem& q = pr[lo];
-    constexpr int BM = WAVES_M * BLK_M * 16;
-    int tiles_m = (q.M + BM - 1) / BM;
-    dgemm_tile<WAVES_M, WAVES_N, BLK_M, BLK_N, TA, TB>(t - tile0[lo], q.M, q.N, q.K, q.alpha, q.A, q.lda, q.B, q.ldb,
-                                                       q.beta, q.C, q.ldc, tiles_m);
-}
-
 template <int WM, int WN, int BMK, int BNK, bool TA, bool TB>
 void launch(hipStream_t s, int M, int N, int K, double alpha, const double* A, long long lda, const double* B,
             long long ldb, double beta, double* C, long long ldc, const Batch& bt) {
@@ -185,46 +171,118 @@ void launch(hipStream_t s, int M, int N, int K, double alpha, const double* A, l
     QK_HIP(hipGetLastError());
 }
 
+// Tile configurations: 0: 256 x 32, 1: 64 x 64, 2: 128 x 64, 3: 128 x 128.
+constexpr int NCFG = 4;
+const char* const CFG_NAME[NCFG] = {"256x32", "64x64", "128x64", "128x128"};
+
 template <bool TA, bool TB>
-void dispatch(hipStream_t s, int M, int N, int K, double alpha, const double* A, long long lda, const double* B,
-              long long ldb, double beta, double* C, long long ldc, const Batch& bt) {
-    if (N <= 32)
-        launch<4, 1, 4, 2, TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);  // 256 x 32
-    else if (N <= 64 || M <= 64)
-        launch<2, 2, 2, 2, TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);  // 64 x 64
-    else
-        launch<2, 2, 4, 4, TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);  // 128 x 128
+void launch_cfg(int cfg, hipStream_t s, int M, int N, int K, double alpha, const double* A, long long lda,
+                const double* B, long long ldb, double beta, double* C, long long ldc, const Batch& bt) {
+    switch (cfg) {
+        case 0: launch<4, 1, 4, 2, TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt); break;
+        case 1: launch<2, 2, 2, 2, TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt); break;
+        case 2: launch<2, 2, 4, 2, TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt); break;
+        default: launch<2, 2, 4, 4, TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt); break;
+    }
 }
 
-template <int WM, int WN, int BMK, int BNK, bool TA, bool TB>
-void launch_grouped(hipStream_t s, const GemmProblem* dprob, const int* dtile0, int ngroup, int ntiles) {
-    hipLaunchKernelGGL((dgemm_mfma_grouped<WM, WN, BMK, BNK, TA, TB>), dim3(ntiles), dim3(64 * WM * WN), 0, s, dprob,
-                       dtile0, ngroup);
-    QK_HIP(hipGetLastError());
+// fixed shape rule, used when tuning is off or not worth it
+int default_cfg(int M, int N) {
+    if (N <= 32) return 0;
+    if (N <= 64 || M <= 64) return 1;
+    return 3;
+}
+
+// Autotuner: the first call of a (TA, TB, M, N, K, batch, upper) shape with
+// beta = 0 and at least 1e9 flop times every configuration once with HIP
+// events on the caller's stream and keeps the fastest for the process.  All
+// configurations add the k-terms of an element in the same order, so the
+// tuning runs leave the same C and later calls give identical results.
+struct TuneKey {
+    int ta, tb, M, N, K, count, upper;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(ta, tb, M, N, K, count, upper) < std::tie(o.ta, o.tb, o.M, o.N, o.K, o.count, o.upper);
+    }
+};
+struct TuneEntry {
+    int cfg;
+    float ms[NCFG];
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry> g_tune;
+std::atomic<int> g_autotune{1};
+
+template <bool TA, bool TB>
+int dispatch(hipStream_t s, int M, int N, int K, double alpha, const double* A, long long lda, const double* B,
+              long long ldb, double beta, double* C, long long ldc, const Batch& bt) {
+    const double flop = 2.0 * M * N * (double)K * bt.count;
+    if (!g_autotune.load() || beta != 0.0 || flop < 1e9) {
+        launch_cfg<TA, TB>(default_cfg(M, N), s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
+        return 1;
+    }
+    TuneKey key{TA, TB, M, N, K, bt.count, bt.upper};
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+        launch_cfg<TA, TB>(it->second.cfg, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
+        return 1;
+    }
+    TuneEntry e{default_cfg(M, N), {0.f, 0.f, 0.f, 0.f}};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    QK_HIP(hipEventCreate(&ev[0]));
+    QK_HIP(hipEventCreate(&ev[1]));
+    float best = 1e30f;
+    for (int c = 0; c < NCFG; ++c) {
+        (void)hipEventRecord(ev[0], s);
+        launch_cfg<TA, TB>(c, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
+        (void)hipEventRecord(ev[1], s);
+        float ms = 1e30f;
+        if (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess)
+            ms = 1e30f;
+        e.ms[c] = ms;
+        if (ms < best) {
+            best = ms;
+            e.cfg = c;
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    g_tune.emplace(key, e);
+    return NCFG;
 }
 
 }  // namespace
 
-int dgemm_grouped_tile_rows(int cfg) { return cfg == 0 ? 64 : 128; }
+void set_autotune(bool on) { g_autotune.store(on ? 1 : 0); }
+bool autotune_on() { return g_autotune.load() != 0; }
 
-void dgemm_grouped_tt(hipStream_t s, int cfg, const GemmProblem* dprob, const int* dtile0, int ngroup, int ntiles) {
-    if (ntiles <= 0) return;
-    if (cfg == 0)
-        launch_grouped<2, 2, 2, 2, true, true>(s, dprob, dtile0, ngroup, ntiles);
-    else
-        launch_grouped<2, 2, 4, 4, true, true>(s, dprob, dtile0, ngroup, ntiles);
+std::string autotune_report() {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    std::string r;
+    for (const auto& kv : g_tune) {
+        const TuneKey& k = kv.first;
+        char buf[256];
+        std::snprintf(buf, sizeof buf, "dgemm %c%c M=%d N=%d K=%d batch=%d%s -> %s (ms:", k.ta ? 'T' : 'N',
+                      k.tb ? 'T' : 'N', k.M, k.N, k.K, k.count, k.upper ? " upper" : "", CFG_NAME[kv.second.cfg]);
+        r += buf;
+        for (int c = 0; c < NCFG; ++c) {
+            std::snprintf(buf, sizeof buf, " %s %.3f", CFG_NAME[c], kv.second.ms[c]);
+            r += buf;
+        }
+        r += ")\n";
+    }
+    return r;
 }
 
-void dgemm(hipStream_t s, bool transA, bool transB, int M, int N, int K, double alpha, const double* A, long long lda,
-           const double* B, long long ldb, double beta, double* C, long long ldc, const Batch& bt) {
-    if (M <= 0 || N <= 0 || bt.count <= 0) return;
+int dgemm(hipStream_t s, bool transA, bool transB, int M, int N, int K, double alpha, const double* A, long long lda,
+          const double* B, long long ldb, double beta, double* C, long long ldc, const Batch& bt) {
+    if (M <= 0 || N <= 0 || bt.count <= 0) return 0;
     if (transA) {
-        if (transB) dispatch<true, true>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
-        else dispatch<true, false>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
-    } else {
-        if (transB) dispatch<false, true>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
-        else dispatch<false, false>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
+        if (transB) return dispatch<true, true>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
+        return dispatch<true, false>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
     }
+    if (transB) return dispatch<false, true>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
+    return dispatch<false, false>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
 }
 
 }  // namespace qk

@@ -16,6 +16,11 @@
 // k = 8*(l/16) .. +7.  The f32 accumulator holds D(i, j) in lane j + 16*(i/4),
 // register i%4.  srcA is fed from B and srcB from A, so lane l of D holds
 // row l%16 of C and stores are 16 consecutive doubles per column.
+#include <array>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "qk_split.h"
 
 namespace qk {
@@ -84,17 +89,17 @@ __global__ void k_split3_rc(const double* __restrict__ src, long long ld, int ro
 // n-tiles, m fastest, so the ~32 workgroups an XCD holds at once share about
 // 8 A row-blocks and 4 B row-blocks: each slab fetched from HBM feeds several
 // workgroups from L2.
-constexpr int BM = 128, BN = 128, NTH = 256, GM = 8, NXCD = 8;
+constexpr int BM = 128, BN = 128, NTH = 256, NXCD = 8;
 constexpr int LROW = 40;  // bf16 per LDS row: 32 k + 8 pad (80 B) against bank conflicts
 static_assert(BM == 128 && BN == 128 && NTH == 256, "slab staging assumes a 128 x 128 tile and 256 threads");
 inline int grid_for(long long total) { return (int)std::min<long long>((total + 255) / 256, 65536); }
 
-__device__ __forceinline__ void tile_of(int pid, int tm, int tn, int& mb, int& nb) {
+__device__ __forceinline__ void tile_of(int pid, int tm, int tn, int gm, int& mb, int& nb) {
     const int T = tm * tn, q = T / NXCD, r = T % NXCD;
     const int x = pid % NXCD, slot = pid / NXCD;
     const int L = x * q + min(x, r) + slot;
-    const int per = GM * tn, g = L / per, first = g * GM;
-    const int gsm = min(tm - first, GM), w = L % per;
+    const int per = gm * tn, g = L / per, first = g * gm;
+    const int gsm = min(tm - first, gm), w = L % per;
     mb = first + w % gsm;
     nb = w / gsm;
 }
@@ -103,11 +108,11 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
                                                      const __bf16* __restrict__ A1, const __bf16* __restrict__ A2,
                                                      const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
                                                      const __bf16* __restrict__ B2, double* __restrict__ C,
-                                                     long long ldc, int tiles_m, int tiles_n) {
+                                                     long long ldc, int tiles_m, int tiles_n, int gm) {
     __shared__ __attribute__((aligned(16))) __bf16 As[3][BM][LROW];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[3][BN][LROW];
     int mb, nb;
-    tile_of(blockIdx.x, tiles_m, tiles_n, mb, nb);
+    tile_of(blockIdx.x, tiles_m, tiles_n, gm, mb, nb);
     const int m0 = mb * BM, n0 = nb * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
@@ -233,13 +238,78 @@ void split3(hipStream_t s, const double* src, long long ld, bool k_contiguous, i
     QK_HIP(hipGetLastError());
 }
 
-void gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc) {
+namespace {
+// Autotuned group size of the tile order, per (M, N, kpad): the first call of a
+// shape with at least 1e10 flop times each candidate once (HIP events on the
+// caller's stream); every candidate writes the same C.
+constexpr int NGM = 4;
+const int GM_CAND[NGM] = {2, 4, 8, 16};
+std::mutex g_mu;
+std::map<std::tuple<int, int, int>, std::pair<int, std::array<float, NGM>>> g_gm;
+
+void launch_gm(hipStream_t s, int gm, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc, int tm,
+               int tn) {
+    hipLaunchKernelGGL(k_gemm_bf16x3, dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1], A.s[2], B.s[0],
+                       B.s[1], B.s[2], C, ldc, tm, tn, gm);
+    QK_HIP(hipGetLastError());
+}
+}  // namespace
+
+int gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc) {
     QK_CHECK(A.kpad == B.kpad && A.rows == M && B.rows == N, QK_E_ARG, "gemm_bf16x3: operand shapes");
     int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
     QK_CHECK((long long)tm * tn < (1LL << 31), QK_E_ARG, "gemm_bf16x3: grid too large");
-    hipLaunchKernelGGL(k_gemm_bf16x3, dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1], A.s[2], B.s[0],
-                       B.s[1], B.s[2], C, ldc, tm, tn);
-    QK_HIP(hipGetLastError());
+    const double flop = 12.0 * M * (double)N * A.kpad;
+    if (!autotune_on() || flop < 1e10) {
+        launch_gm(s, 8, M, N, A, B, C, ldc, tm, tn);
+        return 1;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto key = std::make_tuple(M, N, A.kpad);
+    auto it = g_gm.find(key);
+    if (it != g_gm.end()) {
+        launch_gm(s, it->second.first, M, N, A, B, C, ldc, tm, tn);
+        return 1;
+    }
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    QK_HIP(hipEventCreate(&ev[0]));
+    QK_HIP(hipEventCreate(&ev[1]));
+    std::array<float, NGM> ms{};
+    int best = 8;
+    float tbest = 1e30f;
+    for (int c = 0; c < NGM; ++c) {
+        (void)hipEventRecord(ev[0], s);
+        launch_gm(s, GM_CAND[c], M, N, A, B, C, ldc, tm, tn);
+        (void)hipEventRecord(ev[1], s);
+        float t = 1e30f;
+        if (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&t, ev[0], ev[1]) != hipSuccess) t = 1e30f;
+        ms[c] = t;
+        if (t < tbest) {
+            tbest = t;
+            best = GM_CAND[c];
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    g_gm.emplace(key, std::make_pair(best, ms));
+    return NGM;
+}
+
+std::string autotune_report_bf16() {
+    std::lock_guard<std::mutex> lock(g_mu);
+    std::string r;
+    for (const auto& kv : g_gm) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, "gemm_bf16x3 M=%d N=%d kpad=%d -> group %d (ms:", std::get<0>(kv.first),
+                      std::get<1>(kv.first), std::get<2>(kv.first), kv.second.first);
+        r += buf;
+        for (int c = 0; c < NGM; ++c) {
+            std::snprintf(buf, sizeof buf, " g%d %.3f", GM_CAND[c], kv.second.second[c]);
+            r += buf;
+        }
+        r += ")\n";
+    }
+    return r;
 }
 
 }  // namespace qk

@@ -595,12 +595,16 @@ void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors)
 // rhoT[m + nS (p + n q)] = sum_jb W[jb, m] M[pq, jb]  (rho^T = W^T M^T)
 void excitation_df(Ctx* c);
 
+// excitation GEMM launches (several on a shape's first, tuning call)
+void count_gemm(Ctx* c, int launches) {
+    c->t_stage[6] += launches;
+    c->n_gemm_total += launches;
+}
+
 void excitation(Ctx* c) {
     Stage st(c, 2);
     long long n2 = (long long)c->nOrb * c->nOrb;
     QK_CHECK(n2 < (1LL << 31), QK_E_ARG, "nOrb^2 too large");
-    c->t_stage[6] += 1.0;
-    c->n_gemm_total += 1.0;
     if (c->use_df) {
         excitation_df(c);
         return;
@@ -612,7 +616,8 @@ void excitation(Ctx* c) {
     }
     if (c->precision == 0) {
         Stage g(c, 5, true);
-        dgemm(c->s, true, true, c->nS, (int)n2, c->nS, 1.0, c->Cm, c->nS, c->M, n2, 0.0, c->rhoT, c->nS);
+        count_gemm(c, dgemm(c->s, true, true, c->nS, (int)n2, c->nS, 1.0, c->Cm, c->nS, c->M, n2, 0.0, c->rhoT,
+                            c->nS));
         return;
     }
     // rho^T(m, pq) = sum_jb W(jb, m) M(pq, jb): A rows m (k = jb contiguous in W),
@@ -623,7 +628,7 @@ void excitation(Ctx* c) {
     }
     split3(c->s, c->Cm, c->nS, true, c->nS, c->nS, c->Wsplit);
     Stage g(c, 5, true);
-    gemm_bf16x3(c->s, c->nS, (int)n2, c->Wsplit, c->Msplit, c->rhoT, c->nS);
+    count_gemm(c, gemm_bf16x3(c->s, c->nS, (int)n2, c->Wsplit, c->Msplit, c->rhoT, c->nS));
 }
 
 // Density-fitted excitation densities: T(P, m) = sum_jb Bov(P, jb) W(jb, m),
@@ -635,7 +640,7 @@ void excitation_df(Ctx* c) {
     if (c->precision == 0) {
         dgemm(c->s, false, false, naux, nS, nS, 1.0, c->Bov, naux, c->Cm, nS, 0.0, c->Tdf, naux);
         Stage g(c, 5, true);
-        dgemm(c->s, true, false, nS, (int)n2, naux, 1.0, c->Tdf, naux, c->Bdf, naux, 0.0, c->rhoT, nS);
+        count_gemm(c, dgemm(c->s, true, false, nS, (int)n2, naux, 1.0, c->Tdf, naux, c->Bdf, naux, 0.0, c->rhoT, nS));
         return;
     }
     if (!c->Wsplit.s[0]) alloc_split(c->Wsplit, nS, nS);
@@ -654,7 +659,7 @@ void excitation_df(Ctx* c) {
     gemm_bf16x3(c->s, naux, nS, c->BovTsplit, c->Wsplit, c->Tdf, naux);
     split3(c->s, c->Tdf, naux, true, nS, naux, c->Tsplit);
     Stage g(c, 5, true);
-    gemm_bf16x3(c->s, nS, (int)n2, c->Tsplit, c->Bsplit, c->rhoT, nS);
+    count_gemm(c, gemm_bf16x3(c->s, nS, (int)n2, c->Tsplit, c->Bsplit, c->rhoT, nS));
 }
 
 void sigma_static(Ctx* c, const double* e, double eta, int srg) {
@@ -1783,6 +1788,19 @@ int qk_gwb_stage_times(qk_gwb_handle h, double* ms, int n) {
     if (!h || !ms || n < 0 || n > 4) return QK_E_ARG;
     for (int i = 0; i < n; ++i) ms[i] = h->t_stage[i];
     return QK_OK;
+}
+
+
+int qk_set_autotune(int on) {
+    qk::set_autotune(on != 0);
+    return QK_OK;
+}
+
+int qk_autotune_report(char* buf, int len) {
+    if (!buf || len <= 0) return QK_E_ARG;
+    std::string r = qk::autotune_report() + qk::autotune_report_bf16();
+    std::snprintf(buf, (size_t)len, "%s", r.c_str());
+    return (int)r.size() < len ? QK_OK : QK_E_ARG;
 }
 
 }  // extern "C"

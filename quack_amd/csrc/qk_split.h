@@ -20,6 +20,7 @@ inline int pad32(int k) { return (k + 31) / 32 * 32; }
 void split3(hipStream_t s, const double* src, long long ld, bool k_contiguous, int rows, int K, Split3& out);
 
 // C[i + ldc*j] = sum_k A(i, k) B(j, k), FP64 result from BF16 slice products.
-void gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc);
+// Returns the number of kernel launches it issued (more than one while tuning).
+int gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc);
 
 }  // namespace qk

@@ -29,7 +29,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats",
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
-           "qk_gwb_stage_times"]
+           "qk_gwb_stage_times", "qk_set_autotune", "qk_autotune_report"]
 
 
 class QKError(RuntimeError):
@@ -91,6 +91,8 @@ def load():
     lib.qk_gwb_set_eri_mo.argtypes = [_H, _I, _P]
     lib.qk_gwb_g0w0.argtypes = [_H, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_gwb_stage_times.argtypes = [_H, _P, _I]
+    lib.qk_set_autotune.argtypes = [_I]
+    lib.qk_autotune_report.argtypes = [ctypes.c_char_p, _I]
     lib.qk_gw_set_eri_df_device.argtypes = [_H, _H, _I]
     lib.qk_gw_eig_stats.argtypes = [_H, ctypes.POINTER(ctypes.c_longlong), _I]
     _F = ctypes.POINTER(ctypes.c_float)
@@ -251,6 +253,19 @@ class GWEngine:
         return dict(zip(["aotomo", "rpa_build", "eigensolve", "excitation", "sigma_qp", "total", "excitation_gemm",
                          "excitation_gemm_launches", "excitation_gemm_all_calls", "excitation_gemm_launches_all_calls"],
                         ms))
+
+
+def set_autotune(on):
+    """Process-wide GEMM tile autotuner (on by default; identical results either way)."""
+    load().qk_set_autotune(int(bool(on)))
+
+
+def autotune_report():
+    buf = ctypes.create_string_buffer(1 << 16)
+    rc = load().qk_autotune_report(buf, len(buf))
+    if rc != 0:
+        raise QKError(rc, "autotune report larger than 64 KiB")
+    return [l for l in buf.value.decode().splitlines() if l]
 
 
 class GWBatch:
