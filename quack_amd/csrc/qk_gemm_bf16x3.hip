@@ -78,20 +78,18 @@ __global__ void k_split3_rc(const double* __restrict__ src, long long ld, int ro
 }
 
 // Workgroup tile BM x BN = 128 x 128, 4 wavefronts as 2 x 2, each 64 x 64 =
-// 4 x 4 MFMA tiles.  Per 32-wide k-slab the workgroup stages the three slices
-// of its A rows and B rows in LDS (60 KB); the next slab is fetched into
-// registers while the current one feeds the matrix cores.
+// 4 x 4 MFMA tiles; a 32-wide k-slab of the three slices of its A rows and B
+// rows is 48 KB of LDS.
 //
 // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs, each
 // with its own L2; workgroup p runs logical tile L = x q + min(x, r) + p / 8
 // (x = p % 8, T = 8 q + r tiles), so every XCD works through a contiguous
-// range of logical tiles.  Logical tiles run in groups of GM m-tiles by all
+// range of logical tiles.  Logical tiles run in groups of gm m-tiles (tuned) by all
 // n-tiles, m fastest, so the ~32 workgroups an XCD holds at once share about
 // 8 A row-blocks and 4 B row-blocks: each slab fetched from HBM feeds several
 // workgroups from L2.
 constexpr int BM = 128, BN = 128, NTH = 256, NXCD = 8;
-constexpr int LROW = 40;  // bf16 per LDS row: 32 k + 8 pad (80 B) against bank conflicts
-static_assert(BM == 128 && BN == 128 && NTH == 256, "slab staging assumes a 128 x 128 tile and 256 threads");
+static_assert(BM == 128 && BN == 128 && NTH == 256, "slab moves assume a 128 x 128 tile and 256 threads");
 inline int grid_for(long long total) { return (int)std::min<long long>((total + 255) / 256, 65536); }
 
 __device__ __forceinline__ void tile_of(int pid, int tm, int tn, int gm, int& mb, int& nb) {
@@ -104,45 +102,57 @@ __device__ __forceinline__ void tile_of(int pid, int tm, int tn, int gm, int& mb
     nb = w / gsm;
 }
 
+// The slabs move straight from HBM into LDS (global_load_lds_dwordx4: each
+// lane's 16 bytes land at the wave's LDS base + 16 * lane), with three slabs in
+// flight in a ring of three LDS stages of 48 KB.
+// LDS rows are 64 bytes (32 k of one slice); the 16-byte k-chunk c of row r is
+// stored in slot c ^ ((r >> 1) & 3), so the matrix-core operand reads (16 rows
+// x one chunk per lane group) hit distinct banks.  Rows past M or N load row
+// 0: their products land only in outputs that are never stored.
+constexpr int SLAB_B = 2 * 3 * 128 * 64;  // A and B, three slices, 128 rows of 64 bytes
+// s_waitcnt immediate for "vector-memory count <= n" (gfx9 layout), other counters left at their maximum
+#define QK_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8))
+
 __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, const __bf16* __restrict__ A0,
-                                                     const __bf16* __restrict__ A1, const __bf16* __restrict__ A2,
-                                                     const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
-                                                     const __bf16* __restrict__ B2, double* __restrict__ C,
-                                                     long long ldc, int tiles_m, int tiles_n, int gm) {
-    __shared__ __attribute__((aligned(16))) __bf16 As[3][BM][LROW];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[3][BN][LROW];
+                                                         const __bf16* __restrict__ A1, const __bf16* __restrict__ A2,
+                                                         const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
+                                                         const __bf16* __restrict__ B2, double* __restrict__ C,
+                                                         long long ldc, int tiles_m, int tiles_n, int gm) {
+    __shared__ __attribute__((aligned(1024))) char st0[SLAB_B];
+    __shared__ __attribute__((aligned(1024))) char st1[SLAB_B];
+    __shared__ __attribute__((aligned(1024))) char st2[SLAB_B];
     int mb, nb;
     tile_of(blockIdx.x, tiles_m, tiles_n, gm, mb, nb);
     const int m0 = mb * BM, n0 = nb * BN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
-    const int l16 = lane & 15, lk = (lane >> 4) * 8;
-
-    // slab staging: thread tid owns rows tid/4 and tid/4 + 64 and the 16-byte
-    // k-chunk tid%4 of every slice of A and of B (12 chunks)
-    const int rr = tid >> 2, kc8 = (tid & 3) * 8;
-    const int ia0 = m0 + rr, ia1 = m0 + rr + 64, jb0 = n0 + rr, jb1 = n0 + rr + 64;
-    const long long oa0 = (long long)(ia0 < M ? ia0 : 0) * kpad + kc8;
-    const long long oa1 = (long long)(ia1 < M ? ia1 : 0) * kpad + kc8;
-    const long long ob0 = (long long)(jb0 < N ? jb0 : 0) * kpad + kc8;
-    const long long ob1 = (long long)(jb1 < N ? jb1 : 0) * kpad + kc8;
-    // rows past M / N read row 0 (in bounds) and are zeroed by a mask: no branches
-    const unsigned ma0 = ia0 < M ? ~0u : 0u, ma1 = ia1 < M ? ~0u : 0u;
-    const unsigned mb0 = jb0 < N ? ~0u : 0u, mb1 = jb1 < N ? ~0u : 0u;
-    // one register set holds the next slab while the current one is in LDS;
-    // the row mask is applied when a slab is written to LDS, not at the load,
-    // so the loads stay in flight during the current slab's products
-    uint4 pa00, pa01, pa10, pa11, pa20, pa21, pb00, pb01, pb10, pb11, pb20, pb21;
-#define QK_LD(dst, base, off) dst = *reinterpret_cast<const uint4*>((base) + (off))
-#define QK_FETCH(k0)                                                                                        \
-    do {                                                                                                    \
-        QK_LD(pa00, A0, oa0 + (k0)); QK_LD(pa01, A0, oa1 + (k0)); QK_LD(pa10, A1, oa0 + (k0));             \
-        QK_LD(pa11, A1, oa1 + (k0)); QK_LD(pa20, A2, oa0 + (k0)); QK_LD(pa21, A2, oa1 + (k0));             \
-        QK_LD(pb00, B0, ob0 + (k0)); QK_LD(pb01, B0, ob1 + (k0)); QK_LD(pb10, B1, ob0 + (k0));             \
-        QK_LD(pb11, B1, ob1 + (k0)); QK_LD(pb20, B2, ob0 + (k0)); QK_LD(pb21, B2, ob1 + (k0));             \
+    const int l16 = lane & 15;
+    // operand reads: lane reads row (.. + l16), k-chunk lane/16, stored in slot
+    const int rslot = ((lane >> 4) ^ ((l16 >> 1) & 3)) * 16;
+    // slab writes: wave instruction j of 12 fills the 1 KB block idx = 12 wave + j:
+    // operand idx / 24 (A, B), slice (idx % 24) / 8, rows 16 (idx % 8) .. +15;
+    // lane writes row 16 (idx % 8) + lane / 4, slot lane % 4 = chunk (lane % 4) ^ ((lane >> 3) & 3)
+    const int wchunk = ((lane & 3) ^ ((lane >> 3) & 3)) * 8;
+    const __bf16* src[12];
+    int ldo[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+        const int idx = wave * 12 + j;
+        const int op = idx / 24, sl = (idx % 24) / 8, g = idx % 8;
+        const int row = g * 16 + (lane >> 2);
+        int gr = (op ? n0 : m0) + row;
+        if (gr >= (op ? N : M)) gr = 0;
+        const __bf16* base = op ? (sl == 0 ? B0 : sl == 1 ? B1 : B2) : (sl == 0 ? A0 : sl == 1 ? A1 : A2);
+        src[j] = base + (long long)gr * kpad + wchunk;
+        ldo[j] = ((op * 3 + sl) * 128 + g * 16) * 64;
+    }
+#define QK_ISSUE(stage, k0)                                                                                  \
+    do {                                                                                                     \
+        _Pragma("unroll") for (int j = 0; j < 12; ++j)                                                       \
+            __builtin_amdgcn_global_load_lds((const void*)(src[j] + (k0)),                                   \
+                                             (__attribute__((address_space(3))) void*)((stage) + ldo[j]), 16, 0, 0); \
     } while (0)
-#define QK_ST(arr, row, v, msk) \
-    *reinterpret_cast<uint4*>(&arr[row][kc8]) = uint4{v.x & (msk), v.y & (msk), v.z & (msk), v.w & (msk)}
 
     d4 acc64[4][4];
     f4 acc[4][4];
@@ -154,59 +164,66 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
             acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
         }
 
-    QK_FETCH(0);
-    for (int k0 = 0; k0 < kpad; k0 += KS) {
-        __syncthreads();
-        QK_ST(As[0], rr, pa00, ma0); QK_ST(As[0], rr + 64, pa01, ma1);
-        QK_ST(As[1], rr, pa10, ma0); QK_ST(As[1], rr + 64, pa11, ma1);
-        QK_ST(As[2], rr, pa20, ma0); QK_ST(As[2], rr + 64, pa21, ma1);
-        QK_ST(Bs[0], rr, pb00, mb0); QK_ST(Bs[0], rr + 64, pb01, mb1);
-        QK_ST(Bs[1], rr, pb10, mb0); QK_ST(Bs[1], rr + 64, pb11, mb1);
-        QK_ST(Bs[2], rr, pb20, mb0); QK_ST(Bs[2], rr + 64, pb21, mb1);
-        __syncthreads();
-        if (k0 + KS < kpad) QK_FETCH(k0 + KS);
-        bf16x8 fb[4][3];
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int sl = 0; sl < 3; ++sl)
-                fb[b][sl] = *reinterpret_cast<const bf16x8*>(&Bs[sl][wn + b * 16 + l16][lk]);
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            bf16x8 fa[3];
-#pragma unroll
-            for (int sl = 0; sl < 3; ++sl) fa[sl] = *reinterpret_cast<const bf16x8*>(&As[sl][wm + a * 16 + l16][lk]);
-            // six slice products, smallest weight first; the four b tiles are
-            // independent, so consecutive matrix-core ops never wait on each other
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][2], fa[0], acc[a][b], 0, 0, 0);
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[1], acc[a][b], 0, 0, 0);
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[2], acc[a][b], 0, 0, 0);
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[0], acc[a][b], 0, 0, 0);
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[1], acc[a][b], 0, 0, 0);
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[0], acc[a][b], 0, 0, 0);
-        }
-        // FP64 recombination every KC k-values
-        if ((k0 + KS) % KC == 0 || k0 + KS >= kpad) {
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc[a][b][r];
-                    acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-                }
-        }
-    }
+#define QK_FRAG(stage, opsl, row) (*reinterpret_cast<const bf16x8*>((stage) + ((opsl) * 128 + (row)) * 64 + rslot))
+#define QK_SLAB(stage, k0)                                                                                   \
+    do {                                                                                                     \
+        bf16x8 fb[4][3];                                                                                     \
+        _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                        \
+            _Pragma("unroll") for (int sl = 0; sl < 3; ++sl) fb[b][sl] = QK_FRAG(stage, 3 + sl, wn + b * 16 + l16); \
+        _Pragma("unroll") for (int a = 0; a < 4; ++a) {                                                      \
+            bf16x8 fa[3];                                                                                    \
+            _Pragma("unroll") for (int sl = 0; sl < 3; ++sl) fa[sl] = QK_FRAG(stage, sl, wm + a * 16 + l16); \
+            _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][2], fa[0], acc[a][b], 0, 0, 0);   \
+            _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[1], acc[a][b], 0, 0, 0);   \
+            _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[2], acc[a][b], 0, 0, 0);   \
+            _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[0], acc[a][b], 0, 0, 0);   \
+            _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[1], acc[a][b], 0, 0, 0);   \
+            _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[0], acc[a][b], 0, 0, 0);   \
+        }                                                                                                    \
+        if (((k0) + KS) % KC == 0 || (k0) + KS >= kpad) {                                                    \
+            _Pragma("unroll") for (int a = 0; a < 4; ++a)                                                    \
+                _Pragma("unroll") for (int b = 0; b < 4; ++b) {                                              \
+                    _Pragma("unroll") for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc[a][b][r];    \
+                    acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};                                                      \
+                }                                                                                            \
+        }                                                                                                    \
+    } while (0)
+// slab s from stage cur; stage nxt2 (read two slabs ago) receives slab s + 2.
+// The issue is unconditional (past the last slab it reloads the last slab into
+// a stage nobody reads again), so every path keeps exactly three slabs in flight
+// and the same wait applies everywhere.
+// (plain s_barrier: a __syncthreads() fence would also wait for the slabs still
+// in flight; the explicit vector-memory wait covers exactly the slab needed)
+#define QK_STEP(cur, nxt2, s)                                              \
+    do {                                                                   \
+        __builtin_amdgcn_s_waitcnt(0xC07F); /* own LDS reads of nxt2 done */ \
+        __builtin_amdgcn_s_barrier();                                      \
+        QK_ISSUE(nxt2, min((s) + 2, nslab - 1) * KS);                      \
+        QK_VMCNT(24);                                                      \
+        __builtin_amdgcn_s_barrier();                                      \
+        if ((s) < nslab) QK_SLAB(cur, (s) * KS);                           \
+    } while (0)
 
-#undef QK_ST
-#undef QK_FETCH
-#undef QK_LD
+    const int nslab = kpad / KS;
+    QK_ISSUE(st0, 0);
+    QK_ISSUE(st1, min(1, nslab - 1) * KS);
+    for (int s = 0; s < nslab; s += 3) {
+        QK_STEP(st0, st2, s);
+        QK_STEP(st1, st0, s + 1);
+        QK_STEP(st2, st1, s + 2);
+    }
+    QK_VMCNT(0);  // no slab may land in LDS after the workgroup ends
+#undef QK_STEP
+#undef QK_SLAB
+#undef QK_FRAG
+#undef QK_ISSUE
+
     // acc64[a][b][r] = C(row m0+wm+a*16+l16, col n0+wn+b*16+4*(lane/16)+r)
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -221,6 +238,7 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
             }
     }
 }
+#undef QK_VMCNT
 
 }  // namespace
 
@@ -249,8 +267,8 @@ std::map<std::tuple<int, int, int>, std::pair<int, std::array<float, NGM>>> g_gm
 
 void launch_gm(hipStream_t s, int gm, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc, int tm,
                int tn) {
-    hipLaunchKernelGGL(k_gemm_bf16x3, dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1], A.s[2], B.s[0],
-                       B.s[1], B.s[2], C, ldc, tm, tn, gm);
+    hipLaunchKernelGGL(k_gemm_bf16x3, dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1], A.s[2],
+                       B.s[0], B.s[1], B.s[2], C, ldc, tm, tn, gm);
     QK_HIP(hipGetLastError());
 }
 }  // namespace
