@@ -104,7 +104,10 @@ int qk_gw_set_precision(qk_gw_handle h, int mode);
  * RPA of qk_rg0w0 (LR/phRLR.f90:77, diagonalize_matrix = dsyev):
  *   mode 0 (default): rocSOLVER dsyevd for every RPA, as QuAcK re-diagonalises
  *           every cycle;
- *   mode 1: when no orbital energy moved by more than de_max (Ha) since the
+ *   mode 2: full solves by the library's own Householder tridiagonalisation
+ *           (LAPACK dsytrd layout), rocSOLVER dstedc/dsterf and dormtr;
+ *   mode 1 (or 3 = 1 with the full solves of mode 2):
+ *           when no orbital energy moved by more than de_max (Ha) since the
  *           handle's last eigenvectors were computed (same TDA_W), refine those
  *           eigenvectors by Newton steps for X^T X = I, X^T C X diagonal (FP64
  *           matrix-core GEMMs, quadratic convergence, stopped when the largest
@@ -113,10 +116,11 @@ int qk_gw_set_precision(qk_gw_handle h, int mode);
  * Both modes return FP64 eigenpairs of the same matrix.  No counterpart in
  * QuAcK.  Mode 1 keeps four nS x nS FP64 work arrays on the device. */
 int qk_gw_set_eigensolver(qk_gw_handle h, int mode, double de_max);
-/* counts[0..n-1], n <= 8, since the handle was created: dsyevd calls, refined
- * solves, Newton steps, refinements abandoned for dsyevd; of those, abandoned
- * because a cluster of close eigenvalues outgrew 32, because merging clusters
- * did not settle, because the steps did not converge; the largest cluster met. */
+/* counts[0..n-1], n <= 9, since the handle was created: full solves, refined
+ * solves, Newton steps, refinements abandoned for a full solve; of those,
+ * abandoned because a cluster of close eigenvalues outgrew 32, because merging
+ * clusters did not settle, because the steps did not converge; the largest
+ * cluster met; full solves by the own tridiagonalisation. */
 int qk_gw_eig_stats(qk_gw_handle h, long long *counts, int n);
 
 /* Device time (ms, HIP events on the handle's stream) of the stages of the

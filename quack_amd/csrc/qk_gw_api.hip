@@ -62,6 +62,28 @@ void syevd(hipStream_t s, rocblas_handle rb, bool vectors, int n, double* A, dou
     QK_CHECK(info == 0, QK_E_EIG, "rocsolver_dsyevd failed, info = " + std::to_string(info));
 }
 
+// Full eigen-decomposition without dsyevd: own Householder tridiagonalisation
+// (qk_sytrd.hip), rocSOLVER's divide-and-conquer dstedc on the tridiagonal
+// (dsterf for eigenvalues only), rocSOLVER's dormtr for the back-transformation
+// with the reflectors.  Same outputs as syevd(): w ascending, A = eigenvectors.
+void syevd_own(hipStream_t s, rocblas_handle rb, bool vectors, int n, double*& A, double* w, double* E, double* tau,
+               double*& Z, double* work, int* dinfo) {
+    sytrd_upper(s, n, A, w, E, tau, work, 32);
+    if (vectors) {
+        rb_check(rocsolver_dstedc(rb, rocblas_evect_tridiagonal, n, w, E, Z, n, dinfo), "rocsolver_dstedc");
+        rb_check(rocsolver_dormtr(rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, n, n, A, n, tau,
+                                  Z, n),
+                 "rocsolver_dormtr");
+        std::swap(A, Z);
+    } else {
+        rb_check(rocsolver_dsterf(rb, n, w, E, dinfo), "rocsolver_dsterf");
+    }
+    int info = 0;
+    QK_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+    QK_HIP(hipStreamSynchronize(s));
+    QK_CHECK(info == 0, QK_E_EIG, "tridiagonal eigensolver failed, info = " + std::to_string(info));
+}
+
 // Tiny symmetric solve for DIIS: LU with partial pivoting; rcond is the exact
 // 1-norm reciprocal condition number (what dsysvx estimates,
 // utils/wrap_lapack.f90:560-598) from the explicit inverse of the (<= 33)x(<= 33) matrix.
@@ -175,8 +197,9 @@ struct qk_gw_ctx {
     // eigensolver: 0 = rocSOLVER dsyevd for every RPA (QuAcK's scheme); 1 = refine
     // the previous eigenvectors by Newton steps when the energies moved by at
     // most eig_demax since they were computed, dsyevd otherwise or on failure
-    int eig_mode = 0;
+    int eig_mode = 0;  // bit 0: refine; bit 1: own tridiagonalisation instead of dsyevd
     double eig_demax = 5e-3;
+    double *Zev = nullptr, *tau = nullptr, *trd_work = nullptr;  // own tridiagonalisation path
     double *Xv = nullptr, *Ym = nullptr, *Sm = nullptr, *Gm = nullptr, *lam = nullptr, *eig_dbuf = nullptr;
     int *perm = nullptr, *eig_ibuf = nullptr;
     long long* eig_lbuf = nullptr;
@@ -186,7 +209,7 @@ struct qk_gw_ctx {
     // dsyevd calls, refined solves, Newton steps, fallbacks; fallbacks because a
     // cluster outgrew EIG_KMAX, because merging did not settle, because the
     // steps did not converge; largest cluster met
-    long long eig_count[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long eig_count[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // [8]: full solves by the own tridiagonalisation
     double t_ao = 0.0;
     // [0..4] RPA build, eigensolve, excitation densities, self-energy + QP, total;
     // [5] excitation-density GEMM launches alone, [6] their number
@@ -563,7 +586,7 @@ void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors)
     }
     {
         Stage st(c, 1);
-        bool use_refine = c->eig_mode == 1 && vectors && c->have_vec && c->tda_vec == tda;
+        bool use_refine = (c->eig_mode & 1) && vectors && c->have_vec && c->tda_vec == tda;
         if (use_refine) {
             double d = 0.0;
             for (int p = 0; p < c->nOrb; ++p) d = std::max(d, std::fabs(eh[p] - c->e_vec[p]));
@@ -575,12 +598,17 @@ void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors)
             ++c->eig_count[done ? 1 : 3];
         }
         if (!done) {
-            syevd(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->info);
+            if (c->eig_mode & 2) {
+                syevd_own(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->tau, c->Zev, c->trd_work, c->info);
+                ++c->eig_count[8];
+            } else {
+                syevd(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->info);
+            }
             ++c->eig_count[0];
-            if (c->eig_mode == 1 && vectors)
+            if ((c->eig_mode & 1) && vectors)
                 QK_HIP(hipMemcpyAsync(c->Xv, c->Cm, sizeof(double) * c->nS * c->nS, hipMemcpyDeviceToDevice, c->s));
         }
-        if (c->eig_mode == 1 && vectors) {
+        if ((c->eig_mode & 1) && vectors) {
             c->have_vec = true;
             c->tda_vec = tda;
             c->e_vec.assign(eh, eh + c->nOrb);
@@ -827,7 +855,7 @@ void release(Ctx* c) {
     int* is[] = {c->nit, c->done, c->info, c->active};
     for (int* p : is)
         if (p) (void)hipFree(p);
-    for (double* p : {c->Xv, c->Ym, c->Sm, c->Gm, c->lam, c->eig_dbuf})
+    for (double* p : {c->Xv, c->Ym, c->Sm, c->Gm, c->lam, c->eig_dbuf, c->Zev, c->tau, c->trd_work})
         if (p) (void)hipFree(p);
     for (int* p : {c->perm, c->eig_ibuf})
         if (p) (void)hipFree(p);
@@ -1059,9 +1087,15 @@ int qk_gw_set_precision(qk_gw_handle c, int mode) {
 }
 
 int qk_gw_set_eigensolver(qk_gw_handle c, int mode, double de_max) {
-    if (!c || mode < 0 || mode > 1 || !(de_max >= 0.0)) return QK_E_ARG;
+    if (!c || mode < 0 || mode > 3 || !(de_max >= 0.0)) return QK_E_ARG;
     return guarded(&c->err, [&] {
-        if (mode == 1 && !c->Xv) {
+        if ((mode & 2) && !c->Zev) {
+            size_t S = c->nS;
+            c->Zev = dalloc<double>(S * S);
+            c->tau = dalloc<double>(S);
+            c->trd_work = dalloc<double>(sytrd_work_doubles(c->nS, 32));
+        }
+        if ((mode & 1) && !c->Xv) {
             size_t S = c->nS;
             c->Xv = dalloc<double>(S * S);
             c->Ym = dalloc<double>(S * S);
@@ -1080,7 +1114,7 @@ int qk_gw_set_eigensolver(qk_gw_handle c, int mode, double de_max) {
 }
 
 int qk_gw_eig_stats(qk_gw_handle c, long long* counts, int n) {
-    if (!c || !counts || n < 0 || n > 8) return QK_E_ARG;
+    if (!c || !counts || n < 0 || n > 9) return QK_E_ARG;
     for (int i = 0; i < n; ++i) counts[i] = c->eig_count[i];
     return QK_OK;
 }

@@ -54,6 +54,10 @@ void mix_rows(hipStream_t s, double* A, int n, const int* idx, const int* ptr, c
 void gather_cols(hipStream_t s, const double* X, int n, const int* idx, int m, double* out);
 // (i, a) block of density-fitting factors: Bov[P + naux ia] = B[P + naux (i + n a)]
 void gather_bov(hipStream_t s, const double* B, double* Bov, int naux, int n, int nC, int nO, int v, int nS);
+// Householder tridiagonalisation, upper triangle, LAPACK dsytrd layout (qk_sytrd.hip);
+// work holds sytrd_work_doubles(n, nb) doubles, nb <= 64.
+size_t sytrd_work_doubles(int n, int nb);
+void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* TAU, double* work, int nb);
 // A symmetric from its upper triangle: A[i, j] = A[j, i] for i > j
 void sym_lower(hipStream_t s, double* A, int n);
 void permute_cols(hipStream_t s, const double* X, const int* perm, double* out, int n);

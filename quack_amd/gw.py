@@ -174,14 +174,16 @@ class GWEngine:
     def set_eigensolver(self, mode, de_max=5e-3):
         """"syevd" (0): dsyevd for every RPA, QuAcK's scheme; "refine" (1):
         Newton refinement of the previous eigenvectors when no orbital energy
-        moved by more than de_max Ha, dsyevd otherwise."""
-        _check(load().qk_gw_set_eigensolver(self.h, {"syevd": 0, "refine": 1}.get(mode, mode), de_max), self.h)
+        moved by more than de_max Ha, dsyevd otherwise; "sytrd" (2): full solves
+        through the library's own tridiagonalisation; "refine+sytrd" (3)."""
+        _check(load().qk_gw_set_eigensolver(self.h, {"syevd": 0, "refine": 1, "sytrd": 2, "refine+sytrd": 3}.get(
+            mode, mode), de_max), self.h)
 
     def eig_stats(self):
-        c = (ctypes.c_longlong * 8)()
-        _check(load().qk_gw_eig_stats(self.h, c, 8), self.h)
+        c = (ctypes.c_longlong * 9)()
+        _check(load().qk_gw_eig_stats(self.h, c, 9), self.h)
         return {"syevd": c[0], "refined": c[1], "newton_steps": c[2], "fallbacks": c[3], "fail_cluster_size": c[4],
-                "fail_unsettled": c[5], "fail_noconv": c[6], "largest_cluster": c[7]}
+                "fail_unsettled": c[5], "fail_noconv": c[6], "largest_cluster": c[7], "own_tridiagonal": c[8]}
 
     def set_eri_df(self, B):
         """Density-fitted MO integrals: B of shape (naux, nOrb, nOrb) (any float64
