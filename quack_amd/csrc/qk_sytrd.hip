@@ -24,8 +24,10 @@
 namespace qk {
 namespace {
 
-constexpr int TB = 256;  // threads per block
-constexpr int TS = 64;   // symv tile
+constexpr int TB = 256;    // threads per block (tile products)
+constexpr int TBW = 1024;  // threads per block (row-wise kernels: 64 rows x 16 waves)
+constexpr int TS = 64;     // symv tile
+constexpr int GT = 4;      // tiles per symv workgroup (one column strip)
 
 __device__ inline double wg_sum(double v, double* sh) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -59,49 +61,68 @@ __device__ inline void householder(const double* A, int n, int c, const double* 
     }
 }
 
-// K_a: rows r <= c.  The previous W column piw (= iw + 1) still lacks
-// palpha v_prev (v_prev = A(:, c+1), its unit element A(c, c+1) stored as 1):
-// rows r < c are finalised in memory; row c, read by every thread, is only
-// ever used here and is finalised on the fly.  Then
-// A(r, c) -= sum_k A(r, c+1+k) W(c, iw+1+k) + W(r, iw+1+k) A(c, c+1+k), k < j;
-// pnorm[block] = sum over r < c-1 of the updated A(r, c)^2;  D[c] = A(c, c).
-__global__ __launch_bounds__(TB) void k_col_update(double* __restrict__ A, int n, double* __restrict__ W, int ldw,
-                                                   int c, int iw, int j, int piw, const double* __restrict__ pdot,
-                                                   int npdot, const double* __restrict__ scal_prev,
-                                                   double* __restrict__ pnorm, double* __restrict__ D) {
-    __shared__ double sh[TB / 64];
-    __shared__ double red[4][64];
+// strips of GT row tiles: strip J (columns 64J..64J+63) holds tiles I = 0..J
+// in J/GT + 1 groups; groups_before(J) = sum over J' < J of (J'/GT + 1)
+__host__ __device__ inline int groups_before(int J) {
+    const int a = J / GT, b = J % GT;
+    return GT * a * (a - 1) / 2 + a * b + J;
+}
+
+// K_a: rows r <= c, 64 rows per block, the sixteen waves split the panel
+// columns.  The previous W column piw (= iw + 1) still lacks palpha v_prev
+// (v_prev = A(:, c+1), its unit element A(c, c+1) stored as 1): rows r < c are
+// finalised in memory; row c, read by every block, is finalised on the fly.
+// Then A(r, c) -= sum_k A(r, c+1+k) W(c, iw+1+k) + W(r, iw+1+k) A(c, c+1+k),
+// k < j; pnorm[block] = sum over r < c-1 of the updated A(r, c)^2; D[c] = A(c, c).
+__global__ __launch_bounds__(TBW) void k_col_update(double* __restrict__ A, int n, double* __restrict__ W, int ldw,
+                                                    int c, int iw, int j, int piw, const double* __restrict__ pdot,
+                                                    int npdot, const double* __restrict__ scal_prev,
+                                                    double* __restrict__ pnorm, double* __restrict__ D) {
+    __shared__ double sh[TBW / 64];
+    __shared__ double red[TBW / 64][64];
+    constexpr int NW = TBW / 64;
+    const int rr = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int r = blockIdx.x * 64 + rr;
+    const bool on = r <= c;
+    // panel loads first (they do not depend on palpha)
+    constexpr int KPW = 64 / NW;  // panel columns per wave (nb <= 64)
+    double ar[KPW], wr[KPW], wc[KPW], ac[KPW];
+#pragma unroll
+    for (int t = 0; t < KPW; ++t) {
+        const int k = q + NW * t;
+        const bool use = on && k < j;
+        const long long col = (long long)n * (c + 1 + k), wcol = (long long)ldw * (iw + 1 + k);
+        ar[t] = use ? A[r + col] : 0.0;
+        wr[t] = use ? W[r + wcol] : 0.0;
+        wc[t] = use ? W[c + wcol] : 0.0;
+        ac[t] = use ? A[c + col] : 0.0;
+    }
+    const double aold = on ? A[r + (long long)n * c] : 0.0;
     double palpha = 0.0;
     if (piw >= 0) {
         double s = 0.0;
         for (int k = threadIdx.x; k < npdot; k += blockDim.x) s += pdot[k];
         palpha = -0.5 * scal_prev[1] * wg_sum(s, sh);
     }
-    // 64 rows per block; the four waves split the panel columns k = q, q+4, ...
-    const int rr = threadIdx.x & 63, q = threadIdx.x >> 6;
-    const int r = blockIdx.x * 64 + rr;
-    const bool on = r <= c;
-    double wfin = 0.0;  // finalised W(r, piw)
-    if (on && piw >= 0) wfin = W[r + (long long)ldw * piw] + palpha * A[r + (long long)n * (c + 1)];
     double acc = 0.0;
-    if (on) {
-        for (int k = q; k < j; k += 4) {
-            const long long col = (long long)n * (c + 1 + k);
-            double wck = W[c + (long long)ldw * (iw + 1 + k)];
-            double wrk = W[r + (long long)ldw * (iw + 1 + k)];
-            if (k == 0 && piw >= 0) {
-                wck += palpha;  // v_prev(c) = 1
-                wrk = wfin;
-            }
-            acc += A[r + col] * wck + wrk * A[c + col];
+#pragma unroll
+    for (int t = 0; t < KPW; ++t) {
+        double wck = wc[t], wrk = wr[t];
+        if (q + NW * t == 0 && piw >= 0) {
+            wck += palpha;               // v_prev(c) = 1
+            wrk += palpha * ar[t];       // finalised W(r, piw)
         }
+        acc += ar[t] * wck + wrk * ac[t];
     }
     red[q][rr] = acc;
     __syncthreads();
     double sq = 0.0;
     if (q == 0 && on) {
-        if (piw >= 0 && r < c) W[r + (long long)ldw * piw] = wfin;
-        const double a = A[r + (long long)n * c] - (red[0][rr] + red[1][rr] + red[2][rr] + red[3][rr]);
+        if (piw >= 0 && r < c && j > 0) W[r + (long long)ldw * piw] = wr[0] + palpha * ar[0];
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) s += red[k][rr];
+        const double a = aold - s;
         A[r + (long long)n * c] = a;
         if (r < c - 1) sq = a * a;
         if (r == c) D[c] = a;
@@ -110,97 +131,134 @@ __global__ __launch_bounds__(TB) void k_col_update(double* __restrict__ A, int n
     if (threadIdx.x == 0) pnorm[blockIdx.x] = t;
 }
 
-// K_b1: blocks [0, ntiles) are upper tiles (I <= J) of A(0:L, 0:L), L = c; the
-// next 2j blocks are the panel dots.  v(r) = A(r, c) scale (r < c-1), v(c-1) = 1.
-// part[tile][0..63] = row sums for block I, part[tile][64..127] = column sums
-// for block J (strictly upper part; zero on the diagonal tile's lower half).
+// K_b1: blocks [0, 2j) are the panel dots t1 = W_p^T v, t2 = V_p^T v; then one
+// block per group of up to GT upper tiles (I, J), I = GT g .. of one column
+// strip J of A(0:L, 0:L), L = c.  v(r) = A(r, c) scale (r < c-1), v(c-1) = 1.
+// Every block issues its tile loads, then recomputes the Householder scalars
+// from the norm partials.  Each tile's row sums (its product with v_J) go to
+// prow[tile], the strip's column sums (A_IJ^T v_I summed over the group's
+// tiles, transposed through LDS) to pcol[group]; on a diagonal tile the row
+// sums take the upper part with the diagonal, the column sums the strict part.
 __global__ __launch_bounds__(TB) void k_symv_tiles(const double* __restrict__ A, int n, const double* __restrict__ W,
                                                    int ldw, int c, int iw, int j, const double* __restrict__ pnorm,
-                                                   int npart, int nt, int ntiles, double* __restrict__ part,
+                                                   int npart, double* __restrict__ prow, double* __restrict__ pcol,
                                                    double* __restrict__ t12, double* __restrict__ scal) {
     __shared__ double sh[TB / 64];
-    __shared__ double vI[TS], vJ[TS];
-    __shared__ double rs[4][TS], cs[4][TS];
-    double beta, tau, scale;
-    householder(A, n, c, pnorm, npart, sh, beta, tau, scale);
+    __shared__ double vJ[TS], vI[GT][TS];
+    __shared__ double rs[GT][4][TS];
+    __shared__ double cq[4][TS];
+    __shared__ double ct[TS][TS + 1];
     const int L = c;
-    auto vof = [&](int r) -> double {
-        if (r >= L) return 0.0;
-        if (r == L - 1) return 1.0;
-        return A[r + (long long)n * c] * scale;
-    };
     const int b = blockIdx.x;
-    if (b < ntiles) {
-        if (b == 0 && threadIdx.x == 0) {
-            scal[0] = beta;
-            scal[1] = tau;
-            scal[2] = scale;
+    if (b < 2 * j) {
+        double beta, tau, scale;
+        householder(A, n, c, pnorm, npart, sh, beta, tau, scale);
+        const double* colp = b < j ? W + (long long)ldw * (iw + 1 + b) : A + (long long)n * (c + 1 + (b - j));
+        double s = 0.0;
+        for (int r = threadIdx.x; r < L; r += blockDim.x) {
+            const double v = r == L - 1 ? 1.0 : A[r + (long long)n * c] * scale;
+            s += colp[r] * v;
         }
-        // tile index -> (I, J), I <= J, column-major over the upper triangle
-        int J = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
-        while ((J + 1) * (J + 2) / 2 <= b) ++J;
-        while (J * (J + 1) / 2 > b) --J;
-        const int I = b - J * (J + 1) / 2;
-        const int r0 = I * TS, c0 = J * TS;
-        if (threadIdx.x < TS) vI[threadIdx.x] = vof(r0 + threadIdx.x);
-        else if (threadIdx.x < 2 * TS) vJ[threadIdx.x - TS] = vof(c0 + threadIdx.x - TS);
-        __syncthreads();
-        // thread: row rr = tid % 64 of the tile, column quarter q = tid / 64 (16 columns)
-        const int rr = threadIdx.x & 63, q = threadIdx.x >> 6;
-        const int r = r0 + rr;
-        double rsum = 0.0;
-        for (int cc = q * 16; cc < q * 16 + 16; ++cc) {
-            const int col = c0 + cc;
-            double a = 0.0;
-            bool use = r < L && col < L && (I < J || rr <= cc);
-            if (use) a = A[r + (long long)n * col];
-            rsum += a * vJ[cc];
-            // column sums: strictly upper only (the diagonal entry is counted once, in the row sum)
-            double cpart = (use && (I < J || rr < cc)) ? a * vI[rr] : 0.0;
-            // reduce cpart over the 64 rows of this wave
-            for (int off = 32; off > 0; off >>= 1) cpart += __shfl_down(cpart, off, 64);
-            if (rr == 0) cs[q][cc - q * 16] = cpart;
-        }
-        rs[q][rr] = rsum;
-        __syncthreads();
-        double* o = part + (long long)b * 2 * TS;
-        if (threadIdx.x < TS) {
-            o[threadIdx.x] = rs[0][threadIdx.x] + rs[1][threadIdx.x] + rs[2][threadIdx.x] + rs[3][threadIdx.x];
-        } else if (threadIdx.x < 2 * TS) {
-            const int cc = threadIdx.x - TS;
-            o[TS + cc] = cs[cc / 16][cc % 16];
-        }
+        const double t = wg_sum(s, sh);
+        if (threadIdx.x == 0) t12[b] = t;
         return;
     }
-    // panel dots: k < j -> t1[k] = W(0:L, iw+1+k) . v ; k >= j -> t2[k-j] = A(0:L, c+1+k-j) . v
-    const int k = b - ntiles;
-    const double* colp = k < j ? W + (long long)ldw * (iw + 1 + k) : A + (long long)n * (c + 1 + (k - j));
+    const int gb = b - 2 * j;
+    int J = (int)sqrt(8.0 * gb);
+    while (J > 0 && groups_before(J) > gb) --J;
+    while (groups_before(J + 1) <= gb) ++J;
+    const int I0 = GT * (gb - groups_before(J));
+    const int nI = min(GT, J - I0 + 1);
+    const int c0 = J * TS;
+    const int rr = threadIdx.x & 63, q = threadIdx.x >> 6;
+    // thread (rr, q): row rr of each tile, columns q*16 .. q*16+15 of the strip
+    double a[GT][16];
+#pragma unroll
+    for (int t = 0; t < GT; ++t) {
+        const int I = I0 + t;
+        const int r = I * TS + rr;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int cc = q * 16 + k;
+            const int col = c0 + cc;
+            const bool use = t < nI && r < L && col < L && (I < J || rr <= cc);
+            a[t][k] = use ? A[r + (long long)n * col] : 0.0;
+        }
+    }
+    double beta, tau, scale;
+    householder(A, n, c, pnorm, npart, sh, beta, tau, scale);
+    if (gb == 0 && threadIdx.x == 0) {
+        scal[0] = beta;
+        scal[1] = tau;
+        scal[2] = scale;
+    }
+    for (int e = threadIdx.x; e < (GT + 1) * TS; e += TB) {
+        const int t = e / TS, x = e % TS;
+        const int r = t == 0 ? c0 + x : (I0 + t - 1) * TS + x;
+        double v = 0.0;
+        if (r < L - 1) v = A[r + (long long)n * c] * scale;
+        else if (r == L - 1) v = 1.0;
+        if (t == 0) vJ[x] = v;
+        else vI[t - 1][x] = v;
+    }
+    __syncthreads();
+    double colacc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) colacc[k] = 0.0;
+#pragma unroll
+    for (int t = 0; t < GT; ++t) {
+        const double vi = vI[t][rr];
+        const bool diag = I0 + t == J;
+        double rsum = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int cc = q * 16 + k;
+            rsum += a[t][k] * vJ[cc];
+            colacc[k] += (!diag || rr < cc) ? a[t][k] * vi : 0.0;
+        }
+        rs[t][q][rr] = rsum;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) ct[q * 16 + k][rr] = colacc[k];
+    __syncthreads();
+    if (q < nI) prow[(long long)(J * (J + 1) / 2 + I0 + q) * TS + rr] = rs[q][0][rr] + rs[q][1][rr] + rs[q][2][rr] + rs[q][3][rr];
     double s = 0.0;
-    for (int r = threadIdx.x; r < L; r += blockDim.x) s += colp[r] * vof(r);
-    const double t = wg_sum(s, sh);
-    if (threadIdx.x == 0) t12[k] = t;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += ct[rr][q * 16 + k];
+    cq[q][rr] = s;
+    __syncthreads();
+    if (q == 0) pcol[(long long)gb * TS + rr] = cq[0][rr] + cq[1][rr] + cq[2][rr] + cq[3][rr];
 }
 
-// K_b2: rows r < L.  y(r) = sum of tile partials; w = tau (y - A_p t1 - W_p t2);
-// W(r, iw) = w; pdot[block] = sum w v; stores v into A(r, c) and the unit, E, TAU.
-__global__ __launch_bounds__(TB) void k_symv_finish(double* __restrict__ A, int n, double* __restrict__ W, int ldw,
-                                                    int c, int iw, int j, int nt, const double* __restrict__ part,
-                                                    const double* __restrict__ t12, const double* __restrict__ scal,
-                                                    double* __restrict__ pdot, double* __restrict__ E,
-                                                    double* __restrict__ TAU) {
-    __shared__ double sh[TB / 64];
-    __shared__ double red[4][64];
+// K_b2: rows r < L, 64 rows per block, the sixteen waves split the partial
+// sums and the panel columns.  y(r) = row sums of tiles (Ib, J >= Ib) + column
+// sums of strip Ib's groups; w = tau (y - A_p t1 - W_p t2); W(r, iw) = w;
+// pdot[block] = sum w v; stores v into A(r, c) and the unit, E, TAU.
+__global__ __launch_bounds__(TBW) void k_symv_finish(double* __restrict__ A, int n, double* __restrict__ W, int ldw,
+                                                     int c, int iw, int j, int nt, const double* __restrict__ prow,
+                                                     const double* __restrict__ pcol,
+                                                     const double* __restrict__ t12, const double* __restrict__ scal,
+                                                     double* __restrict__ pdot, double* __restrict__ E,
+                                                     double* __restrict__ TAU) {
+    __shared__ double sh[TBW / 64];
+    __shared__ double red[TBW / 64][64];
+    constexpr int NW = TBW / 64;
     const double tau = scal[1], scale = scal[2];
     const int L = c;
-    // block Ib = 64 rows; the four waves split the tile partials and the panel columns
     const int Ib = blockIdx.x, rr = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int r = Ib * TS + rr;
     double y = 0.0;
     if (r < L) {
-        // row sums of tiles (Ib, J >= Ib) and column sums of tiles (I <= Ib, Ib)
-        for (int J = Ib + q; J < nt; J += 4) y += part[(long long)(J * (J + 1) / 2 + Ib) * 2 * TS + rr];
-        for (int I = q; I <= Ib; I += 4) y += part[(long long)(Ib * (Ib + 1) / 2 + I) * 2 * TS + TS + rr];
-        for (int k = q; k < j; k += 4) {
+        const int nrow = nt - Ib, ng = Ib / GT + 1, g0 = groups_before(Ib);
+        for (int e = q; e < nrow + ng; e += NW) {
+            if (e < nrow) {
+                const int J = Ib + e;
+                y += prow[(long long)(J * (J + 1) / 2 + Ib) * TS + rr];
+            } else {
+                y += pcol[(long long)(g0 + e - nrow) * TS + rr];
+            }
+        }
+        for (int k = q; k < j; k += NW) {
             y -= A[r + (long long)n * (c + 1 + k)] * t12[k];
             y -= W[r + (long long)ldw * (iw + 1 + k)] * t12[j + k];
         }
@@ -209,7 +267,10 @@ __global__ __launch_bounds__(TB) void k_symv_finish(double* __restrict__ A, int 
     __syncthreads();
     double d = 0.0;
     if (q == 0 && r < L) {
-        const double w = tau * (red[0][rr] + red[1][rr] + red[2][rr] + red[3][rr]);
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) s += red[k][rr];
+        const double w = tau * s;
         W[r + (long long)ldw * iw] = w;
         double v;
         if (r == L - 1) {
@@ -248,7 +309,7 @@ size_t sytrd_work_doubles(int n, int nb) {
     const int nt = (n + TS - 1) / TS;
     const size_t ntiles = (size_t)nt * (nt + 1) / 2;
     const size_t nblk = (n + 63) / 64;
-    return (size_t)n * nb + ntiles * 2 * TS + 2 * nb + 2 * nblk + 8;
+    return (size_t)n * nb + (ntiles + groups_before(nt)) * TS + 2 * nb + 2 * nblk + 8;
 }
 
 void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* TAU, double* work, int nb) {
@@ -257,8 +318,9 @@ void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* 
     const size_t ntiles_all = (size_t)nt_all * (nt_all + 1) / 2;
     const int nblk_all = (n + 63) / 64;
     double* W = work;
-    double* part = W + (size_t)n * nb;
-    double* t12 = part + ntiles_all * 2 * TS;
+    double* prow = W + (size_t)n * nb;
+    double* pcol = prow + ntiles_all * TS;
+    double* t12 = pcol + (size_t)groups_before(nt_all) * TS;
     double* pnorm = t12 + 2 * nb;
     double* pdot = pnorm + nblk_all;
     double* scal = pdot + nblk_all;  // two slots of (beta, tau, scale): [0..2] and [4..6]
@@ -275,20 +337,17 @@ void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* 
             // scalars ping-pong between scal[0..2] and scal[4..6] by column parity
             double* sc_cur = scal + 4 * (c & 1);
             double* sc_prev = scal + 4 * ((c + 1) & 1);
-            hipLaunchKernelGGL(k_col_update, dim3(nbu), dim3(TB), 0, s, A, n, W, ldw, c, iw, j, piw, pdot,
+            hipLaunchKernelGGL(k_col_update, dim3(nbu), dim3(TBW), 0, s, A, n, W, ldw, c, iw, j, piw, pdot,
                                npdot_prev, sc_prev, pnorm, D);
             const int L = c;
             const int nt = (L + TS - 1) / TS;
-            const int ntiles = nt * (nt + 1) / 2;
-            const int npart = nbu;
-            hipLaunchKernelGGL(k_symv_tiles, dim3(ntiles + 2 * j), dim3(TB), 0, s, A, n, W, ldw, c, iw, j, pnorm,
-                               npart, nt, ntiles, part, t12, sc_cur);
-            const int nbl = nt;  // one block per 64-row block
-            hipLaunchKernelGGL(k_symv_finish, dim3(nbl), dim3(TB), 0, s, A, n, W, ldw, c, iw, j, nt, part, t12,
-                               sc_cur, pdot, E, TAU);
+            hipLaunchKernelGGL(k_symv_tiles, dim3(2 * j + groups_before(nt)), dim3(TB), 0, s, A, n, W, ldw, c, iw,
+                               j, pnorm, nbu, prow, pcol, t12, sc_cur);
+            hipLaunchKernelGGL(k_symv_finish, dim3(nt), dim3(TBW), 0, s, A, n, W, ldw, c, iw, j, nt, prow, pcol,
+                               t12, sc_cur, pdot, E, TAU);
             QK_HIP(hipGetLastError());
             piw = iw;
-            npdot_prev = nbl;
+            npdot_prev = nt;
         }
         // finalise the panel's last W column (that of c_lo), then the trailing update
         const int m = c_lo;  // rows/cols 0..c_lo-1 remain
