@@ -14,14 +14,16 @@ while with SRG the same perturbation moves them by < 1e-7 meV.
 
 Forms: the plain FP64 form (QuAcK's arithmetic, dsyevd every cycle, one
 molecule at a time) is measured first as the accuracy reference and the
-baseline rate; the timed form (default) runs the excitation densities in split
-precision, refines the previous cycle's eigenvectors where it can, and keeps
-two molecules in flight (one engine, stream and host thread each).  A step is
-one such batch of molecules.  Inputs (AO ERIs, MO coefficients, eHF) are
+baseline rate, then the same FP64 arithmetic with the project's own
+tridiagonalisation in every full solve; the timed form (default) runs the
+excitation densities in split precision, refines the previous cycle's
+eigenvectors where it can, does the remaining full solves through the own
+tridiagonalisation, and keeps three molecules in flight (one engine, stream
+and host thread each).  A step is one such batch of molecules.  Inputs (AO ERIs, MO coefficients, eHF) are
 produced before the timed region by the integral producer and an RHF whose Fock
 builds run through torch on the device.
 
-    python bench.py [--steps K] [--warmup W] [--form fp64|mixed] [--eig syevd|refine]
+    python bench.py [--steps K] [--warmup W] [--form fp64|mixed] [--eig syevd|refine|sytrd|refine+sytrd]
                     [--concurrent C] [--method evgw|g0w0] [--cpu-baseline auto|off]
                     [--learned auto|off] [--workload h50|df600|batch]
 
@@ -156,9 +158,11 @@ def main():
     ap.add_argument("--max-diis", type=int, default=1)
     ap.add_argument("--form", choices=["fp64", "mixed"], default="mixed",
                     help="fp64: QuAcK's arithmetic; mixed: split-BF16 excitation densities")
-    ap.add_argument("--eig", choices=["syevd", "refine"], default="refine",
-                    help="syevd: dsyevd every cycle (QuAcK); refine: Newton refinement of the previous eigenvectors")
-    ap.add_argument("--concurrent", type=int, default=2,
+    ap.add_argument("--eig", choices=["syevd", "refine", "sytrd", "refine+sytrd"], default="refine+sytrd",
+                    help="syevd: dsyevd every cycle (QuAcK); refine: Newton refinement of the previous eigenvectors, "
+                         "dsyevd for the rest; sytrd: full solves through the project's own tridiagonalisation; "
+                         "refine+sytrd: refinement, own-tridiagonalisation full solves for the rest")
+    ap.add_argument("--concurrent", type=int, default=3,
                     help="molecules in flight at once (one engine, stream and host thread each); a step is one "
                          "such batch")
     ap.add_argument("--learned", choices=["auto", "off"], default="auto",
@@ -212,6 +216,13 @@ def main():
     ref = step_on(eng)
     t_fp64 = time.time() - t
     log("plain fp64 form: %.3f s/molecule%s" % (t_fp64, (" nSCF=%d" % ref["nSCF"]) if "nSCF" in ref else ""))
+    # the same FP64 arithmetic with the project's own tridiagonalisation in every full solve
+    eng.set_eigensolver("sytrd")
+    t = time.time()
+    ref_own = step_on(eng)
+    t_fp64_own = time.time() - t
+    log("plain fp64 form, own tridiagonalisation: %.3f s/molecule, max |dE| vs dsyevd %.2e Ha" %
+        (t_fp64_own, float(np.abs(ref_own["eGW"] - ref["eGW"]).max())))
     st0 = eng.stage_times()  # the plain-form launches, part of the fp64 kernel's population only
 
     for g in engs:
@@ -303,7 +314,11 @@ def main():
                    "srg_flow": 500.0 if reg else None, "cycles_per_molecule": r.get("nSCF", 1), "form": args.form,
                    "eigensolver": args.eig, "molecules_in_flight": K, "step": "%d molecules" % K},
         "plain_fp64_form": {"molecules_per_s": 1.0 / t_fp64, "cycles": ref.get("nSCF"),
-                            "speedup_of_this_form": value * t_fp64},
+                            "eigensolver": "dsyevd every cycle", "speedup_of_this_form": value * t_fp64},
+        "plain_fp64_form_own_tridiagonal": {"molecules_per_s": 1.0 / t_fp64_own,
+                                            "eigensolver": "own tridiagonalisation every cycle",
+                                            "max_abs_dE_vs_dsyevd_Ha": float(np.abs(ref_own["eGW"] - ref["eGW"]).max()),
+                                            "speedup_of_this_form": value * t_fp64_own},
         "this_form_one_molecule_alone": {"s": t_single, "speedup_vs_plain_fp64": t_fp64 / t_single,
                                          "stages_ms": st1, "eig_stats": eng.eig_stats()},
         "accuracy_vs_fp64": {"max_abs_dE_meV_all": float(dE.max() / mev),

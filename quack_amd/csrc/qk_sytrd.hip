@@ -42,13 +42,10 @@ __device__ inline double wg_sum(double v, double* sh) {
 }
 
 // scalars of a column: [0] beta, [1] tau, [2] 1/(alpha - beta) (the scale of x)
-// Householder scalars of column c from the norm partials (dlarfg)
-__device__ inline void householder(const double* A, int n, int c, const double* pnorm, int npart, double* sh,
-                                   double& beta, double& tau, double& scale) {
-    double s = 0.0;
-    for (int k = threadIdx.x; k < npart; k += blockDim.x) s += pnorm[k];
-    const double xx = wg_sum(s, sh);
-    const double alpha = A[(c - 1) + (long long)n * c];
+// Householder scalars (dlarfg) from this thread's share of the norm partials
+// and alpha = A(c-1, c); every thread of the block gets them
+__device__ inline void householder(double pn, double alpha, double* sh, double& beta, double& tau, double& scale) {
+    const double xx = wg_sum(pn, sh);
     if (xx == 0.0) {
         beta = alpha;
         tau = 0.0;
@@ -77,9 +74,11 @@ __host__ __device__ inline int groups_before(int J) {
 __global__ __launch_bounds__(TBW) void k_col_update(double* __restrict__ A, int n, double* __restrict__ W, int ldw,
                                                     int c, int iw, int j, int piw, const double* __restrict__ pdot,
                                                     int npdot, const double* __restrict__ scal_prev,
-                                                    double* __restrict__ pnorm, double* __restrict__ D) {
+                                                    double* __restrict__ pnorm, double* __restrict__ D,
+                                                    double* __restrict__ tpart, int ldt) {
     __shared__ double sh[TBW / 64];
     __shared__ double red[TBW / 64][64];
+    __shared__ double xs[64];
     constexpr int NW = TBW / 64;
     const int rr = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int r = blockIdx.x * 64 + rr;
@@ -117,6 +116,7 @@ __global__ __launch_bounds__(TBW) void k_col_update(double* __restrict__ A, int 
     red[q][rr] = acc;
     __syncthreads();
     double sq = 0.0;
+    if (q == 0) xs[rr] = 0.0;
     if (q == 0 && on) {
         if (piw >= 0 && r < c && j > 0) W[r + (long long)ldw * piw] = wr[0] + palpha * ar[0];
         double s = 0.0;
@@ -124,46 +124,53 @@ __global__ __launch_bounds__(TBW) void k_col_update(double* __restrict__ A, int 
         for (int k = 0; k < NW; ++k) s += red[k][rr];
         const double a = aold - s;
         A[r + (long long)n * c] = a;
-        if (r < c - 1) sq = a * a;
+        if (r < c - 1) {
+            sq = a * a;
+            xs[rr] = a;
+        }
         if (r == c) D[c] = a;
     }
-    const double t = wg_sum(sq, sh);
+    const double t = wg_sum(sq, sh);  // its barriers also publish xs
     if (threadIdx.x == 0) pnorm[blockIdx.x] = t;
+    // panel-dot partials over this block's rows of x (= A(r, c), r < c-1):
+    // tpart[k][block] = W(:, iw+1+k) . x, tpart[j+k][block] = A(:, c+1+k) . x
+    const double x = xs[rr];
+#pragma unroll
+    for (int t2 = 0; t2 < KPW; ++t2) {
+        const int k = q + NW * t2;
+        if (k >= j) break;
+        double pw = (k == 0 && piw >= 0 ? wr[t2] + palpha * ar[t2] : wr[t2]) * x;
+        double pa = ar[t2] * x;
+        for (int off = 32; off > 0; off >>= 1) {
+            pw += __shfl_down(pw, off, 64);
+            pa += __shfl_down(pa, off, 64);
+        }
+        if (rr == 0) {
+            tpart[(long long)k * ldt + blockIdx.x] = pw;
+            tpart[(long long)(j + k) * ldt + blockIdx.x] = pa;
+        }
+    }
 }
 
-// K_b1: blocks [0, 2j) are the panel dots t1 = W_p^T v, t2 = V_p^T v; then one
-// block per group of up to GT upper tiles (I, J), I = GT g .. of one column
-// strip J of A(0:L, 0:L), L = c.  v(r) = A(r, c) scale (r < c-1), v(c-1) = 1.
-// Every block issues its tile loads, then recomputes the Householder scalars
-// from the norm partials.  Each tile's row sums (its product with v_J) go to
-// prow[tile], the strip's column sums (A_IJ^T v_I summed over the group's
-// tiles, transposed through LDS) to pcol[group]; on a diagonal tile the row
-// sums take the upper part with the diagonal, the column sums the strict part.
-__global__ __launch_bounds__(TB) void k_symv_tiles(const double* __restrict__ A, int n, const double* __restrict__ W,
-                                                   int ldw, int c, int iw, int j, const double* __restrict__ pnorm,
-                                                   int npart, double* __restrict__ prow, double* __restrict__ pcol,
-                                                   double* __restrict__ t12, double* __restrict__ scal) {
+// K_b1: one block per group of up to GT upper tiles (I, J), I = GT g .., of
+// one column strip J of A(0:L, 0:L), L = c.  v(r) = A(r, c) scale (r < c-1),
+// v(c-1) = 1.  Every block loads its norm partial and vector entries, then its
+// tiles, and recomputes the Householder scalars while the tiles arrive.  Each
+// tile's row sums (its product with v_J) go to prow[tile], the strip's column
+// sums (A_IJ^T v_I summed over the group's tiles, transposed through LDS) to
+// pcol[group]; on a diagonal tile the row sums take the upper part with the
+// diagonal, the column sums the strict part.
+__global__ __launch_bounds__(TB) void k_symv_tiles(const double* __restrict__ A, int n, int c,
+                                                   const double* __restrict__ pnorm, int npart,
+                                                   double* __restrict__ prow, double* __restrict__ pcol,
+                                                   double* __restrict__ scal) {
     __shared__ double sh[TB / 64];
     __shared__ double vJ[TS], vI[GT][TS];
     __shared__ double rs[GT][4][TS];
     __shared__ double cq[4][TS];
     __shared__ double ct[TS][TS + 1];
     const int L = c;
-    const int b = blockIdx.x;
-    if (b < 2 * j) {
-        double beta, tau, scale;
-        householder(A, n, c, pnorm, npart, sh, beta, tau, scale);
-        const double* colp = b < j ? W + (long long)ldw * (iw + 1 + b) : A + (long long)n * (c + 1 + (b - j));
-        double s = 0.0;
-        for (int r = threadIdx.x; r < L; r += blockDim.x) {
-            const double v = r == L - 1 ? 1.0 : A[r + (long long)n * c] * scale;
-            s += colp[r] * v;
-        }
-        const double t = wg_sum(s, sh);
-        if (threadIdx.x == 0) t12[b] = t;
-        return;
-    }
-    const int gb = b - 2 * j;
+    const int gb = blockIdx.x;
     int J = (int)sqrt(8.0 * gb);
     while (J > 0 && groups_before(J) > gb) --J;
     while (groups_before(J + 1) <= gb) ++J;
@@ -171,6 +178,19 @@ __global__ __launch_bounds__(TB) void k_symv_tiles(const double* __restrict__ A,
     const int nI = min(GT, J - I0 + 1);
     const int c0 = J * TS;
     const int rr = threadIdx.x & 63, q = threadIdx.x >> 6;
+    // small loads first: norm partials, alpha, raw vector entries (x(r) = A(r, c), r < L-1)
+    double pn = 0.0;
+    for (int k = threadIdx.x; k < npart; k += TB) pn += pnorm[k];
+    const double alpha = A[(c - 1) + (long long)n * c];
+    constexpr int NV = ((GT + 1) * TS + TB - 1) / TB;
+    double xr[NV];
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+        const int e = threadIdx.x + u * TB;
+        const int t = e / TS, x = e % TS;
+        const int r = t == 0 ? c0 + x : (I0 + t - 1) * TS + x;
+        xr[u] = (e < (GT + 1) * TS && r < L - 1) ? A[r + (long long)n * c] : 0.0;
+    }
     // thread (rr, q): row rr of each tile, columns q*16 .. q*16+15 of the strip
     double a[GT][16];
 #pragma unroll
@@ -186,20 +206,22 @@ __global__ __launch_bounds__(TB) void k_symv_tiles(const double* __restrict__ A,
         }
     }
     double beta, tau, scale;
-    householder(A, n, c, pnorm, npart, sh, beta, tau, scale);
+    householder(pn, alpha, sh, beta, tau, scale);
     if (gb == 0 && threadIdx.x == 0) {
         scal[0] = beta;
         scal[1] = tau;
         scal[2] = scale;
     }
-    for (int e = threadIdx.x; e < (GT + 1) * TS; e += TB) {
-        const int t = e / TS, x = e % TS;
-        const int r = t == 0 ? c0 + x : (I0 + t - 1) * TS + x;
-        double v = 0.0;
-        if (r < L - 1) v = A[r + (long long)n * c] * scale;
-        else if (r == L - 1) v = 1.0;
-        if (t == 0) vJ[x] = v;
-        else vI[t - 1][x] = v;
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+        const int e = threadIdx.x + u * TB;
+        if (e < (GT + 1) * TS) {
+            const int t = e / TS, x = e % TS;
+            const int r = t == 0 ? c0 + x : (I0 + t - 1) * TS + x;
+            const double v = r == L - 1 ? 1.0 : xr[u] * scale;
+            if (t == 0) vJ[x] = v;
+            else vI[t - 1][x] = v;
+        }
     }
     __syncthreads();
     double colacc[16];
@@ -230,21 +252,37 @@ __global__ __launch_bounds__(TB) void k_symv_tiles(const double* __restrict__ A,
     if (q == 0) pcol[(long long)gb * TS + rr] = cq[0][rr] + cq[1][rr] + cq[2][rr] + cq[3][rr];
 }
 
-// K_b2: rows r < L, 64 rows per block, the sixteen waves split the partial
-// sums and the panel columns.  y(r) = row sums of tiles (Ib, J >= Ib) + column
-// sums of strip Ib's groups; w = tau (y - A_p t1 - W_p t2); W(r, iw) = w;
-// pdot[block] = sum w v; stores v into A(r, c) and the unit, E, TAU.
+// K_b2: rows r < L, 64 rows per block, sixteen waves.  First the panel dots
+// t1 = W_p^T v, t2 = V_p^T v from the column update's partials (16 lanes per
+// dot) plus the unit row L-1 of v; then y(r) = row sums of tiles (Ib, J >= Ib)
+// + column sums of strip Ib's groups, w = tau (y - V_p t1 - W_p t2),
+// W(r, iw) = w, pdot[block] = sum w v; stores v into A(r, c), the unit, E, TAU.
 __global__ __launch_bounds__(TBW) void k_symv_finish(double* __restrict__ A, int n, double* __restrict__ W, int ldw,
                                                      int c, int iw, int j, int nt, const double* __restrict__ prow,
                                                      const double* __restrict__ pcol,
-                                                     const double* __restrict__ t12, const double* __restrict__ scal,
-                                                     double* __restrict__ pdot, double* __restrict__ E,
-                                                     double* __restrict__ TAU) {
+                                                     const double* __restrict__ tpart, int ldt, int ntp,
+                                                     const double* __restrict__ scal, double* __restrict__ pdot,
+                                                     double* __restrict__ E, double* __restrict__ TAU) {
     __shared__ double sh[TBW / 64];
     __shared__ double red[TBW / 64][64];
+    __shared__ double t12[128];
     constexpr int NW = TBW / 64;
     const double tau = scal[1], scale = scal[2];
     const int L = c;
+    // panel dots: thread (kk = tid / 16, s = tid % 16)
+    {
+        const int s16 = threadIdx.x & 15;
+        for (int kk = threadIdx.x >> 4; kk < 2 * j; kk += TBW / 16) {
+            double t = 0.0;
+            for (int b = s16; b < ntp; b += 16) t += tpart[(long long)kk * ldt + b];
+            for (int off = 8; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+            if (s16 == 0) {
+                const double unit = kk < j ? W[(L - 1) + (long long)ldw * (iw + 1 + kk)]
+                                           : A[(L - 1) + (long long)n * (c + 1 + (kk - j))];
+                t12[kk] = scale * t + unit;
+            }
+        }
+    }
     const int Ib = blockIdx.x, rr = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int r = Ib * TS + rr;
     double y = 0.0;
@@ -258,6 +296,9 @@ __global__ __launch_bounds__(TBW) void k_symv_finish(double* __restrict__ A, int
                 y += pcol[(long long)(g0 + e - nrow) * TS + rr];
             }
         }
+    }
+    __syncthreads();  // t12 complete
+    if (r < L) {
         for (int k = q; k < j; k += NW) {
             y -= A[r + (long long)n * (c + 1 + k)] * t12[k];
             y -= W[r + (long long)ldw * (iw + 1 + k)] * t12[j + k];
@@ -309,7 +350,7 @@ size_t sytrd_work_doubles(int n, int nb) {
     const int nt = (n + TS - 1) / TS;
     const size_t ntiles = (size_t)nt * (nt + 1) / 2;
     const size_t nblk = (n + 63) / 64;
-    return (size_t)n * nb + (ntiles + groups_before(nt)) * TS + 2 * nb + 2 * nblk + 8;
+    return (size_t)n * nb + (ntiles + groups_before(nt)) * TS + 2 * (size_t)nb * nblk + 2 * nblk + 8;
 }
 
 void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* TAU, double* work, int nb) {
@@ -320,11 +361,11 @@ void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* 
     double* W = work;
     double* prow = W + (size_t)n * nb;
     double* pcol = prow + ntiles_all * TS;
-    double* t12 = pcol + (size_t)groups_before(nt_all) * TS;
-    double* pnorm = t12 + 2 * nb;
+    double* tpart = pcol + (size_t)groups_before(nt_all) * TS;  // [2 nb][nblk_all]
+    double* pnorm = tpart + 2 * (size_t)nb * nblk_all;
     double* pdot = pnorm + nblk_all;
     double* scal = pdot + nblk_all;  // two slots of (beta, tau, scale): [0..2] and [4..6]
-    const int ldw = n;
+    const int ldw = n, ldt = nblk_all;
     int c_hi = n - 1;  // 0-based columns c_hi down to 1 carry reflectors
     while (c_hi >= 1) {
         const int nbp = std::min(nb, c_hi);
@@ -338,13 +379,13 @@ void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* 
             double* sc_cur = scal + 4 * (c & 1);
             double* sc_prev = scal + 4 * ((c + 1) & 1);
             hipLaunchKernelGGL(k_col_update, dim3(nbu), dim3(TBW), 0, s, A, n, W, ldw, c, iw, j, piw, pdot,
-                               npdot_prev, sc_prev, pnorm, D);
+                               npdot_prev, sc_prev, pnorm, D, tpart, ldt);
             const int L = c;
             const int nt = (L + TS - 1) / TS;
-            hipLaunchKernelGGL(k_symv_tiles, dim3(2 * j + groups_before(nt)), dim3(TB), 0, s, A, n, W, ldw, c, iw,
-                               j, pnorm, nbu, prow, pcol, t12, sc_cur);
+            hipLaunchKernelGGL(k_symv_tiles, dim3(groups_before(nt)), dim3(TB), 0, s, A, n, c, pnorm, nbu, prow,
+                               pcol, sc_cur);
             hipLaunchKernelGGL(k_symv_finish, dim3(nt), dim3(TBW), 0, s, A, n, W, ldw, c, iw, j, nt, prow, pcol,
-                               t12, sc_cur, pdot, E, TAU);
+                               tpart, ldt, nbu, sc_cur, pdot, E, TAU);
             QK_HIP(hipGetLastError());
             piw = iw;
             npdot_prev = nt;
