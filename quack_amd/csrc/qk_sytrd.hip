@@ -5,18 +5,18 @@
 // ones rocSOLVER's dormtr applies for the back-transformation.
 //
 // Blocked as LAPACK's dsytrd/dlatrd (Golub & Van Loan, sec. 8.3; the panel
-// recurrence of dlatrd): panels of nb columns from the right; per column
-//   K_a  finalise the previous W column, update column c with the panel,
-//        partial sums of squares of the vector to annihilate;
-//   K_b1 Householder scalars (every workgroup re-reduces the norm), symmetric
-//        matrix-vector product on the upper triangle in 64 x 64 tiles (row and
-//        column sums of each tile to a work array), panel dots t1 = W^T v,
-//        t2 = V^T v;
-//   K_b2 sums the tile partials, panel corrections, scaling by tau, partial
-//        dots w^T v (the alpha of the next W update is applied lazily by the
-//        next K_a), stores v and the scalars;
-// and after each panel the trailing update A -= V W^T + W V^T runs as two
-// matrix-core GEMMs on the upper tiles.  No host synchronisation inside.
+// recurrence of dlatrd): panels of nb columns from the right.  Two kernels per
+// column c:
+//   K_b1 Householder scalars (every workgroup re-reduces the norm partials),
+//        symmetric matrix-vector product y = A v on the upper triangle in
+//        64 x 64 tiles, grouped by column strip (row sums per tile, column
+//        sums per group), partial sums of the quadratic form v^T A v;
+//   K_b2 finishes column c (panel dots, y, the W column with its alpha term,
+//        which v^T A v gives every workgroup without a further pass) and
+//        updates column c-1 with the panel (its norm and panel-dot partials);
+// a small kernel prepares the first column of each panel, and after each
+// panel the trailing update A -= V W^T + W V^T runs as two matrix-core GEMMs
+// on the upper tiles.  No host synchronisation inside.
 #include <algorithm>
 
 #include "qk_kernels.h"
@@ -65,8 +65,8 @@ __host__ __device__ inline int groups_before(int J) {
     return GT * a * (a - 1) / 2 + a * b + J;
 }
 
-// K_a: rows r <= c, 64 rows per block, the sixteen waves split the panel
-// columns.  The previous W column piw (= iw + 1) still lacks palpha v_prev
+// K_a (first column of a panel, j = 0, piw = -1; general form kept): rows
+// r <= c, 64 rows per block, the sixteen waves split the panel columns.  The previous W column piw (= iw + 1) still lacks palpha v_prev
 // (v_prev = A(:, c+1), its unit element A(c, c+1) stored as 1): rows r < c are
 // finalised in memory; row c, read by every block, is finalised on the fly.
 // Then A(r, c) -= sum_k A(r, c+1+k) W(c, iw+1+k) + W(r, iw+1+k) A(c, c+1+k),
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(TBW) void k_col_update(double* __restrict__ A, int 
 __global__ __launch_bounds__(TB) void k_symv_tiles(const double* __restrict__ A, int n, int c,
                                                    const double* __restrict__ pnorm, int npart,
                                                    double* __restrict__ prow, double* __restrict__ pcol,
-                                                   double* __restrict__ scal) {
+                                                   double* __restrict__ qpart, double* __restrict__ scal) {
     __shared__ double sh[TB / 64];
     __shared__ double vJ[TS], vI[GT][TS];
     __shared__ double rs[GT][4][TS];
@@ -249,32 +249,72 @@ __global__ __launch_bounds__(TB) void k_symv_tiles(const double* __restrict__ A,
     for (int k = 0; k < 16; ++k) s += ct[rr][q * 16 + k];
     cq[q][rr] = s;
     __syncthreads();
-    if (q == 0) pcol[(long long)gb * TS + rr] = cq[0][rr] + cq[1][rr] + cq[2][rr] + cq[3][rr];
+    // v^T A v over the group: sum_t v_It . (A_IJ v_J) + v_J . (column sums); off-diagonal
+    // tiles are so counted twice (A_IJ and A_JI), the diagonal tile once
+    double qv = 0.0;
+    if (q < nI) qv = vI[q][rr] * (rs[q][0][rr] + rs[q][1][rr] + rs[q][2][rr] + rs[q][3][rr]);
+    if (q == 0) {
+        const double cs = cq[0][rr] + cq[1][rr] + cq[2][rr] + cq[3][rr];
+        pcol[(long long)gb * TS + rr] = cs;
+        qv += vJ[rr] * cs;
+    }
+    const double qs = wg_sum(qv, sh);
+    if (threadIdx.x == 0) qpart[gb] = qs;
 }
 
-// K_b2: rows r < L, 64 rows per block, sixteen waves.  First the panel dots
-// t1 = W_p^T v, t2 = V_p^T v from the column update's partials (16 lanes per
-// dot) plus the unit row L-1 of v; then y(r) = row sums of tiles (Ib, J >= Ib)
-// + column sums of strip Ib's groups, w = tau (y - V_p t1 - W_p t2),
-// W(r, iw) = w, pdot[block] = sum w v; stores v into A(r, c), the unit, E, TAU.
-__global__ __launch_bounds__(TBW) void k_symv_finish(double* __restrict__ A, int n, double* __restrict__ W, int ldw,
-                                                     int c, int iw, int j, int nt, const double* __restrict__ prow,
-                                                     const double* __restrict__ pcol,
-                                                     const double* __restrict__ tpart, int ldt, int ntp,
-                                                     const double* __restrict__ scal, double* __restrict__ pdot,
-                                                     double* __restrict__ E, double* __restrict__ TAU) {
+// K_b2: rows r < L, 64 rows per block, sixteen waves; finishes column c and,
+// unless c is the last column of its panel, updates column c-1.
+//  1. panel dots t1 = W_p^T v, t2 = V_p^T v from the partials of x = A(:, c)
+//     (16 lanes per dot) plus the unit row L-1 of v = x scale + e_{L-1};
+//  2. w^T v = tau (v^T A v - 2 t1 . t2) from the tile kernel's quadratic-form
+//     partials (w = tau (A v - V_p t1 - W_p t2)), so every block has
+//     palpha = -tau w^T v / 2 without another pass;
+//  3. y(r) from the tile partials, w = tau (y - V_p t1 - W_p t2), the finished
+//     W(r, iw) = w + palpha v, A(r, c) = v (the unit at L-1); row L-1 of w is
+//     computed by every block in the same order, for the update below;
+//  4. column c-1: A(r, c-1) -= sum over panel columns k = c .. c_hi of
+//     A(r, k) W(c-1, k) + W(r, k) A(c-1, k) (dlatrd's update of the next
+//     column), D(c-1), the norm partial of the next vector x' (rows < c-2)
+//     and its panel-dot partials W(:, k) . x', A(:, k) . x'.
+__global__ __launch_bounds__(TBW) void k_finish_update(
+    double* __restrict__ A, int n, double* __restrict__ W, int ldw, int c, int iw, int j, int nt, int last,
+    const double* __restrict__ prow, const double* __restrict__ pcol, const double* __restrict__ qpart, int ngroups,
+    const double* __restrict__ tcur, double* __restrict__ tnext, int ldt, int ntp, const double* __restrict__ scal,
+    double* __restrict__ pnorm, double* __restrict__ D, double* __restrict__ E, double* __restrict__ TAU) {
     __shared__ double sh[TBW / 64];
     __shared__ double red[TBW / 64][64];
+    __shared__ double red2[TBW / 64][64];
     __shared__ double t12[128];
+    __shared__ double rowL[2];  // finished W(L-1, iw), palpha
+    __shared__ double wsh[64], vsh[64], xsh[64];
     constexpr int NW = TBW / 64;
-    const double tau = scal[1], scale = scal[2];
+    constexpr int KPW = 64 / NW;
+    const double beta = scal[0], tau = scal[1], scale = scal[2];
     const int L = c;
-    // panel dots: thread (kk = tid / 16, s = tid % 16)
+    const int Ib = blockIdx.x, rr = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int r = Ib * TS + rr;
+    const bool on = r < L;
+    // loads: panel columns of this block's rows and of row L-1
+    double ar[KPW], wr[KPW], aL[KPW], wL[KPW];
+#pragma unroll
+    for (int t = 0; t < KPW; ++t) {
+        const int k = q + NW * t;
+        const bool use = k < j;
+        const long long col = (long long)n * (c + 1 + k), wcol = (long long)ldw * (iw + 1 + k);
+        ar[t] = use && on ? A[r + col] : 0.0;
+        wr[t] = use && on ? W[r + wcol] : 0.0;
+        aL[t] = use ? A[(L - 1) + col] : 0.0;
+        wL[t] = use ? W[(L - 1) + wcol] : 0.0;
+    }
+    // A(r, c) and A(r, c-1) are rewritten below by wave 0 only, which alone reads them
+    const double xr = q == 0 && on && r < L - 1 ? A[r + (long long)n * c] : 0.0;
+    const double anext = q == 0 && !last && on ? A[r + (long long)n * (c - 1)] : 0.0;
+    // 1. panel dots
     {
         const int s16 = threadIdx.x & 15;
         for (int kk = threadIdx.x >> 4; kk < 2 * j; kk += TBW / 16) {
             double t = 0.0;
-            for (int b = s16; b < ntp; b += 16) t += tpart[(long long)kk * ldt + b];
+            for (int b = s16; b < ntp; b += 16) t += tcur[(long long)kk * ldt + b];
             for (int off = 8; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
             if (s16 == 0) {
                 const double unit = kk < j ? W[(L - 1) + (long long)ldw * (iw + 1 + kk)]
@@ -283,10 +323,11 @@ __global__ __launch_bounds__(TBW) void k_symv_finish(double* __restrict__ A, int
             }
         }
     }
-    const int Ib = blockIdx.x, rr = threadIdx.x & 63, q = threadIdx.x >> 6;
-    const int r = Ib * TS + rr;
+    // 2. quadratic form (every thread's share), 3a. y partial sums of this row
+    double qf = 0.0;
+    for (int g = threadIdx.x; g < ngroups; g += TBW) qf += qpart[g];
     double y = 0.0;
-    if (r < L) {
+    if (on) {
         const int nrow = nt - Ib, ng = Ib / GT + 1, g0 = groups_before(Ib);
         for (int e = q; e < nrow + ng; e += NW) {
             if (e < nrow) {
@@ -297,49 +338,114 @@ __global__ __launch_bounds__(TBW) void k_symv_finish(double* __restrict__ A, int
             }
         }
     }
-    __syncthreads();  // t12 complete
-    if (r < L) {
-        for (int k = q; k < j; k += NW) {
-            y -= A[r + (long long)n * (c + 1 + k)] * t12[k];
-            y -= W[r + (long long)ldw * (iw + 1 + k)] * t12[j + k];
+    const double vAv = wg_sum(qf, sh);  // its barriers also publish t12
+    // row L-1: wave 0, the same sum order in every block
+    if (q == 0) {
+        const int IbL = (L - 1) / TS, rL = (L - 1) % TS;
+        const int nrow = nt - IbL, ng = IbL / GT + 1, g0 = groups_before(IbL);
+        double yl = 0.0;
+        for (int e = rr; e < nrow + ng; e += 64) {
+            if (e < nrow) {
+                const int J = IbL + e;
+                yl += prow[(long long)(J * (J + 1) / 2 + IbL) * TS + rL];
+            } else {
+                yl += pcol[(long long)(g0 + e - nrow) * TS + rL];
+            }
+        }
+        for (int k = rr; k < j; k += 64) {
+            yl -= A[(L - 1) + (long long)n * (c + 1 + k)] * t12[k];
+            yl -= W[(L - 1) + (long long)ldw * (iw + 1 + k)] * t12[j + k];
+        }
+        for (int off = 32; off > 0; off >>= 1) yl += __shfl_xor(yl, off, 64);
+        if (rr == 0) {
+            double tt = 0.0;
+            for (int k = 0; k < j; ++k) tt += t12[k] * t12[j + k];
+            const double palpha = -0.5 * tau * (tau * (vAv - 2.0 * tt));
+            rowL[0] = tau * yl + palpha;  // v(L-1) = 1
+            rowL[1] = palpha;
+        }
+    }
+    // 3b. panel corrections; column update partials of c-1 (k >= 1 columns)
+    double u = 0.0;
+#pragma unroll
+    for (int t = 0; t < KPW; ++t) {
+        const int k = q + NW * t;
+        if (k < j) {
+            y -= ar[t] * t12[k] + wr[t] * t12[j + k];
+            u += ar[t] * wL[t] + wr[t] * aL[t];
         }
     }
     red[q][rr] = y;
+    red2[q][rr] = u;
     __syncthreads();
-    double d = 0.0;
-    if (q == 0 && r < L) {
-        double s = 0.0;
+    const double wLfin = rowL[0], palpha = rowL[1];
+    double sq = 0.0;
+    if (q == 0) {
+        wsh[rr] = 0.0;
+        vsh[rr] = 0.0;
+        xsh[rr] = 0.0;
+        if (on) {
+            double ys = 0.0, us = 0.0;
 #pragma unroll
-        for (int k = 0; k < NW; ++k) s += red[k][rr];
-        const double w = tau * s;
-        W[r + (long long)ldw * iw] = w;
-        double v;
-        if (r == L - 1) {
-            v = 1.0;
-        } else {
-            v = A[r + (long long)n * c] * scale;
+            for (int k = 0; k < NW; ++k) {
+                ys += red[k][rr];
+                us += red2[k][rr];
+            }
+            const double v = r == L - 1 ? 1.0 : xr * scale;
+            const double wf = r == L - 1 ? wLfin : tau * ys + palpha * v;
+            W[r + (long long)ldw * iw] = wf;
             A[r + (long long)n * c] = v;
+            wsh[rr] = wf;
+            vsh[rr] = v;
+            if (!last) {
+                // column c-1, rows r <= c-1: the k = 0 panel column is c itself (A(c-1, c) = 1)
+                const double a = anext - (us + v * wLfin + wf);
+                A[r + (long long)n * (c - 1)] = a;
+                if (r == L - 1) D[c - 1] = a;
+                if (r < L - 2) {
+                    sq = a * a;
+                    xsh[rr] = a;
+                }
+            }
         }
-        d = w * v;
     }
-    const double t = wg_sum(d, sh);
-    if (threadIdx.x == 0) pdot[blockIdx.x] = t;
+    if (!last) {
+        const double t = wg_sum(sq, sh);  // its barriers also publish wsh, vsh, xsh
+        if (threadIdx.x == 0) pnorm[blockIdx.x] = t;
+        // panel-dot partials of x' for column c-1: panel columns k = 0 (c itself) .. j
+        const double x = xsh[rr];
+        const int jn = j + 1;
+#pragma unroll
+        for (int t = 0; t <= KPW; ++t) {
+            // wave q handles k = 0 (wave 0 only, t = 0) and k = 1 + q + NW (t - 1)
+            int k;
+            double wv, av;
+            if (t == 0) {
+                if (q != 0) continue;
+                k = 0;
+                wv = wsh[rr];
+                av = vsh[rr];
+            } else {
+                k = 1 + q + NW * (t - 1);
+                if (k > j) break;
+                wv = wr[t - 1];
+                av = ar[t - 1];
+            }
+            double pw = wv * x, pa = av * x;
+            for (int off = 32; off > 0; off >>= 1) {
+                pw += __shfl_down(pw, off, 64);
+                pa += __shfl_down(pa, off, 64);
+            }
+            if (rr == 0) {
+                tnext[(long long)k * ldt + blockIdx.x] = pw;
+                tnext[(long long)(jn + k) * ldt + blockIdx.x] = pa;
+            }
+        }
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        A[(c - 1) + (long long)n * c] = 1.0;
-        E[c - 1] = scal[0];
+        E[c - 1] = beta;
         TAU[c - 1] = tau;
     }
-}
-
-// W(r, piw) += palpha A(r, c+1) for r <= c (the last column of a panel)
-__global__ void k_w_finish(const double* __restrict__ A, int n, double* __restrict__ W, int ldw, int c, int piw,
-                           const double* __restrict__ pdot, int npdot, const double* __restrict__ scal_prev) {
-    __shared__ double sh[TB / 64];
-    double s = 0.0;
-    for (int k = threadIdx.x; k < npdot; k += blockDim.x) s += pdot[k];
-    const double palpha = -0.5 * scal_prev[1] * wg_sum(s, sh);
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r <= c) W[r + (long long)ldw * piw] += palpha * A[r + (long long)n * (c + 1)];
 }
 
 __global__ void k_d0(const double* __restrict__ A, double* __restrict__ D) { D[0] = A[0]; }
@@ -349,52 +455,52 @@ __global__ void k_d0(const double* __restrict__ A, double* __restrict__ D) { D[0
 size_t sytrd_work_doubles(int n, int nb) {
     const int nt = (n + TS - 1) / TS;
     const size_t ntiles = (size_t)nt * (nt + 1) / 2;
+    const size_t ng = groups_before(nt);
     const size_t nblk = (n + 63) / 64;
-    return (size_t)n * nb + (ntiles + groups_before(nt)) * TS + 2 * (size_t)nb * nblk + 2 * nblk + 8;
+    return (size_t)n * nb + (ntiles + ng) * TS + ng + 2 * (2 * (size_t)nb * nblk) + nblk + 8;
 }
 
 void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* TAU, double* work, int nb) {
     QK_CHECK(n >= 1 && nb >= 1 && nb <= 64, QK_E_ARG, "sytrd: bad size");
     const int nt_all = (n + TS - 1) / TS;
     const size_t ntiles_all = (size_t)nt_all * (nt_all + 1) / 2;
+    const size_t ng_all = groups_before(nt_all);
     const int nblk_all = (n + 63) / 64;
     double* W = work;
     double* prow = W + (size_t)n * nb;
     double* pcol = prow + ntiles_all * TS;
-    double* tpart = pcol + (size_t)groups_before(nt_all) * TS;  // [2 nb][nblk_all]
-    double* pnorm = tpart + 2 * (size_t)nb * nblk_all;
-    double* pdot = pnorm + nblk_all;
-    double* scal = pdot + nblk_all;  // two slots of (beta, tau, scale): [0..2] and [4..6]
+    double* qpart = pcol + ng_all * TS;
+    double* tpart0 = qpart + ng_all;  // two buffers of [2 nb][nblk_all], by column parity
+    double* tpart1 = tpart0 + 2 * (size_t)nb * nblk_all;
+    double* pnorm = tpart1 + 2 * (size_t)nb * nblk_all;
+    double* scal = pnorm + nblk_all;  // two slots of (beta, tau, scale): [0..2] and [4..6]
     const int ldw = n, ldt = nblk_all;
     int c_hi = n - 1;  // 0-based columns c_hi down to 1 carry reflectors
     while (c_hi >= 1) {
         const int nbp = std::min(nb, c_hi);
         const int c_lo = c_hi - nbp + 1;
-        int piw = -1, npdot_prev = 0;
+        // first column of the panel: no panel terms; its norm partials and diagonal
+        hipLaunchKernelGGL(k_col_update, dim3((c_hi + 1 + 63) / 64), dim3(TBW), 0, s, A, n, W, ldw, c_hi, 0, 0, -1,
+                           (const double*)nullptr, 0, (const double*)nullptr, pnorm, D, tpart0, ldt);
         for (int c = c_hi; c >= c_lo; --c) {
             const int iw = c - c_lo;
             const int j = c_hi - c;  // finished panel columns to the right
-            const int nbu = (c + 1 + 63) / 64;  // 64 rows per block
-            // scalars ping-pong between scal[0..2] and scal[4..6] by column parity
+            const int npart = (c + 1 + 63) / 64;  // partial blocks over rows 0..c
             double* sc_cur = scal + 4 * (c & 1);
-            double* sc_prev = scal + 4 * ((c + 1) & 1);
-            hipLaunchKernelGGL(k_col_update, dim3(nbu), dim3(TBW), 0, s, A, n, W, ldw, c, iw, j, piw, pdot,
-                               npdot_prev, sc_prev, pnorm, D, tpart, ldt);
+            double* tcur = (c & 1) ? tpart1 : tpart0;
+            double* tnext = (c & 1) ? tpart0 : tpart1;
             const int L = c;
             const int nt = (L + TS - 1) / TS;
-            hipLaunchKernelGGL(k_symv_tiles, dim3(groups_before(nt)), dim3(TB), 0, s, A, n, c, pnorm, nbu, prow,
-                               pcol, sc_cur);
-            hipLaunchKernelGGL(k_symv_finish, dim3(nt), dim3(TBW), 0, s, A, n, W, ldw, c, iw, j, nt, prow, pcol,
-                               tpart, ldt, nbu, sc_cur, pdot, E, TAU);
+            const int ng = groups_before(nt);
+            hipLaunchKernelGGL(k_symv_tiles, dim3(ng), dim3(TB), 0, s, A, n, c, pnorm, npart, prow, pcol, qpart,
+                               sc_cur);
+            hipLaunchKernelGGL(k_finish_update, dim3(nt), dim3(TBW), 0, s, A, n, W, ldw, c, iw, j, nt,
+                               c == c_lo ? 1 : 0, prow, pcol, qpart, ng, tcur, tnext, ldt, npart, sc_cur, pnorm, D, E,
+                               TAU);
             QK_HIP(hipGetLastError());
-            piw = iw;
-            npdot_prev = nt;
         }
-        // finalise the panel's last W column (that of c_lo), then the trailing update
+        // trailing update A -= V W^T + W V^T on the upper tiles
         const int m = c_lo;  // rows/cols 0..c_lo-1 remain
-        hipLaunchKernelGGL(k_w_finish, dim3((c_lo + TB - 1) / TB), dim3(TB), 0, s, A, n, W, ldw, c_lo - 1, piw, pdot,
-                           npdot_prev, scal + 4 * (c_lo & 1));
-        QK_HIP(hipGetLastError());
         if (m > 0) {
             Batch up;
             up.upper = 1;
