@@ -29,15 +29,25 @@ constexpr int TBW = 1024;  // threads per block (row-wise kernels: 64 rows x 16 
 constexpr int TS = 64;     // symv tile
 constexpr int GT = 4;      // tiles per symv workgroup (one column strip)
 
+// Workgroup barrier that orders LDS only: no kernel here exchanges global
+// memory between its waves, so outstanding global loads and stores (the tile
+// loads in flight, the column stores) need not drain at every barrier, as a
+// __syncthreads() fence would make them.
+__device__ inline void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ inline double wg_sum(double v, double* sh) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    __syncthreads();
+    lds_sync();
     if (lane == 0) sh[w] = v;
-    __syncthreads();
+    lds_sync();
     double r = 0.0;
     for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) r += sh[k];
-    __syncthreads();
+    lds_sync();
     return r;  // valid in every thread
 }
 
@@ -114,7 +124,7 @@ __global__ __launch_bounds__(TBW) void k_col_update(double* __restrict__ A, int 
         acc += ar[t] * wck + wrk * ac[t];
     }
     red[q][rr] = acc;
-    __syncthreads();
+    lds_sync();
     double sq = 0.0;
     if (q == 0) xs[rr] = 0.0;
     if (q == 0 && on) {
@@ -223,7 +233,7 @@ __global__ __launch_bounds__(TB) void k_symv_tiles(const double* __restrict__ A,
             else vI[t - 1][x] = v;
         }
     }
-    __syncthreads();
+    lds_sync();
     double colacc[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) colacc[k] = 0.0;
@@ -242,13 +252,13 @@ __global__ __launch_bounds__(TB) void k_symv_tiles(const double* __restrict__ A,
     }
 #pragma unroll
     for (int k = 0; k < 16; ++k) ct[q * 16 + k][rr] = colacc[k];
-    __syncthreads();
+    lds_sync();
     if (q < nI) prow[(long long)(J * (J + 1) / 2 + I0 + q) * TS + rr] = rs[q][0][rr] + rs[q][1][rr] + rs[q][2][rr] + rs[q][3][rr];
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < 16; ++k) s += ct[rr][q * 16 + k];
     cq[q][rr] = s;
-    __syncthreads();
+    lds_sync();
     // v^T A v over the group: sum_t v_It . (A_IJ v_J) + v_J . (column sums); off-diagonal
     // tiles are so counted twice (A_IJ and A_JI), the diagonal tile once
     double qv = 0.0;
@@ -285,7 +295,7 @@ __global__ __launch_bounds__(TBW) void k_finish_update(
     __shared__ double red[TBW / 64][64];
     __shared__ double red2[TBW / 64][64];
     __shared__ double t12[128];
-    __shared__ double rowL[2];  // finished W(L-1, iw), palpha
+    __shared__ double ylp[TBW / 64];  // per-wave panel corrections of row L-1
     __shared__ double wsh[64], vsh[64], xsh[64];
     constexpr int NW = TBW / 64;
     constexpr int KPW = 64 / NW;
@@ -294,7 +304,7 @@ __global__ __launch_bounds__(TBW) void k_finish_update(
     const int Ib = blockIdx.x, rr = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int r = Ib * TS + rr;
     const bool on = r < L;
-    // loads: panel columns of this block's rows and of row L-1
+    // every load first.  Panel columns of this block's rows and of row L-1:
     double ar[KPW], wr[KPW], aL[KPW], wL[KPW];
 #pragma unroll
     for (int t = 0; t < KPW; ++t) {
@@ -309,23 +319,23 @@ __global__ __launch_bounds__(TBW) void k_finish_update(
     // A(r, c) and A(r, c-1) are rewritten below by wave 0 only, which alone reads them
     const double xr = q == 0 && on && r < L - 1 ? A[r + (long long)n * c] : 0.0;
     const double anext = q == 0 && !last && on ? A[r + (long long)n * (c - 1)] : 0.0;
-    // 1. panel dots
-    {
-        const int s16 = threadIdx.x & 15;
-        for (int kk = threadIdx.x >> 4; kk < 2 * j; kk += TBW / 16) {
-            double t = 0.0;
-            for (int b = s16; b < ntp; b += 16) t += tcur[(long long)kk * ldt + b];
-            for (int off = 8; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-            if (s16 == 0) {
-                const double unit = kk < j ? W[(L - 1) + (long long)ldw * (iw + 1 + kk)]
-                                           : A[(L - 1) + (long long)n * (c + 1 + (kk - j))];
-                t12[kk] = scale * t + unit;
-            }
+    // panel-dot partials of x = A(:, c) (thread (kk, s16): 16 lanes per dot) and their unit-row terms
+    const int s16 = threadIdx.x & 15, kk0 = threadIdx.x >> 4;
+    double tp[2] = {0.0, 0.0}, tu[2] = {0.0, 0.0};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int kk = kk0 + (TBW / 16) * h;
+        if (kk < 2 * j) {
+            for (int b = s16; b < ntp; b += 16) tp[h] += tcur[(long long)kk * ldt + b];
+            if (s16 == 0)
+                tu[h] = kk < j ? W[(L - 1) + (long long)ldw * (iw + 1 + kk)]
+                               : A[(L - 1) + (long long)n * (c + 1 + (kk - j))];
         }
     }
-    // 2. quadratic form (every thread's share), 3a. y partial sums of this row
+    // quadratic-form partials (this thread's share)
     double qf = 0.0;
     for (int g = threadIdx.x; g < ngroups; g += TBW) qf += qpart[g];
+    // tile partials of y for this row
     double y = 0.0;
     if (on) {
         const int nrow = nt - Ib, ng = Ib / GT + 1, g0 = groups_before(Ib);
@@ -338,12 +348,11 @@ __global__ __launch_bounds__(TBW) void k_finish_update(
             }
         }
     }
-    const double vAv = wg_sum(qf, sh);  // its barriers also publish t12
-    // row L-1: wave 0, the same sum order in every block
+    // tile partials of y for row L-1 (wave 0, one per lane; the same order in every block)
+    double yl = 0.0;
     if (q == 0) {
         const int IbL = (L - 1) / TS, rL = (L - 1) % TS;
         const int nrow = nt - IbL, ng = IbL / GT + 1, g0 = groups_before(IbL);
-        double yl = 0.0;
         for (int e = rr; e < nrow + ng; e += 64) {
             if (e < nrow) {
                 const int J = IbL + e;
@@ -352,35 +361,43 @@ __global__ __launch_bounds__(TBW) void k_finish_update(
                 yl += pcol[(long long)(g0 + e - nrow) * TS + rL];
             }
         }
-        for (int k = rr; k < j; k += 64) {
-            yl -= A[(L - 1) + (long long)n * (c + 1 + k)] * t12[k];
-            yl -= W[(L - 1) + (long long)ldw * (iw + 1 + k)] * t12[j + k];
-        }
         for (int off = 32; off > 0; off >>= 1) yl += __shfl_xor(yl, off, 64);
-        if (rr == 0) {
-            double tt = 0.0;
-            for (int k = 0; k < j; ++k) tt += t12[k] * t12[j + k];
-            const double palpha = -0.5 * tau * (tau * (vAv - 2.0 * tt));
-            rowL[0] = tau * yl + palpha;  // v(L-1) = 1
-            rowL[1] = palpha;
-        }
     }
-    // 3b. panel corrections; column update partials of c-1 (k >= 1 columns)
-    double u = 0.0;
+    // panel dots t1 = W_p^T v, t2 = V_p^T v (v = x scale + e_{L-1})
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        double t = tp[h];
+        for (int off = 8; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        const int kk = kk0 + (TBW / 16) * h;
+        if (s16 == 0 && kk < 2 * j) t12[kk] = scale * t + tu[h];
+    }
+    const double vAv = wg_sum(qf, sh);  // its barriers also publish t12
+    // panel corrections of y; update partials of column c-1 (panel columns k >= 1);
+    // the corrections of row L-1, the same in every lane of a wave
+    double u = 0.0, ylc = 0.0;
 #pragma unroll
     for (int t = 0; t < KPW; ++t) {
         const int k = q + NW * t;
         if (k < j) {
             y -= ar[t] * t12[k] + wr[t] * t12[j + k];
             u += ar[t] * wL[t] + wr[t] * aL[t];
+            ylc += aL[t] * t12[k] + wL[t] * t12[j + k];
         }
     }
     red[q][rr] = y;
     red2[q][rr] = u;
-    __syncthreads();
-    const double wLfin = rowL[0], palpha = rowL[1];
+    if (rr == 0) ylp[q] = ylc;
+    lds_sync();
     double sq = 0.0;
     if (q == 0) {
+        // row L-1 of w and alpha = -tau w^T v / 2, w^T v = tau (v^T A v - 2 t1 . t2)
+        double corr = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) corr += ylp[k];
+        double tt = 0.0;
+        for (int k = 0; k < j; ++k) tt += t12[k] * t12[j + k];
+        const double palpha = -0.5 * tau * (tau * (vAv - 2.0 * tt));
+        const double wLfin = tau * (yl - corr) + palpha;  // v(L-1) = 1
         wsh[rr] = 0.0;
         vsh[rr] = 0.0;
         xsh[rr] = 0.0;
