@@ -231,6 +231,7 @@ def main():
     # first call of every GEMM shape in this form: the tile autotuner's trials
     step_on(eng)
     # one molecule alone: per-stage device times and the dominant kernel's solo launch time
+    st_pre = eng.stage_times()
     t = time.time()
     r1 = step_on(eng)
     t_single = time.time() - t
@@ -260,7 +261,7 @@ def main():
     dE = np.max([np.abs(x["eGW"] - ref["eGW"]) for x in rs], axis=0)
     window = np.abs(scf["eHF"] - 0.5 * (scf["eHF"][nO - 1] + scf["eHF"][nO])) < 1.0
 
-    # dominant hand-written kernel: the excitation-density GEMM rho^T = W^T M^T,
+    # the excitation-density GEMM rho^T = W^T M^T (the largest matrix-core kernel),
     # one launch per screening cycle, timed with HIP events on each engine's own
     # stream over the timed region (launches of concurrent molecules overlap, so
     # each takes longer than alone; the solo figure is reported beside it).
@@ -287,6 +288,34 @@ def main():
         flops, peak, kname = 12.0 * nOrb * nOrb * nS * nS, PEAK_BF16, "k_gemm_bf16x3 (excitation densities, 6 BF16 slice products)"
     achieved = flops / (gemm_ms * 1e-3) / 1e12
     traffic = read_traffic(args.form)
+    gemm_roof = {"kernel": kname, "bound": "mfma",
+                 "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                 "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": gemm_ms,
+                 "launches_timed": int(gemm_n), "concurrent_molecules": K,
+                 "all_launches": {"n": int(all_n), "avg_launch_ms": all_t / max(1.0, all_n)},
+                 "solo": {"avg_launch_ms": solo_ms, "achieved": flops / (solo_ms * 1e-3) / 1e12,
+                          "frac": flops / (solo_ms * 1e-3) / 1e12 / peak}}
+    # the own Householder tridiagonalisation (two kernels per column plus the panel
+    # GEMMs; by device time the largest consumer of the mixed form): one "launch" is
+    # one tridiagonalisation of the nS x nS RPA matrix, timed with HIP events on each
+    # engine's stream.  Algorithmic bytes: the symmetric product of every column reads
+    # the trailing upper triangle once, 8 sum_{L<nS} L (L+1) / 2 bytes per launch.
+    trd_bytes = 8.0 * sum(L * (L + 1) / 2.0 for L in range(1, nS))
+    trd_t = sum(b["tridiagonalisation_all_calls"] - a["tridiagonalisation_all_calls"] for a, b in zip(c0, c1))
+    trd_n = sum(b["tridiagonalisations_all_calls"] - a["tridiagonalisations_all_calls"] for a, b in zip(c0, c1))
+    trd_solo_n = st1["tridiagonalisations_all_calls"] - st_pre["tridiagonalisations_all_calls"]
+    trd_roof = None
+    if trd_n > 0:
+        trd_ms = trd_t / trd_n
+        trd_gbs = trd_bytes / (trd_ms * 1e-3) / 1e9
+        trd_roof = {"kernel": "own tridiagonalisation (k_symv_tiles + k_finish_update per column, panel GEMMs)",
+                    "bound": "hbm", "achieved": trd_gbs, "peak": PEAK_HBM, "unit": "GB/s", "frac": trd_gbs / PEAK_HBM,
+                    "traffic": read_trd_traffic(), "bytes_per_launch": trd_bytes, "avg_launch_ms": trd_ms,
+                    "launches_timed": int(trd_n), "concurrent_molecules": K}
+        if trd_solo_n > 0:
+            solo_trd = (st1["tridiagonalisation_all_calls"] - st_pre["tridiagonalisation_all_calls"]) / trd_solo_n
+            trd_roof["solo"] = {"avg_launch_ms": solo_trd, "achieved": trd_bytes / (solo_trd * 1e-3) / 1e9,
+                                "frac": trd_bytes / (solo_trd * 1e-3) / 1e9 / PEAK_HBM}
 
     cpu = None
     if args.cpu_baseline == "auto":
@@ -326,13 +355,8 @@ def main():
                              "max_abs_dE_meV_within_1Ha_of_gap": float(dE[window].max() / mev),
                              "HOMO_dE_meV": float(dE[nO - 1] / mev), "LUMO_dE_meV": float(dE[nO] / mev)},
         "learned_form": learned,
-        "roofline": {"kernel": kname, "bound": "mfma",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": gemm_ms,
-                     "launches_timed": int(gemm_n), "concurrent_molecules": K,
-                     "all_launches": {"n": int(all_n), "avg_launch_ms": all_t / max(1.0, all_n)},
-                     "solo": {"avg_launch_ms": solo_ms, "achieved": flops / (solo_ms * 1e-3) / 1e12,
-                              "frac": flops / (solo_ms * 1e-3) / 1e12 / peak}},
+        "roofline": trd_roof if trd_roof is not None else gemm_roof,
+        "roofline_excitation_gemm": gemm_roof,
         "cpu_baseline": cpu,
         "result": {"HOMO": float(r["eGW"][nO - 1]), "LUMO": float(r["eGW"][nO]), "EcRPA": r["EcRPA"],
                    "nSCF": r.get("nSCF")},
@@ -454,6 +478,21 @@ def run_batch(args):
            "accuracy_vs_fp64_per_molecule": {"max_abs_dE_meV": float(d.max() / mev),
                                              "molecules_over_1meV": int((d > mev).sum())}}
     print(json.dumps(out), flush=True)
+
+
+def read_trd_traffic():
+    """HBM bytes per tridiagonalisation (nS = 5600) from the committed rocprofv3
+    PMC summary of its kernels (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+    f = os.path.join(ROOT, "profiles", "pmc_tridiagonalisation_h50_lattice.md")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("| total per tridiagonalisation |"):
+            try:
+                return float(line.split("|")[-2])
+            except ValueError:
+                return None
+    return None
 
 
 def read_traffic(form):

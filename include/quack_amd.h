@@ -129,8 +129,10 @@ int qk_gw_eig_stats(qk_gw_handle h, long long *counts, int n);
  * [4] self-energy + QP solve, [5] total of the GW call (host clock),
  * [6] the excitation-density GEMM launches alone (the product that writes rho),
  * [7] their number (tuning trials of the autotuner included);
- * [8], [9] the same two summed over every call since the handle was created.
- * n <= 10. */
+ * [8], [9] the same two summed over every call since the handle was created;
+ * [10] device time of the own Householder tridiagonalisations (eigensolver
+ * modes 2 and 3) since the handle was created, [11] their number.
+ * n <= 12. */
 int qk_gw_stage_times(qk_gw_handle h, double *ms, int n);
 
 /* ---- per-routine entry points (host in, host out) ---------------------- */

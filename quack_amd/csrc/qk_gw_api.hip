@@ -67,8 +67,10 @@ void syevd(hipStream_t s, rocblas_handle rb, bool vectors, int n, double* A, dou
 // (dsterf for eigenvalues only), rocSOLVER's dormtr for the back-transformation
 // with the reflectors.  Same outputs as syevd(): w ascending, A = eigenvectors.
 void syevd_own(hipStream_t s, rocblas_handle rb, bool vectors, int n, double*& A, double* w, double* E, double* tau,
-               double*& Z, double* work, int* dinfo) {
+               double*& Z, double* work, int* dinfo, hipEvent_t* tev = nullptr) {
+    if (tev) QK_HIP(hipEventRecord(tev[0], s));
     sytrd_upper(s, n, A, w, E, tau, work, 32);
+    if (tev) QK_HIP(hipEventRecord(tev[1], s));
     if (vectors) {
         rb_check(rocsolver_dstedc(rb, rocblas_evect_tridiagonal, n, w, E, Z, n, dinfo), "rocsolver_dstedc");
         rb_check(rocsolver_dormtr(rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, n, n, A, n, tau,
@@ -217,6 +219,8 @@ struct qk_gw_ctx {
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t ev2[2] = {nullptr, nullptr};
     double t_gemm_total = 0.0, n_gemm_total = 0.0;  // excitation GEMM ms and launches since creation
+    hipEvent_t ev3[2] = {nullptr, nullptr};
+    double t_trd_total = 0.0, n_trd_total = 0.0;    // own tridiagonalisations: ms and count since creation
     bool shared = false;  // stream, rocBLAS handle and the batch slots belong to a qk_gwb handle
     std::string err;
 };
@@ -599,7 +603,13 @@ void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors)
         }
         if (!done) {
             if (c->eig_mode & 2) {
-                syevd_own(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->tau, c->Zev, c->trd_work, c->info);
+                syevd_own(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->tau, c->Zev, c->trd_work, c->info,
+                          c->ev3);
+                float ms = 0.f;
+                QK_HIP(hipEventSynchronize(c->ev3[1]));
+                QK_HIP(hipEventElapsedTime(&ms, c->ev3[0], c->ev3[1]));
+                c->t_trd_total += ms;
+                c->n_trd_total += 1.0;
                 ++c->eig_count[8];
             } else {
                 syevd(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->info);
@@ -871,6 +881,8 @@ void release(Ctx* c) {
         if (e) (void)hipEventDestroy(e);
     for (auto& e : c->ev2)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->ev3)
+        if (e) (void)hipEventDestroy(e);
     if (!c->shared) {
         if (c->rb) (void)rocblas_destroy_handle(c->rb);
         if (c->s) (void)hipStreamDestroy(c->s);
@@ -900,7 +912,7 @@ int qk_gw_create(qk_gw_handle* h, int nBas, int nOrb, int nC, int nO, int nV, in
         QK_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
         rb_check(rocblas_create_handle(&c->rb), "rocblas_create_handle");
         rb_check(rocblas_set_stream(c->rb, c->s), "rocblas_set_stream");
-        for (auto* e : {&c->ev[0], &c->ev[1], &c->ev2[0], &c->ev2[1]}) QK_HIP(hipEventCreate(e));
+        for (auto* e : {&c->ev[0], &c->ev[1], &c->ev2[0], &c->ev2[1], &c->ev3[0], &c->ev3[1]}) QK_HIP(hipEventCreate(e));
         size_t n = nOrb, S = nS;
         c->K = dalloc<double>(S * S);
         c->Cm = dalloc<double>(S * S);
@@ -1120,9 +1132,9 @@ int qk_gw_eig_stats(qk_gw_handle c, long long* counts, int n) {
 }
 
 int qk_gw_stage_times(qk_gw_handle c, double* ms, int n) {
-    if (!c || !ms || n < 0 || n > 10) return QK_E_ARG;
-    double all[10] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4],
-                      c->t_stage[5], c->t_stage[6], c->t_gemm_total, c->n_gemm_total};
+    if (!c || !ms || n < 0 || n > 12) return QK_E_ARG;
+    double all[12] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4],
+                      c->t_stage[5], c->t_stage[6], c->t_gemm_total, c->n_gemm_total, c->t_trd_total, c->n_trd_total};
     for (int i = 0; i < n; ++i) ms[i] = all[i];
     return QK_OK;
 }
