@@ -64,18 +64,16 @@ void syevd(hipStream_t s, rocblas_handle rb, bool vectors, int n, double* A, dou
 
 // Full eigen-decomposition without dsyevd: own Householder tridiagonalisation
 // (qk_sytrd.hip), rocSOLVER's divide-and-conquer dstedc on the tridiagonal
-// (dsterf for eigenvalues only), rocSOLVER's dormtr for the back-transformation
-// with the reflectors.  Same outputs as syevd(): w ascending, A = eigenvectors.
+// (dsterf for eigenvalues only), the own blocked back-transformation with the
+// reflectors.  Same outputs as syevd(): w ascending, A = eigenvectors.
 void syevd_own(hipStream_t s, rocblas_handle rb, bool vectors, int n, double*& A, double* w, double* E, double* tau,
-               double*& Z, double* work, int* dinfo, hipEvent_t* tev = nullptr) {
+               double*& Z, double* work, double* owork, int* dinfo, hipEvent_t* tev = nullptr) {
     if (tev) QK_HIP(hipEventRecord(tev[0], s));
     sytrd_upper(s, n, A, w, E, tau, work, 32);
     if (tev) QK_HIP(hipEventRecord(tev[1], s));
     if (vectors) {
         rb_check(rocsolver_dstedc(rb, rocblas_evect_tridiagonal, n, w, E, Z, n, dinfo), "rocsolver_dstedc");
-        rb_check(rocsolver_dormtr(rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none, n, n, A, n, tau,
-                                  Z, n),
-                 "rocsolver_dormtr");
+        ormtr_upper(s, n, A, tau, Z, owork);
         std::swap(A, Z);
     } else {
         rb_check(rocsolver_dsterf(rb, n, w, E, dinfo), "rocsolver_dsterf");
@@ -201,7 +199,7 @@ struct qk_gw_ctx {
     // most eig_demax since they were computed, dsyevd otherwise or on failure
     int eig_mode = 0;  // bit 0: refine; bit 1: own tridiagonalisation instead of dsyevd
     double eig_demax = 5e-3;
-    double *Zev = nullptr, *tau = nullptr, *trd_work = nullptr;  // own tridiagonalisation path
+    double *Zev = nullptr, *tau = nullptr, *trd_work = nullptr, *orm_work = nullptr;  // own full-solve path
     double *Xv = nullptr, *Ym = nullptr, *Sm = nullptr, *Gm = nullptr, *lam = nullptr, *eig_dbuf = nullptr;
     int *perm = nullptr, *eig_ibuf = nullptr;
     long long* eig_lbuf = nullptr;
@@ -603,8 +601,8 @@ void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors)
         }
         if (!done) {
             if (c->eig_mode & 2) {
-                syevd_own(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->tau, c->Zev, c->trd_work, c->info,
-                          c->ev3);
+                syevd_own(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->tau, c->Zev, c->trd_work,
+                          c->orm_work, c->info, c->ev3);
                 float ms = 0.f;
                 QK_HIP(hipEventSynchronize(c->ev3[1]));
                 QK_HIP(hipEventElapsedTime(&ms, c->ev3[0], c->ev3[1]));
@@ -865,7 +863,7 @@ void release(Ctx* c) {
     int* is[] = {c->nit, c->done, c->info, c->active};
     for (int* p : is)
         if (p) (void)hipFree(p);
-    for (double* p : {c->Xv, c->Ym, c->Sm, c->Gm, c->lam, c->eig_dbuf, c->Zev, c->tau, c->trd_work})
+    for (double* p : {c->Xv, c->Ym, c->Sm, c->Gm, c->lam, c->eig_dbuf, c->Zev, c->tau, c->trd_work, c->orm_work})
         if (p) (void)hipFree(p);
     for (int* p : {c->perm, c->eig_ibuf})
         if (p) (void)hipFree(p);
@@ -1106,6 +1104,7 @@ int qk_gw_set_eigensolver(qk_gw_handle c, int mode, double de_max) {
             c->Zev = dalloc<double>(S * S);
             c->tau = dalloc<double>(S);
             c->trd_work = dalloc<double>(sytrd_work_doubles(c->nS, 32));
+            c->orm_work = dalloc<double>(ormtr_work_doubles(c->nS));
         }
         if ((mode & 1) && !c->Xv) {
             size_t S = c->nS;

@@ -58,6 +58,10 @@ void gather_bov(hipStream_t s, const double* B, double* Bov, int naux, int n, in
 // work holds sytrd_work_doubles(n, nb) doubles, nb <= 64.
 size_t sytrd_work_doubles(int n, int nb);
 void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* TAU, double* work, int nb);
+// Z <- Q Z with the reflectors sytrd_upper left in A, TAU (LAPACK dormtr,
+// SIDE = 'L', UPLO = 'U', TRANS = 'N'); work holds ormtr_work_doubles(n).
+size_t ormtr_work_doubles(int n);
+void ormtr_upper(hipStream_t s, int n, const double* A, const double* TAU, double* Z, double* work);
 // A symmetric from its upper triangle: A[i, j] = A[j, i] for i > j
 void sym_lower(hipStream_t s, double* A, int n);
 void permute_cols(hipStream_t s, const double* X, const int* perm, double* out, int n);

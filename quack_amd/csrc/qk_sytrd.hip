@@ -467,6 +467,63 @@ __global__ __launch_bounds__(TBW) void k_finish_update(
 
 __global__ void k_d0(const double* __restrict__ A, double* __restrict__ D) { D[0] = A[0]; }
 
+
+// ---- back-transformation Z <- Q Z (dormtr, SIDE = 'L', UPLO = 'U', TRANS = 'N') ----
+// Q = H(n-1) ... H(1); reflector j (0-based, column c = j + 1) is v_j with
+// v_j(r) = A(r, j+1) for r < j, v_j(j) = 1, zero below; tau_j = TAU[j].  Blocks of
+// OK reflectors in increasing j are applied in turn, each in compact WY form
+// H_{j0+OK-1} ... H_{j0} = I - V T V^T with T lower triangular (LAPACK dlarft,
+// DIRECT = 'B'): Z(0:m, :) -= V (T (V^T Z(0:m, :))), m = j0 + OK.
+constexpr int OK = 128;   // reflectors per block
+constexpr int OKC = 512;  // rows per split-K chunk of V^T Z
+
+// V (n x nbk, zero-padded columns past n-1) and the padded taus
+__global__ void k_build_v(const double* __restrict__ A, int n, const double* __restrict__ TAU, int nbk,
+                          double* __restrict__ V, double* __restrict__ taup) {
+    const long long total = (long long)n * nbk;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(t % n), j = (int)(t / n);
+        double v = 0.0;
+        if (j < n - 1) v = r < j ? A[r + (long long)n * (j + 1)] : (r == j ? 1.0 : 0.0);
+        V[t] = v;
+        if (r == 0) taup[j] = j < n - 1 ? TAU[j] : 0.0;
+    }
+}
+
+// T of each block (one workgroup per block) from G = V^T V:
+// T(j, j) = tau_j; T(j+1:, j) = -tau_j T(j+1:, j+1:) G(j+1:, j), j descending.
+// Written OKS times side by side, [T T ... T] (OK x OKS OK), so that one GEMM
+// applies T to the sum of the split-K partials.
+__global__ __launch_bounds__(256) void k_build_t(const double* __restrict__ G, const double* __restrict__ taup,
+                                                 double* __restrict__ Trep, int oks) {
+    __shared__ double T[OK][OK + 1];
+    __shared__ double gc[OK];
+    const int b = blockIdx.x;
+    const double* Gb = G + (long long)b * OK * OK;
+    const double* tb = taup + (long long)b * OK;
+    for (int e = threadIdx.x; e < OK * OK; e += blockDim.x) T[e % OK][e / OK] = 0.0;
+    lds_sync();
+    if (threadIdx.x == 0) T[OK - 1][OK - 1] = tb[OK - 1];
+    for (int j = OK - 2; j >= 0; --j) {
+        if (threadIdx.x < OK) gc[threadIdx.x] = Gb[threadIdx.x + (long long)OK * j];
+        lds_sync();
+        const int i = threadIdx.x;
+        if (i > j && i < OK) {
+            double t = 0.0;
+            for (int l = j + 1; l <= i; ++l) t += T[i][l] * gc[l];
+            T[i][j] = -tb[j] * t;
+        }
+        if (i == j) T[j][j] = tb[j];
+        lds_sync();
+    }
+    double* o = Trep + (long long)b * OK * OK * oks;
+    for (long long e = threadIdx.x; e < (long long)OK * OK * oks; e += blockDim.x) {
+        const int i = (int)(e % OK), col = (int)((e / OK) % OK);
+        o[e] = T[i][col];
+    }
+}
+
 }  // namespace
 
 size_t sytrd_work_doubles(int n, int nb) {
@@ -529,6 +586,69 @@ void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* 
     }
     hipLaunchKernelGGL(k_d0, dim3(1), dim3(1), 0, s, A, D);
     QK_HIP(hipGetLastError());
+}
+
+
+size_t ormtr_work_doubles(int n) {
+    const int nblk = (n - 1 + OK - 1) / OK;
+    const size_t nbk = (size_t)nblk * OK;
+    const int oks = (n + OKC - 1) / OKC;
+    return (size_t)n * nbk + nbk + (size_t)nblk * OK * OK + (size_t)nblk * OK * OK * oks + (size_t)oks * OK * n +
+           (size_t)OK * n;
+}
+
+void ormtr_upper(hipStream_t s, int n, const double* A, const double* TAU, double* Z, double* work) {
+    QK_CHECK(n >= 1, QK_E_ARG, "ormtr: bad size");
+    if (n == 1) return;
+    const int nblk = (n - 1 + OK - 1) / OK;
+    const int nbk = nblk * OK;
+    const int oks = (n + OKC - 1) / OKC;  // largest number of split-K chunks
+    double* V = work;
+    double* taup = V + (size_t)n * nbk;
+    double* G = taup + nbk;
+    double* Trep = G + (size_t)nblk * OK * OK;
+    double* W1p = Trep + (size_t)nblk * OK * OK * oks;  // (oks OK) x n, partial V^T Z stacked by chunk
+    double* W2 = W1p + (size_t)oks * OK * n;            // OK x n
+    const long long tot = (long long)n * nbk;
+    hipLaunchKernelGGL(k_build_v, dim3((unsigned)std::min<long long>((tot + 255) / 256, 65536)), dim3(256), 0, s, A,
+                       n, TAU, nbk, V, taup);
+    QK_HIP(hipGetLastError());
+    {
+        Batch bg;
+        bg.sa0 = (long long)OK * n;
+        bg.sb0 = (long long)OK * n;
+        bg.sc0 = (long long)OK * OK;
+        bg.count = nblk;
+        bg.zdiv = nblk;  // one-level batch: z0 = z runs over the sa0/sb0/sc0 strides
+        dgemm(s, true, false, OK, OK, n, 1.0, V, n, V, n, 0.0, G, OK, bg);
+    }
+    hipLaunchKernelGGL(k_build_t, dim3(nblk), dim3(256), 0, s, G, taup, Trep, oks);
+    QK_HIP(hipGetLastError());
+    const long long ldp = (long long)oks * OK;
+    for (int b = 0; b < nblk; ++b) {
+        const int j0 = b * OK;
+        const int m = std::min(j0 + OK, n);
+        const double* Vb = V + (long long)n * j0;
+        // W1p = V(0:m, :)^T Z(0:m, :) in chunks of OKC rows, stacked
+        const int s1 = m / OKC, rem = m % OKC;
+        if (s1 > 0) {
+            Batch bp;
+            bp.sa0 = OKC;
+            bp.sb0 = OKC;
+            bp.sc0 = OK;
+            bp.count = s1;
+            bp.zdiv = s1;
+            dgemm(s, true, false, OK, n, OKC, 1.0, Vb, n, Z, n, 0.0, W1p, ldp, bp);
+        }
+        if (rem > 0)
+            dgemm(s, true, false, OK, n, rem, 1.0, Vb + (long long)s1 * OKC, n, Z + (long long)s1 * OKC, n, 0.0,
+                  W1p + (long long)s1 * OK, ldp);
+        const int sc = s1 + (rem > 0 ? 1 : 0);
+        // W2 = T (sum of the partials) = [T ... T] W1p
+        dgemm(s, false, false, OK, n, sc * OK, 1.0, Trep + (long long)b * OK * OK * oks, OK, W1p, ldp, 0.0, W2, OK);
+        // Z(0:m, :) -= V(0:m, :) W2
+        dgemm(s, false, false, m, n, OK, -1.0, Vb, n, W2, OK, 1.0, Z, n);
+    }
 }
 
 }  // namespace qk
