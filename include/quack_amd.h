@@ -105,14 +105,16 @@ int qk_gw_set_precision(qk_gw_handle h, int mode);
  *   mode 0 (default): rocSOLVER dsyevd for every RPA, as QuAcK re-diagonalises
  *           every cycle;
  *   mode 2: full solves by the library's own Householder tridiagonalisation
- *           (LAPACK dsytrd layout), rocSOLVER dstedc/dsterf and dormtr;
+ *           (LAPACK dsytrd layout), rocSOLVER dstedc on the tridiagonal and the
+ *           library's own blocked back-transformation (in place of dormtr);
  *   mode 1 (or 3 = 1 with the full solves of mode 2):
  *           when no orbital energy moved by more than de_max (Ha) since the
  *           handle's last eigenvectors were computed (same TDA_W), refine those
  *           eigenvectors by Newton steps for X^T X = I, X^T C X diagonal (FP64
  *           matrix-core GEMMs, quadratic convergence, stopped when the largest
- *           correction is below 1e-8); dsyevd otherwise, or when a correction
- *           exceeds 0.25 or six steps do not converge.
+ *           correction is below 1e-8; close eigenvalues handled as clusters);
+ *           a full solve otherwise, or when a cluster outgrows 32 vectors,
+ *           merging clusters does not settle, or eight steps do not converge.
  * Both modes return FP64 eigenpairs of the same matrix.  No counterpart in
  * QuAcK.  Mode 1 keeps four nS x nS FP64 work arrays on the device. */
 int qk_gw_set_eigensolver(qk_gw_handle h, int mode, double de_max);

@@ -64,7 +64,7 @@ void syevd(hipStream_t s, rocblas_handle rb, bool vectors, int n, double* A, dou
 
 // Full eigen-decomposition without dsyevd: own Householder tridiagonalisation
 // (qk_sytrd.hip), rocSOLVER's divide-and-conquer dstedc on the tridiagonal
-// (dsterf for eigenvalues only), the own blocked back-transformation with the
+// (also for eigenvalues only), the own blocked back-transformation with the
 // reflectors.  Same outputs as syevd(): w ascending, A = eigenvectors.
 void syevd_own(hipStream_t s, rocblas_handle rb, bool vectors, int n, double*& A, double* w, double* E, double* tau,
                double*& Z, double* work, double* owork, int* dinfo, hipEvent_t* tev = nullptr) {
@@ -76,7 +76,9 @@ void syevd_own(hipStream_t s, rocblas_handle rb, bool vectors, int n, double*& A
         ormtr_upper(s, n, A, tau, Z, owork);
         std::swap(A, Z);
     } else {
-        rb_check(rocsolver_dsterf(rb, n, w, E, dinfo), "rocsolver_dsterf");
+        // eigenvalues only: divide-and-conquer into the spare matrix (rocSOLVER's
+        // dsterf, a sequential QL sweep, takes seconds at n ~ 5000)
+        rb_check(rocsolver_dstedc(rb, rocblas_evect_tridiagonal, n, w, E, Z, n, dinfo), "rocsolver_dstedc");
     }
     int info = 0;
     QK_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
