@@ -178,6 +178,13 @@ const char *qk_last_error(void);
 int qk_set_autotune(int on);
 int qk_autotune_report(char *buf, int len);
 
+/* hipGraph replay of the own full eigensolve's launch sequences (process-wide;
+ * off by default, since the replay measured slower than stream launches): the
+ * tridiagonalisation (two launches per column) and the back-transformation are
+ * captured once per matrix buffer of a handle and then replayed by one graph
+ * launch each.  Results are identical either way.  No counterpart in QuAcK. */
+int qk_set_graphs(int on);
+
 /* ---- batched G0W0: many molecules of one size in one handle ------------- */
 
 /* nmol molecules with the same (nBas, nOrb, nC, nO, nV, nR, nS) on one stream:

@@ -38,6 +38,13 @@ struct Batch {
     int upper = 0;
 };
 
+// True while the stream records a graph capture (no host timing or
+// synchronisation may run then).
+inline bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
 // Process-wide GEMM tile autotuner (on by default): see qk_gemm.hip.
 void set_autotune(bool on);
 bool autotune_on();

@@ -227,6 +227,10 @@ int dispatch(hipStream_t s, int M, int N, int K, double alpha, const double* A, 
         launch_cfg<TA, TB>(it->second.cfg, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
         return 1;
     }
+    if (stream_capturing(s)) {  // no timing inside a graph capture: an untuned shape keeps its default
+        launch_cfg<TA, TB>(default_cfg(M, N), s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
+        return 1;
+    }
     TuneEntry e{default_cfg(M, N), {0.f, 0.f, 0.f, 0.f}};
     hipEvent_t ev[2] = {nullptr, nullptr};
     QK_HIP(hipEventCreate(&ev[0]));

@@ -113,10 +113,13 @@ constexpr int SLAB_B = 2 * 3 * 128 * 64;  // A and B, three slices, 128 rows of 
 // s_waitcnt immediate for "vector-memory count <= n" (gfx9 layout), other counters left at their maximum
 #define QK_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8))
 
+// OutT: double, or float where the consumer needs only the FP32-level accuracy
+// the slicing gives (the FP64 recombination is then rounded once on the store).
+template <typename OutT>
 __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, const __bf16* __restrict__ A0,
                                                          const __bf16* __restrict__ A1, const __bf16* __restrict__ A2,
                                                          const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
-                                                         const __bf16* __restrict__ B2, double* __restrict__ C,
+                                                         const __bf16* __restrict__ B2, OutT* __restrict__ C,
                                                          long long ldc, int tiles_m, int tiles_n, int gm) {
     __shared__ __attribute__((aligned(1024))) char st0[SLAB_B];
     __shared__ __attribute__((aligned(1024))) char st1[SLAB_B];
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 int j = n0 + wn + b * 16 + 4 * (lane >> 4) + r;
-                if (j < N) C[i + ldc * (long long)j] = acc64[a][b][r];
+                if (j < N) C[i + ldc * (long long)j] = (OutT)acc64[a][b][r];
             }
     }
 }
@@ -263,17 +266,18 @@ namespace {
 constexpr int NGM = 4;
 const int GM_CAND[NGM] = {2, 4, 8, 16};
 std::mutex g_mu;
-std::map<std::tuple<int, int, int>, std::pair<int, std::array<float, NGM>>> g_gm;
+std::map<std::tuple<int, int, int, int>, std::pair<int, std::array<float, NGM>>> g_gm;
 
-void launch_gm(hipStream_t s, int gm, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc, int tm,
+template <typename OutT>
+void launch_gm(hipStream_t s, int gm, int M, int N, const Split3& A, const Split3& B, OutT* C, long long ldc, int tm,
                int tn) {
-    hipLaunchKernelGGL(k_gemm_bf16x3, dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1], A.s[2],
+    hipLaunchKernelGGL(k_gemm_bf16x3<OutT>, dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1], A.s[2],
                        B.s[0], B.s[1], B.s[2], C, ldc, tm, tn, gm);
     QK_HIP(hipGetLastError());
 }
-}  // namespace
 
-int gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc) {
+template <typename OutT>
+int gemm_bf16x3_t(hipStream_t s, int M, int N, const Split3& A, const Split3& B, OutT* C, long long ldc) {
     QK_CHECK(A.kpad == B.kpad && A.rows == M && B.rows == N, QK_E_ARG, "gemm_bf16x3: operand shapes");
     int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
     QK_CHECK((long long)tm * tn < (1LL << 31), QK_E_ARG, "gemm_bf16x3: grid too large");
@@ -283,10 +287,14 @@ int gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, d
         return 1;
     }
     std::lock_guard<std::mutex> lock(g_mu);
-    auto key = std::make_tuple(M, N, A.kpad);
+    auto key = std::make_tuple(M, N, A.kpad, (int)sizeof(OutT));
     auto it = g_gm.find(key);
     if (it != g_gm.end()) {
         launch_gm(s, it->second.first, M, N, A, B, C, ldc, tm, tn);
+        return 1;
+    }
+    if (stream_capturing(s)) {  // no timing inside a graph capture
+        launch_gm(s, 8, M, N, A, B, C, ldc, tm, tn);
         return 1;
     }
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -312,14 +320,23 @@ int gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, d
     g_gm.emplace(key, std::make_pair(best, ms));
     return NGM;
 }
+}  // namespace
+
+int gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc) {
+    return gemm_bf16x3_t(s, M, N, A, B, C, ldc);
+}
+
+int gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, float* C, long long ldc) {
+    return gemm_bf16x3_t(s, M, N, A, B, C, ldc);
+}
 
 std::string autotune_report_bf16() {
     std::lock_guard<std::mutex> lock(g_mu);
     std::string r;
     for (const auto& kv : g_gm) {
         char buf[256];
-        std::snprintf(buf, sizeof buf, "gemm_bf16x3 M=%d N=%d kpad=%d -> group %d (ms:", std::get<0>(kv.first),
-                      std::get<1>(kv.first), std::get<2>(kv.first), kv.second.first);
+        std::snprintf(buf, sizeof buf, "gemm_bf16x3 M=%d N=%d kpad=%d out=f%d -> group %d (ms:", std::get<0>(kv.first),
+                      std::get<1>(kv.first), std::get<2>(kv.first), 8 * std::get<3>(kv.first), kv.second.first);
         r += buf;
         for (int c = 0; c < NGM; ++c) {
             std::snprintf(buf, sizeof buf, " g%d %.3f", GM_CAND[c], kv.second.second[c]);

@@ -11,8 +11,11 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -62,18 +65,89 @@ void syevd(hipStream_t s, rocblas_handle rb, bool vectors, int n, double* A, dou
     QK_CHECK(info == 0, QK_E_EIG, "rocsolver_dsyevd failed, info = " + std::to_string(info));
 }
 
+// hipGraph replay of the own full solve's launch sequences (process-wide
+// switch, qk_set_graphs; off by default).  The tridiagonalisation (two
+// launches per column) and the back-transformation contain only the
+// library's own kernels, with fixed arguments for a given matrix buffer, so
+// each is captured once per buffer and then replayed by one graph launch.  The
+// solve swaps the matrix and eigenvector buffers, so a handle meets two keys.
+// Measured at n = 5600 on MI355X (ROCm 7.2): 142 ms per tridiagonalisation
+// replayed against 137 ms launched on the stream, hence off by default.
+std::atomic<int> g_graphs{0};
+
+struct GraphSlot {
+    const void* key = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+struct SolveGraphs {
+    GraphSlot trd[2], orm[2];
+    void clear() {
+        for (GraphSlot* g : {&trd[0], &trd[1], &orm[0], &orm[1]}) {
+            if (g->exec) (void)hipGraphExecDestroy(g->exec);
+            *g = GraphSlot();
+        }
+    }
+};
+
+// Run body on stream s: replayed from the slot keyed by key, captured into a
+// free slot on first use (thread-local capture mode, so other host threads
+// working on other streams are unaffected), or run eagerly when the slots are
+// taken or graphs are off.
+template <class F>
+void run_graph(hipStream_t s, GraphSlot* slots, const void* key, F&& body) {
+    if (!g_graphs.load()) {
+        body();
+        return;
+    }
+    for (int i = 0; i < 2; ++i)
+        if (slots[i].exec && slots[i].key == key) {
+            QK_HIP(hipGraphLaunch(slots[i].exec, s));
+            return;
+        }
+    GraphSlot* slot = !slots[0].exec ? &slots[0] : !slots[1].exec ? &slots[1] : nullptr;
+    if (!slot) {
+        body();
+        return;
+    }
+    QK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        body();
+    } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(s, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+    }
+    hipGraph_t g = nullptr;
+    QK_HIP(hipStreamEndCapture(s, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    QK_HIP(e);
+    slot->key = key;
+    slot->exec = ex;
+    QK_HIP(hipGraphLaunch(ex, s));
+}
+
 // Full eigen-decomposition without dsyevd: own Householder tridiagonalisation
 // (qk_sytrd.hip), rocSOLVER's divide-and-conquer dstedc on the tridiagonal
 // (also for eigenvalues only), the own blocked back-transformation with the
 // reflectors.  Same outputs as syevd(): w ascending, A = eigenvectors.
 void syevd_own(hipStream_t s, rocblas_handle rb, bool vectors, int n, double*& A, double* w, double* E, double* tau,
-               double*& Z, double* work, double* owork, int* dinfo, hipEvent_t* tev = nullptr) {
+               double*& Z, double* work, double* owork, int* dinfo, hipEvent_t* tev = nullptr,
+               SolveGraphs* gr = nullptr) {
     if (tev) QK_HIP(hipEventRecord(tev[0], s));
-    sytrd_upper(s, n, A, w, E, tau, work, 32);
+    double* Ain = A;
+    auto trd = [&] { sytrd_upper(s, n, Ain, w, E, tau, work, 32); };
+    if (gr) run_graph(s, gr->trd, Ain, trd);
+    else trd();
     if (tev) QK_HIP(hipEventRecord(tev[1], s));
     if (vectors) {
         rb_check(rocsolver_dstedc(rb, rocblas_evect_tridiagonal, n, w, E, Z, n, dinfo), "rocsolver_dstedc");
-        ormtr_upper(s, n, A, tau, Z, owork);
+        double* Zin = Z;
+        auto orm = [&] { ormtr_upper(s, n, Ain, tau, Zin, owork); };
+        if (gr) run_graph(s, gr->orm, Ain, orm);
+        else orm();
         std::swap(A, Z);
     } else {
         // eigenvalues only: divide-and-conquer into the spare matrix (rocSOLVER's
@@ -179,6 +253,9 @@ struct qk_gw_ctx {
     hipStream_t s = nullptr;
     rocblas_handle rb = nullptr;
     double *M = nullptr, *K = nullptr, *Cm = nullptr, *rhoT = nullptr;
+    // rhoT holds FP32 values (first half of the allocation) after a
+    // split-precision contraction, whose accuracy is FP32-level; FP64 otherwise
+    bool rho_f32 = false;
     double *e = nullptr, *e0 = nullptr, *eG = nullptr, *dE = nullptr, *sd = nullptr, *Om = nullptr, *Om0 = nullptr;
     double *Sig = nullptr, *Z = nullptr, *eLin = nullptr, *eGW = nullptr, *w = nullptr, *f = nullptr, *df = nullptr;
     double *part = nullptr, *scal = nullptr, *work = nullptr;
@@ -202,6 +279,7 @@ struct qk_gw_ctx {
     int eig_mode = 0;  // bit 0: refine; bit 1: own tridiagonalisation instead of dsyevd
     double eig_demax = 5e-3;
     double *Zev = nullptr, *tau = nullptr, *trd_work = nullptr, *orm_work = nullptr;  // own full-solve path
+    qk::SolveGraphs graphs;  // captured launch sequences of the own full solve
     double *Xv = nullptr, *Ym = nullptr, *Sm = nullptr, *Gm = nullptr, *lam = nullptr, *eig_dbuf = nullptr;
     int *perm = nullptr, *eig_ibuf = nullptr;
     long long* eig_lbuf = nullptr;
@@ -285,6 +363,14 @@ constexpr double EIG_BIG = 0.25;   // |E_ij| above this: outside the convergence
 constexpr double EIG_TOL = 1e-8;   // last step: the next correction would be O(1e-16)
 constexpr double EIG_DEG = 1e-9;   // eigenvalues closer than this (relative) form a cluster
 constexpr int EIG_KMAX = 32;       // largest cluster handled
+// QK_EIG_TRACE=1 prints the largest correction of every refinement step to stderr
+bool eig_trace() {
+    static const bool on = [] {
+        const char* v = std::getenv("QK_EIG_TRACE");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
 
 // Cyclic Jacobi eigen-decomposition of a small symmetric matrix A (k x k,
 // column-major, destroyed): V = eigenvectors (columns), w = eigenvalues,
@@ -544,6 +630,9 @@ bool refine_eig(Ctx* c) {
         }
         double emax = 0.0;
         for (double v : colmax) emax = std::max(emax, v);
+        if (eig_trace())
+            std::fprintf(stderr, "refine: n %d step %d max|E| %.3e clusters %d largest %d\n", n, it, emax, cl.ncl,
+                         cl.biggest);
         eig_update(s, c->Sm, c->Gm, c->lam, d_lab, n);
         dgemm(s, false, false, n, n, n, 1.0, X, n, c->Sm, n, 0.0, Y, n);
         std::swap(X, Y);
@@ -604,7 +693,7 @@ void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors)
         if (!done) {
             if (c->eig_mode & 2) {
                 syevd_own(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->tau, c->Zev, c->trd_work,
-                          c->orm_work, c->info, c->ev3);
+                          c->orm_work, c->info, c->ev3, &c->graphs);
                 float ms = 0.f;
                 QK_HIP(hipEventSynchronize(c->ev3[1]));
                 QK_HIP(hipEventElapsedTime(&ms, c->ev3[0], c->ev3[1]));
@@ -652,6 +741,7 @@ void excitation(Ctx* c) {
         alloc_split(c->Msplit, (int)n2, c->nS);
         c->msplit_valid = false;
     }
+    c->rho_f32 = c->precision == 1;
     if (c->precision == 0) {
         Stage g(c, 5, true);
         count_gemm(c, dgemm(c->s, true, true, c->nS, (int)n2, c->nS, 1.0, c->Cm, c->nS, c->M, n2, 0.0, c->rhoT,
@@ -666,7 +756,7 @@ void excitation(Ctx* c) {
     }
     split3(c->s, c->Cm, c->nS, true, c->nS, c->nS, c->Wsplit);
     Stage g(c, 5, true);
-    count_gemm(c, gemm_bf16x3(c->s, c->nS, (int)n2, c->Wsplit, c->Msplit, c->rhoT, c->nS));
+    count_gemm(c, gemm_bf16x3(c->s, c->nS, (int)n2, c->Wsplit, c->Msplit, reinterpret_cast<float*>(c->rhoT), c->nS));
 }
 
 // Density-fitted excitation densities: T(P, m) = sum_jb Bov(P, jb) W(jb, m),
@@ -675,6 +765,7 @@ void excitation(Ctx* c) {
 void excitation_df(Ctx* c) {
     const int nS = c->nS, naux = c->naux;
     const long long n2 = (long long)c->nOrb * c->nOrb;
+    c->rho_f32 = c->precision == 1;
     if (c->precision == 0) {
         dgemm(c->s, false, false, naux, nS, nS, 1.0, c->Bov, naux, c->Cm, nS, 0.0, c->Tdf, naux);
         Stage g(c, 5, true);
@@ -697,12 +788,12 @@ void excitation_df(Ctx* c) {
     gemm_bf16x3(c->s, naux, nS, c->BovTsplit, c->Wsplit, c->Tdf, naux);
     split3(c->s, c->Tdf, naux, true, nS, naux, c->Tsplit);
     Stage g(c, 5, true);
-    count_gemm(c, gemm_bf16x3(c->s, nS, (int)n2, c->Tsplit, c->Bsplit, c->rhoT, nS));
+    count_gemm(c, gemm_bf16x3(c->s, nS, (int)n2, c->Tsplit, c->Bsplit, reinterpret_cast<float*>(c->rhoT), nS));
 }
 
 void sigma_static(Ctx* c, const double* e, double eta, int srg) {
     Stage st(c, 3);
-    sigma_partial(c->s, c->rhoT, e, e, c->Om, nullptr, eta, srg, SRG_FLOW, c->nOrb, c->nC, c->nO, c->nR, c->nS,
+    sigma_partial(c->s, c->rhoT, c->rho_f32, e, e, c->Om, nullptr, eta, srg, SRG_FLOW, c->nOrb, c->nC, c->nO, c->nR, c->nS,
                   c->nchunk, c->part);
     sigma_finalize(c->s, c->part, c->nchunk, c->nOrb, c->nC, c->nO, c->nR, c->Sig, c->Z, c->scal);
 }
@@ -715,7 +806,7 @@ void qp_graph(Ctx* c, const double* e0, const double* eG, const double* start, d
     qp_init(c->s, start, c->w, c->f, c->nit, c->done, n, c->nC, c->nR);
     int h_active = 1;
     for (int it = 0; it < QP_MAXIT && h_active > 0; ++it) {
-        sigma_partial(c->s, c->rhoT, c->w, eG, c->Om, c->done, eta, srg, flow, n, c->nC, c->nO, c->nR, c->nS,
+        sigma_partial(c->s, c->rhoT, c->rho_f32, c->w, eG, c->Om, c->done, eta, srg, flow, n, c->nC, c->nO, c->nR, c->nS,
                       c->nchunk, c->part);
         qp_update(c->s, c->part, c->nchunk, e0, c->w, c->f, c->df, c->nit, c->done, n, QP_THRESH, QP_MAXIT, c->active);
         QK_HIP(hipMemcpyAsync(&h_active, c->active, sizeof(int), hipMemcpyDeviceToHost, c->s));
@@ -854,6 +945,7 @@ int guarded(std::string* slot, F&& fn) {
 
 void release(Ctx* c) {
     if (c->s) (void)hipStreamSynchronize(c->s);
+    c->graphs.clear();
     if (c->shared) {  // slots of the batch's contiguous arrays
         c->M = c->Cm = c->rhoT = c->Om = c->work = nullptr;
         c->info = c->active = nullptr;
@@ -1082,7 +1174,7 @@ int qk_gw_sigma_at(qk_gw_handle c, const double* eG, const double* w, double eta
         int n = c->nOrb;
         h2d(c, c->eG, eG, n);
         h2d(c, c->w, w, n);
-        sigma_partial(c->s, c->rhoT, c->w, c->eG, c->Om, nullptr, eta, doSRG, SRG_FLOW, n, c->nC, c->nO, c->nR,
+        sigma_partial(c->s, c->rhoT, c->rho_f32, c->w, c->eG, c->Om, nullptr, eta, doSRG, SRG_FLOW, n, c->nC, c->nO, c->nR,
                       c->nS, c->nchunk, c->part);
         sigma_out(c->s, c->part, c->nchunk, n, c->nC, c->nR, c->f, c->df);
         d2h(c, Sig, c->f, n);
@@ -1461,7 +1553,7 @@ int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int
             double* sc = get(4);
             QK_HIP(hipMemcpyAsync(de, e, sizeof(double) * n, hipMemcpyHostToDevice, s));
             QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
-            sigma_partial(s, rT, de, de, dO, nullptr, eta, doSRG, SRG_FLOW, n, nC, nO, nR, nS, nch, part);
+            sigma_partial(s, rT, false, de, de, dO, nullptr, eta, doSRG, SRG_FLOW, n, nC, nO, nR, nS, nch, part);
             sigma_finalize(s, part, nch, n, nC, nO, nR, dS, dZ, sc);
             QK_HIP(hipMemcpyAsync(SigC, dS, sizeof(double) * n, hipMemcpyDeviceToHost, s));
             QK_HIP(hipMemcpyAsync(Z, dZ, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -1514,7 +1606,7 @@ int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO
             qp_init(s, dL, w, f, nit, done, n, nC, nR);
             int h_active = 1;
             for (int it = 0; it < QP_MAXIT && h_active > 0; ++it) {
-                sigma_partial(s, rT, w, dG, dO, done, eta, doSRG, flow, n, nC, nO, nR, nS, nch, part);
+                sigma_partial(s, rT, false, w, dG, dO, done, eta, doSRG, flow, n, nC, nO, nR, nS, nch, part);
                 qp_update(s, part, nch, d0, w, f, df, nit, done, n, QP_THRESH, QP_MAXIT, act);
                 QK_HIP(hipMemcpyAsync(&h_active, act, sizeof(int), hipMemcpyDeviceToHost, s));
                 QK_HIP(hipStreamSynchronize(s));
@@ -1621,6 +1713,7 @@ void excitation_batch(BCtx* b) {
     BStage st(b, 2);
     Ctx* c0 = b->mol[0];
     const long long n2 = (long long)c0->nOrb * c0->nOrb, nS = c0->nS;
+    for (Ctx* c : b->mol) c->rho_f32 = b->precision == 1;
     if (b->precision == 0) {
         Batch bt;
         bt.count = b->nmol;
@@ -1642,7 +1735,7 @@ void excitation_batch(BCtx* b) {
             c->msplit_valid = true;
         }
         split3(c->s, c->Cm, c->nS, true, c->nS, c->nS, c->Wsplit);
-        gemm_bf16x3(c->s, c->nS, (int)n2, c->Wsplit, c->Msplit, c->rhoT, c->nS);
+        gemm_bf16x3(c->s, c->nS, (int)n2, c->Wsplit, c->Msplit, reinterpret_cast<float*>(c->rhoT), c->nS);
     }
 }
 
@@ -1657,7 +1750,7 @@ void qp_graph_batch(BCtx* b, double eta, int srg) {
         for (int m = 0; m < nm; ++m) {
             if (!act[m]) continue;
             Ctx* c = b->mol[m];
-            sigma_partial(c->s, c->rhoT, c->w, c->e0, c->Om, c->done, eta, srg, SRG_FLOW, c->nOrb, c->nC, c->nO,
+            sigma_partial(c->s, c->rhoT, c->rho_f32, c->w, c->e0, c->Om, c->done, eta, srg, SRG_FLOW, c->nOrb, c->nC, c->nO,
                           c->nR, c->nS, c->nchunk, c->part);
             qp_update(c->s, c->part, c->nchunk, c->e0, c->w, c->f, c->df, c->nit, c->done, c->nOrb, QP_THRESH,
                       QP_MAXIT, c->active);
@@ -1795,7 +1888,7 @@ int qk_gwb_g0w0(qk_gwb_handle h, const double* eHF, double eta, int linearize, i
         {
             BStage st(h, 3);
             for (Ctx* c : h->mol) {
-                sigma_partial(c->s, c->rhoT, c->e0, c->e0, c->Om, nullptr, eta, doSRG, SRG_FLOW, n, c->nC, c->nO,
+                sigma_partial(c->s, c->rhoT, c->rho_f32, c->e0, c->e0, c->Om, nullptr, eta, doSRG, SRG_FLOW, n, c->nC, c->nO,
                               c->nR, nS, c->nchunk, c->part);
                 sigma_finalize(c->s, c->part, c->nchunk, n, c->nC, c->nO, c->nR, c->Sig, c->Z, c->scal);
                 lin_qp(c->s, c->e0, c->Z, c->Sig, c->eLin, n, 1);
@@ -1837,6 +1930,11 @@ int qk_gwb_stage_times(qk_gwb_handle h, double* ms, int n) {
     return QK_OK;
 }
 
+
+int qk_set_graphs(int on) {
+    qk::g_graphs.store(on ? 1 : 0);
+    return QK_OK;
+}
 
 int qk_set_autotune(int on) {
     qk::set_autotune(on != 0);

@@ -125,11 +125,16 @@ __global__ void k_scale_w(double* __restrict__ W, const double* __restrict__ sd,
 // part[3*(p*nchunk + c) + {0,1,2}] = {sig_occ, sig_vir, dsig}.
 // p runs over the active orbitals p = nC + blockIdx.y; skip[p] != 0 exits.
 // Each thread owns SIG_MPT modes m (chunk = RB * SIG_MPT), keeps their Om in
-// registers and streams rho(p, q, m) for all q with SIG_MPT independent loads.
+// registers and streams rho(p, q, m) for all q: the loads of row q+1 are issued
+// before the arithmetic of row q, and every load is unconditional (padding
+// lanes read the last mode and discard it), so no load waits inside a branch.
 // For 2 s eps^2 > 38, exp(-2 s eps^2) < 2^-54 and 1 - exp(...) rounds to 1.0,
 // so the exponential is skipped there with bit-identical results.
+// RhoT: double, or float where rho comes from the split-precision contraction
+// (whose FP32-level accuracy the FP32 copy keeps); every sum is FP64.
 constexpr int SIG_MPT = 4;
-__global__ __launch_bounds__(RB) void k_sigma_partial(const double* __restrict__ rhoT, const double* __restrict__ w,
+template <typename RhoT>
+__global__ __launch_bounds__(RB) void k_sigma_partial(const RhoT* __restrict__ rhoT, const double* __restrict__ w,
                                                       const double* __restrict__ e, const double* __restrict__ Om,
                                                       const int* __restrict__ skip, double eta, int srg, double flow,
                                                       int n, int nC, int nO, int nR, int nS,
@@ -142,27 +147,36 @@ __global__ __launch_bounds__(RB) void k_sigma_partial(const double* __restrict__
     const double eta2 = eta * eta;
     const int mb = c * (RB * SIG_MPT) + threadIdx.x;
     double om[SIG_MPT];
+    int mk[SIG_MPT];
+    bool vk[SIG_MPT];
 #pragma unroll
     for (int k = 0; k < SIG_MPT; ++k) {
-        int m = mb + k * RB;
-        om[k] = m < nS ? Om[m] : 0.0;
+        const int m = mb + k * RB;
+        vk[k] = m < nS;
+        mk[k] = vk[k] ? m : nS - 1;
+        om[k] = vk[k] ? Om[m] : 0.0;
+    }
+    const int q0 = nC, q1 = n - nR;
+    RhoT cur[SIG_MPT], nxt[SIG_MPT];
+    if (q0 < q1) {
+        const RhoT* rq = rhoT + (long long)nS * (p + (long long)n * q0);
+#pragma unroll
+        for (int k = 0; k < SIG_MPT; ++k) cur[k] = rq[mk[k]];
     }
     double so = 0.0, sv = 0.0, ds = 0.0;
-    for (int q = nC; q < n - nR; ++q) {
+    for (int q = q0; q < q1; ++q) {
+        const int qn = q + 1 < q1 ? q + 1 : q;  // the last row reloads itself
+        const RhoT* rn = rhoT + (long long)nS * (p + (long long)n * qn);  // rho(p, q+1, :)
+#pragma unroll
+        for (int k = 0; k < SIG_MPT; ++k) nxt[k] = rn[mk[k]];
         const double sgn = q < nO ? 1.0 : -1.0;
         const double wq = wp - e[q];
-        const double* rq = rhoT + (long long)nS * (p + (long long)n * q);  // rho(p, q, :)
-        double r[SIG_MPT];
-#pragma unroll
-        for (int k = 0; k < SIG_MPT; ++k) {
-            int m = mb + k * RB;
-            r[k] = m < nS ? rq[m] : 0.0;
-        }
         double s_acc = 0.0, d_acc = 0.0;
 #pragma unroll
         for (int k = 0; k < SIG_MPT; ++k) {
-            if (mb + k * RB >= nS) break;  // padding modes contribute nothing (eps may be 0 there)
-            double num = 2.0 * r[k] * r[k];
+            if (!vk[k]) continue;  // padding modes contribute nothing (eps may be 0 there)
+            const double r = (double)cur[k];
+            double num = 2.0 * r * r;
             double eps = wq + sgn * om[k];
             double e2 = eps * eps;
             if (!srg) {
@@ -180,6 +194,8 @@ __global__ __launch_bounds__(RB) void k_sigma_partial(const double* __restrict__
         }
         if (q < nO) so += s_acc; else sv += s_acc;
         ds += d_acc;
+#pragma unroll
+        for (int k = 0; k < SIG_MPT; ++k) cur[k] = nxt[k];
     }
     __shared__ double sh[RB / 64];
     double r0 = block_sum(so, sh);
@@ -591,13 +607,17 @@ int sigma_chunks(int nS, int nact) {
     (void)nact;
     return (nS + RB * SIG_MPT - 1) / (RB * SIG_MPT);
 }
-void sigma_partial(hipStream_t s, const double* rhoT, const double* w, const double* e, const double* Om,
+void sigma_partial(hipStream_t s, const void* rhoT, bool rho_f32, const double* w, const double* e, const double* Om,
                    const int* skip, double eta, int srg, double flow, int n, int nC, int nO, int nR, int nS,
                    int nchunk, double* part) {
     int nact = n - nR - nC;
     QK_CHECK((long long)nchunk * RB * SIG_MPT >= nS, QK_E_ARG, "sigma_partial: too few chunks");
-    hipLaunchKernelGGL(k_sigma_partial, dim3(nchunk, nact), dim3(RB), 0, s, rhoT, w, e, Om, skip, eta, srg, flow, n,
-                       nC, nO, nR, nS, part);
+    if (rho_f32)
+        hipLaunchKernelGGL(k_sigma_partial<float>, dim3(nchunk, nact), dim3(RB), 0, s,
+                           static_cast<const float*>(rhoT), w, e, Om, skip, eta, srg, flow, n, nC, nO, nR, nS, part);
+    else
+        hipLaunchKernelGGL(k_sigma_partial<double>, dim3(nchunk, nact), dim3(RB), 0, s,
+                           static_cast<const double*>(rhoT), w, e, Om, skip, eta, srg, flow, n, nC, nO, nR, nS, part);
     QK_HIP(hipGetLastError());
 }
 void sigma_finalize(hipStream_t s, const double* part, int nchunk, int n, int nC, int nO, int nR, double* Sig,

@@ -13,7 +13,8 @@ void rpa_build(hipStream_t s, const double* e, const double* K, double* dE, doub
                int nC, int nO, int v, int nS, int tda);
 void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w);
 int sigma_chunks(int nS, int nact);
-void sigma_partial(hipStream_t s, const double* rhoT, const double* w, const double* e, const double* Om,
+// rhoT holds doubles, or floats when rho_f32
+void sigma_partial(hipStream_t s, const void* rhoT, bool rho_f32, const double* w, const double* e, const double* Om,
                    const int* skip, double eta, int srg, double flow, int n, int nC, int nO, int nR, int nS,
                    int nchunk, double* part);
 void sigma_finalize(hipStream_t s, const double* part, int nchunk, int n, int nC, int nO, int nR, double* Sig,

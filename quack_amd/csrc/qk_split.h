@@ -22,5 +22,8 @@ void split3(hipStream_t s, const double* src, long long ld, bool k_contiguous, i
 // C[i + ldc*j] = sum_k A(i, k) B(j, k), FP64 result from BF16 slice products.
 // Returns the number of kernel launches it issued (more than one while tuning).
 int gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, double* C, long long ldc);
+// The same product rounded to FP32 on the store (for consumers that need only
+// the FP32-level accuracy of the slicing).
+int gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, float* C, long long ldc);
 
 }  // namespace qk

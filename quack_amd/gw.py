@@ -29,7 +29,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats",
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
-           "qk_gwb_stage_times", "qk_set_autotune", "qk_autotune_report"]
+           "qk_gwb_stage_times", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs"]
 
 
 class QKError(RuntimeError):
@@ -92,6 +92,7 @@ def load():
     lib.qk_gwb_g0w0.argtypes = [_H, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_gwb_stage_times.argtypes = [_H, _P, _I]
     lib.qk_set_autotune.argtypes = [_I]
+    lib.qk_set_graphs.argtypes = [_I]
     lib.qk_autotune_report.argtypes = [ctypes.c_char_p, _I]
     lib.qk_gw_set_eri_df_device.argtypes = [_H, _H, _I]
     lib.qk_gw_eig_stats.argtypes = [_H, ctypes.POINTER(ctypes.c_longlong), _I]
@@ -260,6 +261,12 @@ class GWEngine:
 def set_autotune(on):
     """Process-wide GEMM tile autotuner (on by default; identical results either way)."""
     load().qk_set_autotune(int(bool(on)))
+
+
+def set_graphs(on):
+    """Process-wide hipGraph replay of the own full eigensolve (off by default;
+    identical results either way)."""
+    load().qk_set_graphs(int(bool(on)))
 
 
 def autotune_report():
