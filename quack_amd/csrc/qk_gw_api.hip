@@ -633,10 +633,15 @@ bool refine_eig(Ctx* c) {
         if (eig_trace())
             std::fprintf(stderr, "refine: n %d step %d max|E| %.3e clusters %d largest %d\n", n, it, emax, cl.ncl,
                          cl.biggest);
+        ok = emax < EIG_TOL;
+        // The last correction is below 1e-8.  The eigenvalues S_ii/G_ii of this
+        // step are already exact to O(1e-16); in the split-precision form the
+        // vectors are used only through the FP32-level contraction, so the
+        // final update X (I + E) is skipped there.
+        if (ok && c->precision == 1) break;
         eig_update(s, c->Sm, c->Gm, c->lam, d_lab, n);
         dgemm(s, false, false, n, n, n, 1.0, X, n, c->Sm, n, 0.0, Y, n);
         std::swap(X, Y);
-        ok = emax < EIG_TOL;
     }
     c->Xv = X;
     c->Ym = Y;
