@@ -467,6 +467,21 @@ __global__ __launch_bounds__(TBW) void k_finish_update(
 
 __global__ void k_d0(const double* __restrict__ A, double* __restrict__ D) { D[0] = A[0]; }
 
+// The panel's reflector columns V = A(0:m, c_lo .. c_lo+nbp-1) copied on both
+// sides of W in the work array, so that [V W] and [W V] are column blocks of
+// one array: A -= [V W] [W V]^T = V W^T + W V^T is a single rank-2nbp update.
+__global__ void k_copy_v(const double* __restrict__ V, int n, int m, int nbp, double* __restrict__ left,
+                         double* __restrict__ right) {
+    const long long total = (long long)m * nbp;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(t % m), k = (int)(t / m);
+        const double v = V[r + (long long)n * k];
+        left[r + (long long)n * k] = v;
+        right[r + (long long)n * k] = v;
+    }
+}
+
 
 // ---- back-transformation Z <- Q Z (dormtr, SIDE = 'L', UPLO = 'U', TRANS = 'N') ----
 // Q = H(n-1) ... H(1); reflector j (0-based, column c = j + 1) is v_j with
@@ -531,7 +546,7 @@ size_t sytrd_work_doubles(int n, int nb) {
     const size_t ntiles = (size_t)nt * (nt + 1) / 2;
     const size_t ng = groups_before(nt);
     const size_t nblk = (n + 63) / 64;
-    return (size_t)n * nb + (ntiles + ng) * TS + ng + 2 * (2 * (size_t)nb * nblk) + nblk + 8;
+    return 3 * (size_t)n * nb + (ntiles + ng) * TS + ng + 2 * (2 * (size_t)nb * nblk) + nblk + 8;
 }
 
 void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* TAU, double* work, int nb) {
@@ -540,8 +555,10 @@ void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* 
     const size_t ntiles_all = (size_t)nt_all * (nt_all + 1) / 2;
     const size_t ng_all = groups_before(nt_all);
     const int nblk_all = (n + 63) / 64;
-    double* W = work;
-    double* prow = W + (size_t)n * nb;
+    // work: P = [V copy | W | V copy] (n x 3nb), then the partial sums
+    double* P = work;
+    double* W = P + (size_t)n * nb;
+    double* prow = P + 3 * (size_t)n * nb;
     double* pcol = prow + ntiles_all * TS;
     double* qpart = pcol + ng_all * TS;
     double* tpart0 = qpart + ng_all;  // two buffers of [2 nb][nblk_all], by column parity
@@ -573,14 +590,20 @@ void sytrd_upper(hipStream_t s, int n, double* A, double* D, double* E, double* 
                                TAU);
             QK_HIP(hipGetLastError());
         }
-        // trailing update A -= V W^T + W V^T on the upper tiles
+        // trailing update A -= V W^T + W V^T on the upper tiles, as one
+        // product of K = 2 nbp: [V W] = P(:, nb-nbp .. nb+nbp), [W V] = P(:, nb .. nb+2nbp)
         const int m = c_lo;  // rows/cols 0..c_lo-1 remain
         if (m > 0) {
             Batch up;
             up.upper = 1;
             const double* V = A + (long long)n * c_lo;
-            dgemm(s, false, true, m, m, nbp, -1.0, V, n, W, ldw, 1.0, A, n, up);
-            dgemm(s, false, true, m, m, nbp, -1.0, W, ldw, V, n, 1.0, A, n, up);
+            double* left = P + (long long)n * (nb - nbp);
+            double* right = P + (long long)n * (nb + nbp);
+            const long long tot = (long long)m * nbp;
+            hipLaunchKernelGGL(k_copy_v, dim3((unsigned)std::min<long long>((tot + 255) / 256, 4096)), dim3(256), 0,
+                               s, V, n, m, nbp, left, right);
+            QK_HIP(hipGetLastError());
+            dgemm(s, false, true, m, m, 2 * nbp, -1.0, left, n, W, n, 1.0, A, n, up);
         }
         c_hi = c_lo - 1;
     }
