@@ -162,6 +162,9 @@ def main():
                     help="syevd: dsyevd every cycle (QuAcK); refine: Newton refinement of the previous eigenvectors, "
                          "dsyevd for the rest; sytrd: full solves through the project's own tridiagonalisation; "
                          "refine+sytrd: refinement, own-tridiagonalisation full solves for the rest")
+    ap.add_argument("--de-max", type=float, default=0.01,
+                    help="refinement is tried when no orbital energy moved by more than this (Ha) since the stored "
+                         "eigenvectors")
     ap.add_argument("--concurrent", type=int, default=3,
                     help="molecules in flight at once (one engine, stream and host thread each); a step is one "
                          "such batch")
@@ -227,7 +230,7 @@ def main():
 
     for g in engs:
         g.set_precision(args.form)
-        g.set_eigensolver(args.eig, 0.05)
+        g.set_eigensolver(args.eig, args.de_max)
     # first call of every GEMM shape in this form: the tile autotuner's trials
     step_on(eng)
     # one molecule alone: per-stage device times and the dominant kernel's solo launch time
@@ -341,7 +344,8 @@ def main():
                    "nBas": nBas, "nOrb": nOrb, "nO": nO, "nS": nS, "method": args.method,
                    "thresh": 1e-6 if args.method == "evgw" else None, "max_diis": args.max_diis, "eta": 0.0,
                    "srg_flow": 500.0 if reg else None, "cycles_per_molecule": r.get("nSCF", 1), "form": args.form,
-                   "eigensolver": args.eig, "molecules_in_flight": K, "step": "%d molecules" % K},
+                   "eigensolver": args.eig, "refine_de_max": args.de_max, "molecules_in_flight": K,
+                   "step": "%d molecules" % K},
         "plain_fp64_form": {"molecules_per_s": 1.0 / t_fp64, "cycles": ref.get("nSCF"),
                             "eigensolver": "dsyevd every cycle", "speedup_of_this_form": value * t_fp64},
         "plain_fp64_form_own_tridiagonal": {"molecules_per_s": 1.0 / t_fp64_own,
@@ -519,7 +523,7 @@ def learned_form(eng, scf, nO, ref, args):
         return None
     ws = learned.WarmStart(learned.DEFAULT_MODEL)
     sysd = {"eHF": scf["eHF"], "Sx": scf["Sx"], "nO": nO}
-    eng.set_eigensolver(args.eig, 0.05)
+    eng.set_eigensolver(args.eig, args.de_max)
     t = time.time()
     r = ws.evgw(eng, sysd, doSRG=True, thresh=1e-6)
     dt = time.time() - t

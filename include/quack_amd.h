@@ -133,8 +133,10 @@ int qk_gw_eig_stats(qk_gw_handle h, long long *counts, int n);
  * [7] their number (tuning trials of the autotuner included);
  * [8], [9] the same two summed over every call since the handle was created;
  * [10] device time of the own Householder tridiagonalisations (eigensolver
- * modes 2 and 3) since the handle was created, [11] their number.
- * n <= 12. */
+ * modes 2 and 3) since the handle was created, [11] their number;
+ * [12], [13] device time of the tridiagonal divide-and-conquer (rocSOLVER
+ * dstedc) and of the own back-transformation in those solves with vectors.
+ * n <= 14. */
 int qk_gw_stage_times(qk_gw_handle h, double *ms, int n);
 
 /* ---- per-routine entry points (host in, host out) ---------------------- */

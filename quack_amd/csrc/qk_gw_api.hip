@@ -144,10 +144,12 @@ void syevd_own(hipStream_t s, rocblas_handle rb, bool vectors, int n, double*& A
     if (tev) QK_HIP(hipEventRecord(tev[1], s));
     if (vectors) {
         rb_check(rocsolver_dstedc(rb, rocblas_evect_tridiagonal, n, w, E, Z, n, dinfo), "rocsolver_dstedc");
+        if (tev) QK_HIP(hipEventRecord(tev[2], s));
         double* Zin = Z;
         auto orm = [&] { ormtr_upper(s, n, Ain, tau, Zin, owork); };
         if (gr) run_graph(s, gr->orm, Ain, orm);
         else orm();
+        if (tev) QK_HIP(hipEventRecord(tev[3], s));
         std::swap(A, Z);
     } else {
         // eigenvalues only: divide-and-conquer into the spare matrix (rocSOLVER's
@@ -297,8 +299,9 @@ struct qk_gw_ctx {
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t ev2[2] = {nullptr, nullptr};
     double t_gemm_total = 0.0, n_gemm_total = 0.0;  // excitation GEMM ms and launches since creation
-    hipEvent_t ev3[2] = {nullptr, nullptr};
+    hipEvent_t ev3[4] = {nullptr, nullptr, nullptr, nullptr};
     double t_trd_total = 0.0, n_trd_total = 0.0;    // own tridiagonalisations: ms and count since creation
+    double t_stedc_total = 0.0, t_ormtr_total = 0.0; // their dstedc and back-transformation (vector solves)
     bool shared = false;  // stream, rocBLAS handle and the batch slots belong to a qk_gwb handle
     std::string err;
 };
@@ -359,7 +362,12 @@ void free_split(Split3& sp) {
 }
 
 constexpr int EIG_MAXIT = 8;       // Newton steps before falling back to dsyevd
-constexpr double EIG_BIG = 0.25;   // |E_ij| above this: outside the convergence region
+// |E_ij| above this: the pair is outside the Newton step's convergence region
+// and joins a cluster.  Measured at nS = 5600: 0.25 let one evGW cycle wander
+// through 8 steps with corrections near 0.25; at 0.1 that cycle's pairs form a
+// cluster too large for EIG_KMAX and it takes a full solve instead, and the
+// later cycles converge in 3-5 steps.
+constexpr double EIG_BIG = 0.1;
 constexpr double EIG_TOL = 1e-8;   // last step: the next correction would be O(1e-16)
 constexpr double EIG_DEG = 1e-9;   // eigenvalues closer than this (relative) form a cluster
 constexpr int EIG_KMAX = 32;       // largest cluster handled
@@ -704,6 +712,13 @@ void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors)
                 QK_HIP(hipEventElapsedTime(&ms, c->ev3[0], c->ev3[1]));
                 c->t_trd_total += ms;
                 c->n_trd_total += 1.0;
+                if (vectors) {
+                    QK_HIP(hipEventSynchronize(c->ev3[3]));
+                    QK_HIP(hipEventElapsedTime(&ms, c->ev3[1], c->ev3[2]));
+                    c->t_stedc_total += ms;
+                    QK_HIP(hipEventElapsedTime(&ms, c->ev3[2], c->ev3[3]));
+                    c->t_ormtr_total += ms;
+                }
                 ++c->eig_count[8];
             } else {
                 syevd(c->s, c->rb, vectors, c->nS, c->Cm, c->Om, c->work, c->info);
@@ -1009,7 +1024,8 @@ int qk_gw_create(qk_gw_handle* h, int nBas, int nOrb, int nC, int nO, int nV, in
         QK_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
         rb_check(rocblas_create_handle(&c->rb), "rocblas_create_handle");
         rb_check(rocblas_set_stream(c->rb, c->s), "rocblas_set_stream");
-        for (auto* e : {&c->ev[0], &c->ev[1], &c->ev2[0], &c->ev2[1], &c->ev3[0], &c->ev3[1]}) QK_HIP(hipEventCreate(e));
+        for (auto* e : {&c->ev[0], &c->ev[1], &c->ev2[0], &c->ev2[1], &c->ev3[0], &c->ev3[1], &c->ev3[2], &c->ev3[3]})
+            QK_HIP(hipEventCreate(e));
         size_t n = nOrb, S = nS;
         c->K = dalloc<double>(S * S);
         c->Cm = dalloc<double>(S * S);
@@ -1230,9 +1246,10 @@ int qk_gw_eig_stats(qk_gw_handle c, long long* counts, int n) {
 }
 
 int qk_gw_stage_times(qk_gw_handle c, double* ms, int n) {
-    if (!c || !ms || n < 0 || n > 12) return QK_E_ARG;
-    double all[12] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4],
-                      c->t_stage[5], c->t_stage[6], c->t_gemm_total, c->n_gemm_total, c->t_trd_total, c->n_trd_total};
+    if (!c || !ms || n < 0 || n > 14) return QK_E_ARG;
+    double all[14] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4],
+                      c->t_stage[5], c->t_stage[6], c->t_gemm_total, c->n_gemm_total, c->t_trd_total, c->n_trd_total,
+                      c->t_stedc_total, c->t_ormtr_total};
     for (int i = 0; i < n; ++i) ms[i] = all[i];
     return QK_OK;
 }

@@ -251,11 +251,12 @@ class GWEngine:
         return Sig, dSig
 
     def stage_times(self):
-        ms = np.zeros(12)
-        _check(load().qk_gw_stage_times(self.h, _p(ms), 12), self.h)
+        ms = np.zeros(14)
+        _check(load().qk_gw_stage_times(self.h, _p(ms), 14), self.h)
         return dict(zip(["aotomo", "rpa_build", "eigensolve", "excitation", "sigma_qp", "total", "excitation_gemm",
                          "excitation_gemm_launches", "excitation_gemm_all_calls", "excitation_gemm_launches_all_calls",
-                         "tridiagonalisation_all_calls", "tridiagonalisations_all_calls"], ms))
+                         "tridiagonalisation_all_calls", "tridiagonalisations_all_calls", "stedc_all_calls",
+                         "back_transformation_all_calls"], ms))
 
 
 def set_autotune(on):
