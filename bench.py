@@ -162,7 +162,7 @@ def main():
                     help="syevd: dsyevd every cycle (QuAcK); refine: Newton refinement of the previous eigenvectors, "
                          "dsyevd for the rest; sytrd: full solves through the project's own tridiagonalisation; "
                          "refine+sytrd: refinement, own-tridiagonalisation full solves for the rest")
-    ap.add_argument("--de-max", type=float, default=0.01,
+    ap.add_argument("--de-max", type=float, default=0.003,
                     help="refinement is tried when no orbital energy moved by more than this (Ha) since the stored "
                          "eigenvectors")
     ap.add_argument("--concurrent", type=int, default=3,
