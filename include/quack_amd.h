@@ -52,6 +52,13 @@ int qk_gw_set_eri_ao(qk_gw_handle h, const double *ERI_AO, const double *c);
 /* Same with ERI_AO and c already in device memory (the handle keeps no
  * reference to them after return). */
 int qk_gw_set_eri_ao_device(qk_gw_handle h, const double *ERI_AO_device, const double *c_device);
+/* The same from the 8-fold packed AO integrals that PyDuck writes to
+ * int/ERI_chem.bin (chemists' notation, pyscf's s8 order) and QuAcK's HPC
+ * branch reads (utils/read_2e_integrals.f90:85-107; element (ab|cd) at the
+ * Yoshimine index of utils/utils.f90:1186-1217).  n_packed must be
+ * p(p+1)/2 with p = nBas(nBas+1)/2.  Only the packed array crosses to the
+ * device (an eighth of nBas^4); it is unpacked there. */
+int qk_gw_set_eri_ao_packed(qk_gw_handle h, const double *ERI_chem, long long n_packed, const double *c);
 /* MO integrals as QuAcK passes them to RG0W0/evRGW (GW/RGW.f90:85-86,102-103):
  * ERI_MO is nOrb^4; only the (pq|jb) block is copied to the device. */
 int qk_gw_set_eri_mo(qk_gw_handle h, const double *ERI_MO);

@@ -1102,6 +1102,45 @@ int qk_gw_set_eri_ao(qk_gw_handle c, const double* ERI_AO, const double* cc) {
     });
 }
 
+int qk_gw_set_eri_ao_packed(qk_gw_handle c, const double* ERI_chem, long long n_packed, const double* cc) {
+    if (!c || !ERI_chem || !cc) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        const long long N = c->nBas, P = N * (N + 1) / 2;
+        QK_CHECK(n_packed == P * (P + 1) / 2, QK_E_ARG,
+                 "packed ERI: expected nBas(nBas+1)/2 pairs packed to " + std::to_string(P * (P + 1) / 2) +
+                     " elements, got " + std::to_string(n_packed));
+        double *dp = nullptr, *df = nullptr, *dc = nullptr;
+        try {
+            ensure_m(c);
+            dp = dalloc<double>((size_t)n_packed);
+            h2d(c, dp, ERI_chem, (size_t)n_packed);
+            df = dalloc<double>((size_t)(N * N * N * N));
+            unpack_eri_s8(c->s, dp, df, (int)N);
+            QK_HIP(hipStreamSynchronize(c->s));
+            dfree(dp);
+            dp = nullptr;
+            dc = dalloc<double>((size_t)N * c->nOrb);
+            h2d(c, dc, cc, (size_t)N * c->nOrb);
+            QK_HIP(hipEventRecord(c->ev[0], c->s));
+            aotomo_block(c->s, df, dc, c->M, c->nBas, c->nOrb, c->nC, c->nO, c->nR);
+            QK_HIP(hipEventRecord(c->ev[1], c->s));
+            QK_HIP(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            QK_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            c->t_ao = ms;
+            after_eri(c);
+        } catch (...) {
+            (void)hipStreamSynchronize(c->s);
+            dfree(dp);
+            dfree(df);
+            dfree(dc);
+            throw;
+        }
+        dfree(df);
+        dfree(dc);
+    });
+}
+
 int qk_gw_set_eri_mo(qk_gw_handle c, const double* ERI_MO) {
     if (!c || !ERI_MO) return QK_E_ARG;
     return guarded(&c->err, [&] {

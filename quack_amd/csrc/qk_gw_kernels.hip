@@ -591,6 +591,28 @@ inline int grid_for(long long total, int bs = 256) {
 
 // ---- host wrappers --------------------------------------------------------
 
+// Full AO array in QuAcK's ERI.bin order, f[mu + N la + N^2 nu + N^3 si] =
+// (mu nu|la si), from the 8-fold packed chemists' array of int/ERI_chem.bin
+// (utils/read_2e_integrals.f90:85-107): element (ab|cd) sits at
+// pair(pair(a, b), pair(c, d)), pair(x, y) = x (x + 1) / 2 + y for x >= y
+// (0-based; the 1-based Yoshimine index of utils/utils.f90:1186-1217 minus one).
+__global__ void k_unpack_s8(const double* __restrict__ packed, double* __restrict__ f, int N) {
+    const long long total = (long long)N * N * N * N;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long mu = t % N, la = (t / N) % N, nu = (t / ((long long)N * N)) % N,
+                        si = t / ((long long)N * N * N);
+        const long long ij = mu >= nu ? mu * (mu + 1) / 2 + nu : nu * (nu + 1) / 2 + mu;
+        const long long kl = la >= si ? la * (la + 1) / 2 + si : si * (si + 1) / 2 + la;
+        f[t] = packed[ij >= kl ? ij * (ij + 1) / 2 + kl : kl * (kl + 1) / 2 + ij];
+    }
+}
+
+void unpack_eri_s8(hipStream_t s, const double* packed, double* f, int N) {
+    hipLaunchKernelGGL(k_unpack_s8, dim3(grid_for((long long)N * N * N * N)), dim3(256), 0, s, packed, f, N);
+    QK_HIP(hipGetLastError());
+}
+
 void permute_block(hipStream_t s, const double* X, double* M, int n, int o, int v) {
     hipLaunchKernelGGL(k_permute_block, dim3(grid_for((long long)n * n * o * v)), dim3(256), 0, s, X, M, n, o, v);
     QK_HIP(hipGetLastError());

@@ -7,6 +7,9 @@
 namespace qk {
 
 void permute_block(hipStream_t s, const double* X, double* M, int n, int o, int v);
+// f (N^4, QuAcK's ERI.bin order) from the 8-fold packed chemists' array
+// (N(N+1)/2 pairs, pair-of-pairs packed)
+void unpack_eri_s8(hipStream_t s, const double* packed, double* f, int N);
 void extract_k(hipStream_t s, const double* M, double* K, int n, int nC, int nO, int v, int nS);
 // scal[0] = Tr(A), scal[1] = EcRPA, scal[2] = EcGM
 void rpa_build(hipStream_t s, const double* e, const double* K, double* dE, double* sd, double* Cm, double* scal,

@@ -29,7 +29,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats",
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
-           "qk_gwb_stage_times", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs"]
+           "qk_gwb_stage_times", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed"]
 
 
 class QKError(RuntimeError):
@@ -72,6 +72,7 @@ def load():
     lib.qk_last_error.restype = ctypes.c_char_p
     lib.qk_gw_set_eri_ao.argtypes = [_H, _P, _P]
     lib.qk_gw_set_eri_ao_device.argtypes = [_H, _H, _H]
+    lib.qk_gw_set_eri_ao_packed.argtypes = [_H, _P, ctypes.c_longlong, _P]
     lib.qk_gw_set_eri_mo.argtypes = [_H, _P]
     lib.qk_rg0w0.argtypes = [_H, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_evrgw.argtypes = [_H, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
@@ -205,6 +206,14 @@ class GWEngine:
         c = _f64(c)
         assert c.shape == (self.nBas, self.nOrb)
         _check(load().qk_gw_set_eri_ao(self.h, _p(eri_ao), _p(c)), self.h)
+
+    def set_eri_ao_packed(self, eri_s8, c):
+        """AO integrals in the 8-fold packed chemists' form of int/ERI_chem.bin
+        (quackio.pack_eri_s8 writes one); unpacked on the device."""
+        eri_s8 = np.ascontiguousarray(eri_s8, dtype=np.float64).reshape(-1)
+        c = _f64(c)
+        assert c.shape == (self.nBas, self.nOrb)
+        _check(load().qk_gw_set_eri_ao_packed(self.h, _p(eri_s8), eri_s8.size, _p(c)), self.h)
 
     def set_eri_ao_device(self, eri_ptr, c_ptr):
         """Device pointers (e.g. torch tensor.data_ptr()) of ERI_AO and c (column-major)."""

@@ -100,6 +100,60 @@ def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=
         np.ascontiguousarray(eri_mo, dtype=np.float64).tofile(os.path.join(wd, "int", "ERI_MO.bin"))
 
 
+def s8_pairs(nBas):
+    """0-based (a, b), a >= b, in packed pair order: pair(a, b) = a (a + 1) / 2 + b."""
+    a, b = np.tril_indices(nBas)
+    return a, b
+
+
+def pack_eri_s8(eri, nBas):
+    """The 8-fold packed chemists' array of int/ERI_chem.bin (PyDuck.py:318-326,
+    pyscf's s8 order) from the full AO array in ERI.bin order
+    (flat index mu + N la + N^2 nu + N^3 si holds (mu nu|la si)): element
+    (ab|cd), a >= b, c >= d, pair(a, b) >= pair(c, d), sits at
+    pair(pair(a, b), pair(c, d)) -- the Yoshimine index of
+    utils/utils.f90:1186-1217 minus one."""
+    N = nBas
+    f = np.asarray(eri, dtype=np.float64).reshape(N, N, N, N)  # f[si, nu, la, mu] = (mu nu|la si)
+    a, b = s8_pairs(N)
+    P = a.size
+    i, j = np.tril_indices(P)  # pair indices ij >= kl
+    return np.ascontiguousarray(f[b[j], b[i], a[j], a[i]])
+
+
+def unpack_eri_s8(packed, nBas):
+    """The full AO array in ERI.bin order from the packed one (what the device
+    kernel k_unpack_s8 does), for host-side readers of int/ERI_chem.bin."""
+    N = nBas
+    mu, la, nu, si = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+
+    def pair(x, y):
+        hi, lo = np.maximum(x, y), np.minimum(x, y)
+        return hi * (hi + 1) // 2 + lo
+
+    idx = pair(pair(mu, nu), pair(la, si))
+    # axes (mu, la, nu, si) -> flat order mu + N la + N^2 nu + N^3 si
+    return np.ascontiguousarray(np.asarray(packed)[idx].transpose(3, 2, 1, 0)).reshape(-1)
+
+
+def write_eri_chem(wd, eri, nBas):
+    """int/ERI_chem.bin as PyDuck writes it (raw float64 stream)."""
+    os.makedirs(os.path.join(wd, "int"), exist_ok=True)
+    path = os.path.join(wd, "int", "ERI_chem.bin")
+    pack_eri_s8(eri, nBas).tofile(path)
+    return path
+
+
+def read_eri_chem(wd, nBas):
+    """The packed array of int/ERI_chem.bin (utils/read_2e_integrals.f90:85-107)."""
+    P = nBas * (nBas + 1) // 2
+    x = np.fromfile(os.path.join(wd, "int", "ERI_chem.bin"), dtype=np.float64)
+    if x.size != P * (P + 1) // 2:
+        raise ValueError("int/ERI_chem.bin holds %d values, expected %d for nBas = %d" % (x.size, P * (P + 1) // 2,
+                                                                                          nBas))
+    return x
+
+
 def read_test_dump(path):
     """Parse QuAcK's test dump (description line, value line), src/test/dump_test_value.f90:19-20."""
     out = {}
