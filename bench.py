@@ -55,10 +55,12 @@ def build_inputs(nx, ny, seed):
     mol = ig.h_lattice(nx, ny, seed=seed)
     t = time.time()
     S, T, V = mol.one_electron()
-    eri = mol.eri()
-    log("inputs: AO integrals nBas=%d in %.1f s" % (mol.nBas, time.time() - t))
+    eri_t = torch.empty(mol.nBas ** 4, dtype=torch.float64, device="cuda")
+    mol.eri_device(eri_t.data_ptr(), eri_t.numel())
+    torch.cuda.synchronize()
+    eri = None  # the AO integrals live only on the device
+    log("inputs: AO integrals nBas=%d generated on the device in %.2f s" % (mol.nBas, time.time() - t))
     t = time.time()
-    eri_t = torch.from_numpy(eri).to("cuda")
     scf = ig.rhf(S, T, V, eri, mol.nO, jk=ig.jk_torch(eri_t))
     log("inputs: RHF %d iterations, nOrb=%d, ERHF=%.10f, gap=%.4f Ha in %.1f s" %
         (scf["niter"], scf["nOrb"], scf["ERHF"], scf["eHF"][mol.nO] - scf["eHF"][mol.nO - 1], time.time() - t))

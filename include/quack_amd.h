@@ -59,6 +59,18 @@ int qk_gw_set_eri_ao_device(qk_gw_handle h, const double *ERI_AO_device, const d
  * p(p+1)/2 with p = nBas(nBas+1)/2.  Only the packed array crosses to the
  * device (an eighth of nBas^4); it is unpacked there. */
 int qk_gw_set_eri_ao_packed(qk_gw_handle h, const double *ERI_chem, long long n_packed, const double *c);
+/* Two-electron integrals of contracted s and p shells (e.g. cc-pVDZ hydrogen)
+ * generated on the device into eri_device (n_eri = nBas^4 doubles, nBas =
+ * sum of 2l+1), in the order QuAcK streams from int/ERI.bin
+ * (utils/read_2e_integrals.f90:55-63): eri[a + N b + N^2 c + N^3 d] = <ab|cd>.
+ * No counterpart in QuAcK: it stands in for PySCF's mol.intor('int2e') of
+ * PyDuck.py:329-345, with the conventions of the project's host producer
+ * (p functions x, y, z; contracted functions normalised to one).  Shell i has
+ * angular momentum sl[i] (0 or 1; QK_E_UNSUPPORTED otherwise), snp[i] <= 8
+ * primitives with exponents and raw contraction coefficients taken in order
+ * from exps/coefs, and sits on atom satom[i] at xyz[3*satom[i] .. +2] (bohr). */
+int qk_int_eri_device(int nshell, const int *sl, const int *snp, const int *satom, const double *exps,
+                      const double *coefs, const double *xyz, double *eri_device, long long n_eri);
 /* MO integrals as QuAcK passes them to RG0W0/evRGW (GW/RGW.f90:85-86,102-103):
  * ERI_MO is nOrb^4; only the (pq|jb) block is copied to the device. */
 int qk_gw_set_eri_mo(qk_gw_handle h, const double *ERI_MO);

@@ -32,15 +32,24 @@ def workload(nmol, seed=0, kmin=2, kmax=16):
 
 
 def prepare(mols, log=None):
-    """AO integrals (host producer) and RHF (Fock builds through torch on the
-    device); the AO integrals and MO coefficients stay on the device."""
+    """AO integrals (generated on the device for s/p bases such as the hydrogen
+    systems here, the host producer otherwise) and RHF (Fock builds through
+    torch on the device); the AO integrals and MO coefficients stay on the
+    device."""
     import torch
+
+    from . import gw
     out = []
     for i, mol in enumerate(mols):
         S, T, V = mol.one_electron()
-        eri = mol.eri()
-        eri_t = torch.from_numpy(eri).to("cuda")
-        scf = ig.rhf(S, T, V, eri, mol.nO, jk=ig.jk_torch(eri_t))
+        eri_t = torch.empty(mol.nBas ** 4, dtype=torch.float64, device="cuda")
+        try:
+            mol.eri_device(eri_t.data_ptr(), eri_t.numel())
+        except gw.QKError as ex:
+            if ex.code != gw.QK_E_UNSUPPORTED:
+                raise
+            eri_t.copy_(torch.from_numpy(mol.eri()))
+        scf = ig.rhf(S, T, V, None, mol.nO, jk=ig.jk_torch(eri_t))
         c_t = torch.from_numpy(np.asfortranarray(scf["c"]).ravel(order="F").copy()).to("cuda")
         out.append({"nBas": mol.nBas, "nOrb": scf["nOrb"], "nO": mol.nO, "eHF": scf["eHF"], "eri_t": eri_t,
                     "c_t": c_t})

@@ -29,7 +29,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats",
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
-           "qk_gwb_stage_times", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed"]
+           "qk_gwb_stage_times", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device"]
 
 
 class QKError(RuntimeError):
@@ -73,6 +73,8 @@ def load():
     lib.qk_gw_set_eri_ao.argtypes = [_H, _P, _P]
     lib.qk_gw_set_eri_ao_device.argtypes = [_H, _H, _H]
     lib.qk_gw_set_eri_ao_packed.argtypes = [_H, _P, ctypes.c_longlong, _P]
+    _IPs = ctypes.POINTER(ctypes.c_int)
+    lib.qk_int_eri_device.argtypes = [_I, _IPs, _IPs, _IPs, _P, _P, _P, ctypes.c_void_p, ctypes.c_longlong]
     lib.qk_gw_set_eri_mo.argtypes = [_H, _P]
     lib.qk_rg0w0.argtypes = [_H, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_evrgw.argtypes = [_H, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]

@@ -118,6 +118,21 @@ class Molecule:
         return out
 
 
+    def eri_device(self, ptr, count):
+        """The same array generated on the GPU (s and p shells only) into device
+        memory at ptr (count = nBas^4 doubles), by libquack_amd's
+        qk_int_eri_device; the host producer above stays the reference."""
+        from quack_amd import gw
+        lib = gw.load()
+        d, i = ctypes.c_double, ctypes.c_int
+        rc = lib.qk_int_eri_device(len(self.shell_l), _ptr(self.shell_l, i), _ptr(self.shell_np, i),
+                                   _ptr(self.shell_atom, i), _ptr(self.exps, d), _ptr(self.coefs, d),
+                                   _ptr(self.xyz, d), ctypes.c_void_p(ptr), ctypes.c_longlong(count))
+        if rc != 0:
+            raise gw.QKError(rc, lib.qk_last_error().decode() if rc != 5 else "s and p shells only")
+        return ptr
+
+
 def h_chain(natom, spacing=1.40, jitter=0.10, seed=0, basis="cc-pvdz"):
     """Linear H chain along z: atom k at spacing*k + U(-jitter, +jitter) bohr."""
     rng = np.random.default_rng(seed)
