@@ -49,7 +49,13 @@ def prepare(mols, log=None):
             if ex.code != gw.QK_E_UNSUPPORTED:
                 raise
             eri_t.copy_(torch.from_numpy(mol.eri()))
-        scf = ig.rhf(S, T, V, None, mol.nO, jk=ig.jk_torch(eri_t))
+        try:
+            scf = ig.rhf(S, T, V, None, mol.nO, maxit=600, jk=ig.jk_torch(eri_t))
+        except RuntimeError:
+            if log:
+                log("molecule %d: RHF did not converge in 600 iterations; left out" % i)
+            del eri_t
+            continue
         c_t = torch.from_numpy(np.asfortranarray(scf["c"]).ravel(order="F").copy()).to("cuda")
         out.append({"nBas": mol.nBas, "nOrb": scf["nOrb"], "nO": mol.nO, "eHF": scf["eHF"], "eri_t": eri_t,
                     "c_t": c_t})
