@@ -16,14 +16,16 @@ Forms: the plain FP64 form (QuAcK's arithmetic, dsyevd every cycle, one
 molecule at a time) is measured first as the accuracy reference and the
 baseline rate, then the same FP64 arithmetic with the project's own
 tridiagonalisation in every full solve; the timed form (default) runs the
-excitation densities in split precision, refines the previous cycle's
+excitation densities in split precision (two BF16 slices, three slice
+products: `--form mixed2`; `--form mixed` keeps three slices and six
+products), refines the previous cycle's
 eigenvectors where it can, does the remaining full solves through the own
 tridiagonalisation, and keeps three molecules in flight (one engine, stream
 and host thread each).  A step is one such batch of molecules.  Inputs (AO ERIs, MO coefficients, eHF) are
 produced before the timed region by the integral producer and an RHF whose Fock
 builds run through torch on the device.
 
-    python bench.py [--steps K] [--warmup W] [--form fp64|mixed] [--eig syevd|refine|sytrd|refine+sytrd]
+    python bench.py [--steps K] [--warmup W] [--form fp64|mixed|mixed2] [--eig syevd|refine|sytrd|refine+sytrd]
                     [--concurrent C] [--method evgw|g0w0] [--cpu-baseline auto|off]
                     [--learned auto|off] [--workload h50|df600|batch]
 
@@ -41,6 +43,10 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6       # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0      # GB/s
 PEAK_BF16 = 2500.0     # TFLOP/s, MI355X dense BF16 matrix
+# BF16 slice products per element product of the split-precision forms
+PRODUCTS = {"mixed": 6, "mixed2": 3}
+DTYPE = {"fp64": "f64", "mixed": "bf16x3-slices/f32-acc/f64 (contractions), f64 (eigensolve, QP)",
+         "mixed2": "bf16x2-slices/f32-acc/f64 (contractions), f64 (eigensolve, QP)"}
 
 
 def log(*a):
@@ -158,8 +164,9 @@ def main():
     ap.add_argument("--ny", type=int, default=10)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--max-diis", type=int, default=1)
-    ap.add_argument("--form", choices=["fp64", "mixed"], default="mixed",
-                    help="fp64: QuAcK's arithmetic; mixed: split-BF16 excitation densities")
+    ap.add_argument("--form", choices=["fp64", "mixed", "mixed2"], default="mixed2",
+                    help="fp64: QuAcK's arithmetic; mixed: excitation densities from three BF16 slices "
+                         "(six products); mixed2: two slices (three products)")
     ap.add_argument("--eig", choices=["syevd", "refine", "sytrd", "refine+sytrd"], default="refine+sytrd",
                     help="syevd: dsyevd every cycle (QuAcK); refine: Newton refinement of the previous eigenvectors, "
                          "dsyevd for the rest; sytrd: full solves through the project's own tridiagonalisation; "
@@ -273,8 +280,8 @@ def main():
     # one launch per screening cycle, timed with HIP events on each engine's own
     # stream over the timed region (launches of concurrent molecules overlap, so
     # each takes longer than alone; the solo figure is reported beside it).
-    # FP64 form: 2 nOrb^2 nS^2 flop on FP64 MFMA; mixed form: six BF16 slice
-    # products, 6 x 2 nOrb^2 nS^2 flop on BF16 MFMA.
+    # FP64 form: 2 nOrb^2 nS^2 flop on FP64 MFMA; split forms: six (mixed) or
+    # three (mixed2) BF16 slice products, 6 or 3 x 2 nOrb^2 nS^2 flop on BF16 MFMA.
     gemm_t = sum(b["excitation_gemm_all_calls"] - a["excitation_gemm_all_calls"] for a, b in zip(c0, c1))
     gemm_n = sum(b["excitation_gemm_launches_all_calls"] - a["excitation_gemm_launches_all_calls"] for a, b in zip(c0, c1))
     gemm_ms = gemm_t / max(1.0, gemm_n)
@@ -293,7 +300,9 @@ def main():
     if args.form == "fp64":
         flops, peak, kname = 2.0 * nOrb * nOrb * nS * nS, PEAK_FP64, "dgemm_mfma (excitation densities rho = M W)"
     else:
-        flops, peak, kname = 12.0 * nOrb * nOrb * nS * nS, PEAK_BF16, "k_gemm_bf16x3 (excitation densities, 6 BF16 slice products)"
+        npr = PRODUCTS[args.form]
+        flops, peak, kname = 2.0 * npr * nOrb * nOrb * nS * nS, PEAK_BF16, \
+            "k_gemm_bf16x3 (excitation densities, %d BF16 slice products)" % npr
     achieved = flops / (gemm_ms * 1e-3) / 1e12
     traffic = read_traffic(args.form)
     gemm_roof = {"kernel": kname, "bound": "mfma",
@@ -342,7 +351,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "bf16x3-slices/f32-acc/f64 (contractions), f64 (eigensolve, QP)",
+        "dtype": DTYPE[args.form],
         "data": "synthetic: seeded H50 lattice, integrals from the project's own Gaussian-integral producer",
         "config": {"workload": "H50 5x10 lattice cc-pVDZ %s%s, %s form, %s eigensolver, %d molecules in flight" %
                                ("SRG-" if reg else "", args.method, args.form, args.eig, K),
@@ -394,7 +403,7 @@ def run_df600(args):
     g = gw.GWEngine(n, n, nO)
     nS = g.nS
     res = {}
-    for form in ("fp64", "mixed"):
+    for form in ("fp64", "mixed", "mixed2"):
         g.set_precision(form)
         t = time.time()
         g.set_eri_df_device(Bt.data_ptr(), naux)
@@ -406,11 +415,16 @@ def run_df600(args):
         res[form] = (r, dt, st, t_k)
         log("df600 %s: (ia|jb) %.2f s, G0W0 %.1f s, stages %s" % (form, t_k, dt, {k: round(v, 1) for k, v in st.items()}))
     mev = 1e-3 / 27.211386245988
-    d = np.abs(res["mixed"][0]["eGW"] - res["fp64"][0]["eGW"])
-    dom = np.abs(res["mixed"][0]["Om"] - res["fp64"][0]["Om"])
     flops_rho = 2.0 * nS * n * n * naux
     forms = {}
-    for form, peak, mult in (("fp64", PEAK_FP64, 1.0), ("mixed", PEAK_BF16, 6.0)):
+    acc = {}
+    for form in ("mixed", "mixed2"):
+        d = np.abs(res[form][0]["eGW"] - res["fp64"][0]["eGW"])
+        dom = np.abs(res[form][0]["Om"] - res["fp64"][0]["Om"])
+        acc[form] = {"max_abs_dE_meV_all": float(d.max() / mev), "orbitals_over_1meV": int((d > mev).sum()),
+                     "max_abs_dOm_Ha": float(dom.max()), "HOMO_dE_meV": float(d[nO - 1] / mev),
+                     "LUMO_dE_meV": float(d[nO] / mev)}
+    for form, peak, mult in (("fp64", PEAK_FP64, 1.0), ("mixed", PEAK_BF16, 6.0), ("mixed2", PEAK_BF16, 3.0)):
         r, dt, st, t_k = res[form]
         ms = st["excitation_gemm"] / max(1.0, st["excitation_gemm_launches"])
         ach = mult * flops_rho / (ms * 1e-3) / 1e12
@@ -421,12 +435,11 @@ def run_df600(args):
            "value": res["fp64"][2]["excitation_gemm"] / max(1e-9, res["mixed"][2]["excitation_gemm"]),
            "unit": "x (rho GEMM speed-up, mixed over fp64)", "n_gpus": 1, "higher_is_better": True,
            "dtype": "f64 vs bf16x3-slices/f32-acc/f64", "data": "synthetic: seeded DF factors (dfgen)",
+           "mixed2_rho_gemm_speedup_vs_fp64": res["fp64"][2]["excitation_gemm"] / max(1e-9, res["mixed2"][2]["excitation_gemm"]),
            "config": {"workload": "synthetic DF N=600 o=60 naux=1800 SRG-G0W0", "nOrb": n, "nO": nO, "nS": nS,
                       "naux": naux, "decay": 40.0, "kmax_Ha": 0.1, "scale": scale, "seed": args.seed},
            "forms": forms,
-           "accuracy_mixed_vs_fp64": {"max_abs_dE_meV_all": float(d.max() / mev), "orbitals_over_1meV": int((d > mev).sum()),
-                                      "max_abs_dOm_Ha": float(dom.max()),
-                                      "HOMO_dE_meV": float(d[nO - 1] / mev), "LUMO_dE_meV": float(d[nO] / mev)}}
+           "accuracy_mixed_vs_fp64": acc["mixed"], "accuracy_mixed2_vs_fp64": acc["mixed2"]}
     print(json.dumps(out), flush=True)
     g.close()
 
@@ -509,7 +522,7 @@ def run_batch(args):
     d = np.array(d_all)
     out = {"metric": "batched G0W0 molecules/sec (uneven H chains/lattices)", "value": n_done / tb,
            "unit": "molecules/s", "n_gpus": 1, "steps": reps, "higher_is_better": True,
-           "dtype": "f64" if args.form == "fp64" else "bf16x3-slices/f32-acc/f64 (contractions), f64 (eigensolve, QP)",
+           "dtype": DTYPE[args.form],
            "data": "synthetic: seeded H chains and 2 x k lattices, integrals from the project's own producer",
            "config": {"workload": "%d seeded H_2k chains/lattices, k in [2, %d], cc-pVDZ, SRG-G0W0, %s batch" %
                                   (n_done, args.kmax, args.form), "nBas_range": [min(nbas_all), max(nbas_all)],

@@ -115,8 +115,13 @@ int qk_gw_sigma_at(qk_gw_handle h, const double *eG, const double *w, double eta
  * (GW/RGW_excitation_density.f90:48-50): 0 = FP64 MFMA (default, QuAcK's
  * arithmetic); 1 = split precision: each FP64 operand as three BF16 slices,
  * six slice products on BF16 matrix cores, FP32 accumulation over 256-term
- * chunks, FP64 recombination.  The RPA eigensolve and the QP Newton solve stay
- * FP64 in both modes.  No counterpart in QuAcK (which is FP64 throughout). */
+ * chunks, FP64 recombination (dropped terms O(2^-24) relative); 2 = the same
+ * with two BF16 slices and three slice products (dropped terms O(2^-16)
+ * relative, half the matrix-core work).  Modes 1 and 2 store rho in FP32.  The
+ * RPA eigensolve and the QP Newton solve stay FP64 in every mode; with
+ * density-fitted input the (ia|jb) product that builds the RPA matrix uses
+ * three slices in both split modes.  No counterpart in QuAcK (which is FP64
+ * throughout). */
 int qk_gw_set_precision(qk_gw_handle h, int mode);
 
 /* Eigensolver of the RPA problem inside qk_evrgw, qk_gw_screen and the first

@@ -6,7 +6,10 @@
 //   x = s0 + s1 + s2 + O(2^-24 |x|),   s0 = bf16(x), s1 = bf16(x - s0), s2 = bf16(x - s0 - s1)
 // and the product keeps the six slice products whose weights are >= 2^-16
 // relative: s0t0 + s0t1 + s1t0 + s0t2 + s1t1 + s2t0 (Ozaki-style slicing; the
-// dropped terms are O(2^-24)).  Each wave accumulates KC = 256 k-values in
+// dropped terms are O(2^-24)).  The two-slice form (x = s0 + s1 + O(2^-16 |x|))
+// keeps the three products of weight >= 2^-8, s0t0 + s0t1 + s1t0 (dropped
+// terms O(2^-16)): half the matrix-core work and two thirds of the operand
+// bytes.  Each wave accumulates KC = 256 k-values in
 // FP32 registers, then adds them to FP64 accumulators, so the FP32
 // accumulation error is bounded per chunk instead of over all of K.
 //
@@ -33,7 +36,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int KS = 32;    // k per MFMA
 constexpr int KC = 256;   // k per FP32 accumulation chunk
 
-// out_s[r * kpad + k] = slice s of src(r, k); src(r, k) = src[k + ld*r] (k contiguous)
+// out_s[r * kpad + k] = slice s of src(r, k); src(r, k) = src[k + ld*r] (k contiguous);
+// s2 == nullptr: two slices only
 __global__ void k_split3_kc(const double* __restrict__ src, long long ld, int rows, int K, int kpad,
                             __bf16* __restrict__ s0, __bf16* __restrict__ s1, __bf16* __restrict__ s2) {
     long long total = (long long)rows * kpad;
@@ -47,7 +51,7 @@ __global__ void k_split3_kc(const double* __restrict__ src, long long ld, int ro
         double z = y - (double)(float)b;
         s0[t] = a;
         s1[t] = b;
-        s2[t] = (__bf16)(float)z;
+        if (s2) s2[t] = (__bf16)(float)z;
     }
 }
 
@@ -72,7 +76,7 @@ __global__ void k_split3_rc(const double* __restrict__ src, long long ld, int ro
             long long o = (long long)r * kpad + k;
             s0[o] = a;
             s1[o] = b;
-            s2[o] = (__bf16)(float)z;
+            if (s2) s2[o] = (__bf16)(float)z;
         }
     }
 }
@@ -109,18 +113,23 @@ __device__ __forceinline__ void tile_of(int pid, int tm, int tn, int gm, int& mb
 // stored in slot c ^ ((r >> 1) & 3), so the matrix-core operand reads (16 rows
 // x one chunk per lane group) hit distinct banks.  Rows past M or N load row
 // 0: their products land only in outputs that are never stored.
-constexpr int SLAB_B = 2 * 3 * 128 * 64;  // A and B, three slices, 128 rows of 64 bytes
+// bytes of one slab stage: A and B, NSL slices each, 128 rows of 64 bytes
+template <int NSL>
+constexpr int slab_bytes() { return 2 * NSL * 128 * 64; }
 // s_waitcnt immediate for "vector-memory count <= n" (gfx9 layout), other counters left at their maximum
 #define QK_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | (7 << 4) | (15 << 8))
 
 // OutT: double, or float where the consumer needs only the FP32-level accuracy
 // the slicing gives (the FP64 recombination is then rounded once on the store).
-template <typename OutT>
+template <typename OutT, int NSL>
 __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, const __bf16* __restrict__ A0,
                                                          const __bf16* __restrict__ A1, const __bf16* __restrict__ A2,
                                                          const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
                                                          const __bf16* __restrict__ B2, OutT* __restrict__ C,
                                                          long long ldc, int tiles_m, int tiles_n, int gm) {
+    static_assert(NSL == 2 || NSL == 3, "two or three slices");
+    constexpr int SLAB_B = slab_bytes<NSL>();
+    constexpr int LPW = 4 * NSL;  // 1 KB slab loads per wave per slab
     __shared__ __attribute__((aligned(1024))) char st0[SLAB_B];
     __shared__ __attribute__((aligned(1024))) char st1[SLAB_B];
     __shared__ __attribute__((aligned(1024))) char st2[SLAB_B];
@@ -133,26 +142,26 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
     const int l16 = lane & 15;
     // operand reads: lane reads row (.. + l16), k-chunk lane/16, stored in slot
     const int rslot = ((lane >> 4) ^ ((l16 >> 1) & 3)) * 16;
-    // slab writes: wave instruction j of 12 fills the 1 KB block idx = 12 wave + j:
-    // operand idx / 24 (A, B), slice (idx % 24) / 8, rows 16 (idx % 8) .. +15;
+    // slab writes: wave instruction j of LPW fills the 1 KB block idx = LPW wave + j:
+    // operand idx / (8 NSL) (A, B), slice (idx % (8 NSL)) / 8, rows 16 (idx % 8) .. +15;
     // lane writes row 16 (idx % 8) + lane / 4, slot lane % 4 = chunk (lane % 4) ^ ((lane >> 3) & 3)
     const int wchunk = ((lane & 3) ^ ((lane >> 3) & 3)) * 8;
-    const __bf16* src[12];
-    int ldo[12];
+    const __bf16* src[LPW];
+    int ldo[LPW];
 #pragma unroll
-    for (int j = 0; j < 12; ++j) {
-        const int idx = wave * 12 + j;
-        const int op = idx / 24, sl = (idx % 24) / 8, g = idx % 8;
+    for (int j = 0; j < LPW; ++j) {
+        const int idx = wave * LPW + j;
+        const int op = idx / (8 * NSL), sl = (idx % (8 * NSL)) / 8, g = idx % 8;
         const int row = g * 16 + (lane >> 2);
         int gr = (op ? n0 : m0) + row;
         if (gr >= (op ? N : M)) gr = 0;
         const __bf16* base = op ? (sl == 0 ? B0 : sl == 1 ? B1 : B2) : (sl == 0 ? A0 : sl == 1 ? A1 : A2);
         src[j] = base + (long long)gr * kpad + wchunk;
-        ldo[j] = ((op * 3 + sl) * 128 + g * 16) * 64;
+        ldo[j] = ((op * NSL + sl) * 128 + g * 16) * 64;
     }
 #define QK_ISSUE(stage, k0)                                                                                  \
     do {                                                                                                     \
-        _Pragma("unroll") for (int j = 0; j < 12; ++j)                                                       \
+        _Pragma("unroll") for (int j = 0; j < LPW; ++j)                                                      \
             __builtin_amdgcn_global_load_lds((const void*)(src[j] + (k0)),                                   \
                                              (__attribute__((address_space(3))) void*)((stage) + ldo[j]), 16, 0, 0); \
     } while (0)
@@ -170,18 +179,21 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
 #define QK_FRAG(stage, opsl, row) (*reinterpret_cast<const bf16x8*>((stage) + ((opsl) * 128 + (row)) * 64 + rslot))
 #define QK_SLAB(stage, k0)                                                                                   \
     do {                                                                                                     \
-        bf16x8 fb[4][3];                                                                                     \
+        bf16x8 fb[4][NSL];                                                                                   \
         _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                        \
-            _Pragma("unroll") for (int sl = 0; sl < 3; ++sl) fb[b][sl] = QK_FRAG(stage, 3 + sl, wn + b * 16 + l16); \
+            _Pragma("unroll") for (int sl = 0; sl < NSL; ++sl)                                               \
+                fb[b][sl] = QK_FRAG(stage, NSL + sl, wn + b * 16 + l16);                                     \
         _Pragma("unroll") for (int a = 0; a < 4; ++a) {                                                      \
-            bf16x8 fa[3];                                                                                    \
-            _Pragma("unroll") for (int sl = 0; sl < 3; ++sl) fa[sl] = QK_FRAG(stage, sl, wm + a * 16 + l16); \
-            _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][2], fa[0], acc[a][b], 0, 0, 0);   \
-            _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[1], acc[a][b], 0, 0, 0);   \
-            _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[2], acc[a][b], 0, 0, 0);   \
+            bf16x8 fa[NSL];                                                                                  \
+            _Pragma("unroll") for (int sl = 0; sl < NSL; ++sl) fa[sl] = QK_FRAG(stage, sl, wm + a * 16 + l16); \
+            if constexpr (NSL == 3) {                                                                        \
+                _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                \
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][2], fa[0], acc[a][b], 0, 0, 0); \
+                _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                \
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[1], acc[a][b], 0, 0, 0); \
+                _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                \
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[2], acc[a][b], 0, 0, 0); \
+            }                                                                                                \
             _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][1], fa[0], acc[a][b], 0, 0, 0);   \
             _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                    \
@@ -208,7 +220,7 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
         __builtin_amdgcn_s_waitcnt(0xC07F); /* own LDS reads of nxt2 done */ \
         __builtin_amdgcn_s_barrier();                                      \
         QK_ISSUE(nxt2, min((s) + 2, nslab - 1) * KS);                      \
-        QK_VMCNT(24);                                                      \
+        QK_VMCNT(2 * LPW);                                                 \
         __builtin_amdgcn_s_barrier();                                      \
         if ((s) < nslab) QK_SLAB(cur, (s) * KS);                           \
     } while (0)
@@ -246,48 +258,55 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
 }  // namespace
 
 void split3(hipStream_t s, const double* src, long long ld, bool k_contiguous, int rows, int K, Split3& out) {
-    QK_CHECK(out.s[0] && out.rows == rows && out.kpad >= K && out.kpad % KS == 0, QK_E_ARG, "split3: bad target");
+    QK_CHECK(out.s[0] && out.s[1] && (out.nsl == 2 || (out.nsl == 3 && out.s[2])) && out.rows == rows &&
+                 out.kpad >= K && out.kpad % KS == 0,
+             QK_E_ARG, "split3: bad target");
+    __bf16* s2 = out.nsl == 3 ? out.s[2] : nullptr;
     if (k_contiguous) {
         hipLaunchKernelGGL(k_split3_kc, dim3(grid_for((long long)rows * out.kpad)), dim3(256), 0, s, src, ld, rows, K,
-                           out.kpad, out.s[0], out.s[1], out.s[2]);
+                           out.kpad, out.s[0], out.s[1], s2);
     } else {
         dim3 g((rows + 31) / 32, (out.kpad + 31) / 32);
         QK_CHECK(g.y <= 65535, QK_E_ARG, "split3: K too large");
-        hipLaunchKernelGGL(k_split3_rc, g, dim3(32, 8), 0, s, src, ld, rows, K, out.kpad, out.s[0], out.s[1],
-                           out.s[2]);
+        hipLaunchKernelGGL(k_split3_rc, g, dim3(32, 8), 0, s, src, ld, rows, K, out.kpad, out.s[0], out.s[1], s2);
     }
     QK_HIP(hipGetLastError());
 }
 
 namespace {
-// Autotuned group size of the tile order, per (M, N, kpad): the first call of a
+// Autotuned group size of the tile order, per (M, N, kpad, output type, slices): the first call of a
 // shape with at least 1e10 flop times each candidate once (HIP events on the
 // caller's stream); every candidate writes the same C.
 constexpr int NGM = 4;
 const int GM_CAND[NGM] = {2, 4, 8, 16};
 std::mutex g_mu;
-std::map<std::tuple<int, int, int, int>, std::pair<int, std::array<float, NGM>>> g_gm;
+std::map<std::tuple<int, int, int, int, int>, std::pair<int, std::array<float, NGM>>> g_gm;
 
 template <typename OutT>
 void launch_gm(hipStream_t s, int gm, int M, int N, const Split3& A, const Split3& B, OutT* C, long long ldc, int tm,
                int tn) {
-    hipLaunchKernelGGL(k_gemm_bf16x3<OutT>, dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1], A.s[2],
-                       B.s[0], B.s[1], B.s[2], C, ldc, tm, tn, gm);
+    if (A.nsl == 3)
+        hipLaunchKernelGGL((k_gemm_bf16x3<OutT, 3>), dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1],
+                           A.s[2], B.s[0], B.s[1], B.s[2], C, ldc, tm, tn, gm);
+    else
+        hipLaunchKernelGGL((k_gemm_bf16x3<OutT, 2>), dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1],
+                           (const __bf16*)nullptr, B.s[0], B.s[1], (const __bf16*)nullptr, C, ldc, tm, tn, gm);
     QK_HIP(hipGetLastError());
 }
 
 template <typename OutT>
 int gemm_bf16x3_t(hipStream_t s, int M, int N, const Split3& A, const Split3& B, OutT* C, long long ldc) {
     QK_CHECK(A.kpad == B.kpad && A.rows == M && B.rows == N, QK_E_ARG, "gemm_bf16x3: operand shapes");
+    QK_CHECK(A.nsl == B.nsl && (A.nsl == 2 || A.nsl == 3), QK_E_ARG, "gemm_bf16x3: slice counts");
     int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
     QK_CHECK((long long)tm * tn < (1LL << 31), QK_E_ARG, "gemm_bf16x3: grid too large");
-    const double flop = 12.0 * M * (double)N * A.kpad;
+    const double flop = 2.0 * (A.nsl == 3 ? 6 : 3) * M * (double)N * A.kpad;  // slice products
     if (!autotune_on() || flop < 1e10) {
         launch_gm(s, 8, M, N, A, B, C, ldc, tm, tn);
         return 1;
     }
     std::lock_guard<std::mutex> lock(g_mu);
-    auto key = std::make_tuple(M, N, A.kpad, (int)sizeof(OutT));
+    auto key = std::make_tuple(M, N, A.kpad, (int)sizeof(OutT), A.nsl);
     auto it = g_gm.find(key);
     if (it != g_gm.end()) {
         launch_gm(s, it->second.first, M, N, A, B, C, ldc, tm, tn);
@@ -335,8 +354,9 @@ std::string autotune_report_bf16() {
     std::string r;
     for (const auto& kv : g_gm) {
         char buf[256];
-        std::snprintf(buf, sizeof buf, "gemm_bf16x3 M=%d N=%d kpad=%d out=f%d -> group %d (ms:", std::get<0>(kv.first),
-                      std::get<1>(kv.first), std::get<2>(kv.first), 8 * std::get<3>(kv.first), kv.second.first);
+        std::snprintf(buf, sizeof buf, "gemm_bf16x3 M=%d N=%d kpad=%d out=f%d slices=%d -> group %d (ms:",
+                      std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), 8 * std::get<3>(kv.first),
+                      std::get<4>(kv.first), kv.second.first);
         r += buf;
         for (int c = 0; c < NGM; ++c) {
             std::snprintf(buf, sizeof buf, " g%d %.3f", GM_CAND[c], kv.second.second[c]);

@@ -265,7 +265,9 @@ struct qk_gw_ctx {
     int nchunk = 1;
     bool have_eri = false;
     bool have_screen = false;
-    int precision = 0;        // 0: FP64 MFMA, 1: BF16x3 split with FP64 recombination
+    // 0: FP64 MFMA; split precision with FP64 recombination: 1 three BF16 slices
+    // (six products), 2 two slices (three products) in the excitation densities
+    int precision = 0;
     qk::Split3 Msplit, Wsplit;  // M^T and W slices (mixed precision)
     bool msplit_valid = false;
     // density-fitted integrals (pq|rs) = sum_P B(P,pq) B(P,rs): B[P + naux (p + nOrb q)],
@@ -352,14 +354,17 @@ double d2h_scalar(Ctx* c, const double* src) {
     return v;
 }
 
-void alloc_split(Split3& sp, int rows, int K) {
+void alloc_split(Split3& sp, int rows, int K, int nsl = 3) {
     sp.rows = rows;
     sp.kpad = pad32(K);
-    for (auto& p : sp.s) p = dalloc<__bf16>((size_t)rows * sp.kpad);
+    sp.nsl = nsl;
+    for (int t = 0; t < nsl; ++t) sp.s[t] = dalloc<__bf16>((size_t)rows * sp.kpad);
 }
 void free_split(Split3& sp) {
     for (auto& p : sp.s) dfree(p);
 }
+// slices per operand of a split-precision mode (1: three, 2: two)
+int slices_of(int precision) { return precision == 2 ? 2 : 3; }
 
 constexpr int EIG_MAXIT = 8;       // Newton steps before falling back to dsyevd
 // |E_ij| above this: the pair is outside the Newton step's convergence region
@@ -646,7 +651,7 @@ bool refine_eig(Ctx* c) {
         // step are already exact to O(1e-16); in the split-precision form the
         // vectors are used only through the FP32-level contraction, so the
         // final update X (I + E) is skipped there.
-        if (ok && c->precision == 1) break;
+        if (ok && c->precision != 0) break;
         eig_update(s, c->Sm, c->Gm, c->lam, d_lab, n);
         dgemm(s, false, false, n, n, n, 1.0, X, n, c->Sm, n, 0.0, Y, n);
         std::swap(X, Y);
@@ -756,12 +761,14 @@ void excitation(Ctx* c) {
         excitation_df(c);
         return;
     }
-    if (c->precision == 1 && !c->Msplit.s[0]) {
-        alloc_split(c->Wsplit, c->nS, c->nS);
-        alloc_split(c->Msplit, (int)n2, c->nS);
+    if (c->precision != 0 && (!c->Msplit.s[0] || c->Msplit.nsl != slices_of(c->precision))) {
+        free_split(c->Wsplit);
+        free_split(c->Msplit);
+        alloc_split(c->Wsplit, c->nS, c->nS, slices_of(c->precision));
+        alloc_split(c->Msplit, (int)n2, c->nS, slices_of(c->precision));
         c->msplit_valid = false;
     }
-    c->rho_f32 = c->precision == 1;
+    c->rho_f32 = c->precision != 0;
     if (c->precision == 0) {
         Stage g(c, 5, true);
         count_gemm(c, dgemm(c->s, true, true, c->nS, (int)n2, c->nS, 1.0, c->Cm, c->nS, c->M, n2, 0.0, c->rhoT,
@@ -785,18 +792,25 @@ void excitation(Ctx* c) {
 void excitation_df(Ctx* c) {
     const int nS = c->nS, naux = c->naux;
     const long long n2 = (long long)c->nOrb * c->nOrb;
-    c->rho_f32 = c->precision == 1;
+    c->rho_f32 = c->precision != 0;
     if (c->precision == 0) {
         dgemm(c->s, false, false, naux, nS, nS, 1.0, c->Bov, naux, c->Cm, nS, 0.0, c->Tdf, naux);
         Stage g(c, 5, true);
         count_gemm(c, dgemm(c->s, true, false, nS, (int)n2, naux, 1.0, c->Tdf, naux, c->Bdf, naux, 0.0, c->rhoT, nS));
         return;
     }
-    if (!c->Wsplit.s[0]) alloc_split(c->Wsplit, nS, nS);
-    if (!c->Tsplit.s[0]) {
-        alloc_split(c->Tsplit, nS, naux);
-        alloc_split(c->BovTsplit, naux, nS);
-        alloc_split(c->Bsplit, (int)n2, naux);
+    const int nsl = slices_of(c->precision);
+    if (!c->Wsplit.s[0] || c->Wsplit.nsl != nsl) {
+        free_split(c->Wsplit);
+        alloc_split(c->Wsplit, nS, nS, nsl);
+    }
+    if (!c->Tsplit.s[0] || c->Tsplit.nsl != nsl) {
+        free_split(c->Tsplit);
+        free_split(c->BovTsplit);
+        free_split(c->Bsplit);
+        alloc_split(c->Tsplit, nS, naux, nsl);
+        alloc_split(c->BovTsplit, naux, nS, nsl);
+        alloc_split(c->Bsplit, (int)n2, naux, nsl);
         c->bsplit_valid = false;
     }
     if (!c->bsplit_valid) {
@@ -1244,7 +1258,7 @@ int qk_gw_sigma_at(qk_gw_handle c, const double* eG, const double* w, double eta
 }
 
 int qk_gw_set_precision(qk_gw_handle c, int mode) {
-    if (!c || mode < 0 || mode > 1) return QK_E_ARG;
+    if (!c || mode < 0 || mode > 2) return QK_E_ARG;
     return guarded(&c->err, [&] {
         c->precision = mode;  // slices are allocated on first use
     });
@@ -1774,7 +1788,7 @@ void excitation_batch(BCtx* b) {
     BStage st(b, 2);
     Ctx* c0 = b->mol[0];
     const long long n2 = (long long)c0->nOrb * c0->nOrb, nS = c0->nS;
-    for (Ctx* c : b->mol) c->rho_f32 = b->precision == 1;
+    for (Ctx* c : b->mol) c->rho_f32 = b->precision != 0;
     if (b->precision == 0) {
         Batch bt;
         bt.count = b->nmol;
@@ -1786,9 +1800,11 @@ void excitation_batch(BCtx* b) {
         return;
     }
     for (Ctx* c : b->mol) {
-        if (!c->Msplit.s[0]) {
-            alloc_split(c->Wsplit, c->nS, c->nS);
-            alloc_split(c->Msplit, (int)n2, c->nS);
+        if (!c->Msplit.s[0] || c->Msplit.nsl != slices_of(b->precision)) {
+            free_split(c->Wsplit);
+            free_split(c->Msplit);
+            alloc_split(c->Wsplit, c->nS, c->nS, slices_of(b->precision));
+            alloc_split(c->Msplit, (int)n2, c->nS, slices_of(b->precision));
             c->msplit_valid = false;
         }
         if (!c->msplit_valid) {
@@ -1896,7 +1912,7 @@ int qk_gwb_destroy(qk_gwb_handle h) {
 const char* qk_gwb_last_error(qk_gwb_handle h) { return h ? h->err.c_str() : g_last_error.c_str(); }
 
 int qk_gwb_set_precision(qk_gwb_handle h, int mode) {
-    if (!h || mode < 0 || mode > 1) return QK_E_ARG;
+    if (!h || mode < 0 || mode > 2) return QK_E_ARG;
     h->precision = mode;
     for (Ctx* c : h->mol) c->precision = mode;
     return QK_OK;

@@ -138,6 +138,11 @@ def _check(rc, handle=None):
         raise QKError(rc, msg.decode(errors="replace"))
 
 
+# names of the qk_gw_set_precision modes: FP64, three BF16 slices (six slice
+# products), two BF16 slices (three slice products) in the excitation densities
+PRECISION = {"fp64": 0, "mixed": 1, "mixed2": 2}
+
+
 def dims(nOrb, nO, nC=0, nR=0):
     nV = nOrb - nO
     return nC, nO, nV, nR, (nO - nC) * (nV - nR)
@@ -172,8 +177,10 @@ class GWEngine:
             pass
 
     def set_precision(self, mode):
-        """0: FP64 (QuAcK's arithmetic); 1: split-BF16 excitation densities ("mixed")."""
-        _check(load().qk_gw_set_precision(self.h, {"fp64": 0, "mixed": 1}.get(mode, mode)), self.h)
+        """0 / "fp64": QuAcK's FP64 arithmetic; 1 / "mixed": excitation densities
+        from three BF16 slices (six products); 2 / "mixed2": two slices (three
+        products)."""
+        _check(load().qk_gw_set_precision(self.h, PRECISION.get(mode, mode)), self.h)
 
     def set_eigensolver(self, mode, de_max=5e-3):
         """"syevd" (0): dsyevd for every RPA, QuAcK's scheme; "refine" (1):
@@ -325,7 +332,7 @@ class GWBatch:
             pass
 
     def set_precision(self, mode):
-        self._check(load().qk_gwb_set_precision(self.h, {"fp64": 0, "mixed": 1}.get(mode, mode)))
+        self._check(load().qk_gwb_set_precision(self.h, PRECISION.get(mode, mode)))
 
     def set_eri_ao_device(self, eri_ptr, stride_eri, c_ptr, stride_c):
         """Device pointers: molecule m's AO ERIs at eri_ptr + 8*m*stride_eri bytes

@@ -62,11 +62,12 @@ def test_batch_mixed_within_1meV(chains):
     eHF = np.stack([s["eHF"] for s in chains])
     out = {}
     with gw.GWBatch(3, s0["nBas"], s0["nOrb"], s0["nO"]) as b:
-        for form in ("fp64", "mixed"):
+        for form in ("fp64", "mixed", "mixed2"):
             b.set_precision(form)
             b.set_eri_ao_device(eri.data_ptr(), eri.shape[1], cc.data_ptr(), cc.shape[1])
             out[form] = b.g0w0(eHF, doSRG=True)
     assert np.abs(out["mixed"]["eGW"] - out["fp64"]["eGW"]).max() < MEV
+    assert np.abs(out["mixed2"]["eGW"] - out["fp64"]["eGW"]).max() < MEV
 
 
 def test_batch_mo_path_vs_oracle(chains):

@@ -71,16 +71,19 @@ def test_df_mixed_within_1meV():
     e, Bt, scale = dfgen.synthetic_df(n, nO, naux, seed=3, device="cuda")
     out = {}
     with gw.GWEngine(n, n, nO) as g:
-        for form in ("fp64", "mixed"):
+        for form in ("fp64", "mixed", "mixed2"):
             g.set_precision(form)
             g.set_eri_df_device(Bt.data_ptr(), naux)  # K is formed in the handle's precision
             out[form] = (g.g0w0(e, doSRG=True), g.evgw(e, max_diis=1, thresh=1e-6, doSRG=True))
     torch.cuda.synchronize()
-    for k in range(2):
-        d = np.abs(out["mixed"][k]["eGW"] - out["fp64"][k]["eGW"]).max()
-        say("form", k, "max |dE| meV", d / MEV)
-        assert d < MEV
-    assert np.abs(out["mixed"][0]["Om"] - out["fp64"][0]["Om"]).max() < 1e-6
+    for form in ("mixed", "mixed2"):
+        for k in range(2):
+            d = np.abs(out[form][k]["eGW"] - out["fp64"][k]["eGW"]).max()
+            say(form, k, "max |dE| meV", d / MEV)
+            assert d < MEV
+        # (ia|jb) uses three slices in both split forms: the same screening
+        assert np.abs(out[form][0]["Om"] - out["fp64"][0]["Om"]).max() < 1e-6
+    assert np.array_equal(out["mixed"][0]["Om"], out["mixed2"][0]["Om"])
 
 
 def test_df_device_layout_matches_host(small):

@@ -1,6 +1,8 @@
-"""Split-precision form (BF16 slices, FP32 accumulation, FP64 recombination)
+"""Split-precision forms (BF16 slices, FP32 accumulation, FP64 recombination)
 against the FP64 form and QuAcK's outputs: QP energies within 1 meV
-(3.675e-5 Ha), RPA energies within 1e-8 Ha where they do not depend on rho."""
+(3.675e-5 Ha), RPA energies within 1e-8 Ha where they do not depend on rho.
+"mixed" keeps three slices per operand (six products, dropped terms 2^-24
+relative), "mixed2" two slices (three products, dropped terms 2^-16)."""
 import os
 
 import numpy as np
@@ -30,6 +32,26 @@ def test_mixed_g0w0_within_1mev(systems, name):
         assert np.abs(mix["eGW"] - H10["g0w0_eGW"]).max() < MEV
 
 
+@pytest.mark.parametrize("name", ["He", "H2O", "H10"])
+def test_mixed2_g0w0_within_1mev(systems, name):
+    """Two-slice form against FP64 on the same handle: the SRG-regularised
+    graphical solve, and the unregularised linearised one (eHF + Z Sigma(eHF),
+    no root search).  SigC within 1e-4 relative (products to 2^-16)."""
+    from quack_amd import gw
+    print("mixed2 g0w0", name, flush=True)
+    s = systems[name]
+    out = {}
+    with gw.GWEngine(s["nBas"], s["nOrb"], s["nO"]) as g:
+        g.set_eri_ao(s["eri"], s["c"])
+        for form in ("fp64", "mixed2"):
+            g.set_precision(form)
+            out[form] = (g.g0w0(s["eHF"], doSRG=True), g.g0w0(s["eHF"], linearize=True))
+    for ref, mix in zip(out["fp64"], out["mixed2"]):
+        assert np.abs(mix["Om"] - ref["Om"]).max() < 1e-12
+        assert np.abs(mix["eGW"] - ref["eGW"]).max() < MEV
+        assert np.abs(mix["SigC"] - ref["SigC"]).max() < 1e-4 * max(1.0, np.abs(ref["SigC"]).max())
+
+
 def test_mixed_evgw_within_1mev(systems):
     from quack_amd import gw
     print("mixed evgw H10", flush=True)
@@ -40,3 +62,42 @@ def test_mixed_evgw_within_1mev(systems):
         r = g.evgw(s["eHF"], thresh=1e-6, max_diis=1)
     assert np.abs(r["eGW"] - H10["evgw_nodiis_eGW"]).max() < MEV
     assert abs(r["nSCF"] - int(H10["evgw_nodiis_nSCF"])) <= 1
+
+
+def test_mixed2_srg_evgw_within_1mev(systems):
+    """Two-slice form, SRG-regularised evGW (the method of the throughput
+    workload) against the FP64 form on the same handle: same cycle count."""
+    from quack_amd import gw
+    print("mixed2 srg evgw H10", flush=True)
+    s = systems["H10"]
+    out = {}
+    with gw.GWEngine(s["nBas"], s["nOrb"], s["nO"]) as g:
+        g.set_eri_ao(s["eri"], s["c"])
+        for form in ("fp64", "mixed2"):
+            g.set_precision(form)
+            out[form] = g.evgw(s["eHF"], thresh=1e-6, max_diis=1, doSRG=True)
+    assert np.abs(out["mixed2"]["eGW"] - out["fp64"]["eGW"]).max() < MEV
+    assert out["mixed2"]["nSCF"] == out["fp64"]["nSCF"]
+
+
+def test_precision_switch_reallocates_slices(systems):
+    """One handle switched between the split forms gives what a fresh handle
+    of each form gives (the slice buffers are re-made for the new count)."""
+    from quack_amd import gw
+    print("precision switch H10", flush=True)
+    s = systems["H10"]
+    fresh = {}
+    for form in ("mixed", "mixed2"):
+        with gw.GWEngine(s["nBas"], s["nOrb"], s["nO"]) as g:
+            g.set_eri_ao(s["eri"], s["c"])
+            g.set_precision(form)
+            fresh[form] = g.g0w0(s["eHF"], doSRG=True)
+    with gw.GWEngine(s["nBas"], s["nOrb"], s["nO"]) as g:
+        g.set_eri_ao(s["eri"], s["c"])
+        for form in ("mixed", "mixed2", "mixed"):
+            g.set_precision(form)
+            r = g.g0w0(s["eHF"], doSRG=True)
+            for k in ("eGW", "SigC", "Z"):
+                assert np.array_equal(r[k], fresh[form][k]), (form, k)
+    d = np.abs(fresh["mixed2"]["SigC"] - fresh["mixed"]["SigC"]).max()
+    assert 0.0 < d < 1e-4  # two slices differ from three, within the two-slice bound
