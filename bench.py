@@ -66,10 +66,14 @@ def build_inputs(nx, ny, seed):
     torch.cuda.synchronize()
     eri = None  # the AO integrals live only on the device
     log("inputs: AO integrals nBas=%d generated on the device in %.2f s" % (mol.nBas, time.time() - t))
+    from quack_amd import scf as qscf
     t = time.time()
-    scf = ig.rhf(S, T, V, eri, mol.nO, jk=ig.jk_torch(eri_t))
-    log("inputs: RHF %d iterations, nOrb=%d, ERHF=%.10f, gap=%.4f Ha in %.1f s" %
-        (scf["niter"], scf["nOrb"], scf["ERHF"], scf["eHF"][mol.nO] - scf["eHF"][mol.nO - 1], time.time() - t))
+    jk = qscf.jk_closure(mol.nBas, eri_t.data_ptr())  # HIP Fock builds over the resident AO integrals
+    scf = ig.rhf(S, T, V, eri, mol.nO, jk=jk)
+    log("inputs: RHF %d iterations, nOrb=%d, ERHF=%.10f, gap=%.4f Ha in %.2f s (last Fock build %.2f ms)" %
+        (scf["niter"], scf["nOrb"], scf["ERHF"], scf["eHF"][mol.nO] - scf["eHF"][mol.nO - 1], time.time() - t,
+         jk.engine.last_ms()))
+    jk.engine.close()
     c_t = torch.from_numpy(np.asfortranarray(scf["c"]).ravel(order="F").copy()).to("cuda")
     torch.cuda.synchronize()
     return mol, scf, eri, eri_t, c_t

@@ -255,6 +255,29 @@ int qk_mlp_train(qk_mlp_handle h, const float *X, const float *T, int n, int epo
                  unsigned long long seed, float *loss_per_epoch);
 const char *qk_mlp_last_error(qk_mlp_handle h);
 
+/* ---- restricted Hartree-Fock Fock builds (the SCF before the GW path) ----
+ * The Coulomb and exchange matrices of QuAcK's RHF loop (HF/RHF.f90:178-180)
+ * from the nBas^4 AO integrals resident on the device, in QuAcK's ERI.bin
+ * order (ERI(mu,la,nu,si) = <mu la|nu si> at mu + N la + N^2 nu + N^3 si).
+ * One handle per basis size; it owns the device copies of P, J, K and the
+ * partial sums.  Not re-entrant per handle; synchronous. */
+typedef struct qk_scf_ctx *qk_scf_handle;
+int qk_rhf_create(qk_scf_handle *h, int nBas); /* nBas <= 512 */
+int qk_rhf_destroy(qk_scf_handle h);
+const char *qk_rhf_last_error(qk_scf_handle h);
+/* Copy the host AO integrals (nBas^4, the ERI array QuAcK's RHF receives,
+ * HF/RHF.f90:1-2) to device memory owned by the handle, for callers without
+ * device memory of their own (Fortran). */
+int qk_rhf_set_eri_ao(qk_scf_handle h, const double *ERI_AO);
+/* J(mu,nu) = sum P(la,si) ERI(mu,la,nu,si)   (AOtoMO/Hartree_matrix_AO_basis.f90:22-32)
+ * K(mu,nu) = -sum P(la,si) ERI(mu,la,si,nu)  (AOtoMO/exchange_matrix_AO_basis.f90:22-32)
+ * P, J, K: host nBas x nBas, column-major; n_eri must be nBas^4.  ERI_device
+ * NULL uses the handle's copy from qk_rhf_set_eri_ao.  One pass over the
+ * integrals; the sums run in a fixed order (same result every call). */
+int qk_rhf_jk(qk_scf_handle h, const double *ERI_device, long long n_eri, const double *P, double *J, double *K);
+/* device time (ms, HIP events) of the last qk_rhf_jk's kernels */
+int qk_rhf_jk_ms(qk_scf_handle h, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,13 +33,14 @@ def workload(nmol, seed=0, kmin=2, kmax=16):
 
 def prepare(mols, log=None):
     """AO integrals (generated on the device for s/p bases such as the hydrogen
-    systems here, the host producer otherwise) and RHF (Fock builds through
-    torch on the device); the AO integrals and MO coefficients stay on the
-    device."""
+    systems here, the host producer otherwise) and RHF (Fock builds by the HIP
+    J/K kernel over the resident integrals); the AO integrals and MO
+    coefficients stay on the device."""
     import torch
 
-    from . import gw
+    from . import gw, scf as qscf
     out = []
+    engines = {}  # one J/K handle per basis size
     for i, mol in enumerate(mols):
         S, T, V = mol.one_electron()
         eri_t = torch.empty(mol.nBas ** 4, dtype=torch.float64, device="cuda")
@@ -49,8 +50,17 @@ def prepare(mols, log=None):
             if ex.code != gw.QK_E_UNSUPPORTED:
                 raise
             eri_t.copy_(torch.from_numpy(mol.eri()))
+        if mol.nBas not in engines:
+            engines[mol.nBas] = qscf.JK(mol.nBas)
+        eng, ptr = engines[mol.nBas], eri_t.data_ptr()
+
+        def jk(_eri, D):  # the producer's convention: F = H + 2 J - K for D = C_occ C_occ^T
+            J, K = eng(ptr, D)
+            return J, -K
         try:
-            scf = ig.rhf(S, T, V, None, mol.nO, maxit=600, jk=ig.jk_torch(eri_t))
+            scf = ig.rhf(S, T, V, None, mol.nO, maxit=600, jk=jk)
+        except gw.QKError:
+            raise
         except RuntimeError:
             if log:
                 log("molecule %d: RHF did not converge in 600 iterations; left out" % i)
@@ -61,6 +71,8 @@ def prepare(mols, log=None):
                     "c_t": c_t})
         if log and (i + 1) % 16 == 0:
             log("prepared %d/%d molecules" % (i + 1, len(mols)))
+    for eng in engines.values():
+        eng.close()
     return out
 
 

@@ -70,6 +70,32 @@ module quack_amd_gw
       integer(c_int), intent(out) :: nSCF
     end function
 
+    ! RHF Fock builds (HF/RHF.f90:178-179) on the device
+    integer(c_int) function qk_rhf_create(h, nBas) bind(C, name='qk_rhf_create')
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: h
+      integer(c_int), value :: nBas
+    end function
+
+    integer(c_int) function qk_rhf_destroy(h) bind(C, name='qk_rhf_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    integer(c_int) function qk_rhf_set_eri_ao(h, ERI_AO) bind(C, name='qk_rhf_set_eri_ao')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: ERI_AO(*)
+    end function
+
+    integer(c_int) function qk_rhf_jk(h, ERI_device, n_eri, P, J, K) bind(C, name='qk_rhf_jk')
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: h, ERI_device
+      integer(c_long_long), value :: n_eri
+      real(c_double), intent(in) :: P(*)
+      real(c_double), intent(out) :: J(*), K(*)
+    end function
+
   end interface
 
 contains

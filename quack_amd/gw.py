@@ -29,7 +29,8 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats",
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
-           "qk_gwb_stage_times", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device"]
+           "qk_gwb_stage_times", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
+           "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms"]
 
 
 class QKError(RuntimeError):
@@ -117,6 +118,13 @@ def load():
     lib.qk_rgw_excitation_density.argtypes = [_I] * 5 + [_P, _P, _P]
     lib.qk_rgw_self_energy_diag.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
     lib.qk_rgw_qp_graph.argtypes = [_I, _D, _D] + [_I] * 6 + [_P] * 7
+    lib.qk_rhf_create.argtypes = [ctypes.POINTER(_H), _I]
+    lib.qk_rhf_destroy.argtypes = [_H]
+    lib.qk_rhf_last_error.argtypes = [_H]
+    lib.qk_rhf_last_error.restype = ctypes.c_char_p
+    lib.qk_rhf_set_eri_ao.argtypes = [_H, _P]
+    lib.qk_rhf_jk.argtypes = [_H, _H, _LL, _P, _P, _P]
+    lib.qk_rhf_jk_ms.argtypes = [_H, _P]
     _lib = lib
     return lib
 
