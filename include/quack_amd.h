@@ -232,6 +232,17 @@ int qk_gwb_set_eri_mo(qk_gwb_handle h, int mol, const double *ERI_MO); /* host n
 int qk_gwb_g0w0(qk_gwb_handle h, const double *eHF, double eta, int linearize, int TDA_W, int doSRG, double *eGW,
                 double *SigC, double *Z, double *Om, double *EcRPA, double *EcGM);
 /* [0] AO->MO, [1] eigensolves, [2] excitation densities, [3] self-energy + QP (ms), n <= 4 */
+/* evGW (GW/evRGW.f90:104-200) for every molecule of the batch, cycle by cycle
+ * in step: one batched eigensolve, one excitation stage and one QP stage per
+ * cycle for the whole batch.  Each molecule keeps its own DIIS history,
+ * convergence test and cycle count; one that has converged keeps its last
+ * cycle's outputs while the others go on.  Arrays as qk_gwb_g0w0 (nOrb x nmol,
+ * nS x nmol), eGW_start NULL or nOrb x nmol (as qk_evrgw), nSCF: nmol.
+ * QK_E_NOCONV when some molecule reached maxSCF + 1 cycles (its outputs are
+ * those of its last cycle). */
+int qk_gwb_evgw(qk_gwb_handle h, const double *eHF, const double *eGW_start, int maxSCF, double thresh, int max_diis,
+                double eta, int linearize, int TDA_W, int doSRG, double *eGW, double *SigC, double *Z, double *Om,
+                double *EcRPA, double *EcGM, int *nSCF);
 int qk_gwb_stage_times(qk_gwb_handle h, double *ms, int n);
 
 /* ---- learned warm start: per-orbital MLP (no counterpart in QuAcK) ------ */

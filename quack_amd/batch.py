@@ -100,15 +100,16 @@ class Runner:
             self.plan.append((idx, gw.GWBatch(len(idx), nBas, nOrb, nO), eri, cc, eHF))
         torch.cuda.synchronize()
 
-    def run(self, precision="mixed", doSRG=True, **kw):
-        """G0W0 for every molecule (AO->MO included); returns per-molecule
-        results in input order and the wall time."""
+    def run(self, precision="mixed", doSRG=True, method="g0w0", **kw):
+        """G0W0 (or evGW, method="evgw") for every molecule (AO->MO
+        included); returns per-molecule results in input order and the wall
+        time."""
         res = [None] * len(self.systems)
         t = time.time()
         for idx, b, eri, cc, eHF in self.plan:
             b.set_precision(precision)
             b.set_eri_ao_device(eri.data_ptr(), eri.shape[1], cc.data_ptr(), cc.shape[1])
-            r = b.g0w0(eHF, doSRG=doSRG, **kw)
+            r = b.evgw(eHF, doSRG=doSRG, **kw) if method == "evgw" else b.g0w0(eHF, doSRG=doSRG, **kw)
             for j, i in enumerate(idx):
                 res[i] = {k: (v[j] if np.ndim(v) else v) for k, v in r.items()}
         return res, time.time() - t

@@ -1817,18 +1817,20 @@ void excitation_batch(BCtx* b) {
 }
 
 // GW/RGW_QP_graph.f90:50-92 for every active orbital of every molecule, one
-// host synchronisation per Newton sweep for the whole batch
-void qp_graph_batch(BCtx* b, double eta, int srg) {
+// host synchronisation per Newton sweep for the whole batch.  e0 = eHF; eG:
+// energies in G (eHF in G0W0, eOld in evGW); start: eGWlin (eGWlin in G0W0,
+// eOld in evGW), also the fallback of a failed solve.
+void qp_graph_batch(BCtx* b, double eta, int srg, double* Ctx::*eG, double* Ctx::*start) {
     const int nm = b->nmol;
-    for (Ctx* c : b->mol) qp_init(c->s, c->eLin, c->w, c->f, c->nit, c->done, c->nOrb, c->nC, c->nR);
+    for (Ctx* c : b->mol) qp_init(c->s, c->*start, c->w, c->f, c->nit, c->done, c->nOrb, c->nC, c->nR);
     std::vector<int> act(nm, 1);
     for (int it = 0; it < QP_MAXIT; ++it) {
         bool any = false;
         for (int m = 0; m < nm; ++m) {
             if (!act[m]) continue;
             Ctx* c = b->mol[m];
-            sigma_partial(c->s, c->rhoT, c->rho_f32, c->w, c->e0, c->Om, c->done, eta, srg, SRG_FLOW, c->nOrb, c->nC, c->nO,
-                          c->nR, c->nS, c->nchunk, c->part);
+            sigma_partial(c->s, c->rhoT, c->rho_f32, c->w, c->*eG, c->Om, c->done, eta, srg, SRG_FLOW, c->nOrb, c->nC,
+                          c->nO, c->nR, c->nS, c->nchunk, c->part);
             qp_update(c->s, c->part, c->nchunk, c->e0, c->w, c->f, c->df, c->nit, c->done, c->nOrb, QP_THRESH,
                       QP_MAXIT, c->active);
             any = true;
@@ -1839,7 +1841,7 @@ void qp_graph_batch(BCtx* b, double eta, int srg) {
         if (std::all_of(act.begin(), act.end(), [](int a) { return a == 0; })) break;
     }
     for (Ctx* c : b->mol)
-        qp_finish(c->s, c->w, c->df, c->nit, c->eLin, c->eGW, c->Z, c->nOrb, c->nC, c->nR, QP_MAXIT);
+        qp_finish(c->s, c->w, c->df, c->nit, c->*start, c->eGW, c->Z, c->nOrb, c->nC, c->nR, QP_MAXIT);
 }
 
 }  // namespace
@@ -1974,7 +1976,7 @@ int qk_gwb_g0w0(qk_gwb_handle h, const double* eHF, double eta, int linearize, i
                 for (Ctx* c : h->mol)
                     QK_HIP(hipMemcpyAsync(c->eGW, c->eLin, sizeof(double) * n, hipMemcpyDeviceToDevice, h->s));
             } else {
-                qp_graph_batch(h, eta, doSRG);
+                qp_graph_batch(h, eta, doSRG, &Ctx::e0, &Ctx::eLin);
             }
         }
         std::vector<std::vector<double>> sig(nm, std::vector<double>(n)), sc1(nm, std::vector<double>(8));
@@ -1999,6 +2001,102 @@ int qk_gwb_g0w0(qk_gwb_handle h, const double* eHF, double eta, int linearize, i
             if (EcGM) EcGM[m] = sc1[m][2];
         }
     });
+}
+
+int qk_gwb_evgw(qk_gwb_handle h, const double* eHF, const double* eGW_start, int maxSCF, double thresh, int max_diis,
+                double eta, int linearize, int TDA_W, int doSRG, double* eGW, double* SigC, double* Z, double* Om,
+                double* EcRPA, double* EcGM, int* nSCF_out) {
+    if (!h || !eHF || !eGW) return QK_E_ARG;
+    int conv_status = QK_OK;
+    int rc = guarded(&h->err, [&] {
+        QK_CHECK(h->have_eri, QK_E_ARG, "no integrals for some molecule: call qk_gwb_set_eri_* first");
+        QK_CHECK(maxSCF >= 0, QK_E_ARG, "maxSCF < 0");
+        for (double& t : h->t_stage) t = 0.0;
+        const int n = h->mol[0]->nOrb, nS = h->mol[0]->nS, nm = h->nmol;
+        const double* start = eGW_start ? eGW_start : eHF;
+        // per molecule: GW/evRGW.f90:104-183 (current and previous energies,
+        // DIIS history, cycle count, Conv); a molecule that has converged keeps
+        // its last cycle's outputs and rides along with frozen energies
+        std::vector<std::vector<double>> hGW(nm), hOld(nm);
+        std::vector<Diis> diis;
+        std::vector<double> herr(n), conv(nm, 1.0), ecrpa(nm, 0.0), ecgm(nm, 0.0), sc((size_t)nm * 8);
+        std::vector<int> nscf(nm, 0);
+        std::vector<char> live(nm, 1);
+        std::vector<std::vector<double>> outSig(nm, std::vector<double>(n)), outZ(nm, std::vector<double>(n)),
+            outOm(nm, std::vector<double>(nS));
+        for (int m = 0; m < nm; ++m) {
+            hGW[m].assign(start + (size_t)m * n, start + (size_t)(m + 1) * n);
+            hOld[m] = hGW[m];
+            diis.emplace_back(n, std::max(max_diis, 1));
+            h2d(h->mol[m], h->mol[m]->e0, eHF + (size_t)m * n, n);
+            live[m] = conv[m] > thresh && maxSCF >= 0;
+        }
+        std::vector<std::vector<double>> hnew(nm, std::vector<double>(n));
+        while (std::any_of(live.begin(), live.end(), [](char x) { return x != 0; })) {
+            for (int m = 0; m < nm; ++m) h2d(h->mol[m], h->mol[m]->e, hGW[m].data(), n);
+            screening_batch(h, &Ctx::e, TDA_W, true);
+            for (int m = 0; m < nm; ++m) d2h(h->mol[m], sc.data() + (size_t)m * 8, h->mol[m]->scal, 8);
+            QK_HIP(hipStreamSynchronize(h->s));
+            for (int m = 0; m < nm; ++m)
+                if (live[m]) ecrpa[m] = sc[(size_t)m * 8 + 1];
+            excitation_batch(h);
+            {
+                BStage st(h, 3);
+                for (Ctx* c : h->mol) {
+                    sigma_partial(c->s, c->rhoT, c->rho_f32, c->e, c->e, c->Om, nullptr, eta, doSRG, SRG_FLOW, n, c->nC,
+                                  c->nO, c->nR, nS, c->nchunk, c->part);
+                    sigma_finalize(c->s, c->part, c->nchunk, n, c->nC, c->nO, c->nR, c->Sig, c->Z, c->scal);
+                }
+                if (linearize) {
+                    for (Ctx* c : h->mol) lin_qp(c->s, c->e0, c->Z, c->Sig, c->eGW, n, 0);
+                } else {
+                    qp_graph_batch(h, eta, doSRG, &Ctx::e, &Ctx::e);
+                }
+            }
+            for (int m = 0; m < nm; ++m) {
+                Ctx* c = h->mol[m];
+                d2h(c, sc.data() + (size_t)m * 8, c->scal, 8);
+                d2h(c, hnew[m].data(), c->eGW, n);
+                if (live[m]) {
+                    d2h(c, outSig[m].data(), c->Sig, n);
+                    d2h(c, outZ[m].data(), c->Z, n);
+                    d2h(c, outOm[m].data(), c->Om, nS);
+                }
+            }
+            QK_HIP(hipStreamSynchronize(h->s));
+            for (int m = 0; m < nm; ++m) {
+                if (!live[m]) continue;
+                ecgm[m] = sc[(size_t)m * 8 + 2];
+                hGW[m] = hnew[m];
+                double cv = 0.0;
+                for (int p = 0; p < n; ++p) {
+                    herr[p] = hGW[m][p] - hOld[m][p];
+                    cv = std::max(cv, std::fabs(herr[p]));
+                }
+                conv[m] = cv;
+                if (max_diis > 1) diis[m].step(herr.data(), hGW[m].data());
+                hOld[m] = hGW[m];
+                ++nscf[m];
+                live[m] = conv[m] > thresh && nscf[m] <= maxSCF;
+            }
+        }
+        bool failed = false;
+        for (int m = 0; m < nm; ++m) {
+            std::memcpy(eGW + (size_t)m * n, hGW[m].data(), sizeof(double) * n);
+            if (SigC) std::memcpy(SigC + (size_t)m * n, outSig[m].data(), sizeof(double) * n);
+            if (Z) std::memcpy(Z + (size_t)m * n, outZ[m].data(), sizeof(double) * n);
+            if (Om) std::memcpy(Om + (size_t)m * nS, outOm[m].data(), sizeof(double) * nS);
+            if (EcRPA) EcRPA[m] = ecrpa[m];
+            if (EcGM) EcGM[m] = ecgm[m];
+            if (nSCF_out) nSCF_out[m] = nscf[m];
+            failed = failed || nscf[m] == maxSCF + 1;
+        }
+        if (failed) {
+            conv_status = QK_E_NOCONV;
+            h->err = "evGW: convergence failed for some molecule (nSCF = maxSCF + 1)";
+        }
+    });
+    return rc != QK_OK ? rc : conv_status;
 }
 
 int qk_gwb_stage_times(qk_gwb_handle h, double* ms, int n) {

@@ -29,7 +29,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats",
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
-           "qk_gwb_stage_times", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
+           "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
            "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms"]
 
 
@@ -95,6 +95,7 @@ def load():
     lib.qk_gwb_set_eri_mo.argtypes = [_H, _I, _P]
     lib.qk_gwb_g0w0.argtypes = [_H, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_gwb_stage_times.argtypes = [_H, _P, _I]
+    lib.qk_gwb_evgw.argtypes = [_H, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
     lib.qk_set_autotune.argtypes = [_I]
     lib.qk_set_graphs.argtypes = [_I]
     lib.qk_autotune_report.argtypes = [ctypes.c_char_p, _I]
@@ -305,8 +306,9 @@ def autotune_report():
 
 
 class GWBatch:
-    """nmol molecules of one size (qk_gwb_*): G0W0 for all of them stage by
-    stage on one stream.  Uneven batches: group by size (quack_amd/batch.py)."""
+    """nmol molecules of one size (qk_gwb_*): G0W0 or evGW for all of them
+    stage by stage on one stream.  Uneven batches: group by size
+    (quack_amd/batch.py)."""
 
     def __init__(self, nmol, nBas, nOrb, nO, nC=0, nR=0):
         self.nmol, self.nBas, self.nOrb, self.nO = nmol, nBas, nOrb, nO
@@ -362,6 +364,29 @@ class GWBatch:
                                        _p(Sig), _p(Z), _p(Om), _p(Ec), _p(Egm)))
         return {"eGW": eGW.T.copy(), "SigC": Sig.T.copy(), "Z": Z.T.copy(), "Om": Om.T.copy(), "EcRPA": Ec,
                 "EcGM": Egm}
+
+    def evgw(self, eHF, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, linearize=False, TDA_W=False, doSRG=False,
+             eGW_start=None):
+        """evGW of every molecule (qk_gwb_evgw); eHF (and eGW_start) nmol x nOrb.
+        Raises NotConverged (with the results) when some molecule reached
+        maxSCF + 1 cycles."""
+        n, nS, nm = self.nOrb, self.nS, self.nmol
+        eHF = np.asfortranarray(np.asarray(eHF, dtype=np.float64).reshape(nm, n).T)
+        start = None if eGW_start is None else \
+            np.asfortranarray(np.asarray(eGW_start, dtype=np.float64).reshape(nm, n).T)
+        eGW, Sig, Z = (np.zeros((n, nm), order="F") for _ in range(3))
+        Om = np.zeros((nS, nm), order="F")
+        Ec, Egm = np.zeros(nm), np.zeros(nm)
+        ns = np.zeros(nm, dtype=np.int32)
+        rc = load().qk_gwb_evgw(self.h, _p(eHF), _p(start), maxSCF, thresh, max_diis, eta, int(linearize),
+                                int(TDA_W), int(doSRG), _p(eGW), _p(Sig), _p(Z), _p(Om), _p(Ec), _p(Egm),
+                                ns.ctypes.data_as(_IP))
+        res = {"eGW": eGW.T.copy(), "SigC": Sig.T.copy(), "Z": Z.T.copy(), "Om": Om.T.copy(), "EcRPA": Ec,
+               "EcGM": Egm, "nSCF": ns.astype(int)}
+        if rc == QK_E_NOCONV:
+            raise NotConverged(rc, load().qk_gwb_last_error(self.h).decode(), res)
+        self._check(rc)
+        return res
 
     def stage_times(self):
         ms = np.zeros(4)
