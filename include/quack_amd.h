@@ -193,6 +193,21 @@ int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int
 int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO, int nV, int nR, int nS,
                     const double *eHF, const double *Om, const double *rho, const double *eGWlin, const double *eOld,
                     double *eGW, double *Z);
+/* Static phBSE@GW (GW/RGW_phBSE.f90:1-201, with GW/RGW_phBSE_static_kernel_A.f90
+ * and _B.f90): dRPA screening at eW (TDA_W), its excitation densities, the
+ * static kernels KA/KB (two FP64 matrix-core products over the RPA modes), then
+ * the singlet (singlet != 0) and triplet (triplet != 0) BSE matrices at eGW
+ * (phRLR_A/B with dRPA = false, LR/phRLR_A.f90:43-115, LR/phRLR_B.f90:34-95)
+ * plus the kernels, each solved by phRLR (TDA for the BSE).  ERI: nOrb^4 MO
+ * integrals as RG0W0 receives them.  Outputs: EcBSE[2] (triplet x 3 as at
+ * :193, both halved with exchange_kernel), OmBSE[2 nS] (singlet, then
+ * triplet; zeros for a manifold not asked).  dophBSE2 or dBSE:
+ * QK_E_UNSUPPORTED (the second-order kernel and the dynamical correction are
+ * not built); dTDA is read only by the latter.  Oscillator strengths
+ * (phLR_transition_vectors) are not computed. */
+int qk_rgw_phbse(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE, int dTDA, int singlet, int triplet,
+                 double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double *ERI, const double *eW,
+                 const double *eGW, double *EcBSE, double *OmBSE);
 
 const char *qk_last_error(void);
 

@@ -60,14 +60,17 @@ def ph_dims(nOrb, nO, nC=0, nR=0):
     return nC, nO, nV, nR, (nO - nC) * (nV - nR)
 
 
-def phRLR_A(e, eri, nO, lam=1.0, dRPA=True, nC=0, nR=0):
-    """LR/phRLR_A.f90:140-186 (singlet): A(ia,jb) = 2 lam <bi|ja> - (1-dRPA) lam <bj|ai>
-    + delta_ij delta_ab (e_a - e_i)."""
+def phRLR_A(e, eri, nO, lam=1.0, dRPA=True, nC=0, nR=0, ispin=1):
+    """LR/phRLR_A.f90:43-115: A(ia,jb) = 2 lam <bi|ja> (singlet only)
+    - (1-dRPA) lam <bj|ai> + delta_ij delta_ab (e_a - e_i); ispin 1 singlet,
+    2 triplet."""
     n = len(e)
     G = G4(eri, n)
     occ, vir = slice(nC, nO), slice(nO, n - nR)
     nS = (nO - nC) * (n - nR - nO)
-    A = 2.0 * lam * G[vir, occ, occ, vir].transpose(1, 3, 2, 0).reshape(nS, nS)
+    A = np.zeros((nS, nS))
+    if ispin == 1:
+        A += 2.0 * lam * G[vir, occ, occ, vir].transpose(1, 3, 2, 0).reshape(nS, nS)
     if not dRPA:
         A -= lam * G[vir, occ, vir, occ].transpose(3, 2, 1, 0).reshape(nS, nS)
     dE = (e[None, vir] - e[occ, None]).reshape(-1)
@@ -75,12 +78,15 @@ def phRLR_A(e, eri, nO, lam=1.0, dRPA=True, nC=0, nR=0):
     return A
 
 
-def phRLR_B(eri, n, nO, lam=1.0, dRPA=True, nC=0, nR=0):
-    """LR/phRLR_B.f90:271-312 (singlet): B(ia,jb) = 2 lam <bi|ja> - (1-dRPA) lam <bj|ia>."""
+def phRLR_B(eri, n, nO, lam=1.0, dRPA=True, nC=0, nR=0, ispin=1):
+    """LR/phRLR_B.f90:34-95: B(ia,jb) = 2 lam <bi|ja> (singlet only)
+    - (1-dRPA) lam <bj|ia>; ispin 1 singlet, 2 triplet."""
     G = G4(eri, n)
     occ, vir = slice(nC, nO), slice(nO, n - nR)
     nS = (nO - nC) * (n - nR - nO)
-    B = 2.0 * lam * G[vir, occ, occ, vir].transpose(1, 3, 2, 0).reshape(nS, nS)
+    B = np.zeros((nS, nS))
+    if ispin == 1:
+        B += 2.0 * lam * G[vir, occ, occ, vir].transpose(1, 3, 2, 0).reshape(nS, nS)
     if not dRPA:
         B -= lam * G[vir, occ, occ, vir].transpose(2, 3, 1, 0).reshape(nS, nS)
     return B
@@ -309,3 +315,55 @@ def evrgw(eri_mo, eHF, nO, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, lineari
     status = 1 if nSCF == maxSCF + 1 else 0
     return {"eGW": eGW, "SigC": SigC, "Z": Z, "Om": Om, "EcRPA": EcRPA, "EcGM": EcGM, "nSCF": nSCF,
             "status": status, "Conv": Conv, "history": hist}
+
+
+def phbse_static_kernel_A(eta, Om, rho, nO, lam=1.0, nC=0, nR=0):
+    """GW/RGW_phBSE_static_kernel_A.f90:20-40:
+    KA(ia,jb) = 4 lam^2 sum_m rho(i,j,m) rho(a,b,m) Om_m / (Om_m^2 + eta^2)."""
+    n = rho.shape[0]
+    occ, vir = slice(nC, nO), slice(nO, n - nR)
+    w = Om / (Om ** 2 + eta ** 2)
+    K = np.einsum("ijm,abm,m->iajb", rho[occ, occ], rho[vir, vir], w, optimize=True)
+    nS = (nO - nC) * (n - nR - nO)
+    return 4.0 * lam ** 2 * K.reshape(nS, nS)
+
+
+def phbse_static_kernel_B(eta, Om, rho, nO, lam=1.0, nC=0, nR=0):
+    """GW/RGW_phBSE_static_kernel_B.f90:20-40:
+    KB(ia,jb) = 4 lam^2 sum_m rho(i,b,m) rho(a,j,m) Om_m / (Om_m^2 + eta^2)."""
+    n = rho.shape[0]
+    occ, vir = slice(nC, nO), slice(nO, n - nR)
+    w = Om / (Om ** 2 + eta ** 2)
+    K = np.einsum("ibm,ajm,m->iajb", rho[occ, vir], rho[vir, occ], w, optimize=True)
+    nS = (nO - nC) * (n - nR - nO)
+    return 4.0 * lam ** 2 * K.reshape(nS, nS)
+
+
+def rgw_phbse(eri, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, singlet=True, triplet=True,
+              exchange_kernel=False):
+    """GW/RGW_phBSE.f90:100-195 with the static kernel (no phBSE2, no dynamical
+    correction): dRPA screening at eW, rho, KA/KB, then the singlet and
+    triplet BSE matrices at eGW (phRLR_A/B with dRPA = false) plus the
+    kernels, each solved by phRLR.  Returns EcBSE(2) (triplet x 3, halved with
+    exchange_kernel) and the singlet and triplet excitation energies."""
+    n = len(eW)
+    nS = ph_dims(n, nO)[4]
+    A = phRLR_A(eW, eri, nO, dRPA=True)
+    B = np.zeros((nS, nS)) if TDA_W else phRLR_B(eri, n, nO, dRPA=True)
+    _, OmRPA, XpY, _ = phRLR(TDA_W, A, B)
+    rho = excitation_density(eri, XpY, n, nO)
+    KA = phbse_static_kernel_A(eta, OmRPA, rho, nO)
+    KB = np.zeros((nS, nS)) if TDA else phbse_static_kernel_B(eta, OmRPA, rho, nO)
+    EcBSE = np.zeros(2)
+    Om = {1: np.zeros(nS), 2: np.zeros(nS)}
+    for ispin, on in ((1, singlet), (2, triplet)):
+        if not on:
+            continue
+        A = phRLR_A(eGW, eri, nO, dRPA=False, ispin=ispin) + KA
+        B = np.zeros((nS, nS)) if TDA else phRLR_B(eri, n, nO, dRPA=False, ispin=ispin) + KB
+        EcBSE[ispin - 1], Om[ispin], _, _ = phRLR(TDA, A, B)
+    EcBSE[1] *= 3.0
+    if exchange_kernel:
+        EcBSE *= 0.5
+    return EcBSE, Om[1], Om[2]
+

@@ -42,6 +42,7 @@ def load():
         lib.qkref_rg0w0.argtypes = [_I, _I, _I, _D] + [_I] * 6 + [_D, _D] + [_P] * 8
         lib.qkref_evrgw.argtypes = [_I, _I, _I, _I, _D, _I, _D] + [_I] * 6 + [_D, _D] + [_P] * 7 + [_IP]
         lib.qkref_evrgw.restype = _I
+        lib.qkref_phbse.argtypes = [_I] * 4 + [_D] + [_I] * 6 + [_P] * 6
         _lib = lib
     return _lib
 
@@ -131,3 +132,16 @@ def evrgw(eri_mo, eHF, nO, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, lineari
                             nR, nS, ENuc, ERHF, _p(np.ascontiguousarray(eri_mo)), _p(_f(eHF)), _p(eGW), _p(Sig),
                             _p(Z), _p(Ec), _p(Egm), ctypes.byref(nscf))
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "EcRPA": Ec[0], "EcGM": Egm[0], "nSCF": nscf.value, "status": st}
+
+
+def phbse(eri_mo, eW, eGW, nO, eta=0.0, tda_w=False, tda=False, singlet=True, triplet=True, nC=0, nR=0):
+    """Static phBSE@GW through QuAcK's own routines (ref_driver.f90
+    qkref_phbse, the order of GW/RGW_phBSE.f90:113-190)."""
+    nOrb = len(eW)
+    nC, nO, nV, nR, nS = _dims(nOrb, nO, nC, nR)
+    Ec = np.zeros(2)
+    OmS, OmT = np.zeros(nS), np.zeros(nS)
+    load().qkref_phbse(int(tda_w), int(tda), int(singlet), int(triplet), eta, nOrb, nC, nO, nV, nR, nS,
+                       _p(np.ascontiguousarray(eri_mo)), _p(_f(eW)), _p(_f(eGW)), _p(Ec), _p(OmS), _p(OmT))
+    return {"EcBSE": Ec, "OmS": OmS, "OmT": OmT}
+

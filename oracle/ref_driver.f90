@@ -186,3 +186,51 @@ function qkref_evrgw(tda, lin, srg, maxSCF, thresh, max_diis, eta, nOrb, nC, nO,
   if(nSCF == maxSCF+1) status = 1
   deallocate(Aph, Bph, Om, XpY, XmY, rho, eOld, error_diis, e_diis)
 end function
+
+! GW/RGW_phBSE.f90:113-190, static-kernel path (no phBSE2, no dynamical
+! correction, no transition vectors), composed of QuAcK's own routines in the
+! same order: dRPA screening at eW, excitation densities, static kernels, then
+! the singlet (ispin 1) and triplet (ispin 2) BSE at eGW with dRPA = false.
+subroutine qkref_phbse(tda_w, tda, dosing, dotrip, eta, nOrb, nC, nO, nV, nR, nS, ERI, eW, eGW, EcBSE, OmS, OmT) &
+    bind(C, name='qkref_phbse')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: tda_w, tda, dosing, dotrip, nOrb, nC, nO, nV, nR, nS
+  real(c_double), value :: eta
+  real(c_double), intent(in) :: ERI(nOrb, nOrb, nOrb, nOrb), eW(nOrb), eGW(nOrb)
+  real(c_double), intent(out) :: EcBSE(2), OmS(nS), OmT(nS)
+  double precision, allocatable :: Aph(:,:), Bph(:,:), KA(:,:), KB(:,:), OmRPA(:), XpY(:,:), XmY(:,:), rho(:,:,:)
+  double precision :: EcRPA
+  logical :: TDAW, TDAB
+  TDAW = tda_w /= 0
+  TDAB = tda /= 0
+  allocate(Aph(nS, nS), Bph(nS, nS), KA(nS, nS), KB(nS, nS), OmRPA(nS), XpY(nS, nS), XmY(nS, nS), &
+           rho(nOrb, nOrb, nS))
+  Aph = 0d0; Bph = 0d0; KA = 0d0; KB = 0d0
+  EcBSE = 0d0; OmS = 0d0; OmT = 0d0
+  call phRLR_A(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, eW, ERI, Aph)
+  if(.not. TDAW) call phRLR_B(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, ERI, Bph)
+  call phRLR(TDAW, nS, Aph, Bph, EcRPA, OmRPA, XpY, XmY)
+  call RGW_excitation_density(nOrb, nC, nO, nR, nS, ERI, XpY, rho)
+  call RGW_phBSE_static_kernel_A(eta, nOrb, nC, nO, nV, nR, nS, 1d0, ERI, OmRPA, rho, KA)
+  if(.not. TDAB) call RGW_phBSE_static_kernel_B(eta, nOrb, nC, nO, nV, nR, nS, 1d0, ERI, OmRPA, rho, KB)
+  if(dosing /= 0) then
+    Bph = 0d0
+    call phRLR_A(1, .false., nOrb, nC, nO, nV, nR, nS, 1d0, eGW, ERI, Aph)
+    if(.not. TDAB) call phRLR_B(1, .false., nOrb, nC, nO, nV, nR, nS, 1d0, ERI, Bph)
+    Aph = Aph + KA
+    if(.not. TDAB) Bph = Bph + KB
+    call phRLR(TDAB, nS, Aph, Bph, EcBSE(1), OmS, XpY, XmY)
+  end if
+  if(dotrip /= 0) then
+    Bph = 0d0
+    call phRLR_A(2, .false., nOrb, nC, nO, nV, nR, nS, 1d0, eGW, ERI, Aph)
+    if(.not. TDAB) call phRLR_B(2, .false., nOrb, nC, nO, nV, nR, nS, 1d0, ERI, Bph)
+    Aph = Aph + KA
+    if(.not. TDAB) Bph = Bph + KB
+    call phRLR(TDAB, nS, Aph, Bph, EcBSE(2), OmT, XpY, XmY)
+  end if
+  EcBSE(2) = 3d0 * EcBSE(2)
+  flush(6)
+  deallocate(Aph, Bph, KA, KB, OmRPA, XpY, XmY, rho)
+end subroutine

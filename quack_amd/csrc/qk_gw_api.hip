@@ -1700,6 +1700,119 @@ int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO
 }
 
 
+int qk_rgw_phbse(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE, int dTDA, int singlet, int triplet,
+                 double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double* ERI, const double* eW,
+                 const double* eGW, double* EcBSE, double* OmBSE) {
+    (void)dTDA;  // only read by the dynamical correction, which is not built
+    if (nOrb <= 0 || nC < 0 || nO <= nC || nR < 0 || nV != nOrb - nO || nS != (nO - nC) * (nV - nR) || nS <= 0 ||
+        !ERI || !eW || !eGW || !EcBSE || !OmBSE)
+        return QK_E_ARG;
+    if (dophBSE2 || dBSE) {
+        g_last_error = "qk_rgw_phbse: the second-order kernel and the dynamical correction are not built";
+        return QK_E_UNSUPPORTED;
+    }
+    qk_gw_handle c = nullptr;
+    int rc = qk_gw_create(&c, nOrb, nOrb, nC, nO, nV, nR, nS);
+    if (rc != QK_OK) return rc;
+    rc = qk_gw_set_eri_mo(c, ERI);
+    if (rc == QK_OK) rc = guarded(&c->err, [&] {
+        const int n = nOrb, o = nO - nC, v = nV - nR;
+        const size_t S2 = (size_t)nS * nS;
+        // GW/RGW_phBSE.f90:113-118: dRPA screening at eW and its excitation densities (FP64)
+        c->precision = 0;
+        c->eig_mode = 0;
+        h2d(c, c->e, eW, n);
+        screening(c, c->e, eW, TDA_W, true);
+        excitation(c);
+        // static kernels (GW/RGW_phBSE_static_kernel_A.f90, _B.f90), as two
+        // matrix-core products over the RPA modes m:
+        //   KA(ia,jb) = 4 sum_m [w_m rho(i,j,m)] rho(a,b,m) -> G1[(i + o j), (a + v b)]
+        //   KB(ia,jb) = 4 sum_m [w_m rho(i,b,m)] rho(a,j,m) -> G2[(i + o b), (a + v j)]
+        std::vector<double> KA(S2, 0.0), KB(S2, 0.0);
+        {
+            double* X = dalloc<double>((size_t)nS * o * std::max(o, v));
+            double* Y = nullptr;
+            double* G = nullptr;
+            try {
+                Y = dalloc<double>((size_t)nS * v * std::max(o, v));
+                G = dalloc<double>(S2 > (size_t)o * o * v * v ? S2 : (size_t)o * o * v * v);
+                std::vector<double> hG((size_t)o * o * v * v);
+                rho_block(c->s, c->rhoT, c->Om, eta, 1, n, nS, nC, o, nC, o, X);
+                rho_block(c->s, c->rhoT, c->Om, eta, 0, n, nS, nO, v, nO, v, Y);
+                dgemm(c->s, true, false, o * o, v * v, nS, 1.0, X, nS, Y, nS, 0.0, G, o * o);
+                QK_HIP(hipMemcpyAsync(hG.data(), G, sizeof(double) * hG.size(), hipMemcpyDeviceToHost, c->s));
+                QK_HIP(hipStreamSynchronize(c->s));
+                for (int i = 0; i < o; ++i)
+                    for (int a = 0; a < v; ++a)
+                        for (int j = 0; j < o; ++j)
+                            for (int b = 0; b < v; ++b)
+                                KA[(a + (size_t)v * i) + nS * (b + (size_t)v * j)] =
+                                    4.0 * hG[(i + (size_t)o * j) + (size_t)o * o * (a + (size_t)v * b)];
+                if (!TDA) {
+                    rho_block(c->s, c->rhoT, c->Om, eta, 1, n, nS, nC, o, nO, v, X);
+                    rho_block(c->s, c->rhoT, c->Om, eta, 0, n, nS, nO, v, nC, o, Y);
+                    dgemm(c->s, true, false, o * v, v * o, nS, 1.0, X, nS, Y, nS, 0.0, G, o * v);
+                    hG.resize(S2);
+                    QK_HIP(hipMemcpyAsync(hG.data(), G, sizeof(double) * S2, hipMemcpyDeviceToHost, c->s));
+                    QK_HIP(hipStreamSynchronize(c->s));
+                    for (int i = 0; i < o; ++i)
+                        for (int a = 0; a < v; ++a)
+                            for (int j = 0; j < o; ++j)
+                                for (int b = 0; b < v; ++b)
+                                    KB[(a + (size_t)v * i) + nS * (b + (size_t)v * j)] =
+                                        4.0 * hG[(i + (size_t)o * b) + (size_t)o * v * (a + (size_t)v * j)];
+                }
+            } catch (...) {
+                (void)hipStreamSynchronize(c->s);
+                dfree(X); dfree(Y); dfree(G);
+                throw;
+            }
+            dfree(X); dfree(Y); dfree(G);
+        }
+        // singlet (ispin 1) and triplet (ispin 2): phRLR_A/B with dRPA = false at
+        // eGW (LR/phRLR_A.f90:43-115, LR/phRLR_B.f90:34-95) plus the kernels,
+        // then phRLR (GW/RGW_phBSE.f90:133-190)
+        auto G = [&](int p, int q, int r, int t) {  // ERI(p,q,r,s), 0-based
+            return ERI[p + (size_t)n * (q + (size_t)n * (r + (size_t)n * t))];
+        };
+        std::vector<double> A(S2), B(S2), XpY(S2), XmY(S2);
+        for (int k = 0; k < 2 * nS; ++k) OmBSE[k] = 0.0;
+        EcBSE[0] = EcBSE[1] = 0.0;
+        for (int ispin = 1; ispin <= 2; ++ispin) {
+            if ((ispin == 1 && !singlet) || (ispin == 2 && !triplet)) continue;
+            const double ct1 = ispin == 1 ? 2.0 : 0.0, ct2 = -1.0;
+            for (int i = 0; i < o; ++i)
+                for (int a = 0; a < v; ++a) {
+                    const size_t ia = a + (size_t)v * i;
+                    const int I = nC + i, Aa = nO + a;
+                    for (int j = 0; j < o; ++j)
+                        for (int b = 0; b < v; ++b) {
+                            const size_t jb = b + (size_t)v * j;
+                            const int J = nC + j, Bb = nO + b;
+                            const double d = G(Bb, I, J, Aa);
+                            double aij = ct1 * d + ct2 * G(Bb, J, Aa, I) + KA[ia + nS * jb];
+                            if (i == j && a == b) aij += eGW[Aa] - eGW[I];
+                            A[ia + nS * jb] = aij;
+                            B[ia + nS * jb] = TDA ? 0.0 : ct1 * d + ct2 * G(Bb, J, I, Aa) + KB[ia + nS * jb];
+                        }
+                }
+            double ec = 0.0;
+            const int r = qk_phrlr(TDA, nS, A.data(), B.data(), &ec, OmBSE + (size_t)(ispin - 1) * nS, XpY.data(),
+                                   XmY.data());
+            QK_CHECK(r == QK_OK, r, "qk_rgw_phbse: phRLR failed: " + g_last_error);
+            EcBSE[ispin - 1] = ec;
+        }
+        EcBSE[1] *= 3.0;  // GW/RGW_phBSE.f90:193
+        if (exchange_kernel) {
+            EcBSE[0] *= 0.5;
+            EcBSE[1] *= 0.5;
+        }
+    });
+    if (rc != QK_OK && c) g_last_error = c->err;
+    qk_gw_destroy(c);
+    return rc;
+}
+
 }  // extern "C"
 
 // ---- batched G0W0: many molecules of one size, stage by stage -----------------

@@ -660,6 +660,35 @@ void sigma_out(hipStream_t s, const double* part, int nchunk, int n, int nC, int
     hipLaunchKernelGGL(k_sigma_out, dim3((n + 255) / 256), dim3(256), 0, s, part, nchunk, n, nC, nR, Sig, dSig);
     QK_HIP(hipGetLastError());
 }
+// out[m + nS (p' + P q')] = w_m rho(p0 + p', q0 + q', m), w_m = Om_m / (Om_m^2 + eta^2)
+// when weighted, else 1; rho(p, q, m) = rhoT[m + nS (p + n q)] (FP64).  The
+// (occ, occ), (virt, virt), (occ, virt) and (virt, occ) blocks that the static
+// BSE kernels contract over m (GW/RGW_phBSE_static_kernel_A.f90, _B.f90).
+__global__ void k_rho_block(const double* __restrict__ rhoT, const double* __restrict__ Om, double eta, int weighted,
+                            int n, int nS, int p0, int P, int q0, int Q, double* __restrict__ out) {
+    const long long total = (long long)nS * P * Q;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int m = (int)(t % nS);
+        const long long pq = t / nS;
+        const int pp = (int)(pq % P), qq = (int)(pq / P);
+        double v = rhoT[m + (long long)nS * ((p0 + pp) + (long long)n * (q0 + qq))];
+        if (weighted) {
+            const double om = Om[m];
+            v *= om / (om * om + eta * eta);
+        }
+        out[t] = v;
+    }
+}
+
+void rho_block(hipStream_t s, const double* rhoT, const double* Om, double eta, int weighted, int n, int nS, int p0,
+               int P, int q0, int Q, double* out) {
+    const long long total = (long long)nS * P * Q;
+    hipLaunchKernelGGL(k_rho_block, dim3((unsigned)std::min<long long>((total + 255) / 256, 65536)), dim3(256), 0, s,
+                       rhoT, Om, eta, weighted, n, nS, p0, P, q0, Q, out);
+    QK_HIP(hipGetLastError());
+}
+
 void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag) {
     hipLaunchKernelGGL(k_lin, dim3((n + 255) / 256), dim3(256), 0, s, e0, Z, Sig, out, n, zflag);
     QK_HIP(hipGetLastError());

@@ -59,7 +59,8 @@ subroutine RG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,dophBSE2,TDA_W
   type(c_ptr) :: h
   integer(c_int) :: rc
 
-  if(dophBSE .or. dophBSE2) call qk_unsupported('phBSE@G0W0')
+  if(dophBSE2) call qk_unsupported('phBSE2@G0W0')
+  if(dophBSE .and. dBSE) call qk_unsupported('dynamical phBSE@G0W0')
   if(doppBSE) call qk_unsupported('ppBSE@G0W0')
   if(doACFDT) call qk_unsupported('ACFDT@G0W0')
   if(do_linDM_GW) call qk_unsupported('linDM@G0W0')
@@ -110,6 +111,10 @@ subroutine RG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,dophBSE2,TDA_W
   write(*,*)'-------------------------------------------------------------------------------'
   write(*,*)
 
+  ! static phBSE at the G0W0 energies (GW/RG0W0.f90:239-255): W from eHF
+  if(dophBSE) call qk_phbse_print('G0W0', exchange_kernel, TDA_W, TDA, dTDA, singlet, triplet, eta, &
+                                  nOrb, nC, nO, nV, nR, nS, ENuc, ERHF, ERI, eHF, eGW_out)
+
   if(dotest) then
     call dump_test_value('R','G0W0 correlation energy',EcRPA)
     call dump_test_value('R','G0W0 HOMO energy',eGW_out(nO))
@@ -117,6 +122,64 @@ subroutine RG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,dophBSE2,TDA_W
   end if
 
   deallocate(SigC, Z, Om)
+end subroutine
+
+! Static phBSE (qk_rgw_phbse) after a GW calculation, with QuAcK's output:
+! the excitation tables that RGW_phBSE prints (GW/RGW_phBSE.f90:145,180) and
+! the Tr@BSE energies of GW/RG0W0.f90:248-255 / GW/evRGW.f90:219-226.
+subroutine qk_phbse_print(tag, exchange_kernel, TDA_W, TDA, dTDA, singlet, triplet, eta, nOrb, nC, nO, nV, nR, nS, &
+                          ENuc, ERHF, ERI, eW, eGW)
+  use quack_amd_gw
+  implicit none
+  character(len=*), intent(in) :: tag
+  logical, intent(in) :: exchange_kernel, TDA_W, TDA, dTDA, singlet, triplet
+  double precision, intent(in) :: eta, ENuc, ERHF
+  integer, intent(in) :: nOrb, nC, nO, nV, nR, nS
+  double precision, intent(in) :: ERI(nOrb,nOrb,nOrb,nOrb), eW(nOrb), eGW(nOrb)
+  double precision, allocatable :: OmBSE(:)
+  double precision :: EcBSE(2)
+  integer(c_int) :: rc
+  allocate(OmBSE(2*nS))
+  rc = qk_rgw_phbse(0_c_int, merge(1_c_int, 0_c_int, exchange_kernel), merge(1_c_int, 0_c_int, TDA_W), &
+                    merge(1_c_int, 0_c_int, TDA), 0_c_int, merge(1_c_int, 0_c_int, dTDA),                 &
+                    merge(1_c_int, 0_c_int, singlet), merge(1_c_int, 0_c_int, triplet), eta,             &
+                    int(nOrb,c_int), int(nC,c_int), int(nO,c_int), int(nV,c_int), int(nR,c_int),        &
+                    int(nS,c_int), ERI, eW, eGW, EcBSE, OmBSE)
+  call qk_check(rc, c_null_ptr, 'qk_rgw_phbse')
+  if(singlet) call qk_print_excitations('phBSE@GW@RHF', 'singlet', nS, OmBSE(1:nS))
+  if(triplet) call qk_print_excitations('phBSE@GW@RHF', 'triplet', nS, OmBSE(nS+1:2*nS))
+  write(*,*)
+  write(*,*)'-------------------------------------------------------------------------------'
+  write(*,'(2X,A50,F20.10,A3)') 'Tr@BSE@'//tag//'@RHF correlation energy (singlet) = ',EcBSE(1),' au'
+  write(*,'(2X,A50,F20.10,A3)') 'Tr@BSE@'//tag//'@RHF correlation energy (triplet) = ',EcBSE(2),' au'
+  write(*,'(2X,A50,F20.10,A3)') 'Tr@BSE@'//tag//'@RHF correlation energy           = ',sum(EcBSE),' au'
+  write(*,'(2X,A50,F20.10,A3)') 'Tr@BSE@'//tag//'@RHF total       energy           = ',ENuc + ERHF + sum(EcBSE),' au'
+  write(*,*)'-------------------------------------------------------------------------------'
+  write(*,*)
+  deallocate(OmBSE)
+end subroutine
+
+! Excitation energies of one spin manifold, in the table of QuAcK's
+! LR/print_excitation_energies.f90 (at most the first 50 states).
+subroutine qk_print_excitations(method, manifold, nS, Om)
+  implicit none
+  character(len=*), intent(in) :: method, manifold
+  integer, intent(in) :: nS
+  double precision, intent(in) :: Om(nS)
+  double precision, parameter :: HaToeV = 27.211386245988d0
+  integer :: m
+  write(*,*)
+  write(*,*)'-------------------------------------------------------------'
+  write(*,'(1X,A15,A15,A15,A9)') trim(method),' calculation: ',trim(manifold),' manifold'
+  write(*,*)'-------------------------------------------------------------'
+  write(*,'(1X,A1,1X,A5,1X,A1,1X,A23,1X,A1,1X,A23,1X,A1,1X)') &
+            '|','State','|',' Excitation energy (au) ','|',' Excitation energy (eV) ','|'
+  write(*,*)'-------------------------------------------------------------'
+  do m = 1, min(50, nS)
+    write(*,'(1X,A1,1X,I5,1X,A1,1X,F23.6,1X,A1,1X,F23.6,1X,A1,1X)') '|',m,'|',Om(m),'|',Om(m)*HaToeV,'|'
+  end do
+  write(*,*)'-------------------------------------------------------------'
+  write(*,*)
 end subroutine
 
 subroutine evRGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS,dophBSE,dophBSE2,TDA_W,TDA,dBSE,dTDA, &
@@ -137,7 +200,8 @@ subroutine evRGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS
   integer(c_int) :: rc, nSCF
   type(c_ptr) :: h
 
-  if(dophBSE .or. dophBSE2) call qk_unsupported('phBSE@evGW')
+  if(dophBSE2) call qk_unsupported('phBSE2@evGW')
+  if(dophBSE .and. dBSE) call qk_unsupported('dynamical phBSE@evGW')
   if(doppBSE) call qk_unsupported('ppBSE@evGW')
   if(doACFDT) call qk_unsupported('ACFDT@evGW')
 
@@ -190,6 +254,10 @@ subroutine evRGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS
   write(*,'(2X,A60,F15.6,A3)') '   GM@evGW@RHF correlation energy = ',EcGM,' au'
   write(*,*)'-------------------------------------------------------------------------------'
   write(*,*)
+
+  ! static phBSE at the converged energies (GW/evRGW.f90:214-226): W from eGW too
+  if(dophBSE) call qk_phbse_print('evGW', exchange_kernel, TDA_W, TDA, dTDA, singlet, triplet, eta, &
+                                  nOrb, nC, nO, nV, nR, nS, ENuc, ERHF, ERI, eGW, eGW)
 
   if(dotest) then
     call dump_test_value('R','evGW correlation energy',EcRPA)

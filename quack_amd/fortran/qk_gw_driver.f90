@@ -22,7 +22,7 @@ program qk_gw_driver
   integer :: maxSCF, max_diis, nfreqs, nAt, nEla, nElb, nC, nR, nO, nV, nS, nOrb, ios
   double precision :: thresh, eta, shift, ENuc, ERHF
   character(len=1) :: a1, a2, a3, a4, a5, a6, a7, a8
-  logical :: lin, TDA_W, reg
+  logical :: lin, TDA_W, reg, TDA, singlet, triplet, dophBSE, dophBSE2, doppBSE, dBSE, dTDA
   double precision, allocatable :: eHF(:), ERI(:,:,:,:), dip(:,:,:), eGW(:)
 
   call get_command_argument(1, wd)
@@ -46,6 +46,8 @@ program qk_gw_driver
 
   ! options (defaults of src/QuAcK/read_options.f90:242-254)
   maxSCF = 64; thresh = 1d-5; max_diis = 1; lin = .false.; eta = 0d0; TDA_W = .false.; reg = .false.
+  TDA = .false.; singlet = .true.; triplet = .true.
+  dophBSE = .false.; dophBSE2 = .false.; doppBSE = .false.; dBSE = .false.; dTDA = .true.
   open(unit=12, file=trim(wd)//'/input/options', status='old', action='read')
   do
     read(12,'(A)',iostat=ios) line
@@ -53,6 +55,14 @@ program qk_gw_driver
     if(index(line, '# GW:') == 1) then
       read(12,*) maxSCF, thresh, max_diis, a1, eta, a2, a3, nfreqs, shift, a4, a5, a6, a7, a8
       lin = a1 == 'T'; TDA_W = a2 == 'T'; reg = a3 == 'T'
+    else if(index(line, '# spin:') == 1) then
+      ! src/QuAcK/read_options.f90: TDA singlet triplet
+      read(12,*) a1, a2, a3
+      TDA = a1 == 'T'; singlet = a2 == 'T'; triplet = a3 == 'T'
+    else if(index(line, '# BSE:') == 1) then
+      ! src/QuAcK/read_options.f90:300-315: phBSE phBSE2 ppBSE dBSE dTDA
+      read(12,*) a1, a2, a3, a4, a5
+      dophBSE = a1 == 'T'; dophBSE2 = a2 == 'T'; doppBSE = a3 == 'T'; dBSE = a4 == 'T'; dTDA = a5 /= 'F'
     end if
   end do
   close(12)
@@ -89,11 +99,11 @@ program qk_gw_driver
 
   if(dotest) open(unit=1231597, file=trim(wd)//'/test/Rtest.dat', status='replace')
 
-  if(doG0W0) call RG0W0_hip(dotest, .false., .false., .false., .false., .false., TDA_W, .false., .false., .false., &
-                            .false., .true., .false., lin, eta, reg, .false., nOrb, nOrb, nC, nO, nV, nR, nS, ENuc, &
+  if(doG0W0) call RG0W0_hip(dotest, .false., .false., .false., dophBSE, dophBSE2, TDA_W, TDA, dBSE, dTDA, &
+                            doppBSE, singlet, triplet, lin, eta, reg, .false., nOrb, nOrb, nC, nO, nV, nR, nS, ENuc, &
                             ERHF, ERI, dip, eHF, eGW)
-  if(doevGW) call evRGW_hip(dotest, maxSCF, thresh, max_diis, .false., .false., .false., .false., .false., TDA_W, &
-                            .false., .false., .false., .false., .true., .false., lin, eta, reg, nOrb, nOrb, nC, nO, &
+  if(doevGW) call evRGW_hip(dotest, maxSCF, thresh, max_diis, .false., .false., .false., dophBSE, dophBSE2, TDA_W, &
+                            TDA, dBSE, dTDA, doppBSE, singlet, triplet, lin, eta, reg, nOrb, nOrb, nC, nO, &
                             nV, nR, nS, ENuc, ERHF, ERI, dip, eHF)
 
   if(dotest) close(1231597)

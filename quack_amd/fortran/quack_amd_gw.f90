@@ -70,6 +70,17 @@ module quack_amd_gw
       integer(c_int), intent(out) :: nSCF
     end function
 
+    ! static phBSE@GW (GW/RGW_phBSE.f90) from the MO integrals
+    integer(c_int) function qk_rgw_phbse(dophBSE2, exchange_kernel, TDA_W, TDA, dBSE, dTDA, singlet, triplet, eta, &
+                                         nOrb, nC, nO, nV, nR, nS, ERI, eW, eGW, EcBSE, OmBSE) bind(C, name='qk_rgw_phbse')
+      import :: c_int, c_double
+      integer(c_int), value :: dophBSE2, exchange_kernel, TDA_W, TDA, dBSE, dTDA, singlet, triplet
+      real(c_double), value :: eta
+      integer(c_int), value :: nOrb, nC, nO, nV, nR, nS
+      real(c_double), intent(in) :: ERI(*), eW(*), eGW(*)
+      real(c_double), intent(out) :: EcBSE(2), OmBSE(*)
+    end function
+
     ! RHF Fock builds (HF/RHF.f90:178-179) on the device
     integer(c_int) function qk_rhf_create(h, nBas) bind(C, name='qk_rhf_create')
       import :: c_int, c_ptr

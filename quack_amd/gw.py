@@ -30,7 +30,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
            "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
-           "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms"]
+           "qk_rgw_phbse", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms"]
 
 
 class QKError(RuntimeError):
@@ -119,6 +119,7 @@ def load():
     lib.qk_rgw_excitation_density.argtypes = [_I] * 5 + [_P, _P, _P]
     lib.qk_rgw_self_energy_diag.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
     lib.qk_rgw_qp_graph.argtypes = [_I, _D, _D] + [_I] * 6 + [_P] * 7
+    lib.qk_rgw_phbse.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 5
     lib.qk_rhf_create.argtypes = [ctypes.POINTER(_H), _I]
     lib.qk_rhf_destroy.argtypes = [_H]
     lib.qk_rhf_last_error.argtypes = [_H]
@@ -545,3 +546,20 @@ class MLP:
         hist = np.zeros(max(epochs, 1), np.float32)
         self._chk(load().qk_mlp_train(self.h, px, pt, X.shape[0], epochs, batch, lr, seed, self._f(hist)[1]))
         return hist[:epochs]
+
+
+def RGW_phBSE(ERI, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, singlet=True, triplet=True, exchange_kernel=False,
+              nC=0, nR=0):
+    """GW/RGW_phBSE.f90 with the static kernel (qk_rgw_phbse).  Returns
+    dict(EcBSE (2), OmS, OmT)."""
+    eW, eGW = _f64(eW), _f64(eGW)
+    n = len(eW)
+    nV = n - nO
+    nS = (nO - nC) * (nV - nR)
+    ERI = _f64(ERI)  # QuAcK's column-major nOrb^4 array (a flat array as it is)
+    Ec = np.zeros(2)
+    Om = np.zeros(2 * nS)
+    _check(load().qk_rgw_phbse(0, int(exchange_kernel), int(TDA_W), int(TDA), 0, 0, int(singlet), int(triplet), eta,
+                               n, nC, nO, nV, nR, nS, _p(ERI), _p(eW), _p(eGW), _p(Ec), _p(Om)))
+    return {"EcBSE": Ec, "OmS": Om[:nS].copy(), "OmT": Om[nS:].copy()}
+

@@ -62,7 +62,7 @@ OPTIONS = """# HF: maxSCF thresh    DIIS guess mix shift stab search aordm readF
 # ACFDT: AC Kx  XBS
          F  F   F
 # BSE: phBSE phBSE2 ppBSE dBSE dTDA
-       F     F      F     F    F
+       {bse}     F      F     F    F
 """
 
 
@@ -71,8 +71,9 @@ def tf(b):
 
 
 def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=1e-5, max_diis=5, lin=False,
-                 eta=0.0, TDA_W=False, reg=False, eri_mo=None):
-    """sysd: a dict from quack_amd.intgen.system(); eri_mo: MO integrals (nOrb^4, flat)."""
+                 eta=0.0, TDA_W=False, reg=False, eri_mo=None, bse=False):
+    """sysd: a dict from quack_amd.intgen.system(); eri_mo: MO integrals (nOrb^4, flat);
+    bse: the phBSE flag of the "# BSE:" options line (static kernel)."""
     for d in ("input", "int", "test"):
         os.makedirs(os.path.join(wd, d), exist_ok=True)
     mol = sysd["mol"]
@@ -80,7 +81,7 @@ def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=
         f.write(METHODS.format(g0w0=tf(g0w0), evgw=tf(evgw), test=tf(test)))
     with open(os.path.join(wd, "input", "options"), "w") as f:
         f.write(OPTIONS.format(maxscf=maxSCF, thresh="%.10g" % thresh, diis=max_diis, lin=tf(lin), eta="%.10g" % eta,
-                               tda=tf(TDA_W), reg=tf(reg)))
+                               tda=tf(TDA_W), reg=tf(reg), bse=tf(bse)))
     with open(os.path.join(wd, "input", "hpc_flags"), "w") as f:
         f.write("# if True (T), switch to HPC mode\n  F\n# if True (T), use GPU\n  T\n")
     with open(os.path.join(wd, "input", "molecule"), "w") as f:

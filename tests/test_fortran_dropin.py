@@ -46,3 +46,34 @@ def test_driver_reproduces_quack_goldens(tmp_path, mo_systems, name):
     for key in ("G0W0 correlation energy", "G0W0 HOMO energy", "G0W0 LUMO energy", "evGW correlation energy",
                 "evGW HOMO energy", "evGW LUMO energy"):
         assert abs(dump[key] - g[key]) < 1e-7 * abs(g[key]), key
+
+
+@pytest.mark.gpu
+def test_driver_phbse_matches_oracle(tmp_path, mo_systems):
+    """phBSE switched on in input/options: the drop-ins run the static BSE
+    after G0W0 and evGW and print QuAcK's Tr@BSE lines, which match the CPU
+    restatement of RGW_phBSE at the printed precision."""
+    import numpy as np
+
+    from oracle import gw_oracle as go
+    print("fortran driver phBSE H2O", flush=True)
+    s = mo_systems["H2O"]
+    quackio.write_inputs(str(tmp_path), s, g0w0=True, evgw=True, maxSCF=256, thresh=1e-5, max_diis=1,
+                         eri_mo=s["eri_mo"], bse=True)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    vals = {}
+    for ln in r.stdout.splitlines():
+        for tag in ("G0W0", "evGW"):
+            for man in ("singlet", "triplet"):
+                key = "Tr@BSE@%s@RHF correlation energy (%s) =" % (tag, man)
+                if key in ln:
+                    vals[(tag, man)] = float(ln.split("=")[1].split()[0])
+    assert len(vals) == 4, r.stdout[-3000:]
+    nO, mo = s["nO"], s["eri_mo"]
+    eG0W0 = go.rg0w0(mo, s["eHF"], nO)["eGW"]
+    Ec, _, _ = go.rgw_phbse(mo, s["eHF"], eG0W0, nO)
+    assert abs(vals[("G0W0", "singlet")] - Ec[0]) < 2e-10 and abs(vals[("G0W0", "triplet")] - Ec[1]) < 2e-10
+    # the excitation tables of LR/print_excitation_energies.f90 (method and manifold in A15 fields)
+    assert "phBSE@GW@RHF" in r.stdout and "singlet manifold" in r.stdout and "triplet manifold" in r.stdout
+    assert np.isfinite(vals[("evGW", "singlet")])

@@ -256,3 +256,22 @@ def test_dfgen_torch_layout():
     assert B.shape == (12, 12, 9) and np.allclose(B, np.transpose(B, (1, 0, 2)))
     ov = B[4:, :4, :]
     assert abs((ov * ov).sum(axis=2).max() - 0.1) < 1e-14
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+@pytest.mark.parametrize("name", ["He", "H2O", "H10"])
+def test_phbse_vs_compiled_reference(mo_systems, name):
+    """Static phBSE@G0W0 (GW/RGW_phBSE.f90): the restatement against QuAcK's
+    own phRLR_A/B, phRLR, RGW_excitation_density and static kernels composed
+    in RGW_phBSE's order, for full and TDA BSE, with full and TDA screening."""
+    s = mo_systems[name]
+    nO, mo = s["nO"], s["eri_mo"]
+    eGW = go.rg0w0(mo, s["eHF"], nO)["eGW"]
+    for tda_w, tda, eta in ((False, False, 0.0), (False, True, 0.0), (True, False, 0.05)):
+        r = qkref.phbse(mo, s["eHF"], eGW, nO, eta=eta, tda_w=tda_w, tda=tda)
+        Ec, OmS, OmT = go.rgw_phbse(mo, s["eHF"], eGW, nO, eta=eta, TDA_W=tda_w, TDA=tda)
+        assert np.abs(OmS - r["OmS"]).max() < 1e-10, (tda_w, tda)
+        assert np.abs(OmT - r["OmT"]).max() < 1e-10, (tda_w, tda)
+        assert np.abs(Ec - r["EcBSE"]).max() < 1e-10, (tda_w, tda)
+        # the BSE kernel moves the excitations off the RPA ones it starts from
+        assert np.abs(OmS - OmT).max() > 1e-3
