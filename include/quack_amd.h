@@ -108,6 +108,13 @@ int qk_gw_screen(qk_gw_handle h, const double *e, int TDA_W);
  * Sig(p) = Sigma_c(w_p) and dSig(p) = dSigma_c/dw(w_p)
  * for every active orbital, G built from eG (GW/RGW_Re_SigC.f90,
  * GW/RGW_Re_dSigC.f90 and the SRG variants; zero for frozen orbitals). */
+/* The full correlation self-energy matrix of qsGW at energies e
+ * (GW/RGW_self_energy.f90, or GW/RGW_SRG_self_energy.f90 with doSRG), from the
+ * screening of the last qk_gw_screen (which must run in precision 0: FP64
+ * excitation densities, else QK_E_UNSUPPORTED): SigC nOrb x nOrb
+ * column-major (not yet symmetrised), Z, EcGM, and the screening's EcRPA. */
+int qk_gw_sigma_full(qk_gw_handle h, const double *e, double eta, int doSRG, double *SigC, double *Z, double *EcGM,
+                     double *EcRPA);
 int qk_gw_sigma_at(qk_gw_handle h, const double *eG, const double *w, double eta, int doSRG, double *Sig,
                    double *dSig);
 

@@ -367,3 +367,92 @@ def rgw_phbse(eri, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, singlet=True, t
         EcBSE *= 0.5
     return EcBSE, Om[1], Om[2]
 
+
+
+def rgw_self_energy(eta, e, Om, rho, nO, nC=0, nR=0):
+    """GW/RGW_self_energy.f90: the full correlation self-energy
+    SigC(p,q) = sum_{r,m} 2 rho(p,r,m) rho(q,r,m) eps/(eps^2 + eta^2) with
+    eps = e_p - e_r + Om_m (r occupied) or e_p - e_r - Om_m (r virtual), the
+    renormalisation Z(p) and EcGM.  Returns EcGM, SigC, Z."""
+    n = len(e)
+    act, occ, vir = slice(nC, n - nR), slice(nC, nO), slice(nO, n - nR)
+    Sig = np.zeros((n, n))
+    Zs = np.zeros(n)
+    for rs, sgn in ((occ, 1.0), (vir, -1.0)):
+        eps = e[act, None, None] - e[None, rs, None] + sgn * Om[None, None, :]
+        r_ = rho[act, rs, :]
+        w = eps / (eps ** 2 + eta ** 2)
+        Sig[act, act] += 2.0 * np.einsum("prm,prm,qrm->pq", r_, w, r_, optimize=True)
+        Zs[act] -= np.einsum("prm,prm->p", 2.0 * r_ ** 2, (eps ** 2 - eta ** 2) / (eps ** 2 + eta ** 2) ** 2)
+    Z = 1.0 / (1.0 - Zs)
+    eps = e[vir, None, None] - e[None, occ, None] + Om[None, None, :]
+    EcGM = -np.sum(4.0 * rho[vir, occ, :] ** 2 * eps / (eps ** 2 + eta ** 2))
+    return EcGM, Sig, Z
+
+
+def rgw_srg_self_energy(flow, e, Om, rho, nO, nC=0, nR=0):
+    """GW/RGW_SRG_self_energy.f90: SigC(p,q) = sum_{r,m} 2 rho(p,r,m) rho(q,r,m)
+    (1 - exp(-s D_p^2) exp(-s D_q^2)) (D_p + D_q)/(D_p^2 + D_q^2), D_x = e_x -
+    e_r +- Om_m; Z(p) and EcGM with the SRG factors.  Returns EcGM, SigC, Z."""
+    n = len(e)
+    s = flow
+    act, occ, vir = slice(nC, n - nR), slice(nC, nO), slice(nO, n - nR)
+    Sig = np.zeros((n, n))
+    Zs = np.zeros(n)
+    for rs, sgn in ((occ, 1.0), (vir, -1.0)):
+        D = e[act, None, None] - e[None, rs, None] + sgn * Om[None, None, :]       # [p, r, m]
+        r_ = rho[act, rs, :]
+        Dp, Dq = D[:, None], D[None, :]                                            # [p, q, r, m]
+        w = (1.0 - np.exp(-s * Dp * Dp) * np.exp(-s * Dq * Dq)) * (Dp + Dq) / (Dp * Dp + Dq * Dq)
+        Sig[act, act] += 2.0 * np.einsum("prm,qrm,pqrm->pq", r_, r_, w, optimize=True)
+        Zs[act] -= np.sum(2.0 * r_ ** 2 * (1.0 - np.exp(-2.0 * s * D * D)) / D ** 2, axis=(1, 2))
+    Z = 1.0 / (1.0 - Zs)
+    D = e[vir, None, None] - e[None, occ, None] + Om[None, None, :]
+    EcGM = -np.sum(4.0 * rho[vir, occ, :] ** 2 * (1.0 - np.exp(-2.0 * s * D * D)) / D)
+    return EcGM, Sig, Z
+
+
+def qsrgw(S, T, V, eri_ao, X, cHF, PHF, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, eta=0.0, doSRG=False,
+          TDA_W=False):
+    """GW/qsRGW.f90:137-222 for nBas == nOrb (no linDM, no BSE): per cycle J and
+    K from the AO integrals, the MO integrals of the current orbitals, dRPA
+    screening at eGW, rho, the full self-energy (symmetrised, to the AO basis),
+    F = Hc + J + K/2 + SigC_AO, the commutator (Conv from nSCF > 1, nSCF
+    starting at -1), DIIS on F, diagonalisation of X^T F X.  Returns a dict
+    with eGW, c, EqsGW (electronic), EcRPA, EcGM, Z, nSCF, Conv, converged."""
+    from .scf_oracle import exchange_matrix, hartree_matrix
+    n = S.shape[0]
+    Hc = T + V
+    nSCF, Conv = -1, 1.0
+    P, eGW, c = PHF.copy(), eHF.copy(), cHF.copy()
+    diis = DIIS(n * n, max(1, max_diis))
+    EqsGW = EcRPA = EcGM = 0.0
+    Z = np.ones(n)
+    while Conv > thresh and nSCF < maxSCF:
+        nSCF += 1
+        J = hartree_matrix(P, eri_ao, n)
+        K = exchange_matrix(P, eri_ao, n)
+        mo = aotomo_eri_rhf(c, eri_ao)
+        A = phRLR_A(eGW, mo, nO, dRPA=True)
+        B = np.zeros_like(A) if TDA_W else phRLR_B(mo, n, nO, dRPA=True)
+        EcRPA, Om, XpY, _ = phRLR(TDA_W, A, B)
+        rho = excitation_density(mo, XpY, n, nO)
+        if doSRG:
+            EcGM, SigC, Z = rgw_srg_self_energy(FLOW, eGW, Om, rho, nO)
+        else:
+            EcGM, SigC, Z = rgw_self_energy(eta, eGW, Om, rho, nO)
+        SigC = 0.5 * (SigC + SigC.T)
+        SigCp = S @ c @ SigC @ c.T @ S                       # AOtoMO/MOtoAO.f90
+        F = Hc + J + 0.5 * K + SigCp
+        err = F @ P @ S - S @ P @ F
+        if nSCF > 1:
+            Conv = np.abs(err).max()
+        EqsGW = np.trace(P @ T) + np.trace(P @ V) + 0.5 * np.trace(P @ J) + 0.25 * np.trace(P @ K)
+        if max_diis > 1:
+            f, _ = diis(err.ravel(order="F"), F.ravel(order="F").copy())
+            F = f.reshape(n, n, order="F")
+        eGW, cp = diagonalize_matrix(X.T @ F @ X)
+        c = X @ cp
+        P = 2.0 * c[:, :nO] @ c[:, :nO].T
+    return {"eGW": eGW, "c": c, "EqsGW": EqsGW, "EcRPA": EcRPA, "EcGM": EcGM, "Z": Z, "nSCF": nSCF, "Conv": Conv,
+            "converged": nSCF < maxSCF}

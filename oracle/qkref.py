@@ -43,6 +43,8 @@ def load():
         lib.qkref_evrgw.argtypes = [_I, _I, _I, _I, _D, _I, _D] + [_I] * 6 + [_D, _D] + [_P] * 7 + [_IP]
         lib.qkref_evrgw.restype = _I
         lib.qkref_phbse.argtypes = [_I] * 4 + [_D] + [_I] * 6 + [_P] * 6
+        lib.qkref_self_energy_full.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
+        lib.qkref_qsgw_cycle.argtypes = [_I, _I, _D] + [_I] * 6 + [_P] * 10
         _lib = lib
     return _lib
 
@@ -144,4 +146,29 @@ def phbse(eri_mo, eW, eGW, nO, eta=0.0, tda_w=False, tda=False, singlet=True, tr
     load().qkref_phbse(int(tda_w), int(tda), int(singlet), int(triplet), eta, nOrb, nC, nO, nV, nR, nS,
                        _p(np.ascontiguousarray(eri_mo)), _p(_f(eW)), _p(_f(eGW)), _p(Ec), _p(OmS), _p(OmT))
     return {"EcBSE": Ec, "OmS": OmS, "OmT": OmT}
+
+
+def self_energy_full(e, Om, rho, nO, eta=0.0, srg=False, nC=0, nR=0):
+    """GW/RGW_self_energy.f90 (or RGW_SRG_self_energy.f90): full SigC, Z, EcGM."""
+    nOrb = len(e)
+    nC, nO, nV, nR, nS = _dims(nOrb, nO, nC, nR)
+    Ec = np.zeros(1)
+    Sig = np.zeros((nOrb, nOrb), order="F")
+    Z = np.zeros(nOrb)
+    load().qkref_self_energy_full(int(srg), eta, nOrb, nC, nO, nV, nR, nS, _p(_f(e)), _p(_f(Om)), _p(_f(rho)),
+                                  _p(Ec), _p(Sig), _p(Z))
+    return {"EcGM": Ec[0], "SigC": Sig, "Z": Z}
+
+
+def qsgw_cycle(S, T, V, X, c, P, eGW, eri_ao, nO, eta=0.0, srg=False, tda_w=False):
+    """One cycle of GW/qsRGW.f90 through QuAcK's own routines: F and the new
+    quasiparticle energies."""
+    n = S.shape[0]
+    nC, nO, nV, nR, nS = _dims(n, nO)
+    F = np.zeros((n, n), order="F")
+    e = np.zeros(n)
+    load().qkref_qsgw_cycle(int(srg), int(tda_w), eta, n, nC, nO, nV, nR, nS, _p(_f(S)), _p(_f(T)), _p(_f(V)),
+                            _p(_f(X)), _p(_f(c)), _p(_f(P)), _p(_f(eGW)), _p(np.ascontiguousarray(eri_ao)), _p(F),
+                            _p(e))
+    return {"F": F, "eGW": e}
 

@@ -234,3 +234,61 @@ subroutine qkref_phbse(tda_w, tda, dosing, dotrip, eta, nOrb, nC, nO, nV, nR, nS
   flush(6)
   deallocate(Aph, Bph, KA, KB, OmRPA, XpY, XmY, rho)
 end subroutine
+
+! GW/RGW_self_energy.f90 / GW/RGW_SRG_self_energy.f90: the full self-energy
+subroutine qkref_self_energy_full(srg, eta, nOrb, nC, nO, nV, nR, nS, e, Om, rho, EcGM, SigC, Z) &
+    bind(C, name='qkref_self_energy_full')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: srg, nOrb, nC, nO, nV, nR, nS
+  real(c_double), value :: eta
+  real(c_double), intent(in) :: e(nOrb), Om(nS), rho(nOrb, nOrb, nS)
+  real(c_double), intent(out) :: EcGM, SigC(nOrb, nOrb), Z(nOrb)
+  if(srg /= 0) then
+    call RGW_SRG_self_energy(500d0, nOrb, nOrb, nC, nO, nV, nR, nS, e, Om, rho, EcGM, SigC, Z)
+  else
+    call RGW_self_energy(eta, nOrb, nOrb, nC, nO, nV, nR, nS, e, Om, rho, EcGM, SigC, Z)
+  end if
+  flush(6)
+end subroutine
+
+! One cycle of GW/qsRGW.f90:137-207 (nBas == nOrb, DIIS history of one, which
+! leaves F unchanged), composed of QuAcK's own routines: returns F and the new
+! quasiparticle energies.
+subroutine qkref_qsgw_cycle(srg, tda_w, eta, nBas, nC, nO, nV, nR, nS, S, T, V, X, c, P, eGW, ERI_AO, F, eNew) &
+    bind(C, name='qkref_qsgw_cycle')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: srg, tda_w, nBas, nC, nO, nV, nR, nS
+  real(c_double), value :: eta
+  real(c_double), intent(in) :: S(nBas,nBas), T(nBas,nBas), V(nBas,nBas), X(nBas,nBas), c(nBas,nBas), P(nBas,nBas)
+  real(c_double), intent(in) :: eGW(nBas), ERI_AO(nBas,nBas,nBas,nBas)
+  real(c_double), intent(out) :: F(nBas,nBas), eNew(nBas)
+  double precision, allocatable :: J(:,:), K(:,:), ERI_MO(:,:,:,:), Aph(:,:), Bph(:,:), Om(:), XpY(:,:), XmY(:,:)
+  double precision, allocatable :: rho(:,:,:), SigC(:,:), SigCp(:,:), Z(:), cp(:,:)
+  double precision :: EcRPA, EcGM
+  logical :: TDAW
+  TDAW = tda_w /= 0
+  allocate(J(nBas,nBas), K(nBas,nBas), ERI_MO(nBas,nBas,nBas,nBas), Aph(nS,nS), Bph(nS,nS), Om(nS), &
+           XpY(nS,nS), XmY(nS,nS), rho(nBas,nBas,nS), SigC(nBas,nBas), SigCp(nBas,nBas), Z(nBas), cp(nBas,nBas))
+  Bph = 0d0
+  call Hartree_matrix_AO_basis(nBas, P, ERI_AO, J)
+  call exchange_matrix_AO_basis(nBas, P, ERI_AO, K)
+  call AOtoMO_ERI_RHF(nBas, nBas, c, ERI_AO, ERI_MO)
+  call phRLR_A(1, .true., nBas, nC, nO, nV, nR, nS, 1d0, eGW, ERI_MO, Aph)
+  if(.not. TDAW) call phRLR_B(1, .true., nBas, nC, nO, nV, nR, nS, 1d0, ERI_MO, Bph)
+  call phRLR(TDAW, nS, Aph, Bph, EcRPA, Om, XpY, XmY)
+  call RGW_excitation_density(nBas, nC, nO, nR, nS, ERI_MO, XpY, rho)
+  if(srg /= 0) then
+    call RGW_SRG_self_energy(500d0, nBas, nBas, nC, nO, nV, nR, nS, eGW, Om, rho, EcGM, SigC, Z)
+  else
+    call RGW_self_energy(eta, nBas, nBas, nC, nO, nV, nR, nS, eGW, Om, rho, EcGM, SigC, Z)
+  end if
+  SigC = 0.5d0*(SigC + transpose(SigC))
+  call MOtoAO(nBas, nBas, S, c, SigC, SigCp)
+  F = T + V + J + 0.5d0*K + SigCp
+  cp = matmul(transpose(X), matmul(F, X))
+  call diagonalize_matrix(nBas, cp, eNew)
+  flush(6)
+  deallocate(J, K, ERI_MO, Aph, Bph, Om, XpY, XmY, rho, SigC, SigCp, Z, cp)
+end subroutine

@@ -1240,6 +1240,37 @@ int qk_gw_screen(qk_gw_handle c, const double* e, int TDA_W) {
     });
 }
 
+int qk_gw_sigma_full(qk_gw_handle c, const double* e, double eta, int doSRG, double* SigC, double* Z, double* EcGM,
+                     double* EcRPA) {
+    if (!c || !e || !SigC || !Z) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        QK_CHECK(c->have_screen, QK_E_ARG, "no screening: call qk_gw_screen first");
+        QK_CHECK(!c->rho_f32, QK_E_UNSUPPORTED, "the full self-energy reads FP64 excitation densities (precision 0)");
+        const int n = c->nOrb;
+        const double ecrpa = d2h_scalar(c, c->scal + 1);  // the screening's, before the sums below overwrite scal
+        h2d(c, c->eG, e, n);
+        // Z and EcGM: the diagonal routine's (GW/RGW_self_energy.f90 computes them with the same formulas)
+        sigma_partial(c->s, c->rhoT, false, c->eG, c->eG, c->Om, nullptr, eta, doSRG, SRG_FLOW, n, c->nC, c->nO, c->nR,
+                      c->nS, c->nchunk, c->part);
+        sigma_finalize(c->s, c->part, c->nchunk, n, c->nC, c->nO, c->nR, c->Sig, c->Z, c->scal);
+        double* full = dalloc<double>((size_t)n * n);
+        try {
+            sigma_full(c->s, c->rhoT, c->eG, c->Om, eta, doSRG, SRG_FLOW, n, c->nC, c->nO, c->nR, c->nS, full);
+            QK_HIP(hipMemcpyAsync(SigC, full, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->s));
+            d2h(c, Z, c->Z, n);
+            const double ecgm = d2h_scalar(c, c->scal + 2);
+            QK_HIP(hipStreamSynchronize(c->s));
+            if (EcGM) *EcGM = ecgm;
+            if (EcRPA) *EcRPA = ecrpa;
+        } catch (...) {
+            (void)hipStreamSynchronize(c->s);
+            dfree(full);
+            throw;
+        }
+        dfree(full);
+    });
+}
+
 int qk_gw_sigma_at(qk_gw_handle c, const double* eG, const double* w, double eta, int doSRG, double* Sig,
                    double* dSig) {
     if (!c || !eG || !w || !Sig || !dSig) return QK_E_ARG;

@@ -689,6 +689,71 @@ void rho_block(hipStream_t s, const double* rhoT, const double* Om, double eta, 
     QK_HIP(hipGetLastError());
 }
 
+// The full correlation self-energy of qsGW (GW/RGW_self_energy.f90,
+// GW/RGW_SRG_self_energy.f90): Sig(p,q) = 2 sum_{r,m} rho(p,r,m) rho(q,r,m) w,
+// D_x = e_x - e_r + Om_m (r occupied) or e_x - e_r - Om_m (r virtual),
+//   w = D_p / (D_p^2 + eta^2)                                      (plain)
+//   w = (1 - exp(-s D_p^2) exp(-s D_q^2)) (D_p + D_q) / (D_p^2 + D_q^2)  (SRG).
+// One thread per (p, q) of a 16 x 16 tile of the active orbitals; the tile's
+// rho(p, r, m) and rho(q, r, m) rows come through LDS in chunks of 64 modes
+// (m contiguous in rhoT, so every row load is coalesced).
+template <bool SRG>
+__global__ __launch_bounds__(256) void k_sigma_full(const double* __restrict__ rhoT, const double* __restrict__ e,
+                                                    const double* __restrict__ Om, double eta, double flow, int n,
+                                                    int nC, int nO, int nR, int nS, double* __restrict__ Sig) {
+    constexpr int TT = 16, MC = 64;
+    __shared__ double sp[TT][MC + 1], sq[TT][MC + 1], om[MC];
+    const int hi = n - nR;  // active orbitals nC .. hi-1
+    const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;
+    const int p0 = nC + blockIdx.x * TT, q0 = nC + blockIdx.y * TT;
+    const int p = p0 + tx, q = q0 + ty;
+    const bool on = p < hi && q < hi;
+    const double ep = p < hi ? e[p] : 0.0, eq = q < hi ? e[q] : 0.0;
+    double acc = 0.0;
+    for (int r = nC; r < hi; ++r) {
+        const double er = e[r];
+        const double sgn = r < nO ? 1.0 : -1.0;
+        for (int m0 = 0; m0 < nS; m0 += MC) {
+            for (int t = threadIdx.x; t < TT * MC; t += blockDim.x) {
+                const int row = t / MC, mm = t % MC, m = m0 + mm;
+                const int pr = p0 + row, qr = q0 + row;
+                sp[row][mm] = (m < nS && pr < hi) ? rhoT[m + (long long)nS * (pr + (long long)n * r)] : 0.0;
+                sq[row][mm] = (m < nS && qr < hi) ? rhoT[m + (long long)nS * (qr + (long long)n * r)] : 0.0;
+            }
+            if (threadIdx.x < MC) om[threadIdx.x] = m0 + (int)threadIdx.x < nS ? Om[m0 + threadIdx.x] : 0.0;
+            __syncthreads();
+            const int mmax = min(MC, nS - m0);
+            for (int mm = 0; mm < mmax; ++mm) {
+                const double o = sgn * om[mm];
+                const double dp = ep - er + o;
+                double w;
+                if constexpr (SRG) {
+                    const double dq = eq - er + o;
+                    w = (1.0 - exp(-flow * dp * dp) * exp(-flow * dq * dq)) * (dp + dq) / (dp * dp + dq * dq);
+                } else {
+                    w = dp / (dp * dp + eta * eta);
+                }
+                acc += sp[tx][mm] * sq[ty][mm] * w;
+            }
+            __syncthreads();
+        }
+    }
+    if (on) Sig[p + (long long)n * q] = 2.0 * acc;
+}
+
+void sigma_full(hipStream_t s, const double* rhoT, const double* e, const double* Om, double eta, int srg, double flow,
+                int n, int nC, int nO, int nR, int nS, double* Sig) {
+    QK_HIP(hipMemsetAsync(Sig, 0, sizeof(double) * (size_t)n * n, s));
+    const int nact = n - nR - nC;
+    if (nact <= 0) return;
+    dim3 g((nact + 15) / 16, (nact + 15) / 16);
+    if (srg)
+        hipLaunchKernelGGL(k_sigma_full<true>, g, dim3(256), 0, s, rhoT, e, Om, eta, flow, n, nC, nO, nR, nS, Sig);
+    else
+        hipLaunchKernelGGL(k_sigma_full<false>, g, dim3(256), 0, s, rhoT, e, Om, eta, flow, n, nC, nO, nR, nS, Sig);
+    QK_HIP(hipGetLastError());
+}
+
 void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag) {
     hipLaunchKernelGGL(k_lin, dim3((n + 255) / 256), dim3(256), 0, s, e0, Z, Sig, out, n, zflag);
     QK_HIP(hipGetLastError());

@@ -30,7 +30,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
            "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
-           "qk_rgw_phbse", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms"]
+           "qk_rgw_phbse", "qk_gw_sigma_full", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms"]
 
 
 class QKError(RuntimeError):
@@ -120,6 +120,7 @@ def load():
     lib.qk_rgw_self_energy_diag.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
     lib.qk_rgw_qp_graph.argtypes = [_I, _D, _D] + [_I] * 6 + [_P] * 7
     lib.qk_rgw_phbse.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 5
+    lib.qk_gw_sigma_full.argtypes = [_H, _P, _D, _I, _P, _P, _P, _P]
     lib.qk_rhf_create.argtypes = [ctypes.POINTER(_H), _I]
     lib.qk_rhf_destroy.argtypes = [_H]
     lib.qk_rhf_last_error.argtypes = [_H]
@@ -271,6 +272,17 @@ class GWEngine:
 
     def screen(self, e, TDA_W=False):
         _check(load().qk_gw_screen(self.h, _p(_f64(e)), int(TDA_W)), self.h)
+
+    def sigma_full(self, e, eta=0.0, doSRG=False):
+        """Full self-energy matrix of qsGW at e (qk_gw_sigma_full) from the last
+        screen(); returns dict(SigC (nOrb x nOrb), Z, EcGM, EcRPA)."""
+        n = self.nOrb
+        Sig = np.zeros((n, n), order="F")
+        Z = np.zeros(n)
+        Ec, Er = _D(0.0), _D(0.0)
+        _check(load().qk_gw_sigma_full(self.h, _p(_f64(e)), eta, int(doSRG), _p(Sig), _p(Z), ctypes.byref(Ec),
+                                       ctypes.byref(Er)), self.h)
+        return {"SigC": Sig, "Z": Z, "EcGM": Ec.value, "EcRPA": Er.value}
 
     def sigma_at(self, eG, w, eta=0.0, doSRG=False):
         n = self.nOrb

@@ -275,3 +275,47 @@ def test_phbse_vs_compiled_reference(mo_systems, name):
         assert np.abs(Ec - r["EcBSE"]).max() < 1e-10, (tda_w, tda)
         # the BSE kernel moves the excitations off the RPA ones it starts from
         assert np.abs(OmS - OmT).max() > 1e-3
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+@pytest.mark.parametrize("name", ["H2O", "H10"])
+def test_full_self_energy_vs_compiled_reference(mo_systems, name):
+    """GW/RGW_self_energy.f90 and RGW_SRG_self_energy.f90 (the full matrix qsGW
+    uses): restatement against QuAcK's compiled routines; the diagonal, Z and
+    EcGM also equal the diagonal routines'."""
+    s = mo_systems[name]
+    n, nO, mo = len(s["eHF"]), s["nO"], s["eri_mo"]
+    A, B = go.phRLR_A(s["eHF"], mo, nO), go.phRLR_B(mo, n, nO)
+    _, Om, XpY, _ = go.phRLR(False, A, B)
+    rho = go.excitation_density(mo, XpY, n, nO)
+    for srg, eta in ((False, 0.0), (False, 0.05), (True, 0.0)):
+        r = qkref.self_energy_full(s["eHF"], Om, rho, nO, eta=eta, srg=srg)
+        if srg:
+            Ec, Sig, Z = go.rgw_srg_self_energy(go.FLOW, s["eHF"], Om, rho, nO)
+            Sd, Zd, Ecd = go.srg_self_energy_diag(s["eHF"], Om, rho, nO)
+        else:
+            Ec, Sig, Z = go.rgw_self_energy(eta, s["eHF"], Om, rho, nO)
+            Sd, Zd, Ecd = go.self_energy_diag(s["eHF"], Om, rho, nO, eta)
+        assert np.abs(Sig - r["SigC"]).max() < 1e-11, (srg, eta)
+        assert np.abs(Z - r["Z"]).max() < 1e-11 and abs(Ec - r["EcGM"]) < 1e-11
+        assert np.abs(np.diag(Sig) - Sd).max() < 1e-12 and np.abs(Z - Zd).max() < 1e-12 and abs(Ec - Ecd) < 1e-12
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+@pytest.mark.parametrize("name", ["He", "H2O"])
+def test_qsgw_cycle_vs_compiled_reference(systems, name):
+    """The first qsGW cycle of the restatement (J, K, MO integrals, screening,
+    full self-energy, MOtoAO, F, diagonalisation) against the same cycle
+    composed of QuAcK's own routines."""
+    s = systems[name]
+    mol = s["mol"]
+    S, T, V = mol.one_electron()
+    s_, U = np.linalg.eigh(S)
+    X = U / np.sqrt(s_)
+    nO, c = s["nO"], np.asarray(s["c"])
+    P = 2.0 * c[:, :nO] @ c[:, :nO].T
+    for srg, eta, tda_w in ((False, 0.0, False), (True, 0.0, False), (False, 0.05, True)):
+        ref = qkref.qsgw_cycle(S, T, V, X, c, P, s["eHF"], s["eri"], nO, eta=eta, srg=srg, tda_w=tda_w)
+        r = go.qsrgw(S, T, V, s["eri"], X, c, P, s["eHF"], nO, maxSCF=0, eta=eta, doSRG=srg, TDA_W=tda_w)
+        assert r["nSCF"] == 0 and not r["converged"]
+        assert np.abs(r["eGW"] - ref["eGW"]).max() < 1e-10, (name, srg, eta, tda_w)
