@@ -414,24 +414,26 @@ def rgw_srg_self_energy(flow, e, Om, rho, nO, nC=0, nR=0):
 
 def qsrgw(S, T, V, eri_ao, X, cHF, PHF, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, eta=0.0, doSRG=False,
           TDA_W=False):
-    """GW/qsRGW.f90:137-222 for nBas == nOrb (no linDM, no BSE): per cycle J and
+    """GW/qsRGW.f90:193-352 (no linDM, no BSE): per cycle J and
     K from the AO integrals, the MO integrals of the current orbitals, dRPA
     screening at eGW, rho, the full self-energy (symmetrised, to the AO basis),
     F = Hc + J + K/2 + SigC_AO, the commutator (Conv from nSCF > 1, nSCF
-    starting at -1), DIIS on F, diagonalisation of X^T F X.  Returns a dict
-    with eGW, c, EqsGW (electronic), EcRPA, EcGM, Z, nSCF, Conv, converged."""
+    starting at -1), DIIS on F, diagonalisation of X^T F X (nBas == nOrb) or
+    of C^T F C (nBas > nOrb, where F is first projected on the kept orbitals,
+    :281-284, :323-333).  X: nBas x nOrb.  Returns a dict with eGW, c, EqsGW
+    (electronic), EcRPA, EcGM, Z, nSCF, Conv, converged."""
     from .scf_oracle import exchange_matrix, hartree_matrix
-    n = S.shape[0]
+    nB, n = X.shape
     Hc = T + V
     nSCF, Conv = -1, 1.0
     P, eGW, c = PHF.copy(), eHF.copy(), cHF.copy()
-    diis = DIIS(n * n, max(1, max_diis))
+    diis = DIIS(nB * nB, max(1, max_diis))
     EqsGW = EcRPA = EcGM = 0.0
     Z = np.ones(n)
     while Conv > thresh and nSCF < maxSCF:
         nSCF += 1
-        J = hartree_matrix(P, eri_ao, n)
-        K = exchange_matrix(P, eri_ao, n)
+        J = hartree_matrix(P, eri_ao, nB)
+        K = exchange_matrix(P, eri_ao, nB)
         mo = aotomo_eri_rhf(c, eri_ao)
         A = phRLR_A(eGW, mo, nO, dRPA=True)
         B = np.zeros_like(A) if TDA_W else phRLR_B(mo, n, nO, dRPA=True)
@@ -444,15 +446,21 @@ def qsrgw(S, T, V, eri_ao, X, cHF, PHF, eHF, nO, maxSCF=64, thresh=1e-5, max_dii
         SigC = 0.5 * (SigC + SigC.T)
         SigCp = S @ c @ SigC @ c.T @ S                       # AOtoMO/MOtoAO.f90
         F = Hc + J + 0.5 * K + SigCp
+        if nB != n:
+            F = S @ c @ (c.T @ F @ c) @ c.T @ S
         err = F @ P @ S - S @ P @ F
         if nSCF > 1:
             Conv = np.abs(err).max()
         EqsGW = np.trace(P @ T) + np.trace(P @ V) + 0.5 * np.trace(P @ J) + 0.25 * np.trace(P @ K)
         if max_diis > 1:
             f, _ = diis(err.ravel(order="F"), F.ravel(order="F").copy())
-            F = f.reshape(n, n, order="F")
-        eGW, cp = diagonalize_matrix(X.T @ F @ X)
-        c = X @ cp
+            F = f.reshape(nB, nB, order="F")
+        if nB == n:
+            eGW, cp = diagonalize_matrix(X.T @ F @ X)
+            c = X @ cp
+        else:
+            eGW, cp = diagonalize_matrix(c.T @ F @ c)
+            c = c @ cp
         P = 2.0 * c[:, :nO] @ c[:, :nO].T
     return {"eGW": eGW, "c": c, "EqsGW": EqsGW, "EcRPA": EcRPA, "EcGM": EcGM, "Z": Z, "nSCF": nSCF, "Conv": Conv,
             "converged": nSCF < maxSCF}

@@ -252,7 +252,7 @@ subroutine qkref_self_energy_full(srg, eta, nOrb, nC, nO, nV, nR, nS, e, Om, rho
   flush(6)
 end subroutine
 
-! One cycle of GW/qsRGW.f90:137-207 (nBas == nOrb, DIIS history of one, which
+! One cycle of GW/qsRGW.f90:193-333 (nBas == nOrb, DIIS history of one, which
 ! leaves F unchanged), composed of QuAcK's own routines: returns F and the new
 ! quasiparticle energies.
 subroutine qkref_qsgw_cycle(srg, tda_w, eta, nBas, nC, nO, nV, nR, nS, S, T, V, X, c, P, eGW, ERI_AO, F, eNew) &

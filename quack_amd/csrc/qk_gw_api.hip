@@ -275,6 +275,8 @@ struct qk_gw_ctx {
     bool use_df = false;
     int naux = 0;
     double *Bdf = nullptr, *Bov = nullptr, *Tdf = nullptr;
+    // work of the full self-energy's matrix-core path (qk_gw_sigma_full), kept across calls
+    double* sigw = nullptr;
     qk::Split3 Bsplit, BovTsplit, Tsplit;  // B (rows pq), Bov^T (rows P, k = jb), T (rows m)
     bool bsplit_valid = false;
     // eigensolver: 0 = rocSOLVER dsyevd for every RPA (QuAcK's scheme); 1 = refine
@@ -985,7 +987,7 @@ void release(Ctx* c) {
         c->info = c->active = nullptr;
     }
     double* ds[] = {c->M, c->K, c->Cm, c->rhoT, c->e, c->e0, c->eG, c->dE, c->sd, c->Om, c->Om0, c->Sig, c->Z,
-                    c->eLin, c->eGW, c->w, c->f, c->df, c->part, c->scal, c->work};
+                    c->eLin, c->eGW, c->w, c->f, c->df, c->part, c->scal, c->work, c->sigw};
     for (double* p : ds)
         if (p) (void)hipFree(p);
     int* is[] = {c->nit, c->done, c->info, c->active};
@@ -1255,7 +1257,10 @@ int qk_gw_sigma_full(qk_gw_handle c, const double* e, double eta, int doSRG, dou
         sigma_finalize(c->s, c->part, c->nchunk, n, c->nC, c->nO, c->nR, c->Sig, c->Z, c->scal);
         double* full = dalloc<double>((size_t)n * n);
         try {
-            sigma_full(c->s, c->rhoT, c->eG, c->Om, eta, doSRG, SRG_FLOW, n, c->nC, c->nO, c->nR, c->nS, full);
+            // weighted densities (n^2 nS) and one n x n product per orbital r (n^3)
+            if (!doSRG && !c->sigw) c->sigw = dalloc<double>((size_t)n * n * c->nS + (size_t)n * n * n);
+            sigma_full(c->s, c->rhoT, c->eG, c->Om, eta, doSRG, SRG_FLOW, n, c->nC, c->nO, c->nR, c->nS, full,
+                       doSRG ? nullptr : c->sigw);
             QK_HIP(hipMemcpyAsync(SigC, full, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->s));
             d2h(c, Z, c->Z, n);
             const double ecgm = d2h_scalar(c, c->scal + 2);

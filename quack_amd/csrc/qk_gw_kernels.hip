@@ -741,11 +741,68 @@ __global__ __launch_bounds__(256) void k_sigma_full(const double* __restrict__ r
     if (on) Sig[p + (long long)n * q] = 2.0 * acc;
 }
 
+// Plain weight, which depends on p, r and m only: X[m + nS (p + n r)] =
+// 2 rho(p,r,m) D_p / (D_p^2 + eta^2) for active p and r, else 0; then
+// Sig = sum_r X_r^T R_r (R_r the nS x n slab of rhoT at r) is a sum of
+// matrix-core products.
+__global__ void k_sigma_weight(const double* __restrict__ rhoT, const double* __restrict__ e,
+                               const double* __restrict__ Om, double eta, int n, int nC, int nO, int nR, int nS,
+                               double* __restrict__ X) {
+    const long long total = (long long)nS * n * n;
+    const int hi = n - nR;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int m = (int)(t % nS);
+        const long long pr = t / nS;
+        const int p = (int)(pr % n), r = (int)(pr / n);
+        double v = 0.0;
+        if (p >= nC && p < hi && r >= nC && r < hi) {
+            const double d = e[p] - e[r] + (r < nO ? Om[m] : -Om[m]);
+            v = 2.0 * rhoT[t] * d / (d * d + eta * eta);
+        }
+        X[t] = v;
+    }
+}
+
+// Sig(nC + i, nC + j) = sum over the nact products of parts[k nn + i + nact j]
+__global__ void k_sum_parts(const double* __restrict__ parts, int nact, long long nn, int n, int nC,
+                            double* __restrict__ Sig) {
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nn; t += (long long)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int k = 0; k < nact; ++k) v += parts[k * nn + t];
+        const int i = (int)(t % nact), j = (int)(t / nact);
+        Sig[(nC + i) + (long long)n * (nC + j)] = v;
+    }
+}
+
 void sigma_full(hipStream_t s, const double* rhoT, const double* e, const double* Om, double eta, int srg, double flow,
-                int n, int nC, int nO, int nR, int nS, double* Sig) {
+                int n, int nC, int nO, int nR, int nS, double* Sig, double* work) {
     QK_HIP(hipMemsetAsync(Sig, 0, sizeof(double) * (size_t)n * n, s));
     const int nact = n - nR - nC;
     if (nact <= 0) return;
+    if (!srg && work) {
+        const long long total = (long long)nS * n * n;
+        hipLaunchKernelGGL(k_sigma_weight, dim3((unsigned)std::min<long long>((total + 255) / 256, 65536)), dim3(256),
+                           0, s, rhoT, e, Om, eta, n, nC, nO, nR, nS, work);
+        QK_HIP(hipGetLastError());
+        // the active block only (inactive rows and columns stay zero): one
+        // product per active r, batched, then their sum in a fixed order
+        const long long slab = (long long)nS * n, off = (long long)nS * nC;
+        double* parts = work + slab * n;  // nact products of nact x nact
+        Batch bt;
+        bt.count = nact;
+        bt.zdiv = nact;
+        bt.sa0 = slab;
+        bt.sb0 = slab;
+        bt.sc0 = (long long)nact * nact;
+        dgemm(s, true, false, nact, nact, nS, 1.0, work + slab * nC + off, nS, rhoT + slab * nC + off, nS, 0.0, parts,
+              nact, bt);
+        const long long nn = (long long)nact * nact;
+        hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<long long>((nn + 255) / 256, 4096)), dim3(256), 0, s,
+                           parts, nact, nn, n, nC, Sig);
+        QK_HIP(hipGetLastError());
+        return;
+    }
     dim3 g((nact + 15) / 16, (nact + 15) / 16);
     if (srg)
         hipLaunchKernelGGL(k_sigma_full<true>, g, dim3(256), 0, s, rhoT, e, Om, eta, flow, n, nC, nO, nR, nS, Sig);

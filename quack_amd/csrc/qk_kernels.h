@@ -24,8 +24,9 @@ void sigma_finalize(hipStream_t s, const double* part, int nchunk, int n, int nC
                     double* Z, double* scal);
 void sigma_out(hipStream_t s, const double* part, int nchunk, int n, int nC, int nR, double* Sig, double* dSig);
 void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag);
+// work: nS n^2 + n^3 doubles for the plain weight's matrix-core path (NULL: the direct kernel)
 void sigma_full(hipStream_t s, const double* rhoT, const double* e, const double* Om, double eta, int srg, double flow,
-                int n, int nC, int nO, int nR, int nS, double* Sig);
+                int n, int nC, int nO, int nR, int nS, double* Sig, double* work);
 void rho_block(hipStream_t s, const double* rhoT, const double* Om, double eta, int weighted, int n, int nS, int p0,
                int P, int q0, int Q, double* out);
 void qp_init(hipStream_t s, const double* eGWlin, double* w, double* f, int* nit, int* done, int n, int nC, int nR);

@@ -1,6 +1,6 @@
 """Quasiparticle self-consistent GW (qsGW@RHF) on the device.
 
-QuAcK's qsRGW loop (GW/qsRGW.f90:137-222, for nBas == nOrb, without linDM):
+QuAcK's qsRGW loop (GW/qsRGW.f90:193-352, without linDM):
 per cycle
   * J and K from the AO integrals resident in HBM (qk_rhf_jk),
   * the MO integral block of the current orbitals (the engine's AO->MO block)
@@ -30,26 +30,24 @@ class QsGWNotConverged(RuntimeError):
 
 def qsRGW(eri_ptr, S, T, V, X, cHF, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, eta=0.0, doSRG=False,
           TDA_W=False, log=None):
-    """qsGW from the RHF orbitals cHF and energies eHF.  eri_ptr: device address
-    of the nBas^4 AO integrals (QuAcK's ERI.bin order).  X: the orthogonalisation
-    matrix (nBas x nBas).  Returns dict(eGW, c, EqsGW (electronic), EcRPA, EcGM,
-    Z, SigC (MO basis, symmetrised), nSCF, Conv)."""
+    """qsGW from the RHF orbitals cHF (nBas x nOrb) and energies eHF.  eri_ptr:
+    device address of the nBas^4 AO integrals (QuAcK's ERI.bin order).  X: the
+    orthogonalisation matrix (nBas x nOrb).  Returns dict(eGW, c, EqsGW
+    (electronic), EcRPA, EcGM, Z, SigC (MO basis, symmetrised), nSCF, Conv)."""
     import torch  # device memory for the MO coefficients handed to the AO->MO block
 
-    n = S.shape[0]
-    if X.shape[1] != n:
-        raise NotImplementedError("qsGW here needs nBas == nOrb (no near-linear dependencies)")
+    nBas, n = X.shape  # n = nOrb
     Hc = T + V
     c = np.array(cHF, dtype=np.float64, order="F")
     eGW = np.array(eHF, dtype=np.float64)
     P = 2.0 * c[:, :nO] @ c[:, :nO].T
     nS = nO * (n - nO)
-    err_diis = np.zeros((n * n, max(1, max_diis)))
-    F_diis = np.zeros((n * n, max(1, max_diis)))
+    err_diis = np.zeros((nBas * nBas, max(1, max_diis)))
+    F_diis = np.zeros((nBas * nBas, max(1, max_diis)))
     n_diis, nSCF, Conv = 0, -1, 1.0
     res = {}
-    c_t = torch.empty(n * n, dtype=torch.float64, device="cuda")
-    with scf.JK(n) as jk, gw.GWEngine(n, n, nO) as eng:
+    c_t = torch.empty(nBas * n, dtype=torch.float64, device="cuda")
+    with scf.JK(nBas) as jk, gw.GWEngine(nBas, n, nO) as eng:
         while Conv > thresh and nSCF < maxSCF:
             nSCF += 1
             J, K = jk(eri_ptr, P)
@@ -60,6 +58,8 @@ def qsRGW(eri_ptr, S, T, V, X, cHF, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5,
             SigC = 0.5 * (r["SigC"] + r["SigC"].T)
             SigCp = S @ c @ SigC @ c.T @ S
             F = Hc + J + 0.5 * K + SigCp
+            if nBas != n:  # project F onto the kept orbitals (qsRGW.f90:281-284)
+                F = S @ c @ (c.T @ F @ c) @ c.T @ S
             err = F @ P @ S - S @ P @ F
             if nSCF > 1:
                 Conv = float(np.abs(err).max())
@@ -68,14 +68,18 @@ def qsRGW(eri_ptr, S, T, V, X, cHF, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5,
                 n_diis = min(n_diis + 1, max_diis)
                 f = np.asfortranarray(F).ravel(order="F").copy()
                 scf.diis_extrapolation(n_diis, err_diis, F_diis, np.asfortranarray(err).ravel(order="F"), f)
-                F = f.reshape(n, n, order="F")
-            eGW, cp = np.linalg.eigh(X.T @ F @ X)
-            c = np.asfortranarray(X @ cp)
+                F = f.reshape(nBas, nBas, order="F")
+            if nBas == n:
+                eGW, cp = np.linalg.eigh(X.T @ F @ X)
+                c = np.asfortranarray(X @ cp)
+            else:  # qsRGW.f90:323-333: rotate the current orbitals
+                eGW, cp = np.linalg.eigh(c.T @ F @ c)
+                c = np.asfortranarray(c @ cp)
             P = 2.0 * c[:, :nO] @ c[:, :nO].T
             res = {"eGW": eGW, "c": c, "EqsGW": EqsGW, "EcRPA": r["EcRPA"], "EcGM": r["EcGM"], "Z": r["Z"],
                    "SigC": c.T @ SigCp @ c, "nSCF": nSCF, "Conv": Conv, "nS": nS}
             if log:
                 log("qsGW %3d  E %.10f  Conv %.2e" % (nSCF, EqsGW, Conv))
-    if nSCF == maxSCF:  # GW/qsRGW.f90:224-232
+    if nSCF == maxSCF:  # GW/qsRGW.f90:354-362
         raise QsGWNotConverged("qsGW: convergence failed after %d cycles (Conv %.2e)" % (nSCF, Conv), res)
     return res

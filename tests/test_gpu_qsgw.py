@@ -64,3 +64,31 @@ def test_qsgw_matches_oracle(systems, name):
         assert ro["converged"] and r["nSCF"] == ro["nSCF"], kw
         assert np.abs(r["eGW"] - ro["eGW"]).max() < 1e-8, kw
         assert abs(r["EqsGW"] - ro["EqsGW"]) < 1e-9 and abs(r["EcGM"] - ro["EcGM"]) < 1e-9
+
+
+def test_qsgw_dropped_functions():
+    """nBas > nOrb (a compressed H4 chain: canonical orthogonalisation drops one
+    of 20 functions): QuAcK's second branch (F projected on the kept
+    orbitals, the current orbitals rotated) against the restatement."""
+    import torch
+
+    from quack_amd import intgen as ig
+    from quack_amd import qsgw, scf
+    print("qsgw H4 nOrb < nBas", flush=True)
+    mol = ig.h_chain(4, spacing=0.3, jitter=0.0, seed=0)
+    s = ig.system(mol)
+    S, T, V = mol.one_electron()
+    X = scf.orthogonalization_matrix(S)
+    assert X.shape == (20, 19)
+    nO, c = s["nO"], np.asarray(s["c"])
+    P = 2.0 * c[:, :nO] @ c[:, :nO].T
+    eri_t = torch.from_numpy(np.ascontiguousarray(s["eri"])).to("cuda")
+    r = qsgw.qsRGW(eri_t.data_ptr(), S, T, V, X, c, s["eHF"], nO, doSRG=True)
+    ro = go.qsrgw(S, T, V, s["eri"], X, c, P, s["eHF"], nO, doSRG=True)
+    assert ro["converged"] and r["nSCF"] == ro["nSCF"]
+    assert np.abs(r["eGW"] - ro["eGW"]).max() < 1e-8
+    # eta = 0.05 does not converge here in QuAcK's 64 cycles (nor in the
+    # restatement): the loop stops with the error and the last cycle's results
+    with pytest.raises(qsgw.QsGWNotConverged) as ex:
+        qsgw.qsRGW(eri_t.data_ptr(), S, T, V, X, c, s["eHF"], nO, eta=0.05)
+    assert ex.value.result["nSCF"] == 64 and np.all(np.isfinite(ex.value.result["eGW"]))
