@@ -200,6 +200,20 @@ int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int
 int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO, int nV, int nR, int nS,
                     const double *eHF, const double *Om, const double *rho, const double *eGWlin, const double *eOld,
                     double *eGW, double *Z);
+/* Unrestricted G0W0 (GW/UG0W0.f90:92-150) with nC = nR = 0: spin-conserving
+ * dRPA screening at eHF over both spin blocks (LR/phULR_A.f90, LR/phULR_B.f90,
+ * LR/phULR.f90; nSa + nSb modes), excitation densities
+ * (GW/UGW_excitation_density.f90) as FP64 matrix-core products, diagonal
+ * self-energies (GW/UGW_self_energy_diag.f90 or the SRG form), linearised or
+ * graphical (GW/UGW_QP_graph.f90) quasiparticle energies per spin, and the RPA
+ * again at eGW for EcRPA.  ERI_aaaa, ERI_aabb, ERI_bbbb: nBas^4 MO integrals
+ * (physicist, column-major; in ERI_aabb the first and third indices are
+ * alpha).  eHF, eGW, SigC, Z: nBas x 2 (alpha column, then beta).  Om: nSa +
+ * nSb.  EcGM[2]: per spin (QuAcK's SRG routine returns their sum). */
+int qk_ug0w0(int nBas, int nOa, int nOb, const double *ERI_aaaa, const double *ERI_aabb, const double *ERI_bbbb,
+             const double *eHF, double eta, int linearize, int TDA_W, int doSRG, double *eGW, double *SigC, double *Z,
+             double *Om, double *EcRPA, double *EcGM);
+
 /* Static phBSE@GW (GW/RGW_phBSE.f90:1-201, with GW/RGW_phBSE_static_kernel_A.f90
  * and _B.f90): dRPA screening at eW (TDA_W), its excitation densities, the
  * static kernels KA/KB (two FP64 matrix-core products over the RPA modes), then

@@ -464,3 +464,97 @@ def qsrgw(S, T, V, eri_ao, X, cHF, PHF, eHF, nO, maxSCF=64, thresh=1e-5, max_dii
         P = 2.0 * c[:, :nO] @ c[:, :nO].T
     return {"eGW": eGW, "c": c, "EqsGW": EqsGW, "EcRPA": EcRPA, "EcGM": EcGM, "Z": Z, "nSCF": nSCF, "Conv": Conv,
             "converged": nSCF < maxSCF}
+
+
+# ---- unrestricted G0W0 (GW/UG0W0.f90) ----------------------------------------
+# Spin-resolved MO integrals: G_aa, G_ab, G_bb = G4(ERI_aaaa), G4(ERI_aabb),
+# G4(ERI_bbbb) (in ERI_aabb the first and third indices are alpha, the second
+# and fourth beta); e[:, 0] alpha, e[:, 1] beta; nO = (nO_alpha, nO_beta).
+
+
+def _ublocks(n, nO):
+    oa, ob = slice(0, nO[0]), slice(0, nO[1])
+    va, vb = slice(nO[0], n), slice(nO[1], n)
+    return oa, va, ob, vb, nO[0] * (n - nO[0]), nO[1] * (n - nO[1])
+
+
+def phULR_A(e, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True):
+    """LR/phULR_A.f90:26-80 (spin-conserving, ispin 1), nC = nR = 0."""
+    n = e.shape[0]
+    Ga, Gm, Gb = G4(eri_aa, n), G4(eri_ab, n), G4(eri_bb, n)
+    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO)
+    dr = 1.0 if dRPA else 0.0
+    A = np.zeros((nSa + nSb, nSa + nSb))
+    A[:nSa, :nSa] = lam * Ga[oa, va, va, oa].transpose(0, 2, 3, 1).reshape(nSa, nSa) \
+        - (1 - dr) * lam * Ga[oa, va, oa, va].transpose(0, 3, 2, 1).reshape(nSa, nSa)
+    A[:nSa, nSa:] = lam * Gm[oa, vb, va, ob].transpose(0, 2, 3, 1).reshape(nSa, nSb)
+    A[nSa:, :nSa] = lam * Gm[va, ob, oa, vb].transpose(1, 3, 2, 0).reshape(nSb, nSa)
+    A[nSa:, nSa:] = lam * Gb[ob, vb, vb, ob].transpose(0, 2, 3, 1).reshape(nSb, nSb) \
+        - (1 - dr) * lam * Gb[ob, vb, ob, vb].transpose(0, 3, 2, 1).reshape(nSb, nSb)
+    d = np.concatenate([(e[va, 0][None, :] - e[oa, 0][:, None]).reshape(-1),
+                        (e[vb, 1][None, :] - e[ob, 1][:, None]).reshape(-1)])
+    A[np.diag_indices(nSa + nSb)] += d
+    return A
+
+
+def phULR_B(n, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True):
+    """LR/phULR_B.f90:26-80 (spin-conserving, ispin 1), nC = nR = 0."""
+    Ga, Gm, Gb = G4(eri_aa, n), G4(eri_ab, n), G4(eri_bb, n)
+    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO)
+    dr = 1.0 if dRPA else 0.0
+    B = np.zeros((nSa + nSb, nSa + nSb))
+    B[:nSa, :nSa] = lam * Ga[oa, oa, va, va].transpose(0, 2, 1, 3).reshape(nSa, nSa) \
+        - (1 - dr) * lam * Ga[oa, oa, va, va].transpose(0, 3, 1, 2).reshape(nSa, nSa)
+    B[:nSa, nSa:] = lam * Gm[oa, ob, va, vb].transpose(0, 2, 1, 3).reshape(nSa, nSb)
+    B[nSa:, :nSa] = lam * Gm[oa, ob, va, vb].transpose(1, 3, 0, 2).reshape(nSb, nSa)
+    B[nSa:, nSa:] = lam * Gb[ob, ob, vb, vb].transpose(0, 2, 1, 3).reshape(nSb, nSb) \
+        - (1 - dr) * lam * Gb[ob, ob, vb, vb].transpose(0, 3, 1, 2).reshape(nSb, nSb)
+    return B
+
+
+def ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO):
+    """GW/UGW_excitation_density.f90: rho[p, q, m, spin]."""
+    Ga, Gm, Gb = G4(eri_aa, n), G4(eri_ab, n), G4(eri_bb, n)
+    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO)
+    Xa, Xb = XpY[:, :nSa], XpY[:, nSa:]
+    rho = np.zeros((n, n, nSa + nSb, 2))
+    # alpha: sum_jb(a) <pj|qb>_aaaa X(m, jb) + sum_jb(b) <pj|qb>_aabb X(m, nSa + jb)
+    rho[:, :, :, 0] = np.einsum("pjqb,mjb->pqm", Ga[:, oa, :, va], Xa.reshape(-1, nO[0], n - nO[0]), optimize=True) \
+        + np.einsum("pjqb,mjb->pqm", Gm[:, ob, :, vb], Xb.reshape(-1, nO[1], n - nO[1]), optimize=True)
+    # beta: sum_jb(a) <jp|bq>_aabb X(m, jb) + sum_jb(b) <pj|qb>_bbbb X(m, nSa + jb)
+    rho[:, :, :, 1] = np.einsum("jpbq,mjb->pqm", Gm[oa, :, va, :], Xa.reshape(-1, nO[0], n - nO[0]), optimize=True) \
+        + np.einsum("pjqb,mjb->pqm", Gb[:, ob, :, vb], Xb.reshape(-1, nO[1], n - nO[1]), optimize=True)
+    return rho
+
+
+def ug0w0(eri_aa, eri_ab, eri_bb, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False):
+    """GW/UG0W0.f90:92-150 (nC = nR = 0): spin-conserving dRPA screening at
+    eHF, excitation densities, diagonal self-energies (GW/UGW_self_energy_diag.f90
+    or the SRG form), linearised energies and the graphical QP solve of each
+    spin (GW/UGW_QP_graph.f90), then the RPA again at eGW for EcRPA.  The
+    unrestricted sums carry rho^2 where the restricted ones carry 2 rho^2, so
+    the restricted restatements are used with rho / sqrt(2) (and EcGM, 4 rho^2
+    there, halved).  Returns dict(eGW, SigC, Z (n x 2), Om, EcRPA, EcGM (2))."""
+    n = eHF.shape[0]
+    A = phULR_A(eHF, eri_aa, eri_ab, eri_bb, nO)
+    B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO)
+    _, Om, XpY, _ = phRLR(TDA_W, A, B)
+    rho = ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO) / np.sqrt(2.0)
+    Sig, Z, EcGM = np.zeros((n, 2)), np.zeros((n, 2)), np.zeros(2)
+    eGW = np.zeros((n, 2))
+    for s in range(2):
+        if doSRG:
+            Sig[:, s], Z[:, s], ec = srg_self_energy_diag(eHF[:, s], Om, rho[:, :, :, s], nO[s])
+        else:
+            Sig[:, s], Z[:, s], ec = self_energy_diag(eHF[:, s], Om, rho[:, :, :, s], nO[s], eta)
+        EcGM[s] = 0.5 * ec
+        lin = eHF[:, s] + Z[:, s] * Sig[:, s]
+        if linearize:
+            eGW[:, s] = lin
+        else:
+            eGW[:, s], Z[:, s], _ = qp_graph(eHF[:, s], Om, rho[:, :, :, s], lin, eHF[:, s], Z[:, s], nO[s], eta,
+                                             srg=doSRG)
+    A = phULR_A(eGW, eri_aa, eri_ab, eri_bb, nO)
+    B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO)
+    EcRPA, _, _, _ = phRLR(TDA_W, A, B)
+    return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": EcRPA, "EcGM": EcGM}

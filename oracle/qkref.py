@@ -45,6 +45,7 @@ def load():
         lib.qkref_phbse.argtypes = [_I] * 4 + [_D] + [_I] * 6 + [_P] * 6
         lib.qkref_self_energy_full.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
         lib.qkref_qsgw_cycle.argtypes = [_I, _I, _D] + [_I] * 6 + [_P] * 10
+        lib.qkref_ug0w0.argtypes = [_I, _I, _I, _D, _I, _I, _I] + [_P] * 9
         _lib = lib
     return _lib
 
@@ -171,4 +172,17 @@ def qsgw_cycle(S, T, V, X, c, P, eGW, eri_ao, nO, eta=0.0, srg=False, tda_w=Fals
                             _p(_f(X)), _p(_f(c)), _p(_f(P)), _p(_f(eGW)), _p(np.ascontiguousarray(eri_ao)), _p(F),
                             _p(e))
     return {"F": F, "eGW": e}
+
+
+def ug0w0(eri_aa, eri_ab, eri_bb, eHF, nO, eta=0.0, linearize=False, tda_w=False, srg=False):
+    """GW/UG0W0.f90 through QuAcK's own unrestricted routines
+    (ref_driver.f90 qkref_ug0w0); eHF n x 2, nO = (alpha, beta)."""
+    n = eHF.shape[0]
+    eGW, Sig, Z = (np.zeros((n, 2), order="F") for _ in range(3))
+    Ec = np.zeros(1)
+    Egm = np.zeros(2)
+    load().qkref_ug0w0(int(linearize), int(tda_w), int(srg), eta, n, nO[0], nO[1],
+                       _p(np.ascontiguousarray(eri_aa)), _p(np.ascontiguousarray(eri_ab)),
+                       _p(np.ascontiguousarray(eri_bb)), _p(_f(eHF)), _p(eGW), _p(Sig), _p(Z), _p(Ec), _p(Egm))
+    return {"eGW": eGW, "SigC": Sig, "Z": Z, "EcRPA": Ec[0], "EcGM": Egm}
 

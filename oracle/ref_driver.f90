@@ -292,3 +292,56 @@ subroutine qkref_qsgw_cycle(srg, tda_w, eta, nBas, nC, nO, nV, nR, nS, S, T, V, 
   flush(6)
   deallocate(J, K, ERI_MO, Aph, Bph, Om, XpY, XmY, rho, SigC, SigCp, Z, cp)
 end subroutine
+
+! GW/UG0W0.f90:92-150 (nC = nR = 0, no printing), composed of QuAcK's own
+! unrestricted routines in the same order.  EcGM: the non-SRG routine returns
+! one value per spin; the SRG routine one scalar (their sum), put in EcGM(1).
+subroutine qkref_ug0w0(lin, tda_w, srg, eta, nBas, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, eGW, SigC, Z, &
+                       EcRPA, EcGM) bind(C, name='qkref_ug0w0')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: lin, tda_w, srg, nBas, nOa, nOb
+  real(c_double), value :: eta
+  real(c_double), intent(in) :: ERI_aaaa(nBas,nBas,nBas,nBas), ERI_aabb(nBas,nBas,nBas,nBas)
+  real(c_double), intent(in) :: ERI_bbbb(nBas,nBas,nBas,nBas), eHF(nBas,2)
+  real(c_double), intent(out) :: eGW(nBas,2), SigC(nBas,2), Z(nBas,2), EcRPA, EcGM(2)
+  integer :: nC(2), nO(2), nV(2), nR(2), nS(2), nSa, nSb, nSt, is
+  double precision, allocatable :: Aph(:,:), Bph(:,:), Om(:), XpY(:,:), XmY(:,:), rho(:,:,:,:), eGWlin(:,:)
+  double precision :: ecs
+  logical :: TDAW, doSRG
+  TDAW = tda_w /= 0
+  doSRG = srg /= 0
+  nC = 0; nR = 0
+  nO(1) = nOa; nO(2) = nOb
+  nV = nBas - nO
+  nS = nO*nV
+  nSa = nS(1); nSb = nS(2); nSt = nSa + nSb
+  allocate(Aph(nSt,nSt), Bph(nSt,nSt), Om(nSt), XpY(nSt,nSt), XmY(nSt,nSt), rho(nBas,nBas,nSt,2), eGWlin(nBas,2))
+  Bph = 0d0
+  call phULR_A(1, .true., nBas, nC, nO, nV, nR, nSa, nSb, nSt, 1d0, eHF, ERI_aaaa, ERI_aabb, ERI_bbbb, Aph)
+  if(.not. TDAW) call phULR_B(1, .true., nBas, nC, nO, nV, nR, nSa, nSb, nSt, 1d0, ERI_aaaa, ERI_aabb, ERI_bbbb, Bph)
+  call phULR(TDAW, nSa, nSb, nSt, Aph, Bph, EcRPA, Om, XpY, XmY)
+  call UGW_excitation_density(nBas, nC, nO, nR, nSa, nSb, nSt, ERI_aaaa, ERI_aabb, ERI_bbbb, XpY, rho)
+  EcGM = 0d0
+  if(doSRG) then
+    call UGW_SRG_self_energy_diag(500d0, nBas, nC, nO, nV, nR, nSt, eHF, Om, rho, ecs, SigC, Z)
+    EcGM(1) = ecs
+  else
+    call UGW_self_energy_diag(eta, nBas, nC, nO, nV, nR, nSt, eHF, Om, rho, EcGM, SigC, Z)
+  end if
+  eGWlin = eHF + Z*SigC
+  if(lin /= 0) then
+    eGW = eGWlin
+  else
+    do is = 1, 2
+      call UGW_QP_graph(doSRG, eta, 500d0, nBas, nC(is), nO(is), nV(is), nR(is), nSt, eHF(:,is), Om, &
+                        rho(:,:,:,is), eGWlin(:,is), eHF(:,is), eGW(:,is), Z(:,is))
+    end do
+  end if
+  Bph = 0d0
+  call phULR_A(1, .true., nBas, nC, nO, nV, nR, nSa, nSb, nSt, 1d0, eGW, ERI_aaaa, ERI_aabb, ERI_bbbb, Aph)
+  if(.not. TDAW) call phULR_B(1, .true., nBas, nC, nO, nV, nR, nSa, nSb, nSt, 1d0, ERI_aaaa, ERI_aabb, ERI_bbbb, Bph)
+  call phULR(TDAW, nSa, nSb, nSt, Aph, Bph, EcRPA, Om, XpY, XmY)
+  flush(6)
+  deallocate(Aph, Bph, Om, XpY, XmY, rho, eGWlin)
+end subroutine

@@ -1643,41 +1643,105 @@ double* upload_rho(hipStream_t s, const double* rho, int n, int nS) {
 }
 }  // namespace
 
+}  // extern "C"
+
+namespace qk {
+namespace {
+// Diagonal self-energy (GW/RGW_self_energy_diag.f90 or the SRG form) from host
+// rho(p,q,m) (n x n x nS, column-major) on the scratch stream.  nS is the
+// number of RPA modes, which the unrestricted caller sets to nSa + nSb.
+void sigma_diag_host(int doSRG, double eta, int n, int nC, int nO, int nR, int nS, const double* e, const double* Om,
+                     const double* rho, double* EcGM, double* SigC, double* Z) {
+    hipStream_t s = scratch().s;
+    int nch = sigma_chunks(nS, n - nR - nC);
+    std::vector<double*> bufs;
+    auto get = [&](size_t cnt) { bufs.push_back(dalloc<double>(cnt)); return bufs.back(); };
+    try {
+        double* rT = upload_rho(s, rho, n, nS);
+        bufs.push_back(rT);
+        double* de = get(n);
+        double* dO = get(nS);
+        double* part = get((size_t)3 * n * nch);
+        double* dS = get(n);
+        double* dZ = get(n);
+        double* sc = get(4);
+        QK_HIP(hipMemcpyAsync(de, e, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
+        sigma_partial(s, rT, false, de, de, dO, nullptr, eta, doSRG, SRG_FLOW, n, nC, nO, nR, nS, nch, part);
+        sigma_finalize(s, part, nch, n, nC, nO, nR, dS, dZ, sc);
+        QK_HIP(hipMemcpyAsync(SigC, dS, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipMemcpyAsync(Z, dZ, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        double gm = 0.0;
+        QK_HIP(hipMemcpyAsync(&gm, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+        QK_HIP(hipStreamSynchronize(s));
+        if (EcGM) *EcGM = gm;
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        for (double* p : bufs) (void)hipFree(p);
+        throw;
+    }
+    for (double* p : bufs) (void)hipFree(p);
+}
+
+// Graphical QP solve (GW/RGW_QP_graph.f90:50-92) from host arrays, as above.
+void qp_graph_host(int doSRG, double eta, double flow, int n, int nC, int nO, int nR, int nS, const double* eHF,
+                   const double* Om, const double* rho, const double* eGWlin, const double* eOld, double* eGW, double* Z) {
+    hipStream_t s = scratch().s;
+    int nch = sigma_chunks(nS, n - nR - nC);
+    std::vector<void*> bufs;
+    auto get = [&](size_t cnt) { double* p = dalloc<double>(cnt); bufs.push_back(p); return p; };
+    auto geti = [&](size_t cnt) { int* p = dalloc<int>(cnt); bufs.push_back(p); return p; };
+    try {
+        double* rT = upload_rho(s, rho, n, nS);
+        bufs.push_back(rT);
+        double* d0 = get(n);
+        double* dG = get(n);
+        double* dL = get(n);
+        double* dO = get(nS);
+        double* w = get(n);
+        double* f = get(n);
+        double* df = get(n);
+        double* dE = get(n);
+        double* dZ = get(n);
+        double* part = get((size_t)3 * n * nch);
+        int* nit = geti(n);
+        int* done = geti(n);
+        int* act = geti(1);
+        QK_HIP(hipMemcpyAsync(d0, eHF, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        QK_HIP(hipMemcpyAsync(dG, eOld, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        QK_HIP(hipMemcpyAsync(dL, eGWlin, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
+        QK_HIP(hipMemcpyAsync(dZ, Z, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        qp_init(s, dL, w, f, nit, done, n, nC, nR);
+        int h_active = 1;
+        for (int it = 0; it < QP_MAXIT && h_active > 0; ++it) {
+            sigma_partial(s, rT, false, w, dG, dO, done, eta, doSRG, flow, n, nC, nO, nR, nS, nch, part);
+            qp_update(s, part, nch, d0, w, f, df, nit, done, n, QP_THRESH, QP_MAXIT, act);
+            QK_HIP(hipMemcpyAsync(&h_active, act, sizeof(int), hipMemcpyDeviceToHost, s));
+            QK_HIP(hipStreamSynchronize(s));
+        }
+        qp_finish(s, w, df, nit, dL, dE, dZ, n, nC, nR, QP_MAXIT);
+        QK_HIP(hipMemcpyAsync(eGW, dE, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipMemcpyAsync(Z, dZ, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        for (void* p : bufs) (void)hipFree(p);
+        throw;
+    }
+    for (void* p : bufs) (void)hipFree(p);
+}
+}  // namespace
+}  // namespace qk
+
+extern "C" {
+
 int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double* e,
                             const double* Om, const double* rho, double* EcGM, double* SigC, double* Z) {
     if (!e || !Om || !rho || !SigC || !Z) return QK_E_ARG;
     return guarded(nullptr, [&] {
         check_sizes(nOrb, nOrb, nC, nO, nV, nR, nS);
-        hipStream_t s = scratch().s;
-        int n = nOrb;
-        int nch = sigma_chunks(nS, n - nR - nC);
-        std::vector<double*> bufs;
-        auto get = [&](size_t cnt) { bufs.push_back(dalloc<double>(cnt)); return bufs.back(); };
-        try {
-            double* rT = upload_rho(s, rho, n, nS);
-            bufs.push_back(rT);
-            double* de = get(n);
-            double* dO = get(nS);
-            double* part = get((size_t)3 * n * nch);
-            double* dS = get(n);
-            double* dZ = get(n);
-            double* sc = get(4);
-            QK_HIP(hipMemcpyAsync(de, e, sizeof(double) * n, hipMemcpyHostToDevice, s));
-            QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
-            sigma_partial(s, rT, false, de, de, dO, nullptr, eta, doSRG, SRG_FLOW, n, nC, nO, nR, nS, nch, part);
-            sigma_finalize(s, part, nch, n, nC, nO, nR, dS, dZ, sc);
-            QK_HIP(hipMemcpyAsync(SigC, dS, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            QK_HIP(hipMemcpyAsync(Z, dZ, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            double gm = 0.0;
-            QK_HIP(hipMemcpyAsync(&gm, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
-            QK_HIP(hipStreamSynchronize(s));
-            if (EcGM) *EcGM = gm;
-        } catch (...) {
-            (void)hipStreamSynchronize(s);
-            for (double* p : bufs) (void)hipFree(p);
-            throw;
-        }
-        for (double* p : bufs) (void)hipFree(p);
+        sigma_diag_host(doSRG, eta, nOrb, nC, nO, nR, nS, e, Om, rho, EcGM, SigC, Z);
     });
 }
 
@@ -1687,54 +1751,137 @@ int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO
     if (!eHF || !Om || !rho || !eGWlin || !eOld || !eGW || !Z) return QK_E_ARG;
     return guarded(nullptr, [&] {
         check_sizes(nOrb, nOrb, nC, nO, nV, nR, nS);
-        hipStream_t s = scratch().s;
-        int n = nOrb;
-        int nch = sigma_chunks(nS, n - nR - nC);
-        std::vector<void*> bufs;
-        auto get = [&](size_t cnt) { double* p = dalloc<double>(cnt); bufs.push_back(p); return p; };
-        auto geti = [&](size_t cnt) { int* p = dalloc<int>(cnt); bufs.push_back(p); return p; };
-        try {
-            double* rT = upload_rho(s, rho, n, nS);
-            bufs.push_back(rT);
-            double* d0 = get(n);
-            double* dG = get(n);
-            double* dL = get(n);
-            double* dO = get(nS);
-            double* w = get(n);
-            double* f = get(n);
-            double* df = get(n);
-            double* dE = get(n);
-            double* dZ = get(n);
-            double* part = get((size_t)3 * n * nch);
-            int* nit = geti(n);
-            int* done = geti(n);
-            int* act = geti(1);
-            QK_HIP(hipMemcpyAsync(d0, eHF, sizeof(double) * n, hipMemcpyHostToDevice, s));
-            QK_HIP(hipMemcpyAsync(dG, eOld, sizeof(double) * n, hipMemcpyHostToDevice, s));
-            QK_HIP(hipMemcpyAsync(dL, eGWlin, sizeof(double) * n, hipMemcpyHostToDevice, s));
-            QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
-            QK_HIP(hipMemcpyAsync(dZ, Z, sizeof(double) * n, hipMemcpyHostToDevice, s));
-            qp_init(s, dL, w, f, nit, done, n, nC, nR);
-            int h_active = 1;
-            for (int it = 0; it < QP_MAXIT && h_active > 0; ++it) {
-                sigma_partial(s, rT, false, w, dG, dO, done, eta, doSRG, flow, n, nC, nO, nR, nS, nch, part);
-                qp_update(s, part, nch, d0, w, f, df, nit, done, n, QP_THRESH, QP_MAXIT, act);
-                QK_HIP(hipMemcpyAsync(&h_active, act, sizeof(int), hipMemcpyDeviceToHost, s));
-                QK_HIP(hipStreamSynchronize(s));
-            }
-            qp_finish(s, w, df, nit, dL, dE, dZ, n, nC, nR, QP_MAXIT);
-            QK_HIP(hipMemcpyAsync(eGW, dE, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            QK_HIP(hipMemcpyAsync(Z, dZ, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-            QK_HIP(hipStreamSynchronize(s));
-        } catch (...) {
-            (void)hipStreamSynchronize(s);
-            for (void* p : bufs) (void)hipFree(p);
-            throw;
-        }
-        for (void* p : bufs) (void)hipFree(p);
+        qp_graph_host(doSRG, eta, flow, nOrb, nC, nO, nR, nS, eHF, Om, rho, eGWlin, eOld, eGW, Z);
     });
 }
 
+
+int qk_ug0w0(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* ERI_aabb, const double* ERI_bbbb,
+             const double* eHF, double eta, int linearize, int TDA_W, int doSRG, double* eGW, double* SigC, double* Z,
+             double* Om, double* EcRPA, double* EcGM) {
+    if (nBas <= 0 || nOa <= 0 || nOb <= 0 || nOa >= nBas || nOb >= nBas || !ERI_aaaa || !ERI_aabb || !ERI_bbbb ||
+        !eHF || !eGW || !SigC || !Z)
+        return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        const int n = nBas;
+        const int nO[2] = {nOa, nOb}, v[2] = {n - nOa, n - nOb};
+        const int nSa = nO[0] * v[0], nSb = nO[1] * v[1], nSt = nSa + nSb;
+        QK_CHECK((long long)nSt * nSt < (1LL << 40), QK_E_ARG, "nSa + nSb too large");
+        const size_t S2 = (size_t)nSt * nSt, n2 = (size_t)n * n;
+        const double* G[3] = {ERI_aaaa, ERI_aabb, ERI_bbbb};
+        auto at = [&](int k, int p, int q, int r, int t) {  // ERI_k(p,q,r,s), 0-based
+            return G[k][p + (size_t)n * (q + (size_t)n * (r + (size_t)n * t))];
+        };
+        auto e = [&](int p, int sp) { return eHF[p + (size_t)n * sp]; };
+        // spin-conserving dRPA A and B (LR/phULR_A.f90:26-80, LR/phULR_B.f90:26-80, nC = nR = 0);
+        // pair index ia = a' + v i' within each spin block, the beta block after the alpha one
+        auto build = [&](const double* en, std::vector<double>& A, std::vector<double>& B) {
+            A.assign(S2, 0.0);
+            B.assign(S2, 0.0);
+            for (int si = 0; si < 2; ++si)
+                for (int sj = 0; sj < 2; ++sj) {
+                    const int oi = si ? nSa : 0, oj = sj ? nSa : 0;
+                    for (int i = 0; i < nO[si]; ++i)
+                        for (int a = 0; a < v[si]; ++a) {
+                            const size_t ia = oi + a + (size_t)v[si] * i;
+                            const int I = i, Aa = nO[si] + a;
+                            for (int j = 0; j < nO[sj]; ++j)
+                                for (int b = 0; b < v[sj]; ++b) {
+                                    const size_t jb = oj + b + (size_t)v[sj] * j;
+                                    const int J = j, Bb = nO[sj] + b;
+                                    double x, y;
+                                    if (si == sj) {  // ERI_ssss(i,b,a,j), ERI_ssss(i,j,a,b)
+                                        const int k = si ? 2 : 0;
+                                        x = at(k, I, Bb, Aa, J);
+                                        y = at(k, I, J, Aa, Bb);
+                                    } else if (si == 0) {  // alpha ia, beta jb: ERI_aabb(i,b,a,j), (i,j,a,b)
+                                        x = at(1, I, Bb, Aa, J);
+                                        y = at(1, I, J, Aa, Bb);
+                                    } else {  // beta ia, alpha jb: ERI_aabb(b,i,j,a), (j,i,b,a)
+                                        x = at(1, Bb, I, J, Aa);
+                                        y = at(1, J, I, Bb, Aa);
+                                    }
+                                    if (si == sj && i == j && a == b) x += en[Aa + (size_t)n * si] - en[I + (size_t)n * si];
+                                    A[ia + nSt * jb] = x;
+                                    B[ia + nSt * jb] = TDA_W ? 0.0 : y;
+                                }
+                        }
+                }
+        };
+        std::vector<double> A, B, XpY(S2), XmY(S2), om(nSt);
+        build(eHF, A, B);
+        double ec = 0.0;
+        int r = qk_phrlr(TDA_W, nSt, A.data(), B.data(), &ec, om.data(), XpY.data(), XmY.data());
+        QK_CHECK(r == QK_OK, r, "qk_ug0w0: phULR failed: " + g_last_error);
+        // excitation densities (GW/UGW_excitation_density.f90) as matrix-core products
+        // rhoT_s(m, pq) = sum_k XpY(m, k) M_s(pq, k) / sqrt(2): the unrestricted sums carry
+        // rho^2 where the restricted kernels carry 2 rho^2
+        std::vector<double> Ms(n2 * nSt), rhoq[2];
+        hipStream_t st = scratch().s;
+        double *dX = nullptr, *dM = nullptr, *dR = nullptr;
+        try {
+            dX = dalloc<double>(S2);
+            dM = dalloc<double>(n2 * nSt);
+            dR = dalloc<double>(n2 * nSt);
+            QK_HIP(hipMemcpyAsync(dX, XpY.data(), sizeof(double) * S2, hipMemcpyHostToDevice, st));
+            std::vector<double> hR(n2 * nSt);
+            for (int sp = 0; sp < 2; ++sp) {
+                for (int sj = 0; sj < 2; ++sj) {
+                    const int oj = sj ? nSa : 0;
+                    for (int j = 0; j < nO[sj]; ++j)
+                        for (int b = 0; b < v[sj]; ++b) {
+                            const size_t k = oj + b + (size_t)v[sj] * j;
+                            const int J = j, Bb = nO[sj] + b;
+                            for (int q = 0; q < n; ++q)
+                                for (int p = 0; p < n; ++p) {
+                                    double x;
+                                    if (sp == 0) x = sj == 0 ? at(0, p, J, q, Bb) : at(1, p, J, q, Bb);
+                                    else x = sj == 0 ? at(1, J, p, Bb, q) : at(2, p, J, q, Bb);
+                                    Ms[(p + (size_t)n * q) + n2 * k] = x;
+                                }
+                        }
+                }
+                QK_HIP(hipMemcpyAsync(dM, Ms.data(), sizeof(double) * n2 * nSt, hipMemcpyHostToDevice, st));
+                dgemm(st, false, true, nSt, (int)n2, nSt, 1.0 / std::sqrt(2.0), dX, nSt, dM, n2, 0.0, dR, nSt);
+                QK_HIP(hipMemcpyAsync(hR.data(), dR, sizeof(double) * n2 * nSt, hipMemcpyDeviceToHost, st));
+                QK_HIP(hipStreamSynchronize(st));
+                // to QuAcK's rho(p, q, m) layout for the per-spin sums
+                rhoq[sp].resize(n2 * nSt);
+                for (int m = 0; m < nSt; ++m)
+                    for (size_t pq = 0; pq < n2; ++pq) rhoq[sp][pq + n2 * m] = hR[m + (size_t)nSt * pq];
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(st);
+            dfree(dX); dfree(dM); dfree(dR);
+            throw;
+        }
+        dfree(dX); dfree(dM); dfree(dR);
+        // per spin: diagonal self-energy (GW/UGW_self_energy_diag.f90 or the SRG form),
+        // linearised energies, graphical QP (GW/UGW_QP_graph.f90) with G from eHF
+        for (int sp = 0; sp < 2; ++sp) {
+            const double* es = eHF + (size_t)n * sp;
+            double gm = 0.0;
+            std::vector<double> sig(n), z(n), lin(n), eg(n);
+            sigma_diag_host(doSRG, eta, n, 0, nO[sp], 0, nSt, es, om.data(), rhoq[sp].data(), &gm, sig.data(), z.data());
+            if (EcGM) EcGM[sp] = 0.5 * gm;  // the restricted sum carries 4 rho^2, the unrestricted rho^2 (x2 here)
+            for (int p = 0; p < n; ++p) lin[p] = es[p] + z[p] * sig[p];
+            if (linearize) eg = lin;
+            else qp_graph_host(doSRG, eta, SRG_FLOW, n, 0, nO[sp], 0, nSt, es, om.data(), rhoq[sp].data(), lin.data(), es,
+                               eg.data(), z.data());
+            for (int p = 0; p < n; ++p) {
+                eGW[p + (size_t)n * sp] = eg[p];
+                SigC[p + (size_t)n * sp] = sig[p];
+                Z[p + (size_t)n * sp] = z[p];
+            }
+        }
+        if (Om) std::memcpy(Om, om.data(), sizeof(double) * nSt);
+        // RPA correlation energy at eGW (GW/UG0W0.f90:140-142)
+        build(eGW, A, B);
+        r = qk_phrlr(TDA_W, nSt, A.data(), B.data(), &ec, om.data(), XpY.data(), XmY.data());
+        QK_CHECK(r == QK_OK, r, "qk_ug0w0: phULR at eGW failed: " + g_last_error);
+        if (EcRPA) *EcRPA = ec;
+    });
+}
 
 int qk_rgw_phbse(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE, int dTDA, int singlet, int triplet,
                  double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double* ERI, const double* eW,

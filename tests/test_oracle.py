@@ -319,3 +319,29 @@ def test_qsgw_cycle_vs_compiled_reference(systems, name):
         r = go.qsrgw(S, T, V, s["eri"], X, c, P, s["eHF"], nO, maxSCF=0, eta=eta, doSRG=srg, TDA_W=tda_w)
         assert r["nSCF"] == 0 and not r["converged"]
         assert np.abs(r["eGW"] - ref["eGW"]).max() < 1e-10, (name, srg, eta, tda_w)
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+def test_ug0w0_vs_compiled_reference(mo_systems):
+    """Unrestricted G0W0 (GW/UG0W0.f90): the restatement against QuAcK's own
+    phULR_A/B, phULR, UGW_excitation_density, UGW_(SRG_)self_energy_diag and
+    UGW_QP_graph composed in UG0W0's order, for closed-shell and open-shell
+    occupations (one more alpha electron) with shifted beta energies; and the
+    closed-shell reduction to the restricted restatement."""
+    s = mo_systems["H2O"]
+    mo, nO = s["eri_mo"], s["nO"]
+    n = len(s["eHF"])
+    e2 = np.stack([s["eHF"], s["eHF"] + 0.01 * np.sin(np.arange(n))], axis=1)
+    for occ in ((nO, nO), (nO + 1, nO)):
+        for srg, lin, eta, tda in ((False, False, 0.0, False), (True, False, 0.0, False), (False, True, 0.05, False),
+                                   (False, False, 0.0, True)):
+            r = qkref.ug0w0(mo, mo, mo, e2, occ, eta=eta, linearize=lin, tda_w=tda, srg=srg)
+            u = go.ug0w0(mo, mo, mo, e2, occ, eta=eta, linearize=lin, TDA_W=tda, doSRG=srg)
+            assert np.abs(u["eGW"] - r["eGW"]).max() < 1e-10, (occ, srg, lin, tda)
+            assert np.abs(u["Z"] - r["Z"]).max() < 1e-9 and abs(u["EcRPA"] - r["EcRPA"]) < 1e-10
+            ecgm = u["EcGM"].sum() if srg else u["EcGM"]
+            assert np.abs(ecgm - (r["EcGM"][0] if srg else r["EcGM"])).max() < 1e-10
+    e2 = np.stack([s["eHF"], s["eHF"]], axis=1)
+    u = go.ug0w0(mo, mo, mo, e2, (nO, nO))
+    rr = go.rg0w0(mo, s["eHF"], nO)
+    assert np.abs(u["eGW"] - rr["eGW"][:, None]).max() < 1e-11 and abs(u["EcRPA"] - rr["EcRPA"]) < 1e-11
