@@ -214,6 +214,19 @@ int qk_ug0w0(int nBas, int nOa, int nOb, const double *ERI_aaaa, const double *E
              const double *eHF, double eta, int linearize, int TDA_W, int doSRG, double *eGW, double *SigC, double *Z,
              double *Om, double *EcRPA, double *EcGM);
 
+/* Unrestricted evGW (GW/evUGW.f90:103-215) with nC = nR = 0: per cycle the
+ * screening of qk_ug0w0 at the current energies (both spins), the diagonal
+ * self-energies at those energies, then eHF + SigC (linearize) or the graphical
+ * solve started from the previous energies; Conv = max |eGW - eOld| over both
+ * spins; DIIS per spin (utils/DIIS_extrapolation.f90) when max_diis > 1.
+ * Screening always uses TDA_W (QuAcK builds B unless its separate TDA flag is
+ * set).  Outputs as qk_ug0w0 from the last cycle (EcRPA is the last cycle's
+ * screening at its input energies); nSCF_out = cycles run.  Returns
+ * QK_E_NOCONV, outputs filled, where QuAcK stops after maxSCF + 1 cycles. */
+int qk_evugw(int nBas, int nOa, int nOb, const double *ERI_aaaa, const double *ERI_aabb, const double *ERI_bbbb,
+             const double *eHF, int maxSCF, double thresh, int max_diis, double eta, int linearize, int TDA_W,
+             int doSRG, double *eGW, double *SigC, double *Z, double *Om, double *EcRPA, double *EcGM, int *nSCF_out);
+
 /* Static phBSE@GW (GW/RGW_phBSE.f90:1-201, with GW/RGW_phBSE_static_kernel_A.f90
  * and _B.f90): dRPA screening at eW (TDA_W), its excitation densities, the
  * static kernels KA/KB (two FP64 matrix-core products over the RPA modes), then

@@ -558,3 +558,46 @@ def ug0w0(eri_aa, eri_ab, eri_bb, eHF, nO, eta=0.0, linearize=False, TDA_W=False
     B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO)
     EcRPA, _, _, _ = phRLR(TDA_W, A, B)
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": EcRPA, "EcGM": EcGM}
+
+
+def evugw(eri_aa, eri_ab, eri_bb, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, eta=0.0, linearize=False,
+          TDA_W=False, doSRG=False):
+    """GW/evUGW.f90:103-215 (nC = nR = 0): eGW = eOld = eHF; per cycle the
+    screening at eGW, excitation densities, self-energies at eGW, then eHF + SigC
+    (linearize) or the graphical solve with guess and G from eOld; Conv =
+    max |eGW - eOld| over both spins; DIIS per spin with one shared history
+    length.  B is built unless TDA_W (QuAcK tests its separate TDA flag there).
+    Returns dict(eGW, SigC, Z, Om, EcRPA, EcGM, nSCF, converged)."""
+    n = eHF.shape[0]
+    eHF = np.asarray(eHF, dtype=np.float64)
+    eGW, eOld = eHF.copy(), eHF.copy()
+    Z = np.ones((n, 2))
+    Sig, EcGM = np.zeros((n, 2)), np.zeros(2)
+    diis = [DIIS(n, max(max_diis, 1)) for _ in range(2)]
+    nSCF, Conv, EcRPA, Om = 0, 1.0, 0.0, None
+    while Conv > thresh and nSCF <= maxSCF:
+        A = phULR_A(eGW, eri_aa, eri_ab, eri_bb, nO)
+        B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO)
+        EcRPA, Om, XpY, _ = phRLR(TDA_W, A, B)
+        rho = ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO) / np.sqrt(2.0)
+        new = np.zeros((n, 2))
+        for s in range(2):
+            if doSRG:
+                Sig[:, s], Z[:, s], ec = srg_self_energy_diag(eGW[:, s], Om, rho[:, :, :, s], nO[s])
+            else:
+                Sig[:, s], Z[:, s], ec = self_energy_diag(eGW[:, s], Om, rho[:, :, :, s], nO[s], eta)
+            EcGM[s] = 0.5 * ec
+            if linearize:
+                new[:, s] = eHF[:, s] + Sig[:, s]
+            else:
+                new[:, s], Z[:, s], _ = qp_graph(eHF[:, s], Om, rho[:, :, :, s], eOld[:, s], eOld[:, s], Z[:, s],
+                                                 nO[s], eta, srg=doSRG)
+        eGW = new
+        Conv = np.abs(eGW - eOld).max()
+        if max_diis > 1:
+            for s in range(2):
+                eGW[:, s], _ = diis[s](eGW[:, s] - eOld[:, s], eGW[:, s].copy())
+        eOld = eGW.copy()
+        nSCF += 1
+    return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": EcRPA, "EcGM": EcGM, "nSCF": nSCF,
+            "converged": nSCF <= maxSCF}

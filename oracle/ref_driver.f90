@@ -345,3 +345,69 @@ subroutine qkref_ug0w0(lin, tda_w, srg, eta, nBas, nOa, nOb, ERI_aaaa, ERI_aabb,
   flush(6)
   deallocate(Aph, Bph, Om, XpY, XmY, rho, eGWlin)
 end subroutine
+
+! evUGW's cycle (GW/evUGW.f90:103-215) composed of QuAcK's own routines, with
+! TDA = TDA_W for the B build and the printing left out
+subroutine qkref_evugw(maxSCF, thresh, max_diis, lin, tda_w, srg, eta, nBas, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, &
+                       eHF, eGW, SigC, Z, EcRPA, EcGM, nSCF_out) bind(C, name='qkref_evugw')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: maxSCF, max_diis, lin, tda_w, srg, nBas, nOa, nOb
+  real(c_double), value :: thresh, eta
+  real(c_double), intent(in) :: ERI_aaaa(nBas,nBas,nBas,nBas), ERI_aabb(nBas,nBas,nBas,nBas)
+  real(c_double), intent(in) :: ERI_bbbb(nBas,nBas,nBas,nBas), eHF(nBas,2)
+  real(c_double), intent(out) :: eGW(nBas,2), SigC(nBas,2), Z(nBas,2), EcRPA, EcGM(2)
+  integer(c_int), intent(out) :: nSCF_out
+  integer :: nC(2), nO(2), nV(2), nR(2), nS(2), nSa, nSb, nSt, is, nSCF, n_diis
+  double precision, allocatable :: Aph(:,:), Bph(:,:), Om(:), XpY(:,:), XmY(:,:), rho(:,:,:,:), eOld(:,:)
+  double precision, allocatable :: error_diis(:,:,:), e_diis(:,:,:)
+  double precision :: ecs, Conv, rcond(2)
+  logical :: TDAW, doSRG
+  TDAW = tda_w /= 0
+  doSRG = srg /= 0
+  nC = 0; nR = 0
+  nO(1) = nOa; nO(2) = nOb
+  nV = nBas - nO
+  nS = nO*nV
+  nSa = nS(1); nSb = nS(2); nSt = nSa + nSb
+  allocate(Aph(nSt,nSt), Bph(nSt,nSt), Om(nSt), XpY(nSt,nSt), XmY(nSt,nSt), rho(nBas,nBas,nSt,2), eOld(nBas,2))
+  allocate(error_diis(nBas,max(max_diis,1),2), e_diis(nBas,max(max_diis,1),2))
+  error_diis = 0d0; e_diis = 0d0
+  nSCF = 0; n_diis = 0; Conv = 1d0
+  eGW = eHF; eOld = eHF; Z = 1d0; EcGM = 0d0
+  do while(Conv > thresh .and. nSCF <= maxSCF)
+    Bph = 0d0
+    call phULR_A(1, .true., nBas, nC, nO, nV, nR, nSa, nSb, nSt, 1d0, eGW, ERI_aaaa, ERI_aabb, ERI_bbbb, Aph)
+    if(.not. TDAW) call phULR_B(1, .true., nBas, nC, nO, nV, nR, nSa, nSb, nSt, 1d0, ERI_aaaa, ERI_aabb, ERI_bbbb, Bph)
+    call phULR(TDAW, nSa, nSb, nSt, Aph, Bph, EcRPA, Om, XpY, XmY)
+    call UGW_excitation_density(nBas, nC, nO, nR, nSa, nSb, nSt, ERI_aaaa, ERI_aabb, ERI_bbbb, XpY, rho)
+    if(doSRG) then
+      call UGW_SRG_self_energy_diag(500d0, nBas, nC, nO, nV, nR, nSt, eGW, Om, rho, ecs, SigC, Z)
+      EcGM(1) = ecs
+    else
+      call UGW_self_energy_diag(eta, nBas, nC, nO, nV, nR, nSt, eGW, Om, rho, EcGM, SigC, Z)
+    end if
+    if(lin /= 0) then
+      eGW = eHF + SigC
+    else
+      do is = 1, 2
+        call UGW_QP_graph(doSRG, eta, 500d0, nBas, nC(is), nO(is), nV(is), nR(is), nSt, eHF(:,is), Om, &
+                          rho(:,:,:,is), eOld(:,is), eOld(:,is), eGW(:,is), Z(:,is))
+      end do
+    end if
+    Conv = maxval(abs(eGW - eOld))
+    if(max_diis > 1) then
+      n_diis = min(n_diis+1, max_diis)
+      do is = 1, 2
+        call DIIS_extrapolation(rcond(1), nBas, nBas, n_diis, error_diis(:,1:n_diis,is), e_diis(:,1:n_diis,is), &
+                                eGW(:,is) - eOld(:,is), eGW(:,is))
+      end do
+    end if
+    eOld = eGW
+    nSCF = nSCF + 1
+  end do
+  nSCF_out = nSCF
+  flush(6)
+  deallocate(Aph, Bph, Om, XpY, XmY, rho, eOld, error_diis, e_diis)
+end subroutine
+

@@ -1651,14 +1651,17 @@ namespace {
 // rho(p,q,m) (n x n x nS, column-major) on the scratch stream.  nS is the
 // number of RPA modes, which the unrestricted caller sets to nSa + nSb.
 void sigma_diag_host(int doSRG, double eta, int n, int nC, int nO, int nR, int nS, const double* e, const double* Om,
-                     const double* rho, double* EcGM, double* SigC, double* Z) {
+                     const double* rho, double* EcGM, double* SigC, double* Z, const double* rT_dev = nullptr) {
     hipStream_t s = scratch().s;
     int nch = sigma_chunks(nS, n - nR - nC);
     std::vector<double*> bufs;
     auto get = [&](size_t cnt) { bufs.push_back(dalloc<double>(cnt)); return bufs.back(); };
     try {
-        double* rT = upload_rho(s, rho, n, nS);
-        bufs.push_back(rT);
+        const double* rT = rT_dev;  // rho(p, q, m) at m + nS (p + n q) on the device, or uploaded here
+        if (!rT) {
+            bufs.push_back(upload_rho(s, rho, n, nS));
+            rT = bufs.back();
+        }
         double* de = get(n);
         double* dO = get(nS);
         double* part = get((size_t)3 * n * nch);
@@ -1685,15 +1688,20 @@ void sigma_diag_host(int doSRG, double eta, int n, int nC, int nO, int nR, int n
 
 // Graphical QP solve (GW/RGW_QP_graph.f90:50-92) from host arrays, as above.
 void qp_graph_host(int doSRG, double eta, double flow, int n, int nC, int nO, int nR, int nS, const double* eHF,
-                   const double* Om, const double* rho, const double* eGWlin, const double* eOld, double* eGW, double* Z) {
+                   const double* Om, const double* rho, const double* eGWlin, const double* eOld, double* eGW, double* Z,
+                   const double* rT_dev = nullptr) {
     hipStream_t s = scratch().s;
     int nch = sigma_chunks(nS, n - nR - nC);
     std::vector<void*> bufs;
     auto get = [&](size_t cnt) { double* p = dalloc<double>(cnt); bufs.push_back(p); return p; };
     auto geti = [&](size_t cnt) { int* p = dalloc<int>(cnt); bufs.push_back(p); return p; };
     try {
-        double* rT = upload_rho(s, rho, n, nS);
-        bufs.push_back(rT);
+        const double* rT = rT_dev;
+        if (!rT) {
+            double* u = upload_rho(s, rho, n, nS);
+            bufs.push_back(u);
+            rT = u;
+        }
         double* d0 = get(n);
         double* dG = get(n);
         double* dL = get(n);
@@ -1731,6 +1739,135 @@ void qp_graph_host(int doSRG, double eta, double flow, int n, int nC, int nO, in
     }
     for (void* p : bufs) (void)hipFree(p);
 }
+
+// Unrestricted screening and excitation densities (nC = nR = 0), shared by
+// UG0W0 and evUGW.  Pair index ia = a' + v_s i' within spin block s, the beta
+// block after the alpha one (LR/phULR_A.f90, LR/phULR_B.f90).  The integral
+// slabs M_s(pq, k) = ERI(p, j, q, b) of each spin's densities
+// (GW/UGW_excitation_density.f90:24-62) stay on the device for the whole
+// calculation; each screening leaves rho_s = M_s XpY^T / sqrt(2) there in the
+// restricted kernels' rhoT layout (m contiguous): the unrestricted sums weigh
+// rho^2 where the restricted kernels weigh 2 rho^2.
+struct UScreen {
+    int n, nO[2], v[2], nSa, nSb, nSt, TDA_W;
+    size_t S2, n2;
+    const double* G[3];
+    hipStream_t st;
+    double *dX = nullptr, *dM[2] = {nullptr, nullptr}, *dR[2] = {nullptr, nullptr};
+    std::vector<double> A, B, XpY, XmY, om;
+
+    UScreen(int n_, int nOa, int nOb, const double* Gaaaa, const double* Gaabb, const double* Gbbbb, int tda)
+        : n(n_), nO{nOa, nOb}, v{n_ - nOa, n_ - nOb}, TDA_W(tda), G{Gaaaa, Gaabb, Gbbbb}, st(scratch().s) {
+        nSa = nO[0] * v[0];
+        nSb = nO[1] * v[1];
+        nSt = nSa + nSb;
+        QK_CHECK((long long)nSt * nSt < (1LL << 40), QK_E_ARG, "nSa + nSb too large");
+        S2 = (size_t)nSt * nSt;
+        n2 = (size_t)n * n;
+        XpY.resize(S2);
+        XmY.resize(S2);
+        om.resize(nSt);
+        try {
+            dX = dalloc<double>(S2);
+            std::vector<double> Ms(n2 * nSt);
+            for (int sp = 0; sp < 2; ++sp) {
+                dM[sp] = dalloc<double>(n2 * nSt);
+                dR[sp] = dalloc<double>(n2 * nSt);
+                for (int sj = 0; sj < 2; ++sj)
+                    for (int j = 0; j < nO[sj]; ++j)
+                        for (int b = 0; b < v[sj]; ++b) {
+                            const size_t k = (sj ? nSa : 0) + b + (size_t)v[sj] * j;
+                            const int Bb = nO[sj] + b;
+                            for (int q = 0; q < n; ++q)
+                                for (int p = 0; p < n; ++p)
+                                    Ms[(p + (size_t)n * q) + n2 * k] =
+                                        sp == 0 ? at(sj == 0 ? 0 : 1, p, j, q, Bb)
+                                                : (sj == 0 ? at(1, j, p, Bb, q) : at(2, p, j, q, Bb));
+                        }
+                QK_HIP(hipMemcpyAsync(dM[sp], Ms.data(), sizeof(double) * n2 * nSt, hipMemcpyHostToDevice, st));
+                QK_HIP(hipStreamSynchronize(st));  // Ms is refilled for the next spin
+            }
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    ~UScreen() { release(); }
+    void release() {
+        (void)hipStreamSynchronize(st);
+        dfree(dX);
+        for (int sp = 0; sp < 2; ++sp) {
+            dfree(dM[sp]);
+            dfree(dR[sp]);
+        }
+        dX = dM[0] = dM[1] = dR[0] = dR[1] = nullptr;
+    }
+    double at(int k, int p, int q, int r, int t) const {  // ERI_k(p, q, r, t), 0-based
+        return G[k][p + (size_t)n * (q + (size_t)n * (r + (size_t)n * t))];
+    }
+    // spin-conserving dRPA A (orbital energies en, n x 2) and B (LR/phULR_A.f90:26-80, LR/phULR_B.f90:26-80)
+    void build(const double* en) {
+        A.assign(S2, 0.0);
+        B.assign(S2, 0.0);
+        for (int si = 0; si < 2; ++si)
+            for (int sj = 0; sj < 2; ++sj) {
+                const int oi = si ? nSa : 0, oj = sj ? nSa : 0;
+                for (int i = 0; i < nO[si]; ++i)
+                    for (int a = 0; a < v[si]; ++a) {
+                        const size_t ia = oi + a + (size_t)v[si] * i;
+                        const int Aa = nO[si] + a;
+                        for (int j = 0; j < nO[sj]; ++j)
+                            for (int b = 0; b < v[sj]; ++b) {
+                                const size_t jb = oj + b + (size_t)v[sj] * j;
+                                const int Bb = nO[sj] + b;
+                                double x, y;
+                                if (si == sj) {  // ERI_ssss(i,b,a,j), ERI_ssss(i,j,a,b)
+                                    const int k = si ? 2 : 0;
+                                    x = at(k, i, Bb, Aa, j);
+                                    y = at(k, i, j, Aa, Bb);
+                                } else if (si == 0) {  // alpha ia, beta jb: ERI_aabb(i,b,a,j), (i,j,a,b)
+                                    x = at(1, i, Bb, Aa, j);
+                                    y = at(1, i, j, Aa, Bb);
+                                } else {  // beta ia, alpha jb: ERI_aabb(b,i,j,a), (j,i,b,a)
+                                    x = at(1, Bb, i, j, Aa);
+                                    y = at(1, j, i, Bb, Aa);
+                                }
+                                if (si == sj && i == j && a == b) x += en[Aa + (size_t)n * si] - en[i + (size_t)n * si];
+                                A[ia + nSt * jb] = x;
+                                B[ia + nSt * jb] = TDA_W ? 0.0 : y;
+                            }
+                    }
+            }
+    }
+    // LR/phULR.f90 at energies en; returns EcRPA, leaves Om and XpY
+    double rpa(const double* en, const char* who) {
+        build(en);
+        double ec = 0.0;
+        int r = qk_phrlr(TDA_W, nSt, A.data(), B.data(), &ec, om.data(), XpY.data(), XmY.data());
+        QK_CHECK(r == QK_OK, r, std::string(who) + ": phULR failed: " + g_last_error);
+        return ec;
+    }
+    // GW/UGW_excitation_density.f90 for both spins from the last rpa()
+    void densities() {
+        QK_HIP(hipMemcpyAsync(dX, XpY.data(), sizeof(double) * S2, hipMemcpyHostToDevice, st));
+        for (int sp = 0; sp < 2; ++sp)
+            dgemm(st, false, true, nSt, (int)n2, nSt, 1.0 / std::sqrt(2.0), dX, nSt, dM[sp], n2, 0.0, dR[sp], nSt);
+        QK_HIP(hipStreamSynchronize(st));
+    }
+    // per spin: diagonal self-energy at e_s (GW/UGW_self_energy_diag.f90 or the SRG form);
+    // EcGM per spin, halved (the restricted sum carries 4 rho^2, the unrestricted rho^2, x2 here)
+    void sigma(int sp, const double* es, double eta, int doSRG, double* sig, double* z, double* ecgm) {
+        double gm = 0.0;
+        sigma_diag_host(doSRG, eta, n, 0, nO[sp], 0, nSt, es, om.data(), nullptr, &gm, sig, z, dR[sp]);
+        if (ecgm) *ecgm = 0.5 * gm;
+    }
+    // GW/UGW_QP_graph.f90 for spin sp
+    void qp(int sp, int doSRG, double eta, const double* eHF_s, const double* eLin, const double* eOld, double* eGW_s,
+            double* z) {
+        qp_graph_host(doSRG, eta, SRG_FLOW, n, 0, nO[sp], 0, nSt, eHF_s, om.data(), nullptr, eLin, eOld, eGW_s, z,
+                      dR[sp]);
+    }
+};
 }  // namespace
 }  // namespace qk
 
@@ -1764,123 +1901,85 @@ int qk_ug0w0(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* E
         return QK_E_ARG;
     return guarded(nullptr, [&] {
         const int n = nBas;
-        const int nO[2] = {nOa, nOb}, v[2] = {n - nOa, n - nOb};
-        const int nSa = nO[0] * v[0], nSb = nO[1] * v[1], nSt = nSa + nSb;
-        QK_CHECK((long long)nSt * nSt < (1LL << 40), QK_E_ARG, "nSa + nSb too large");
-        const size_t S2 = (size_t)nSt * nSt, n2 = (size_t)n * n;
-        const double* G[3] = {ERI_aaaa, ERI_aabb, ERI_bbbb};
-        auto at = [&](int k, int p, int q, int r, int t) {  // ERI_k(p,q,r,s), 0-based
-            return G[k][p + (size_t)n * (q + (size_t)n * (r + (size_t)n * t))];
-        };
-        auto e = [&](int p, int sp) { return eHF[p + (size_t)n * sp]; };
-        // spin-conserving dRPA A and B (LR/phULR_A.f90:26-80, LR/phULR_B.f90:26-80, nC = nR = 0);
-        // pair index ia = a' + v i' within each spin block, the beta block after the alpha one
-        auto build = [&](const double* en, std::vector<double>& A, std::vector<double>& B) {
-            A.assign(S2, 0.0);
-            B.assign(S2, 0.0);
-            for (int si = 0; si < 2; ++si)
-                for (int sj = 0; sj < 2; ++sj) {
-                    const int oi = si ? nSa : 0, oj = sj ? nSa : 0;
-                    for (int i = 0; i < nO[si]; ++i)
-                        for (int a = 0; a < v[si]; ++a) {
-                            const size_t ia = oi + a + (size_t)v[si] * i;
-                            const int I = i, Aa = nO[si] + a;
-                            for (int j = 0; j < nO[sj]; ++j)
-                                for (int b = 0; b < v[sj]; ++b) {
-                                    const size_t jb = oj + b + (size_t)v[sj] * j;
-                                    const int J = j, Bb = nO[sj] + b;
-                                    double x, y;
-                                    if (si == sj) {  // ERI_ssss(i,b,a,j), ERI_ssss(i,j,a,b)
-                                        const int k = si ? 2 : 0;
-                                        x = at(k, I, Bb, Aa, J);
-                                        y = at(k, I, J, Aa, Bb);
-                                    } else if (si == 0) {  // alpha ia, beta jb: ERI_aabb(i,b,a,j), (i,j,a,b)
-                                        x = at(1, I, Bb, Aa, J);
-                                        y = at(1, I, J, Aa, Bb);
-                                    } else {  // beta ia, alpha jb: ERI_aabb(b,i,j,a), (j,i,b,a)
-                                        x = at(1, Bb, I, J, Aa);
-                                        y = at(1, J, I, Bb, Aa);
-                                    }
-                                    if (si == sj && i == j && a == b) x += en[Aa + (size_t)n * si] - en[I + (size_t)n * si];
-                                    A[ia + nSt * jb] = x;
-                                    B[ia + nSt * jb] = TDA_W ? 0.0 : y;
-                                }
-                        }
-                }
-        };
-        std::vector<double> A, B, XpY(S2), XmY(S2), om(nSt);
-        build(eHF, A, B);
-        double ec = 0.0;
-        int r = qk_phrlr(TDA_W, nSt, A.data(), B.data(), &ec, om.data(), XpY.data(), XmY.data());
-        QK_CHECK(r == QK_OK, r, "qk_ug0w0: phULR failed: " + g_last_error);
-        // excitation densities (GW/UGW_excitation_density.f90) as matrix-core products
-        // rhoT_s(m, pq) = sum_k XpY(m, k) M_s(pq, k) / sqrt(2): the unrestricted sums carry
-        // rho^2 where the restricted kernels carry 2 rho^2
-        std::vector<double> Ms(n2 * nSt), rhoq[2];
-        hipStream_t st = scratch().s;
-        double *dX = nullptr, *dM = nullptr, *dR = nullptr;
-        try {
-            dX = dalloc<double>(S2);
-            dM = dalloc<double>(n2 * nSt);
-            dR = dalloc<double>(n2 * nSt);
-            QK_HIP(hipMemcpyAsync(dX, XpY.data(), sizeof(double) * S2, hipMemcpyHostToDevice, st));
-            std::vector<double> hR(n2 * nSt);
-            for (int sp = 0; sp < 2; ++sp) {
-                for (int sj = 0; sj < 2; ++sj) {
-                    const int oj = sj ? nSa : 0;
-                    for (int j = 0; j < nO[sj]; ++j)
-                        for (int b = 0; b < v[sj]; ++b) {
-                            const size_t k = oj + b + (size_t)v[sj] * j;
-                            const int J = j, Bb = nO[sj] + b;
-                            for (int q = 0; q < n; ++q)
-                                for (int p = 0; p < n; ++p) {
-                                    double x;
-                                    if (sp == 0) x = sj == 0 ? at(0, p, J, q, Bb) : at(1, p, J, q, Bb);
-                                    else x = sj == 0 ? at(1, J, p, Bb, q) : at(2, p, J, q, Bb);
-                                    Ms[(p + (size_t)n * q) + n2 * k] = x;
-                                }
-                        }
-                }
-                QK_HIP(hipMemcpyAsync(dM, Ms.data(), sizeof(double) * n2 * nSt, hipMemcpyHostToDevice, st));
-                dgemm(st, false, true, nSt, (int)n2, nSt, 1.0 / std::sqrt(2.0), dX, nSt, dM, n2, 0.0, dR, nSt);
-                QK_HIP(hipMemcpyAsync(hR.data(), dR, sizeof(double) * n2 * nSt, hipMemcpyDeviceToHost, st));
-                QK_HIP(hipStreamSynchronize(st));
-                // to QuAcK's rho(p, q, m) layout for the per-spin sums
-                rhoq[sp].resize(n2 * nSt);
-                for (int m = 0; m < nSt; ++m)
-                    for (size_t pq = 0; pq < n2; ++pq) rhoq[sp][pq + n2 * m] = hR[m + (size_t)nSt * pq];
-            }
-        } catch (...) {
-            (void)hipStreamSynchronize(st);
-            dfree(dX); dfree(dM); dfree(dR);
-            throw;
-        }
-        dfree(dX); dfree(dM); dfree(dR);
-        // per spin: diagonal self-energy (GW/UGW_self_energy_diag.f90 or the SRG form),
-        // linearised energies, graphical QP (GW/UGW_QP_graph.f90) with G from eHF
+        UScreen u(n, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
+        u.rpa(eHF, "qk_ug0w0");
+        u.densities();
+        if (Om) std::memcpy(Om, u.om.data(), sizeof(double) * u.nSt);
+        // GW/UG0W0.f90:104-134 per spin: self-energy at eHF, linearised energies,
+        // graphical QP with the linearised guess and G from eHF
         for (int sp = 0; sp < 2; ++sp) {
             const double* es = eHF + (size_t)n * sp;
-            double gm = 0.0;
-            std::vector<double> sig(n), z(n), lin(n), eg(n);
-            sigma_diag_host(doSRG, eta, n, 0, nO[sp], 0, nSt, es, om.data(), rhoq[sp].data(), &gm, sig.data(), z.data());
-            if (EcGM) EcGM[sp] = 0.5 * gm;  // the restricted sum carries 4 rho^2, the unrestricted rho^2 (x2 here)
+            double *sig = SigC + (size_t)n * sp, *z = Z + (size_t)n * sp, *eg = eGW + (size_t)n * sp;
+            u.sigma(sp, es, eta, doSRG, sig, z, EcGM ? EcGM + sp : nullptr);
+            std::vector<double> lin(n);
             for (int p = 0; p < n; ++p) lin[p] = es[p] + z[p] * sig[p];
-            if (linearize) eg = lin;
-            else qp_graph_host(doSRG, eta, SRG_FLOW, n, 0, nO[sp], 0, nSt, es, om.data(), rhoq[sp].data(), lin.data(), es,
-                               eg.data(), z.data());
-            for (int p = 0; p < n; ++p) {
-                eGW[p + (size_t)n * sp] = eg[p];
-                SigC[p + (size_t)n * sp] = sig[p];
-                Z[p + (size_t)n * sp] = z[p];
-            }
+            if (linearize) std::memcpy(eg, lin.data(), sizeof(double) * n);
+            else u.qp(sp, doSRG, eta, es, lin.data(), es, eg, z);
         }
-        if (Om) std::memcpy(Om, om.data(), sizeof(double) * nSt);
         // RPA correlation energy at eGW (GW/UG0W0.f90:140-142)
-        build(eGW, A, B);
-        r = qk_phrlr(TDA_W, nSt, A.data(), B.data(), &ec, om.data(), XpY.data(), XmY.data());
-        QK_CHECK(r == QK_OK, r, "qk_ug0w0: phULR at eGW failed: " + g_last_error);
+        const double ec = u.rpa(eGW, "qk_ug0w0 (RPA at eGW)");
         if (EcRPA) *EcRPA = ec;
     });
+}
+
+int qk_evugw(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* ERI_aabb, const double* ERI_bbbb,
+             const double* eHF, int maxSCF, double thresh, int max_diis, double eta, int linearize, int TDA_W,
+             int doSRG, double* eGW, double* SigC, double* Z, double* Om, double* EcRPA, double* EcGM, int* nSCF_out) {
+    if (nBas <= 0 || nOa <= 0 || nOb <= 0 || nOa >= nBas || nOb >= nBas || !ERI_aaaa || !ERI_aabb || !ERI_bbbb ||
+        !eHF || !eGW || !SigC || !Z || maxSCF < 0)
+        return QK_E_ARG;
+    int conv_status = QK_OK;
+    int rc = guarded(nullptr, [&] {
+        const int n = nBas;
+        const size_t n2s = (size_t)n * 2;
+        UScreen u(n, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
+        // GW/evUGW.f90:103-215: eGW = eOld = eHF, Z = 1; per cycle screening at eGW,
+        // self-energy at eGW, QP energies (linearised: eHF + SigC; graphical: guess and
+        // G from eOld), Conv = max |eGW - eOld| over both spins, DIIS per spin with
+        // one shared history length
+        std::vector<double> e(eHF, eHF + n2s), eOld = e, err(n2s);
+        for (size_t p = 0; p < n2s; ++p) Z[p] = 1.0;
+        Diis diis[2] = {Diis(n, std::max(max_diis, 1)), Diis(n, std::max(max_diis, 1))};
+        int nSCF = 0;
+        double Conv = 1.0, ecrpa = 0.0, ecgm[2] = {0.0, 0.0};
+        while (Conv > thresh && nSCF <= maxSCF) {
+            ecrpa = u.rpa(e.data(), "qk_evugw");
+            u.densities();
+            for (int sp = 0; sp < 2; ++sp) {
+                const size_t o = (size_t)n * sp;
+                u.sigma(sp, e.data() + o, eta, doSRG, SigC + o, Z + o, ecgm + sp);
+                if (linearize) {
+                    for (int p = 0; p < n; ++p) e[o + p] = eHF[o + p] + SigC[o + p];
+                } else {
+                    std::vector<double> eg(n);
+                    u.qp(sp, doSRG, eta, eHF + o, eOld.data() + o, eOld.data() + o, eg.data(), Z + o);
+                    std::memcpy(e.data() + o, eg.data(), sizeof(double) * n);
+                }
+            }
+            Conv = 0.0;
+            for (size_t p = 0; p < n2s; ++p) {
+                err[p] = e[p] - eOld[p];
+                Conv = std::max(Conv, std::fabs(err[p]));
+            }
+            if (max_diis > 1)
+                for (int sp = 0; sp < 2; ++sp) diis[sp].step(err.data() + (size_t)n * sp, e.data() + (size_t)n * sp);
+            eOld = e;
+            ++nSCF;
+        }
+        std::memcpy(eGW, e.data(), sizeof(double) * n2s);
+        if (Om) std::memcpy(Om, u.om.data(), sizeof(double) * u.nSt);
+        if (EcRPA) *EcRPA = ecrpa;
+        if (EcGM) {
+            EcGM[0] = ecgm[0];
+            EcGM[1] = ecgm[1];
+        }
+        if (nSCF_out) *nSCF_out = nSCF;
+        if (nSCF == maxSCF + 1) {
+            conv_status = QK_E_NOCONV;
+            g_last_error = "evUGW: convergence failed after " + std::to_string(nSCF) + " cycles";
+        }
+    });
+    return rc != QK_OK ? rc : conv_status;
 }
 
 int qk_rgw_phbse(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE, int dTDA, int singlet, int triplet,

@@ -30,7 +30,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
            "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
-           "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms"]
+           "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms"]
 
 
 class QKError(RuntimeError):
@@ -122,6 +122,7 @@ def load():
     lib.qk_rgw_phbse.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 5
     lib.qk_gw_sigma_full.argtypes = [_H, _P, _D, _I, _P, _P, _P, _P]
     lib.qk_ug0w0.argtypes = [_I, _I, _I, _P, _P, _P, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
+    lib.qk_evugw.argtypes = [_I, _I, _I, _P, _P, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
     lib.qk_rhf_create.argtypes = [ctypes.POINTER(_H), _I]
     lib.qk_rhf_destroy.argtypes = [_H]
     lib.qk_rhf_last_error.argtypes = [_H]
@@ -592,4 +593,27 @@ def UG0W0(ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, nO, eta=0.0, linearize=False, TDA_W
                            _p(eHF), eta, int(linearize), int(TDA_W), int(doSRG), _p(eGW), _p(Sig), _p(Z), _p(Om),
                            ctypes.byref(Ec), _p(Egm)))
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": Ec.value, "EcGM": Egm}
+
+
+def evUGW(ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, eta=0.0, linearize=False,
+          TDA_W=False, doSRG=False):
+    """GW/evUGW.f90 on the GPU (qk_evugw), nC = nR = 0, QuAcK's defaults for
+    maxSCF, thresh and max_diis.  Returns dict(eGW, SigC, Z, Om, EcRPA, EcGM,
+    nSCF, converged); non-convergence is reported in "converged" with the last
+    cycle's outputs (QuAcK stops there)."""
+    eHF = np.asfortranarray(eHF, dtype=np.float64)
+    n = eHF.shape[0]
+    nSt = nO[0] * (n - nO[0]) + nO[1] * (n - nO[1])
+    eGW, Sig, Z = (np.zeros((n, 2), order="F") for _ in range(3))
+    Om = np.zeros(nSt)
+    Ec = _D(0.0)
+    Egm = np.zeros(2)
+    nscf = _I(0)
+    rc = load().qk_evugw(n, int(nO[0]), int(nO[1]), _p(_f64(ERI_aaaa)), _p(_f64(ERI_aabb)), _p(_f64(ERI_bbbb)),
+                         _p(eHF), int(maxSCF), thresh, int(max_diis), eta, int(linearize), int(TDA_W), int(doSRG),
+                         _p(eGW), _p(Sig), _p(Z), _p(Om), ctypes.byref(Ec), _p(Egm), ctypes.byref(nscf))
+    if rc != QK_E_NOCONV:
+        _check(rc)
+    return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": Ec.value, "EcGM": Egm, "nSCF": nscf.value,
+            "converged": rc == QK_OK}
 

@@ -345,3 +345,31 @@ def test_ug0w0_vs_compiled_reference(mo_systems):
     u = go.ug0w0(mo, mo, mo, e2, (nO, nO))
     rr = go.rg0w0(mo, s["eHF"], nO)
     assert np.abs(u["eGW"] - rr["eGW"][:, None]).max() < 1e-11 and abs(u["EcRPA"] - rr["EcRPA"]) < 1e-11
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+def test_evugw_vs_compiled_reference(mo_systems):
+    """Unrestricted evGW (GW/evUGW.f90): the restatement against the same cycle
+    composed of QuAcK's compiled phULR_A/B, phULR, UGW_excitation_density,
+    UGW_(SRG_)self_energy_diag, UGW_QP_graph and DIIS_extrapolation; the same
+    cycle counts and energies to 1e-11.  (Closed-shell SRG linearised evGW on
+    H2O does not converge in 64 cycles; there the two agree only to 1e-5, so it
+    is not a case here.)  With alpha = beta it is the restricted evGW."""
+    s = mo_systems["H2O"]
+    mo, nO = s["eri_mo"], s["nO"]
+    n = len(s["eHF"])
+    e2 = np.stack([s["eHF"], s["eHF"] + 0.01 * np.sin(np.arange(n))], axis=1)
+    for occ, kw in (((nO, nO), {"doSRG": True}), ((nO, nO), {"eta": 0.05}), ((nO + 1, nO), {"eta": 0.0}),
+                    ((nO + 1, nO), {"doSRG": True, "linearize": True}), ((nO + 1, nO), {"doSRG": True, "TDA_W": True})):
+        u = go.evugw(mo, mo, mo, e2, occ, **kw)
+        r = qkref.evugw(mo, mo, mo, e2, occ, eta=kw.get("eta", 0.0), linearize=kw.get("linearize", False),
+                        tda_w=kw.get("TDA_W", False), srg=kw.get("doSRG", False))
+        assert u["nSCF"] == r["nSCF"] and u["converged"], (occ, kw)
+        assert np.abs(u["eGW"] - r["eGW"]).max() < 1e-11, (occ, kw)
+        assert np.abs(u["Z"] - r["Z"]).max() < 1e-10 and abs(u["EcRPA"] - r["EcRPA"]) < 1e-11
+        ecgm = u["EcGM"].sum() if kw.get("doSRG") else u["EcGM"]
+        assert np.abs(ecgm - (r["EcGM"][0] if kw.get("doSRG") else r["EcGM"])).max() < 1e-11
+    e2 = np.stack([s["eHF"], s["eHF"]], axis=1)
+    u = go.evugw(mo, mo, mo, e2, (nO, nO), doSRG=True)
+    rr = go.evrgw(mo, s["eHF"], nO, maxSCF=64, doSRG=True)
+    assert np.abs(u["eGW"] - rr["eGW"][:, None]).max() < 1e-11 and abs(u["EcRPA"] - rr["EcRPA"]) < 1e-11
