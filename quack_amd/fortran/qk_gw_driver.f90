@@ -18,23 +18,28 @@ program qk_gw_driver
 
   implicit none
   character(len=512) :: wd, line
-  logical :: doG0W0, doevGW, doqsGW, doscGW, dotest, dotU, dotG
+  logical :: doG0W0, doevGW, doqsGW, doscGW, dotest, dotU, dotG, doRHF, doUHF
   integer :: maxSCF, max_diis, nfreqs, nAt, nEla, nElb, nC, nR, nO, nV, nS, nOrb, ios
   double precision :: thresh, eta, shift, ENuc, ERHF
   character(len=1) :: a1, a2, a3, a4, a5, a6, a7, a8
   logical :: lin, TDA_W, reg, TDA, singlet, triplet, dophBSE, dophBSE2, doppBSE, dBSE, dTDA
-  double precision, allocatable :: eHF(:), ERI(:,:,:,:), dip(:,:,:), eGW(:)
+  double precision, allocatable :: eHF(:), ERI(:,:,:,:), dip(:,:,:), eGW(:), eU(:,:), SU(:,:), cU(:,:,:)
+  integer :: nOU(2), nCU(2), nVU(2), nRU(2), nSU(2)
 
   call get_command_argument(1, wd)
   if(len_trim(wd) == 0) wd = '.'
 
   ! methods
-  doG0W0 = .false.; doevGW = .false.; dotest = .false.
+  doG0W0 = .false.; doevGW = .false.; dotest = .false.; doRHF = .true.; doUHF = .false.
   open(unit=11, file=trim(wd)//'/input/methods', status='old', action='read')
   do
     read(11,'(A)',iostat=ios) line
     if(ios /= 0) exit
-    if(index(line, '# G0W0') == 1) then
+    if(index(line, '# RHF UHF') == 1) then
+      ! src/QuAcK/read_methods.f90: the HF line selects the restricted or unrestricted branch
+      read(11,*) a1, a2
+      doRHF = a1 == 'T'; doUHF = a2 == 'T'
+    else if(index(line, '# G0W0') == 1) then
       read(11,*) a1, a2, a3, a4
       doG0W0 = a1 == 'T'; doevGW = a2 == 'T'; doqsGW = a3 == 'T'; doscGW = a4 == 'T'
     else if(index(line, '# Rtest') == 1) then
@@ -72,8 +77,8 @@ program qk_gw_driver
   read(13,*)
   read(13,*) nAt, nEla, nElb, nC, nR
   close(13)
-  if(nEla /= nElb) then
-    write(*,*) 'qk_gw_driver: restricted (closed-shell) molecules only'
+  if(nEla /= nElb .and. .not. doUHF) then
+    write(*,*) 'qk_gw_driver: open-shell molecules need the UHF branch'
     error stop 2
   end if
   nO = nEla
@@ -98,6 +103,25 @@ program qk_gw_driver
   close(16)
 
   if(dotest) open(unit=1231597, file=trim(wd)//'/test/Rtest.dat', status='replace')
+  if(dotest .and. doUHF) open(unit=1232584, file=trim(wd)//'/test/Utest.dat', status='replace')
+
+  if(doUHF) then
+    ! unrestricted branch (src/GW/UGW.f90:82-84,114-116) on the orbitals of int/MO.dat for both
+    ! spins, nEla and nElb electrons: the three spin blocks are the same MO integrals
+    nOU = [nEla, nElb]; nCU = nC; nRU = nR; nVU = nOrb - nOU; nSU = (nOU - nCU)*(nVU - nRU)
+    allocate(eU(nOrb,2), SU(nOrb,nOrb), cU(nOrb,nOrb,2))
+    eU(:,1) = eHF; eU(:,2) = eHF; SU = 0d0; cU = 0d0
+    if(doG0W0) call UG0W0_hip(dotest, .false., .false., .false., dophBSE, TDA_W, TDA, dBSE, dTDA, .true., .false., &
+                              lin, eta, reg, nOrb, nCU, nOU, nVU, nRU, nSU, ENuc, ERHF, SU, ERI, ERI, ERI, dip, dip, &
+                              cU, eU)
+    if(doevGW) call evUGW_hip(dotest, maxSCF, thresh, max_diis, .false., .false., .false., dophBSE, TDA_W, TDA, &
+                              dBSE, dTDA, .true., .false., lin, eta, reg, nOrb, nCU, nOU, nVU, nRU, nSU, ENuc, ERHF, &
+                              SU, ERI, ERI, ERI, dip, dip, cU, eU)
+    deallocate(eU, SU, cU)
+  end if
+  if(.not. doRHF) then
+    doG0W0 = .false.; doevGW = .false.
+  end if
 
   if(doG0W0) call RG0W0_hip(dotest, .false., .false., .false., dophBSE, dophBSE2, TDA_W, TDA, dBSE, dTDA, &
                             doppBSE, singlet, triplet, lin, eta, reg, .false., nOrb, nOrb, nC, nO, nV, nR, nS, ENuc, &
@@ -107,6 +131,7 @@ program qk_gw_driver
                             nV, nR, nS, ENuc, ERHF, ERI, dip, eHF)
 
   if(dotest) close(1231597)
+  if(dotest .and. doUHF) close(1232584)
 
 end program
 
@@ -120,5 +145,8 @@ subroutine dump_test_value(branch, description, val)
   if(branch == 'R') then
     write(1231597,*) trim(description)
     write(1231597,'(F20.15)') val
+  else if(branch == 'U') then
+    write(1232584,*) trim(description)
+    write(1232584,'(F20.15)') val
   end if
 end subroutine

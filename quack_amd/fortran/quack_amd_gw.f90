@@ -81,6 +81,27 @@ module quack_amd_gw
       real(c_double), intent(out) :: EcBSE(2), OmBSE(*)
     end function
 
+    ! unrestricted G0W0 and evGW (GW/UG0W0.f90, GW/evUGW.f90), nC = nR = 0
+    integer(c_int) function qk_ug0w0(nBas, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, eta, linearize, TDA_W, doSRG, &
+                                     eGW, SigC, Z, Om, EcRPA, EcGM) bind(C, name='qk_ug0w0')
+      import :: c_int, c_double
+      integer(c_int), value :: nBas, nOa, nOb, linearize, TDA_W, doSRG
+      real(c_double), intent(in) :: ERI_aaaa(*), ERI_aabb(*), ERI_bbbb(*), eHF(*)
+      real(c_double), value :: eta
+      real(c_double), intent(out) :: eGW(*), SigC(*), Z(*), Om(*), EcRPA, EcGM(2)
+    end function
+
+    integer(c_int) function qk_evugw(nBas, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, maxSCF, thresh, max_diis, eta, &
+                                     linearize, TDA_W, doSRG, eGW, SigC, Z, Om, EcRPA, EcGM, nSCF) &
+                                     bind(C, name='qk_evugw')
+      import :: c_int, c_double
+      integer(c_int), value :: nBas, nOa, nOb, maxSCF, max_diis, linearize, TDA_W, doSRG
+      real(c_double), intent(in) :: ERI_aaaa(*), ERI_aabb(*), ERI_bbbb(*), eHF(*)
+      real(c_double), value :: thresh, eta
+      real(c_double), intent(out) :: eGW(*), SigC(*), Z(*), Om(*), EcRPA, EcGM(2)
+      integer(c_int), intent(out) :: nSCF
+    end function
+
     ! RHF Fock builds (HF/RHF.f90:178-179) on the device
     integer(c_int) function qk_rhf_create(h, nBas) bind(C, name='qk_rhf_create')
       import :: c_int, c_ptr

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 METHODS = """# RHF UHF GHF ROHF HFB cRHF cUHF eRHF scGHF
-  T   F   F   F    F   F    F    F    F
+  {rhf}   {uhf}   F   F    F   F    F    F    F
 # MP2 MP3
   F   F
 # CCD pCCD DCD CCSD CCSD(T)
@@ -71,21 +71,24 @@ def tf(b):
 
 
 def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=1e-5, max_diis=5, lin=False,
-                 eta=0.0, TDA_W=False, reg=False, eri_mo=None, bse=False):
+                 eta=0.0, TDA_W=False, reg=False, eri_mo=None, bse=False, uhf=False, nel=None):
     """sysd: a dict from quack_amd.intgen.system(); eri_mo: MO integrals (nOrb^4, flat);
-    bse: the phBSE flag of the "# BSE:" options line (static kernel)."""
+    bse: the phBSE flag of the "# BSE:" options line (static kernel); uhf: the
+    methods file selects UHF instead of RHF (the unrestricted GW branch);
+    nel: (nEla, nElb) for input/molecule (default: the closed shell)."""
     for d in ("input", "int", "test"):
         os.makedirs(os.path.join(wd, d), exist_ok=True)
     mol = sysd["mol"]
     with open(os.path.join(wd, "input", "methods"), "w") as f:
-        f.write(METHODS.format(g0w0=tf(g0w0), evgw=tf(evgw), test=tf(test)))
+        f.write(METHODS.format(g0w0=tf(g0w0), evgw=tf(evgw), test=tf(test), rhf=tf(not uhf), uhf=tf(uhf)))
     with open(os.path.join(wd, "input", "options"), "w") as f:
         f.write(OPTIONS.format(maxscf=maxSCF, thresh="%.10g" % thresh, diis=max_diis, lin=tf(lin), eta="%.10g" % eta,
                                tda=tf(TDA_W), reg=tf(reg), bse=tf(bse)))
     with open(os.path.join(wd, "input", "hpc_flags"), "w") as f:
         f.write("# if True (T), switch to HPC mode\n  F\n# if True (T), use GPU\n  T\n")
     with open(os.path.join(wd, "input", "molecule"), "w") as f:
-        f.write("# nAt nEla nElb nCore nRyd\n%d %d %d 0 0\n# Znuc x  y  z\n" % (len(mol.symbols), mol.nO, mol.nO))
+        nea, neb = nel if nel is not None else (mol.nO, mol.nO)
+        f.write("# nAt nEla nElb nCore nRyd\n%d %d %d 0 0\n# Znuc x  y  z\n" % (len(mol.symbols), nea, neb))
         for s, r in zip(mol.symbols, mol.xyz):
             f.write("%s %.17g %.17g %.17g\n" % (s, r[0], r[1], r[2]))
     with open(os.path.join(wd, "int", "ENuc.dat"), "w") as f:
@@ -153,6 +156,13 @@ def read_eri_chem(wd, nBas):
         raise ValueError("int/ERI_chem.bin holds %d values, expected %d for nBas = %d" % (x.size, P * (P + 1) // 2,
                                                                                           nBas))
     return x
+
+
+def read_test_dump_pairs(path):
+    """The same dump as (description, value) pairs in file order (the
+    unrestricted drivers write some descriptions twice, once per spin)."""
+    lines = [ln.strip() for ln in open(path) if ln.strip()]
+    return [(lines[k], float(lines[k + 1])) for k in range(0, len(lines) - 1, 2)]
 
 
 def read_test_dump(path):

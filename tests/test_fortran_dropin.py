@@ -77,3 +77,54 @@ def test_driver_phbse_matches_oracle(tmp_path, mo_systems):
     # the excitation tables of LR/print_excitation_energies.f90 (method and manifold in A15 fields)
     assert "phBSE@GW@RHF" in r.stdout and "singlet manifold" in r.stdout and "triplet manifold" in r.stdout
     assert np.isfinite(vals[("evGW", "singlet")])
+
+
+def test_input_files_uhf_flag(tmp_path, systems):
+    """The unrestricted branch is selected by the HF line of input/methods and
+    the electron counts of input/molecule, as QuAcK reads them."""
+    s = systems["H2O"]
+    quackio.write_inputs(str(tmp_path), s, uhf=True, nel=(s["nO"] + 1, s["nO"]))
+    m = open(tmp_path / "input" / "methods").read().splitlines()
+    assert m[0].startswith("# RHF UHF") and m[1].split()[:2] == ["F", "T"]
+    mol = open(tmp_path / "input" / "molecule").read().splitlines()
+    assert mol[1].split()[1:3] == [str(s["nO"] + 1), str(s["nO"])]
+
+
+@pytest.mark.gpu
+def test_driver_unrestricted(tmp_path, mo_systems):
+    """UG0W0_hip and evUGW_hip through the driver's UHF branch: a closed shell
+    reproduces QuAcK's restricted goldens (UHF = RHF there), and one more
+    alpha electron matches the CPU restatement of UG0W0/evUGW on the same
+    orbitals; the values come from test/Utest.dat in QuAcK's dump format."""
+    import numpy as np
+
+    from oracle import gw_oracle as go
+    s = mo_systems["H2O"]
+    nO, mo = s["nO"], s["eri_mo"]
+    print("fortran driver UHF H2O closed shell", flush=True)
+    wd = tmp_path / "closed"
+    quackio.write_inputs(str(wd), s, g0w0=True, evgw=True, maxSCF=256, thresh=1e-5, max_diis=1, eri_mo=mo, uhf=True)
+    r = subprocess.run([DRIVER, str(wd)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "G0W0@UHF HOMO      energy" in r.stdout and "evGW@UHF HOMO      energy" in r.stdout
+    d = quackio.read_test_dump_pairs(str(wd / "test" / "Utest.dat"))
+    g = GOLD["H2O"]["feather"]
+    for tag, k0 in (("G0W0", 0), ("evGW", 5)):
+        assert d[k0][0] == tag + " correlation energy"
+        assert abs(d[k0][1] - g[tag + " correlation energy"]) < 1e-7 * abs(g[tag + " correlation energy"])
+        for k, key in ((1, "HOMO"), (2, "LUMO"), (3, "HOMO"), (4, "LUMO")):
+            ref = g["%s %s energy" % (tag, key)]
+            assert abs(d[k0 + k][1] - ref) < 1e-7 * abs(ref), (tag, k)
+    print("fortran driver UHF H2O open shell", flush=True)
+    wd = tmp_path / "open"
+    occ = (nO + 1, nO)
+    quackio.write_inputs(str(wd), s, g0w0=True, evgw=True, maxSCF=64, thresh=1e-5, max_diis=5, eri_mo=mo, uhf=True,
+                         nel=occ)
+    r = subprocess.run([DRIVER, str(wd)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    d = quackio.read_test_dump_pairs(str(wd / "test" / "Utest.dat"))
+    e2 = np.stack([s["eHF"], s["eHF"]], axis=1)
+    for k0, o in ((0, go.ug0w0(mo, mo, mo, e2, occ)), (5, go.evugw(mo, mo, mo, e2, occ, maxSCF=64, max_diis=5))):
+        ref = [o["EcRPA"], o["eGW"][occ[0] - 1, 0], o["eGW"][occ[0], 0], o["eGW"][occ[1] - 1, 1], o["eGW"][occ[1], 1]]
+        for k in range(5):
+            assert abs(d[k0 + k][1] - ref[k]) < 1e-8, (k0, k, d[k0 + k], ref[k])
