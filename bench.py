@@ -185,7 +185,8 @@ def main():
                     help="also run the learned warm start once and report it beside the headline")
     ap.add_argument("--no-reg", action="store_true", help="plain eta=0 evGW instead of SRG-regularised")
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
-    ap.add_argument("--workload", choices=["h50", "df600", "batch", "learned"], default="h50",
+    ap.add_argument("--ugw-atoms", type=int, default=10, help="ugw workload: H chain length (one more alpha electron)")
+    ap.add_argument("--workload", choices=["h50", "df600", "batch", "learned", "ugw"], default="h50",
                     help="h50: the headline N=250 lattice; df600: the precision-form measurement on the synthetic "
                          "density-fitted system (N=600, o=60, naux=1800), FP64 against split precision; batch: "
                          "batched G0W0 over seeded H chains/lattices of uneven size; learned: the MLP warm start "
@@ -206,6 +207,8 @@ def main():
         return run_df600(args)
     if args.workload == "batch":
         return run_batch(args)
+    if args.workload == "ugw":
+        return run_ugw(args)
     if args.workload == "learned":
         return run_learned(args)
     K = max(1, args.concurrent)
@@ -653,5 +656,70 @@ def learned_form(eng, scf, nO, ref, args):
     return out
 
 
+
+UGW_SAMPLE = r"""
+import sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from quack_amd import intgen as ig
+from oracle import qkref
+s = ig.system(ig.h_chain(int(sys.argv[2]), seed=0))
+mo = qkref.aotomo(s["c"], s["eri"])
+e2 = np.stack([s["eHF"], s["eHF"]], axis=1)
+occ = (s["nO"] + 1, s["nO"])
+t = time.time()
+r = qkref.evugw(mo, mo, mo, e2, occ, srg=True)
+t = time.time() - t
+np.savez(sys.argv[3], mo=mo, e2=e2, occ=np.array(occ), eGW=r["eGW"], nSCF=r["nSCF"], t=t)
+"""
+
+
+def run_ugw(args):
+    """Unrestricted SRG-evGW (GW/evUGW.f90) on an open-shell H chain (cc-pVDZ,
+    RHF orbitals for both spins, one more alpha electron): one complete
+    molecule on the GPU (qk_evugw) against QuAcK's own unrestricted routines
+    compiled from source (oracle/_ref, flang + MKL on the host cores, in a
+    child process without torch) on the same inputs."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from quack_amd import gw
+    path = os.path.join(tempfile.mkdtemp(), "ugw.npz")
+    p = subprocess.run([sys.executable, "-c", UGW_SAMPLE, ROOT, str(args.ugw_atoms), path], capture_output=True,
+                       text=True, timeout=900)
+    if p.returncode != 0:
+        raise RuntimeError("reference run failed: " + p.stderr[-2000:])
+    d = np.load(path)
+    mo, e2, occ = d["mo"], d["e2"], tuple(int(x) for x in d["occ"])
+    n = e2.shape[0]
+    torch.cuda.init()
+    for _ in range(max(args.warmup, 1)):
+        gw.evUGW(mo, mo, mo, e2, occ, doSRG=True, maxSCF=1)
+    ts = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = gw.evUGW(mo, mo, mo, e2, occ, doSRG=True)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t = float(np.mean(ts))
+    print(json.dumps({
+        "metric": "unrestricted SRG-evGW molecules per second", "value": 1.0 / t, "unit": "mol/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "ugw: H%d chain cc-pVDZ, nBas %d, nO %s, nSa + nSb %d" % (
+            args.ugw_atoms, n, list(occ), occ[0] * (n - occ[0]) + occ[1] * (n - occ[1]))},
+        "cycles": int(g["nSCF"]), "cycles_reference": int(d["nSCF"]),
+        "max_abs_dE_vs_reference_Ha": float(np.abs(g["eGW"] - d["eGW"]).max()),
+        "cpu_baseline": {"value": 1.0 / float(d["t"]), "unit": "mol/s", "cores": int(os.environ.get("OMP_NUM_THREADS",
+                                                                                                   os.cpu_count())),
+                         "kind": "reference", "sample": "the same molecule, QuAcK's compiled unrestricted routines"},
+    }), flush=True)
+
+
 if __name__ == "__main__":
     main()
+
