@@ -82,7 +82,7 @@ def build_inputs(nx, ny, seed):
 CPU_SAMPLE = r"""
 import json, os, sys, time
 sys.path.insert(0, sys.argv[1])
-method, max_diis, reg = sys.argv[2], int(sys.argv[3]), bool(int(sys.argv[4]))
+method, max_diis, srg, eta = sys.argv[2], int(sys.argv[3]), bool(int(sys.argv[4])), float(sys.argv[5])
 from oracle import qkref
 from quack_amd import intgen as ig
 s = ig.system(ig.h_chain(10, seed=0))
@@ -90,32 +90,78 @@ nO = s["nO"]
 def one():
     mo = qkref.aotomo(s["c"], s["eri"])
     if method == "evgw":
-        return qkref.evrgw(mo, s["eHF"], nO, maxSCF=256, thresh=1e-6, max_diis=max_diis, srg=reg)
-    return qkref.rg0w0(mo, s["eHF"], nO, srg=reg)
+        return qkref.evrgw(mo, s["eHF"], nO, maxSCF=256, thresh=1e-6, max_diis=max_diis, srg=srg, eta=eta)
+    return qkref.rg0w0(mo, s["eHF"], nO, srg=srg, eta=eta)
 sys.stdout.flush()
 saved = os.dup(1); null = os.open(os.devnull, os.O_WRONLY); os.dup2(null, 1)
 r = one()
-reps, t0 = 0, time.time()
-while time.time() - t0 < 10.0:
-    r = one(); reps += 1
-t = (time.time() - t0) / reps
+ts = []
+t0 = time.time()
+while time.time() - t0 < 10.0 or len(ts) < 5:
+    t = time.time(); r = one(); ts.append(time.time() - t)
+ts.sort()
 os.dup2(saved, 1)
-print(json.dumps({"t": t, "reps": reps, "eGW": list(r["eGW"]), "EcRPA": float(r["EcRPA"])}), flush=True)
+print(json.dumps({"t": ts[len(ts) // 2], "reps": len(ts), "eGW": list(r["eGW"]), "EcRPA": float(r["EcRPA"])}), flush=True)
+"""
+
+# QuAcK's compiled path at N = 250 on the host cores: AOtoMO_ERI_RHF and one
+# evGW cycle (evRGW.f90:119-183 with maxSCF = 0: phRLR_A/B, phRLR with its two
+# dsyev, RGW_excitation_density, RGW_self_energy_diag, RGW_QP_graph), each
+# timed once, on the RHF solution the GPU computed (passed in a file).
+CPU250_SAMPLE = r"""
+import json, os, sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from oracle import qkref
+from quack_amd import intgen as ig
+d = np.load(sys.argv[2])
+mol = ig.h_lattice(int(d["nx"]), int(d["ny"]), seed=int(d["seed"]))
+t = time.time(); eri = mol.eri(); t_int = time.time() - t
+print("cpu250: AO integrals on the host in %.1f s" % t_int, file=sys.stderr, flush=True)
+saved = os.dup(1); null = os.open(os.devnull, os.O_WRONLY); os.dup2(null, 1)
+t = time.time(); mo = qkref.aotomo(d["c"], eri); t_ao = time.time() - t
+del eri
+print("cpu250: AOtoMO_ERI_RHF %.1f s" % t_ao, file=sys.stderr, flush=True)
+t = time.time()
+r = qkref.evrgw(mo, d["eHF"], int(d["nO"]), maxSCF=0, thresh=1e-6, max_diis=int(d["max_diis"]),
+                eta=float(d["eta"]), srg=bool(d["srg"]))
+t_cyc = time.time() - t
+os.dup2(saved, 1)
+print(json.dumps({"t_aotomo": t_ao, "t_cycle": t_cyc, "eGW1": list(r["eGW"]), "t_integrals": t_int}), flush=True)
 """
 
 
-def cpu_baseline(method, max_diis, reg):
-    """QuAcK's own routines (oracle/_ref: GW/, LR/, AOtoMO/ compiled from
-    source with flang + MKL) on the host cores, on a bounded sample.
+def host_cpu():
+    """Physical cores, logical CPUs and model of the host (lscpu)."""
+    import subprocess
+    info = {"model": None, "physical_cores": None, "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = dict((ln.split(":", 1)[0].strip(), ln.split(":", 1)[1].strip()) for ln in out.splitlines() if ":" in ln)
+        info["model"] = kv.get("Model name")
+        if kv.get("Core(s) per socket") and kv.get("Socket(s)"):
+            info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except Exception:
+        pass
+    return info
 
-    The N = 250 lattice is out of reach for a bounded sample (QuAcK's
-    AOtoMO_ERI_RHF alone takes ~3 min there on the 16 host cores, and one
-    dsyev of nS = 5600 several times longer than a whole GPU molecule), so the
-    sample is one complete molecule of the smallest member of the workload
-    family: the H10 chain (cc-pVDZ, N = 50, nS = 225), AO->MO + the same
-    method.  It runs in a child process (the reference's MKL must not share a
-    process with torch).  The GPU runs the same molecule right after, so the
-    ratio at equal work is reported beside it."""
+
+def sample_threads():
+    """Host threads the reference's MKL/OpenMP use (OMP_NUM_THREADS, as set on the host)."""
+    return int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
+
+
+def cpu_baseline(method, max_diis, mk):
+    """QuAcK's own routines (oracle/_ref: GW/, LR/, AOtoMO/ compiled from
+    source with flang + MKL) on the host cores, on a bounded sample: one
+    complete molecule of the smallest member of the workload family, the H10
+    chain (cc-pVDZ, N = 50, nS = 225), AO->MO + the same method, median of at
+    least 5 runs over ~10 s.  It runs in a child process (the reference's MKL
+    must not share a process with torch).  The GPU runs the same molecule
+    right after, so the ratio at equal work is reported beside it.  The N = 250
+    figure (QuAcK's AOtoMO_ERI_RHF plus one evGW cycle, timed by
+    `bench.py --workload cpu250`, extrapolated over the cycles) is read from
+    profiles/cpu_reference_h50_lattice.md when present."""
     import subprocess
 
     import numpy as np
@@ -125,8 +171,8 @@ def cpu_baseline(method, max_diis, reg):
     from quack_amd import intgen as ig
     if not qkref.available():
         return None
-    p = subprocess.run([sys.executable, "-c", CPU_SAMPLE, ROOT, method, str(max_diis), str(int(reg))], capture_output=True,
-                       text=True, timeout=300)
+    p = subprocess.run([sys.executable, "-c", CPU_SAMPLE, ROOT, method, str(max_diis), str(int(mk["doSRG"])),
+                        str(mk["eta"])], capture_output=True, text=True, timeout=300)
     if p.returncode != 0:
         raise RuntimeError("cpu sample failed: " + p.stderr[-2000:])
     ref = json.loads(p.stdout.strip().splitlines()[-1])
@@ -136,26 +182,108 @@ def cpu_baseline(method, max_diis, reg):
         def gpu_one():
             g.set_eri_ao(s["eri"], s["c"])
             if method == "evgw":
-                return g.evgw(s["eHF"], maxSCF=256, thresh=1e-6, max_diis=max_diis, doSRG=reg)
-            return g.g0w0(s["eHF"], doSRG=reg)
+                return g.evgw(s["eHF"], maxSCF=256, thresh=1e-6, max_diis=max_diis, **mk)
+            return g.g0w0(s["eHF"], **mk)
         r = gpu_one()
-        reps_g, t0 = 0, time.time()
-        while time.time() - t0 < 2.0:
+        ts, t0 = [], time.time()
+        while time.time() - t0 < 2.0 or len(ts) < 5:
+            t = time.time()
             gpu_one()
-            reps_g += 1
-        t_gpu = (time.time() - t0) / reps_g
+            ts.append(time.time() - t)
+        t_gpu = float(np.median(ts))
     err = float(np.abs(np.asarray(ref["eGW"]) - r["eGW"]).max())
-    log("cpu baseline: H10 %s reference %.3f s/molecule, GPU %.4f s/molecule, max|eGW diff| %.2e Ha" %
+    log("cpu baseline: H10 %s reference %.3f s/molecule (median), GPU %.4f s/molecule, max|eGW diff| %.2e Ha" %
         (method, ref["t"], t_gpu, err))
-    threads = os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
-    return {"value": 1.0 / ref["t"], "unit": "molecules/s", "cores": int(threads), "kind": "reference",
-            "sample": ("%d x one H10-chain molecule (cc-pVDZ, N=50, nS=225), same options: QuAcK's AOtoMO_ERI_RHF + %s "
-                       "(phRLR_A/B, phRLR, RGW_excitation_density, RGW_self_energy_diag, RGW_QP_graph%s), "
-                       "host ERIs; the N=250 lattice itself does not fit a bounded CPU sample" %
-                       (ref["reps"], "evRGW loop" if method == "evgw" else "RG0W0 sequence",
-                        ", DIIS" if (method == "evgw" and max_diis > 1) else "")),
-            "gpu_same_sample": 1.0 / t_gpu, "gpu_over_cpu_same_sample": ref["t"] / t_gpu,
-            "max_abs_eGW_diff_same_sample": err}
+    hc = host_cpu()
+    out = {"value": 1.0 / ref["t"], "unit": "molecules/s", "cores": sample_threads(), "kind": "reference",
+           "host": hc,
+           "sample": ("median of %d runs of one H10-chain molecule (cc-pVDZ, N=50, nS=225), same options: QuAcK's "
+                      "AOtoMO_ERI_RHF + %s (phRLR_A/B, phRLR, RGW_excitation_density, RGW_self_energy_diag, "
+                      "RGW_QP_graph%s), host ERIs" %
+                      (ref["reps"], "evRGW loop" if method == "evgw" else "RG0W0 sequence",
+                       ", DIIS" if (method == "evgw" and max_diis > 1) else "")),
+           "gpu_same_sample": 1.0 / t_gpu, "gpu_over_cpu_same_sample": ref["t"] / t_gpu,
+           "max_abs_eGW_diff_same_sample": err}
+    n250 = read_cpu250()
+    if n250:
+        out["n250"] = n250
+    return out
+
+
+def read_cpu250():
+    """The recorded N = 250 reference figure (profiles/cpu_reference_h50_lattice.md)."""
+    f = os.path.join(ROOT, "profiles", "cpu_reference_h50_lattice.md")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("json: "):
+            try:
+                return json.loads(line[6:])
+            except ValueError:
+                return None
+    return None
+
+
+def run_cpu250(args):
+    """QuAcK's compiled path on the host cores at N = 250 (the first H50
+    lattice): AOtoMO_ERI_RHF and one evGW cycle, each timed once on the RHF
+    solution and AO integrals of the GPU inputs, against the plain FP64 GPU
+    form of the same molecule; the reference's whole-molecule time is
+    extrapolated as AO->MO + cycles x one cycle (labelled so)."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from quack_amd import gw, intgen, workloads
+    torch.cuda.init()
+    gw.load()
+    mk = method_kw(args)
+    mol = intgen.h_lattice(args.nx, args.ny, seed=args.seed)
+    scf, eri_t, c_t = workloads.device_inputs(mol, log=log)
+    with gw.GWEngine(mol.nBas, scf["nOrb"], mol.nO) as g:
+        g.set_precision("fp64")
+        g.set_eigensolver("syevd")
+        g.set_eri_ao_device(eri_t.data_ptr(), c_t.data_ptr())
+        g.evgw(scf["eHF"], maxSCF=256, thresh=1e-6, max_diis=args.max_diis, **mk)
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t = time.time()
+            g.set_eri_ao_device(eri_t.data_ptr(), c_t.data_ptr())
+            ref = g.evgw(scf["eHF"], maxSCF=256, thresh=1e-6, max_diis=args.max_diis, **mk)
+            ts.append(time.time() - t)
+        try:
+            r1 = g.evgw(scf["eHF"], maxSCF=0, thresh=0.0, max_diis=args.max_diis, **mk)
+        except gw.NotConverged as ex:
+            r1 = ex.result
+    del eri_t, c_t
+    torch.cuda.empty_cache()
+    t_gpu = float(np.median(ts))
+    path = os.path.join(tempfile.mkdtemp(), "in.npz")
+    np.savez(path, c=scf["c"], eHF=scf["eHF"], nO=mol.nO, nx=args.nx, ny=args.ny, seed=args.seed,
+             max_diis=args.max_diis, eta=mk["eta"], srg=int(mk["doSRG"]))
+    log("cpu250: GPU plain FP64 %.3f s/molecule (%d cycles); reference on the host cores next" % (t_gpu, ref["nSCF"]))
+    p = subprocess.Popen([sys.executable, "-u", "-c", CPU250_SAMPLE, ROOT, path], stdout=subprocess.PIPE, text=True)
+    outp, _ = p.communicate(timeout=3000)
+    if p.returncode != 0:
+        raise RuntimeError("reference run failed")
+    d = json.loads(outp.strip().splitlines()[-1])
+    err1 = float(np.abs(np.asarray(d["eGW1"]) - r1["eGW"]).max())
+    cycles = int(ref["nSCF"])
+    t_ref = d["t_aotomo"] + cycles * d["t_cycle"]
+    out = {"metric": "QuAcK's compiled restricted evGW path at N=250 on the host cores (extrapolated)",
+           "value": 1.0 / t_ref, "unit": "molecules/s", "n_gpus": 1, "higher_is_better": True,
+           "config": {"workload": "H50 5x10 lattice seed %d cc-pVDZ, nBas %d, nOrb %d, %s evGW DIIS %d" %
+                                  (args.seed, mol.nBas, scf["nOrb"],
+                                   "SRG" if mk["doSRG"] else "eta=%g" % mk["eta"], args.max_diis)},
+           "reference_aotomo_s": d["t_aotomo"], "reference_one_cycle_s": d["t_cycle"], "cycles": cycles,
+           "reference_s_per_molecule_extrapolated": t_ref,
+           "extrapolation": "AOtoMO_ERI_RHF + cycles x (one evGW cycle), each timed once",
+           "first_cycle_max_abs_dE_vs_gpu_Ha": err1, "gpu_plain_fp64_s_per_molecule_median_of_3": t_gpu,
+           "gpu_plain_fp64_over_reference": t_ref / t_gpu, "cores": sample_threads(), "host": host_cpu()}
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -167,7 +295,11 @@ def main():
     ap.add_argument("--nx", type=int, default=5)
     ap.add_argument("--ny", type=int, default=10)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--max-diis", type=int, default=1)
+    ap.add_argument("--reg", choices=["eta", "srg", "none"], default="eta",
+                    help="eta: QuAcK's broadened self-energy (eta = --eta); srg: QuAcK's SRG regularisation; none: "
+                         "eta = 0 as in QuAcK's default options")
+    ap.add_argument("--eta", type=float, default=0.05)
+    ap.add_argument("--max-diis", type=int, default=5)
     ap.add_argument("--form", choices=["fp64", "mixed", "mixed2"], default="mixed2",
                     help="fp64: QuAcK's arithmetic; mixed: excitation densities from three BF16 slices "
                          "(six products); mixed2: two slices (three products)")
@@ -179,18 +311,19 @@ def main():
                     help="refinement is tried when no orbital energy moved by more than this (Ha) since the stored "
                          "eigenvectors")
     ap.add_argument("--concurrent", type=int, default=4,
-                    help="molecules in flight at once (one engine, stream and host thread each); a step is one "
-                         "such batch")
-    ap.add_argument("--learned", choices=["auto", "off"], default="auto",
-                    help="also run the learned warm start once and report it beside the headline")
-    ap.add_argument("--no-reg", action="store_true", help="plain eta=0 evGW instead of SRG-regularised")
+                    help="molecules in flight at once (one engine, stream and host thread each, each its own seeded "
+                         "lattice); a step is one such batch")
+    ap.add_argument("--learned", choices=["on", "off"], default="on",
+                    help="on: the timed form starts evGW from the MLP's prediction (quack_amd/learned.py)")
+    ap.add_argument("--baseline-reps", type=int, default=3, help="runs of the plain FP64 form (median reported)")
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--ugw-atoms", type=int, default=10, help="ugw workload: H chain length (one more alpha electron)")
-    ap.add_argument("--workload", choices=["h50", "df600", "batch", "learned", "ugw"], default="h50",
-                    help="h50: the headline N=250 lattice; df600: the precision-form measurement on the synthetic "
+    ap.add_argument("--workload", choices=["h50", "df600", "batch", "learned", "ugw", "cpu250"], default="h50",
+                    help="h50: the headline N=250 lattices; df600: the precision-form measurement on the synthetic "
                          "density-fitted system (N=600, o=60, naux=1800), FP64 against split precision; batch: "
                          "batched G0W0 over seeded H chains/lattices of uneven size; learned: the MLP warm start "
-                         "against plain evGW on held-out systems")
+                         "against plain evGW on held-out systems; cpu250: QuAcK's compiled path on the host cores "
+                         "at N=250 (AO->MO + one evGW cycle)")
     ap.add_argument("--nmol", type=int, default=64, help="batch and learned workloads: number of molecules")
     ap.add_argument("--kmax", type=int, default=12, help="batch workload: largest H_2k (N = 10k)")
     ap.add_argument("--chunk", type=int, default=256, help="batch workload: most molecules resident at once")
@@ -211,6 +344,20 @@ def main():
         return run_ugw(args)
     if args.workload == "learned":
         return run_learned(args)
+    if args.workload == "cpu250":
+        return run_cpu250(args)
+    return run_h50(args)
+
+
+def method_kw(args):
+    if args.reg == "srg":
+        return {"doSRG": True, "eta": 0.0}
+    return {"doSRG": False, "eta": args.eta if args.reg == "eta" else 0.0}
+
+
+def run_h50(args):
+    """The headline: --concurrent seeded H50 lattices (seeds seed .. seed+K-1),
+    one engine each, evGW at eta = 0.05 with DIIS 5 by default."""
     K = max(1, args.concurrent)
     # one hardware queue per engine stream: with four (HIP's default), a fourth
     # molecule's stream shares a queue with another (plus torch's) and waits
@@ -226,140 +373,175 @@ def main():
     import numpy as np
     import torch
 
-    from quack_amd import gw
+    from quack_amd import gw, intgen, learned, workloads
+    torch.cuda.init()  # torch's HIP runtime first, then the library binds to it
     gw.load()
-    mol, scf, eri, eri_t, c_t = build_inputs(args.nx, args.ny, args.seed)
-    nBas, nOrb, nO = mol.nBas, scf["nOrb"], mol.nO
-    engs = [gw.GWEngine(nBas, nOrb, nO) for _ in range(K)]
+    mk = method_kw(args)
+    evkw = {"maxSCF": 256, "thresh": 1e-6, "max_diis": args.max_diis, **mk}
+    mols, inputs = [], []
+    for k in range(K):
+        mol = intgen.h_lattice(args.nx, args.ny, seed=args.seed + k)
+        mols.append(mol)
+        inputs.append(workloads.device_inputs(mol, log=log))
+    nBas, nO = mols[0].nBas, mols[0].nO
+    engs = [gw.GWEngine(nBas, inp[0]["nOrb"], nO) for inp in inputs]
     eng = engs[0]
-    nS = eng.nS
-    reg = not args.no_reg
+    nOrb, nS = eng.nOrb, eng.nS
+    use_learned = args.learned == "on" and args.method == "evgw" and os.path.exists(learned.DEFAULT_MODEL)
+    if use_learned:
+        wsm = learned.WarmStart(learned.DEFAULT_MODEL)
+        if wsm.method["eta"] != mk["eta"] or wsm.method["doSRG"] != mk["doSRG"] or \
+                wsm.method["max_diis"] != args.max_diis:
+            log("learned model trained for %s, not this method: learned form off" % wsm.method)
+            use_learned = False
+        wsm.close()
+    wss = [learned.WarmStart(learned.DEFAULT_MODEL) for _ in range(K)] if use_learned else [None] * K
 
-    def step_on(g):
+    def step_on(i, learned_on=use_learned):
+        g, (scf, eri_t, c_t) = engs[i], inputs[i]
         g.set_eri_ao_device(eri_t.data_ptr(), c_t.data_ptr())
-        if args.method == "evgw":
-            return g.evgw(scf["eHF"], maxSCF=256, thresh=1e-6, max_diis=args.max_diis, doSRG=reg)
-        return g.g0w0(scf["eHF"], doSRG=reg)
+        if args.method == "g0w0":
+            return g.g0w0(scf["eHF"], **mk)
+        if learned_on:
+            return wss[i].evgw(g, {"eHF": scf["eHF"], "Sx": scf["Sx"], "nO": nO})
+        return g.evgw(scf["eHF"], **evkw)
 
     # plain FP64 form (dsyevd every cycle, one molecule at a time): the accuracy
-    # reference and the baseline rate
-    eng.set_precision("fp64")
-    eng.set_eigensolver("syevd")
-    step_on(eng)
-    torch.cuda.synchronize()
-    t = time.time()
-    ref = step_on(eng)
-    t_fp64 = time.time() - t
-    log("plain fp64 form: %.3f s/molecule%s" % (t_fp64, (" nSCF=%d" % ref["nSCF"]) if "nSCF" in ref else ""))
+    # reference of every molecule and, as the median of --baseline-reps runs on
+    # the first molecule, the baseline rate
+    for g in engs:
+        g.set_precision("fp64")
+        g.set_eigensolver("syevd")
+    step_on(0, False)  # warm-up of the FP64 shapes
+    refs, t_fp64s = [], []
+    for rep in range(max(1, args.baseline_reps)):
+        torch.cuda.synchronize()
+        t = time.time()
+        refs_0 = step_on(0, False)
+        t_fp64s.append(time.time() - t)
+    refs.append(refs_0)
+    for i in range(1, K):
+        refs.append(step_on(i, False))
+    t_fp64 = float(np.median(t_fp64s))
+    log("plain fp64 form: %s s/molecule (median %.3f)%s" % ([round(x, 3) for x in t_fp64s], t_fp64,
+                                                            (" nSCF=%s" % [r.get("nSCF") for r in refs])))
     # the same FP64 arithmetic with the project's own tridiagonalisation in every full solve
     eng.set_eigensolver("sytrd")
     t = time.time()
-    ref_own = step_on(eng)
+    ref_own = step_on(0, False)
     t_fp64_own = time.time() - t
     log("plain fp64 form, own tridiagonalisation: %.3f s/molecule, max |dE| vs dsyevd %.2e Ha" %
-        (t_fp64_own, float(np.abs(ref_own["eGW"] - ref["eGW"]).max())))
+        (t_fp64_own, float(np.abs(ref_own["eGW"] - refs[0]["eGW"]).max())))
     st0 = eng.stage_times()  # the plain-form launches, part of the fp64 kernel's population only
 
     for g in engs:
         g.set_precision(args.form)
         g.set_eigensolver(args.eig, args.de_max)
     # first call of every GEMM shape in this form: the tile autotuner's trials
-    step_on(eng)
-    # one molecule alone: per-stage device times and the dominant kernel's solo launch time
-    st_pre = eng.stage_times()
+    step_on(0, False)
+    # one molecule alone, without and with the learned start: per-stage device
+    # times, full-solve counts and the dominant kernels' solo launch times
+    st_a = eng.stage_times()
+    e_a = eng.eig_stats()
     t = time.time()
-    r1 = step_on(eng)
+    r_nolearn = step_on(0, False)
+    t_nolearn = time.time() - t
+    e_b = eng.eig_stats()
+    st_b = eng.stage_times()
+    t = time.time()
+    r1 = step_on(0)
     t_single = time.time() - t
     st1 = eng.stage_times()
-    log("%s/%s form, one molecule alone: %.3f s, stages %s, eig %s" %
-        (args.form, args.eig, t_single, {k: round(v, 1) for k, v in st1.items()}, eng.eig_stats()))
+    e_c = eng.eig_stats()
+    solo_stages = {k: st1[k] for k in ("aotomo", "rpa_build", "eigensolve", "excitation", "sigma_qp", "total")}
+    log("%s/%s form, one molecule alone: without the learned start %.3f s (%d cycles, %d full solves); with it "
+        "%.3f s (%s cycles, %d full solves); stages %s" %
+        (args.form, args.eig, t_nolearn, r_nolearn.get("nSCF", 1), e_b["syevd"] - e_a["syevd"], t_single,
+         r1.get("nSCF_total", r1.get("nSCF")), e_c["syevd"] - e_b["syevd"],
+         {k: round(v, 1) for k, v in solo_stages.items()}))
     pool = ThreadPoolExecutor(K)
     for w in range(max(0, args.warmup - 1)):
-        list(pool.map(step_on, engs))
+        list(pool.map(step_on, range(K)))
         log("warmup %d done" % (w + 1))
     torch.cuda.synchronize()
     c0 = [g.stage_times() for g in engs]
+    ec0 = [g.eig_stats() for g in engs]
     log("timed region starts, monotonic ns %d" % time.monotonic_ns())
     t0 = time.time()
     for k in range(args.steps):
-        rs = list(pool.map(step_on, engs))
+        rs = list(pool.map(step_on, range(K)))
         log("step %d done (%d molecules)" % (k, K))
     torch.cuda.synchronize()
     t1 = time.time()
     log("timed region ends, monotonic ns %d" % time.monotonic_ns())
     c1 = [g.stage_times() for g in engs]
+    ec1 = [g.eig_stats() for g in engs]
     pool.shutdown()
-    r = rs[0]
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = K * args.steps / (t1 - t0)
     mev = 1e-3 / 27.211386245988
-    dE = np.max([np.abs(x["eGW"] - ref["eGW"]) for x in rs], axis=0)
-    window = np.abs(scf["eHF"] - 0.5 * (scf["eHF"][nO - 1] + scf["eHF"][nO])) < 1.0
+    dEs = [np.abs(x["eGW"] - ref["eGW"]) for x, ref in zip(rs, refs)]
+    dE = np.max(dEs, axis=0)
+    eHF0 = inputs[0][0]["eHF"]
+    window = np.abs(eHF0 - 0.5 * (eHF0[nO - 1] + eHF0[nO])) < 1.0
+    full_timed = sum(b["syevd"] - a["syevd"] for a, b in zip(ec0, ec1))
+    refined_timed = sum(b["refined"] - a["refined"] for a, b in zip(ec0, ec1))
 
-    # the excitation-density GEMM rho^T = W^T M^T (the largest matrix-core kernel),
-    # one launch per screening cycle, timed with HIP events on each engine's own
-    # stream over the timed region (launches of concurrent molecules overlap, so
-    # each takes longer than alone; the solo figure is reported beside it).
+    # the excitation-density GEMM rho^T = W^T M^T (the largest matrix-core
+    # kernel), one launch per screening cycle: HIP events on the engine's own
+    # stream.  "solo" = one molecule alone (the kernel's own device time, the
+    # roofline figure); "concurrent" = averaged over the timed region, where the
+    # launches of the molecules in flight overlap and each takes longer.
     # FP64 form: 2 nOrb^2 nS^2 flop on FP64 MFMA; split forms: six (mixed) or
-    # three (mixed2) BF16 slice products, 6 or 3 x 2 nOrb^2 nS^2 flop on BF16 MFMA.
+    # three (mixed2) BF16 slice products, 6 or 3 x 2 nOrb^2 nS^2 flop.
     gemm_t = sum(b["excitation_gemm_all_calls"] - a["excitation_gemm_all_calls"] for a, b in zip(c0, c1))
     gemm_n = sum(b["excitation_gemm_launches_all_calls"] - a["excitation_gemm_launches_all_calls"] for a, b in zip(c0, c1))
     gemm_ms = gemm_t / max(1.0, gemm_n)
-    solo_ms = st1["excitation_gemm"] / max(1.0, st1["excitation_gemm_launches"])
-    learned = None
-    if args.learned == "auto" and args.method == "evgw" and reg:
-        learned = learned_form(eng, scf, nO, ref, args)
-    # every launch of this form's kernel in the process (tuning trials, warm-up,
-    # solo, timed and learned runs): the population of a rocprofv3 kernel
-    # summary of this command
-    tot = [g.stage_times() for g in engs]
-    sub = args.form != "fp64"
-    all_t = sum(x["excitation_gemm_all_calls"] for x in tot) - (st0["excitation_gemm_all_calls"] if sub else 0.0)
-    all_n = sum(x["excitation_gemm_launches_all_calls"] for x in tot) - \
-        (st0["excitation_gemm_launches_all_calls"] if sub else 0.0)
+    solo_ms = (st1["excitation_gemm_all_calls"] - st_a["excitation_gemm_all_calls"]) / \
+        max(1.0, st1["excitation_gemm_launches_all_calls"] - st_a["excitation_gemm_launches_all_calls"])
     if args.form == "fp64":
         flops, peak, kname = 2.0 * nOrb * nOrb * nS * nS, PEAK_FP64, "dgemm_mfma (excitation densities rho = M W)"
     else:
         npr = PRODUCTS[args.form]
         flops, peak, kname = 2.0 * npr * nOrb * nOrb * nS * nS, PEAK_BF16, \
             "k_gemm_bf16x3 (excitation densities, %d BF16 slice products)" % npr
-    achieved = flops / (gemm_ms * 1e-3) / 1e12
-    traffic = read_traffic(args.form)
-    gemm_roof = {"kernel": kname, "bound": "mfma",
-                 "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                 "traffic": traffic, "flops_per_launch": flops, "avg_launch_ms": gemm_ms,
-                 "launches_timed": int(gemm_n), "concurrent_molecules": K,
-                 "all_launches": {"n": int(all_n), "avg_launch_ms": all_t / max(1.0, all_n)},
-                 "solo": {"avg_launch_ms": solo_ms, "achieved": flops / (solo_ms * 1e-3) / 1e12,
-                          "frac": flops / (solo_ms * 1e-3) / 1e12 / peak}}
-    # the own Householder tridiagonalisation (two kernels per column plus the panel
-    # GEMMs; by device time the largest consumer of the mixed form): one "launch" is
-    # one tridiagonalisation of the nS x nS RPA matrix, timed with HIP events on each
-    # engine's stream.  Algorithmic bytes: the symmetric product of every column reads
-    # the trailing upper triangle once, 8 sum_{L<nS} L (L+1) / 2 bytes per launch.
+    achieved = flops / (solo_ms * 1e-3) / 1e12
+    gemm_roof = {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                 "frac": achieved / peak, "traffic": read_traffic(args.form), "flops_per_launch": flops,
+                 "avg_launch_ms": solo_ms, "timing": "solo: one molecule alone, HIP events on the engine's stream",
+                 "concurrent": {"avg_launch_ms": gemm_ms, "launches": int(gemm_n), "molecules_in_flight": K,
+                                "achieved": flops / (gemm_ms * 1e-3) / 1e12}}
+    # the own Householder tridiagonalisation (two kernels per column plus the
+    # panel GEMMs): one "launch" is one tridiagonalisation of the nS x nS RPA
+    # matrix.  Algorithmic bytes: the symmetric product of every column reads
+    # the trailing upper triangle once, 8 sum_{L<nS} L (L+1) / 2 bytes.
     trd_bytes = 8.0 * sum(L * (L + 1) / 2.0 for L in range(1, nS))
+    trd_solo_n = st1["tridiagonalisations_all_calls"] - st_a["tridiagonalisations_all_calls"]
     trd_t = sum(b["tridiagonalisation_all_calls"] - a["tridiagonalisation_all_calls"] for a, b in zip(c0, c1))
     trd_n = sum(b["tridiagonalisations_all_calls"] - a["tridiagonalisations_all_calls"] for a, b in zip(c0, c1))
-    trd_solo_n = st1["tridiagonalisations_all_calls"] - st_pre["tridiagonalisations_all_calls"]
     trd_roof = None
-    if trd_n > 0:
-        trd_ms = trd_t / trd_n
-        trd_gbs = trd_bytes / (trd_ms * 1e-3) / 1e9
+    if trd_solo_n > 0:
+        solo_trd = (st1["tridiagonalisation_all_calls"] - st_a["tridiagonalisation_all_calls"]) / trd_solo_n
+        gbs = trd_bytes / (solo_trd * 1e-3) / 1e9
         trd_roof = {"kernel": "own tridiagonalisation (k_symv_tiles + k_finish_update per column, panel GEMMs)",
-                    "bound": "hbm", "achieved": trd_gbs, "peak": PEAK_HBM, "unit": "GB/s", "frac": trd_gbs / PEAK_HBM,
-                    "traffic": read_trd_traffic(), "bytes_per_launch": trd_bytes, "avg_launch_ms": trd_ms,
-                    "launches_timed": int(trd_n), "concurrent_molecules": K}
-        if trd_solo_n > 0:
-            solo_trd = (st1["tridiagonalisation_all_calls"] - st_pre["tridiagonalisation_all_calls"]) / trd_solo_n
-            trd_roof["solo"] = {"avg_launch_ms": solo_trd, "achieved": trd_bytes / (solo_trd * 1e-3) / 1e9,
-                                "frac": trd_bytes / (solo_trd * 1e-3) / 1e9 / PEAK_HBM}
+                    "bound": "hbm", "achieved": gbs, "peak": PEAK_HBM, "unit": "GB/s", "frac": gbs / PEAK_HBM,
+                    "traffic": read_trd_traffic(), "bytes_per_launch": trd_bytes, "avg_launch_ms": solo_trd,
+                    "timing": "solo: one molecule alone, HIP events on the engine's stream"}
+        if trd_n > 0:
+            trd_roof["concurrent"] = {"avg_launch_ms": trd_t / trd_n, "launches": int(trd_n),
+                                      "molecules_in_flight": K}
+    # the dominant kernel by device time of one molecule alone
+    dom = trd_roof if (trd_roof is not None and trd_solo_n * trd_roof["avg_launch_ms"] >
+                       (st1["excitation_gemm_launches_all_calls"] - st_a["excitation_gemm_launches_all_calls"]) *
+                       solo_ms) else gemm_roof
 
     cpu = None
     if args.cpu_baseline == "auto":
         try:
-            cpu = cpu_baseline(args.method, args.max_diis, reg)
+            cpu = cpu_baseline(args.method, args.max_diis, mk)
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline failed: %r" % (e,))
+    tag = "SRG-" if mk["doSRG"] else ("eta=%g " % mk["eta"])
     out = {
         "metric": "G0W0/evGW molecules/sec at N=250, mixed+learned vs FP64 GPU at 1 meV QP error",
         "value": value,
@@ -372,37 +554,56 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": DTYPE[args.form],
-        "data": "synthetic: seeded H50 lattice, integrals from the project's own Gaussian-integral producer",
-        "config": {"workload": "H50 5x10 lattice cc-pVDZ %s%s, %s form, %s eigensolver, %d molecules in flight" %
-                               ("SRG-" if reg else "", args.method, args.form, args.eig, K),
-                   "nBas": nBas, "nOrb": nOrb, "nO": nO, "nS": nS, "method": args.method,
-                   "thresh": 1e-6 if args.method == "evgw" else None, "max_diis": args.max_diis, "eta": 0.0,
-                   "srg_flow": 500.0 if reg else None, "cycles_per_molecule": r.get("nSCF", 1), "form": args.form,
-                   "eigensolver": args.eig, "refine_de_max": args.de_max, "molecules_in_flight": K,
-                   "hip_hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+        "data": "synthetic: %d seeded H50 lattices, integrals from the project's own Gaussian-integral producer" % K,
+        "config": {"workload": "H50 5x10 lattices (seeds %d..%d) cc-pVDZ %s%s DIIS %d, %s form, %s eigensolver%s, "
+                               "%d molecules in flight" %
+                               (args.seed, args.seed + K - 1, tag, args.method, args.max_diis, args.form, args.eig,
+                                ", learned warm start" if use_learned else "", K),
+                   "nBas": nBas, "nOrb": [e.nOrb for e in engs], "nO": nO, "nS": [e.nS for e in engs],
+                   "method": args.method, "thresh": 1e-6 if args.method == "evgw" else None,
+                   "max_diis": args.max_diis, "eta": mk["eta"], "srg_flow": 500.0 if mk["doSRG"] else None,
+                   "cycles_per_molecule": [x.get("nSCF_total", x.get("nSCF", 1)) for x in rs],
+                   "plain_cycles_per_molecule": [x.get("nSCF", 1) for x in refs], "form": args.form,
+                   "eigensolver": args.eig, "refine_de_max": args.de_max, "learned": use_learned,
+                   "molecules_in_flight": K, "hip_hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+                   "full_rpa_solves_per_molecule_timed": full_timed / (K * args.steps),
+                   "refined_rpa_solves_per_molecule_timed": refined_timed / (K * args.steps),
                    "step": "%d molecules" % K},
-        "plain_fp64_form": {"molecules_per_s": 1.0 / t_fp64, "cycles": ref.get("nSCF"),
-                            "eigensolver": "dsyevd every cycle", "speedup_of_this_form": value * t_fp64},
+        "plain_fp64_form": {"molecules_per_s": 1.0 / t_fp64, "s_per_molecule_runs": t_fp64s,
+                            "statistic": "median of %d runs, one molecule at a time" % len(t_fp64s),
+                            "cycles": refs[0].get("nSCF"), "eigensolver": "dsyevd every cycle",
+                            "speedup_of_this_form": value * t_fp64},
         "plain_fp64_form_own_tridiagonal": {"molecules_per_s": 1.0 / t_fp64_own,
                                             "eigensolver": "own tridiagonalisation every cycle",
-                                            "max_abs_dE_vs_dsyevd_Ha": float(np.abs(ref_own["eGW"] - ref["eGW"]).max()),
+                                            "max_abs_dE_vs_dsyevd_Ha": float(np.abs(ref_own["eGW"] - refs[0]["eGW"]).max()),
                                             "speedup_of_this_form": value * t_fp64_own},
         "this_form_one_molecule_alone": {"s": t_single, "speedup_vs_plain_fp64": t_fp64 / t_single,
-                                         "stages_ms": st1, "eig_stats": eng.eig_stats()},
+                                         "without_learned_start_s": t_nolearn,
+                                         "cycles": r1.get("nSCF_total", r1.get("nSCF")),
+                                         "cycles_without_learned_start": r_nolearn.get("nSCF"),
+                                         "full_rpa_solves": e_c["syevd"] - e_b["syevd"],
+                                         "full_rpa_solves_without_learned_start": e_b["syevd"] - e_a["syevd"],
+                                         "stages_ms": {k: st1[k] - st_b[k] for k in solo_stages}},
         "accuracy_vs_fp64": {"max_abs_dE_meV_all": float(dE.max() / mev),
-                             "orbitals_over_1meV": int((dE > mev).sum()),
+                             "orbitals_over_1meV": int(sum((d > mev).sum() for d in dEs)),
+                             "orbitals": int(sum(len(d) for d in dEs)),
                              "max_abs_dE_meV_within_1Ha_of_gap": float(dE[window].max() / mev),
-                             "HOMO_dE_meV": float(dE[nO - 1] / mev), "LUMO_dE_meV": float(dE[nO] / mev)},
-        "learned_form": learned,
-        "roofline": trd_roof if trd_roof is not None else gemm_roof,
+                             "HOMO_dE_meV": float(dE[nO - 1] / mev), "LUMO_dE_meV": float(dE[nO] / mev),
+                             "learned_start_max_err_meV": (float(np.abs(r1["start"] - refs[0]["eGW"]).max() / mev)
+                                                           if "start" in r1 else None)},
+        "roofline": dom,
         "roofline_excitation_gemm": gemm_roof,
+        "roofline_tridiagonalisation": trd_roof,
         "cpu_baseline": cpu,
-        "result": {"HOMO": float(r["eGW"][nO - 1]), "LUMO": float(r["eGW"][nO]), "EcRPA": r["EcRPA"],
-                   "nSCF": r.get("nSCF")},
+        "result": {"HOMO": float(rs[0]["eGW"][nO - 1]), "LUMO": float(rs[0]["eGW"][nO]), "EcRPA": rs[0]["EcRPA"],
+                   "nSCF": rs[0].get("nSCF")},
     }
     print(json.dumps(out), flush=True)
     for g in engs:
         g.close()
+    for w in wss:
+        if w is not None:
+            w.close()
 
 
 def run_df600(args):
@@ -566,35 +767,38 @@ def run_batch(args):
 
 
 def run_learned(args):
-    """Learned form on held-out systems (seeded H chains and lattices, N =
-    20-100, drawn with a seed the training never used): plain SRG-evGW and the
-    learned warm start on each, inputs produced on the device.  Reports the
-    evGW cycles saved, the wall time and the QP errors against plain evGW."""
+    """Learned form on held-out systems (seeded H chains and planar lattices,
+    N = 20-250, drawn with a seed the training never used): plain evGW and the
+    learned warm start on each (the model's method: eta = 0.05, DIIS 5, FP64
+    with refined and own full RPA solves), inputs produced on the device.
+    Reports full RPA diagonalisations and evGW cycles saved, wall time, and
+    the QP errors against plain evGW."""
     import numpy as np
     import torch
 
     from quack_amd import gw, learned
     torch.cuda.init()  # torch's HIP runtime first, then the library binds to it
     gw.load()
-    rows = learned.evaluate_device(learned.DEFAULT_MODEL, nsys=args.nmol, seed=12345 + args.seed, log=log)
+    rows = learned.evaluate_device(learned.DEFAULT_MODEL, nsys=args.nmol, seed=12345 + args.seed, log=log,
+                                   precision=args.form, eig=args.eig)
     t_plain = sum(r["t_plain"] for r in rows)
-    t_warm = sum(r["t_warm"] for r in rows)
-    cyc_plain = sum(r["plain_cycles"] for r in rows)
-    cyc_warm = sum(r["warm_cycles"] for r in rows)
+    t_warm = sum(r["t_learned"] for r in rows)
     err = np.array([r["max_err_meV"] for r in rows])
-    win = np.array([r["max_err_meV_1Ha_window"] for r in rows])
     hl = np.array([r["homo_lumo_err_meV"] for r in rows])
-    out = {"metric": "learned warm start: held-out evGW molecules/sec (plain SRG-evGW vs MLP warm start)",
+    out = {"metric": "learned warm start: held-out evGW molecules/sec (plain evGW vs MLP warm start)",
            "value": len(rows) / t_warm, "unit": "molecules/s", "n_gpus": 1, "steps": 1, "higher_is_better": True,
-           "dtype": "f64 (evGW), f32/bf16 (MLP)", "data": "synthetic: seeded held-out H chains and lattices",
-           "config": {"workload": "%d held-out seeded H chains/lattices, cc-pVDZ, SRG-evGW to 1e-6 Ha, DIIS off"
-                                  % len(rows), "nBas_range": [min(r["nBas"] for r in rows), max(r["nBas"] for r in rows)],
+           "dtype": "f64 (evGW), f32/bf16 (MLP)", "data": "synthetic: seeded held-out H chains and planar lattices",
+           "config": {"workload": "%d held-out seeded H chains/lattices, cc-pVDZ, evGW eta=0.05 DIIS 5 to 1e-6 Ha, "
+                                  "%s form, %s eigensolver" % (len(rows), args.form, args.eig),
+                      "nBas_range": [min(r["nBas"] for r in rows), max(r["nBas"] for r in rows)],
                       "model": os.path.relpath(learned.DEFAULT_MODEL, ROOT), "seed": 12345 + args.seed},
            "plain_molecules_per_s": len(rows) / t_plain, "speedup_vs_plain": t_plain / t_warm,
-           "evgw_cycles": {"plain": cyc_plain, "learned_incl_first_cycle": cyc_warm, "saved": cyc_plain - cyc_warm},
+           "full_rpa_solves": {"plain": sum(r["plain_full_solves"] for r in rows),
+                               "learned": sum(r["learned_full_solves"] for r in rows)},
+           "evgw_cycles": {"plain": sum(r["plain_cycles"] for r in rows),
+                           "learned_incl_first_cycle": sum(r["learned_cycles"] for r in rows)},
            "accuracy_vs_plain_evgw": {"max_abs_dE_meV_all": float(err.max()),
                                       "molecules_all_orbitals_within_1meV": int((err <= 1.0).sum()),
-                                      "max_abs_dE_meV_within_1Ha_of_gap": float(win.max()),
                                       "max_HOMO_LUMO_dE_meV": float(hl.max()),
                                       "orbitals_over_1meV": int(sum(r["n_over_1meV"] for r in rows)),
                                       "orbitals": int(sum(r["n_orbitals"] for r in rows))},
@@ -630,31 +834,6 @@ def read_traffic(form):
             except ValueError:
                 return None
     return None
-
-
-def learned_form(eng, scf, nO, ref, args):
-    """Learned warm start (quack_amd/learned.py) on the same molecule, one run."""
-    import numpy as np
-
-    from quack_amd import learned
-    if not os.path.exists(learned.DEFAULT_MODEL):
-        return None
-    ws = learned.WarmStart(learned.DEFAULT_MODEL)
-    sysd = {"eHF": scf["eHF"], "Sx": scf["Sx"], "nO": nO}
-    eng.set_eigensolver(args.eig, args.de_max)
-    t = time.time()
-    r = ws.evgw(eng, sysd, doSRG=True, thresh=1e-6)
-    dt = time.time() - t
-    mev = 1e-3 / 27.211386245988
-    dE = np.abs(r["eGW"] - ref["eGW"])
-    window = np.abs(scf["eHF"] - 0.5 * (scf["eHF"][nO - 1] + scf["eHF"][nO])) < 1.0
-    out = {"s_per_molecule": dt, "cycles": int(r["nSCF"]) + 1, "plain_cycles": ref.get("nSCF"),
-           "max_abs_dE_meV_all": float(dE.max() / mev), "orbitals_over_1meV": int((dE > mev).sum()),
-           "max_abs_dE_meV_within_1Ha_of_gap": float(dE[window].max() / mev),
-           "HOMO_dE_meV": float(dE[nO - 1] / mev), "LUMO_dE_meV": float(dE[nO] / mev)}
-    log("learned form: %s" % out)
-    return out
-
 
 
 UGW_SAMPLE = r"""

@@ -31,7 +31,10 @@ enum {
     QK_E_EIG = 3,          /* eigensolver failure (rocSOLVER info != 0) */
     QK_E_NOCONV = 4,       /* evGW reached maxSCF+1 cycles (GW/evRGW.f90:190-199) */
     QK_E_UNSUPPORTED = 5,  /* option not implemented on the GPU path */
-    QK_E_ALLOC = 6         /* device memory exhausted */
+    QK_E_ALLOC = 6,        /* device memory exhausted */
+    QK_E_UNSTABLE = 7      /* A-B not positive definite: the linear-response instability
+                              QuAcK warns about (LR/phRLR.f90:57-58) before it takes
+                              square roots of the negative modes */
 };
 
 typedef struct qk_gw_ctx *qk_gw_handle;
@@ -176,7 +179,9 @@ int qk_gw_stage_times(qk_gw_handle h, double *ms, int n);
 int qk_aotomo_eri_rhf(int nBas, int nOrb, const double *c, const double *ERI_AO, double *ERI_MO);
 
 /* LR/phRLR.f90:1-97 for general Aph, Bph (nS x nS): EcRPA, Om (ascending),
- * XpY, XmY.  TDA != 0 uses Aph only (phRLR.f90:38-43). */
+ * XpY, XmY.  TDA != 0 uses Aph only (phRLR.f90:38-43).  Returns QK_E_UNSTABLE,
+ * outputs untouched, when the smallest eigenvalue of A - B is not positive
+ * (QuAcK prints its warning there and goes on with NaN square roots). */
 int qk_phrlr(int TDA, int nS, const double *Aph, const double *Bph, double *EcRPA, double *Om, double *XpY,
              double *XmY);
 

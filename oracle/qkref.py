@@ -60,6 +60,15 @@ def _f(a):
     return np.asfortranarray(a, dtype=np.float64)
 
 
+def _eri(a):
+    """A four-index array for the Fortran side: a flat array is taken as
+    QuAcK's column-major data as it is; an (n, n, n, n) array indexed
+    [p, q, r, s] is laid out column-major, so Fortran reads ERI(p, q, r, s)
+    (a C-order buffer would reach it as ERI(s, r, q, p), which differs for the
+    mixed-spin block ERI_aabb)."""
+    return np.asfortranarray(a, dtype=np.float64)
+
+
 def _dims(nOrb, nO, nC=0, nR=0):
     nV = nOrb - nO
     nS = (nO - nC) * (nV - nR)
@@ -70,7 +79,7 @@ def aotomo(c, eri_ao):
     nBas, nOrb = c.shape
     c = _f(c)
     out = np.zeros(nOrb ** 4)
-    load().qkref_aotomo(nBas, nOrb, _p(c), _p(np.ascontiguousarray(eri_ao)), _p(out))
+    load().qkref_aotomo(nBas, nOrb, _p(c), _p(_eri(eri_ao)), _p(out))
     return out
 
 
@@ -81,7 +90,7 @@ def phrlr(e, eri_mo, nO, tda=False, nC=0, nR=0):
     Om = np.zeros(nS)
     XpY = np.zeros((nS, nS), order="F")
     XmY = np.zeros((nS, nS), order="F")
-    load().qkref_phrlr(int(tda), nOrb, nC, nO, nV, nR, nS, _p(_f(e)), _p(np.ascontiguousarray(eri_mo)),
+    load().qkref_phrlr(int(tda), nOrb, nC, nO, nV, nR, nS, _p(_f(e)), _p(_eri(eri_mo)),
                        _p(Ec), _p(Om), _p(XpY), _p(XmY))
     return {"EcRPA": Ec[0], "Om": Om, "XpY": XpY, "XmY": XmY}
 
@@ -89,7 +98,7 @@ def phrlr(e, eri_mo, nO, tda=False, nC=0, nR=0):
 def excitation_density(eri_mo, XpY, nOrb, nO, nC=0, nR=0):
     nC, nO, nV, nR, nS = _dims(nOrb, nO, nC, nR)
     rho = np.zeros((nOrb, nOrb, nS), order="F")
-    load().qkref_excitation_density(nOrb, nC, nO, nR, nS, _p(np.ascontiguousarray(eri_mo)), _p(_f(XpY)), _p(rho))
+    load().qkref_excitation_density(nOrb, nC, nO, nR, nS, _p(_eri(eri_mo)), _p(_f(XpY)), _p(rho))
     return rho
 
 
@@ -120,7 +129,7 @@ def rg0w0(eri_mo, eHF, nO, eta=0.0, linearize=False, tda_w=False, srg=False, ENu
     eGW, Sig, Z = np.zeros(nOrb), np.zeros(nOrb), np.zeros(nOrb)
     Om, Ec, Egm = np.zeros(nS), np.zeros(1), np.zeros(1)
     load().qkref_rg0w0(int(tda_w), int(linearize), int(srg), eta, nOrb, nC, nO, nV, nR, nS, ENuc, ERHF,
-                       _p(np.ascontiguousarray(eri_mo)), _p(_f(eHF)), _p(eGW), _p(Sig), _p(Z), _p(Om),
+                       _p(_eri(eri_mo)), _p(_f(eHF)), _p(eGW), _p(Sig), _p(Z), _p(Om),
                        _p(Ec), _p(Egm))
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": Ec[0], "EcGM": Egm[0]}
 
@@ -133,7 +142,7 @@ def evrgw(eri_mo, eHF, nO, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, lineari
     Ec, Egm = np.zeros(1), np.zeros(1)
     nscf = ctypes.c_int(0)
     st = load().qkref_evrgw(int(tda_w), int(linearize), int(srg), maxSCF, thresh, max_diis, eta, nOrb, nC, nO, nV,
-                            nR, nS, ENuc, ERHF, _p(np.ascontiguousarray(eri_mo)), _p(_f(eHF)), _p(eGW), _p(Sig),
+                            nR, nS, ENuc, ERHF, _p(_eri(eri_mo)), _p(_f(eHF)), _p(eGW), _p(Sig),
                             _p(Z), _p(Ec), _p(Egm), ctypes.byref(nscf))
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "EcRPA": Ec[0], "EcGM": Egm[0], "nSCF": nscf.value, "status": st}
 
@@ -146,7 +155,7 @@ def phbse(eri_mo, eW, eGW, nO, eta=0.0, tda_w=False, tda=False, singlet=True, tr
     Ec = np.zeros(2)
     OmS, OmT = np.zeros(nS), np.zeros(nS)
     load().qkref_phbse(int(tda_w), int(tda), int(singlet), int(triplet), eta, nOrb, nC, nO, nV, nR, nS,
-                       _p(np.ascontiguousarray(eri_mo)), _p(_f(eW)), _p(_f(eGW)), _p(Ec), _p(OmS), _p(OmT))
+                       _p(_eri(eri_mo)), _p(_f(eW)), _p(_f(eGW)), _p(Ec), _p(OmS), _p(OmT))
     return {"EcBSE": Ec, "OmS": OmS, "OmT": OmT}
 
 
@@ -170,7 +179,7 @@ def qsgw_cycle(S, T, V, X, c, P, eGW, eri_ao, nO, eta=0.0, srg=False, tda_w=Fals
     F = np.zeros((n, n), order="F")
     e = np.zeros(n)
     load().qkref_qsgw_cycle(int(srg), int(tda_w), eta, n, nC, nO, nV, nR, nS, _p(_f(S)), _p(_f(T)), _p(_f(V)),
-                            _p(_f(X)), _p(_f(c)), _p(_f(P)), _p(_f(eGW)), _p(np.ascontiguousarray(eri_ao)), _p(F),
+                            _p(_f(X)), _p(_f(c)), _p(_f(P)), _p(_f(eGW)), _p(_eri(eri_ao)), _p(F),
                             _p(e))
     return {"F": F, "eGW": e}
 
@@ -183,8 +192,8 @@ def ug0w0(eri_aa, eri_ab, eri_bb, eHF, nO, eta=0.0, linearize=False, tda_w=False
     Ec = np.zeros(1)
     Egm = np.zeros(2)
     load().qkref_ug0w0(int(linearize), int(tda_w), int(srg), eta, n, nO[0], nO[1],
-                       _p(np.ascontiguousarray(eri_aa)), _p(np.ascontiguousarray(eri_ab)),
-                       _p(np.ascontiguousarray(eri_bb)), _p(_f(eHF)), _p(eGW), _p(Sig), _p(Z), _p(Ec), _p(Egm))
+                       _p(_eri(eri_aa)), _p(_eri(eri_ab)),
+                       _p(_eri(eri_bb)), _p(_f(eHF)), _p(eGW), _p(Sig), _p(Z), _p(Ec), _p(Egm))
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "EcRPA": Ec[0], "EcGM": Egm}
 
 
@@ -197,8 +206,8 @@ def evugw(eri_aa, eri_ab, eri_bb, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, e
     Ec = np.zeros(1)
     Egm = np.zeros(2)
     nscf = ctypes.c_int(0)
-    load().qkref_evugw(int(maxSCF), thresh, int(max_diis), int(linearize), int(tda_w), int(srg), eta, n, nO[0], nO[1], _p(np.ascontiguousarray(eri_aa)),
-                       _p(np.ascontiguousarray(eri_ab)), _p(np.ascontiguousarray(eri_bb)), _p(_f(eHF)), _p(eGW),
+    load().qkref_evugw(int(maxSCF), thresh, int(max_diis), int(linearize), int(tda_w), int(srg), eta, n, nO[0], nO[1], _p(_eri(eri_aa)),
+                       _p(_eri(eri_ab)), _p(_eri(eri_bb)), _p(_f(eHF)), _p(eGW),
                        _p(Sig), _p(Z), _p(Ec), _p(Egm), ctypes.byref(nscf))
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "EcRPA": Ec[0], "EcGM": Egm, "nSCF": nscf.value}
 

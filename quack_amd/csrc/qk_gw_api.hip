@@ -1566,6 +1566,15 @@ int qk_phrlr(int TDA, int nS, const double* Aph, const double* Bph, double* EcRP
                 QK_HIP(hipMemcpyAsync(B, Bph, sizeof(double) * S2, hipMemcpyHostToDevice, s));
                 apb_amb(s, A, B, ApB, U, (long long)S2);
                 syevd(s, rb, true, nS, U, w, work, info);
+                // LR/phRLR.f90:57-58: QuAcK warns when A - B has a negative
+                // eigenvalue and then takes its square root; stop before the
+                // NaN (or the infinite inverse root of a zero mode) spreads
+                double wmin = 0.0;
+                QK_HIP(hipMemcpyAsync(&wmin, w, sizeof(double), hipMemcpyDeviceToHost, s));
+                QK_HIP(hipStreamSynchronize(s));
+                QK_CHECK(wmin > 0.0, QK_E_UNSTABLE,
+                         "phRLR: A-B is not positive definite (smallest eigenvalue " + std::to_string(wmin) +
+                             "): linear-response instability");
                 colscale_pow(s, U, w, T, nS, 0.5);
                 dgemm(s, false, true, nS, nS, nS, 1.0, T, nS, U, nS, 0.0, Sq, nS);   // ADAt(U, sqrt(w))
                 colscale_pow(s, U, w, T, nS, -0.5);

@@ -6,6 +6,11 @@
 !   input/methods    the "# G0W0 evGW qsGW scGW" line and the "# Rtest ..." line
 !                    (src/QuAcK/read_methods.f90:191-199)
 !   input/options    the "# GW:" line, 14 fields (src/QuAcK/read_options.f90:240-266)
+!   input/hpc_flags  HPC and GPU switches (src/QuAcK/read_hpc_flags.f90:1-39); the GW
+!                    drop-ins run only when the GPU switch (line 4) is T, as the
+!                    patched dispatch of src/GW/RGW.f90:85,102 and UGW.f90:82,114
+!                    does; otherwise the run belongs to QuAcK's CPU routines, which
+!                    this driver does not contain, and it stops with code 3
 !   input/molecule   "# nAt nEla nElb nCore nRyd" (PyDuck.py:137-145)
 !   int/ENuc.dat     nuclear repulsion (PyDuck.py:150-153)
 !   int/MO.dat       nOrb, ERHF and eHF(1:nOrb): the RHF reference QuAcK computes
@@ -23,6 +28,7 @@ program qk_gw_driver
   double precision :: thresh, eta, shift, ENuc, ERHF
   character(len=1) :: a1, a2, a3, a4, a5, a6, a7, a8
   logical :: lin, TDA_W, reg, TDA, singlet, triplet, dophBSE, dophBSE2, doppBSE, dBSE, dTDA
+  logical :: switch_hpc, use_gpu
   double precision, allocatable :: eHF(:), ERI(:,:,:,:), dip(:,:,:), eGW(:), eU(:,:), SU(:,:), cU(:,:,:)
   integer :: nOU(2), nCU(2), nVU(2), nRU(2), nSU(2)
 
@@ -71,6 +77,19 @@ program qk_gw_driver
     end if
   end do
   close(12)
+
+  ! GPU switch and the dispatch of the patched RGW/UGW: the options the GPU
+  ! drop-ins do not carry (phBSE2, ppBSE, dynamical BSE) stay on the CPU routines
+  call qk_read_hpc_flags(wd, switch_hpc, use_gpu)
+  if((doG0W0 .or. doevGW) .and. .not.(use_gpu .and. .not.(dophBSE2 .or. doppBSE .or. dBSE))) then
+    if(.not. use_gpu) then
+      write(*,*) 'qk_gw_driver: use_gpu is F in input/hpc_flags: G0W0/evGW belong to QuAcK''s CPU routines'
+    else
+      write(*,*) 'qk_gw_driver: phBSE2, ppBSE and dBSE belong to QuAcK''s CPU routines'
+    end if
+    write(*,*) '              (RG0W0/evRGW, UG0W0/evUGW), which this driver does not contain'
+    error stop 3
+  end if
 
   ! molecule
   open(unit=13, file=trim(wd)//'/input/molecule', status='old', action='read')
@@ -134,6 +153,27 @@ program qk_gw_driver
   if(dotest .and. doUHF) close(1232584)
 
 end program
+
+! The reader of src/QuAcK/read_hpc_flags.f90:1-39: a comment line, the HPC
+! switch, a comment line, the GPU switch; both F when the file is absent.
+subroutine qk_read_hpc_flags(wd, switch_hpc, use_gpu)
+  implicit none
+  character(len=*), intent(in) :: wd
+  logical, intent(out) :: switch_hpc, use_gpu
+  character(len=1) :: ans
+  integer :: ios
+  switch_hpc = .false.
+  use_gpu = .false.
+  open(unit=17, file=trim(wd)//'/input/hpc_flags', status='old', action='read', iostat=ios)
+  if(ios /= 0) return
+  read(17,*,iostat=ios)
+  read(17,*,iostat=ios) ans
+  if(ios == 0 .and. ans == 'T') switch_hpc = .true.
+  read(17,*,iostat=ios)
+  read(17,*,iostat=ios) ans
+  if(ios == 0 .and. ans == 'T') use_gpu = .true.
+  close(17)
+end subroutine
 
 ! QuAcK's test-dump format (src/test/dump_test_value.f90:19-20): the
 ! description on one line, the value as F20.15 on the next.

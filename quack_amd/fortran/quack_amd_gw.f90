@@ -17,6 +17,7 @@ module quack_amd_gw
   integer(c_int), parameter :: QK_E_NOCONV = 4
   integer(c_int), parameter :: QK_E_UNSUPPORTED = 5
   integer(c_int), parameter :: QK_E_ALLOC = 6
+  integer(c_int), parameter :: QK_E_UNSTABLE = 7
 
   interface
 

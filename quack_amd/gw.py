@@ -18,8 +18,9 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libquack_amd.so")
 
-QK_OK, QK_E_ARG, QK_E_HIP, QK_E_EIG, QK_E_NOCONV, QK_E_UNSUPPORTED, QK_E_ALLOC = range(7)
-_NAMES = {1: "QK_E_ARG", 2: "QK_E_HIP", 3: "QK_E_EIG", 4: "QK_E_NOCONV", 5: "QK_E_UNSUPPORTED", 6: "QK_E_ALLOC"}
+QK_OK, QK_E_ARG, QK_E_HIP, QK_E_EIG, QK_E_NOCONV, QK_E_UNSUPPORTED, QK_E_ALLOC, QK_E_UNSTABLE = range(8)
+_NAMES = {1: "QK_E_ARG", 2: "QK_E_HIP", 3: "QK_E_EIG", 4: "QK_E_NOCONV", 5: "QK_E_UNSUPPORTED", 6: "QK_E_ALLOC",
+          7: "QK_E_UNSTABLE"}
 
 EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_ao", "qk_gw_set_eri_ao_device",
            "qk_gw_set_eri_mo", "qk_rg0w0", "qk_evrgw", "qk_gw_stage_times", "qk_aotomo_eri_rhf", "qk_phrlr",

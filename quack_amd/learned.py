@@ -1,24 +1,43 @@
-"""Learned warm start of evGW: a per-orbital MLP trained on QP shifts that the
-GPU engine's own plain evGW produces, run before the evGW cycles.
+"""Learned warm start of evGW: a per-orbital MLP trained on the QP shifts that
+the batched plain evGW produces, run before the evGW cycles.
 
-The learned form runs plain evGW's first cycle unchanged (screening at eHF,
-self-energy, graphical QP: GW/evRGW.f90:123-154), giving eGW1.  Features of
-orbital p (14), from the RHF reference and that cycle's screening:
-  eHF_p, diagonal exchange Sigma_x(p), occupied flag,
-  Sigma_c and dSigma_c/dw at w = eHF_p + {-0.25, 0, +0.25} Ha (6),
-  HOMO-LUMO gap, nO/nOrb, the linearised shift Z(eHF_p) Sigma_c(eHF_p),
-  eHF_p - Fermi level, the first-cycle QP shift eGW1_p - eHF_p.
-Target: the converged evGW energy minus eGW1_p.  evGW then restarts from
-eGW1 + prediction, so the learned form's diagonalisations are 1 + its cycles.
-The network (quack_amd/csrc/qk_mlp.hip: 14 -> 128 -> 128 -> 128 -> 1, SiLU,
-FP32 weights) is trained on the device by the project's own forward/backward
-and Adam kernels and evaluated with the BF16 matrix-core kernel.  evGW then
-starts from the predicted energies (qk_evrgw's eGW_start) instead of eHF;
-plain evGW (eGW_start = NULL) stays the reference that judges its error.
+Method.  evGW with QuAcK's broadened self-energy (GW/RGW_self_energy_diag.f90
+with eta > 0, its exact derivative in the graphical Newton solve of
+GW/RGW_QP_graph.f90), eta = 0.05 Ha, DIIS 5 (QuAcK's default,
+GW/evRGW.f90:168-173), converged to 1e-6 Ha.  Its self-consistent solution is
+unique in the sense that matters here: evGW restarted a few mHa away from it
+returns to it for every orbital.  Neither QuAcK's unbroadened eta = 0 form
+(chaotic: 1e-12 Ha moves QP energies by eV) nor its SRG form has that: the SRG
+Newton solve uses a pseudo-derivative (GW/RGW_SRG_Re_dSigC.f90) and orbitals
+whose solve fails keep their previous energy (GW/RGW_QP_graph.f90:78-81), so
+their final values depend on the path and any warm start moves them.
 
-    python -m quack_amd.learned make-data --out data.npz --nsys 160
-    python -m quack_amd.learned train --data data.npz --out quack_amd/models/mlp_evgw_srg.npz
-    python -m quack_amd.learned eval --model quack_amd/models/mlp_evgw_srg.npz
+Flow of the learned form for one molecule:
+  1. plain evGW's first cycle at eHF (screening, excitation densities,
+     self-energy, graphical QP: GW/evRGW.f90:123-154) gives eGW1;
+  2. features of orbital p (14) from the RHF reference and that cycle's
+     screening: eHF_p, diagonal exchange Sigma_x(p), occupied flag, Sigma_c
+     and dSigma_c/dw at w = eHF_p + {-0.25, 0, +0.25} Ha (6), the HOMO-LUMO
+     gap, nO/nOrb, the linearised shift Z(eHF_p) Sigma_c(eHF_p), eHF_p - Fermi
+     level, the first-cycle shift eGW1_p - eHF_p;
+  3. the MLP (quack_amd/csrc/qk_mlp.hip: 14 -> 128 -> 128 -> 128 -> 1, SiLU,
+     FP32 weights, BF16 matrix-core inference) predicts e*_p - eGW1_p;
+  4. evGW restarts from eGW1 + prediction (qk_evrgw's eGW_start, which
+     replaces the eGW = eHF start of GW/evRGW.f90:110).  Its screenings
+     after the first are refined from the previous eigenvectors when the
+     energies moved little, so the learned form needs two full RPA
+     diagonalisations where plain evGW from eHF needs several.
+Plain evGW (eGW_start = NULL) stays the fallback and the judge of its error.
+
+Training data: seeded H chains and planar lattices of 4 to 50 atoms in
+cc-pVDZ (N = 20..250), AO integrals and RHF produced on the device, the
+targets from the batched plain evGW (qk_gwb_evgw, FP64), the features from a
+single engine's first cycle.  Training runs on the device with the library's
+own forward, backward and Adam kernels.
+
+    python -m quack_amd.learned make-data --out data.npz --nsys 1024
+    python -m quack_amd.learned train --data data.npz --out quack_amd/models/mlp_evgw_eta005.npz
+    python -m quack_amd.learned eval --nsys 64
 """
 import argparse
 import json
@@ -34,10 +53,14 @@ from . import intgen as ig
 OFFSETS = (-0.25, 0.0, 0.25)
 NFEAT = 14
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_MODEL = os.path.join(HERE, "models", "mlp_evgw_srg.npz")
+DEFAULT_MODEL = os.path.join(HERE, "models", "mlp_evgw_eta005.npz")
+# the evGW the model is trained for and judged against
+METHOD = {"eta": 0.05, "doSRG": False, "max_diis": 5, "thresh": 1e-6, "maxSCF": 256}
+# predictions larger than this (Ha) are not trusted: the start is eGW1 there
+CAP = 0.25
 
 
-def first_cycle(eng, sysd, doSRG=True, eta=0.0):
+def first_cycle(eng, sysd, doSRG=False, eta=0.05):
     """Plain evGW's first cycle (maxSCF = 0 runs exactly one): eGW1."""
     try:
         r = eng.evgw(sysd["eHF"], maxSCF=0, thresh=0.0, max_diis=1, eta=eta, doSRG=doSRG)
@@ -46,7 +69,7 @@ def first_cycle(eng, sysd, doSRG=True, eta=0.0):
     return r["eGW"]
 
 
-def features(eng, sysd, eGW1, doSRG=True, eta=0.0):
+def features(eng, sysd, eGW1, doSRG=False, eta=0.05):
     """(n x 14) features from the screening left by the first cycle (at eHF)."""
     e = np.asarray(sysd["eHF"], np.float64)
     n, nO = len(e), sysd["nO"]
@@ -64,50 +87,132 @@ def features(eng, sysd, eGW1, doSRG=True, eta=0.0):
     return X
 
 
-def random_system(rng):
-    """Seeded H chain or planar lattice, cc-pVDZ (N = 20..100)."""
+def random_molecule(rng):
+    """Seeded H chain or planar lattice, cc-pVDZ, 4..50 atoms (N = 20..250),
+    the atom count drawn log-uniformly."""
+    nat = int(round(np.exp(rng.uniform(np.log(4), np.log(50)))))
+    seed = int(rng.integers(1 << 30))
     if rng.random() < 0.5:
-        nat = int(rng.choice([4, 6, 8, 10, 12, 14, 16, 18, 20]))
-        return ig.h_chain(nat, spacing=rng.uniform(1.3, 1.9), jitter=rng.uniform(0.0, 0.15),
-                          seed=int(rng.integers(1 << 30)))
-    nx, ny = [(2, 2), (2, 3), (2, 4), (3, 4), (2, 5), (2, 6), (4, 4), (3, 6), (4, 5)][int(rng.integers(9))]
-    return ig.h_lattice(nx, ny, spacing=rng.uniform(1.5, 1.9), jitter=rng.uniform(0.0, 0.15),
-                        seed=int(rng.integers(1 << 30)))
+        nat += nat % 2
+        return ig.h_chain(min(nat, 50), spacing=rng.uniform(1.35, 1.9), jitter=rng.uniform(0.0, 0.15), seed=seed)
+    shapes = [(nx, ny) for nx in range(2, 6) for ny in range(nx, 11) if 4 <= nx * ny <= 50 and nx * ny % 2 == 0]
+    nx, ny = min(shapes, key=lambda s: (abs(s[0] * s[1] - nat), rng.random()))
+    return ig.h_lattice(nx, ny, spacing=rng.uniform(1.5, 1.9), jitter=rng.uniform(0.0, 0.15), seed=seed)
 
 
-def plain_and_features(mol, doSRG=True, thresh=1e-6):
-    s = ig.system(mol)
-    with gw.GWEngine(s["nBas"], s["nOrb"], s["nO"]) as eng:
-        eng.set_eri_ao(s["eri"], s["c"])
-        e1 = first_cycle(eng, s, doSRG=doSRG)
-        X = features(eng, s, e1, doSRG=doSRG)
-        r = eng.evgw(s["eHF"], maxSCF=128, thresh=thresh, max_diis=1, doSRG=doSRG)
-    y = r["eGW"] - e1
-    return s, X, y, r
+def device_systems(mols, log=None, chunk_gb=64.0):
+    """Yields lists of prepared systems of one basis size whose AO integrals
+    sit in one contiguous device tensor (at most chunk_gb GB): each system has
+    eHF, c (host), Sx, nOrb, nO, nBas, the tensor and its index there.  RHFs
+    that do not converge in 600 iterations are left out."""
+    import torch
+
+    from . import scf as qscf
+    by_size = {}
+    for m in mols:
+        by_size.setdefault(m.nBas, []).append(m)
+    for nBas, ms in sorted(by_size.items()):
+        per = max(1, int(chunk_gb * 1e9 // (8 * nBas ** 4)))
+        for k0 in range(0, len(ms), per):
+            part = ms[k0:k0 + per]
+            eri = torch.empty((len(part), nBas ** 4), dtype=torch.float64, device="cuda")
+            out = []
+            with qscf.JK(nBas) as jkeng:
+                for j, mol in enumerate(part):
+                    mol.eri_device(eri[j].data_ptr(), nBas ** 4)
+                    S, T, V = mol.one_electron()
+                    ptr = eri[j].data_ptr()
+
+                    def jk(_eri, D, ptr=ptr):
+                        J, K = jkeng(ptr, D)
+                        return J, -K
+                    try:
+                        s = ig.rhf(S, T, V, None, mol.nO, maxit=600, jk=jk)
+                    except gw.QKError:
+                        raise
+                    except RuntimeError:
+                        if log:
+                            log("RHF not converged: molecule left out (nBas %d)" % nBas)
+                        continue
+                    s.update({"nO": mol.nO, "nBas": nBas, "eri": eri, "slot": j})
+                    out.append(s)
+            yield out
+            del eri
 
 
-def make_data(out, nsys, seed=0, doSRG=True):
+def plain_targets_batched(systems, **method):
+    """Plain evGW (FP64) of every system through batched handles: systems of
+    one (nBas, nOrb, nO) whose slots are equally strided share one handle.
+    Returns {index in systems: result}."""
+    import torch
+    res = {}
+    groups = {}
+    for i, s in enumerate(systems):
+        groups.setdefault((s["nBas"], s["nOrb"], s["nO"]), []).append(i)
+    for (nBas, nOrb, nO), idx in groups.items():
+        slots = [systems[i]["slot"] for i in idx]
+        runs = [idx] if all(b - a == slots[1] - slots[0] for a, b in zip(slots, slots[1:])) and len(idx) > 1 \
+            else [[i] for i in idx]
+        for run in runs:
+            s0 = systems[run[0]]
+            step = (systems[run[1]]["slot"] - s0["slot"]) if len(run) > 1 else 1
+            c = torch.from_numpy(np.stack([np.asfortranarray(systems[i]["c"]).ravel(order="F") for i in run])).cuda()
+            with gw.GWBatch(len(run), nBas, nOrb, nO) as b:
+                b.set_precision("fp64")
+                b.set_eri_ao_device(s0["eri"][s0["slot"]].data_ptr(), step * nBas ** 4, c.data_ptr(), c.shape[1])
+                try:
+                    r = b.evgw(np.stack([systems[i]["eHF"] for i in run]), **method)
+                except gw.NotConverged as ex:
+                    r = ex.result
+                    r["failed"] = True
+            for j, i in enumerate(run):
+                res[i] = {"eGW": r["eGW"][j], "nSCF": int(r["nSCF"][j]),
+                          "converged": not r.get("failed") or int(r["nSCF"][j]) <= method.get("maxSCF", 256)}
+    return res
+
+
+def make_data_device(out, nsys, seed=0, log=None, t_max=1e9):
+    """Training data: nsys seeded molecules (fewer if t_max seconds pass),
+    targets from the batched plain evGW, features from each molecule's first
+    cycle.  Saves X (n_orbitals x 14), y (e* - eGW1) and per-molecule meta;
+    rewritten after every chunk, so a cut-off run keeps what it finished."""
     rng = np.random.default_rng(seed)
+    mols = [random_molecule(rng) for _ in range(nsys)]
     Xs, ys, meta = [], [], []
     t0 = time.time()
-    while len(meta) < nsys:
-        mol = random_system(rng)
-        try:
-            s, X, y, r = plain_and_features(mol, doSRG=doSRG)
-        except (gw.NotConverged, gw.QKError, RuntimeError) as ex:
-            print("skip system:", ex, flush=True)
+    kw = {k: METHOD[k] for k in ("eta", "doSRG", "max_diis", "thresh", "maxSCF")}
+    for systems in device_systems(mols, log=log):
+        if not systems:
             continue
-        Xs.append(X)
-        ys.append(y)
-        meta.append((s["nOrb"], r["nSCF"]))
-        if len(meta) % 10 == 0:
-            print("data: %d systems, %d orbitals, %.0f s" % (len(meta), sum(m[0] for m in meta), time.time() - t0),
-                  flush=True)
-    np.savez(out, X=np.concatenate(Xs), y=np.concatenate(ys), meta=np.array(meta))
+        tgt = plain_targets_batched(systems, **kw)
+        engines = {}
+        for i, s in enumerate(systems):
+            if not tgt[i]["converged"]:
+                continue
+            key = (s["nBas"], s["nOrb"], s["nO"])
+            if key not in engines:
+                engines[key] = gw.GWEngine(*key)
+            eng = engines[key]
+            import torch
+            c_t = torch.from_numpy(np.asfortranarray(s["c"]).ravel(order="F").copy()).cuda()
+            eng.set_eri_ao_device(s["eri"][s["slot"]].data_ptr(), c_t.data_ptr())
+            e1 = first_cycle(eng, s, doSRG=kw["doSRG"], eta=kw["eta"])
+            X = features(eng, s, e1, doSRG=kw["doSRG"], eta=kw["eta"])
+            Xs.append(X)
+            ys.append(tgt[i]["eGW"] - e1)
+            meta.append((s["nBas"], s["nOrb"], s["nO"], tgt[i]["nSCF"]))
+        for e in engines.values():
+            e.close()
+        np.savez(out, X=np.concatenate(Xs), y=np.concatenate(ys), meta=np.array(meta), method=json.dumps(METHOD))
+        if log:
+            log("data: %d molecules, %d orbitals, largest nBas %d, %.0f s" %
+                (len(meta), sum(m[1] for m in meta), max(m[0] for m in meta), time.time() - t0))
+        if time.time() - t0 > t_max:
+            break
     return out
 
 
-def train(data, out, epochs=400, width=128, depth=3, lr=1e-3, batch=512, seed=0):
+def train(data, out, epochs=600, width=128, depth=3, lr=1e-3, batch=512, seed=0):
     d = np.load(data)
     X, y = d["X"].astype(np.float64), d["y"].astype(np.float64)
     mean, std = X.mean(0), X.std(0) + 1e-12
@@ -123,13 +228,16 @@ def train(data, out, epochs=400, width=128, depth=3, lr=1e-3, batch=512, seed=0)
     hist = net.train(Xn[tr], yn[tr], epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
     t_train = time.time() - t0
     pv = net.forward(Xn[va], bf16=True) * ys
+    err = np.abs(pv - y[va])
     stats = {"train_loss_first": float(hist[0]), "train_loss_last": float(hist[-1]), "epochs": epochs,
-             "n_train": int(len(tr)), "n_val": int(nval), "val_mae_Ha": float(np.mean(np.abs(pv - y[va]))),
-             "val_max_Ha": float(np.max(np.abs(pv - y[va]))), "val_mae_lin_only_Ha": float(np.mean(np.abs(y[va]))),
-             "train_seconds": t_train}
+             "n_train": int(len(tr)), "n_val": int(nval), "n_molecules": int(len(d["meta"])),
+             "largest_nBas": int(d["meta"][:, 0].max()), "val_mae_Ha": float(err.mean()),
+             "val_p99_Ha": float(np.quantile(err, 0.99)), "val_max_Ha": float(err.max()),
+             "val_mae_first_cycle_Ha": float(np.mean(np.abs(y[va]))), "train_seconds": t_train}
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
     np.savez(out, params=net.get_params(), mean=mean, std=std, yscale=ys, width=width, depth=depth,
-             stats=json.dumps(stats))
+             stats=json.dumps(stats), method=json.dumps(METHOD))
+    net.close()
     return stats
 
 
@@ -139,105 +247,82 @@ class WarmStart:
     def __init__(self, path=DEFAULT_MODEL):
         d = np.load(path)
         self.mean, self.std, self.ys = d["mean"], d["std"], float(d["yscale"])
+        self.method = json.loads(str(d["method"]))
         self.net = gw.MLP(nin=NFEAT, width=int(d["width"]), depth=int(d["depth"]), seed=0)
         self.net.set_params(d["params"])
 
-    def predict(self, eng, sysd, doSRG=True, zmin=0.5, cap=0.1):
-        """eGW1 + learned correction.  The correction is applied only where the
-        first-cycle renormalisation Z(eHF) exceeds zmin (a quasiparticle-like
-        orbital, a single well-separated root) and is capped at `cap` Ha;
-        elsewhere the start is eGW1, where plain evGW's second cycle starts."""
-        e1 = first_cycle(eng, sysd, doSRG=doSRG)
-        X = features(eng, sysd, e1, doSRG=doSRG)
+    def close(self):
+        self.net.close()
+
+    def predict(self, eng, sysd):
+        """eGW1 + the learned correction (corrections above CAP Ha are not
+        applied); also returns eGW1."""
+        m = self.method
+        e1 = first_cycle(eng, sysd, doSRG=m["doSRG"], eta=m["eta"])
+        X = features(eng, sysd, e1, doSRG=m["doSRG"], eta=m["eta"])
         r = self.net.forward(((X - self.mean) / self.std).astype(np.float32), bf16=True).astype(np.float64)
         r *= self.ys
-        Z0 = 1.0 / (1.0 - X[:, 7])
-        ok = (Z0 > zmin) & (np.abs(r) < cap)
-        return e1 + np.where(ok, r, 0.0)
+        return e1 + np.where(np.abs(r) < CAP, r, 0.0), e1
 
-    def evgw(self, eng, sysd, doSRG=True, thresh=1e-6, maxSCF=128):
-        start = self.predict(eng, sysd, doSRG=doSRG)
-        r = eng.evgw(sysd["eHF"], maxSCF=maxSCF, thresh=thresh, max_diis=1, doSRG=doSRG, eGW_start=start)
+    def evgw(self, eng, sysd):
+        """The learned form: first cycle, prediction, evGW from the prediction.
+        Returns evGW's result dict with "start", and "nSCF_total" counting the
+        first cycle."""
+        m = self.method
+        start, e1 = self.predict(eng, sysd)
+        r = eng.evgw(sysd["eHF"], maxSCF=m["maxSCF"], thresh=m["thresh"], max_diis=m["max_diis"], eta=m["eta"],
+                     doSRG=m["doSRG"], eGW_start=start)
         r["start"] = start
+        r["nSCF_total"] = r["nSCF"] + 1
         return r
 
 
-def evaluate(model, nsys=8, seed=12345, doSRG=True):
-    ws = WarmStart(model)
-    rng = np.random.default_rng(seed)
-    rows = []
-    for k in range(nsys):
-        mol = random_system(rng)
-        s = ig.system(mol)
-        with gw.GWEngine(s["nBas"], s["nOrb"], s["nO"]) as eng:
-            eng.set_eri_ao(s["eri"], s["c"])
-            t = time.time()
-            ref = eng.evgw(s["eHF"], maxSCF=128, thresh=1e-6, max_diis=1, doSRG=doSRG)
-            t_plain = time.time() - t
-            t = time.time()
-            r = ws.evgw(eng, s, doSRG=doSRG)
-            t_warm = time.time() - t
-        d = np.abs(r["eGW"] - ref["eGW"]) * 27211.386
-        e, nO = s["eHF"], s["nO"]
-        win = np.abs(e - 0.5 * (e[nO - 1] + e[nO])) < 1.0
-        rows.append({"nOrb": s["nOrb"], "plain_cycles": ref["nSCF"], "warm_cycles": r["nSCF"] + 1,
-                     "max_err_meV": float(d.max()), "max_err_meV_1Ha_window": float(d[win].max()),
-                     "homo_lumo_err_meV": float(max(d[nO - 1], d[nO])), "n_over_1meV": int((d > 1.0).sum()),
-                     "start_err_meV": float(np.abs(r["start"] - ref["eGW"]).max() * 27211.386),
-                     "t_plain": t_plain, "t_warm": t_warm})
-        print(json.dumps(rows[-1]), flush=True)
-    return rows
+def plain(eng, sysd, method=METHOD):
+    return eng.evgw(sysd["eHF"], maxSCF=method["maxSCF"], thresh=method["thresh"], max_diis=method["max_diis"],
+                    eta=method["eta"], doSRG=method["doSRG"])
 
 
-def evaluate_device(model, nsys=32, seed=12345, doSRG=True, log=None):
-    """Held-out evaluation with the inputs produced on the device (integrals,
-    RHF Fock builds): for each seeded system, plain evGW (from eHF) and the
-    learned form (first cycle, prediction, evGW from the prediction), both on
-    the same engine.  Returns per-system rows (cycles, wall times, errors
-    against plain evGW in meV)."""
+def evaluate_device(model, nsys=64, seed=12345, log=None, precision="fp64", eig="refine+sytrd"):
+    """Held-out evaluation (molecules drawn with a seed the training never
+    used; inputs produced on the device): per molecule plain evGW from eHF and
+    the learned form on the same engine.  Returns per-molecule rows (cycles,
+    full diagonalisations, wall times, errors against plain evGW in meV)."""
     import torch
-
-    from . import scf
     ws = WarmStart(model)
     rng = np.random.default_rng(seed)
+    mols = [random_molecule(rng) for _ in range(nsys)]
     rows = []
-    while len(rows) < nsys:
-        mol = random_system(rng)
-        S, T, V = mol.one_electron()
-        eri_t = torch.empty(mol.nBas ** 4, dtype=torch.float64, device="cuda")
-        mol.eri_device(eri_t.data_ptr(), eri_t.numel())
-        jk = scf.jk_closure(mol.nBas, eri_t.data_ptr())
-        try:
-            s = ig.rhf(S, T, V, None, mol.nO, maxit=600, jk=jk)
-        except gw.QKError:
-            raise
-        except RuntimeError:
-            continue  # RHF not converged: drawn again
-        finally:
-            jk.engine.close()
-        s["nO"] = mol.nO
-        c_t = torch.from_numpy(np.asfortranarray(s["c"]).ravel(order="F").copy()).to("cuda")
-        with gw.GWEngine(mol.nBas, s["nOrb"], mol.nO) as eng:
-            eng.set_eri_ao_device(eri_t.data_ptr(), c_t.data_ptr())
-            try:
-                t = time.time()
-                ref = eng.evgw(s["eHF"], maxSCF=128, thresh=1e-6, max_diis=1, doSRG=doSRG)
-                t_plain = time.time() - t
-                t = time.time()
-                r = ws.evgw(eng, s, doSRG=doSRG)
-                t_warm = time.time() - t
-            except gw.NotConverged:
-                continue
-        d = np.abs(r["eGW"] - ref["eGW"]) * 27211.386
-        e, nO = s["eHF"], mol.nO
-        win = np.abs(e - 0.5 * (e[nO - 1] + e[nO])) < 1.0
-        rows.append({"nBas": mol.nBas, "nOrb": s["nOrb"], "plain_cycles": ref["nSCF"], "warm_cycles": r["nSCF"] + 1,
-                     "max_err_meV": float(d.max()), "max_err_meV_1Ha_window": float(d[win].max()),
-                     "homo_lumo_err_meV": float(max(d[nO - 1], d[nO])), "n_over_1meV": int((d > 1.0).sum()),
-                     "n_orbitals": int(len(d)), "t_plain": t_plain, "t_warm": t_warm})
-        if log:
-            log(json.dumps(rows[-1]))
-        del eri_t, c_t
+    for systems in device_systems(mols, log=log):
+        for s in systems:
+            c_t = torch.from_numpy(np.asfortranarray(s["c"]).ravel(order="F").copy()).cuda()
+            with gw.GWEngine(s["nBas"], s["nOrb"], s["nO"]) as eng:
+                eng.set_precision(precision)
+                eng.set_eigensolver(eig, 0.003)
+                eng.set_eri_ao_device(s["eri"][s["slot"]].data_ptr(), c_t.data_ptr())
+                try:
+                    plain(eng, s)  # warm-up of this size's GEMM shapes
+                    st0 = eng.eig_stats()
+                    t = time.time()
+                    ref = plain(eng, s)
+                    t_plain = time.time() - t
+                    st1 = eng.eig_stats()
+                    t = time.time()
+                    r = ws.evgw(eng, s)
+                    t_warm = time.time() - t
+                    st2 = eng.eig_stats()
+                except gw.NotConverged:
+                    continue
+            d = np.abs(r["eGW"] - ref["eGW"]) * 27211.386
+            e, nO = s["eHF"], s["nO"]
+            rows.append({"nBas": s["nBas"], "nOrb": s["nOrb"], "plain_cycles": ref["nSCF"],
+                         "learned_cycles": r["nSCF_total"], "plain_full_solves": st1["syevd"] - st0["syevd"],
+                         "learned_full_solves": st2["syevd"] - st1["syevd"], "max_err_meV": float(d.max()),
+                         "homo_lumo_err_meV": float(max(d[nO - 1], d[nO])), "n_over_1meV": int((d > 1.0).sum()),
+                         "n_orbitals": int(len(d)), "start_err_meV": float(np.abs(r["start"] - ref["eGW"]).max() * 27211.386),
+                         "t_plain": t_plain, "t_learned": t_warm})
+            if log:
+                log(json.dumps(rows[-1]))
+    ws.close()
     return rows
 
 
@@ -247,16 +332,27 @@ def main():
     ap.add_argument("--out", default="data.npz")
     ap.add_argument("--data", default="data.npz")
     ap.add_argument("--model", default=DEFAULT_MODEL)
-    ap.add_argument("--nsys", type=int, default=160)
-    ap.add_argument("--epochs", type=int, default=400)
+    ap.add_argument("--nsys", type=int, default=1024)
+    ap.add_argument("--epochs", type=int, default=600)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--t-max", type=float, default=1e9, help="make-data: stop drawing chunks after this many seconds")
     a = ap.parse_args()
+
+    def log(*x):
+        print(*x, flush=True)
+    if a.cmd in ("make-data", "eval"):
+        import torch
+        torch.cuda.init()  # torch's HIP runtime first, then the library binds to it
+    gw.load()
     if a.cmd == "make-data":
-        make_data(a.out, a.nsys, a.seed)
+        make_data_device(a.out, a.nsys, a.seed, log=log, t_max=a.t_max)
     elif a.cmd == "train":
         print(json.dumps(train(a.data, a.out, epochs=a.epochs, seed=a.seed)), flush=True)
     else:
-        evaluate(a.model, a.nsys, seed=12345 + a.seed)
+        rows = evaluate_device(a.model, a.nsys, seed=12345 + a.seed, log=log)
+        print(json.dumps({"molecules": len(rows), "max_err_meV": max(r["max_err_meV"] for r in rows),
+                          "plain_full": sum(r["plain_full_solves"] for r in rows),
+                          "learned_full": sum(r["learned_full_solves"] for r in rows)}), flush=True)
 
 
 if __name__ == "__main__":

@@ -71,11 +71,13 @@ def tf(b):
 
 
 def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=1e-5, max_diis=5, lin=False,
-                 eta=0.0, TDA_W=False, reg=False, eri_mo=None, bse=False, uhf=False, nel=None):
+                 eta=0.0, TDA_W=False, reg=False, eri_mo=None, bse=False, uhf=False, nel=None, use_gpu=True):
     """sysd: a dict from quack_amd.intgen.system(); eri_mo: MO integrals (nOrb^4, flat);
     bse: the phBSE flag of the "# BSE:" options line (static kernel); uhf: the
     methods file selects UHF instead of RHF (the unrestricted GW branch);
-    nel: (nEla, nElb) for input/molecule (default: the closed shell)."""
+    nel: (nEla, nElb) for input/molecule (default: the closed shell); use_gpu:
+    the GPU switch of input/hpc_flags (its fourth line, read_hpc_flags.f90:31-33),
+    None to leave the file out (QuAcK then takes both switches as F)."""
     for d in ("input", "int", "test"):
         os.makedirs(os.path.join(wd, d), exist_ok=True)
     mol = sysd["mol"]
@@ -84,8 +86,13 @@ def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=
     with open(os.path.join(wd, "input", "options"), "w") as f:
         f.write(OPTIONS.format(maxscf=maxSCF, thresh="%.10g" % thresh, diis=max_diis, lin=tf(lin), eta="%.10g" % eta,
                                tda=tf(TDA_W), reg=tf(reg), bse=tf(bse)))
-    with open(os.path.join(wd, "input", "hpc_flags"), "w") as f:
-        f.write("# if True (T), switch to HPC mode\n  F\n# if True (T), use GPU\n  T\n")
+    hpc = os.path.join(wd, "input", "hpc_flags")
+    if use_gpu is None:
+        if os.path.exists(hpc):
+            os.remove(hpc)
+    else:
+        with open(hpc, "w") as f:
+            f.write("# if True (T), switch to HPC mode\n  F\n# if True (T), use GPU\n  %s\n" % tf(use_gpu))
     with open(os.path.join(wd, "input", "molecule"), "w") as f:
         nea, neb = nel if nel is not None else (mol.nO, mol.nO)
         f.write("# nAt nEla nElb nCore nRyd\n%d %d %d 0 0\n# Znuc x  y  z\n" % (len(mol.symbols), nea, neb))

@@ -55,3 +55,39 @@ def mo_systems(systems):
         d["eri_mo"] = go.aotomo_eri_rhf(s["c"], s["eri"])
         out[k] = d
     return out
+
+
+def _mo4(G, c1, c2, c3, c4):
+    """<pq|rs> = sum c1[mu,p] c2[nu,q] c3[la,r] c4[si,s] <mu nu|la si>, indexed [p,q,r,s]."""
+    import numpy as np
+    t = np.tensordot(G, c4, axes=([3], [0]))   # [mu, nu, la, s]
+    t = np.tensordot(t, c3, axes=([2], [0]))   # [mu, nu, s, r]
+    t = np.tensordot(t, c2, axes=([1], [0]))   # [mu, s, r, q]
+    t = np.tensordot(t, c1, axes=([0], [0]))   # [s, r, q, p]
+    return t.transpose(3, 2, 1, 0)
+
+
+@pytest.fixture(scope="session")
+def uhf_h2o(systems):
+    """H2O/cc-pVDZ with genuinely different alpha and beta orbitals: beta's
+    coefficients are the RHF ones rotated by exp(X), X antisymmetric with a
+    seeded occupied-virtual block of scale 0.05, so the three MO blocks
+    (aa|aa), (aa|bb), (bb|bb) all differ and (aa|bb) loses the symmetry that
+    would hide a swapped alpha/beta index slot.  Returned flat, QuAcK's
+    column-major order; ERI_aabb(p,q,r,s) = <p_a q_b|r_a s_b>."""
+    import numpy as np
+    from scipy.linalg import expm
+    s = systems["H2O"]
+    c, nO = s["c"], s["nO"]
+    N, n = c.shape
+    rng = np.random.default_rng(11)
+    X = np.zeros((n, n))
+    K = 0.05 * rng.standard_normal((nO, n - nO))
+    X[:nO, nO:] = K
+    X[nO:, :nO] = -K.T
+    cb = c @ expm(X)
+    G = np.asarray(s["eri"]).reshape((N,) * 4, order="F")
+    flat = lambda t: np.asfortranarray(t).reshape(-1, order="F")
+    e = s["eHF"]
+    return {"aa": flat(_mo4(G, c, c, c, c)), "ab": flat(_mo4(G, c, cb, c, cb)), "bb": flat(_mo4(G, cb, cb, cb, cb)),
+            "e2": np.stack([e, e + 0.01 * np.sin(np.arange(n))], axis=1), "nO": nO, "n": n}

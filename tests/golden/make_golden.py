@@ -10,8 +10,15 @@
   coefficients c, so that tests rebuild identical MO integrals from the
   deterministic AO integrals) and the sign- and rotation-invariant outputs
   (Om, eGW, Z, SigC, EcRPA, EcGM, nSCF).
+* lattice_2x5_ref.npz, lattice_5x6_ref.npz: the same for planar H lattices
+  of the headline recipe (1.6 bohr spacing, +-0.15 bohr in-plane jitter,
+  default_rng(0), cc-pVDZ): 2 x 5 (N = 50, nS = 225) and 5 x 6 (N = 150,
+  nS = 2025).  Both keep every basis function (nOrb = nBas), so QuAcK's
+  RGW_self_energy_diag, which sizes its arrays with nBas, is valid there.
+  G0W0 (linearised at eta = 0, graphical at eta = 0.05, SRG), SRG-evGW to
+  1e-6 Ha with DIIS off and eta = 0.05 evGW with DIIS 5.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [h10] [lattice]
 """
 import json
 import os
@@ -56,7 +63,42 @@ GOLDENS = {
 }
 
 
+def lattice(nx, ny, name):
+    """Reference outputs for one lattice.  The RHF solution (c, eHF) of an
+    existing fixture file is kept, so that regenerating the outputs leaves the
+    inputs bit-identical; the AO integrals are deterministic."""
+    mol = ig.h_lattice(nx, ny, seed=0)
+    path = os.path.join(HERE, name)
+    if os.path.exists(path):
+        old = np.load(path)
+        s = {"eri": mol.eri(), "c": old["c"], "eHF": old["eHF"], "ERHF": old["ERHF"], "nO": int(old["nO"]),
+             "nOrb": old["c"].shape[1], "nBas": mol.nBas}
+    else:
+        s = ig.system(mol)
+    assert s["nOrb"] == s["nBas"], "a lattice whose RHF drops functions has no valid QuAcK result"
+    nO = s["nO"]
+    mo = qkref.aotomo(s["c"], s["eri"])
+    out = {"eHF": s["eHF"], "ERHF": s["ERHF"], "nO": nO, "c": s["c"], "nx": nx, "ny": ny}
+    # eta = 0 only linearised: its graphical solve is sensitive at the 1e-13 level
+    # near poles (a root can switch for one orbital between two FP64 builds)
+    for tag, kw in (("g0w0_lin", {"linearize": True}), ("g0w0_eta", {"eta": 0.05}), ("g0w0_srg", {"srg": True})):
+        r = qkref.rg0w0(mo, s["eHF"], nO, **kw)
+        for k in ("eGW", "SigC", "Z", "Om", "EcRPA", "EcGM"):
+            out[tag + "_" + k] = r[k]
+    for tag, kw in (("evgw_srg", {"max_diis": 1, "srg": True}), ("evgw_eta", {"max_diis": 5, "eta": 0.05})):
+        r = qkref.evrgw(mo, s["eHF"], nO, maxSCF=256, thresh=1e-6, **kw)
+        for k in ("eGW", "SigC", "Z", "EcRPA", "EcGM", "nSCF", "status"):
+            out[tag + "_" + k] = r[k]
+    np.savez(path, **{k: np.asarray(v) for k, v in out.items()})
+
+
 def main():
+    what = sys.argv[1:] or ["h10", "lattice"]
+    if "lattice" in what:
+        lattice(2, 5, "lattice_2x5_ref.npz")
+        lattice(5, 6, "lattice_5x6_ref.npz")
+    if "h10" not in what:
+        return
     with open(os.path.join(HERE, "quack_goldens.json"), "w") as f:
         json.dump(GOLDENS, f, indent=1)
     s = ig.system(ig.h_chain(10, seed=0))

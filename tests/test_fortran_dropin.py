@@ -128,3 +128,18 @@ def test_driver_unrestricted(tmp_path, mo_systems):
         ref = [o["EcRPA"], o["eGW"][occ[0] - 1, 0], o["eGW"][occ[0], 0], o["eGW"][occ[1] - 1, 1], o["eGW"][occ[1], 1]]
         for k in range(5):
             assert abs(d[k0 + k][1] - ref[k]) < 1e-8, (k0, k, d[k0 + k], ref[k])
+
+
+@pytest.mark.parametrize("use_gpu", [False, None])
+def test_driver_gpu_switch_off(tmp_path, systems, use_gpu):
+    """input/hpc_flags with the GPU switch F (or no file, which QuAcK's
+    read_hpc_flags.f90 reads as F): the run belongs to QuAcK's CPU RG0W0 /
+    evRGW, so the driver prints why and stops with code 3 before any device
+    call (the patched dispatch of src/GW/RGW.f90:85,102).  The T case is every
+    GPU test of this file."""
+    s = systems["He"]
+    quackio.write_inputs(str(tmp_path), s, g0w0=True, evgw=True, max_diis=1, use_gpu=use_gpu)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
+    assert "use_gpu is F" in r.stdout
+    assert not os.path.exists(tmp_path / "test" / "Rtest.dat")

@@ -1,41 +1,16 @@
-"""Learned warm start (quack_amd/learned.py + models/mlp_evgw_srg.npz, trained
-on the GPU by the project's MLP kernels) against plain evGW, which judges it:
-frontier orbitals within 1 meV, no more diagonalisations than plain evGW."""
-import os
+"""Learned warm start (quack_amd/learned.py + models/mlp_evgw_eta005.npz,
+trained on the GPU by the project's MLP kernels on QP shifts from the batched
+plain evGW) against plain evGW, which judges it: every QP energy within 1 meV
+of plain evGW (the same method: eta = 0.05, DIIS 5, 1e-6 Ha), and fewer full
+RPA diagonalisations than plain evGW needs.  The held-out molecules are drawn
+with a seed the training data never used."""
+import json
 
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 MEV = 1e-3 / 27.211386245988
-
-
-def test_warm_start_h10(systems):
-    """The warm start saves cycles and lands on plain evGW's solution at the
-    frontier.  SRG-regularised evGW has several self-consistent solutions a
-    few tenths of a meV to a few meV apart for orbitals next to satellites:
-    QuAcK's own DIIS-accelerated run (max_diis 5) converges, from the same eHF
-    start and to 1e-10, to a solution 3.2 meV from the plain (max_diis 1) one
-    for one orbital in the 1 Ha window of this H10 chain.  The warm start is
-    therefore held to 1 meV at HOMO and LUMO and, in the window, to no more
-    than the spread between QuAcK's two iteration schemes."""
-    from quack_amd import gw, learned
-    print("learned warm start H10", flush=True)
-    s = systems["H10"]
-    ws = learned.WarmStart(learned.DEFAULT_MODEL)
-    with gw.GWEngine(s["nBas"], s["nOrb"], s["nO"]) as eng:
-        eng.set_eri_ao(s["eri"], s["c"])
-        ref = eng.evgw(s["eHF"], maxSCF=128, thresh=1e-6, max_diis=1, doSRG=True)
-        ref5 = eng.evgw(s["eHF"], maxSCF=128, thresh=1e-6, max_diis=5, doSRG=True)
-        r = ws.evgw(eng, s, doSRG=True, thresh=1e-6)
-    nO = s["nO"]
-    assert r["nSCF"] + 1 <= ref["nSCF"]
-    assert abs(r["eGW"][nO - 1] - ref["eGW"][nO - 1]) < MEV
-    assert abs(r["eGW"][nO] - ref["eGW"][nO]) < MEV
-    e = s["eHF"]
-    win = np.abs(e - 0.5 * (e[nO - 1] + e[nO])) < 1.0
-    spread = np.abs(ref5["eGW"] - ref["eGW"])[win].max()
-    assert np.abs(r["eGW"] - ref["eGW"])[win].max() <= max(MEV, spread)
 
 
 def test_model_file_consistent():
@@ -45,3 +20,24 @@ def test_model_file_consistent():
     width, depth = int(d["width"]), int(d["depth"])
     n = learned.NFEAT * width + width + (depth - 1) * (width * width + width) + width + 1
     assert d["params"].size == n
+    assert json.loads(str(d["method"])) == learned.METHOD
+
+
+def test_heldout_all_orbitals_within_1meV():
+    """Eight held-out seeded H chains and planar lattices (N = 20..250 drawn
+    log-uniformly): the learned form lands within 1 meV of plain evGW on every
+    orbital of every molecule, and needs fewer full RPA diagonalisations in
+    total than plain evGW (both with refinement of the previous eigenvectors
+    where the energies moved little)."""
+    import torch
+
+    from quack_amd import gw, learned
+    torch.cuda.init()
+    gw.load()
+    print("learned held-out", flush=True)
+    rows = learned.evaluate_device(learned.DEFAULT_MODEL, nsys=8, seed=20241203,
+                                   log=lambda *a: print(*a, flush=True))
+    assert len(rows) >= 6
+    worst = max(r["max_err_meV"] for r in rows)
+    assert worst < 1.0, worst
+    assert sum(r["learned_full_solves"] for r in rows) < sum(r["plain_full_solves"] for r in rows)

@@ -70,6 +70,26 @@ def test_phrlr_ab_and_general(gw, mo_systems, name):
             assert np.abs(sgn[:, None] * XmY - XmY0).max() < 1e-7
 
 
+def test_phrlr_indefinite_a_minus_b(gw):
+    """A - B with a negative eigenvalue: QuAcK warns (LR/phRLR.f90:57-58) and
+    goes on with the square root of a negative number; the engine returns
+    QK_E_UNSTABLE before any NaN reaches the second eigensolve.  A and B are
+    small hand-built symmetric matrices with A - B = diag(-0.1, 0.3, 0.5, 0.7)
+    in a rotated basis."""
+    say("phrlr unstable")
+    rng = np.random.default_rng(3)
+    Q, _ = np.linalg.qr(rng.standard_normal((4, 4)))
+    AmB = Q @ np.diag([-0.1, 0.3, 0.5, 0.7]) @ Q.T
+    B = 0.05 * (lambda m: m + m.T)(rng.standard_normal((4, 4)))
+    A = AmB + B
+    with pytest.raises(gw.QKError) as ex:
+        gw.phRLR(False, A, B)
+    assert ex.value.code == gw.QK_E_UNSTABLE
+    # shifted by 0.2, A - B is positive definite: no error
+    Ec, Om, _, _ = gw.phRLR(False, A + 0.2 * np.eye(4), B)
+    assert np.all(np.isfinite(Om)) and np.isfinite(Ec)
+
+
 @pytest.mark.parametrize("name", ["He", "H2O", "H10"])
 def test_excitation_density_sigma_qp(gw, mo_systems, name):
     say("rho/sigma/qp", name)

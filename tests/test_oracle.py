@@ -373,3 +373,32 @@ def test_evugw_vs_compiled_reference(mo_systems):
     u = go.evugw(mo, mo, mo, e2, (nO, nO), doSRG=True)
     rr = go.evrgw(mo, s["eHF"], nO, maxSCF=64, doSRG=True)
     assert np.abs(u["eGW"] - rr["eGW"][:, None]).max() < 1e-11 and abs(u["EcRPA"] - rr["EcRPA"]) < 1e-11
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+def test_unrestricted_distinct_spins_vs_compiled_reference(uhf_h2o):
+    """UG0W0 and evUGW with genuinely different alpha and beta orbitals, so
+    (aa|bb) has none of the symmetry of a restricted block: the restatement
+    against QuAcK's compiled unrestricted routines.  A planted mistake, the
+    alpha and beta slots of (aa|bb) swapped (its [q,p,s,r] view), must move
+    the result, so the case discriminates the index slots of phULR_A/B
+    (LR/phULR_A.f90, LR/phULR_B.f90) and of UGW_excitation_density."""
+    u = uhf_h2o
+    n, nO = u["n"], u["nO"]
+    for occ in ((nO, nO), (nO + 1, nO)):
+        for kw in ({"doSRG": True}, {"eta": 0.05}, {"TDA_W": True, "doSRG": True}):
+            r = qkref.ug0w0(u["aa"], u["ab"], u["bb"], u["e2"], occ, eta=kw.get("eta", 0.0),
+                            tda_w=kw.get("TDA_W", False), srg=kw.get("doSRG", False))
+            o = go.ug0w0(u["aa"], u["ab"], u["bb"], u["e2"], occ, **kw)
+            assert np.abs(o["eGW"] - r["eGW"]).max() < 1e-10, (occ, kw)
+            assert abs(o["EcRPA"] - r["EcRPA"]) < 1e-10, (occ, kw)
+    ab_swapped = u["ab"].reshape((n,) * 4, order="F").transpose(1, 0, 3, 2).reshape(-1, order="F")
+    assert np.abs(ab_swapped - u["ab"]).max() > 1e-3
+    bad = go.ug0w0(u["aa"], ab_swapped, u["bb"], u["e2"], (nO + 1, nO), doSRG=True)
+    good = go.ug0w0(u["aa"], u["ab"], u["bb"], u["e2"], (nO + 1, nO), doSRG=True)
+    assert np.abs(bad["eGW"] - good["eGW"]).max() > 1e-5
+    occ = (nO + 1, nO)
+    r = qkref.evugw(u["aa"], u["ab"], u["bb"], u["e2"], occ, srg=True)
+    o = go.evugw(u["aa"], u["ab"], u["bb"], u["e2"], occ, doSRG=True)
+    assert o["converged"] and o["nSCF"] == r["nSCF"]
+    assert np.abs(o["eGW"] - r["eGW"]).max() < 1e-10
