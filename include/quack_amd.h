@@ -205,6 +205,25 @@ int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int
 int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO, int nV, int nR, int nS,
                     const double *eHF, const double *Om, const double *rho, const double *eGWlin, const double *eOld,
                     double *eGW, double *Z);
+/* Allocation-free form of qk_rgw_qp_graph.  No counterpart in QuAcK: it is for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Every array is a device
+ * pointer; rhoT holds rho(p, q, m) at rhoT[m + nS (p + nOrb q)] (the engine's
+ * layout, modes contiguous).  The Newton state and partial sums live in a
+ * library-owned static device array of 4 MiB per GPU (held until the process
+ * ends): 8 (3 nOrb nchunk + 3 nOrb) + 4 (2 nOrb + 1) bytes with
+ * nchunk = ceil(nS / 1024) must fit, else QK_E_ARG and nothing runs.  The
+ * call enqueues a fixed 64 Newton sweeps (finished orbitals masked on the
+ * device, results identical to qk_rgw_qp_graph) on `stream` (a hipStream_t;
+ * NULL = the null stream) and returns without synchronising; it calls no
+ * hipMalloc/hipFree.  Calls that overlap on one GPU share the array and race.
+ * Z is in/out as in qk_rgw_qp_graph.  Messages: qk_noalloc_last_error. */
+int qk_rgw_qp_graph_noalloc(int doSRG, double eta, double flow, int nOrb, int nC, int nO, int nV, int nR, int nS,
+                            const double *eHF_device, const double *Om_device, const double *rhoT_device,
+                            const double *eGWlin_device, const double *eOld_device, double *eGW_device,
+                            double *Z_device, void *stream);
+const char *qk_noalloc_last_error(void);
+
 /* Unrestricted G0W0 (GW/UG0W0.f90:92-150) with nC = nR = 0: spin-conserving
  * dRPA screening at eHF over both spin blocks (LR/phULR_A.f90, LR/phULR_B.f90,
  * LR/phULR.f90; nSa + nSb modes), excitation densities

@@ -31,7 +31,8 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
            "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
-           "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms"]
+           "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms",
+           "qk_rgw_qp_graph_noalloc", "qk_noalloc_last_error"]
 
 
 class QKError(RuntimeError):
@@ -120,6 +121,8 @@ def load():
     lib.qk_rgw_excitation_density.argtypes = [_I] * 5 + [_P, _P, _P]
     lib.qk_rgw_self_energy_diag.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
     lib.qk_rgw_qp_graph.argtypes = [_I, _D, _D] + [_I] * 6 + [_P] * 7
+    lib.qk_rgw_qp_graph_noalloc.argtypes = [_I, _D, _D] + [_I] * 6 + [_H] * 8
+    lib.qk_noalloc_last_error.restype = ctypes.c_char_p
     lib.qk_rgw_phbse.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 5
     lib.qk_gw_sigma_full.argtypes = [_H, _P, _D, _I, _P, _P, _P, _P]
     lib.qk_ug0w0.argtypes = [_I, _I, _I, _P, _P, _P, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
