@@ -15,8 +15,8 @@
   default_rng(0), cc-pVDZ): 2 x 5 (N = 50, nS = 225) and 5 x 6 (N = 150,
   nS = 2025).  Both keep every basis function (nOrb = nBas), so QuAcK's
   RGW_self_energy_diag, which sizes its arrays with nBas, is valid there.
-  G0W0 (linearised at eta = 0, graphical at eta = 0.05, SRG), SRG-evGW to
-  1e-6 Ha with DIIS off and eta = 0.05 evGW with DIIS 5.
+  G0W0 (linearised at eta = 0, graphical at eta = 0.05 and 0.1, SRG),
+  SRG-evGW to 1e-6 Ha with DIIS off and eta = 0.05 and 0.1 evGW with DIIS 5.
 
     python tests/golden/make_golden.py [h10] [lattice]
 """
@@ -81,11 +81,13 @@ def lattice(nx, ny, name):
     out = {"eHF": s["eHF"], "ERHF": s["ERHF"], "nO": nO, "c": s["c"], "nx": nx, "ny": ny}
     # eta = 0 only linearised: its graphical solve is sensitive at the 1e-13 level
     # near poles (a root can switch for one orbital between two FP64 builds)
-    for tag, kw in (("g0w0_lin", {"linearize": True}), ("g0w0_eta", {"eta": 0.05}), ("g0w0_srg", {"srg": True})):
+    for tag, kw in (("g0w0_lin", {"linearize": True}), ("g0w0_eta", {"eta": 0.05}), ("g0w0_eta01", {"eta": 0.1}),
+                    ("g0w0_srg", {"srg": True})):
         r = qkref.rg0w0(mo, s["eHF"], nO, **kw)
         for k in ("eGW", "SigC", "Z", "Om", "EcRPA", "EcGM"):
             out[tag + "_" + k] = r[k]
-    for tag, kw in (("evgw_srg", {"max_diis": 1, "srg": True}), ("evgw_eta", {"max_diis": 5, "eta": 0.05})):
+    for tag, kw in (("evgw_srg", {"max_diis": 1, "srg": True}), ("evgw_eta", {"max_diis": 5, "eta": 0.05}),
+                    ("evgw_eta01", {"max_diis": 5, "eta": 0.1})):
         r = qkref.evrgw(mo, s["eHF"], nO, maxSCF=256, thresh=1e-6, **kw)
         for k in ("eGW", "SigC", "Z", "EcRPA", "EcGM", "nSCF", "status"):
             out[tag + "_" + k] = r[k]
