@@ -159,6 +159,12 @@ int qk_gw_set_eigensolver(qk_gw_handle h, int mode, double de_max);
  * cluster met; full solves by the own tridiagonalisation. */
 int qk_gw_eig_stats(qk_gw_handle h, long long *counts, int n);
 
+/* Newton iterations of every orbital in the handle's last graphical QP solve
+ * (nit[nOrb]; 64 = QuAcK's maxIt, where the orbital keeps its starting energy,
+ * GW/RGW_QP_graph.f90:78-81; 0 for frozen orbitals).  No counterpart in QuAcK,
+ * which prints them. */
+int qk_gw_qp_iters(qk_gw_handle h, int *nit);
+
 /* Device time (ms, HIP events on the handle's stream) of the stages of the
  * last qk_rg0w0 / qk_evrgw call: [0] AO->MO block (last set_eri_*),
  * [1] RPA matrix build, [2] eigensolve, [3] excitation densities,
@@ -250,6 +256,18 @@ int qk_ug0w0(int nBas, int nOa, int nOb, const double *ERI_aaaa, const double *E
 int qk_evugw(int nBas, int nOa, int nOb, const double *ERI_aaaa, const double *ERI_aabb, const double *ERI_bbbb,
              const double *eHF, int maxSCF, double thresh, int max_diis, double eta, int linearize, int TDA_W,
              int doSRG, double *eGW, double *SigC, double *Z, double *Om, double *EcRPA, double *EcGM, int *nSCF_out);
+
+/* Eigensolver of the unrestricted screening of qk_ug0w0 and qk_evugw,
+ * process-wide: mode and de_max as qk_gw_set_eigensolver (0: rocSOLVER dsyevd
+ * for every RPA, QuAcK's sequence and the default; 1: Newton refinement of the
+ * previous cycle's eigenvectors when no energy of either spin moved by more
+ * than de_max Ha, dsyevd otherwise; 2: the own tridiagonalisation for every
+ * full solve; 3: both).  No counterpart in QuAcK (LR/phULR.f90 always calls
+ * diagonalize_matrix). */
+int qk_set_unrestricted_eigensolver(int mode, double de_max);
+/* Eigensolver counts (as qk_gw_eig_stats) of the calling thread's last
+ * qk_ug0w0 or qk_evugw. */
+int qk_unrestricted_eig_stats(long long *counts, int n);
 
 /* Static phBSE@GW (GW/RGW_phBSE.f90:1-201, with GW/RGW_phBSE_static_kernel_A.f90
  * and _B.f90): dRPA screening at eW (TDA_W), its excitation densities, the

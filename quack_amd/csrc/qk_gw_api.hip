@@ -291,6 +291,10 @@ struct qk_gw_ctx {
     long long* eig_lbuf = nullptr;
     bool have_vec = false;
     int tda_vec = 0;
+    // unrestricted pairs (the screening of qk_ug0w0 and qk_evugw): alpha nO,
+    // beta nOb, nC = nR = 0, nS = nSa + nSb, energies (e_alpha, e_beta); -1
+    // for the restricted handle
+    int nOb = -1;
     std::vector<double> e_vec;
     // dsyevd calls, refined solves, Newton steps, fallbacks; fallbacks because a
     // cluster outgrew EIG_KMAX, because merging did not settle, because the
@@ -692,17 +696,24 @@ bool refine_eig(Ctx* c) {
 
 // RPA screening at energies e (device; eh = the same energies on the host).
 // Cm = eigenvectors (scaled to W when vectors), Om = excitation energies.
+int n_energies(const Ctx* c) { return c->nOb >= 0 ? 2 * c->nOrb : c->nOrb; }
+
 void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors) {
     {
         Stage st(c, 0);
-        rpa_build(c->s, e, c->K, c->dE, c->sd, c->Cm, c->scal, c->nC, c->nO, c->v, c->nS, tda);
+        if (c->nOb >= 0) {
+            delta_u(c->s, e, c->dE, c->sd, c->nOrb, c->nO, c->nOb);
+            rpa_build_c(c->s, c->K, c->dE, c->sd, c->Cm, c->scal, c->nS, tda);
+        } else {
+            rpa_build(c->s, e, c->K, c->dE, c->sd, c->Cm, c->scal, c->nC, c->nO, c->v, c->nS, tda);
+        }
     }
     {
         Stage st(c, 1);
         bool use_refine = (c->eig_mode & 1) && vectors && c->have_vec && c->tda_vec == tda;
         if (use_refine) {
             double d = 0.0;
-            for (int p = 0; p < c->nOrb; ++p) d = std::max(d, std::fabs(eh[p] - c->e_vec[p]));
+            for (int p = 0; p < n_energies(c); ++p) d = std::max(d, std::fabs(eh[p] - c->e_vec[p]));
             use_refine = d <= c->eig_demax;
         }
         bool done = false;
@@ -737,7 +748,7 @@ void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors)
         if ((c->eig_mode & 1) && vectors) {
             c->have_vec = true;
             c->tda_vec = tda;
-            c->e_vec.assign(eh, eh + c->nOrb);
+            c->e_vec.assign(eh, eh + n_energies(c));
         }
     }
     {
@@ -1334,6 +1345,14 @@ int qk_gw_eig_stats(qk_gw_handle c, long long* counts, int n) {
     return QK_OK;
 }
 
+int qk_gw_qp_iters(qk_gw_handle c, int* nit) {
+    if (!c || !nit) return QK_E_ARG;
+    return guarded(&c->err, [&] {
+        QK_HIP(hipMemcpyAsync(nit, c->nit, sizeof(int) * c->nOrb, hipMemcpyDeviceToHost, c->s));
+        QK_HIP(hipStreamSynchronize(c->s));
+    });
+}
+
 int qk_gw_stage_times(qk_gw_handle c, double* ms, int n) {
     if (!c || !ms || n < 0 || n > 14) return QK_E_ARG;
     double all[14] = {c->t_ao, c->t_stage[0], c->t_stage[1], c->t_stage[2], c->t_stage[3], c->t_stage[4],
@@ -1656,128 +1675,166 @@ double* upload_rho(hipStream_t s, const double* rho, int n, int nS) {
 
 namespace qk {
 namespace {
-// Diagonal self-energy (GW/RGW_self_energy_diag.f90 or the SRG form) from host
-// rho(p,q,m) (n x n x nS, column-major) on the scratch stream.  nS is the
-// number of RPA modes, which the unrestricted caller sets to nSa + nSb.
-void sigma_diag_host(int doSRG, double eta, int n, int nC, int nO, int nR, int nS, const double* e, const double* Om,
-                     const double* rho, double* EcGM, double* SigC, double* Z, const double* rT_dev = nullptr) {
-    hipStream_t s = scratch().s;
-    int nch = sigma_chunks(nS, n - nR - nC);
-    std::vector<double*> bufs;
-    auto get = [&](size_t cnt) { bufs.push_back(dalloc<double>(cnt)); return bufs.back(); };
-    try {
-        const double* rT = rT_dev;  // rho(p, q, m) at m + nS (p + n q) on the device, or uploaded here
-        if (!rT) {
-            bufs.push_back(upload_rho(s, rho, n, nS));
-            rT = bufs.back();
+// Device work of the diagonal self-energy and the graphical QP solve for n
+// orbitals and nch chunks of modes: partial sums, Newton state and the
+// orbital-energy vectors, allocated once by their owner.
+struct QpWork {
+    int n = 0, nch = 0;
+    double *part = nullptr, *w = nullptr, *f = nullptr, *df = nullptr, *sig = nullptr, *z = nullptr, *sc = nullptr;
+    double *e0 = nullptr, *eG = nullptr, *eL = nullptr, *eo = nullptr;
+    int *nit = nullptr, *done = nullptr, *act = nullptr;
+    QpWork(int n_, int nch_) : n(n_), nch(nch_) {
+        try {
+            part = dalloc<double>((size_t)3 * n * nch);
+            for (double** p : {&w, &f, &df, &sig, &z, &e0, &eG, &eL, &eo}) *p = dalloc<double>(n);
+            sc = dalloc<double>(4);
+            nit = dalloc<int>(n);
+            done = dalloc<int>(n);
+            act = dalloc<int>(1);
+        } catch (...) {
+            release();
+            throw;
         }
-        double* de = get(n);
-        double* dO = get(nS);
-        double* part = get((size_t)3 * n * nch);
-        double* dS = get(n);
-        double* dZ = get(n);
-        double* sc = get(4);
-        QK_HIP(hipMemcpyAsync(de, e, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
-        sigma_partial(s, rT, false, de, de, dO, nullptr, eta, doSRG, SRG_FLOW, n, nC, nO, nR, nS, nch, part);
-        sigma_finalize(s, part, nch, n, nC, nO, nR, dS, dZ, sc);
-        QK_HIP(hipMemcpyAsync(SigC, dS, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        QK_HIP(hipMemcpyAsync(Z, dZ, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        double gm = 0.0;
-        QK_HIP(hipMemcpyAsync(&gm, sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
-        QK_HIP(hipStreamSynchronize(s));
-        if (EcGM) *EcGM = gm;
-    } catch (...) {
-        (void)hipStreamSynchronize(s);
-        for (double* p : bufs) (void)hipFree(p);
-        throw;
     }
-    for (double* p : bufs) (void)hipFree(p);
+    QpWork(const QpWork&) = delete;
+    QpWork& operator=(const QpWork&) = delete;
+    ~QpWork() { release(); }
+    void release() {
+        for (double** p : {&part, &w, &f, &df, &sig, &z, &sc, &e0, &eG, &eL, &eo}) dfree(*p);
+        dfree(nit);
+        dfree(done);
+        dfree(act);
+    }
+};
+
+// Diagonal self-energy (GW/RGW_self_energy_diag.f90 or the SRG form) at the
+// host energies e, from device rT (rho(p, q, m) at m + nS (p + n q)) and Om;
+// SigC, Z and EcGM to the host.
+void sigma_dev(hipStream_t s, QpWork& q, const double* rT, const double* dO, int doSRG, double eta, int nC, int nO,
+               int nR, int nS, const double* e, double* SigC, double* Z, double* EcGM) {
+    const int n = q.n;
+    QK_HIP(hipMemcpyAsync(q.e0, e, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    sigma_partial(s, rT, false, q.e0, q.e0, dO, nullptr, eta, doSRG, SRG_FLOW, n, nC, nO, nR, nS, q.nch, q.part);
+    sigma_finalize(s, q.part, q.nch, n, nC, nO, nR, q.sig, q.z, q.sc);
+    QK_HIP(hipMemcpyAsync(SigC, q.sig, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    QK_HIP(hipMemcpyAsync(Z, q.z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    double gm = 0.0;
+    QK_HIP(hipMemcpyAsync(&gm, q.sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+    QK_HIP(hipStreamSynchronize(s));
+    if (EcGM) *EcGM = gm;
 }
 
-// Graphical QP solve (GW/RGW_QP_graph.f90:50-92) from host arrays, as above.
-void qp_graph_host(int doSRG, double eta, double flow, int n, int nC, int nO, int nR, int nS, const double* eHF,
-                   const double* Om, const double* rho, const double* eGWlin, const double* eOld, double* eGW, double* Z,
-                   const double* rT_dev = nullptr) {
-    hipStream_t s = scratch().s;
-    int nch = sigma_chunks(nS, n - nR - nC);
-    std::vector<void*> bufs;
-    auto get = [&](size_t cnt) { double* p = dalloc<double>(cnt); bufs.push_back(p); return p; };
-    auto geti = [&](size_t cnt) { int* p = dalloc<int>(cnt); bufs.push_back(p); return p; };
-    try {
-        const double* rT = rT_dev;
-        if (!rT) {
-            double* u = upload_rho(s, rho, n, nS);
-            bufs.push_back(u);
-            rT = u;
-        }
-        double* d0 = get(n);
-        double* dG = get(n);
-        double* dL = get(n);
-        double* dO = get(nS);
-        double* w = get(n);
-        double* f = get(n);
-        double* df = get(n);
-        double* dE = get(n);
-        double* dZ = get(n);
-        double* part = get((size_t)3 * n * nch);
-        int* nit = geti(n);
-        int* done = geti(n);
-        int* act = geti(1);
-        QK_HIP(hipMemcpyAsync(d0, eHF, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(dG, eOld, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(dL, eGWlin, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(dZ, Z, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        qp_init(s, dL, w, f, nit, done, n, nC, nR);
-        int h_active = 1;
-        for (int it = 0; it < QP_MAXIT && h_active > 0; ++it) {
-            sigma_partial(s, rT, false, w, dG, dO, done, eta, doSRG, flow, n, nC, nO, nR, nS, nch, part);
-            qp_update(s, part, nch, d0, w, f, df, nit, done, n, QP_THRESH, QP_MAXIT, act);
-            QK_HIP(hipMemcpyAsync(&h_active, act, sizeof(int), hipMemcpyDeviceToHost, s));
-            QK_HIP(hipStreamSynchronize(s));
-        }
-        qp_finish(s, w, df, nit, dL, dE, dZ, n, nC, nR, QP_MAXIT);
-        QK_HIP(hipMemcpyAsync(eGW, dE, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        QK_HIP(hipMemcpyAsync(Z, dZ, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+// Graphical QP solve (GW/RGW_QP_graph.f90:50-92) from host energies, device rT
+// and Om; the Newton sweeps stop as soon as no orbital is left active.  Z is
+// read (frozen orbitals keep it) and written on the host.
+void qp_dev(hipStream_t s, QpWork& q, const double* rT, const double* dO, int doSRG, double eta, double flow, int nC,
+            int nO, int nR, int nS, const double* eHF, const double* eGWlin, const double* eOld, double* eGW,
+            double* Z) {
+    const int n = q.n;
+    QK_HIP(hipMemcpyAsync(q.e0, eHF, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(q.eG, eOld, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(q.eL, eGWlin, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(q.z, Z, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    qp_init(s, q.eL, q.w, q.f, q.nit, q.done, n, nC, nR);
+    int h_active = 1;
+    for (int it = 0; it < QP_MAXIT && h_active > 0; ++it) {
+        sigma_partial(s, rT, false, q.w, q.eG, dO, q.done, eta, doSRG, flow, n, nC, nO, nR, nS, q.nch, q.part);
+        qp_update(s, q.part, q.nch, q.e0, q.w, q.f, q.df, q.nit, q.done, n, QP_THRESH, QP_MAXIT, q.act);
+        QK_HIP(hipMemcpyAsync(&h_active, q.act, sizeof(int), hipMemcpyDeviceToHost, s));
         QK_HIP(hipStreamSynchronize(s));
-    } catch (...) {
-        (void)hipStreamSynchronize(s);
-        for (void* p : bufs) (void)hipFree(p);
-        throw;
     }
-    for (void* p : bufs) (void)hipFree(p);
+    qp_finish(s, q.w, q.df, q.nit, q.eL, q.eo, q.z, n, nC, nR, QP_MAXIT);
+    QK_HIP(hipMemcpyAsync(eGW, q.eo, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    QK_HIP(hipMemcpyAsync(Z, q.z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    QK_HIP(hipStreamSynchronize(s));
 }
+
+// Host rho(p,q,m) (n x n x nS, column-major) and Om uploaded for one
+// self-energy or QP call of the C-ABI (qk_rgw_self_energy_diag, qk_rgw_qp_graph).
+struct HostRho {
+    hipStream_t s;
+    double *rT = nullptr, *dO = nullptr;
+    HostRho(hipStream_t s_, const double* rho, const double* Om, int n, int nS) : s(s_) {
+        try {
+            rT = upload_rho(s, rho, n, nS);
+            dO = dalloc<double>(nS);
+            QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    ~HostRho() { release(); }
+    void release() {
+        (void)hipStreamSynchronize(s);
+        dfree(rT);
+        dfree(dO);
+    }
+};
+
+// Eigensolver of the unrestricted screening (qk_set_unrestricted_eigensolver),
+// process-wide, and the eigensolver counts of the calling thread's last
+// qk_ug0w0 / qk_evugw (qk_unrestricted_eig_stats).
+std::atomic<int> g_u_eig_mode{0};
+std::atomic<double> g_u_eig_demax{5e-3};
+thread_local long long g_u_eig_count[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 // Unrestricted screening and excitation densities (nC = nR = 0), shared by
 // UG0W0 and evUGW.  Pair index ia = a' + v_s i' within spin block s, the beta
 // block after the alpha one (LR/phULR_A.f90, LR/phULR_B.f90).  The integral
-// slabs M_s(pq, k) = ERI(p, j, q, b) of each spin's densities
-// (GW/UGW_excitation_density.f90:24-62) stay on the device for the whole
-// calculation; each screening leaves rho_s = M_s XpY^T / sqrt(2) there in the
+// slabs M_s(pq, k) = (pq|k) of each spin's densities
+// (GW/UGW_excitation_density.f90:24-62) and the coupling K = (ia|jb) / 2
+// gathered from them stay on the device for the whole calculation.  The dRPA
+// A = D + K_u, B = K_u (K_u = 2K) has A - B = D, so each screening is the
+// restricted engine's (screening(): C = D^2 + 2 D^1/2 K_u D^1/2, or A under
+// TDA, built on the device; the engine's eigensolver, with Newton refinement
+// of the previous cycle's vectors when it is switched on), held by a
+// screening-only engine handle whose pairs are the unrestricted ones.  It
+// leaves Om and XpY^T = W on the device, then rho_s = M_s W / sqrt(2) in the
 // restricted kernels' rhoT layout (m contiguous): the unrestricted sums weigh
 // rho^2 where the restricted kernels weigh 2 rho^2.
 struct UScreen {
     int n, nO[2], v[2], nSa, nSb, nSt, TDA_W;
     size_t S2, n2;
     const double* G[3];
-    hipStream_t st;
-    double *dX = nullptr, *dM[2] = {nullptr, nullptr}, *dR[2] = {nullptr, nullptr};
-    std::vector<double> A, B, XpY, XmY, om;
+    Ctx* c = nullptr;
+    hipStream_t st = nullptr;
+    double* de = nullptr;
+    double *dM[2] = {nullptr, nullptr}, *dR[2] = {nullptr, nullptr};
+    QpWork* q = nullptr;
 
     UScreen(int n_, int nOa, int nOb, const double* Gaaaa, const double* Gaabb, const double* Gbbbb, int tda)
-        : n(n_), nO{nOa, nOb}, v{n_ - nOa, n_ - nOb}, TDA_W(tda), G{Gaaaa, Gaabb, Gbbbb}, st(scratch().s) {
+        : n(n_), nO{nOa, nOb}, v{n_ - nOa, n_ - nOb}, TDA_W(tda), G{Gaaaa, Gaabb, Gbbbb} {
         nSa = nO[0] * v[0];
         nSb = nO[1] * v[1];
         nSt = nSa + nSb;
         QK_CHECK((long long)nSt * nSt < (1LL << 40), QK_E_ARG, "nSa + nSb too large");
         S2 = (size_t)nSt * nSt;
         n2 = (size_t)n * n;
-        XpY.resize(S2);
-        XmY.resize(S2);
-        om.resize(nSt);
+        QK_CHECK((long long)n2 < (1LL << 31), QK_E_ARG, "nBas^2 too large");
         try {
-            dX = dalloc<double>(S2);
+            c = new Ctx();
+            c->nBas = c->nOrb = n;
+            c->nC = c->nR = 0;
+            c->nO = nOa;
+            c->nOb = nOb;
+            c->nV = n - nOa;
+            c->o = nOa;
+            c->v = n - nOa;
+            c->nS = nSt;
+            QK_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+            rb_check(rocblas_create_handle(&c->rb), "rocblas_create_handle");
+            rb_check(rocblas_set_stream(c->rb, c->s), "rocblas_set_stream");
+            for (auto* e : {&c->ev[0], &c->ev[1], &c->ev2[0], &c->ev2[1], &c->ev3[0], &c->ev3[1], &c->ev3[2],
+                            &c->ev3[3]})
+                QK_HIP(hipEventCreate(e));
+            st = c->s;
+            c->K = dalloc<double>(S2);
+            c->Cm = dalloc<double>(S2);
+            for (double** p : {&c->dE, &c->sd, &c->Om, &c->work}) *p = dalloc<double>(nSt);
+            c->scal = dalloc<double>(8);
+            c->info = dalloc<int>(1);
+            de = dalloc<double>(2 * (size_t)n);
             std::vector<double> Ms(n2 * nSt);
             for (int sp = 0; sp < 2; ++sp) {
                 dM[sp] = dalloc<double>(n2 * nSt);
@@ -1787,94 +1844,85 @@ struct UScreen {
                         for (int b = 0; b < v[sj]; ++b) {
                             const size_t k = (sj ? nSa : 0) + b + (size_t)v[sj] * j;
                             const int Bb = nO[sj] + b;
-                            for (int q = 0; q < n; ++q)
+                            for (int qq = 0; qq < n; ++qq)
                                 for (int p = 0; p < n; ++p)
-                                    Ms[(p + (size_t)n * q) + n2 * k] =
-                                        sp == 0 ? at(sj == 0 ? 0 : 1, p, j, q, Bb)
-                                                : (sj == 0 ? at(1, j, p, Bb, q) : at(2, p, j, q, Bb));
+                                    Ms[(p + (size_t)n * qq) + n2 * k] =
+                                        sp == 0 ? at(sj == 0 ? 0 : 1, p, j, qq, Bb)
+                                                : (sj == 0 ? at(1, j, p, Bb, qq) : at(2, p, j, qq, Bb));
                         }
                 QK_HIP(hipMemcpyAsync(dM[sp], Ms.data(), sizeof(double) * n2 * nSt, hipMemcpyHostToDevice, st));
                 QK_HIP(hipStreamSynchronize(st));  // Ms is refilled for the next spin
             }
+            extract_ku(st, dM[0], dM[1], c->K, n, nO[0], nO[1]);
+            const int rc = qk_gw_set_eigensolver(c, g_u_eig_mode.load(), g_u_eig_demax.load());
+            QK_CHECK(rc == QK_OK, rc, "unrestricted screening: " + c->err);
+            q = new QpWork(n, sigma_chunks(nSt, n));
         } catch (...) {
             release();
             throw;
         }
     }
+    UScreen(const UScreen&) = delete;
+    UScreen& operator=(const UScreen&) = delete;
     ~UScreen() { release(); }
     void release() {
-        (void)hipStreamSynchronize(st);
-        dfree(dX);
-        for (int sp = 0; sp < 2; ++sp) {
-            dfree(dM[sp]);
-            dfree(dR[sp]);
+        if (st) (void)hipStreamSynchronize(st);
+        delete q;
+        q = nullptr;
+        for (double** p : {&de, &dM[0], &dM[1], &dR[0], &dR[1]}) dfree(*p);
+        if (c) {
+            std::copy(c->eig_count, c->eig_count + 9, g_u_eig_count);
+            qk::release(c);
+            delete c;
+            c = nullptr;
         }
-        dX = dM[0] = dM[1] = dR[0] = dR[1] = nullptr;
+        st = nullptr;
     }
-    double at(int k, int p, int q, int r, int t) const {  // ERI_k(p, q, r, t), 0-based
-        return G[k][p + (size_t)n * (q + (size_t)n * (r + (size_t)n * t))];
+    double at(int k, int p, int qq, int r, int t) const {  // ERI_k(p, q, r, t), 0-based
+        return G[k][p + (size_t)n * (qq + (size_t)n * (r + (size_t)n * t))];
     }
-    // spin-conserving dRPA A (orbital energies en, n x 2) and B (LR/phULR_A.f90:26-80, LR/phULR_B.f90:26-80)
-    void build(const double* en) {
-        A.assign(S2, 0.0);
-        B.assign(S2, 0.0);
-        for (int si = 0; si < 2; ++si)
-            for (int sj = 0; sj < 2; ++sj) {
-                const int oi = si ? nSa : 0, oj = sj ? nSa : 0;
-                for (int i = 0; i < nO[si]; ++i)
-                    for (int a = 0; a < v[si]; ++a) {
-                        const size_t ia = oi + a + (size_t)v[si] * i;
-                        const int Aa = nO[si] + a;
-                        for (int j = 0; j < nO[sj]; ++j)
-                            for (int b = 0; b < v[sj]; ++b) {
-                                const size_t jb = oj + b + (size_t)v[sj] * j;
-                                const int Bb = nO[sj] + b;
-                                double x, y;
-                                if (si == sj) {  // ERI_ssss(i,b,a,j), ERI_ssss(i,j,a,b)
-                                    const int k = si ? 2 : 0;
-                                    x = at(k, i, Bb, Aa, j);
-                                    y = at(k, i, j, Aa, Bb);
-                                } else if (si == 0) {  // alpha ia, beta jb: ERI_aabb(i,b,a,j), (i,j,a,b)
-                                    x = at(1, i, Bb, Aa, j);
-                                    y = at(1, i, j, Aa, Bb);
-                                } else {  // beta ia, alpha jb: ERI_aabb(b,i,j,a), (j,i,b,a)
-                                    x = at(1, Bb, i, j, Aa);
-                                    y = at(1, j, i, Bb, Aa);
-                                }
-                                if (si == sj && i == j && a == b) x += en[Aa + (size_t)n * si] - en[i + (size_t)n * si];
-                                A[ia + nSt * jb] = x;
-                                B[ia + nSt * jb] = TDA_W ? 0.0 : y;
-                            }
-                    }
-            }
-    }
-    // LR/phULR.f90 at energies en; returns EcRPA, leaves Om and XpY
+    // LR/phULR.f90 at energies en (n x 2, host); returns EcRPA, leaves Om and
+    // W = XpY^T on the device
     double rpa(const double* en, const char* who) {
-        build(en);
+        if (!TDA_W) {
+            // LR/phRLR.f90:57-58 (the unrestricted solver shares it): A - B
+            // must be positive definite; here it is diag(e_a - e_i)
+            double dmin = HUGE_VAL;
+            for (int sp = 0; sp < 2; ++sp)
+                for (int i = 0; i < nO[sp]; ++i)
+                    for (int a = nO[sp]; a < n; ++a) dmin = std::min(dmin, en[(size_t)n * sp + a] - en[(size_t)n * sp + i]);
+            QK_CHECK(dmin > 0.0, QK_E_UNSTABLE,
+                     std::string(who) + ": A-B is not positive definite (smallest eigenvalue " +
+                         std::to_string(dmin) + "): linear-response instability");
+        }
+        QK_HIP(hipMemcpyAsync(de, en, sizeof(double) * 2 * n, hipMemcpyHostToDevice, st));
+        screening(c, de, en, TDA_W, true);
         double ec = 0.0;
-        int r = qk_phrlr(TDA_W, nSt, A.data(), B.data(), &ec, om.data(), XpY.data(), XmY.data());
-        QK_CHECK(r == QK_OK, r, std::string(who) + ": phULR failed: " + g_last_error);
+        QK_HIP(hipMemcpyAsync(&ec, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
         return ec;
     }
-    // GW/UGW_excitation_density.f90 for both spins from the last rpa()
+    // GW/UGW_excitation_density.f90 for both spins from the last rpa():
+    // rho_s(m, pq) = sum_k W(k, m) M_s(pq, k) / sqrt(2)
     void densities() {
-        QK_HIP(hipMemcpyAsync(dX, XpY.data(), sizeof(double) * S2, hipMemcpyHostToDevice, st));
         for (int sp = 0; sp < 2; ++sp)
-            dgemm(st, false, true, nSt, (int)n2, nSt, 1.0 / std::sqrt(2.0), dX, nSt, dM[sp], n2, 0.0, dR[sp], nSt);
+            dgemm(st, true, true, nSt, (int)n2, nSt, 1.0 / std::sqrt(2.0), c->Cm, nSt, dM[sp], n2, 0.0, dR[sp], nSt);
+    }
+    void om_host(double* out) {
+        QK_HIP(hipMemcpyAsync(out, c->Om, sizeof(double) * nSt, hipMemcpyDeviceToHost, st));
         QK_HIP(hipStreamSynchronize(st));
     }
     // per spin: diagonal self-energy at e_s (GW/UGW_self_energy_diag.f90 or the SRG form);
     // EcGM per spin, halved (the restricted sum carries 4 rho^2, the unrestricted rho^2, x2 here)
     void sigma(int sp, const double* es, double eta, int doSRG, double* sig, double* z, double* ecgm) {
         double gm = 0.0;
-        sigma_diag_host(doSRG, eta, n, 0, nO[sp], 0, nSt, es, om.data(), nullptr, &gm, sig, z, dR[sp]);
+        sigma_dev(st, *q, dR[sp], c->Om, doSRG, eta, 0, nO[sp], 0, nSt, es, sig, z, &gm);
         if (ecgm) *ecgm = 0.5 * gm;
     }
     // GW/UGW_QP_graph.f90 for spin sp
     void qp(int sp, int doSRG, double eta, const double* eHF_s, const double* eLin, const double* eOld, double* eGW_s,
             double* z) {
-        qp_graph_host(doSRG, eta, SRG_FLOW, n, 0, nO[sp], 0, nSt, eHF_s, om.data(), nullptr, eLin, eOld, eGW_s, z,
-                      dR[sp]);
+        qp_dev(st, *q, dR[sp], c->Om, doSRG, eta, SRG_FLOW, 0, nO[sp], 0, nSt, eHF_s, eLin, eOld, eGW_s, z);
     }
 };
 }  // namespace
@@ -1887,7 +1935,10 @@ int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int
     if (!e || !Om || !rho || !SigC || !Z) return QK_E_ARG;
     return guarded(nullptr, [&] {
         check_sizes(nOrb, nOrb, nC, nO, nV, nR, nS);
-        sigma_diag_host(doSRG, eta, nOrb, nC, nO, nR, nS, e, Om, rho, EcGM, SigC, Z);
+        hipStream_t s = scratch().s;
+        HostRho r(s, rho, Om, nOrb, nS);
+        QpWork q(nOrb, sigma_chunks(nS, nOrb - nR - nC));
+        sigma_dev(s, q, r.rT, r.dO, doSRG, eta, nC, nO, nR, nS, e, SigC, Z, EcGM);
     });
 }
 
@@ -1897,7 +1948,10 @@ int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO
     if (!eHF || !Om || !rho || !eGWlin || !eOld || !eGW || !Z) return QK_E_ARG;
     return guarded(nullptr, [&] {
         check_sizes(nOrb, nOrb, nC, nO, nV, nR, nS);
-        qp_graph_host(doSRG, eta, flow, nOrb, nC, nO, nR, nS, eHF, Om, rho, eGWlin, eOld, eGW, Z);
+        hipStream_t s = scratch().s;
+        HostRho r(s, rho, Om, nOrb, nS);
+        QpWork q(nOrb, sigma_chunks(nS, nOrb - nR - nC));
+        qp_dev(s, q, r.rT, r.dO, doSRG, eta, flow, nC, nO, nR, nS, eHF, eGWlin, eOld, eGW, Z);
     });
 }
 
@@ -1913,7 +1967,7 @@ int qk_ug0w0(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* E
         UScreen u(n, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
         u.rpa(eHF, "qk_ug0w0");
         u.densities();
-        if (Om) std::memcpy(Om, u.om.data(), sizeof(double) * u.nSt);
+        if (Om) u.om_host(Om);
         // GW/UG0W0.f90:104-134 per spin: self-energy at eHF, linearised energies,
         // graphical QP with the linearised guess and G from eHF
         for (int sp = 0; sp < 2; ++sp) {
@@ -1976,7 +2030,7 @@ int qk_evugw(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* E
             ++nSCF;
         }
         std::memcpy(eGW, e.data(), sizeof(double) * n2s);
-        if (Om) std::memcpy(Om, u.om.data(), sizeof(double) * u.nSt);
+        if (Om) u.om_host(Om);
         if (EcRPA) *EcRPA = ecrpa;
         if (EcGM) {
             EcGM[0] = ecgm[0];
@@ -2509,6 +2563,19 @@ int qk_gwb_stage_times(qk_gwb_handle h, double* ms, int n) {
     return QK_OK;
 }
 
+
+int qk_set_unrestricted_eigensolver(int mode, double de_max) {
+    if (mode < 0 || mode > 3 || !(de_max >= 0.0)) return QK_E_ARG;
+    qk::g_u_eig_mode.store(mode);
+    qk::g_u_eig_demax.store(de_max);
+    return QK_OK;
+}
+
+int qk_unrestricted_eig_stats(long long* counts, int n) {
+    if (!counts || n < 0 || n > 9) return QK_E_ARG;
+    for (int i = 0; i < n; ++i) counts[i] = qk::g_u_eig_count[i];
+    return QK_OK;
+}
 
 int qk_set_graphs(int on) {
     qk::g_graphs.store(on ? 1 : 0);

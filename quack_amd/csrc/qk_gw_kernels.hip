@@ -51,6 +51,47 @@ __global__ void k_extract_k(const double* __restrict__ M, double* __restrict__ K
     }
 }
 
+// Unrestricted pairs (no frozen orbitals): pair ia of spin s is a' + v_s i'
+// within its spin block, the beta block after the alpha one.  The restricted
+// RPA kernels then build the unrestricted matrices unchanged from
+// K[ia + nSt jb] = (ia|jb) / 2: A = D + K_u, B = K_u give A + B = D + 2 K_u and
+// Tr A = sum D + K_u(ia, ia), the restricted forms with 2K in place of K_u.
+__device__ inline void upair(int ia, int n, int nOa, int nOb, int nSa, int& s, int& i, int& a) {
+    s = ia >= nSa;
+    const int l = ia - (s ? nSa : 0);
+    const int v = n - (s ? nOb : nOa);
+    a = (s ? nOb : nOa) + l % v;
+    i = l / v;
+}
+
+// K[ia + nSt jb] = (ia|jb) / 2 from M_s[p + n q + n^2 jb] = (pq|jb), s = spin of ia
+__global__ void k_extract_ku(const double* __restrict__ M0, const double* __restrict__ M1, double* __restrict__ K,
+                             int n, int nOa, int nOb, int nSa, int nSt) {
+    const long long total = (long long)nSt * nSt;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int ia = (int)(t % nSt);
+        const long long jb = t / nSt;
+        int s, i, a;
+        upair(ia, n, nOa, nOb, nSa, s, i, a);
+        const double* M = s ? M1 : M0;
+        K[t] = 0.5 * M[i + (long long)n * a + (long long)n * n * jb];
+    }
+}
+
+// dE[ia] = e_s[a] - e_s[i], sd[ia] = sqrt(dE), e holding e_alpha then e_beta
+__global__ void k_delta_u(const double* __restrict__ e, double* __restrict__ dE, double* __restrict__ sd, int n,
+                          int nOa, int nOb, int nSa, int nSt) {
+    const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ia < nSt) {
+        int s, i, a;
+        upair(ia, n, nOa, nOb, nSa, s, i, a);
+        const double d = e[n * s + a] - e[n * s + i];
+        dE[ia] = d;
+        sd[ia] = sqrt(d);
+    }
+}
+
 // ---- RPA matrix ---------------------------------------------------------
 
 // dE[ia] = e_a - e_i ; sd[ia] = sqrt(dE)
@@ -624,8 +665,23 @@ void extract_k(hipStream_t s, const double* M, double* K, int n, int nC, int nO,
 void rpa_build(hipStream_t s, const double* e, const double* K, double* dE, double* sd, double* Cm, double* scal,
                int nC, int nO, int v, int nS, int tda) {
     hipLaunchKernelGGL(k_delta, dim3((nS + 255) / 256), dim3(256), 0, s, e, dE, sd, nC, nO, v, nS);
+    rpa_build_c(s, K, dE, sd, Cm, scal, nS, tda);
+}
+void rpa_build_c(hipStream_t s, const double* K, const double* dE, const double* sd, double* Cm, double* scal, int nS,
+                 int tda) {
     hipLaunchKernelGGL(k_build_c, dim3(grid_for((long long)nS * nS)), dim3(256), 0, s, K, dE, sd, Cm, nS, tda);
     hipLaunchKernelGGL(k_trace_a, dim3(1), dim3(RB), 0, s, K, dE, nS, scal);
+    QK_HIP(hipGetLastError());
+}
+void extract_ku(hipStream_t s, const double* M0, const double* M1, double* K, int n, int nOa, int nOb) {
+    const int nSa = nOa * (n - nOa), nSt = nSa + nOb * (n - nOb);
+    hipLaunchKernelGGL(k_extract_ku, dim3(grid_for((long long)nSt * nSt)), dim3(256), 0, s, M0, M1, K, n, nOa, nOb,
+                       nSa, nSt);
+    QK_HIP(hipGetLastError());
+}
+void delta_u(hipStream_t s, const double* e, double* dE, double* sd, int n, int nOa, int nOb) {
+    const int nSa = nOa * (n - nOa), nSt = nSa + nOb * (n - nOb);
+    hipLaunchKernelGGL(k_delta_u, dim3((nSt + 255) / 256), dim3(256), 0, s, e, dE, sd, n, nOa, nOb, nSa, nSt);
     QK_HIP(hipGetLastError());
 }
 void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w) {

@@ -14,6 +14,14 @@ void extract_k(hipStream_t s, const double* M, double* K, int n, int nC, int nO,
 // scal[0] = Tr(A), scal[1] = EcRPA, scal[2] = EcGM
 void rpa_build(hipStream_t s, const double* e, const double* K, double* dE, double* sd, double* Cm, double* scal,
                int nC, int nO, int v, int nS, int tda);
+// rpa_build without the pair energies: dE, sd given
+void rpa_build_c(hipStream_t s, const double* K, const double* dE, const double* sd, double* Cm, double* scal, int nS,
+                 int tda);
+// Unrestricted pairs (nC = nR = 0; alpha block, then beta): K = (ia|jb) / 2
+// from the spin slabs M_s[p + n q + n^2 jb] = (pq|jb), and the pair energies
+// from e = (e_alpha, e_beta)
+void extract_ku(hipStream_t s, const double* M0, const double* M1, double* K, int n, int nOa, int nOb);
+void delta_u(hipStream_t s, const double* e, double* dE, double* sd, int n, int nOa, int nOb);
 void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w);
 int sigma_chunks(int nS, int nact);
 // rhoT holds doubles, or floats when rho_f32

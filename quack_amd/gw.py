@@ -32,7 +32,16 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
            "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
            "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms",
-           "qk_rgw_qp_graph_noalloc", "qk_noalloc_last_error"]
+           "qk_rgw_qp_graph_noalloc", "qk_noalloc_last_error", "qk_gw_qp_iters",
+           "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats"]
+
+
+EIG_MODES = {"syevd": 0, "refine": 1, "sytrd": 2, "refine+sytrd": 3}
+
+
+def _eig_counts(c):
+    return {"syevd": c[0], "refined": c[1], "newton_steps": c[2], "fallbacks": c[3], "fail_cluster_size": c[4],
+            "fail_unsettled": c[5], "fail_noconv": c[6], "largest_cluster": c[7], "own_tridiagonal": c[8]}
 
 
 class QKError(RuntimeError):
@@ -121,12 +130,15 @@ def load():
     lib.qk_rgw_excitation_density.argtypes = [_I] * 5 + [_P, _P, _P]
     lib.qk_rgw_self_energy_diag.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
     lib.qk_rgw_qp_graph.argtypes = [_I, _D, _D] + [_I] * 6 + [_P] * 7
+    lib.qk_gw_qp_iters.argtypes = [_H, _IP]
     lib.qk_rgw_qp_graph_noalloc.argtypes = [_I, _D, _D] + [_I] * 6 + [_H] * 8
     lib.qk_noalloc_last_error.restype = ctypes.c_char_p
     lib.qk_rgw_phbse.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 5
     lib.qk_gw_sigma_full.argtypes = [_H, _P, _D, _I, _P, _P, _P, _P]
     lib.qk_ug0w0.argtypes = [_I, _I, _I, _P, _P, _P, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_evugw.argtypes = [_I, _I, _I, _P, _P, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
+    lib.qk_set_unrestricted_eigensolver.argtypes = [_I, _D]
+    lib.qk_unrestricted_eig_stats.argtypes = [ctypes.POINTER(ctypes.c_longlong), _I]
     lib.qk_rhf_create.argtypes = [ctypes.POINTER(_H), _I]
     lib.qk_rhf_destroy.argtypes = [_H]
     lib.qk_rhf_last_error.argtypes = [_H]
@@ -204,14 +216,12 @@ class GWEngine:
         Newton refinement of the previous eigenvectors when no orbital energy
         moved by more than de_max Ha, dsyevd otherwise; "sytrd" (2): full solves
         through the library's own tridiagonalisation; "refine+sytrd" (3)."""
-        _check(load().qk_gw_set_eigensolver(self.h, {"syevd": 0, "refine": 1, "sytrd": 2, "refine+sytrd": 3}.get(
-            mode, mode), de_max), self.h)
+        _check(load().qk_gw_set_eigensolver(self.h, EIG_MODES.get(mode, mode), de_max), self.h)
 
     def eig_stats(self):
         c = (ctypes.c_longlong * 9)()
         _check(load().qk_gw_eig_stats(self.h, c, 9), self.h)
-        return {"syevd": c[0], "refined": c[1], "newton_steps": c[2], "fallbacks": c[3], "fail_cluster_size": c[4],
-                "fail_unsettled": c[5], "fail_noconv": c[6], "largest_cluster": c[7], "own_tridiagonal": c[8]}
+        return _eig_counts(c)
 
     def set_eri_df(self, B):
         """Density-fitted MO integrals: B of shape (naux, nOrb, nOrb) (any float64
@@ -295,6 +305,13 @@ class GWEngine:
         Sig, dSig = np.zeros(n), np.zeros(n)
         _check(load().qk_gw_sigma_at(self.h, _p(_f64(eG)), _p(_f64(w)), eta, int(doSRG), _p(Sig), _p(dSig)), self.h)
         return Sig, dSig
+
+    def qp_iters(self):
+        """Newton iterations per orbital of the last graphical QP solve (64: the
+        solve failed and the orbital kept its starting energy)."""
+        nit = np.zeros(self.nOrb, dtype=np.int32)
+        _check(load().qk_gw_qp_iters(self.h, nit.ctypes.data_as(_IP)), self.h)
+        return nit
 
     def stage_times(self):
         ms = np.zeros(14)
@@ -580,6 +597,19 @@ def RGW_phBSE(ERI, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, singlet=True, t
     _check(load().qk_rgw_phbse(0, int(exchange_kernel), int(TDA_W), int(TDA), 0, 0, int(singlet), int(triplet), eta,
                                n, nC, nO, nV, nR, nS, _p(ERI), _p(eW), _p(eGW), _p(Ec), _p(Om)))
     return {"EcBSE": Ec, "OmS": Om[:nS].copy(), "OmT": Om[nS:].copy()}
+
+
+def set_unrestricted_eigensolver(mode, de_max=5e-3):
+    """Eigensolver of UG0W0/evUGW's screening (process-wide), the modes of
+    GWEngine.set_eigensolver; "syevd" (QuAcK's sequence) by default."""
+    _check(load().qk_set_unrestricted_eigensolver(EIG_MODES.get(mode, mode), de_max))
+
+
+def unrestricted_eig_stats():
+    """Eigensolver counts of this thread's last UG0W0/evUGW (as GWEngine.eig_stats)."""
+    c = (ctypes.c_longlong * 9)()
+    _check(load().qk_unrestricted_eig_stats(c, 9))
+    return _eig_counts(c)
 
 
 def UG0W0(ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False):
