@@ -3,14 +3,19 @@ the batched plain evGW produces, run before the evGW cycles.
 
 Method.  evGW with QuAcK's broadened self-energy (GW/RGW_self_energy_diag.f90
 with eta > 0, its exact derivative in the graphical Newton solve of
-GW/RGW_QP_graph.f90), eta = 0.05 Ha, DIIS 5 (QuAcK's default,
+GW/RGW_QP_graph.f90), eta = 0.1 Ha, DIIS 5 (QuAcK's default,
 GW/evRGW.f90:168-173), converged to 1e-6 Ha.  Its self-consistent solution is
-unique in the sense that matters here: evGW restarted a few mHa away from it
-returns to it for every orbital.  Neither QuAcK's unbroadened eta = 0 form
-(chaotic: 1e-12 Ha moves QP energies by eV) nor its SRG form has that: the SRG
-Newton solve uses a pseudo-derivative (GW/RGW_SRG_Re_dSigC.f90) and orbitals
-whose solve fails keep their previous energy (GW/RGW_QP_graph.f90:78-81), so
-their final values depend on the path and any warm start moves them.
+unique in the sense that matters here: evGW restarted from its own
+second-cycle input, or 5 mHa away from its solution, returns to it for every
+orbital (measured on 40 seeded H chains and lattices, N = 30..250).  At
+eta = 0.05 with DIIS 5 six of the same 40 are path-dependent (a restart from
+the second-cycle input alone, which only drops the DIIS history, lands on
+another solution), so no warm start can be held to plain evGW there.  Neither
+QuAcK's unbroadened eta = 0 form (chaotic: 1e-12 Ha moves QP energies by eV)
+nor its SRG form has a unique solution: the SRG Newton solve uses a
+pseudo-derivative (GW/RGW_SRG_Re_dSigC.f90) and orbitals whose solve fails keep
+their previous energy (GW/RGW_QP_graph.f90:78-81), so their final values depend
+on the path and any warm start moves them.
 
 Flow of the learned form for one molecule:
   1. plain evGW's first cycle at eHF (screening, excitation densities,
@@ -36,7 +41,7 @@ single engine's first cycle.  Training runs on the device with the library's
 own forward, backward and Adam kernels.
 
     python -m quack_amd.learned make-data --out data.npz --nsys 1024
-    python -m quack_amd.learned train --data data.npz --out quack_amd/models/mlp_evgw_eta005.npz
+    python -m quack_amd.learned train --data data.npz --out quack_amd/models/mlp_evgw_eta01.npz
     python -m quack_amd.learned eval --nsys 64
 """
 import argparse
@@ -53,14 +58,14 @@ from . import intgen as ig
 OFFSETS = (-0.25, 0.0, 0.25)
 NFEAT = 14
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_MODEL = os.path.join(HERE, "models", "mlp_evgw_eta005.npz")
+DEFAULT_MODEL = os.path.join(HERE, "models", "mlp_evgw_eta01.npz")
 # the evGW the model is trained for and judged against
-METHOD = {"eta": 0.05, "doSRG": False, "max_diis": 5, "thresh": 1e-6, "maxSCF": 256}
+METHOD = {"eta": 0.1, "doSRG": False, "max_diis": 5, "thresh": 1e-6, "maxSCF": 256}
 # predictions larger than this (Ha) are not trusted: the start is eGW1 there
 CAP = 0.25
 
 
-def first_cycle(eng, sysd, doSRG=False, eta=0.05):
+def first_cycle(eng, sysd, doSRG=False, eta=0.1):
     """Plain evGW's first cycle (maxSCF = 0 runs exactly one): eGW1."""
     try:
         r = eng.evgw(sysd["eHF"], maxSCF=0, thresh=0.0, max_diis=1, eta=eta, doSRG=doSRG)
@@ -69,7 +74,7 @@ def first_cycle(eng, sysd, doSRG=False, eta=0.05):
     return r["eGW"]
 
 
-def features(eng, sysd, eGW1, doSRG=False, eta=0.05):
+def features(eng, sysd, eGW1, doSRG=False, eta=0.1):
     """(n x 14) features from the screening left by the first cycle (at eHF)."""
     e = np.asarray(sysd["eHF"], np.float64)
     n, nO = len(e), sysd["nO"]
@@ -100,44 +105,49 @@ def random_molecule(rng):
     return ig.h_lattice(nx, ny, spacing=rng.uniform(1.5, 1.9), jitter=rng.uniform(0.0, 0.15), seed=seed)
 
 
-def device_systems(mols, log=None, chunk_gb=64.0):
+def device_systems(mols, log=None, chunk_gb=64.0, rng=None):
     """Yields lists of prepared systems of one basis size whose AO integrals
     sit in one contiguous device tensor (at most chunk_gb GB): each system has
     eHF, c (host), Sx, nOrb, nO, nBas, the tensor and its index there.  RHFs
-    that do not converge in 600 iterations are left out."""
+    that do not converge in 600 iterations are left out.  Chunks come in
+    order of size, or in an order shuffled by rng (so that a run cut short
+    still covers every size)."""
     import torch
 
     from . import scf as qscf
     by_size = {}
     for m in mols:
         by_size.setdefault(m.nBas, []).append(m)
+    chunks = []
     for nBas, ms in sorted(by_size.items()):
         per = max(1, int(chunk_gb * 1e9 // (8 * nBas ** 4)))
-        for k0 in range(0, len(ms), per):
-            part = ms[k0:k0 + per]
-            eri = torch.empty((len(part), nBas ** 4), dtype=torch.float64, device="cuda")
-            out = []
-            with qscf.JK(nBas) as jkeng:
-                for j, mol in enumerate(part):
-                    mol.eri_device(eri[j].data_ptr(), nBas ** 4)
-                    S, T, V = mol.one_electron()
-                    ptr = eri[j].data_ptr()
+        chunks += [(nBas, ms[k0:k0 + per]) for k0 in range(0, len(ms), per)]
+    if rng is not None:
+        chunks = [chunks[i] for i in rng.permutation(len(chunks))]
+    for nBas, part in chunks:
+        eri = torch.empty((len(part), nBas ** 4), dtype=torch.float64, device="cuda")
+        out = []
+        with qscf.JK(nBas) as jkeng:
+            for j, mol in enumerate(part):
+                mol.eri_device(eri[j].data_ptr(), nBas ** 4)
+                S, T, V = mol.one_electron()
+                ptr = eri[j].data_ptr()
 
-                    def jk(_eri, D, ptr=ptr):
-                        J, K = jkeng(ptr, D)
-                        return J, -K
-                    try:
-                        s = ig.rhf(S, T, V, None, mol.nO, maxit=600, jk=jk)
-                    except gw.QKError:
-                        raise
-                    except RuntimeError:
-                        if log:
-                            log("RHF not converged: molecule left out (nBas %d)" % nBas)
-                        continue
-                    s.update({"nO": mol.nO, "nBas": nBas, "eri": eri, "slot": j})
-                    out.append(s)
-            yield out
-            del eri
+                def jk(_eri, D, ptr=ptr):
+                    J, K = jkeng(ptr, D)
+                    return J, -K
+                try:
+                    s = ig.rhf(S, T, V, None, mol.nO, maxit=600, jk=jk)
+                except gw.QKError:
+                    raise
+                except RuntimeError:
+                    if log:
+                        log("RHF not converged: molecule left out (nBas %d)" % nBas)
+                    continue
+                s.update({"nO": mol.nO, "nBas": nBas, "eri": eri, "slot": j})
+                out.append(s)
+        yield out
+        del eri
 
 
 def plain_targets_batched(systems, **method):
@@ -181,7 +191,7 @@ def make_data_device(out, nsys, seed=0, log=None, t_max=1e9):
     Xs, ys, meta = [], [], []
     t0 = time.time()
     kw = {k: METHOD[k] for k in ("eta", "doSRG", "max_diis", "thresh", "maxSCF")}
-    for systems in device_systems(mols, log=log):
+    for systems in device_systems(mols, log=log, rng=rng):
         if not systems:
             continue
         tgt = plain_targets_batched(systems, **kw)
@@ -254,22 +264,35 @@ class WarmStart:
     def close(self):
         self.net.close()
 
-    def predict(self, eng, sysd):
-        """eGW1 + the learned correction (corrections above CAP Ha are not
-        applied); also returns eGW1."""
+    def predict(self, eng, sysd, window=None, cap=CAP, diag=None):
+        """eGW1 + the learned correction, applied where |correction| < cap Ha
+        and, with `window`, only to orbitals whose eHF lies within `window` Ha
+        of the Fermi level; eGW1 elsewhere.  Also returns eGW1.  diag (a
+        dict), when given, receives per-molecule quantities known after the
+        first cycle."""
         m = self.method
         e1 = first_cycle(eng, sysd, doSRG=m["doSRG"], eta=m["eta"])
+        nit1 = eng.qp_iters()
         X = features(eng, sysd, e1, doSRG=m["doSRG"], eta=m["eta"])
         r = self.net.forward(((X - self.mean) / self.std).astype(np.float32), bf16=True).astype(np.float64)
         r *= self.ys
-        return e1 + np.where(np.abs(r) < CAP, r, 0.0), e1
+        ok = np.abs(r) < cap
+        if window is not None:
+            ok &= np.abs(X[:, 12]) < window
+        if diag is not None:
+            Z0 = 1.0 / (1.0 - X[:, 7])
+            diag.update({"first_cycle_newton_failures": int((nit1 >= 64).sum()), "first_cycle_max_newton": int(nit1.max()),
+                         "max_abs_pred": float(np.abs(r).max()), "min_Z0": float(Z0.min()),
+                         "n_Z0_below_half": int((Z0 < 0.5).sum()), "max_first_shift": float(np.abs(e1 - X[:, 0]).max()),
+                         "e1_minus_start_max": float(np.abs(np.where(ok, r, 0.0)).max())})
+        return e1 + np.where(ok, r, 0.0), e1
 
-    def evgw(self, eng, sysd):
+    def evgw(self, eng, sysd, window=None, cap=CAP, diag=None):
         """The learned form: first cycle, prediction, evGW from the prediction.
         Returns evGW's result dict with "start", and "nSCF_total" counting the
         first cycle."""
         m = self.method
-        start, e1 = self.predict(eng, sysd)
+        start, e1 = self.predict(eng, sysd, window=window, cap=cap, diag=diag)
         r = eng.evgw(sysd["eHF"], maxSCF=m["maxSCF"], thresh=m["thresh"], max_diis=m["max_diis"], eta=m["eta"],
                      doSRG=m["doSRG"], eGW_start=start)
         r["start"] = start
@@ -282,7 +305,7 @@ def plain(eng, sysd, method=METHOD):
                     eta=method["eta"], doSRG=method["doSRG"])
 
 
-def evaluate_device(model, nsys=64, seed=12345, log=None, precision="fp64", eig="refine+sytrd"):
+def evaluate_device(model, nsys=64, seed=12345, log=None, precision="fp64", eig="refine+sytrd", de_max=0.03):
     """Held-out evaluation (molecules drawn with a seed the training never
     used; inputs produced on the device): per molecule plain evGW from eHF and
     the learned form on the same engine.  Returns per-molecule rows (cycles,
@@ -297,7 +320,7 @@ def evaluate_device(model, nsys=64, seed=12345, log=None, precision="fp64", eig=
             c_t = torch.from_numpy(np.asfortranarray(s["c"]).ravel(order="F").copy()).cuda()
             with gw.GWEngine(s["nBas"], s["nOrb"], s["nO"]) as eng:
                 eng.set_precision(precision)
-                eng.set_eigensolver(eig, 0.003)
+                eng.set_eigensolver(eig, de_max)
                 eng.set_eri_ao_device(s["eri"][s["slot"]].data_ptr(), c_t.data_ptr())
                 try:
                     plain(eng, s)  # warm-up of this size's GEMM shapes
@@ -306,8 +329,10 @@ def evaluate_device(model, nsys=64, seed=12345, log=None, precision="fp64", eig=
                     ref = plain(eng, s)
                     t_plain = time.time() - t
                     st1 = eng.eig_stats()
+                    nitp = eng.qp_iters()
                     t = time.time()
-                    r = ws.evgw(eng, s)
+                    dg = {}
+                    r = ws.evgw(eng, s, diag=dg)
                     t_warm = time.time() - t
                     st2 = eng.eig_stats()
                 except gw.NotConverged:
@@ -319,7 +344,8 @@ def evaluate_device(model, nsys=64, seed=12345, log=None, precision="fp64", eig=
                          "learned_full_solves": st2["syevd"] - st1["syevd"], "max_err_meV": float(d.max()),
                          "homo_lumo_err_meV": float(max(d[nO - 1], d[nO])), "n_over_1meV": int((d > 1.0).sum()),
                          "n_orbitals": int(len(d)), "start_err_meV": float(np.abs(r["start"] - ref["eGW"]).max() * 27211.386),
-                         "t_plain": t_plain, "t_learned": t_warm})
+                         "t_plain": t_plain, "t_learned": t_warm,
+                         "plain_last_cycle_newton_failures": int((nitp >= 64).sum()), **dg})
             if log:
                 log(json.dumps(rows[-1]))
     ws.close()

@@ -1,7 +1,7 @@
-"""Learned warm start (quack_amd/learned.py + models/mlp_evgw_eta005.npz,
+"""Learned warm start (quack_amd/learned.py + models/mlp_evgw_eta01.npz,
 trained on the GPU by the project's MLP kernels on QP shifts from the batched
 plain evGW) against plain evGW, which judges it: every QP energy within 1 meV
-of plain evGW (the same method: eta = 0.05, DIIS 5, 1e-6 Ha), and fewer full
+of plain evGW (the same method: eta = 0.1, DIIS 5, 1e-6 Ha), and fewer full
 RPA diagonalisations than plain evGW needs.  The held-out molecules are drawn
 with a seed the training data never used."""
 import json
