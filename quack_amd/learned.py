@@ -264,12 +264,15 @@ class WarmStart:
     def close(self):
         self.net.close()
 
-    def predict(self, eng, sysd, window=None, cap=CAP, diag=None):
+    def predict(self, eng, sysd, window=None, cap=CAP, diag=None, zmax=None):
         """eGW1 + the learned correction, applied where |correction| < cap Ha
         and, with `window`, only to orbitals whose eHF lies within `window` Ha
-        of the Fermi level; eGW1 elsewhere.  Also returns eGW1.  diag (a
-        dict), when given, receives per-molecule quantities known after the
-        first cycle."""
+        of the Fermi level, and, with `zmax`, only where the renormalisation
+        factor at eHF, Z0 = 1 / (1 - dSigma_c/dw), lies in (0, zmax] (Z0 > 1
+        means dSigma_c/dw > 0: the orbital sits on the near side of a
+        broadened pole of Sigma_c, where its QP equation can have several
+        roots); eGW1 elsewhere.  Also returns eGW1.  diag (a dict), when
+        given, receives per-molecule quantities known after the first cycle."""
         m = self.method
         e1 = first_cycle(eng, sysd, doSRG=m["doSRG"], eta=m["eta"])
         nit1 = eng.qp_iters()
@@ -279,20 +282,22 @@ class WarmStart:
         ok = np.abs(r) < cap
         if window is not None:
             ok &= np.abs(X[:, 12]) < window
+        Z0 = 1.0 / (1.0 - X[:, 7])
+        if zmax is not None:
+            ok &= (Z0 > 0.0) & (Z0 <= zmax)
         if diag is not None:
-            Z0 = 1.0 / (1.0 - X[:, 7])
             diag.update({"first_cycle_newton_failures": int((nit1 >= 64).sum()), "first_cycle_max_newton": int(nit1.max()),
                          "max_abs_pred": float(np.abs(r).max()), "min_Z0": float(Z0.min()),
                          "n_Z0_below_half": int((Z0 < 0.5).sum()), "max_first_shift": float(np.abs(e1 - X[:, 0]).max()),
                          "e1_minus_start_max": float(np.abs(np.where(ok, r, 0.0)).max())})
         return e1 + np.where(ok, r, 0.0), e1
 
-    def evgw(self, eng, sysd, window=None, cap=CAP, diag=None):
+    def evgw(self, eng, sysd, window=None, cap=CAP, diag=None, zmax=None):
         """The learned form: first cycle, prediction, evGW from the prediction.
         Returns evGW's result dict with "start", and "nSCF_total" counting the
         first cycle."""
         m = self.method
-        start, e1 = self.predict(eng, sysd, window=window, cap=cap, diag=diag)
+        start, e1 = self.predict(eng, sysd, window=window, cap=cap, diag=diag, zmax=zmax)
         r = eng.evgw(sysd["eHF"], maxSCF=m["maxSCF"], thresh=m["thresh"], max_diis=m["max_diis"], eta=m["eta"],
                      doSRG=m["doSRG"], eGW_start=start)
         r["start"] = start
@@ -305,11 +310,15 @@ def plain(eng, sysd, method=METHOD):
                     eta=method["eta"], doSRG=method["doSRG"])
 
 
-def evaluate_device(model, nsys=64, seed=12345, log=None, precision="fp64", eig="refine+sytrd", de_max=0.03):
+def evaluate_device(model, nsys=64, seed=12345, log=None, precision="fp64", eig="refine+sytrd", de_max=0.03,
+                    variants=None):
     """Held-out evaluation (molecules drawn with a seed the training never
     used; inputs produced on the device): per molecule plain evGW from eHF and
     the learned form on the same engine.  Returns per-molecule rows (cycles,
-    full diagonalisations, wall times, errors against plain evGW in meV)."""
+    full diagonalisations, wall times, errors against plain evGW in meV).
+    variants: {name: WarmStart.evgw keyword arguments} run in addition to the
+    default form on the same molecules, reported as name_cycles,
+    name_full_solves, name_max_err_meV, name_n_over_1meV."""
     import torch
     ws = WarmStart(model)
     rng = np.random.default_rng(seed)
@@ -335,6 +344,14 @@ def evaluate_device(model, nsys=64, seed=12345, log=None, precision="fp64", eig=
                     r = ws.evgw(eng, s, diag=dg)
                     t_warm = time.time() - t
                     st2 = eng.eig_stats()
+                    extra = {}
+                    for name, kw in (variants or {}).items():
+                        sa = eng.eig_stats()
+                        rv = ws.evgw(eng, s, **kw)
+                        sb = eng.eig_stats()
+                        dv = np.abs(rv["eGW"] - ref["eGW"]) * 27211.386
+                        extra.update({name + "_cycles": rv["nSCF_total"], name + "_full_solves": sb["syevd"] - sa["syevd"],
+                                      name + "_max_err_meV": float(dv.max()), name + "_n_over_1meV": int((dv > 1.0).sum())})
                 except gw.NotConverged:
                     continue
             d = np.abs(r["eGW"] - ref["eGW"]) * 27211.386
@@ -345,7 +362,7 @@ def evaluate_device(model, nsys=64, seed=12345, log=None, precision="fp64", eig=
                          "homo_lumo_err_meV": float(max(d[nO - 1], d[nO])), "n_over_1meV": int((d > 1.0).sum()),
                          "n_orbitals": int(len(d)), "start_err_meV": float(np.abs(r["start"] - ref["eGW"]).max() * 27211.386),
                          "t_plain": t_plain, "t_learned": t_warm,
-                         "plain_last_cycle_newton_failures": int((nitp >= 64).sum()), **dg})
+                         "plain_last_cycle_newton_failures": int((nitp >= 64).sum()), **dg, **extra})
             if log:
                 log(json.dumps(rows[-1]))
     ws.close()
@@ -362,6 +379,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=600)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--t-max", type=float, default=1e9, help="make-data: stop drawing chunks after this many seconds")
+    ap.add_argument("--zmax", type=float, default=None, help="eval: also run the form gated to 0 < Z0 <= zmax")
     a = ap.parse_args()
 
     def log(*x):
@@ -375,10 +393,20 @@ def main():
     elif a.cmd == "train":
         print(json.dumps(train(a.data, a.out, epochs=a.epochs, seed=a.seed)), flush=True)
     else:
-        rows = evaluate_device(a.model, a.nsys, seed=12345 + a.seed, log=log)
-        print(json.dumps({"molecules": len(rows), "max_err_meV": max(r["max_err_meV"] for r in rows),
-                          "plain_full": sum(r["plain_full_solves"] for r in rows),
-                          "learned_full": sum(r["learned_full_solves"] for r in rows)}), flush=True)
+        variants = {"zgate": {"zmax": a.zmax}} if a.zmax is not None else None
+        rows = evaluate_device(a.model, a.nsys, seed=12345 + a.seed, log=log, variants=variants)
+        summary = {"molecules": len(rows), "max_err_meV": max(r["max_err_meV"] for r in rows),
+                   "molecules_over_1meV": sum(r["n_over_1meV"] > 0 for r in rows),
+                   "plain_full": sum(r["plain_full_solves"] for r in rows),
+                   "learned_full": sum(r["learned_full_solves"] for r in rows),
+                   "plain_cycles": sum(r["plain_cycles"] for r in rows),
+                   "learned_cycles": sum(r["learned_cycles"] for r in rows)}
+        if variants:
+            summary.update({"zgate_max_err_meV": max(r["zgate_max_err_meV"] for r in rows),
+                            "zgate_molecules_over_1meV": sum(r["zgate_n_over_1meV"] > 0 for r in rows),
+                            "zgate_full": sum(r["zgate_full_solves"] for r in rows),
+                            "zgate_cycles": sum(r["zgate_cycles"] for r in rows)})
+        print(json.dumps(summary), flush=True)
 
 
 if __name__ == "__main__":
