@@ -305,11 +305,12 @@ int qk_set_graphs(int on);
 /* ---- batched G0W0: many molecules of one size in one handle ------------- */
 
 /* nmol molecules with the same (nBas, nOrb, nC, nO, nV, nR, nS) on one stream:
- * each stage runs for all of them before the next (one strided-batched
- * rocSOLVER eigensolve per RPA, one batched excitation-density GEMM in the
- * FP64 form, one host synchronisation per QP Newton sweep for the batch).
- * GW/RG0W0.f90:88-233 per molecule; no counterpart in QuAcK, which runs one
- * molecule per process.  Callers with uneven sizes group molecules by size. */
+ * each stage runs for all of them before the next, one launch per kernel for
+ * the whole batch (grid z = molecule): one strided-batched rocSOLVER
+ * eigensolve per RPA, one batched excitation-density GEMM in the FP64 form,
+ * one launch pair per QP Newton sweep.  GW/RG0W0.f90:88-233 per molecule; no
+ * counterpart in QuAcK, which runs one molecule per process.  Callers with
+ * uneven sizes group molecules by size. */
 typedef struct qk_gwb_ctx *qk_gwb_handle;
 int qk_gwb_create(qk_gwb_handle *h, int nmol, int nBas, int nOrb, int nC, int nO, int nV, int nR, int nS);
 int qk_gwb_destroy(qk_gwb_handle h);
@@ -325,9 +326,9 @@ int qk_gwb_g0w0(qk_gwb_handle h, const double *eHF, double eta, int linearize, i
 /* [0] AO->MO, [1] eigensolves, [2] excitation densities, [3] self-energy + QP (ms), n <= 4 */
 /* evGW (GW/evRGW.f90:104-200) for every molecule of the batch, cycle by cycle
  * in step: one batched eigensolve, one excitation stage and one QP stage per
- * cycle for the whole batch.  Each molecule keeps its own DIIS history,
- * convergence test and cycle count; one that has converged keeps its last
- * cycle's outputs while the others go on.  Arrays as qk_gwb_g0w0 (nOrb x nmol,
+ * cycle for the whole batch.  Each molecule keeps its own DIIS history
+ * (max_diis <= 32), convergence test and cycle count, all on the device; one
+ * that has converged keeps its last cycle's outputs while the others go on.  Arrays as qk_gwb_g0w0 (nOrb x nmol,
  * nS x nmol), eGW_start NULL or nOrb x nmol (as qk_evrgw), nSCF: nmol.
  * QK_E_NOCONV when some molecule reached maxSCF + 1 cycles (its outputs are
  * those of its last cycle). */
@@ -335,6 +336,18 @@ int qk_gwb_evgw(qk_gwb_handle h, const double *eHF, const double *eGW_start, int
                 double eta, int linearize, int TDA_W, int doSRG, double *eGW, double *SigC, double *Z, double *Om,
                 double *EcRPA, double *EcGM, int *nSCF);
 int qk_gwb_stage_times(qk_gwb_handle h, double *ms, int n);
+/* hipGraph replay of qk_gwb_evgw's cycles (off by default): from the second
+ * cycle of a call, the launches before the batched eigensolve and those after
+ * it (RPA finalisation, excitation densities, self-energy, all 64 Newton
+ * sweeps with finished orbitals masked on the device, the DIIS and
+ * convergence bookkeeping) are replayed from two graphs captured once per set
+ * of cycle parameters; the host reads only the molecules' live flags and the
+ * eigensolver's info after each cycle.  Results are identical with it on or
+ * off.  No counterpart in QuAcK (GW/evRGW.f90:119-183 is the cycle). */
+int qk_gwb_set_graphs(qk_gwb_handle h, int on);
+/* counts[0] = evGW cycles run eagerly, counts[1] = cycles replayed from graphs
+ * (since the handle was created), n <= 2 */
+int qk_gwb_graph_stats(qk_gwb_handle h, long long *counts, int n);
 
 /* ---- learned warm start: per-orbital MLP (no counterpart in QuAcK) ------ */
 

@@ -114,6 +114,11 @@ class Runner:
                 res[i] = {k: (v[j] if np.ndim(v) else v) for k, v in r.items()}
         return res, time.time() - t
 
+    def set_graphs(self, on):
+        """hipGraph replay of the evGW cycles of every group's handle."""
+        for _, b, _, _, _ in self.plan:
+            b.set_graphs(on)
+
     def close(self):
         for _, b, _, _, _ in self.plan:
             b.close()

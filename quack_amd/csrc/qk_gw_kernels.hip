@@ -96,7 +96,11 @@ __global__ void k_delta_u(const double* __restrict__ e, double* __restrict__ dE,
 
 // dE[ia] = e_a - e_i ; sd[ia] = sqrt(dE)
 __global__ void k_delta(const double* __restrict__ e, double* __restrict__ dE, double* __restrict__ sd, int nC, int nO,
-                        int v, int nS) {
+                        int v, int nS, Zs zs) {
+    const long long z = blockIdx.z;
+    e += z * zs.n;
+    dE += z * zs.S;
+    sd += z * zs.S;
     int ia = blockIdx.x * blockDim.x + threadIdx.x;
     if (ia < nS) {
         int a = ia % v, i = ia / v;
@@ -108,7 +112,12 @@ __global__ void k_delta(const double* __restrict__ e, double* __restrict__ dE, d
 
 // TDA: Cm = A = diag(dE) + 2K ; RPA: Cm = (A-B)^1/2 (A+B) (A-B)^1/2 = diag(dE^2) + 4 sd K sd
 __global__ void k_build_c(const double* __restrict__ K, const double* __restrict__ dE, const double* __restrict__ sd,
-                          double* __restrict__ Cm, int nS, int tda) {
+                          double* __restrict__ Cm, int nS, int tda, Zs zs) {
+    const long long z = blockIdx.z;
+    K += z * zs.S2;
+    dE += z * zs.S;
+    sd += z * zs.S;
+    Cm += z * zs.S2;
     long long total = (long long)nS * nS;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
         int ia = (int)(t % nS), jb = (int)(t / nS);
@@ -126,7 +135,11 @@ __global__ void k_build_c(const double* __restrict__ K, const double* __restrict
 }
 
 // out[0] = Tr(A) = sum_ia dE[ia] + 2 K[ia,ia]
-__global__ void k_trace_a(const double* __restrict__ K, const double* __restrict__ dE, int nS, double* out) {
+__global__ void k_trace_a(const double* __restrict__ K, const double* __restrict__ dE, int nS, double* out, Zs zs) {
+    const long long z = blockIdx.z;
+    K += z * zs.S2;
+    dE += z * zs.S;
+    out += z * zs.c;
     __shared__ double sh[RB / 64];
     double s = 0.0;
     for (int ia = threadIdx.x; ia < nS; ia += blockDim.x) s += dE[ia] + 2.0 * K[(long long)ia * nS + ia];
@@ -135,7 +148,10 @@ __global__ void k_trace_a(const double* __restrict__ K, const double* __restrict
 }
 
 // Om = sqrt|lambda| (RPA) or lambda (TDA); out[1] = EcRPA = (sum Om - TrA)/2 with TrA = out[0]
-__global__ void k_finalize_om(double* __restrict__ Om, int nS, int tda, double* out) {
+__global__ void k_finalize_om(double* __restrict__ Om, int nS, int tda, double* out, Zs zs) {
+    const long long z = blockIdx.z;
+    Om += z * zs.S;
+    out += z * zs.c;
     __shared__ double sh[RB / 64];
     double s = 0.0;
     for (int m = threadIdx.x; m < nS; m += blockDim.x) {
@@ -149,7 +165,12 @@ __global__ void k_finalize_om(double* __restrict__ Om, int nS, int tda, double* 
 }
 
 // W[jb + nS m] = Z[jb + nS m] * sd[jb] / sqrt(Om[m])   (so that rho = M W)
-__global__ void k_scale_w(double* __restrict__ W, const double* __restrict__ sd, const double* __restrict__ Om, int nS) {
+__global__ void k_scale_w(double* __restrict__ W, const double* __restrict__ sd, const double* __restrict__ Om, int nS,
+                          Zs zs) {
+    const long long z = blockIdx.z;
+    W += z * zs.S2;
+    sd += z * zs.S;
+    Om += z * zs.S;
     long long total = (long long)nS * nS;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
         int jb = (int)(t % nS), m = (int)(t / nS);
@@ -188,7 +209,17 @@ __global__ __launch_bounds__(RB) void k_sigma_partial(const RhoT* __restrict__ r
                                                       const double* __restrict__ e, const double* __restrict__ Om,
                                                       const int* __restrict__ skip, double eta, int srg, double flow,
                                                       int n, int nC, int nO, int nR, int nS,
-                                                      double* __restrict__ part) {
+                                                      double* __restrict__ part, Zs zs) {
+    {
+        // a float copy of rho occupies the first half of each molecule's double slot
+        const long long z = blockIdx.z;
+        rhoT += z * zs.R * (long long)(sizeof(double) / sizeof(RhoT));
+        w += z * zs.n;
+        e += z * zs.n;
+        Om += z * zs.S;
+        if (skip) skip += z * zs.n;
+        part += z * zs.P;
+    }
     const int p = nC + blockIdx.y;
     const int c = blockIdx.x;
     const int nchunk = gridDim.x;
@@ -261,7 +292,12 @@ __global__ __launch_bounds__(RB) void k_sigma_partial(const RhoT* __restrict__ r
 // zero outside (GW/RGW_self_energy_diag.f90:115-116,191); EcGM = -2 sum_{p virtual} sig_occ(p)
 // (the same eps = e_a - e_i + Om_m as GW/RGW_self_energy_diag.f90:179-181).
 __global__ void k_sigma_finalize(const double* __restrict__ part, int nchunk, int n, int nC, int nO, int nR,
-                                 double* __restrict__ Sig, double* __restrict__ Z, double* out) {
+                                 double* __restrict__ Sig, double* __restrict__ Z, double* out, Zs zs) {
+    const long long z = blockIdx.z;
+    part += z * zs.P;
+    Sig += z * zs.n;
+    Z += z * zs.n;
+    out += z * zs.c;
     __shared__ double sh[RB / 64];
     double gm = 0.0;
     for (int p = threadIdx.x; p < n; p += blockDim.x) {
@@ -299,7 +335,12 @@ __global__ void k_sigma_out(const double* __restrict__ part, int nchunk, int n, 
 }
 
 // eGWlin = e0 + Z*Sig (GW/RG0W0.f90:188) ; or e0 + Sig (evRGW.f90:147, zflag = 0)
-__global__ void k_lin(const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag) {
+__global__ void k_lin(const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag, Zs zs) {
+    const long long z = blockIdx.z;
+    e0 += z * zs.n;
+    Z += z * zs.n;
+    Sig += z * zs.n;
+    out += z * zs.n;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < n) out[p] = e0[p] + (zflag ? Z[p] * Sig[p] : Sig[p]);
 }
@@ -307,7 +348,13 @@ __global__ void k_lin(const double* e0, const double* Z, const double* Sig, doub
 // ---- QP Newton (GW/RGW_QP_graph.f90:50-92) --------------------------------
 
 __global__ void k_qp_init(const double* __restrict__ eGWlin, double* __restrict__ w, double* __restrict__ f,
-                          int* __restrict__ nit, int* __restrict__ done, int n, int nC, int nR) {
+                          int* __restrict__ nit, int* __restrict__ done, int n, int nC, int nR, Zs zs) {
+    const long long z = blockIdx.z;
+    eGWlin += z * zs.n;
+    w += z * zs.n;
+    f += z * zs.n;
+    nit += z * zs.n;
+    done += z * zs.n;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < n) {
         w[p] = eGWlin[p];
@@ -321,7 +368,18 @@ __global__ void k_qp_init(const double* __restrict__ eGWlin, double* __restrict_
 __global__ void k_qp_update(const double* __restrict__ part, int nchunk, const double* __restrict__ eHF,
                             double* __restrict__ w, double* __restrict__ f, double* __restrict__ df,
                             int* __restrict__ nit, int* __restrict__ done, int n, double thresh, int maxit,
-                            int* __restrict__ active) {
+                            int* __restrict__ active, Zs zs) {
+    {
+        const long long z = blockIdx.z;
+        part += z * zs.P;
+        eHF += z * zs.n;
+        w += z * zs.n;
+        f += z * zs.n;
+        df += z * zs.n;
+        nit += z * zs.n;
+        done += z * zs.n;
+        active += z * zs.i;
+    }
     __shared__ double sh[RB / 64];
     double cnt = 0.0;
     for (int p = threadIdx.x; p < n; p += blockDim.x) {
@@ -350,7 +408,16 @@ __global__ void k_qp_update(const double* __restrict__ part, int nchunk, const d
 // eGW(p) = w, Z(p) = df unless nIt == maxIt (then eGW = eGWlin, Z kept)
 __global__ void k_qp_finish(const double* __restrict__ w, const double* __restrict__ df, const int* __restrict__ nit,
                             const double* __restrict__ eGWlin, double* __restrict__ eGW, double* __restrict__ Z, int n,
-                            int nC, int nR, int maxit) {
+                            int nC, int nR, int maxit, Zs zs) {
+    {
+        const long long z = blockIdx.z;
+        w += z * zs.n;
+        df += z * zs.n;
+        nit += z * zs.n;
+        eGWlin += z * zs.n;
+        eGW += z * zs.n;
+        Z += z * zs.n;
+    }
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     if (p < nC || p >= n - nR) {
@@ -362,6 +429,177 @@ __global__ void k_qp_finish(const double* __restrict__ w, const double* __restri
     } else {
         eGW[p] = w[p];
         Z[p] = df[p];
+    }
+}
+
+// ---- evGW bookkeeping on the device (batched evGW cycles) ------------------
+
+constexpr int DIIS_MAX = 32;  // history length the device DIIS supports
+
+// Bordered DIIS system of order N <= DIIS_MAX + 1 (column-major in A), solved
+// by LU with partial pivoting; returns the exact 1-norm reciprocal condition
+// number from the explicit inverse (what LAPACK's dsysvx estimates) and the
+// solution in x, 0 when the matrix is singular.  One thread; LU, piv and col
+// are that thread's work arrays.
+__device__ double diis_solve(int N, const double* A, const double* b, double* x, double* LU, int* piv, double* col) {
+    double anorm = 0.0;
+    for (int j = 0; j < N; ++j) {
+        double sj = 0.0;
+        for (int i = 0; i < N; ++i) sj += fabs(A[i + N * j]);
+        anorm = fmax(anorm, sj);
+    }
+    for (int t = 0; t < N * N; ++t) LU[t] = A[t];
+    for (int k = 0; k < N; ++k) {
+        int pk = k;
+        for (int i = k + 1; i < N; ++i)
+            if (fabs(LU[i + N * k]) > fabs(LU[pk + N * k])) pk = i;
+        piv[k] = pk;
+        if (LU[pk + N * k] == 0.0) {
+            for (int i = 0; i < N; ++i) x[i] = 0.0;
+            return 0.0;
+        }
+        if (pk != k)
+            for (int j = 0; j < N; ++j) {
+                const double t = LU[k + N * j];
+                LU[k + N * j] = LU[pk + N * j];
+                LU[pk + N * j] = t;
+            }
+        for (int i = k + 1; i < N; ++i) {
+            LU[i + N * k] /= LU[k + N * k];
+            for (int j = k + 1; j < N; ++j) LU[i + N * j] -= LU[i + N * k] * LU[k + N * j];
+        }
+    }
+    auto lusolve = [&](double* r) {
+        for (int k = 0; k < N; ++k) {
+            const double t = r[k];
+            r[k] = r[piv[k]];
+            r[piv[k]] = t;
+        }
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < i; ++j) r[i] -= LU[i + N * j] * r[j];
+        for (int i = N - 1; i >= 0; --i) {
+            for (int j = i + 1; j < N; ++j) r[i] -= LU[i + N * j] * r[j];
+            r[i] /= LU[i + N * i];
+        }
+    };
+    double inorm = 0.0;
+    for (int j = 0; j < N; ++j) {
+        for (int i = 0; i < N; ++i) col[i] = i == j ? 1.0 : 0.0;
+        lusolve(col);
+        double sj = 0.0;
+        for (int i = 0; i < N; ++i) sj += fabs(col[i]);
+        inorm = fmax(inorm, sj);
+    }
+    for (int i = 0; i < N; ++i) x[i] = b[i];
+    lusolve(x);
+    if (!(anorm > 0.0) || !(inorm > 0.0) || !isfinite(inorm)) return 0.0;
+    return 1.0 / (anorm * inorm);
+}
+
+// One evGW cycle's bookkeeping for molecule blockIdx.x (GW/evRGW.f90:156-183),
+// so that a whole cycle can run without the host: for a live molecule the
+// cycle's outputs are kept (Sig, Z, Om; EcRPA and EcGM from scal[1], scal[2]),
+// Conv = max |eNew - e|, DIIS when maxd > 1 (utils/DIIS_extrapolation.f90:
+// history prepended, bordered system, extrapolation only when rcond > 1e-13),
+// e <- eNew (extrapolated), nscf += 1, live <- Conv > thresh and nscf <= maxSCF.
+// A finished molecule is left as it is.  hist holds per molecule the error and
+// energy histories (n x maxd each).
+__global__ __launch_bounds__(RB) void k_evgw_update(const double* __restrict__ eNew, double* __restrict__ e,
+                                                    const double* __restrict__ Sig, const double* __restrict__ Z,
+                                                    const double* __restrict__ Om, const double* __restrict__ scal,
+                                                    double* __restrict__ hist, int* __restrict__ state,
+                                                    double* __restrict__ conv, double* __restrict__ outSig,
+                                                    double* __restrict__ outZ, double* __restrict__ outOm,
+                                                    double* __restrict__ outEc, int n, int nS, int maxd,
+                                                    double thresh, int maxSCF, Zs zs) {
+    const long long m = blockIdx.x;
+    int* st = state + 4 * m;  // [0] live, [1] nscf, [2] history length
+    if (!st[0]) return;
+    eNew += m * zs.n;
+    e += m * zs.n;
+    Sig += m * zs.n;
+    Z += m * zs.n;
+    Om += m * zs.S;
+    scal += m * zs.c;
+    outSig += m * zs.n;
+    outZ += m * zs.n;
+    outOm += m * zs.S;
+    double* errH = hist + m * 2LL * n * maxd;
+    double* eH = errH + (long long)n * maxd;
+    __shared__ double sh[RB / 64];
+    __shared__ double A[(DIIS_MAX + 1) * (DIIS_MAX + 1)], LU[(DIIS_MAX + 1) * (DIIS_MAX + 1)];
+    __shared__ double rhs[DIIS_MAX + 1], x[DIIS_MAX + 1], col[DIIS_MAX + 1];
+    __shared__ int piv[DIIS_MAX + 1];
+    __shared__ int use_x;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        outSig[p] = Sig[p];
+        outZ[p] = Z[p];
+    }
+    for (int k = threadIdx.x; k < nS; k += blockDim.x) outOm[k] = Om[k];
+    if (threadIdx.x == 0) {
+        outEc[2 * m] = scal[1];
+        outEc[2 * m + 1] = scal[2];
+    }
+    double cmax = 0.0;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) cmax = fmax(cmax, fabs(eNew[p] - e[p]));
+    // block maximum through the sum reduction's layout: |x| >= 0, so reduce max by shuffles
+    for (int off = 32; off > 0; off >>= 1) cmax = fmax(cmax, __shfl_down(cmax, off, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = cmax;
+    __syncthreads();
+    double cv = 0.0;
+    for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) cv = fmax(cv, sh[k]);
+    __syncthreads();
+    if (maxd > 1) {
+        const int md = min(st[2] + 1, maxd);
+        for (int p = threadIdx.x; p < n; p += blockDim.x) {
+            for (int j = md - 2; j >= 0; --j) {
+                errH[p + (long long)n * (j + 1)] = errH[p + (long long)n * j];
+                eH[p + (long long)n * (j + 1)] = eH[p + (long long)n * j];
+            }
+            errH[p] = eNew[p] - e[p];
+            eH[p] = eNew[p];
+        }
+        const int N = md + 1;
+        for (int i = 0; i < md; ++i)
+            for (int j = i; j < md; ++j) {
+                double d = 0.0;
+                for (int p = threadIdx.x; p < n; p += blockDim.x)
+                    d += errH[p + (long long)n * i] * errH[p + (long long)n * j];
+                const double r = block_sum(d, sh);
+                if (threadIdx.x == 0) {
+                    A[i + N * j] = r;
+                    A[j + N * i] = r;
+                }
+            }
+        if (threadIdx.x == 0) {
+            for (int i = 0; i < md; ++i) {
+                A[i + N * md] = -1.0;
+                A[md + N * i] = -1.0;
+                rhs[i] = 0.0;
+            }
+            A[md + N * md] = 0.0;
+            rhs[md] = -1.0;
+            const double rc = diis_solve(N, A, rhs, x, LU, piv, col);
+            use_x = rc > 1e-13;
+            st[2] = md;
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < n; p += blockDim.x) {
+            double v = eNew[p];
+            if (use_x) {
+                v = 0.0;
+                for (int j = 0; j < md; ++j) v += x[j] * eH[p + (long long)n * j];
+            }
+            e[p] = v;
+        }
+    } else {
+        for (int p = threadIdx.x; p < n; p += blockDim.x) e[p] = eNew[p];
+    }
+    if (threadIdx.x == 0) {
+        conv[m] = cv;
+        st[1] += 1;
+        st[0] = (cv > thresh && st[1] <= maxSCF) ? 1 : 0;
     }
 }
 
@@ -663,14 +901,15 @@ void extract_k(hipStream_t s, const double* M, double* K, int n, int nC, int nO,
     QK_HIP(hipGetLastError());
 }
 void rpa_build(hipStream_t s, const double* e, const double* K, double* dE, double* sd, double* Cm, double* scal,
-               int nC, int nO, int v, int nS, int tda) {
-    hipLaunchKernelGGL(k_delta, dim3((nS + 255) / 256), dim3(256), 0, s, e, dE, sd, nC, nO, v, nS);
-    rpa_build_c(s, K, dE, sd, Cm, scal, nS, tda);
+               int nC, int nO, int v, int nS, int tda, const Zs& zs, int nz) {
+    hipLaunchKernelGGL(k_delta, dim3((nS + 255) / 256, 1, nz), dim3(256), 0, s, e, dE, sd, nC, nO, v, nS, zs);
+    rpa_build_c(s, K, dE, sd, Cm, scal, nS, tda, zs, nz);
 }
 void rpa_build_c(hipStream_t s, const double* K, const double* dE, const double* sd, double* Cm, double* scal, int nS,
-                 int tda) {
-    hipLaunchKernelGGL(k_build_c, dim3(grid_for((long long)nS * nS)), dim3(256), 0, s, K, dE, sd, Cm, nS, tda);
-    hipLaunchKernelGGL(k_trace_a, dim3(1), dim3(RB), 0, s, K, dE, nS, scal);
+                 int tda, const Zs& zs, int nz) {
+    hipLaunchKernelGGL(k_build_c, dim3(grid_for((long long)nS * nS), 1, nz), dim3(256), 0, s, K, dE, sd, Cm, nS, tda,
+                       zs);
+    hipLaunchKernelGGL(k_trace_a, dim3(1, 1, nz), dim3(RB), 0, s, K, dE, nS, scal, zs);
     QK_HIP(hipGetLastError());
 }
 void extract_ku(hipStream_t s, const double* M0, const double* M1, double* K, int n, int nOa, int nOb) {
@@ -684,10 +923,11 @@ void delta_u(hipStream_t s, const double* e, double* dE, double* sd, int n, int 
     hipLaunchKernelGGL(k_delta_u, dim3((nSt + 255) / 256), dim3(256), 0, s, e, dE, sd, n, nOa, nOb, nSa, nSt);
     QK_HIP(hipGetLastError());
 }
-void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w) {
-    hipLaunchKernelGGL(k_finalize_om, dim3(1), dim3(RB), 0, s, Om, nS, tda, scal);
+void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w,
+                  const Zs& zs, int nz) {
+    hipLaunchKernelGGL(k_finalize_om, dim3(1, 1, nz), dim3(RB), 0, s, Om, nS, tda, scal, zs);
     if (scale_w && !tda)
-        hipLaunchKernelGGL(k_scale_w, dim3(grid_for((long long)nS * nS)), dim3(256), 0, s, W, sd, Om, nS);
+        hipLaunchKernelGGL(k_scale_w, dim3(grid_for((long long)nS * nS), 1, nz), dim3(256), 0, s, W, sd, Om, nS, zs);
     QK_HIP(hipGetLastError());
 }
 int sigma_chunks(int nS, int nact) {
@@ -696,20 +936,22 @@ int sigma_chunks(int nS, int nact) {
 }
 void sigma_partial(hipStream_t s, const void* rhoT, bool rho_f32, const double* w, const double* e, const double* Om,
                    const int* skip, double eta, int srg, double flow, int n, int nC, int nO, int nR, int nS,
-                   int nchunk, double* part) {
+                   int nchunk, double* part, const Zs& zs, int nz) {
     int nact = n - nR - nC;
     QK_CHECK((long long)nchunk * RB * SIG_MPT >= nS, QK_E_ARG, "sigma_partial: too few chunks");
     if (rho_f32)
-        hipLaunchKernelGGL(k_sigma_partial<float>, dim3(nchunk, nact), dim3(RB), 0, s,
-                           static_cast<const float*>(rhoT), w, e, Om, skip, eta, srg, flow, n, nC, nO, nR, nS, part);
+        hipLaunchKernelGGL(k_sigma_partial<float>, dim3(nchunk, nact, nz), dim3(RB), 0, s,
+                           static_cast<const float*>(rhoT), w, e, Om, skip, eta, srg, flow, n, nC, nO, nR, nS, part,
+                           zs);
     else
-        hipLaunchKernelGGL(k_sigma_partial<double>, dim3(nchunk, nact), dim3(RB), 0, s,
-                           static_cast<const double*>(rhoT), w, e, Om, skip, eta, srg, flow, n, nC, nO, nR, nS, part);
+        hipLaunchKernelGGL(k_sigma_partial<double>, dim3(nchunk, nact, nz), dim3(RB), 0, s,
+                           static_cast<const double*>(rhoT), w, e, Om, skip, eta, srg, flow, n, nC, nO, nR, nS, part,
+                           zs);
     QK_HIP(hipGetLastError());
 }
 void sigma_finalize(hipStream_t s, const double* part, int nchunk, int n, int nC, int nO, int nR, double* Sig,
-                    double* Z, double* scal) {
-    hipLaunchKernelGGL(k_sigma_finalize, dim3(1), dim3(RB), 0, s, part, nchunk, n, nC, nO, nR, Sig, Z, scal);
+                    double* Z, double* scal, const Zs& zs, int nz) {
+    hipLaunchKernelGGL(k_sigma_finalize, dim3(1, 1, nz), dim3(RB), 0, s, part, nchunk, n, nC, nO, nR, Sig, Z, scal, zs);
     QK_HIP(hipGetLastError());
 }
 void sigma_out(hipStream_t s, const double* part, int nchunk, int n, int nC, int nR, double* Sig, double* dSig) {
@@ -867,24 +1109,36 @@ void sigma_full(hipStream_t s, const double* rhoT, const double* e, const double
     QK_HIP(hipGetLastError());
 }
 
-void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag) {
-    hipLaunchKernelGGL(k_lin, dim3((n + 255) / 256), dim3(256), 0, s, e0, Z, Sig, out, n, zflag);
+void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag,
+            const Zs& zs, int nz) {
+    hipLaunchKernelGGL(k_lin, dim3((n + 255) / 256, 1, nz), dim3(256), 0, s, e0, Z, Sig, out, n, zflag, zs);
     QK_HIP(hipGetLastError());
 }
-void qp_init(hipStream_t s, const double* eGWlin, double* w, double* f, int* nit, int* done, int n, int nC, int nR) {
-    hipLaunchKernelGGL(k_qp_init, dim3((n + 255) / 256), dim3(256), 0, s, eGWlin, w, f, nit, done, n, nC, nR);
+void qp_init(hipStream_t s, const double* eGWlin, double* w, double* f, int* nit, int* done, int n, int nC, int nR,
+             const Zs& zs, int nz) {
+    hipLaunchKernelGGL(k_qp_init, dim3((n + 255) / 256, 1, nz), dim3(256), 0, s, eGWlin, w, f, nit, done, n, nC, nR,
+                       zs);
     QK_HIP(hipGetLastError());
 }
 void qp_update(hipStream_t s, const double* part, int nchunk, const double* eHF, double* w, double* f, double* df,
-               int* nit, int* done, int n, double thresh, int maxit, int* active) {
-    hipLaunchKernelGGL(k_qp_update, dim3(1), dim3(RB), 0, s, part, nchunk, eHF, w, f, df, nit, done, n, thresh, maxit,
-                       active);
+               int* nit, int* done, int n, double thresh, int maxit, int* active, const Zs& zs, int nz) {
+    hipLaunchKernelGGL(k_qp_update, dim3(1, 1, nz), dim3(RB), 0, s, part, nchunk, eHF, w, f, df, nit, done, n, thresh,
+                       maxit, active, zs);
     QK_HIP(hipGetLastError());
 }
 void qp_finish(hipStream_t s, const double* w, const double* df, const int* nit, const double* eGWlin, double* eGW,
-               double* Z, int n, int nC, int nR, int maxit) {
-    hipLaunchKernelGGL(k_qp_finish, dim3((n + 255) / 256), dim3(256), 0, s, w, df, nit, eGWlin, eGW, Z, n, nC, nR,
-                       maxit);
+               double* Z, int n, int nC, int nR, int maxit, const Zs& zs, int nz) {
+    hipLaunchKernelGGL(k_qp_finish, dim3((n + 255) / 256, 1, nz), dim3(256), 0, s, w, df, nit, eGWlin, eGW, Z, n, nC,
+                       nR, maxit, zs);
+    QK_HIP(hipGetLastError());
+}
+void evgw_update(hipStream_t s, const double* eNew, double* e, const double* Sig, const double* Z, const double* Om,
+                 const double* scal, double* hist, int* state, double* conv, double* outSig, double* outZ,
+                 double* outOm, double* outEc, int n, int nS, int maxd, double thresh, int maxSCF, const Zs& zs,
+                 int nmol) {
+    QK_CHECK(maxd >= 1 && maxd <= DIIS_MAX, QK_E_ARG, "evgw_update: DIIS history longer than 32");
+    hipLaunchKernelGGL(k_evgw_update, dim3(nmol), dim3(RB), 0, s, eNew, e, Sig, Z, Om, scal, hist, state, conv, outSig,
+                       outZ, outOm, outEc, n, nS, maxd, thresh, maxSCF, zs);
     QK_HIP(hipGetLastError());
 }
 void transpose(hipStream_t s, const double* in, double* out, int rows, int cols, long long ld_in, long long ld_out,

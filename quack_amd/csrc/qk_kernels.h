@@ -6,6 +6,16 @@
 
 namespace qk {
 
+// Strides (in elements) between the molecules of a molecule-batched launch,
+// whose grid z index is the molecule: n-vectors (orbital energies, Sig, Z,
+// the Newton state, done/nit flags), nS-vectors (Om, dE, sd), nS x nS
+// matrices (K, the RPA matrix and its eigenvectors), excitation densities
+// (n^2 nS doubles), self-energy partial sums, the scalar block (8 doubles)
+// and the one-int counters.  Zero, with one molecule, for a single engine.
+struct Zs {
+    long long n = 0, S = 0, S2 = 0, R = 0, P = 0, c = 0, i = 0;
+};
+
 void permute_block(hipStream_t s, const double* X, double* M, int n, int o, int v);
 // f (N^4, QuAcK's ERI.bin order) from the 8-fold packed chemists' array
 // (N(N+1)/2 pairs, pair-of-pairs packed)
@@ -13,35 +23,48 @@ void unpack_eri_s8(hipStream_t s, const double* packed, double* f, int N);
 void extract_k(hipStream_t s, const double* M, double* K, int n, int nC, int nO, int v, int nS);
 // scal[0] = Tr(A), scal[1] = EcRPA, scal[2] = EcGM
 void rpa_build(hipStream_t s, const double* e, const double* K, double* dE, double* sd, double* Cm, double* scal,
-               int nC, int nO, int v, int nS, int tda);
+               int nC, int nO, int v, int nS, int tda, const Zs& zs = Zs(), int nz = 1);
 // rpa_build without the pair energies: dE, sd given
 void rpa_build_c(hipStream_t s, const double* K, const double* dE, const double* sd, double* Cm, double* scal, int nS,
-                 int tda);
+                 int tda, const Zs& zs = Zs(), int nz = 1);
 // Unrestricted pairs (nC = nR = 0; alpha block, then beta): K = (ia|jb) / 2
 // from the spin slabs M_s[p + n q + n^2 jb] = (pq|jb), and the pair energies
 // from e = (e_alpha, e_beta)
 void extract_ku(hipStream_t s, const double* M0, const double* M1, double* K, int n, int nOa, int nOb);
 void delta_u(hipStream_t s, const double* e, double* dE, double* sd, int n, int nOa, int nOb);
-void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w);
+void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w,
+                  const Zs& zs = Zs(), int nz = 1);
 int sigma_chunks(int nS, int nact);
 // rhoT holds doubles, or floats when rho_f32
 void sigma_partial(hipStream_t s, const void* rhoT, bool rho_f32, const double* w, const double* e, const double* Om,
                    const int* skip, double eta, int srg, double flow, int n, int nC, int nO, int nR, int nS,
-                   int nchunk, double* part);
+                   int nchunk, double* part, const Zs& zs = Zs(), int nz = 1);
 void sigma_finalize(hipStream_t s, const double* part, int nchunk, int n, int nC, int nO, int nR, double* Sig,
-                    double* Z, double* scal);
+                    double* Z, double* scal, const Zs& zs = Zs(), int nz = 1);
 void sigma_out(hipStream_t s, const double* part, int nchunk, int n, int nC, int nR, double* Sig, double* dSig);
-void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag);
+void lin_qp(hipStream_t s, const double* e0, const double* Z, const double* Sig, double* out, int n, int zflag,
+            const Zs& zs = Zs(), int nz = 1);
 // work: nS n^2 + n^3 doubles for the plain weight's matrix-core path (NULL: the direct kernel)
 void sigma_full(hipStream_t s, const double* rhoT, const double* e, const double* Om, double eta, int srg, double flow,
                 int n, int nC, int nO, int nR, int nS, double* Sig, double* work);
 void rho_block(hipStream_t s, const double* rhoT, const double* Om, double eta, int weighted, int n, int nS, int p0,
                int P, int q0, int Q, double* out);
-void qp_init(hipStream_t s, const double* eGWlin, double* w, double* f, int* nit, int* done, int n, int nC, int nR);
+void qp_init(hipStream_t s, const double* eGWlin, double* w, double* f, int* nit, int* done, int n, int nC, int nR,
+             const Zs& zs = Zs(), int nz = 1);
 void qp_update(hipStream_t s, const double* part, int nchunk, const double* eHF, double* w, double* f, double* df,
-               int* nit, int* done, int n, double thresh, int maxit, int* active);
+               int* nit, int* done, int n, double thresh, int maxit, int* active, const Zs& zs = Zs(), int nz = 1);
 void qp_finish(hipStream_t s, const double* w, const double* df, const int* nit, const double* eGWlin, double* eGW,
-               double* Z, int n, int nC, int nR, int maxit);
+               double* Z, int n, int nC, int nR, int maxit, const Zs& zs = Zs(), int nz = 1);
+// One evGW cycle's bookkeeping for every molecule of a batch (GW/evRGW.f90:156-183,
+// DIIS of utils/DIIS_extrapolation.f90), on the device: state[4 m + {0, 1, 2}] =
+// live flag, cycles run, DIIS history length; hist: 2 n maxd doubles per
+// molecule; conv[m] = Conv of the cycle; outSig/outZ/outOm/outEc (EcRPA, EcGM)
+// keep the live molecules' outputs of the cycle; e <- the next cycle's
+// energies.  maxd <= 32.
+void evgw_update(hipStream_t s, const double* eNew, double* e, const double* Sig, const double* Z, const double* Om,
+                 const double* scal, double* hist, int* state, double* conv, double* outSig, double* outZ,
+                 double* outOm, double* outEc, int n, int nS, int maxd, double thresh, int maxSCF, const Zs& zs,
+                 int nmol);
 void transpose(hipStream_t s, const double* in, double* out, int rows, int cols, long long ld_in, long long ld_out,
                int batch = 1, long long bs_in = 0, long long bs_out = 0);
 void apb_amb(hipStream_t s, const double* A, const double* B, double* ApB, double* AmB, long long total);

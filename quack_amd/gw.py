@@ -33,7 +33,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
            "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms",
            "qk_rgw_qp_graph_noalloc", "qk_noalloc_last_error", "qk_gw_qp_iters",
-           "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats"]
+           "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats", "qk_gwb_set_graphs", "qk_gwb_graph_stats"]
 
 
 EIG_MODES = {"syevd": 0, "refine": 1, "sytrd": 2, "refine+sytrd": 3}
@@ -138,6 +138,8 @@ def load():
     lib.qk_ug0w0.argtypes = [_I, _I, _I, _P, _P, _P, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_evugw.argtypes = [_I, _I, _I, _P, _P, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
     lib.qk_set_unrestricted_eigensolver.argtypes = [_I, _D]
+    lib.qk_gwb_set_graphs.argtypes = [_H, _I]
+    lib.qk_gwb_graph_stats.argtypes = [_H, ctypes.POINTER(ctypes.c_longlong), _I]
     lib.qk_unrestricted_eig_stats.argtypes = [ctypes.POINTER(ctypes.c_longlong), _I]
     lib.qk_rhf_create.argtypes = [ctypes.POINTER(_H), _I]
     lib.qk_rhf_destroy.argtypes = [_H]
@@ -379,6 +381,15 @@ class GWBatch:
 
     def set_precision(self, mode):
         self._check(load().qk_gwb_set_precision(self.h, PRECISION.get(mode, mode)))
+
+    def set_graphs(self, on):
+        """hipGraph replay of evgw()'s cycles from the second cycle of a call."""
+        self._check(load().qk_gwb_set_graphs(self.h, int(bool(on))))
+
+    def graph_stats(self):
+        c = (ctypes.c_longlong * 2)()
+        self._check(load().qk_gwb_graph_stats(self.h, c, 2))
+        return {"eager_cycles": c[0], "graph_cycles": c[1]}
 
     def set_eri_ao_device(self, eri_ptr, stride_eri, c_ptr, stride_c):
         """Device pointers: molecule m's AO ERIs at eri_ptr + 8*m*stride_eri bytes

@@ -86,6 +86,33 @@ def test_df_mixed_within_1meV():
     assert np.array_equal(out["mixed"][0]["Om"], out["mixed2"][0]["Om"])
 
 
+def test_df600_precision_form_within_1meV():
+    """The precision form's full configuration (N = 600, o = 60, nS = 32400,
+    naux = 1800, seed 0, factors generated on the device): SRG-G0W0 in the
+    two-slice split-precision form within 1 meV of FP64 on every orbital, and
+    its RPA energies within 1e-7 Ha ((ia|jb) keeps three BF16 slices).  About
+    70 s: each nS = 32400 eigensolve takes ~20 s."""
+    import torch
+
+    from quack_amd import dfgen, gw
+    say("df600 precision form vs fp64")
+    n, nO, naux = 600, 60, 1800
+    e, Bt, _ = dfgen.synthetic_df(n, nO, naux, seed=0, device="cuda")
+    out = {}
+    with gw.GWEngine(n, n, nO) as g:
+        for form in ("fp64", "mixed2"):
+            g.set_precision(form)
+            g.set_eri_df_device(Bt.data_ptr(), naux)
+            out[form] = g.g0w0(e, doSRG=True)
+            say(" ", form, "done")
+    torch.cuda.synchronize()
+    d = np.abs(out["mixed2"]["eGW"] - out["fp64"]["eGW"])
+    dom = np.abs(out["mixed2"]["Om"] - out["fp64"]["Om"]).max()
+    say("  max |dE| %.2e meV, max |dOm| %.2e Ha" % (d.max() / MEV, dom))
+    assert d.max() < MEV
+    assert dom < 1e-7
+
+
 def test_df_device_layout_matches_host(small):
     """The device-resident layout [q, p, P] (P fastest) is the host array's."""
     import torch
