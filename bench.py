@@ -323,7 +323,7 @@ def main():
                          "lattice); a step is one such batch")
     ap.add_argument("--learned", choices=["on", "off"], default="on",
                     help="on: the timed form starts evGW from the MLP's prediction (quack_amd/learned.py)")
-    ap.add_argument("--baseline-reps", type=int, default=3, help="runs of the plain FP64 form (median reported)")
+    ap.add_argument("--baseline-reps", type=int, default=5, help="runs of the plain FP64 form (median reported)")
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--ugw-atoms", type=int, default=10, help="ugw workload: H chain length (one more alpha electron)")
     ap.add_argument("--workload", choices=["h50", "df600", "batch", "learned", "ugw", "cpu250"], default="h50",

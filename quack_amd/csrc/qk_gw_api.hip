@@ -28,6 +28,7 @@ namespace {
 
 constexpr int QP_MAXIT = 64;       // GW/RGW_QP_graph.f90:31
 constexpr double QP_THRESH = 1e-6; // GW/RGW_QP_graph.f90:32
+constexpr int QP_SYNC_EVERY = 4;   // Newton sweeps issued between host reads of the active count
 constexpr double SRG_FLOW = 500.0; // GW/RG0W0.f90:98
 
 thread_local std::string g_last_error;
@@ -851,11 +852,17 @@ void qp_graph(Ctx* c, const double* e0, const double* eG, const double* start, d
     Stage st(c, 3);
     int n = c->nOrb;
     qp_init(c->s, start, c->w, c->f, c->nit, c->done, n, c->nC, c->nR);
+    // the host reads the active count after every QP_SYNC_EVERY sweeps: the
+    // sweeps issued after the last orbital finished are masked no-ops, so the
+    // results are those of a read after every sweep
     int h_active = 1;
-    for (int it = 0; it < QP_MAXIT && h_active > 0; ++it) {
-        sigma_partial(c->s, c->rhoT, c->rho_f32, c->w, eG, c->Om, c->done, eta, srg, flow, n, c->nC, c->nO, c->nR, c->nS,
-                      c->nchunk, c->part);
-        qp_update(c->s, c->part, c->nchunk, e0, c->w, c->f, c->df, c->nit, c->done, n, QP_THRESH, QP_MAXIT, c->active);
+    for (int it = 0; it < QP_MAXIT && h_active > 0;) {
+        for (int k = 0; k < QP_SYNC_EVERY && it < QP_MAXIT; ++k, ++it) {
+            sigma_partial(c->s, c->rhoT, c->rho_f32, c->w, eG, c->Om, c->done, eta, srg, flow, n, c->nC, c->nO, c->nR,
+                          c->nS, c->nchunk, c->part);
+            qp_update(c->s, c->part, c->nchunk, e0, c->w, c->f, c->df, c->nit, c->done, n, QP_THRESH, QP_MAXIT,
+                      c->active);
+        }
         QK_HIP(hipMemcpyAsync(&h_active, c->active, sizeof(int), hipMemcpyDeviceToHost, c->s));
         QK_HIP(hipStreamSynchronize(c->s));
     }
@@ -1727,7 +1734,8 @@ void sigma_dev(hipStream_t s, QpWork& q, const double* rT, const double* dO, int
 }
 
 // Graphical QP solve (GW/RGW_QP_graph.f90:50-92) from host energies, device rT
-// and Om; the Newton sweeps stop as soon as no orbital is left active.  Z is
+// and Om; the Newton sweeps stop once no orbital is left active (read every
+// QP_SYNC_EVERY sweeps).  Z is
 // read (frozen orbitals keep it) and written on the host.
 void qp_dev(hipStream_t s, QpWork& q, const double* rT, const double* dO, int doSRG, double eta, double flow, int nC,
             int nO, int nR, int nS, const double* eHF, const double* eGWlin, const double* eOld, double* eGW,
@@ -1739,9 +1747,11 @@ void qp_dev(hipStream_t s, QpWork& q, const double* rT, const double* dO, int do
     QK_HIP(hipMemcpyAsync(q.z, Z, sizeof(double) * n, hipMemcpyHostToDevice, s));
     qp_init(s, q.eL, q.w, q.f, q.nit, q.done, n, nC, nR);
     int h_active = 1;
-    for (int it = 0; it < QP_MAXIT && h_active > 0; ++it) {
-        sigma_partial(s, rT, false, q.w, q.eG, dO, q.done, eta, doSRG, flow, n, nC, nO, nR, nS, q.nch, q.part);
-        qp_update(s, q.part, q.nch, q.e0, q.w, q.f, q.df, q.nit, q.done, n, QP_THRESH, QP_MAXIT, q.act);
+    for (int it = 0; it < QP_MAXIT && h_active > 0;) {
+        for (int k = 0; k < QP_SYNC_EVERY && it < QP_MAXIT; ++k, ++it) {
+            sigma_partial(s, rT, false, q.w, q.eG, dO, q.done, eta, doSRG, flow, n, nC, nO, nR, nS, q.nch, q.part);
+            qp_update(s, q.part, q.nch, q.e0, q.w, q.f, q.df, q.nit, q.done, n, QP_THRESH, QP_MAXIT, q.act);
+        }
         QK_HIP(hipMemcpyAsync(&h_active, q.act, sizeof(int), hipMemcpyDeviceToHost, s));
         QK_HIP(hipStreamSynchronize(s));
     }
