@@ -222,9 +222,19 @@ def make_data_device(out, nsys, seed=0, log=None, t_max=1e9):
     return out
 
 
+def load_data(data):
+    """X, y and per-molecule meta of one make-data file, or of several
+    (comma-separated paths, each made with its own seed) concatenated."""
+    parts = [np.load(p) for p in data.split(",")]
+    for d in parts:
+        assert json.loads(str(d["method"])) == METHOD, "training data of another method"
+    return (np.concatenate([d["X"] for d in parts]), np.concatenate([d["y"] for d in parts]),
+            np.concatenate([d["meta"] for d in parts]))
+
+
 def train(data, out, epochs=600, width=128, depth=3, lr=1e-3, batch=512, seed=0):
-    d = np.load(data)
-    X, y = d["X"].astype(np.float64), d["y"].astype(np.float64)
+    X, y, meta = load_data(data)
+    X, y = X.astype(np.float64), y.astype(np.float64)
     mean, std = X.mean(0), X.std(0) + 1e-12
     ys = y.std() + 1e-12
     Xn = ((X - mean) / std).astype(np.float32)
@@ -240,8 +250,8 @@ def train(data, out, epochs=600, width=128, depth=3, lr=1e-3, batch=512, seed=0)
     pv = net.forward(Xn[va], bf16=True) * ys
     err = np.abs(pv - y[va])
     stats = {"train_loss_first": float(hist[0]), "train_loss_last": float(hist[-1]), "epochs": epochs,
-             "n_train": int(len(tr)), "n_val": int(nval), "n_molecules": int(len(d["meta"])),
-             "largest_nBas": int(d["meta"][:, 0].max()), "val_mae_Ha": float(err.mean()),
+             "n_train": int(len(tr)), "n_val": int(nval), "n_molecules": int(len(meta)),
+             "largest_nBas": int(meta[:, 0].max()), "val_mae_Ha": float(err.mean()),
              "val_p99_Ha": float(np.quantile(err, 0.99)), "val_max_Ha": float(err.max()),
              "val_mae_first_cycle_Ha": float(np.mean(np.abs(y[va]))), "train_seconds": t_train}
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
@@ -373,7 +383,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["make-data", "train", "eval"])
     ap.add_argument("--out", default="data.npz")
-    ap.add_argument("--data", default="data.npz")
+    ap.add_argument("--data", default="data.npz", help="train: one make-data file or several, comma-separated")
     ap.add_argument("--model", default=DEFAULT_MODEL)
     ap.add_argument("--nsys", type=int, default=1024)
     ap.add_argument("--epochs", type=int, default=600)
