@@ -302,6 +302,15 @@ int qk_autotune_report(char *buf, int len);
  * launch each.  Results are identical either way.  No counterpart in QuAcK. */
 int qk_set_graphs(int on);
 
+/* Split-precision AO->MO block (process-wide, off by default; applies to the
+ * qk_gw_set_eri_ao* and qk_gwb_set_eri_ao_device calls made while it is on):
+ * the last two quarter transforms of the (pq|jb) block (AOtoMO_ERI_RHF.f90:31-51
+ * restricted to the block) run as three-slice BF16 products with FP32
+ * accumulation over 256-term k-chunks and FP64 recombination, and (ia|jb),
+ * which sets the RPA energies, comes from a separate FP64 transform of the
+ * occupied-virtual block.  No counterpart in QuAcK. */
+int qk_set_split_aotomo(int on);
+
 /* ---- batched G0W0: many molecules of one size in one handle ------------- */
 
 /* nmol molecules with the same (nBas, nOrb, nC, nO, nV, nR, nS) on one stream:

@@ -81,6 +81,35 @@ __global__ void k_split3_rc(const double* __restrict__ src, long long ld, int ro
     }
 }
 
+// Row blocks: src(r, k) = src[r' + ld*k + bs*b] with r = r' + nr*b (r' < nr
+// contiguous within block b), 32x32 tiles through LDS, grid z = block.
+__global__ void k_split3_blk(const double* __restrict__ src, long long ld, long long bs, int nr, int K, int kpad,
+                             __bf16* __restrict__ s0, __bf16* __restrict__ s1, __bf16* __restrict__ s2) {
+    __shared__ double t[32][33];
+    const long long b = blockIdx.z;
+    src += b * bs;
+    int r0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    for (int kk = threadIdx.y; kk < 32; kk += blockDim.y) {
+        int r = r0 + threadIdx.x, k = k0 + kk;
+        t[kk][threadIdx.x] = (r < nr && k < K) ? src[r + ld * k] : 0.0;
+    }
+    __syncthreads();
+    for (int rr = threadIdx.y; rr < 32; rr += blockDim.y) {
+        int r = r0 + rr, k = k0 + threadIdx.x;
+        if (r < nr && k < kpad) {
+            double x = t[threadIdx.x][rr];
+            __bf16 a = (__bf16)(float)x;
+            double y = x - (double)(float)a;
+            __bf16 c = (__bf16)(float)y;
+            double z = y - (double)(float)c;
+            long long o = (r + nr * b) * (long long)kpad + k;
+            s0[o] = a;
+            s1[o] = c;
+            if (s2) s2[o] = (__bf16)(float)z;
+        }
+    }
+}
+
 // Workgroup tile BM x BN = 128 x 128, 4 wavefronts as 2 x 2, each 64 x 64 =
 // 4 x 4 MFMA tiles; a 32-wide k-slab of the three slices of its A rows and B
 // rows is 48 KB of LDS.
@@ -270,6 +299,18 @@ void split3(hipStream_t s, const double* src, long long ld, bool k_contiguous, i
         QK_CHECK(g.y <= 65535, QK_E_ARG, "split3: K too large");
         hipLaunchKernelGGL(k_split3_rc, g, dim3(32, 8), 0, s, src, ld, rows, K, out.kpad, out.s[0], out.s[1], s2);
     }
+    QK_HIP(hipGetLastError());
+}
+
+void split3_blocks(hipStream_t s, const double* src, long long ld, long long bs, int nr, int nb, int K,
+                   Split3& out) {
+    QK_CHECK(out.s[0] && out.s[1] && (out.nsl == 2 || (out.nsl == 3 && out.s[2])) &&
+                 (long long)out.rows == (long long)nr * nb && out.kpad >= K && out.kpad % KS == 0,
+             QK_E_ARG, "split3_blocks: bad target");
+    dim3 g((nr + 31) / 32, (out.kpad + 31) / 32, nb);
+    QK_CHECK(g.y <= 65535 && nb <= 65535, QK_E_ARG, "split3_blocks: K or block count too large");
+    hipLaunchKernelGGL(k_split3_blk, g, dim3(32, 8), 0, s, src, ld, bs, nr, K, out.kpad, out.s[0], out.s[1],
+                       out.nsl == 3 ? out.s[2] : nullptr);
     QK_HIP(hipGetLastError());
 }
 

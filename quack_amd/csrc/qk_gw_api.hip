@@ -870,16 +870,29 @@ void qp_graph(Ctx* c, const double* e0, const double* eG, const double* start, d
 }
 
 
+// Split-precision form of the AO->MO block (qk_set_split_aotomo), process-wide, off by default.
+std::atomic<int> g_split_aotomo{0};
+bool split_aotomo() { return g_split_aotomo.load() != 0; }
+
 // (pq|jb) from AO integrals on the device: four quarter transforms (occupied
 // index first: 2N^4 o + 6 N^3 o v flops instead of 8 N^5 for the full ERI_MO).
-void aotomo_block(hipStream_t s, const double* f, const double* cc, double* M, int N, int n, int nC, int nO, int nR) {
+// With split, the last two quarter transforms of M
+// (2 n N^2 v o flops each) run as three-slice BF16 products with FP32
+// accumulation and FP64 recombination (relative error O(2^-24), far below the
+// excitation densities' own split error), and K = (ia|jb), which sets the RPA
+// energies, comes from a separate FP64 transform of the occupied-virtual block
+// (2 o N^2 v o + 2 o v N v o flops); without split, K is taken from M later.
+void aotomo_block(hipStream_t s, const double* f, const double* cc, double* M, int N, int n, int nC, int nO, int nR,
+                  bool split = false, double* K = nullptr) {
     int o = nO - nC, v = n - nR - nO;
     long long N2 = (long long)N * N, N3 = N2 * N;
     QK_CHECK(N3 < (1LL << 31), QK_E_ARG, "nBas^3 too large");
+    QK_CHECK(!split || K, QK_E_ARG, "aotomo_block: the split form needs K");
     const double* Cocc = cc + (long long)N * nC;
     const double* Cvir = cc + (long long)N * nO;
     double* T1 = dalloc<double>((size_t)N3 * o);
     double* T2 = nullptr;
+    Split3 As, Bs;
     try {
         // T1[mu, lam, nu, j] = sum_sig (mu nu|lam sig) C[sig, j]
         dgemm(s, false, false, (int)N3, o, N, 1.0, f, N3, Cocc, N, 0.0, T1, N3);
@@ -892,10 +905,45 @@ void aotomo_block(hipStream_t s, const double* f, const double* cc, double* M, i
         dgemm(s, false, false, N, v, N, 1.0, T1, N, Cvir, N, 0.0, T2, N2, b2);
         QK_HIP(hipStreamSynchronize(s));
         dfree(T1);
+        long long cols = (long long)N * v * o;
+        QK_CHECK(cols < (1LL << 31) && (long long)n * v * o < (1LL << 31), QK_E_ARG, "block too large");
+        if (split) {
+            const long long nS = (long long)o * v;
+            // K[a' + v i' + nS jb] = sum_nu Cvir[nu, a'] sum_mu Cocc[mu, i'] T2[mu, nu, jb] in FP64
+            T1 = dalloc<double>((size_t)o * cols);
+            dgemm(s, true, false, o, (int)cols, N, 1.0, Cocc, N, T2, N, 0.0, T1, o);  // T3o[i' + o (nu + N jb)]
+            Batch bk;
+            bk.zdiv = v * o; bk.count = v * o;
+            bk.sb0 = (long long)o * N;
+            bk.sc0 = nS;
+            dgemm(s, true, true, v, o, N, 1.0, Cvir, N, T1, o, 0.0, K, v, bk);
+            QK_HIP(hipStreamSynchronize(s));
+            dfree(T1);
+            // T3[p + n (nu + N jb)] = sum_mu C[mu, p] T2[mu + N (nu + N jb)]: A(p, mu) = C[mu + N p],
+            // B(r, mu) = T2[mu + N r], both k contiguous
+            alloc_split(As, n, N, 3);
+            alloc_split(Bs, (int)cols, N, 3);
+            split3(s, cc, N, true, n, N, As);
+            split3(s, T2, N, true, (int)cols, N, Bs);
+            T1 = dalloc<double>((size_t)n * cols);
+            gemm_bf16x3(s, n, (int)cols, As, Bs, T1, n);
+            QK_HIP(hipStreamSynchronize(s));
+            free_split(Bs);
+            dfree(T2);
+            // M[q + n (p + n jb)] = sum_nu C[nu + N q] T3[p + n nu + n N jb]: B rows r = p + n jb,
+            // k = nu, split from the row blocks jb of T3 (M is symmetric in p and q)
+            alloc_split(Bs, n * v * o, N, 3);
+            split3_blocks(s, T1, n, (long long)n * N, n, v * o, N, Bs);
+            QK_HIP(hipStreamSynchronize(s));
+            dfree(T1);
+            gemm_bf16x3(s, n, n * v * o, As, Bs, M, n);
+            QK_HIP(hipStreamSynchronize(s));
+            free_split(As);
+            free_split(Bs);
+            return;
+        }
         // T3[p, nu, b, j] = sum_mu C[mu, p] T2[mu, nu, b, j]
         T1 = dalloc<double>((size_t)n * N * v * o);
-        long long cols = (long long)N * v * o;
-        QK_CHECK(cols < (1LL << 31), QK_E_ARG, "block too large");
         dgemm(s, true, false, n, (int)cols, N, 1.0, cc, N, T2, N, 0.0, T1, n);
         QK_HIP(hipStreamSynchronize(s));
         dfree(T2);
@@ -911,6 +959,8 @@ void aotomo_block(hipStream_t s, const double* f, const double* cc, double* M, i
         (void)hipStreamSynchronize(s);
         dfree(T1);
         dfree(T2);
+        free_split(As);
+        free_split(Bs);
         throw;
     }
 }
@@ -922,11 +972,12 @@ void ensure_m(Ctx* c) {
     c->use_df = false;
 }
 
-void after_eri(Ctx* c) {
+// k_done: K already formed (by the split form of aotomo_block)
+void after_eri(Ctx* c, bool k_done = false) {
     c->msplit_valid = false;
     c->have_screen = false;
     c->have_vec = false;
-    extract_k(c->s, c->M, c->K, c->nOrb, c->nC, c->nO, c->v, c->nS);
+    if (!k_done) extract_k(c->s, c->M, c->K, c->nOrb, c->nC, c->nO, c->v, c->nS);
     QK_HIP(hipStreamSynchronize(c->s));
     c->have_eri = true;
 }
@@ -1097,14 +1148,15 @@ int qk_gw_set_eri_ao_device(qk_gw_handle c, const double* f, const double* cc) {
     if (!c || !f || !cc) return QK_E_ARG;
     return guarded(&c->err, [&] {
         ensure_m(c);
+        const bool sp = split_aotomo();
         QK_HIP(hipEventRecord(c->ev[0], c->s));
-        aotomo_block(c->s, f, cc, c->M, c->nBas, c->nOrb, c->nC, c->nO, c->nR);
+        aotomo_block(c->s, f, cc, c->M, c->nBas, c->nOrb, c->nC, c->nO, c->nR, sp, c->K);
         QK_HIP(hipEventRecord(c->ev[1], c->s));
         QK_HIP(hipEventSynchronize(c->ev[1]));
         float ms = 0.f;
         QK_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
         c->t_ao = ms;
-        after_eri(c);
+        after_eri(c, sp);
     });
 }
 
@@ -1119,14 +1171,15 @@ int qk_gw_set_eri_ao(qk_gw_handle c, const double* ERI_AO, const double* cc) {
             dc = dalloc<double>(N * c->nOrb);
             h2d(c, df, ERI_AO, N * N * N * N);
             h2d(c, dc, cc, N * c->nOrb);
+            const bool sp = split_aotomo();
             QK_HIP(hipEventRecord(c->ev[0], c->s));
-            aotomo_block(c->s, df, dc, c->M, c->nBas, c->nOrb, c->nC, c->nO, c->nR);
+            aotomo_block(c->s, df, dc, c->M, c->nBas, c->nOrb, c->nC, c->nO, c->nR, sp, c->K);
             QK_HIP(hipEventRecord(c->ev[1], c->s));
             QK_HIP(hipEventSynchronize(c->ev[1]));
             float ms = 0.f;
             QK_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
             c->t_ao = ms;
-            after_eri(c);
+            after_eri(c, sp);
         } catch (...) {
             (void)hipStreamSynchronize(c->s);
             dfree(df);
@@ -1157,14 +1210,15 @@ int qk_gw_set_eri_ao_packed(qk_gw_handle c, const double* ERI_chem, long long n_
             dp = nullptr;
             dc = dalloc<double>((size_t)N * c->nOrb);
             h2d(c, dc, cc, (size_t)N * c->nOrb);
+            const bool sp = split_aotomo();
             QK_HIP(hipEventRecord(c->ev[0], c->s));
-            aotomo_block(c->s, df, dc, c->M, c->nBas, c->nOrb, c->nC, c->nO, c->nR);
+            aotomo_block(c->s, df, dc, c->M, c->nBas, c->nOrb, c->nC, c->nO, c->nR, sp, c->K);
             QK_HIP(hipEventRecord(c->ev[1], c->s));
             QK_HIP(hipEventSynchronize(c->ev[1]));
             float ms = 0.f;
             QK_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
             c->t_ao = ms;
-            after_eri(c);
+            after_eri(c, sp);
         } catch (...) {
             (void)hipStreamSynchronize(c->s);
             dfree(dp);
@@ -2510,11 +2564,12 @@ int qk_gwb_set_eri_ao_device(qk_gwb_handle h, const double* ERI_AO, long long st
     if (!h || !ERI_AO || !c) return QK_E_ARG;
     return guarded(&h->err, [&] {
         BStage st(h, 0);
+        const bool sp = split_aotomo();
         for (int m = 0; m < h->nmol; ++m) {
             Ctx* x = h->mol[m];
             aotomo_block(h->s, ERI_AO + m * stride_eri, c + m * stride_c, x->M, x->nBas, x->nOrb, x->nC, x->nO,
-                         x->nR);
-            extract_k(h->s, x->M, x->K, x->nOrb, x->nC, x->nO, x->v, x->nS);
+                         x->nR, sp, x->K);
+            if (!sp) extract_k(h->s, x->M, x->K, x->nOrb, x->nC, x->nO, x->v, x->nS);
             x->msplit_valid = false;
             x->have_eri = true;
         }
@@ -2701,6 +2756,11 @@ int qk_gwb_stage_times(qk_gwb_handle h, double* ms, int n) {
     return QK_OK;
 }
 
+
+int qk_set_split_aotomo(int on) {
+    qk::g_split_aotomo.store(on ? 1 : 0);
+    return QK_OK;
+}
 
 int qk_set_unrestricted_eigensolver(int mode, double de_max) {
     if (mode < 0 || mode > 3 || !(de_max >= 0.0)) return QK_E_ARG;

@@ -21,6 +21,11 @@ inline int pad32(int k) { return (k + 31) / 32 * 32; }
 // src(r, k) = src[k + ld*r] if k_contiguous else src[r + ld*k]; k >= K gives 0.
 void split3(hipStream_t s, const double* src, long long ld, bool k_contiguous, int rows, int K, Split3& out);
 
+// Row blocks of a strided array: out row r = r' + nr*b holds src(r', k) =
+// src[r' + ld*k + bs*b] (r' < nr, b < nb), the slices as split3's.
+void split3_blocks(hipStream_t s, const double* src, long long ld, long long bs, int nr, int nb, int K,
+                   Split3& out);
+
 // C[i + ldc*j] = sum_k A(i, k) B(j, k), FP64 result from BF16 slice products:
 // with three slices per operand the six products of weight >= 2^-16, with two
 // the three of weight >= 2^-8 (A.nsl == B.nsl).

@@ -33,7 +33,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
            "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms",
            "qk_rgw_qp_graph_noalloc", "qk_noalloc_last_error", "qk_gw_qp_iters",
-           "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats", "qk_gwb_set_graphs", "qk_gwb_graph_stats"]
+           "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats", "qk_gwb_set_graphs", "qk_gwb_graph_stats", "qk_set_split_aotomo"]
 
 
 EIG_MODES = {"syevd": 0, "refine": 1, "sytrd": 2, "refine+sytrd": 3}
@@ -139,6 +139,7 @@ def load():
     lib.qk_evugw.argtypes = [_I, _I, _I, _P, _P, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
     lib.qk_set_unrestricted_eigensolver.argtypes = [_I, _D]
     lib.qk_gwb_set_graphs.argtypes = [_H, _I]
+    lib.qk_set_split_aotomo.argtypes = [_I]
     lib.qk_gwb_graph_stats.argtypes = [_H, ctypes.POINTER(ctypes.c_longlong), _I]
     lib.qk_unrestricted_eig_stats.argtypes = [ctypes.POINTER(ctypes.c_longlong), _I]
     lib.qk_rhf_create.argtypes = [ctypes.POINTER(_H), _I]
@@ -608,6 +609,12 @@ def RGW_phBSE(ERI, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, singlet=True, t
     _check(load().qk_rgw_phbse(0, int(exchange_kernel), int(TDA_W), int(TDA), 0, 0, int(singlet), int(triplet), eta,
                                n, nC, nO, nV, nR, nS, _p(ERI), _p(eW), _p(eGW), _p(Ec), _p(Om)))
     return {"EcBSE": Ec, "OmS": Om[:nS].copy(), "OmT": Om[nS:].copy()}
+
+
+def set_split_aotomo(on):
+    """Split-precision AO->MO block for the integrals set while it is on
+    (process-wide, off by default; (ia|jb) stays FP64)."""
+    _check(load().qk_set_split_aotomo(int(bool(on))))
 
 
 def set_unrestricted_eigensolver(mode, de_max=5e-3):
