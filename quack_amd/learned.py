@@ -232,7 +232,7 @@ def load_data(data):
             np.concatenate([d["meta"] for d in parts]))
 
 
-def train(data, out, epochs=600, width=128, depth=3, lr=1e-3, batch=512, seed=0):
+def train(data, out, epochs=600, width=128, depth=3, lr=1e-3, batch=512, seed=0, log=None, every=25):
     X, y, meta = load_data(data)
     X, y = X.astype(np.float64), y.astype(np.float64)
     mean, std = X.mean(0), X.std(0) + 1e-12
@@ -245,7 +245,14 @@ def train(data, out, epochs=600, width=128, depth=3, lr=1e-3, batch=512, seed=0)
     va, tr = perm[:nval], perm[nval:]
     net = gw.MLP(nin=NFEAT, width=width, depth=depth, seed=seed)
     t0 = time.time()
-    hist = net.train(Xn[tr], yn[tr], epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
+    # epochs in segments of `every` (the Adam state stays with the network; each
+    # segment shuffles with its own seed), reporting the loss after each
+    hist = []
+    for k0 in range(0, epochs, every):
+        ne = min(every, epochs - k0)
+        hist += list(net.train(Xn[tr], yn[tr], epochs=ne, batch=batch, lr=lr, seed=seed + 1 + k0))
+        if log:
+            log("train: epoch %d/%d, loss %.4g, %.0f s" % (k0 + ne, epochs, hist[-1], time.time() - t0))
     t_train = time.time() - t0
     pv = net.forward(Xn[va], bf16=True) * ys
     err = np.abs(pv - y[va])
@@ -401,7 +408,7 @@ def main():
     if a.cmd == "make-data":
         make_data_device(a.out, a.nsys, a.seed, log=log, t_max=a.t_max)
     elif a.cmd == "train":
-        print(json.dumps(train(a.data, a.out, epochs=a.epochs, seed=a.seed)), flush=True)
+        print(json.dumps(train(a.data, a.out, epochs=a.epochs, seed=a.seed, log=log)), flush=True)
     else:
         variants = {"zgate": {"zmax": a.zmax}} if a.zmax is not None else None
         rows = evaluate_device(a.model, a.nsys, seed=12345 + a.seed, log=log, variants=variants)
