@@ -147,7 +147,13 @@ def device_systems(mols, log=None, chunk_gb=64.0, rng=None):
                 s.update({"nO": mol.nO, "nBas": nBas, "eri": eri, "slot": j})
                 out.append(s)
         yield out
+        # this chunk's integrals go back to the device before the next chunk's
+        # are allocated (the batched handles allocate outside torch's cache):
+        # the systems handed out drop their reference to the tensor
+        for s in out:
+            s["eri"] = None
         del eri
+        torch.cuda.empty_cache()
 
 
 def plain_targets_batched(systems, **method):
