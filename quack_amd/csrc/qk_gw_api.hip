@@ -8,52 +8,15 @@
 // (A-B)^1/2 (A+B) (A-B)^1/2 = D^2 + 4 D^1/2 K D^1/2: one symmetric eigensolve
 // instead of the two of LR/phRLR.f90:54,67, and XpY/XmY follow from its
 // eigenvectors by diagonal scalings (LR/phRLR.f90:77-81).
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
-
-#include <atomic>
 #include <chrono>
-#include <cmath>
 #include <cstdio>
 #include <cstdlib>
-#include <cstring>
-#include <new>
-#include <vector>
 
-#include "qk_kernels.h"
-#include "qk_split.h"
+#include "qk_gw_internal.h"
 
 namespace qk {
-namespace {
-
-constexpr int QP_MAXIT = 64;       // GW/RGW_QP_graph.f90:31
-constexpr double QP_THRESH = 1e-6; // GW/RGW_QP_graph.f90:32
-constexpr int QP_SYNC_EVERY = 4;   // Newton sweeps issued between host reads of the active count
-constexpr double SRG_FLOW = 500.0; // GW/RG0W0.f90:98
 
 thread_local std::string g_last_error;
-
-template <class T>
-T* dalloc(size_t count) {
-    if (count == 0) count = 1;
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, count * sizeof(T));
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error{QK_E_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
-                                    hipGetErrorString(e)};
-    }
-    return static_cast<T*>(p);
-}
-template <class T>
-void dfree(T*& p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
-void rb_check(rocblas_status st, const char* what) {
-    if (st != rocblas_status_success) throw Error{QK_E_HIP, std::string(what) + ": rocblas status " + std::to_string((int)st)};
-}
 
 // rocSOLVER dsyevd on the stream bound to rb; eigenvalues ascending in w.
 void syevd(hipStream_t s, rocblas_handle rb, bool vectors, int n, double* A, double* w, double* work, int* dinfo) {
@@ -66,6 +29,57 @@ void syevd(hipStream_t s, rocblas_handle rb, bool vectors, int n, double* A, dou
     QK_CHECK(info == 0, QK_E_EIG, "rocsolver_dsyevd failed, info = " + std::to_string(info));
 }
 
+// Tiny symmetric solve for DIIS: LU with partial pivoting; rcond is the exact
+// 1-norm reciprocal condition number (what dsysvx estimates,
+// utils/wrap_lapack.f90:560-598) from the explicit inverse of the (<= 33)x(<= 33) matrix.
+double small_solve(int n, std::vector<double> A, std::vector<double> b, std::vector<double>& x) {
+    std::vector<double> Ainv(n * n, 0.0), LU = A;
+    std::vector<int> piv(n);
+    double anorm = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += std::fabs(A[i + n * j]);
+        anorm = std::max(anorm, s);
+    }
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(LU[i + n * k]) > std::fabs(LU[p + n * k])) p = i;
+        piv[k] = p;
+        if (LU[p + n * k] == 0.0) { x.assign(n, 0.0); return 0.0; }
+        if (p != k)
+            for (int j = 0; j < n; ++j) std::swap(LU[k + n * j], LU[p + n * j]);
+        for (int i = k + 1; i < n; ++i) {
+            LU[i + n * k] /= LU[k + n * k];
+            for (int j = k + 1; j < n; ++j) LU[i + n * j] -= LU[i + n * k] * LU[k + n * j];
+        }
+    }
+    auto lusolve = [&](std::vector<double>& r) {
+        for (int k = 0; k < n; ++k) std::swap(r[k], r[piv[k]]);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) r[i] -= LU[i + n * j] * r[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) r[i] -= LU[i + n * j] * r[j];
+            r[i] /= LU[i + n * i];
+        }
+    };
+    double inorm = 0.0;
+    for (int j = 0; j < n; ++j) {
+        std::vector<double> ej(n, 0.0);
+        ej[j] = 1.0;
+        lusolve(ej);
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += std::fabs(ej[i]);
+        inorm = std::max(inorm, s);
+    }
+    x = b;
+    lusolve(x);
+    if (!(anorm > 0.0) || !(inorm > 0.0) || !std::isfinite(inorm)) return 0.0;
+    return 1.0 / (anorm * inorm);
+}
+
+namespace {
+
 // hipGraph replay of the own full solve's launch sequences (process-wide
 // switch, qk_set_graphs; off by default).  The tridiagonalisation (two
 // launches per column) and the back-transformation contain only the
@@ -75,20 +89,6 @@ void syevd(hipStream_t s, rocblas_handle rb, bool vectors, int n, double* A, dou
 // Measured at n = 5600 on MI355X (ROCm 7.2): 142 ms per tridiagonalisation
 // replayed against 137 ms launched on the stream, hence off by default.
 std::atomic<int> g_graphs{0};
-
-struct GraphSlot {
-    const void* key = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
-struct SolveGraphs {
-    GraphSlot trd[2], orm[2];
-    void clear() {
-        for (GraphSlot* g : {&trd[0], &trd[1], &orm[0], &orm[1]}) {
-            if (g->exec) (void)hipGraphExecDestroy(g->exec);
-            *g = GraphSlot();
-        }
-    }
-};
 
 // Run body on stream s: replayed from the slot keyed by key, captured into a
 // free slot on first use (thread-local capture mode, so other host threads
@@ -163,162 +163,7 @@ void syevd_own(hipStream_t s, rocblas_handle rb, bool vectors, int n, double*& A
     QK_CHECK(info == 0, QK_E_EIG, "tridiagonal eigensolver failed, info = " + std::to_string(info));
 }
 
-// Tiny symmetric solve for DIIS: LU with partial pivoting; rcond is the exact
-// 1-norm reciprocal condition number (what dsysvx estimates,
-// utils/wrap_lapack.f90:560-598) from the explicit inverse of the (<= 33)x(<= 33) matrix.
-double small_solve(int n, std::vector<double> A, std::vector<double> b, std::vector<double>& x) {
-    std::vector<double> Ainv(n * n, 0.0), LU = A;
-    std::vector<int> piv(n);
-    double anorm = 0.0;
-    for (int j = 0; j < n; ++j) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += std::fabs(A[i + n * j]);
-        anorm = std::max(anorm, s);
-    }
-    for (int k = 0; k < n; ++k) {
-        int p = k;
-        for (int i = k + 1; i < n; ++i)
-            if (std::fabs(LU[i + n * k]) > std::fabs(LU[p + n * k])) p = i;
-        piv[k] = p;
-        if (LU[p + n * k] == 0.0) { x.assign(n, 0.0); return 0.0; }
-        if (p != k)
-            for (int j = 0; j < n; ++j) std::swap(LU[k + n * j], LU[p + n * j]);
-        for (int i = k + 1; i < n; ++i) {
-            LU[i + n * k] /= LU[k + n * k];
-            for (int j = k + 1; j < n; ++j) LU[i + n * j] -= LU[i + n * k] * LU[k + n * j];
-        }
-    }
-    auto lusolve = [&](std::vector<double>& r) {
-        for (int k = 0; k < n; ++k) std::swap(r[k], r[piv[k]]);
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j < i; ++j) r[i] -= LU[i + n * j] * r[j];
-        for (int i = n - 1; i >= 0; --i) {
-            for (int j = i + 1; j < n; ++j) r[i] -= LU[i + n * j] * r[j];
-            r[i] /= LU[i + n * i];
-        }
-    };
-    double inorm = 0.0;
-    for (int j = 0; j < n; ++j) {
-        std::vector<double> ej(n, 0.0);
-        ej[j] = 1.0;
-        lusolve(ej);
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += std::fabs(ej[i]);
-        inorm = std::max(inorm, s);
-    }
-    x = b;
-    lusolve(x);
-    if (!(anorm > 0.0) || !(inorm > 0.0) || !std::isfinite(inorm)) return 0.0;
-    return 1.0 / (anorm * inorm);
-}
-
 }  // namespace
-
-// utils/DIIS_extrapolation.f90:1-61 (history prepend utils/utils.f90:506-536).
-struct Diis {
-    int n, max_diis, n_diis = 0;
-    std::vector<double> err, e;  // column-major (n, max_diis)
-    Diis(int n_, int m_) : n(n_), max_diis(m_), err((size_t)n_ * m_, 0.0), e((size_t)n_ * m_, 0.0) {}
-    void step(const double* err_in, double* e_inout) {
-        n_diis = std::min(n_diis + 1, max_diis);
-        int m = n_diis;
-        for (int j = m - 2; j >= 0; --j) {
-            std::memcpy(&err[(size_t)n * (j + 1)], &err[(size_t)n * j], sizeof(double) * n);
-            std::memcpy(&e[(size_t)n * (j + 1)], &e[(size_t)n * j], sizeof(double) * n);
-        }
-        std::memcpy(&err[0], err_in, sizeof(double) * n);
-        std::memcpy(&e[0], e_inout, sizeof(double) * n);
-        int N = m + 1;
-        std::vector<double> A((size_t)N * N, 0.0), b(N, 0.0), w;
-        for (int i = 0; i < m; ++i)
-            for (int j = 0; j < m; ++j) {
-                double s = 0.0;
-                for (int k = 0; k < n; ++k) s += err[(size_t)n * i + k] * err[(size_t)n * j + k];
-                A[i + N * j] = s;
-            }
-        for (int i = 0; i < m; ++i) { A[i + N * m] = -1.0; A[m + N * i] = -1.0; }
-        b[m] = -1.0;
-        double rcond = small_solve(N, A, b, w);
-        if (rcond > 1e-13) {
-            for (int k = 0; k < n; ++k) {
-                double s = 0.0;
-                for (int j = 0; j < m; ++j) s += w[j] * e[(size_t)n * j + k];
-                e_inout[k] = s;
-            }
-        }
-    }
-};
-
-}  // namespace qk
-
-struct qk_gw_ctx {
-    int nBas = 0, nOrb = 0, nC = 0, nO = 0, nV = 0, nR = 0, nS = 0, o = 0, v = 0;
-    hipStream_t s = nullptr;
-    rocblas_handle rb = nullptr;
-    double *M = nullptr, *K = nullptr, *Cm = nullptr, *rhoT = nullptr;
-    // rhoT holds FP32 values (first half of the allocation) after a
-    // split-precision contraction, whose accuracy is FP32-level; FP64 otherwise
-    bool rho_f32 = false;
-    double *e = nullptr, *e0 = nullptr, *eG = nullptr, *dE = nullptr, *sd = nullptr, *Om = nullptr, *Om0 = nullptr;
-    double *Sig = nullptr, *Z = nullptr, *eLin = nullptr, *eGW = nullptr, *w = nullptr, *f = nullptr, *df = nullptr;
-    double *part = nullptr, *scal = nullptr, *work = nullptr;
-    int *nit = nullptr, *done = nullptr, *info = nullptr, *active = nullptr;
-    int nchunk = 1;
-    bool have_eri = false;
-    bool have_screen = false;
-    // 0: FP64 MFMA; split precision with FP64 recombination: 1 three BF16 slices
-    // (six products), 2 two slices (three products) in the excitation densities
-    int precision = 0;
-    qk::Split3 Msplit, Wsplit;  // M^T and W slices (mixed precision)
-    bool msplit_valid = false;
-    // density-fitted integrals (pq|rs) = sum_P B(P,pq) B(P,rs): B[P + naux (p + nOrb q)],
-    // Bov[P + naux ia] its (i, a) block, T = Bov W (naux x nS)
-    bool use_df = false;
-    int naux = 0;
-    double *Bdf = nullptr, *Bov = nullptr, *Tdf = nullptr;
-    // work of the full self-energy's matrix-core path (qk_gw_sigma_full), kept across calls
-    double* sigw = nullptr;
-    qk::Split3 Bsplit, BovTsplit, Tsplit;  // B (rows pq), Bov^T (rows P, k = jb), T (rows m)
-    bool bsplit_valid = false;
-    // eigensolver: 0 = rocSOLVER dsyevd for every RPA (QuAcK's scheme); 1 = refine
-    // the previous eigenvectors by Newton steps when the energies moved by at
-    // most eig_demax since they were computed, dsyevd otherwise or on failure
-    int eig_mode = 0;  // bit 0: refine; bit 1: own tridiagonalisation instead of dsyevd
-    double eig_demax = 5e-3;
-    double *Zev = nullptr, *tau = nullptr, *trd_work = nullptr, *orm_work = nullptr;  // own full-solve path
-    qk::SolveGraphs graphs;  // captured launch sequences of the own full solve
-    double *Xv = nullptr, *Ym = nullptr, *Sm = nullptr, *Gm = nullptr, *lam = nullptr, *eig_dbuf = nullptr;
-    int *perm = nullptr, *eig_ibuf = nullptr;
-    long long* eig_lbuf = nullptr;
-    bool have_vec = false;
-    int tda_vec = 0;
-    // unrestricted pairs (the screening of qk_ug0w0 and qk_evugw): alpha nO,
-    // beta nOb, nC = nR = 0, nS = nSa + nSb, energies (e_alpha, e_beta); -1
-    // for the restricted handle
-    int nOb = -1;
-    std::vector<double> e_vec;
-    // dsyevd calls, refined solves, Newton steps, fallbacks; fallbacks because a
-    // cluster outgrew EIG_KMAX, because merging did not settle, because the
-    // steps did not converge; largest cluster met
-    long long eig_count[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // [8]: full solves by the own tridiagonalisation
-    double t_ao = 0.0;
-    // [0..4] RPA build, eigensolve, excitation densities, self-energy + QP, total;
-    // [5] excitation-density GEMM launches alone, [6] their number
-    double t_stage[7] = {0, 0, 0, 0, 0, 0, 0};
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    hipEvent_t ev2[2] = {nullptr, nullptr};
-    double t_gemm_total = 0.0, n_gemm_total = 0.0;  // excitation GEMM ms and launches since creation
-    hipEvent_t ev3[4] = {nullptr, nullptr, nullptr, nullptr};
-    double t_trd_total = 0.0, n_trd_total = 0.0;    // own tridiagonalisations: ms and count since creation
-    double t_stedc_total = 0.0, t_ormtr_total = 0.0; // their dstedc and back-transformation (vector solves)
-    bool shared = false;  // stream, rocBLAS handle and the batch slots belong to a qk_gwb handle
-    std::string err;
-};
-
-namespace qk {
-namespace {
-
-using Ctx = qk_gw_ctx;
 
 // Event-timed stage on the handle's stream (accumulates into t_stage[k]).
 struct Stage {
@@ -361,7 +206,7 @@ double d2h_scalar(Ctx* c, const double* src) {
     return v;
 }
 
-void alloc_split(Split3& sp, int rows, int K, int nsl = 3) {
+void alloc_split(Split3& sp, int rows, int K, int nsl) {
     sp.rows = rows;
     sp.kpad = pad32(K);
     sp.nsl = nsl;
@@ -883,7 +728,7 @@ bool split_aotomo() { return g_split_aotomo.load() != 0; }
 // energies, comes from a separate FP64 transform of the occupied-virtual block
 // (2 o N^2 v o + 2 o v N v o flops); without split, K is taken from M later.
 void aotomo_block(hipStream_t s, const double* f, const double* cc, double* M, int N, int n, int nC, int nO, int nR,
-                  bool split = false, double* K = nullptr) {
+                  bool split, double* K) {
     int o = nO - nC, v = n - nR - nO;
     long long N2 = (long long)N * N, N3 = N2 * N;
     QK_CHECK(N3 < (1LL << 31), QK_E_ARG, "nBas^3 too large");
@@ -1013,39 +858,28 @@ void upload_mo_block(hipStream_t s, const double* ERI_MO, double* M, int n, int 
     (void)hipHostFree(hX);
 }
 
-// Shared stream and rocBLAS handle of the handle-less entry points.
-struct Scratch {
-    hipStream_t s = nullptr;
-    rocblas_handle rb = nullptr;
-    Scratch() {
-        QK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        rb_check(rocblas_create_handle(&rb), "rocblas_create_handle");
-        rb_check(rocblas_set_stream(rb, s), "rocblas_set_stream");
+// upload rho(p,q,m) (QuAcK layout) into rhoT[m + nS (p + n q)]
+double* upload_rho(hipStream_t s, const double* rho, int n, int nS) {
+    size_t n2 = (size_t)n * n;
+    double* tmp = dalloc<double>(n2 * nS);
+    double* out = nullptr;
+    try {
+        out = dalloc<double>(n2 * nS);
+        QK_HIP(hipMemcpyAsync(tmp, rho, sizeof(double) * n2 * nS, hipMemcpyHostToDevice, s));
+        transpose(s, tmp, out, (int)n2, nS, n2, nS);
+        QK_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        dfree(tmp); dfree(out);
+        throw;
     }
-};
+    dfree(tmp);
+    return out;
+}
+
 Scratch& scratch() {
     static Scratch* sc = new Scratch();  // lives until process exit
     return *sc;
-}
-
-template <class F>
-int guarded(std::string* slot, F&& fn) {
-    try {
-        fn();
-        return QK_OK;
-    } catch (const Error& e) {
-        if (slot) *slot = e.msg;
-        g_last_error = e.msg;
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        if (slot) *slot = "host allocation failed";
-        g_last_error = "host allocation failed";
-        return QK_E_ALLOC;
-    } catch (...) {
-        if (slot) *slot = "unknown error";
-        g_last_error = "unknown error";
-        return QK_E_HIP;
-    }
 }
 
 void release(Ctx* c) {
@@ -1088,7 +922,6 @@ void release(Ctx* c) {
     }
 }
 
-}  // namespace
 }  // namespace qk
 
 using namespace qk;
@@ -1713,404 +1546,6 @@ int qk_rgw_excitation_density(int nOrb, int nC, int nO, int nR, int nS, const do
     });
 }
 
-namespace {
-// upload rho(p,q,m) (QuAcK layout) into rhoT[m + nS (p + n q)]
-double* upload_rho(hipStream_t s, const double* rho, int n, int nS) {
-    size_t n2 = (size_t)n * n;
-    double* tmp = dalloc<double>(n2 * nS);
-    double* out = nullptr;
-    try {
-        out = dalloc<double>(n2 * nS);
-        QK_HIP(hipMemcpyAsync(tmp, rho, sizeof(double) * n2 * nS, hipMemcpyHostToDevice, s));
-        transpose(s, tmp, out, (int)n2, nS, n2, nS);
-        QK_HIP(hipStreamSynchronize(s));
-    } catch (...) {
-        (void)hipStreamSynchronize(s);
-        dfree(tmp); dfree(out);
-        throw;
-    }
-    dfree(tmp);
-    return out;
-}
-}  // namespace
-
-}  // extern "C"
-
-namespace qk {
-namespace {
-// Device work of the diagonal self-energy and the graphical QP solve for n
-// orbitals and nch chunks of modes: partial sums, Newton state and the
-// orbital-energy vectors, allocated once by their owner.
-struct QpWork {
-    int n = 0, nch = 0;
-    double *part = nullptr, *w = nullptr, *f = nullptr, *df = nullptr, *sig = nullptr, *z = nullptr, *sc = nullptr;
-    double *e0 = nullptr, *eG = nullptr, *eL = nullptr, *eo = nullptr;
-    int *nit = nullptr, *done = nullptr, *act = nullptr;
-    QpWork(int n_, int nch_) : n(n_), nch(nch_) {
-        try {
-            part = dalloc<double>((size_t)3 * n * nch);
-            for (double** p : {&w, &f, &df, &sig, &z, &e0, &eG, &eL, &eo}) *p = dalloc<double>(n);
-            sc = dalloc<double>(4);
-            nit = dalloc<int>(n);
-            done = dalloc<int>(n);
-            act = dalloc<int>(1);
-        } catch (...) {
-            release();
-            throw;
-        }
-    }
-    QpWork(const QpWork&) = delete;
-    QpWork& operator=(const QpWork&) = delete;
-    ~QpWork() { release(); }
-    void release() {
-        for (double** p : {&part, &w, &f, &df, &sig, &z, &sc, &e0, &eG, &eL, &eo}) dfree(*p);
-        dfree(nit);
-        dfree(done);
-        dfree(act);
-    }
-};
-
-// Diagonal self-energy (GW/RGW_self_energy_diag.f90 or the SRG form) at the
-// host energies e, from device rT (rho(p, q, m) at m + nS (p + n q)) and Om;
-// SigC, Z and EcGM to the host.
-void sigma_dev(hipStream_t s, QpWork& q, const double* rT, const double* dO, int doSRG, double eta, int nC, int nO,
-               int nR, int nS, const double* e, double* SigC, double* Z, double* EcGM) {
-    const int n = q.n;
-    QK_HIP(hipMemcpyAsync(q.e0, e, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    sigma_partial(s, rT, false, q.e0, q.e0, dO, nullptr, eta, doSRG, SRG_FLOW, n, nC, nO, nR, nS, q.nch, q.part);
-    sigma_finalize(s, q.part, q.nch, n, nC, nO, nR, q.sig, q.z, q.sc);
-    QK_HIP(hipMemcpyAsync(SigC, q.sig, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    QK_HIP(hipMemcpyAsync(Z, q.z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    double gm = 0.0;
-    QK_HIP(hipMemcpyAsync(&gm, q.sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
-    QK_HIP(hipStreamSynchronize(s));
-    if (EcGM) *EcGM = gm;
-}
-
-// Graphical QP solve (GW/RGW_QP_graph.f90:50-92) from host energies, device rT
-// and Om; the Newton sweeps stop once no orbital is left active (read every
-// QP_SYNC_EVERY sweeps).  Z is
-// read (frozen orbitals keep it) and written on the host.
-void qp_dev(hipStream_t s, QpWork& q, const double* rT, const double* dO, int doSRG, double eta, double flow, int nC,
-            int nO, int nR, int nS, const double* eHF, const double* eGWlin, const double* eOld, double* eGW,
-            double* Z) {
-    const int n = q.n;
-    QK_HIP(hipMemcpyAsync(q.e0, eHF, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    QK_HIP(hipMemcpyAsync(q.eG, eOld, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    QK_HIP(hipMemcpyAsync(q.eL, eGWlin, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    QK_HIP(hipMemcpyAsync(q.z, Z, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    qp_init(s, q.eL, q.w, q.f, q.nit, q.done, n, nC, nR);
-    int h_active = 1;
-    for (int it = 0; it < QP_MAXIT && h_active > 0;) {
-        for (int k = 0; k < QP_SYNC_EVERY && it < QP_MAXIT; ++k, ++it) {
-            sigma_partial(s, rT, false, q.w, q.eG, dO, q.done, eta, doSRG, flow, n, nC, nO, nR, nS, q.nch, q.part);
-            qp_update(s, q.part, q.nch, q.e0, q.w, q.f, q.df, q.nit, q.done, n, QP_THRESH, QP_MAXIT, q.act);
-        }
-        QK_HIP(hipMemcpyAsync(&h_active, q.act, sizeof(int), hipMemcpyDeviceToHost, s));
-        QK_HIP(hipStreamSynchronize(s));
-    }
-    qp_finish(s, q.w, q.df, q.nit, q.eL, q.eo, q.z, n, nC, nR, QP_MAXIT);
-    QK_HIP(hipMemcpyAsync(eGW, q.eo, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    QK_HIP(hipMemcpyAsync(Z, q.z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    QK_HIP(hipStreamSynchronize(s));
-}
-
-// Host rho(p,q,m) (n x n x nS, column-major) and Om uploaded for one
-// self-energy or QP call of the C-ABI (qk_rgw_self_energy_diag, qk_rgw_qp_graph).
-struct HostRho {
-    hipStream_t s;
-    double *rT = nullptr, *dO = nullptr;
-    HostRho(hipStream_t s_, const double* rho, const double* Om, int n, int nS) : s(s_) {
-        try {
-            rT = upload_rho(s, rho, n, nS);
-            dO = dalloc<double>(nS);
-            QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
-        } catch (...) {
-            release();
-            throw;
-        }
-    }
-    ~HostRho() { release(); }
-    void release() {
-        (void)hipStreamSynchronize(s);
-        dfree(rT);
-        dfree(dO);
-    }
-};
-
-// Eigensolver of the unrestricted screening (qk_set_unrestricted_eigensolver),
-// process-wide, and the eigensolver counts of the calling thread's last
-// qk_ug0w0 / qk_evugw (qk_unrestricted_eig_stats).
-std::atomic<int> g_u_eig_mode{0};
-std::atomic<double> g_u_eig_demax{5e-3};
-thread_local long long g_u_eig_count[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-
-// Unrestricted screening and excitation densities (nC = nR = 0), shared by
-// UG0W0 and evUGW.  Pair index ia = a' + v_s i' within spin block s, the beta
-// block after the alpha one (LR/phULR_A.f90, LR/phULR_B.f90).  The integral
-// slabs M_s(pq, k) = (pq|k) of each spin's densities
-// (GW/UGW_excitation_density.f90:24-62) and the coupling K = (ia|jb) / 2
-// gathered from them stay on the device for the whole calculation.  The dRPA
-// A = D + K_u, B = K_u (K_u = 2K) has A - B = D, so each screening is the
-// restricted engine's (screening(): C = D^2 + 2 D^1/2 K_u D^1/2, or A under
-// TDA, built on the device; the engine's eigensolver, with Newton refinement
-// of the previous cycle's vectors when it is switched on), held by a
-// screening-only engine handle whose pairs are the unrestricted ones.  It
-// leaves Om and XpY^T = W on the device, then rho_s = M_s W / sqrt(2) in the
-// restricted kernels' rhoT layout (m contiguous): the unrestricted sums weigh
-// rho^2 where the restricted kernels weigh 2 rho^2.
-struct UScreen {
-    int n, nO[2], v[2], nSa, nSb, nSt, TDA_W;
-    size_t S2, n2;
-    const double* G[3];
-    Ctx* c = nullptr;
-    hipStream_t st = nullptr;
-    double* de = nullptr;
-    double *dM[2] = {nullptr, nullptr}, *dR[2] = {nullptr, nullptr};
-    QpWork* q = nullptr;
-
-    UScreen(int n_, int nOa, int nOb, const double* Gaaaa, const double* Gaabb, const double* Gbbbb, int tda)
-        : n(n_), nO{nOa, nOb}, v{n_ - nOa, n_ - nOb}, TDA_W(tda), G{Gaaaa, Gaabb, Gbbbb} {
-        nSa = nO[0] * v[0];
-        nSb = nO[1] * v[1];
-        nSt = nSa + nSb;
-        QK_CHECK((long long)nSt * nSt < (1LL << 40), QK_E_ARG, "nSa + nSb too large");
-        S2 = (size_t)nSt * nSt;
-        n2 = (size_t)n * n;
-        QK_CHECK((long long)n2 < (1LL << 31), QK_E_ARG, "nBas^2 too large");
-        try {
-            c = new Ctx();
-            c->nBas = c->nOrb = n;
-            c->nC = c->nR = 0;
-            c->nO = nOa;
-            c->nOb = nOb;
-            c->nV = n - nOa;
-            c->o = nOa;
-            c->v = n - nOa;
-            c->nS = nSt;
-            QK_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
-            rb_check(rocblas_create_handle(&c->rb), "rocblas_create_handle");
-            rb_check(rocblas_set_stream(c->rb, c->s), "rocblas_set_stream");
-            for (auto* e : {&c->ev[0], &c->ev[1], &c->ev2[0], &c->ev2[1], &c->ev3[0], &c->ev3[1], &c->ev3[2],
-                            &c->ev3[3]})
-                QK_HIP(hipEventCreate(e));
-            st = c->s;
-            c->K = dalloc<double>(S2);
-            c->Cm = dalloc<double>(S2);
-            for (double** p : {&c->dE, &c->sd, &c->Om, &c->work}) *p = dalloc<double>(nSt);
-            c->scal = dalloc<double>(8);
-            c->info = dalloc<int>(1);
-            de = dalloc<double>(2 * (size_t)n);
-            std::vector<double> Ms(n2 * nSt);
-            for (int sp = 0; sp < 2; ++sp) {
-                dM[sp] = dalloc<double>(n2 * nSt);
-                dR[sp] = dalloc<double>(n2 * nSt);
-                for (int sj = 0; sj < 2; ++sj)
-                    for (int j = 0; j < nO[sj]; ++j)
-                        for (int b = 0; b < v[sj]; ++b) {
-                            const size_t k = (sj ? nSa : 0) + b + (size_t)v[sj] * j;
-                            const int Bb = nO[sj] + b;
-                            for (int qq = 0; qq < n; ++qq)
-                                for (int p = 0; p < n; ++p)
-                                    Ms[(p + (size_t)n * qq) + n2 * k] =
-                                        sp == 0 ? at(sj == 0 ? 0 : 1, p, j, qq, Bb)
-                                                : (sj == 0 ? at(1, j, p, Bb, qq) : at(2, p, j, qq, Bb));
-                        }
-                QK_HIP(hipMemcpyAsync(dM[sp], Ms.data(), sizeof(double) * n2 * nSt, hipMemcpyHostToDevice, st));
-                QK_HIP(hipStreamSynchronize(st));  // Ms is refilled for the next spin
-            }
-            extract_ku(st, dM[0], dM[1], c->K, n, nO[0], nO[1]);
-            const int rc = qk_gw_set_eigensolver(c, g_u_eig_mode.load(), g_u_eig_demax.load());
-            QK_CHECK(rc == QK_OK, rc, "unrestricted screening: " + c->err);
-            q = new QpWork(n, sigma_chunks(nSt, n));
-        } catch (...) {
-            release();
-            throw;
-        }
-    }
-    UScreen(const UScreen&) = delete;
-    UScreen& operator=(const UScreen&) = delete;
-    ~UScreen() { release(); }
-    void release() {
-        if (st) (void)hipStreamSynchronize(st);
-        delete q;
-        q = nullptr;
-        for (double** p : {&de, &dM[0], &dM[1], &dR[0], &dR[1]}) dfree(*p);
-        if (c) {
-            std::copy(c->eig_count, c->eig_count + 9, g_u_eig_count);
-            qk::release(c);
-            delete c;
-            c = nullptr;
-        }
-        st = nullptr;
-    }
-    double at(int k, int p, int qq, int r, int t) const {  // ERI_k(p, q, r, t), 0-based
-        return G[k][p + (size_t)n * (qq + (size_t)n * (r + (size_t)n * t))];
-    }
-    // LR/phULR.f90 at energies en (n x 2, host); returns EcRPA, leaves Om and
-    // W = XpY^T on the device
-    double rpa(const double* en, const char* who) {
-        if (!TDA_W) {
-            // LR/phRLR.f90:57-58 (the unrestricted solver shares it): A - B
-            // must be positive definite; here it is diag(e_a - e_i)
-            double dmin = HUGE_VAL;
-            for (int sp = 0; sp < 2; ++sp)
-                for (int i = 0; i < nO[sp]; ++i)
-                    for (int a = nO[sp]; a < n; ++a) dmin = std::min(dmin, en[(size_t)n * sp + a] - en[(size_t)n * sp + i]);
-            QK_CHECK(dmin > 0.0, QK_E_UNSTABLE,
-                     std::string(who) + ": A-B is not positive definite (smallest eigenvalue " +
-                         std::to_string(dmin) + "): linear-response instability");
-        }
-        QK_HIP(hipMemcpyAsync(de, en, sizeof(double) * 2 * n, hipMemcpyHostToDevice, st));
-        screening(c, de, en, TDA_W, true);
-        double ec = 0.0;
-        QK_HIP(hipMemcpyAsync(&ec, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
-        QK_HIP(hipStreamSynchronize(st));
-        return ec;
-    }
-    // GW/UGW_excitation_density.f90 for both spins from the last rpa():
-    // rho_s(m, pq) = sum_k W(k, m) M_s(pq, k) / sqrt(2)
-    void densities() {
-        for (int sp = 0; sp < 2; ++sp)
-            dgemm(st, true, true, nSt, (int)n2, nSt, 1.0 / std::sqrt(2.0), c->Cm, nSt, dM[sp], n2, 0.0, dR[sp], nSt);
-    }
-    void om_host(double* out) {
-        QK_HIP(hipMemcpyAsync(out, c->Om, sizeof(double) * nSt, hipMemcpyDeviceToHost, st));
-        QK_HIP(hipStreamSynchronize(st));
-    }
-    // per spin: diagonal self-energy at e_s (GW/UGW_self_energy_diag.f90 or the SRG form);
-    // EcGM per spin, halved (the restricted sum carries 4 rho^2, the unrestricted rho^2, x2 here)
-    void sigma(int sp, const double* es, double eta, int doSRG, double* sig, double* z, double* ecgm) {
-        double gm = 0.0;
-        sigma_dev(st, *q, dR[sp], c->Om, doSRG, eta, 0, nO[sp], 0, nSt, es, sig, z, &gm);
-        if (ecgm) *ecgm = 0.5 * gm;
-    }
-    // GW/UGW_QP_graph.f90 for spin sp
-    void qp(int sp, int doSRG, double eta, const double* eHF_s, const double* eLin, const double* eOld, double* eGW_s,
-            double* z) {
-        qp_dev(st, *q, dR[sp], c->Om, doSRG, eta, SRG_FLOW, 0, nO[sp], 0, nSt, eHF_s, eLin, eOld, eGW_s, z);
-    }
-};
-}  // namespace
-}  // namespace qk
-
-extern "C" {
-
-int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double* e,
-                            const double* Om, const double* rho, double* EcGM, double* SigC, double* Z) {
-    if (!e || !Om || !rho || !SigC || !Z) return QK_E_ARG;
-    return guarded(nullptr, [&] {
-        check_sizes(nOrb, nOrb, nC, nO, nV, nR, nS);
-        hipStream_t s = scratch().s;
-        HostRho r(s, rho, Om, nOrb, nS);
-        QpWork q(nOrb, sigma_chunks(nS, nOrb - nR - nC));
-        sigma_dev(s, q, r.rT, r.dO, doSRG, eta, nC, nO, nR, nS, e, SigC, Z, EcGM);
-    });
-}
-
-int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO, int nV, int nR, int nS,
-                    const double* eHF, const double* Om, const double* rho, const double* eGWlin, const double* eOld,
-                    double* eGW, double* Z) {
-    if (!eHF || !Om || !rho || !eGWlin || !eOld || !eGW || !Z) return QK_E_ARG;
-    return guarded(nullptr, [&] {
-        check_sizes(nOrb, nOrb, nC, nO, nV, nR, nS);
-        hipStream_t s = scratch().s;
-        HostRho r(s, rho, Om, nOrb, nS);
-        QpWork q(nOrb, sigma_chunks(nS, nOrb - nR - nC));
-        qp_dev(s, q, r.rT, r.dO, doSRG, eta, flow, nC, nO, nR, nS, eHF, eGWlin, eOld, eGW, Z);
-    });
-}
-
-
-int qk_ug0w0(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* ERI_aabb, const double* ERI_bbbb,
-             const double* eHF, double eta, int linearize, int TDA_W, int doSRG, double* eGW, double* SigC, double* Z,
-             double* Om, double* EcRPA, double* EcGM) {
-    if (nBas <= 0 || nOa <= 0 || nOb <= 0 || nOa >= nBas || nOb >= nBas || !ERI_aaaa || !ERI_aabb || !ERI_bbbb ||
-        !eHF || !eGW || !SigC || !Z)
-        return QK_E_ARG;
-    return guarded(nullptr, [&] {
-        const int n = nBas;
-        UScreen u(n, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
-        u.rpa(eHF, "qk_ug0w0");
-        u.densities();
-        if (Om) u.om_host(Om);
-        // GW/UG0W0.f90:104-134 per spin: self-energy at eHF, linearised energies,
-        // graphical QP with the linearised guess and G from eHF
-        for (int sp = 0; sp < 2; ++sp) {
-            const double* es = eHF + (size_t)n * sp;
-            double *sig = SigC + (size_t)n * sp, *z = Z + (size_t)n * sp, *eg = eGW + (size_t)n * sp;
-            u.sigma(sp, es, eta, doSRG, sig, z, EcGM ? EcGM + sp : nullptr);
-            std::vector<double> lin(n);
-            for (int p = 0; p < n; ++p) lin[p] = es[p] + z[p] * sig[p];
-            if (linearize) std::memcpy(eg, lin.data(), sizeof(double) * n);
-            else u.qp(sp, doSRG, eta, es, lin.data(), es, eg, z);
-        }
-        // RPA correlation energy at eGW (GW/UG0W0.f90:140-142)
-        const double ec = u.rpa(eGW, "qk_ug0w0 (RPA at eGW)");
-        if (EcRPA) *EcRPA = ec;
-    });
-}
-
-int qk_evugw(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* ERI_aabb, const double* ERI_bbbb,
-             const double* eHF, int maxSCF, double thresh, int max_diis, double eta, int linearize, int TDA_W,
-             int doSRG, double* eGW, double* SigC, double* Z, double* Om, double* EcRPA, double* EcGM, int* nSCF_out) {
-    if (nBas <= 0 || nOa <= 0 || nOb <= 0 || nOa >= nBas || nOb >= nBas || !ERI_aaaa || !ERI_aabb || !ERI_bbbb ||
-        !eHF || !eGW || !SigC || !Z || maxSCF < 0)
-        return QK_E_ARG;
-    int conv_status = QK_OK;
-    int rc = guarded(nullptr, [&] {
-        const int n = nBas;
-        const size_t n2s = (size_t)n * 2;
-        UScreen u(n, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
-        // GW/evUGW.f90:103-215: eGW = eOld = eHF, Z = 1; per cycle screening at eGW,
-        // self-energy at eGW, QP energies (linearised: eHF + SigC; graphical: guess and
-        // G from eOld), Conv = max |eGW - eOld| over both spins, DIIS per spin with
-        // one shared history length
-        std::vector<double> e(eHF, eHF + n2s), eOld = e, err(n2s);
-        for (size_t p = 0; p < n2s; ++p) Z[p] = 1.0;
-        Diis diis[2] = {Diis(n, std::max(max_diis, 1)), Diis(n, std::max(max_diis, 1))};
-        int nSCF = 0;
-        double Conv = 1.0, ecrpa = 0.0, ecgm[2] = {0.0, 0.0};
-        while (Conv > thresh && nSCF <= maxSCF) {
-            ecrpa = u.rpa(e.data(), "qk_evugw");
-            u.densities();
-            for (int sp = 0; sp < 2; ++sp) {
-                const size_t o = (size_t)n * sp;
-                u.sigma(sp, e.data() + o, eta, doSRG, SigC + o, Z + o, ecgm + sp);
-                if (linearize) {
-                    for (int p = 0; p < n; ++p) e[o + p] = eHF[o + p] + SigC[o + p];
-                } else {
-                    std::vector<double> eg(n);
-                    u.qp(sp, doSRG, eta, eHF + o, eOld.data() + o, eOld.data() + o, eg.data(), Z + o);
-                    std::memcpy(e.data() + o, eg.data(), sizeof(double) * n);
-                }
-            }
-            Conv = 0.0;
-            for (size_t p = 0; p < n2s; ++p) {
-                err[p] = e[p] - eOld[p];
-                Conv = std::max(Conv, std::fabs(err[p]));
-            }
-            if (max_diis > 1)
-                for (int sp = 0; sp < 2; ++sp) diis[sp].step(err.data() + (size_t)n * sp, e.data() + (size_t)n * sp);
-            eOld = e;
-            ++nSCF;
-        }
-        std::memcpy(eGW, e.data(), sizeof(double) * n2s);
-        if (Om) u.om_host(Om);
-        if (EcRPA) *EcRPA = ecrpa;
-        if (EcGM) {
-            EcGM[0] = ecgm[0];
-            EcGM[1] = ecgm[1];
-        }
-        if (nSCF_out) *nSCF_out = nSCF;
-        if (nSCF == maxSCF + 1) {
-            conv_status = QK_E_NOCONV;
-            g_last_error = "evUGW: convergence failed after " + std::to_string(nSCF) + " cycles";
-        }
-    });
-    return rc != QK_OK ? rc : conv_status;
-}
-
 int qk_rgw_phbse(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE, int dTDA, int singlet, int triplet,
                  double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double* ERI, const double* eW,
                  const double* eGW, double* EcBSE, double* OmBSE) {
@@ -2224,554 +1659,8 @@ int qk_rgw_phbse(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE
     return rc;
 }
 
-}  // extern "C"
-
-// ---- batched G0W0: many molecules of one size, stage by stage -----------------
-
-struct qk_gwb_ctx {
-    int nmol = 0;
-    hipStream_t s = nullptr;
-    rocblas_handle rb = nullptr;
-    std::vector<qk_gw_ctx*> mol;
-    // contiguous per-molecule arrays with uniform strides (zs), so that one
-    // launch serves every molecule (grid z = molecule); the molecules'
-    // handles point into them
-    double *M = nullptr, *Cm = nullptr, *rhoT = nullptr, *Om = nullptr, *E = nullptr;
-    double *K = nullptr, *dE = nullptr, *sd = nullptr, *Om0 = nullptr, *part = nullptr, *scal = nullptr;
-    double* vec[10] = {};  // e, e0, eG, Sig, Z, eLin, eGW, w, f, df: nOrb per molecule
-    int *info = nullptr, *active = nullptr, *nit = nullptr, *done = nullptr;
-    qk::Zs zs;
-    int nchunk = 1;
-    // evGW state on the device (qk_gwb_evgw): DIIS histories, per molecule
-    // [live, cycles, history length, -], Conv, and the live molecules' outputs
-    double *hist = nullptr, *conv = nullptr, *outSig = nullptr, *outZ = nullptr, *outOm = nullptr, *outEc = nullptr;
-    int* state = nullptr;
-    int hist_maxd = 0;
-    // hipGraph replay of the evGW cycle (qk_gwb_set_graphs): the launches
-    // before the batched eigensolve and those after it, captured on the
-    // second cycle of a call and replayed while the parameters stay the same
-    int graphs = 0;
-    hipGraphExec_t gpre = nullptr, gpost = nullptr;
-    std::vector<double> gkey;
-    long long graph_cycles = 0, eager_cycles = 0;
-    int precision = 0;
-    bool have_eri = false;
-    double t_stage[4] = {0, 0, 0, 0};  // AO->MO, eigensolves, excitation densities, self-energy + QP
-    std::string err;
-};
-
-namespace qk {
-namespace {
-
-using BCtx = qk_gwb_ctx;
-
-enum { V_E, V_E0, V_EG, V_SIG, V_Z, V_ELIN, V_EGW, V_W, V_F, V_DF };
-
-void drop_graphs(BCtx* b) {
-    for (hipGraphExec_t* g : {&b->gpre, &b->gpost}) {
-        if (*g) (void)hipGraphExecDestroy(*g);
-        *g = nullptr;
-    }
-    b->gkey.clear();
-}
-
-void release_batch(BCtx* b) {
-    if (b->s) (void)hipStreamSynchronize(b->s);
-    drop_graphs(b);
-    for (Ctx* c : b->mol) {
-        release(c);
-        delete c;
-    }
-    b->mol.clear();
-    for (double* p : {b->M, b->Cm, b->rhoT, b->Om, b->E, b->K, b->dE, b->sd, b->Om0, b->part, b->scal, b->hist, b->conv,
-                      b->outSig, b->outZ, b->outOm, b->outEc})
-        if (p) (void)hipFree(p);
-    for (double* p : b->vec)
-        if (p) (void)hipFree(p);
-    for (int* p : {b->info, b->active, b->nit, b->done, b->state})
-        if (p) (void)hipFree(p);
-    if (b->rb) (void)rocblas_destroy_handle(b->rb);
-    if (b->s) (void)hipStreamDestroy(b->s);
-}
-
-// One strided-batched rocSOLVER dsyevd for every molecule; with check, the
-// info flags are read back (the graph-replayed evGW reads them after the cycle)
-void syevd_batched(BCtx* b, bool vectors, bool check) {
-    const int nS = b->mol[0]->nS;
-    const long long S2 = (long long)nS * nS;
-    rb_check(rocsolver_dsyevd_strided_batched(b->rb, vectors ? rocblas_evect_original : rocblas_evect_none,
-                                              rocblas_fill_upper, nS, b->Cm, nS, S2, b->Om, nS, b->E, nS, b->info,
-                                              b->nmol),
-             "rocsolver_dsyevd_strided_batched");
-    if (check) {
-        std::vector<int> info(b->nmol);
-        QK_HIP(hipMemcpyAsync(info.data(), b->info, sizeof(int) * b->nmol, hipMemcpyDeviceToHost, b->s));
-        QK_HIP(hipStreamSynchronize(b->s));
-        for (int m = 0; m < b->nmol; ++m)
-            QK_CHECK(info[m] == 0, QK_E_EIG,
-                     "rocsolver_dsyevd_strided_batched failed for molecule " + std::to_string(m));
-    }
-}
-
-// Event-timed region on the batch stream (nothing is timed inside a graph capture)
-struct BStage {
-    BCtx* b;
-    int k;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    BStage(BCtx* b_, int k_) : b(b_), k(k_) {
-        if (stream_capturing(b->s)) return;
-        QK_HIP(hipEventCreate(&e0));
-        QK_HIP(hipEventCreate(&e1));
-        QK_HIP(hipEventRecord(e0, b->s));
-    }
-    ~BStage() {
-        if (!e0) return;
-        float ms = 0.f;
-        if (hipEventRecord(e1, b->s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
-            hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
-            b->t_stage[k] += ms;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-    }
-};
-
-// RPA matrices of every molecule at its energies e (device, nOrb per molecule):
-// one launch per kernel for the batch
-void rpa_build_batch(BCtx* b, const double* e, int tda) {
-    Ctx* c0 = b->mol[0];
-    rpa_build(b->s, e, b->K, b->dE, b->sd, b->Cm, b->scal, c0->nC, c0->nO, c0->v, c0->nS, tda, b->zs, b->nmol);
-}
-void rpa_finalize_batch(BCtx* b, int tda, bool vectors) {
-    rpa_finalize(b->s, b->Om, b->Cm, b->sd, b->scal, b->mol[0]->nS, tda, vectors, b->zs, b->nmol);
-}
-
-// screening of every molecule at its energies e: RPA build, one batched
-// eigensolve, finalisation (W scaling, EcRPA)
-void screening_batch(BCtx* b, const double* e, int tda, bool vectors) {
-    rpa_build_batch(b, e, tda);
-    {
-        BStage st(b, 1);
-        syevd_batched(b, vectors, true);
-    }
-    rpa_finalize_batch(b, tda, vectors);
-}
-
-void excitation_batch(BCtx* b) {
-    BStage st(b, 2);
-    Ctx* c0 = b->mol[0];
-    const long long n2 = (long long)c0->nOrb * c0->nOrb, nS = c0->nS;
-    for (Ctx* c : b->mol) c->rho_f32 = b->precision != 0;
-    if (b->precision == 0) {
-        Batch bt;
-        bt.count = b->nmol;
-        bt.zdiv = b->nmol;
-        bt.sa0 = nS * nS;
-        bt.sb0 = n2 * nS;
-        bt.sc0 = n2 * nS;
-        dgemm(b->s, true, true, (int)nS, (int)n2, (int)nS, 1.0, b->Cm, nS, b->M, n2, 0.0, b->rhoT, nS, bt);
-        return;
-    }
-    for (Ctx* c : b->mol) {
-        if (!c->Msplit.s[0] || c->Msplit.nsl != slices_of(b->precision)) {
-            QK_CHECK(!stream_capturing(b->s), QK_E_ARG, "split operands allocated inside a graph capture");
-            free_split(c->Wsplit);
-            free_split(c->Msplit);
-            alloc_split(c->Wsplit, c->nS, c->nS, slices_of(b->precision));
-            alloc_split(c->Msplit, (int)n2, c->nS, slices_of(b->precision));
-            c->msplit_valid = false;
-        }
-        if (!c->msplit_valid) {
-            split3(c->s, c->M, n2, false, (int)n2, c->nS, c->Msplit);
-            c->msplit_valid = true;
-        }
-        split3(c->s, c->Cm, c->nS, true, c->nS, c->nS, c->Wsplit);
-        gemm_bf16x3(c->s, c->nS, (int)n2, c->Wsplit, c->Msplit, reinterpret_cast<float*>(c->rhoT), c->nS);
-    }
-}
-
-// Diagonal self-energy of every molecule with w = e (the energies in G): Sig, Z, EcGM in scal[2]
-void sigma_batch(BCtx* b, const double* e, double eta, int srg) {
-    Ctx* c0 = b->mol[0];
-    const int n = c0->nOrb;
-    sigma_partial(b->s, b->rhoT, b->precision != 0, e, e, b->Om, nullptr, eta, srg, SRG_FLOW, n, c0->nC, c0->nO, c0->nR,
-                  c0->nS, b->nchunk, b->part, b->zs, b->nmol);
-    sigma_finalize(b->s, b->part, b->nchunk, n, c0->nC, c0->nO, c0->nR, b->vec[V_SIG], b->vec[V_Z], b->scal, b->zs,
-                   b->nmol);
-}
-
-// GW/RGW_QP_graph.f90:50-92 for every active orbital of every molecule, each
-// Newton sweep one launch pair for the batch.  eG: energies in G (eHF in
-// G0W0, eOld in evGW); start: eGWlin (G0W0) or eOld (evGW), also the fallback
-// of a failed solve.  Eagerly the sweeps stop once no orbital of any molecule
-// is active (one host read per sweep); with fixed_sweeps all QP_MAXIT sweeps
-// are issued, finished orbitals masked on the device, with the same results
-// and no host read (the form a graph captures).
-void qp_graph_batch(BCtx* b, double eta, int srg, const double* eG, const double* start, bool fixed_sweeps) {
-    Ctx* c0 = b->mol[0];
-    const int nm = b->nmol, n = c0->nOrb;
-    qp_init(b->s, start, b->vec[V_W], b->vec[V_F], b->nit, b->done, n, c0->nC, c0->nR, b->zs, nm);
-    std::vector<int> act(nm, 1);
-    for (int it = 0; it < QP_MAXIT; ++it) {
-        sigma_partial(b->s, b->rhoT, b->precision != 0, b->vec[V_W], eG, b->Om, b->done, eta, srg, SRG_FLOW, n, c0->nC,
-                      c0->nO, c0->nR, c0->nS, b->nchunk, b->part, b->zs, nm);
-        qp_update(b->s, b->part, b->nchunk, b->vec[V_E0], b->vec[V_W], b->vec[V_F], b->vec[V_DF], b->nit, b->done, n,
-                  QP_THRESH, QP_MAXIT, b->active, b->zs, nm);
-        if (fixed_sweeps) continue;
-        QK_HIP(hipMemcpyAsync(act.data(), b->active, sizeof(int) * nm, hipMemcpyDeviceToHost, b->s));
-        QK_HIP(hipStreamSynchronize(b->s));
-        if (std::all_of(act.begin(), act.end(), [](int a) { return a == 0; })) break;
-    }
-    qp_finish(b->s, b->vec[V_W], b->vec[V_DF], b->nit, start, b->vec[V_EGW], b->vec[V_Z], n, c0->nC, c0->nR, QP_MAXIT,
-              b->zs, nm);
-}
-
-// Capture what body enqueues on the batch stream into an executable graph
-// (thread-local capture mode: other host threads are unaffected).
-template <class F>
-hipGraphExec_t capture_graph(hipStream_t s, F&& body) {
-    QK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-        body();
-    } catch (...) {
-        hipGraph_t junk = nullptr;
-        (void)hipStreamEndCapture(s, &junk);
-        if (junk) (void)hipGraphDestroy(junk);
-        throw;
-    }
-    hipGraph_t g = nullptr;
-    QK_HIP(hipStreamEndCapture(s, &g));
-    hipGraphExec_t ex = nullptr;
-    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    QK_HIP(e);
-    return ex;
-}
-
-}  // namespace
-}  // namespace qk
-
-extern "C" {
-
-int qk_gwb_create(qk_gwb_handle* h, int nmol, int nBas, int nOrb, int nC, int nO, int nV, int nR, int nS) {
-    if (!h || nmol <= 0) return QK_E_ARG;
-    *h = nullptr;
-    BCtx* b = new (std::nothrow) BCtx();
-    if (!b) return QK_E_ALLOC;
-    int rc = guarded(&b->err, [&] {
-        check_sizes(nBas, nOrb, nC, nO, nV, nR, nS);
-        QK_CHECK(nmol <= 65535, QK_E_ARG, "at most 65535 molecules per batch");
-        b->nmol = nmol;
-        QK_HIP(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
-        rb_check(rocblas_create_handle(&b->rb), "rocblas_create_handle");
-        rb_check(rocblas_set_stream(b->rb, b->s), "rocblas_set_stream");
-        const size_t n = nOrb, S = nS, nm = nmol;
-        b->nchunk = sigma_chunks(nS, nOrb - nR - nC);
-        b->zs.n = (long long)n;
-        b->zs.S = (long long)S;
-        b->zs.S2 = (long long)(S * S);
-        b->zs.R = (long long)(n * n * S);
-        b->zs.P = 3LL * (long long)n * b->nchunk;
-        b->zs.c = 8;
-        b->zs.i = 1;
-        b->M = dalloc<double>(nm * n * n * S);
-        b->Cm = dalloc<double>(nm * S * S);
-        b->rhoT = dalloc<double>(nm * n * n * S);
-        b->Om = dalloc<double>(nm * S);
-        b->E = dalloc<double>(nm * S);
-        b->K = dalloc<double>(nm * S * S);
-        b->dE = dalloc<double>(nm * S);
-        b->sd = dalloc<double>(nm * S);
-        b->Om0 = dalloc<double>(nm * S);
-        b->part = dalloc<double>(nm * (size_t)b->zs.P);
-        b->scal = dalloc<double>(nm * 8);
-        for (double*& v : b->vec) v = dalloc<double>(nm * n);
-        b->info = dalloc<int>(nm);
-        b->active = dalloc<int>(nm);
-        b->nit = dalloc<int>(nm * n);
-        b->done = dalloc<int>(nm * n);
-        for (int m = 0; m < nmol; ++m) {
-            Ctx* c = new Ctx();
-            b->mol.push_back(c);
-            c->shared = true;
-            c->s = b->s;
-            c->rb = b->rb;
-            c->nBas = nBas; c->nOrb = nOrb; c->nC = nC; c->nO = nO; c->nV = nV; c->nR = nR; c->nS = nS;
-            c->o = nO - nC; c->v = nOrb - nR - nO;
-            c->M = b->M + m * n * n * S;
-            c->Cm = b->Cm + m * S * S;
-            c->rhoT = b->rhoT + m * n * n * S;
-            c->Om = b->Om + m * S;
-            c->work = b->E + m * S;
-            c->info = b->info + m;
-            c->active = b->active + m;
-            c->K = b->K + m * S * S;
-            c->dE = b->dE + m * S;
-            c->sd = b->sd + m * S;
-            c->Om0 = b->Om0 + m * S;
-            c->part = b->part + m * (size_t)b->zs.P;
-            c->scal = b->scal + m * 8;
-            double** vp[10] = {&c->e, &c->e0, &c->eG, &c->Sig, &c->Z, &c->eLin, &c->eGW, &c->w, &c->f, &c->df};
-            for (int k = 0; k < 10; ++k) *vp[k] = b->vec[k] + m * n;
-            c->nit = b->nit + m * n;
-            c->done = b->done + m * n;
-            c->nchunk = b->nchunk;
-        }
-    });
-    if (rc != QK_OK) {
-        g_last_error = b->err;
-        release_batch(b);
-        delete b;
-        return rc;
-    }
-    *h = b;
-    return QK_OK;
-}
-
-int qk_gwb_destroy(qk_gwb_handle h) {
-    if (!h) return QK_E_ARG;
-    release_batch(h);
-    delete h;
-    return QK_OK;
-}
-
-const char* qk_gwb_last_error(qk_gwb_handle h) { return h ? h->err.c_str() : g_last_error.c_str(); }
-
-int qk_gwb_set_precision(qk_gwb_handle h, int mode) {
-    if (!h || mode < 0 || mode > 2) return QK_E_ARG;
-    h->precision = mode;
-    for (Ctx* c : h->mol) c->precision = mode;
-    return QK_OK;
-}
-
-int qk_gwb_set_graphs(qk_gwb_handle h, int on) {
-    if (!h) return QK_E_ARG;
-    return guarded(&h->err, [&] {
-        QK_HIP(hipStreamSynchronize(h->s));
-        drop_graphs(h);
-        h->graphs = on ? 1 : 0;
-    });
-}
-
-int qk_gwb_graph_stats(qk_gwb_handle h, long long* counts, int n) {
-    if (!h || !counts || n < 0 || n > 2) return QK_E_ARG;
-    const long long v[2] = {h->eager_cycles, h->graph_cycles};
-    for (int i = 0; i < n; ++i) counts[i] = v[i];
-    return QK_OK;
-}
-
-int qk_gwb_set_eri_ao_device(qk_gwb_handle h, const double* ERI_AO, long long stride_eri, const double* c,
-                             long long stride_c) {
-    if (!h || !ERI_AO || !c) return QK_E_ARG;
-    return guarded(&h->err, [&] {
-        BStage st(h, 0);
-        const bool sp = split_aotomo();
-        for (int m = 0; m < h->nmol; ++m) {
-            Ctx* x = h->mol[m];
-            aotomo_block(h->s, ERI_AO + m * stride_eri, c + m * stride_c, x->M, x->nBas, x->nOrb, x->nC, x->nO,
-                         x->nR, sp, x->K);
-            if (!sp) extract_k(h->s, x->M, x->K, x->nOrb, x->nC, x->nO, x->v, x->nS);
-            x->msplit_valid = false;
-            x->have_eri = true;
-        }
-        QK_HIP(hipStreamSynchronize(h->s));
-        h->have_eri = true;
-    });
-}
-
-int qk_gwb_set_eri_mo(qk_gwb_handle h, int mol, const double* ERI_MO) {
-    if (!h || !ERI_MO || mol < 0 || mol >= h->nmol) return QK_E_ARG;
-    return guarded(&h->err, [&] {
-        Ctx* x = h->mol[mol];
-        upload_mo_block(h->s, ERI_MO, x->M, x->nOrb, x->nC, x->nO, x->nR);
-        extract_k(h->s, x->M, x->K, x->nOrb, x->nC, x->nO, x->v, x->nS);
-        QK_HIP(hipStreamSynchronize(h->s));
-        x->msplit_valid = false;
-        x->have_eri = true;
-        h->have_eri = std::all_of(h->mol.begin(), h->mol.end(), [](Ctx* c) { return c->have_eri; });
-    });
-}
-
-int qk_gwb_g0w0(qk_gwb_handle h, const double* eHF, double eta, int linearize, int TDA_W, int doSRG, double* eGW,
-                double* SigC, double* Z, double* Om, double* EcRPA, double* EcGM) {
-    if (!h || !eHF || !eGW) return QK_E_ARG;
-    return guarded(&h->err, [&] {
-        QK_CHECK(h->have_eri, QK_E_ARG, "no integrals for some molecule: call qk_gwb_set_eri_* first");
-        for (double& t : h->t_stage) t = 0.0;
-        const int n = h->mol[0]->nOrb, nS = h->mol[0]->nS, nm = h->nmol;
-        const size_t nn = (size_t)nm * n, nnS = (size_t)nm * nS;
-        hipStream_t s = h->s;
-        QK_HIP(hipMemcpyAsync(h->vec[V_E0], eHF, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-        // GW/RG0W0.f90:116-127: screening at eHF, one batched eigensolve
-        screening_batch(h, h->vec[V_E0], TDA_W, true);
-        QK_HIP(hipMemcpyAsync(h->Om0, h->Om, sizeof(double) * nnS, hipMemcpyDeviceToDevice, s));
-        excitation_batch(h);
-        {
-            BStage st(h, 3);
-            sigma_batch(h, h->vec[V_E0], eta, doSRG);
-            lin_qp(s, h->vec[V_E0], h->vec[V_Z], h->vec[V_SIG], h->vec[V_ELIN], n, 1, h->zs, nm);
-            if (linearize)
-                QK_HIP(hipMemcpyAsync(h->vec[V_EGW], h->vec[V_ELIN], sizeof(double) * nn, hipMemcpyDeviceToDevice, s));
-            else
-                qp_graph_batch(h, eta, doSRG, h->vec[V_E0], h->vec[V_ELIN], false);
-        }
-        std::vector<double> sig(nn), sc1((size_t)nm * 8), sc2((size_t)nm * 8);
-        QK_HIP(hipMemcpyAsync(sig.data(), h->vec[V_SIG], sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-        QK_HIP(hipMemcpyAsync(sc1.data(), h->scal, sizeof(double) * nm * 8, hipMemcpyDeviceToHost, s));  // [2] = EcGM
-        // RPA correlation energy at eGW (GW/RG0W0.f90:222-225): eigenvalues only
-        screening_batch(h, h->vec[V_EGW], TDA_W, false);
-        QK_HIP(hipMemcpyAsync(sc2.data(), h->scal, sizeof(double) * nm * 8, hipMemcpyDeviceToHost, s));
-        QK_HIP(hipMemcpyAsync(eGW, h->vec[V_EGW], sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-        if (Z) QK_HIP(hipMemcpyAsync(Z, h->vec[V_Z], sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-        if (Om) QK_HIP(hipMemcpyAsync(Om, h->Om0, sizeof(double) * nnS, hipMemcpyDeviceToHost, s));
-        QK_HIP(hipStreamSynchronize(s));
-        if (SigC) std::memcpy(SigC, sig.data(), sizeof(double) * nn);
-        for (int m = 0; m < nm; ++m) {
-            if (EcRPA) EcRPA[m] = sc2[(size_t)m * 8 + 1];
-            if (EcGM) EcGM[m] = sc1[(size_t)m * 8 + 2];
-        }
-    });
-}
-
-int qk_gwb_evgw(qk_gwb_handle h, const double* eHF, const double* eGW_start, int maxSCF, double thresh, int max_diis,
-                double eta, int linearize, int TDA_W, int doSRG, double* eGW, double* SigC, double* Z, double* Om,
-                double* EcRPA, double* EcGM, int* nSCF_out) {
-    if (!h || !eHF || !eGW) return QK_E_ARG;
-    int conv_status = QK_OK;
-    int rc = guarded(&h->err, [&] {
-        QK_CHECK(h->have_eri, QK_E_ARG, "no integrals for some molecule: call qk_gwb_set_eri_* first");
-        QK_CHECK(maxSCF >= 0, QK_E_ARG, "maxSCF < 0");
-        QK_CHECK(max_diis <= 32, QK_E_ARG, "batched evGW: max_diis > 32");
-        for (double& t : h->t_stage) t = 0.0;
-        const int n = h->mol[0]->nOrb, nS = h->mol[0]->nS, nm = h->nmol;
-        const size_t nn = (size_t)nm * n, nnS = (size_t)nm * nS;
-        const int maxd = std::max(max_diis, 1);
-        hipStream_t s = h->s;
-        if (h->hist_maxd < maxd) {
-            QK_HIP(hipStreamSynchronize(s));
-            drop_graphs(h);  // the graphs hold the old history buffer
-            dfree(h->hist);
-            h->hist = dalloc<double>((size_t)nm * 2 * n * maxd);
-            h->hist_maxd = maxd;
-        }
-        if (!h->state) {
-            h->conv = dalloc<double>(nm);
-            h->outSig = dalloc<double>(nn);
-            h->outZ = dalloc<double>(nn);
-            h->outOm = dalloc<double>(nnS);
-            h->outEc = dalloc<double>((size_t)nm * 2);
-            h->state = dalloc<int>((size_t)nm * 4);
-        }
-        // per molecule: GW/evRGW.f90:104-183 (current energies, DIIS history,
-        // cycle count, Conv) on the device; a molecule that has converged
-        // keeps its last cycle's outputs and rides along with frozen energies
-        std::vector<int> st((size_t)nm * 4, 0), info(nm);
-        for (int m = 0; m < nm; ++m) st[(size_t)m * 4] = 1.0 > thresh ? 1 : 0;  // Conv = 1 before the first cycle
-        std::vector<double> zeros(std::max(nn, nnS), 0.0);
-        QK_HIP(hipMemcpyAsync(h->state, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(h->vec[V_E], eGW_start ? eGW_start : eHF, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(h->vec[V_E0], eHF, sizeof(double) * nn, hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(h->outSig, zeros.data(), sizeof(double) * nn, hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(h->outZ, zeros.data(), sizeof(double) * nn, hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(h->outOm, zeros.data(), sizeof(double) * nnS, hipMemcpyHostToDevice, s));
-        QK_HIP(hipMemcpyAsync(h->outEc, zeros.data(), sizeof(double) * nm * 2, hipMemcpyHostToDevice, s));
-        QK_HIP(hipStreamSynchronize(s));
-        // the cycle: launches before the batched eigensolve, the eigensolve,
-        // launches after it (GW/evRGW.f90:119-183)
-        auto pre = [&] { rpa_build_batch(h, h->vec[V_E], TDA_W); };
-        auto post = [&](bool fixed_sweeps) {
-            rpa_finalize_batch(h, TDA_W, true);
-            excitation_batch(h);
-            BStage stq(h, 3);
-            sigma_batch(h, h->vec[V_E], eta, doSRG);
-            if (linearize)
-                lin_qp(s, h->vec[V_E0], h->vec[V_Z], h->vec[V_SIG], h->vec[V_EGW], n, 0, h->zs, nm);
-            else
-                qp_graph_batch(h, eta, doSRG, h->vec[V_E], h->vec[V_E], fixed_sweeps);
-            evgw_update(s, h->vec[V_EGW], h->vec[V_E], h->vec[V_SIG], h->vec[V_Z], h->Om, h->scal, h->hist, h->state,
-                        h->conv, h->outSig, h->outZ, h->outOm, h->outEc, n, nS, maxd, thresh, maxSCF, h->zs, nm);
-        };
-        const std::vector<double> key = {eta, (double)linearize, (double)TDA_W, (double)doSRG, (double)maxd, thresh,
-                                         (double)maxSCF, (double)h->precision};
-        for (int cyc = 0;; ++cyc) {
-            // the first cycle of a call runs eagerly: it tunes the GEMM shapes
-            // and allocates the split operands, which a capture cannot do
-            if (h->graphs && cyc > 0) {
-                if (!h->gpre || h->gkey != key) {
-                    drop_graphs(h);
-                    h->gpre = capture_graph(s, pre);
-                    h->gpost = capture_graph(s, [&] { post(true); });
-                    h->gkey = key;
-                }
-                QK_HIP(hipGraphLaunch(h->gpre, s));
-                {
-                    BStage se(h, 1);
-                    syevd_batched(h, true, false);
-                }
-                QK_HIP(hipGraphLaunch(h->gpost, s));
-                ++h->graph_cycles;
-            } else {
-                pre();
-                {
-                    BStage se(h, 1);
-                    syevd_batched(h, true, false);
-                }
-                post(false);
-                ++h->eager_cycles;
-            }
-            QK_HIP(hipMemcpyAsync(st.data(), h->state, sizeof(int) * st.size(), hipMemcpyDeviceToHost, s));
-            QK_HIP(hipMemcpyAsync(info.data(), h->info, sizeof(int) * nm, hipMemcpyDeviceToHost, s));
-            QK_HIP(hipStreamSynchronize(s));
-            for (int m = 0; m < nm; ++m)
-                QK_CHECK(info[m] == 0, QK_E_EIG,
-                         "rocsolver_dsyevd_strided_batched failed for molecule " + std::to_string(m));
-            bool any = false;
-            for (int m = 0; m < nm; ++m) any = any || st[(size_t)m * 4] != 0;
-            if (!any) break;
-        }
-        std::vector<double> ec((size_t)nm * 2);
-        QK_HIP(hipMemcpyAsync(eGW, h->vec[V_E], sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-        if (SigC) QK_HIP(hipMemcpyAsync(SigC, h->outSig, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-        if (Z) QK_HIP(hipMemcpyAsync(Z, h->outZ, sizeof(double) * nn, hipMemcpyDeviceToHost, s));
-        if (Om) QK_HIP(hipMemcpyAsync(Om, h->outOm, sizeof(double) * nnS, hipMemcpyDeviceToHost, s));
-        QK_HIP(hipMemcpyAsync(ec.data(), h->outEc, sizeof(double) * nm * 2, hipMemcpyDeviceToHost, s));
-        QK_HIP(hipStreamSynchronize(s));
-        bool failed = false;
-        for (int m = 0; m < nm; ++m) {
-            if (EcRPA) EcRPA[m] = ec[(size_t)m * 2];
-            if (EcGM) EcGM[m] = ec[(size_t)m * 2 + 1];
-            if (nSCF_out) nSCF_out[m] = st[(size_t)m * 4 + 1];
-            failed = failed || st[(size_t)m * 4 + 1] == maxSCF + 1;
-        }
-        if (failed) {
-            conv_status = QK_E_NOCONV;
-            h->err = "evGW: convergence failed for some molecule (nSCF = maxSCF + 1)";
-        }
-    });
-    return rc != QK_OK ? rc : conv_status;
-}
-
-int qk_gwb_stage_times(qk_gwb_handle h, double* ms, int n) {
-    if (!h || !ms || n < 0 || n > 4) return QK_E_ARG;
-    for (int i = 0; i < n; ++i) ms[i] = h->t_stage[i];
-    return QK_OK;
-}
-
-
 int qk_set_split_aotomo(int on) {
     qk::g_split_aotomo.store(on ? 1 : 0);
-    return QK_OK;
-}
-
-int qk_set_unrestricted_eigensolver(int mode, double de_max) {
-    if (mode < 0 || mode > 3 || !(de_max >= 0.0)) return QK_E_ARG;
-    qk::g_u_eig_mode.store(mode);
-    qk::g_u_eig_demax.store(de_max);
-    return QK_OK;
-}
-
-int qk_unrestricted_eig_stats(long long* counts, int n) {
-    if (!counts || n < 0 || n > 9) return QK_E_ARG;
-    for (int i = 0; i < n; ++i) counts[i] = qk::g_u_eig_count[i];
     return QK_OK;
 }
 

@@ -1,0 +1,397 @@
+// Unrestricted G0W0 and evGW (qk_ug0w0, qk_evugw; GW/UG0W0.f90:92-150,
+// GW/evUGW.f90:103-215 with nC = nR = 0), and the host-array self-energy and
+// graphical QP entries (qk_rgw_self_energy_diag, qk_rgw_qp_graph) that share
+// their device work buffers.
+#include "qk_gw_internal.h"
+
+namespace qk {
+namespace {
+// Device work of the diagonal self-energy and the graphical QP solve for n
+// orbitals and nch chunks of modes: partial sums, Newton state and the
+// orbital-energy vectors, allocated once by their owner.
+struct QpWork {
+    int n = 0, nch = 0;
+    double *part = nullptr, *w = nullptr, *f = nullptr, *df = nullptr, *sig = nullptr, *z = nullptr, *sc = nullptr;
+    double *e0 = nullptr, *eG = nullptr, *eL = nullptr, *eo = nullptr;
+    int *nit = nullptr, *done = nullptr, *act = nullptr;
+    QpWork(int n_, int nch_) : n(n_), nch(nch_) {
+        try {
+            part = dalloc<double>((size_t)3 * n * nch);
+            for (double** p : {&w, &f, &df, &sig, &z, &e0, &eG, &eL, &eo}) *p = dalloc<double>(n);
+            sc = dalloc<double>(4);
+            nit = dalloc<int>(n);
+            done = dalloc<int>(n);
+            act = dalloc<int>(1);
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    QpWork(const QpWork&) = delete;
+    QpWork& operator=(const QpWork&) = delete;
+    ~QpWork() { release(); }
+    void release() {
+        for (double** p : {&part, &w, &f, &df, &sig, &z, &sc, &e0, &eG, &eL, &eo}) dfree(*p);
+        dfree(nit);
+        dfree(done);
+        dfree(act);
+    }
+};
+
+// Diagonal self-energy (GW/RGW_self_energy_diag.f90 or the SRG form) at the
+// host energies e, from device rT (rho(p, q, m) at m + nS (p + n q)) and Om;
+// SigC, Z and EcGM to the host.
+void sigma_dev(hipStream_t s, QpWork& q, const double* rT, const double* dO, int doSRG, double eta, int nC, int nO,
+               int nR, int nS, const double* e, double* SigC, double* Z, double* EcGM) {
+    const int n = q.n;
+    QK_HIP(hipMemcpyAsync(q.e0, e, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    sigma_partial(s, rT, false, q.e0, q.e0, dO, nullptr, eta, doSRG, SRG_FLOW, n, nC, nO, nR, nS, q.nch, q.part);
+    sigma_finalize(s, q.part, q.nch, n, nC, nO, nR, q.sig, q.z, q.sc);
+    QK_HIP(hipMemcpyAsync(SigC, q.sig, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    QK_HIP(hipMemcpyAsync(Z, q.z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    double gm = 0.0;
+    QK_HIP(hipMemcpyAsync(&gm, q.sc + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+    QK_HIP(hipStreamSynchronize(s));
+    if (EcGM) *EcGM = gm;
+}
+
+// Graphical QP solve (GW/RGW_QP_graph.f90:50-92) from host energies, device rT
+// and Om; the Newton sweeps stop once no orbital is left active (read every
+// QP_SYNC_EVERY sweeps).  Z is
+// read (frozen orbitals keep it) and written on the host.
+void qp_dev(hipStream_t s, QpWork& q, const double* rT, const double* dO, int doSRG, double eta, double flow, int nC,
+            int nO, int nR, int nS, const double* eHF, const double* eGWlin, const double* eOld, double* eGW,
+            double* Z) {
+    const int n = q.n;
+    QK_HIP(hipMemcpyAsync(q.e0, eHF, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(q.eG, eOld, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(q.eL, eGWlin, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    QK_HIP(hipMemcpyAsync(q.z, Z, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    qp_init(s, q.eL, q.w, q.f, q.nit, q.done, n, nC, nR);
+    int h_active = 1;
+    for (int it = 0; it < QP_MAXIT && h_active > 0;) {
+        for (int k = 0; k < QP_SYNC_EVERY && it < QP_MAXIT; ++k, ++it) {
+            sigma_partial(s, rT, false, q.w, q.eG, dO, q.done, eta, doSRG, flow, n, nC, nO, nR, nS, q.nch, q.part);
+            qp_update(s, q.part, q.nch, q.e0, q.w, q.f, q.df, q.nit, q.done, n, QP_THRESH, QP_MAXIT, q.act);
+        }
+        QK_HIP(hipMemcpyAsync(&h_active, q.act, sizeof(int), hipMemcpyDeviceToHost, s));
+        QK_HIP(hipStreamSynchronize(s));
+    }
+    qp_finish(s, q.w, q.df, q.nit, q.eL, q.eo, q.z, n, nC, nR, QP_MAXIT);
+    QK_HIP(hipMemcpyAsync(eGW, q.eo, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    QK_HIP(hipMemcpyAsync(Z, q.z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    QK_HIP(hipStreamSynchronize(s));
+}
+
+// Host rho(p,q,m) (n x n x nS, column-major) and Om uploaded for one
+// self-energy or QP call of the C-ABI (qk_rgw_self_energy_diag, qk_rgw_qp_graph).
+struct HostRho {
+    hipStream_t s;
+    double *rT = nullptr, *dO = nullptr;
+    HostRho(hipStream_t s_, const double* rho, const double* Om, int n, int nS) : s(s_) {
+        try {
+            rT = upload_rho(s, rho, n, nS);
+            dO = dalloc<double>(nS);
+            QK_HIP(hipMemcpyAsync(dO, Om, sizeof(double) * nS, hipMemcpyHostToDevice, s));
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    ~HostRho() { release(); }
+    void release() {
+        (void)hipStreamSynchronize(s);
+        dfree(rT);
+        dfree(dO);
+    }
+};
+
+// Eigensolver of the unrestricted screening (qk_set_unrestricted_eigensolver),
+// process-wide, and the eigensolver counts of the calling thread's last
+// qk_ug0w0 / qk_evugw (qk_unrestricted_eig_stats).
+std::atomic<int> g_u_eig_mode{0};
+std::atomic<double> g_u_eig_demax{5e-3};
+thread_local long long g_u_eig_count[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+// Unrestricted screening and excitation densities (nC = nR = 0), shared by
+// UG0W0 and evUGW.  Pair index ia = a' + v_s i' within spin block s, the beta
+// block after the alpha one (LR/phULR_A.f90, LR/phULR_B.f90).  The integral
+// slabs M_s(pq, k) = (pq|k) of each spin's densities
+// (GW/UGW_excitation_density.f90:24-62) and the coupling K = (ia|jb) / 2
+// gathered from them stay on the device for the whole calculation.  The dRPA
+// A = D + K_u, B = K_u (K_u = 2K) has A - B = D, so each screening is the
+// restricted engine's (screening(): C = D^2 + 2 D^1/2 K_u D^1/2, or A under
+// TDA, built on the device; the engine's eigensolver, with Newton refinement
+// of the previous cycle's vectors when it is switched on), held by a
+// screening-only engine handle whose pairs are the unrestricted ones.  It
+// leaves Om and XpY^T = W on the device, then rho_s = M_s W / sqrt(2) in the
+// restricted kernels' rhoT layout (m contiguous): the unrestricted sums weigh
+// rho^2 where the restricted kernels weigh 2 rho^2.
+struct UScreen {
+    int n, nO[2], v[2], nSa, nSb, nSt, TDA_W;
+    size_t S2, n2;
+    const double* G[3];
+    Ctx* c = nullptr;
+    hipStream_t st = nullptr;
+    double* de = nullptr;
+    double *dM[2] = {nullptr, nullptr}, *dR[2] = {nullptr, nullptr};
+    QpWork* q = nullptr;
+
+    UScreen(int n_, int nOa, int nOb, const double* Gaaaa, const double* Gaabb, const double* Gbbbb, int tda)
+        : n(n_), nO{nOa, nOb}, v{n_ - nOa, n_ - nOb}, TDA_W(tda), G{Gaaaa, Gaabb, Gbbbb} {
+        nSa = nO[0] * v[0];
+        nSb = nO[1] * v[1];
+        nSt = nSa + nSb;
+        QK_CHECK((long long)nSt * nSt < (1LL << 40), QK_E_ARG, "nSa + nSb too large");
+        S2 = (size_t)nSt * nSt;
+        n2 = (size_t)n * n;
+        QK_CHECK((long long)n2 < (1LL << 31), QK_E_ARG, "nBas^2 too large");
+        try {
+            c = new Ctx();
+            c->nBas = c->nOrb = n;
+            c->nC = c->nR = 0;
+            c->nO = nOa;
+            c->nOb = nOb;
+            c->nV = n - nOa;
+            c->o = nOa;
+            c->v = n - nOa;
+            c->nS = nSt;
+            QK_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+            rb_check(rocblas_create_handle(&c->rb), "rocblas_create_handle");
+            rb_check(rocblas_set_stream(c->rb, c->s), "rocblas_set_stream");
+            for (auto* e : {&c->ev[0], &c->ev[1], &c->ev2[0], &c->ev2[1], &c->ev3[0], &c->ev3[1], &c->ev3[2],
+                            &c->ev3[3]})
+                QK_HIP(hipEventCreate(e));
+            st = c->s;
+            c->K = dalloc<double>(S2);
+            c->Cm = dalloc<double>(S2);
+            for (double** p : {&c->dE, &c->sd, &c->Om, &c->work}) *p = dalloc<double>(nSt);
+            c->scal = dalloc<double>(8);
+            c->info = dalloc<int>(1);
+            de = dalloc<double>(2 * (size_t)n);
+            std::vector<double> Ms(n2 * nSt);
+            for (int sp = 0; sp < 2; ++sp) {
+                dM[sp] = dalloc<double>(n2 * nSt);
+                dR[sp] = dalloc<double>(n2 * nSt);
+                for (int sj = 0; sj < 2; ++sj)
+                    for (int j = 0; j < nO[sj]; ++j)
+                        for (int b = 0; b < v[sj]; ++b) {
+                            const size_t k = (sj ? nSa : 0) + b + (size_t)v[sj] * j;
+                            const int Bb = nO[sj] + b;
+                            for (int qq = 0; qq < n; ++qq)
+                                for (int p = 0; p < n; ++p)
+                                    Ms[(p + (size_t)n * qq) + n2 * k] =
+                                        sp == 0 ? at(sj == 0 ? 0 : 1, p, j, qq, Bb)
+                                                : (sj == 0 ? at(1, j, p, Bb, qq) : at(2, p, j, qq, Bb));
+                        }
+                QK_HIP(hipMemcpyAsync(dM[sp], Ms.data(), sizeof(double) * n2 * nSt, hipMemcpyHostToDevice, st));
+                QK_HIP(hipStreamSynchronize(st));  // Ms is refilled for the next spin
+            }
+            extract_ku(st, dM[0], dM[1], c->K, n, nO[0], nO[1]);
+            const int rc = qk_gw_set_eigensolver(c, g_u_eig_mode.load(), g_u_eig_demax.load());
+            QK_CHECK(rc == QK_OK, rc, "unrestricted screening: " + c->err);
+            q = new QpWork(n, sigma_chunks(nSt, n));
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    UScreen(const UScreen&) = delete;
+    UScreen& operator=(const UScreen&) = delete;
+    ~UScreen() { release(); }
+    void release() {
+        if (st) (void)hipStreamSynchronize(st);
+        delete q;
+        q = nullptr;
+        for (double** p : {&de, &dM[0], &dM[1], &dR[0], &dR[1]}) dfree(*p);
+        if (c) {
+            std::copy(c->eig_count, c->eig_count + 9, g_u_eig_count);
+            qk::release(c);
+            delete c;
+            c = nullptr;
+        }
+        st = nullptr;
+    }
+    double at(int k, int p, int qq, int r, int t) const {  // ERI_k(p, q, r, t), 0-based
+        return G[k][p + (size_t)n * (qq + (size_t)n * (r + (size_t)n * t))];
+    }
+    // LR/phULR.f90 at energies en (n x 2, host); returns EcRPA, leaves Om and
+    // W = XpY^T on the device
+    double rpa(const double* en, const char* who) {
+        if (!TDA_W) {
+            // LR/phRLR.f90:57-58 (the unrestricted solver shares it): A - B
+            // must be positive definite; here it is diag(e_a - e_i)
+            double dmin = HUGE_VAL;
+            for (int sp = 0; sp < 2; ++sp)
+                for (int i = 0; i < nO[sp]; ++i)
+                    for (int a = nO[sp]; a < n; ++a) dmin = std::min(dmin, en[(size_t)n * sp + a] - en[(size_t)n * sp + i]);
+            QK_CHECK(dmin > 0.0, QK_E_UNSTABLE,
+                     std::string(who) + ": A-B is not positive definite (smallest eigenvalue " +
+                         std::to_string(dmin) + "): linear-response instability");
+        }
+        QK_HIP(hipMemcpyAsync(de, en, sizeof(double) * 2 * n, hipMemcpyHostToDevice, st));
+        screening(c, de, en, TDA_W, true);
+        double ec = 0.0;
+        QK_HIP(hipMemcpyAsync(&ec, c->scal + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
+        return ec;
+    }
+    // GW/UGW_excitation_density.f90 for both spins from the last rpa():
+    // rho_s(m, pq) = sum_k W(k, m) M_s(pq, k) / sqrt(2)
+    void densities() {
+        for (int sp = 0; sp < 2; ++sp)
+            dgemm(st, true, true, nSt, (int)n2, nSt, 1.0 / std::sqrt(2.0), c->Cm, nSt, dM[sp], n2, 0.0, dR[sp], nSt);
+    }
+    void om_host(double* out) {
+        QK_HIP(hipMemcpyAsync(out, c->Om, sizeof(double) * nSt, hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
+    }
+    // per spin: diagonal self-energy at e_s (GW/UGW_self_energy_diag.f90 or the SRG form);
+    // EcGM per spin, halved (the restricted sum carries 4 rho^2, the unrestricted rho^2, x2 here)
+    void sigma(int sp, const double* es, double eta, int doSRG, double* sig, double* z, double* ecgm) {
+        double gm = 0.0;
+        sigma_dev(st, *q, dR[sp], c->Om, doSRG, eta, 0, nO[sp], 0, nSt, es, sig, z, &gm);
+        if (ecgm) *ecgm = 0.5 * gm;
+    }
+    // GW/UGW_QP_graph.f90 for spin sp
+    void qp(int sp, int doSRG, double eta, const double* eHF_s, const double* eLin, const double* eOld, double* eGW_s,
+            double* z) {
+        qp_dev(st, *q, dR[sp], c->Om, doSRG, eta, SRG_FLOW, 0, nO[sp], 0, nSt, eHF_s, eLin, eOld, eGW_s, z);
+    }
+};
+}  // namespace
+}  // namespace qk
+
+using namespace qk;
+
+extern "C" {
+
+int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double* e,
+                            const double* Om, const double* rho, double* EcGM, double* SigC, double* Z) {
+    if (!e || !Om || !rho || !SigC || !Z) return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        check_sizes(nOrb, nOrb, nC, nO, nV, nR, nS);
+        hipStream_t s = scratch().s;
+        HostRho r(s, rho, Om, nOrb, nS);
+        QpWork q(nOrb, sigma_chunks(nS, nOrb - nR - nC));
+        sigma_dev(s, q, r.rT, r.dO, doSRG, eta, nC, nO, nR, nS, e, SigC, Z, EcGM);
+    });
+}
+
+int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO, int nV, int nR, int nS,
+                    const double* eHF, const double* Om, const double* rho, const double* eGWlin, const double* eOld,
+                    double* eGW, double* Z) {
+    if (!eHF || !Om || !rho || !eGWlin || !eOld || !eGW || !Z) return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        check_sizes(nOrb, nOrb, nC, nO, nV, nR, nS);
+        hipStream_t s = scratch().s;
+        HostRho r(s, rho, Om, nOrb, nS);
+        QpWork q(nOrb, sigma_chunks(nS, nOrb - nR - nC));
+        qp_dev(s, q, r.rT, r.dO, doSRG, eta, flow, nC, nO, nR, nS, eHF, eGWlin, eOld, eGW, Z);
+    });
+}
+
+
+int qk_ug0w0(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* ERI_aabb, const double* ERI_bbbb,
+             const double* eHF, double eta, int linearize, int TDA_W, int doSRG, double* eGW, double* SigC, double* Z,
+             double* Om, double* EcRPA, double* EcGM) {
+    if (nBas <= 0 || nOa <= 0 || nOb <= 0 || nOa >= nBas || nOb >= nBas || !ERI_aaaa || !ERI_aabb || !ERI_bbbb ||
+        !eHF || !eGW || !SigC || !Z)
+        return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        const int n = nBas;
+        UScreen u(n, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
+        u.rpa(eHF, "qk_ug0w0");
+        u.densities();
+        if (Om) u.om_host(Om);
+        // GW/UG0W0.f90:104-134 per spin: self-energy at eHF, linearised energies,
+        // graphical QP with the linearised guess and G from eHF
+        for (int sp = 0; sp < 2; ++sp) {
+            const double* es = eHF + (size_t)n * sp;
+            double *sig = SigC + (size_t)n * sp, *z = Z + (size_t)n * sp, *eg = eGW + (size_t)n * sp;
+            u.sigma(sp, es, eta, doSRG, sig, z, EcGM ? EcGM + sp : nullptr);
+            std::vector<double> lin(n);
+            for (int p = 0; p < n; ++p) lin[p] = es[p] + z[p] * sig[p];
+            if (linearize) std::memcpy(eg, lin.data(), sizeof(double) * n);
+            else u.qp(sp, doSRG, eta, es, lin.data(), es, eg, z);
+        }
+        // RPA correlation energy at eGW (GW/UG0W0.f90:140-142)
+        const double ec = u.rpa(eGW, "qk_ug0w0 (RPA at eGW)");
+        if (EcRPA) *EcRPA = ec;
+    });
+}
+
+int qk_evugw(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* ERI_aabb, const double* ERI_bbbb,
+             const double* eHF, int maxSCF, double thresh, int max_diis, double eta, int linearize, int TDA_W,
+             int doSRG, double* eGW, double* SigC, double* Z, double* Om, double* EcRPA, double* EcGM, int* nSCF_out) {
+    if (nBas <= 0 || nOa <= 0 || nOb <= 0 || nOa >= nBas || nOb >= nBas || !ERI_aaaa || !ERI_aabb || !ERI_bbbb ||
+        !eHF || !eGW || !SigC || !Z || maxSCF < 0)
+        return QK_E_ARG;
+    int conv_status = QK_OK;
+    int rc = guarded(nullptr, [&] {
+        const int n = nBas;
+        const size_t n2s = (size_t)n * 2;
+        UScreen u(n, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
+        // GW/evUGW.f90:103-215: eGW = eOld = eHF, Z = 1; per cycle screening at eGW,
+        // self-energy at eGW, QP energies (linearised: eHF + SigC; graphical: guess and
+        // G from eOld), Conv = max |eGW - eOld| over both spins, DIIS per spin with
+        // one shared history length
+        std::vector<double> e(eHF, eHF + n2s), eOld = e, err(n2s);
+        for (size_t p = 0; p < n2s; ++p) Z[p] = 1.0;
+        Diis diis[2] = {Diis(n, std::max(max_diis, 1)), Diis(n, std::max(max_diis, 1))};
+        int nSCF = 0;
+        double Conv = 1.0, ecrpa = 0.0, ecgm[2] = {0.0, 0.0};
+        while (Conv > thresh && nSCF <= maxSCF) {
+            ecrpa = u.rpa(e.data(), "qk_evugw");
+            u.densities();
+            for (int sp = 0; sp < 2; ++sp) {
+                const size_t o = (size_t)n * sp;
+                u.sigma(sp, e.data() + o, eta, doSRG, SigC + o, Z + o, ecgm + sp);
+                if (linearize) {
+                    for (int p = 0; p < n; ++p) e[o + p] = eHF[o + p] + SigC[o + p];
+                } else {
+                    std::vector<double> eg(n);
+                    u.qp(sp, doSRG, eta, eHF + o, eOld.data() + o, eOld.data() + o, eg.data(), Z + o);
+                    std::memcpy(e.data() + o, eg.data(), sizeof(double) * n);
+                }
+            }
+            Conv = 0.0;
+            for (size_t p = 0; p < n2s; ++p) {
+                err[p] = e[p] - eOld[p];
+                Conv = std::max(Conv, std::fabs(err[p]));
+            }
+            if (max_diis > 1)
+                for (int sp = 0; sp < 2; ++sp) diis[sp].step(err.data() + (size_t)n * sp, e.data() + (size_t)n * sp);
+            eOld = e;
+            ++nSCF;
+        }
+        std::memcpy(eGW, e.data(), sizeof(double) * n2s);
+        if (Om) u.om_host(Om);
+        if (EcRPA) *EcRPA = ecrpa;
+        if (EcGM) {
+            EcGM[0] = ecgm[0];
+            EcGM[1] = ecgm[1];
+        }
+        if (nSCF_out) *nSCF_out = nSCF;
+        if (nSCF == maxSCF + 1) {
+            conv_status = QK_E_NOCONV;
+            g_last_error = "evUGW: convergence failed after " + std::to_string(nSCF) + " cycles";
+        }
+    });
+    return rc != QK_OK ? rc : conv_status;
+}
+
+int qk_set_unrestricted_eigensolver(int mode, double de_max) {
+    if (mode < 0 || mode > 3 || !(de_max >= 0.0)) return QK_E_ARG;
+    qk::g_u_eig_mode.store(mode);
+    qk::g_u_eig_demax.store(de_max);
+    return QK_OK;
+}
+
+int qk_unrestricted_eig_stats(long long* counts, int n) {
+    if (!counts || n < 0 || n > 9) return QK_E_ARG;
+    for (int i = 0; i < n; ++i) counts[i] = qk::g_u_eig_count[i];
+    return QK_OK;
+}
+
+}  // extern "C"
