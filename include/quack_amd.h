@@ -207,7 +207,9 @@ int qk_rgw_self_energy_diag(int doSRG, double eta, int nOrb, int nC, int nO, int
                             const double *Om, const double *rho, double *EcGM, double *SigC, double *Z);
 
 /* GW/RGW_QP_graph.f90:1-97: Newton solve of the QP equation; Z is in/out as
- * in QuAcK (unchanged for an orbital whose solve hits maxIt = 64). */
+ * in QuAcK (unchanged for an orbital whose solve hits maxIt = 64, and for
+ * frozen orbitals).  Frozen entries of eGW, which QuAcK leaves unassigned,
+ * are set to eOld: the value they hold at QuAcK's call (GW/evRGW.f90:154). */
 int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO, int nV, int nR, int nS,
                     const double *eHF, const double *Om, const double *rho, const double *eGWlin, const double *eOld,
                     double *eGW, double *Z);

@@ -196,9 +196,11 @@ def qp_graph(eHF, Om, rho, eGWlin, eOld, Z, nO, eta, srg=False, nC=0, nR=0):
     """GW/RGW_QP_graph.f90:50-92: per-orbital Newton on w - eHF(p) - Sig(w) = 0.
     Start at eGWlin(p); Sig built from eOld; |f| > 1e-6 tested before each
     update; at nIt == 64 eGW(p) = eGWlin(p) and Z(p) keeps its input value,
-    otherwise Z(p) = df of the last step.  Returns eGW, Z, iterations."""
+    otherwise Z(p) = df of the last step.  Frozen orbitals (p < nC, p >= n-nR)
+    are never assigned there, so eGW keeps the caller's values: eOld at the
+    call in GW/evRGW.f90:154.  Returns eGW, Z, iterations."""
     n = len(eHF)
-    eGW = np.zeros(n)
+    eGW = np.array(eOld, dtype=np.float64)
     Z = np.array(Z, dtype=np.float64)
     its = np.zeros(n, dtype=np.int64)
     for p in range(nC, n - nR):
@@ -257,36 +259,40 @@ class DIIS:
         return e_inout, rcond
 
 
-def screening(e, eri, nO, TDA_W=False):
+def screening(e, eri, nO, TDA_W=False, nC=0, nR=0):
     n = len(e)
-    A = phRLR_A(e, eri, nO)
-    B = np.zeros_like(A) if TDA_W else phRLR_B(eri, n, nO)
+    A = phRLR_A(e, eri, nO, nC=nC, nR=nR)
+    B = np.zeros_like(A) if TDA_W else phRLR_B(eri, n, nO, nC=nC, nR=nR)
     return phRLR(TDA_W, A, B)
 
 
-def rg0w0(eri_mo, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False):
-    """GW/RG0W0.f90:88-233 (linDM, BSE, ACFDT and ppBSE branches off)."""
+def rg0w0(eri_mo, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False, nC=0, nR=0):
+    """GW/RG0W0.f90:88-233 (linDM, BSE, ACFDT and ppBSE branches off).
+    Frozen orbitals of eGW, undefined in QuAcK's G0W0 (eGW is freshly
+    allocated and GW/RGW_QP_graph.f90 does not assign them), are eHF here."""
     n = len(eHF)
-    EcRPA, Om, XpY, XmY = screening(eHF, eri_mo, nO, TDA_W)
-    rho = excitation_density(eri_mo, XpY, n, nO)
+    EcRPA, Om, XpY, XmY = screening(eHF, eri_mo, nO, TDA_W, nC, nR)
+    rho = excitation_density(eri_mo, XpY, n, nO, nC, nR)
     if doSRG:
-        SigC, Z, EcGM = srg_self_energy_diag(eHF, Om, rho, nO)
+        SigC, Z, EcGM = srg_self_energy_diag(eHF, Om, rho, nO, nC=nC, nR=nR)
     else:
-        SigC, Z, EcGM = self_energy_diag(eHF, Om, rho, nO, eta)
+        SigC, Z, EcGM = self_energy_diag(eHF, Om, rho, nO, eta, nC=nC, nR=nR)
     eGWlin = eHF + Z * SigC
     its = None
     if linearize:
         eGW = eGWlin.copy()
     else:
-        eGW, Z, its = qp_graph(eHF, Om, rho, eGWlin, eHF, Z, nO, eta, doSRG)
-    EcRPA2, _, _, _ = screening(eGW, eri_mo, nO, TDA_W)
+        eGW, Z, its = qp_graph(eHF, Om, rho, eGWlin, eHF, Z, nO, eta, doSRG, nC=nC, nR=nR)
+    EcRPA2, _, _, _ = screening(eGW, eri_mo, nO, TDA_W, nC, nR)
     return {"eGW": eGW, "SigC": SigC, "Z": Z, "Om": Om, "EcRPA": EcRPA2, "EcGM": EcGM,
             "eGWlin": eGWlin, "rho": rho, "XpY": XpY, "XmY": XmY, "EcRPA0": EcRPA, "qp_iters": its}
 
 
 def evrgw(eri_mo, eHF, nO, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, linearize=False, TDA_W=False,
-          doSRG=False):
-    """GW/evRGW.f90:104-200.  status 1 where QuAcK prints 'Convergence failed'."""
+          doSRG=False, nC=0, nR=0):
+    """GW/evRGW.f90:104-200.  status 1 where QuAcK prints 'Convergence failed'.
+    Frozen orbitals keep eHF throughout (eGW = eHF at :110, never assigned by
+    GW/RGW_QP_graph.f90, zero DIIS error there)."""
     n = len(eHF)
     eGW = eHF.copy()
     eOld = eGW.copy()
@@ -296,16 +302,16 @@ def evrgw(eri_mo, eHF, nO, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, lineari
     Conv = 1.0
     hist = []
     while Conv > thresh and nSCF <= maxSCF:
-        EcRPA, Om, XpY, XmY = screening(eGW, eri_mo, nO, TDA_W)
-        rho = excitation_density(eri_mo, XpY, n, nO)
+        EcRPA, Om, XpY, XmY = screening(eGW, eri_mo, nO, TDA_W, nC, nR)
+        rho = excitation_density(eri_mo, XpY, n, nO, nC, nR)
         if doSRG:
-            SigC, Z, EcGM = srg_self_energy_diag(eGW, Om, rho, nO)
+            SigC, Z, EcGM = srg_self_energy_diag(eGW, Om, rho, nO, nC=nC, nR=nR)
         else:
-            SigC, Z, EcGM = self_energy_diag(eGW, Om, rho, nO, eta)
+            SigC, Z, EcGM = self_energy_diag(eGW, Om, rho, nO, eta, nC=nC, nR=nR)
         if linearize:
             eGW = eHF + SigC
         else:
-            eGW, Z, _ = qp_graph(eHF, Om, rho, eOld, eOld, Z, nO, eta, doSRG)
+            eGW, Z, _ = qp_graph(eHF, Om, rho, eOld, eOld, Z, nO, eta, doSRG, nC=nC, nR=nR)
         Conv = np.max(np.abs(eGW - eOld))
         if max_diis > 1:
             eGW, _ = diis(eGW - eOld, eGW)
@@ -340,27 +346,28 @@ def phbse_static_kernel_B(eta, Om, rho, nO, lam=1.0, nC=0, nR=0):
 
 
 def rgw_phbse(eri, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, singlet=True, triplet=True,
-              exchange_kernel=False):
+              exchange_kernel=False, nC=0, nR=0):
     """GW/RGW_phBSE.f90:100-195 with the static kernel (no phBSE2, no dynamical
     correction): dRPA screening at eW, rho, KA/KB, then the singlet and
     triplet BSE matrices at eGW (phRLR_A/B with dRPA = false) plus the
     kernels, each solved by phRLR.  Returns EcBSE(2) (triplet x 3, halved with
     exchange_kernel) and the singlet and triplet excitation energies."""
     n = len(eW)
-    nS = ph_dims(n, nO)[4]
-    A = phRLR_A(eW, eri, nO, dRPA=True)
-    B = np.zeros((nS, nS)) if TDA_W else phRLR_B(eri, n, nO, dRPA=True)
+    nS = ph_dims(n, nO, nC, nR)[4]
+    A = phRLR_A(eW, eri, nO, dRPA=True, nC=nC, nR=nR)
+    B = np.zeros((nS, nS)) if TDA_W else phRLR_B(eri, n, nO, dRPA=True, nC=nC, nR=nR)
     _, OmRPA, XpY, _ = phRLR(TDA_W, A, B)
-    rho = excitation_density(eri, XpY, n, nO)
-    KA = phbse_static_kernel_A(eta, OmRPA, rho, nO)
-    KB = np.zeros((nS, nS)) if TDA else phbse_static_kernel_B(eta, OmRPA, rho, nO)
+    rho = excitation_density(eri, XpY, n, nO, nC, nR)
+    KA = phbse_static_kernel_A(eta, OmRPA, rho, nO, nC=nC, nR=nR)
+    KB = np.zeros((nS, nS)) if TDA else phbse_static_kernel_B(eta, OmRPA, rho, nO, nC=nC, nR=nR)
     EcBSE = np.zeros(2)
     Om = {1: np.zeros(nS), 2: np.zeros(nS)}
     for ispin, on in ((1, singlet), (2, triplet)):
         if not on:
             continue
-        A = phRLR_A(eGW, eri, nO, dRPA=False, ispin=ispin) + KA
-        B = np.zeros((nS, nS)) if TDA else phRLR_B(eri, n, nO, dRPA=False, ispin=ispin) + KB
+        A = phRLR_A(eGW, eri, nO, dRPA=False, ispin=ispin, nC=nC, nR=nR) + KA
+        B = (np.zeros((nS, nS)) if TDA
+             else phRLR_B(eri, n, nO, dRPA=False, ispin=ispin, nC=nC, nR=nR) + KB)
         EcBSE[ispin - 1], Om[ispin], _, _ = phRLR(TDA, A, B)
     EcBSE[1] *= 3.0
     if exchange_kernel:

@@ -123,9 +123,9 @@ def qp_graph(eHF, Om, rho, eGWlin, eOld, Z, nO, eta=0.0, srg=False, nC=0, nR=0):
     return {"eGW": eGW, "Z": Zio}
 
 
-def rg0w0(eri_mo, eHF, nO, eta=0.0, linearize=False, tda_w=False, srg=False, ENuc=0.0, ERHF=0.0):
+def rg0w0(eri_mo, eHF, nO, eta=0.0, linearize=False, tda_w=False, srg=False, ENuc=0.0, ERHF=0.0, nC=0, nR=0):
     nOrb = len(eHF)
-    nC, nO, nV, nR, nS = _dims(nOrb, nO)
+    nC, nO, nV, nR, nS = _dims(nOrb, nO, nC, nR)
     eGW, Sig, Z = np.zeros(nOrb), np.zeros(nOrb), np.zeros(nOrb)
     Om, Ec, Egm = np.zeros(nS), np.zeros(1), np.zeros(1)
     load().qkref_rg0w0(int(tda_w), int(linearize), int(srg), eta, nOrb, nC, nO, nV, nR, nS, ENuc, ERHF,
@@ -135,9 +135,9 @@ def rg0w0(eri_mo, eHF, nO, eta=0.0, linearize=False, tda_w=False, srg=False, ENu
 
 
 def evrgw(eri_mo, eHF, nO, maxSCF=256, thresh=1e-5, max_diis=5, eta=0.0, linearize=False, tda_w=False,
-          srg=False, ENuc=0.0, ERHF=0.0):
+          srg=False, ENuc=0.0, ERHF=0.0, nC=0, nR=0):
     nOrb = len(eHF)
-    nC, nO, nV, nR, nS = _dims(nOrb, nO)
+    nC, nO, nV, nR, nS = _dims(nOrb, nO, nC, nR)
     eGW, Sig, Z = np.zeros(nOrb), np.zeros(nOrb), np.zeros(nOrb)
     Ec, Egm = np.zeros(1), np.zeros(1)
     nscf = ctypes.c_int(0)

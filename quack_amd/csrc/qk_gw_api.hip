@@ -711,7 +711,7 @@ void qp_graph(Ctx* c, const double* e0, const double* eG, const double* start, d
         QK_HIP(hipMemcpyAsync(&h_active, c->active, sizeof(int), hipMemcpyDeviceToHost, c->s));
         QK_HIP(hipStreamSynchronize(c->s));
     }
-    qp_finish(c->s, c->w, c->df, c->nit, start, c->eGW, c->Z, n, c->nC, c->nR, QP_MAXIT);
+    qp_finish(c->s, c->w, c->df, c->nit, start, eG, c->eGW, c->Z, n, c->nC, c->nR, QP_MAXIT);
 }
 
 

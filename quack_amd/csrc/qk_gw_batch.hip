@@ -200,8 +200,8 @@ void qp_graph_batch(BCtx* b, double eta, int srg, const double* eG, const double
         QK_HIP(hipStreamSynchronize(b->s));
         if (std::all_of(act.begin(), act.end(), [](int a) { return a == 0; })) break;
     }
-    qp_finish(b->s, b->vec[V_W], b->vec[V_DF], b->nit, start, b->vec[V_EGW], b->vec[V_Z], n, c0->nC, c0->nR, QP_MAXIT,
-              b->zs, nm);
+    qp_finish(b->s, b->vec[V_W], b->vec[V_DF], b->nit, start, eG, b->vec[V_EGW], b->vec[V_Z], n, c0->nC, c0->nR,
+              QP_MAXIT, b->zs, nm);
 }
 
 // Capture what body enqueues on the batch stream into an executable graph

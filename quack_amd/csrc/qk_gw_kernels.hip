@@ -407,21 +407,25 @@ __global__ void k_qp_update(const double* __restrict__ part, int nchunk, const d
 
 // eGW(p) = w, Z(p) = df unless nIt == maxIt (then eGW = eGWlin, Z kept)
 __global__ void k_qp_finish(const double* __restrict__ w, const double* __restrict__ df, const int* __restrict__ nit,
-                            const double* __restrict__ eGWlin, double* __restrict__ eGW, double* __restrict__ Z, int n,
-                            int nC, int nR, int maxit, Zs zs) {
+                            const double* __restrict__ eGWlin, const double* __restrict__ eG,
+                            double* __restrict__ eGW, double* __restrict__ Z, int n, int nC, int nR, int maxit, Zs zs) {
     {
         const long long z = blockIdx.z;
         w += z * zs.n;
         df += z * zs.n;
         nit += z * zs.n;
         eGWlin += z * zs.n;
+        eG += z * zs.n;
         eGW += z * zs.n;
         Z += z * zs.n;
     }
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     if (p < nC || p >= n - nR) {
-        eGW[p] = 0.0;  // QuAcK leaves frozen orbitals of eGW unset; they are never read
+        // GW/RGW_QP_graph.f90:50 never assigns frozen entries of eGW, so they
+        // keep what the caller's array held: in GW/evRGW.f90:154 that is
+        // eOld, the energies in G.  Z keeps its input value there.
+        eGW[p] = eG[p];
         return;
     }
     if (nit[p] == maxit) {
@@ -1126,10 +1130,10 @@ void qp_update(hipStream_t s, const double* part, int nchunk, const double* eHF,
                        maxit, active, zs);
     QK_HIP(hipGetLastError());
 }
-void qp_finish(hipStream_t s, const double* w, const double* df, const int* nit, const double* eGWlin, double* eGW,
-               double* Z, int n, int nC, int nR, int maxit, const Zs& zs, int nz) {
-    hipLaunchKernelGGL(k_qp_finish, dim3((n + 255) / 256, 1, nz), dim3(256), 0, s, w, df, nit, eGWlin, eGW, Z, n, nC,
-                       nR, maxit, zs);
+void qp_finish(hipStream_t s, const double* w, const double* df, const int* nit, const double* eGWlin,
+               const double* eG, double* eGW, double* Z, int n, int nC, int nR, int maxit, const Zs& zs, int nz) {
+    hipLaunchKernelGGL(k_qp_finish, dim3((n + 255) / 256, 1, nz), dim3(256), 0, s, w, df, nit, eGWlin, eG, eGW, Z, n,
+                       nC, nR, maxit, zs);
     QK_HIP(hipGetLastError());
 }
 void evgw_update(hipStream_t s, const double* eNew, double* e, const double* Sig, const double* Z, const double* Om,

@@ -77,7 +77,7 @@ void qp_dev(hipStream_t s, QpWork& q, const double* rT, const double* dO, int do
         QK_HIP(hipMemcpyAsync(&h_active, q.act, sizeof(int), hipMemcpyDeviceToHost, s));
         QK_HIP(hipStreamSynchronize(s));
     }
-    qp_finish(s, q.w, q.df, q.nit, q.eL, q.eo, q.z, n, nC, nR, QP_MAXIT);
+    qp_finish(s, q.w, q.df, q.nit, q.eL, q.eG, q.eo, q.z, n, nC, nR, QP_MAXIT);
     QK_HIP(hipMemcpyAsync(eGW, q.eo, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     QK_HIP(hipMemcpyAsync(Z, q.z, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     QK_HIP(hipStreamSynchronize(s));

@@ -53,8 +53,10 @@ void qp_init(hipStream_t s, const double* eGWlin, double* w, double* f, int* nit
              const Zs& zs = Zs(), int nz = 1);
 void qp_update(hipStream_t s, const double* part, int nchunk, const double* eHF, double* w, double* f, double* df,
                int* nit, int* done, int n, double thresh, int maxit, int* active, const Zs& zs = Zs(), int nz = 1);
-void qp_finish(hipStream_t s, const double* w, const double* df, const int* nit, const double* eGWlin, double* eGW,
-               double* Z, int n, int nC, int nR, int maxit, const Zs& zs = Zs(), int nz = 1);
+// eG: energies in G; frozen orbitals of eGW are set to them (eOld in evGW).
+void qp_finish(hipStream_t s, const double* w, const double* df, const int* nit, const double* eGWlin,
+               const double* eG, double* eGW, double* Z, int n, int nC, int nR, int maxit, const Zs& zs = Zs(),
+               int nz = 1);
 // One evGW cycle's bookkeeping for every molecule of a batch (GW/evRGW.f90:156-183,
 // DIIS of utils/DIIS_extrapolation.f90), on the device: state[4 m + {0, 1, 2}] =
 // live flag, cycles run, DIIS history length; hist: 2 n maxd doubles per

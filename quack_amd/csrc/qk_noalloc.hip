@@ -87,7 +87,7 @@ int qk_rgw_qp_graph_noalloc(int doSRG, double eta, double flow, int nOrb, int nC
             sigma_partial(s, rhoT, false, w, eOld, Om, done, eta, doSRG, flow, nOrb, nC, nO, nR, nS, nchunk, part);
             qp_update(s, part, nchunk, eHF, w, f, df, nit, done, nOrb, QP_THRESH, QP_MAXIT, active);
         }
-        qp_finish(s, w, df, nit, eGWlin, eGW, Z, nOrb, nC, nR, QP_MAXIT);
+        qp_finish(s, w, df, nit, eGWlin, eOld, eGW, Z, nOrb, nC, nR, QP_MAXIT);
         return QK_OK;
     } catch (const Error& e) {
         g_err = e.msg;

@@ -120,6 +120,75 @@ def test_oracle_vs_compiled_reference(systems, name):
         assert a["nSCF"] == b["nSCF"] and np.abs(a["eGW"] - b["eGW"]).max() < 1e-10
 
 
+def _active_space(eri_mo, eHF, nC, nR):
+    """The same system on the active orbitals [nC, n - nR) alone."""
+    n = len(eHF)
+    act = slice(nC, n - nR)
+    G = go.G4(eri_mo, n)[act, act, act, act]
+    return np.asfortranarray(G).reshape(-1, order="F"), eHF[act].copy()
+
+
+@pytest.mark.parametrize("nC,nR", [(1, 0), (1, 2), (0, 3)])
+def test_frozen_orbitals_equal_active_space(mo_systems, nC, nR):
+    """Frozen core (nC) and frozen virtuals (nR): every sum of QuAcK's dRPA,
+    excitation densities, self-energy and QP equation runs over active
+    orbitals only, so the answer equals the same calculation on the active
+    orbitals alone with nO - nC occupied; the frozen entries of evGW's eGW
+    stay at eHF (eGW = eHF at GW/evRGW.f90:110, never assigned by
+    GW/RGW_QP_graph.f90), up to DIIS rounding."""
+    s = mo_systems["H2O"]
+    nO, e = s["nO"], s["eHF"]
+    n = len(e)
+    act = slice(nC, n - nR)
+    mo_a, e_a = _active_space(s["eri_mo"], e, nC, nR)
+    for kw in ({"eta": 0.0}, {"eta": 0.1}, {"doSRG": True}):
+        a = go.rg0w0(s["eri_mo"], e, nO, nC=nC, nR=nR, **kw)
+        b = go.rg0w0(mo_a, e_a, nO - nC, **kw)
+        assert np.abs(a["eGW"][act] - b["eGW"]).max() < 1e-12
+        assert np.abs(a["Om"] - b["Om"]).max() < 1e-12 and abs(a["EcRPA"] - b["EcRPA"]) < 1e-12
+        assert np.abs(a["eGW"][:nC] - e[:nC]).max(initial=0) == 0.0
+    a = go.evrgw(s["eri_mo"], e, nO, thresh=1e-6, max_diis=5, eta=0.1, nC=nC, nR=nR)
+    b = go.evrgw(mo_a, e_a, nO - nC, thresh=1e-6, max_diis=5, eta=0.1)
+    assert a["nSCF"] == b["nSCF"] and np.abs(a["eGW"][act] - b["eGW"]).max() < 1e-11
+    frozen = np.r_[np.arange(nC), np.arange(n - nR, n)]
+    assert np.abs(a["eGW"][frozen] - e[frozen]).max(initial=0) < 1e-13
+    Ec, OmS, OmT = go.rgw_phbse(s["eri_mo"], e, a["eGW"], nO, eta=0.1, nC=nC, nR=nR)
+    Ec_a, OmS_a, OmT_a = go.rgw_phbse(mo_a, e_a, b["eGW"], nO - nC, eta=0.1)
+    assert np.abs(OmS - OmS_a).max() < 1e-10 and np.abs(OmT - OmT_a).max() < 1e-10
+    assert np.abs(Ec - Ec_a).max() < 1e-10
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+@pytest.mark.parametrize("nC,nR", [(1, 0), (1, 2), (0, 3)])
+def test_frozen_orbitals_vs_compiled_reference(systems, nC, nR):
+    """G0W0 and evGW (DIIS 5, whose error vector covers the frozen entries)
+    with frozen core / virtuals against QuAcK's own routines.  QuAcK's G0W0
+    leaves the frozen entries of eGW undefined, so only active ones are
+    compared there."""
+    s = systems["H2O"]
+    nO, e = s["nO"], s["eHF"]
+    n = len(e)
+    act = slice(nC, n - nR)
+    mo = qkref.aotomo(s["c"], s["eri"])
+    for kw in ({"eta": 0.0}, {"eta": 0.1}, {"srg": True}):
+        a = qkref.rg0w0(mo, e, nO, nC=nC, nR=nR, **kw)
+        b = go.rg0w0(mo, e, nO, nC=nC, nR=nR, eta=kw.get("eta", 0.0), doSRG=kw.get("srg", False))
+        assert np.abs(a["eGW"][act] - b["eGW"][act]).max() < 1e-12
+        assert np.abs(a["Z"][act] - b["Z"][act]).max() < 1e-10 and abs(a["EcRPA"] - b["EcRPA"]) < 1e-12
+    for kw in ({"max_diis": 5, "thresh": 1e-6, "eta": 0.1}, {"max_diis": 5, "thresh": 1e-6},
+               {"max_diis": 1, "thresh": 1e-6, "tda_w": True}):
+        a = qkref.evrgw(mo, e, nO, nC=nC, nR=nR, **kw)
+        b = go.evrgw(mo, e, nO, max_diis=kw["max_diis"], thresh=kw["thresh"], eta=kw.get("eta", 0.0),
+                     TDA_W=kw.get("tda_w", False), nC=nC, nR=nR)
+        assert a["nSCF"] == b["nSCF"] and np.abs(a["eGW"] - b["eGW"]).max() < 1e-12
+    eGW = go.rg0w0(mo, e, nO, nC=nC, nR=nR)["eGW"]
+    for tda_w, tda in ((False, False), (True, True)):
+        r = qkref.phbse(mo, e, eGW, nO, tda_w=tda_w, tda=tda, nC=nC, nR=nR)
+        Ec, OmS, OmT = go.rgw_phbse(mo, e, eGW, nO, TDA_W=tda_w, TDA=tda, nC=nC, nR=nR)
+        assert np.abs(OmS - r["OmS"]).max() < 1e-10 and np.abs(OmT - r["OmT"]).max() < 1e-10
+        assert np.abs(Ec - r["EcBSE"]).max() < 1e-10
+
+
 def test_he_closed_form(mo_systems):
     """He/6-31G has nS = 1: Om = sqrt(D (D + 4K)), XpY = sqrt(D/Om),
     rho_pq = (pq|12) XpY and the QP equation is a scalar rational equation."""
