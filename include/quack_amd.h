@@ -232,7 +232,7 @@ int qk_rgw_qp_graph_noalloc(int doSRG, double eta, double flow, int nOrb, int nC
                             double *Z_device, void *stream);
 const char *qk_noalloc_last_error(void);
 
-/* Unrestricted G0W0 (GW/UG0W0.f90:92-150) with nC = nR = 0: spin-conserving
+/* Unrestricted G0W0 (GW/UG0W0.f90:92-150): spin-conserving
  * dRPA screening at eHF over both spin blocks (LR/phULR_A.f90, LR/phULR_B.f90,
  * LR/phULR.f90; nSa + nSb modes), excitation densities
  * (GW/UGW_excitation_density.f90) as FP64 matrix-core products, diagonal
@@ -240,13 +240,18 @@ const char *qk_noalloc_last_error(void);
  * graphical (GW/UGW_QP_graph.f90) quasiparticle energies per spin, and the RPA
  * again at eGW for EcRPA.  ERI_aaaa, ERI_aabb, ERI_bbbb: nBas^4 MO integrals
  * (physicist, column-major; in ERI_aabb the first and third indices are
- * alpha).  eHF, eGW, SigC, Z: nBas x 2 (alpha column, then beta).  Om: nSa +
+ * alpha).  nC[2], nO[2], nR[2]: frozen core, occupied and frozen virtual
+ * counts per spin (alpha, beta), as QuAcK's nC(nspin), nO(nspin), nR(nspin);
+ * per spin 0 <= nC < nO < nBas - nR, nS_s = (nO - nC)(nBas - nR - nO).
+ * eHF, eGW, SigC, Z: nBas x 2 (alpha column, then beta); frozen entries of
+ * eGW are eHF (QuAcK leaves them undefined), SigC 0 and Z 1 there.  Om: nSa +
  * nSb.  EcGM[2]: per spin (QuAcK's SRG routine returns their sum). */
-int qk_ug0w0(int nBas, int nOa, int nOb, const double *ERI_aaaa, const double *ERI_aabb, const double *ERI_bbbb,
-             const double *eHF, double eta, int linearize, int TDA_W, int doSRG, double *eGW, double *SigC, double *Z,
-             double *Om, double *EcRPA, double *EcGM);
+int qk_ug0w0(int nBas, const int *nC, const int *nO, const int *nR, const double *ERI_aaaa, const double *ERI_aabb,
+             const double *ERI_bbbb, const double *eHF, double eta, int linearize, int TDA_W, int doSRG, double *eGW,
+             double *SigC, double *Z, double *Om, double *EcRPA, double *EcGM);
 
-/* Unrestricted evGW (GW/evUGW.f90:103-215) with nC = nR = 0: per cycle the
+/* Unrestricted evGW (GW/evUGW.f90:103-215), orbital counts as qk_ug0w0
+ * (frozen entries of eGW stay at eHF, as in QuAcK): per cycle the
  * screening of qk_ug0w0 at the current energies (both spins), the diagonal
  * self-energies at those energies, then eHF + SigC (linearize) or the graphical
  * solve started from the previous energies; Conv = max |eGW - eOld| over both
@@ -255,9 +260,10 @@ int qk_ug0w0(int nBas, int nOa, int nOb, const double *ERI_aaaa, const double *E
  * set).  Outputs as qk_ug0w0 from the last cycle (EcRPA is the last cycle's
  * screening at its input energies); nSCF_out = cycles run.  Returns
  * QK_E_NOCONV, outputs filled, where QuAcK stops after maxSCF + 1 cycles. */
-int qk_evugw(int nBas, int nOa, int nOb, const double *ERI_aaaa, const double *ERI_aabb, const double *ERI_bbbb,
-             const double *eHF, int maxSCF, double thresh, int max_diis, double eta, int linearize, int TDA_W,
-             int doSRG, double *eGW, double *SigC, double *Z, double *Om, double *EcRPA, double *EcGM, int *nSCF_out);
+int qk_evugw(int nBas, const int *nC, const int *nO, const int *nR, const double *ERI_aaaa, const double *ERI_aabb,
+             const double *ERI_bbbb, const double *eHF, int maxSCF, double thresh, int max_diis, double eta,
+             int linearize, int TDA_W, int doSRG, double *eGW, double *SigC, double *Z, double *Om, double *EcRPA,
+             double *EcGM, int *nSCF_out);
 
 /* Eigensolver of the unrestricted screening of qk_ug0w0 and qk_evugw,
  * process-wide: mode and de_max as qk_gw_set_eigensolver (0: rocSOLVER dsyevd

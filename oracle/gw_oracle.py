@@ -479,17 +479,20 @@ def qsrgw(S, T, V, eri_ao, X, cHF, PHF, eHF, nO, maxSCF=64, thresh=1e-5, max_dii
 # and fourth beta); e[:, 0] alpha, e[:, 1] beta; nO = (nO_alpha, nO_beta).
 
 
-def _ublocks(n, nO):
-    oa, ob = slice(0, nO[0]), slice(0, nO[1])
-    va, vb = slice(nO[0], n), slice(nO[1], n)
-    return oa, va, ob, vb, nO[0] * (n - nO[0]), nO[1] * (n - nO[1])
+def _ublocks(n, nO, nC=(0, 0), nR=(0, 0)):
+    """Active occupied and virtual slices of each spin (nC(s)+1..nO(s) and
+    nO(s)+1..nBas-nR(s) of LR/phULR_A.f90:55-118) and the pair counts."""
+    oa, ob = slice(nC[0], nO[0]), slice(nC[1], nO[1])
+    va, vb = slice(nO[0], n - nR[0]), slice(nO[1], n - nR[1])
+    return (oa, va, ob, vb, (nO[0] - nC[0]) * (n - nR[0] - nO[0]), (nO[1] - nC[1]) * (n - nR[1] - nO[1]))
 
 
-def phULR_A(e, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True):
-    """LR/phULR_A.f90:26-80 (spin-conserving, ispin 1), nC = nR = 0."""
+def phULR_A(e, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True, nC=(0, 0), nR=(0, 0)):
+    """LR/phULR_A.f90:26-80 (spin-conserving, ispin 1); pairs ia with i, a
+    active in their spin, a fastest."""
     n = e.shape[0]
     Ga, Gm, Gb = G4(eri_aa, n), G4(eri_ab, n), G4(eri_bb, n)
-    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO)
+    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO, nC, nR)
     dr = 1.0 if dRPA else 0.0
     A = np.zeros((nSa + nSb, nSa + nSb))
     A[:nSa, :nSa] = lam * Ga[oa, va, va, oa].transpose(0, 2, 3, 1).reshape(nSa, nSa) \
@@ -504,10 +507,10 @@ def phULR_A(e, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True):
     return A
 
 
-def phULR_B(n, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True):
-    """LR/phULR_B.f90:26-80 (spin-conserving, ispin 1), nC = nR = 0."""
+def phULR_B(n, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True, nC=(0, 0), nR=(0, 0)):
+    """LR/phULR_B.f90:26-80 (spin-conserving, ispin 1)."""
     Ga, Gm, Gb = G4(eri_aa, n), G4(eri_ab, n), G4(eri_bb, n)
-    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO)
+    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO, nC, nR)
     dr = 1.0 if dRPA else 0.0
     B = np.zeros((nSa + nSb, nSa + nSb))
     B[:nSa, :nSa] = lam * Ga[oa, oa, va, va].transpose(0, 2, 1, 3).reshape(nSa, nSa) \
@@ -519,57 +522,64 @@ def phULR_B(n, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True):
     return B
 
 
-def ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO):
-    """GW/UGW_excitation_density.f90: rho[p, q, m, spin]."""
+def ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO, nC=(0, 0), nR=(0, 0)):
+    """GW/UGW_excitation_density.f90: rho[p, q, m, spin], summed over the
+    active pairs jb; QuAcK fills only active p, q of each spin (:43-44,
+    :86-87), the others are never read."""
     Ga, Gm, Gb = G4(eri_aa, n), G4(eri_ab, n), G4(eri_bb, n)
-    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO)
-    Xa, Xb = XpY[:, :nSa], XpY[:, nSa:]
+    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO, nC, nR)
+    Xa = XpY[:, :nSa].reshape(-1, oa.stop - oa.start, va.stop - va.start)
+    Xb = XpY[:, nSa:].reshape(-1, ob.stop - ob.start, vb.stop - vb.start)
     rho = np.zeros((n, n, nSa + nSb, 2))
     # alpha: sum_jb(a) <pj|qb>_aaaa X(m, jb) + sum_jb(b) <pj|qb>_aabb X(m, nSa + jb)
-    rho[:, :, :, 0] = np.einsum("pjqb,mjb->pqm", Ga[:, oa, :, va], Xa.reshape(-1, nO[0], n - nO[0]), optimize=True) \
-        + np.einsum("pjqb,mjb->pqm", Gm[:, ob, :, vb], Xb.reshape(-1, nO[1], n - nO[1]), optimize=True)
+    rho[:, :, :, 0] = np.einsum("pjqb,mjb->pqm", Ga[:, oa, :, va], Xa, optimize=True) \
+        + np.einsum("pjqb,mjb->pqm", Gm[:, ob, :, vb], Xb, optimize=True)
     # beta: sum_jb(a) <jp|bq>_aabb X(m, jb) + sum_jb(b) <pj|qb>_bbbb X(m, nSa + jb)
-    rho[:, :, :, 1] = np.einsum("jpbq,mjb->pqm", Gm[oa, :, va, :], Xa.reshape(-1, nO[0], n - nO[0]), optimize=True) \
-        + np.einsum("pjqb,mjb->pqm", Gb[:, ob, :, vb], Xb.reshape(-1, nO[1], n - nO[1]), optimize=True)
+    rho[:, :, :, 1] = np.einsum("jpbq,mjb->pqm", Gm[oa, :, va, :], Xa, optimize=True) \
+        + np.einsum("pjqb,mjb->pqm", Gb[:, ob, :, vb], Xb, optimize=True)
     return rho
 
 
-def ug0w0(eri_aa, eri_ab, eri_bb, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False):
-    """GW/UG0W0.f90:92-150 (nC = nR = 0): spin-conserving dRPA screening at
+def ug0w0(eri_aa, eri_ab, eri_bb, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False, nC=(0, 0),
+          nR=(0, 0)):
+    """GW/UG0W0.f90:92-150 (nC, nR per spin): spin-conserving dRPA screening at
     eHF, excitation densities, diagonal self-energies (GW/UGW_self_energy_diag.f90
     or the SRG form), linearised energies and the graphical QP solve of each
     spin (GW/UGW_QP_graph.f90), then the RPA again at eGW for EcRPA.  The
     unrestricted sums carry rho^2 where the restricted ones carry 2 rho^2, so
     the restricted restatements are used with rho / sqrt(2) (and EcGM, 4 rho^2
-    there, halved).  Returns dict(eGW, SigC, Z (n x 2), Om, EcRPA, EcGM (2))."""
+    there, halved).  Frozen entries of eGW (undefined in QuAcK's G0W0) are
+    eHF.  Returns dict(eGW, SigC, Z (n x 2), Om, EcRPA, EcGM (2))."""
     n = eHF.shape[0]
-    A = phULR_A(eHF, eri_aa, eri_ab, eri_bb, nO)
-    B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO)
+    A = phULR_A(eHF, eri_aa, eri_ab, eri_bb, nO, nC=nC, nR=nR)
+    B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO, nC=nC, nR=nR)
     _, Om, XpY, _ = phRLR(TDA_W, A, B)
-    rho = ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO) / np.sqrt(2.0)
+    rho = ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO, nC, nR) / np.sqrt(2.0)
     Sig, Z, EcGM = np.zeros((n, 2)), np.zeros((n, 2)), np.zeros(2)
     eGW = np.zeros((n, 2))
     for s in range(2):
         if doSRG:
-            Sig[:, s], Z[:, s], ec = srg_self_energy_diag(eHF[:, s], Om, rho[:, :, :, s], nO[s])
+            Sig[:, s], Z[:, s], ec = srg_self_energy_diag(eHF[:, s], Om, rho[:, :, :, s], nO[s], nC=nC[s],
+                                                          nR=nR[s])
         else:
-            Sig[:, s], Z[:, s], ec = self_energy_diag(eHF[:, s], Om, rho[:, :, :, s], nO[s], eta)
+            Sig[:, s], Z[:, s], ec = self_energy_diag(eHF[:, s], Om, rho[:, :, :, s], nO[s], eta, nC=nC[s],
+                                                      nR=nR[s])
         EcGM[s] = 0.5 * ec
         lin = eHF[:, s] + Z[:, s] * Sig[:, s]
         if linearize:
             eGW[:, s] = lin
         else:
             eGW[:, s], Z[:, s], _ = qp_graph(eHF[:, s], Om, rho[:, :, :, s], lin, eHF[:, s], Z[:, s], nO[s], eta,
-                                             srg=doSRG)
-    A = phULR_A(eGW, eri_aa, eri_ab, eri_bb, nO)
-    B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO)
+                                             srg=doSRG, nC=nC[s], nR=nR[s])
+    A = phULR_A(eGW, eri_aa, eri_ab, eri_bb, nO, nC=nC, nR=nR)
+    B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO, nC=nC, nR=nR)
     EcRPA, _, _, _ = phRLR(TDA_W, A, B)
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": EcRPA, "EcGM": EcGM}
 
 
 def evugw(eri_aa, eri_ab, eri_bb, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, eta=0.0, linearize=False,
-          TDA_W=False, doSRG=False):
-    """GW/evUGW.f90:103-215 (nC = nR = 0): eGW = eOld = eHF; per cycle the
+          TDA_W=False, doSRG=False, nC=(0, 0), nR=(0, 0)):
+    """GW/evUGW.f90:103-215 (nC, nR per spin): eGW = eOld = eHF; per cycle the
     screening at eGW, excitation densities, self-energies at eGW, then eHF + SigC
     (linearize) or the graphical solve with guess and G from eOld; Conv =
     max |eGW - eOld| over both spins; DIIS per spin with one shared history
@@ -583,22 +593,24 @@ def evugw(eri_aa, eri_ab, eri_bb, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, e
     diis = [DIIS(n, max(max_diis, 1)) for _ in range(2)]
     nSCF, Conv, EcRPA, Om = 0, 1.0, 0.0, None
     while Conv > thresh and nSCF <= maxSCF:
-        A = phULR_A(eGW, eri_aa, eri_ab, eri_bb, nO)
-        B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO)
+        A = phULR_A(eGW, eri_aa, eri_ab, eri_bb, nO, nC=nC, nR=nR)
+        B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO, nC=nC, nR=nR)
         EcRPA, Om, XpY, _ = phRLR(TDA_W, A, B)
-        rho = ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO) / np.sqrt(2.0)
+        rho = ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO, nC, nR) / np.sqrt(2.0)
         new = np.zeros((n, 2))
         for s in range(2):
             if doSRG:
-                Sig[:, s], Z[:, s], ec = srg_self_energy_diag(eGW[:, s], Om, rho[:, :, :, s], nO[s])
+                Sig[:, s], Z[:, s], ec = srg_self_energy_diag(eGW[:, s], Om, rho[:, :, :, s], nO[s], nC=nC[s],
+                                                              nR=nR[s])
             else:
-                Sig[:, s], Z[:, s], ec = self_energy_diag(eGW[:, s], Om, rho[:, :, :, s], nO[s], eta)
+                Sig[:, s], Z[:, s], ec = self_energy_diag(eGW[:, s], Om, rho[:, :, :, s], nO[s], eta, nC=nC[s],
+                                                          nR=nR[s])
             EcGM[s] = 0.5 * ec
             if linearize:
                 new[:, s] = eHF[:, s] + Sig[:, s]
             else:
                 new[:, s], Z[:, s], _ = qp_graph(eHF[:, s], Om, rho[:, :, :, s], eOld[:, s], eOld[:, s], Z[:, s],
-                                                 nO[s], eta, srg=doSRG)
+                                                 nO[s], eta, srg=doSRG, nC=nC[s], nR=nR[s])
         eGW = new
         Conv = np.abs(eGW - eOld).max()
         if max_diis > 1:

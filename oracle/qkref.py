@@ -45,8 +45,9 @@ def load():
         lib.qkref_phbse.argtypes = [_I] * 4 + [_D] + [_I] * 6 + [_P] * 6
         lib.qkref_self_energy_full.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
         lib.qkref_qsgw_cycle.argtypes = [_I, _I, _D] + [_I] * 6 + [_P] * 10
-        lib.qkref_ug0w0.argtypes = [_I, _I, _I, _D, _I, _I, _I] + [_P] * 9
-        lib.qkref_evugw.argtypes = [_I, _D, _I, _I, _I, _I, _D, _I, _I, _I] + [_P] * 9 + [ctypes.POINTER(_I)]
+        lib.qkref_ug0w0.argtypes = [_I, _I, _I, _D, _I, _I, _I] + [_I] * 4 + [_P] * 9
+        lib.qkref_evugw.argtypes = [_I, _D, _I, _I, _I, _I, _D, _I, _I, _I] + [_I] * 4 + [_P] * 9 + \
+            [ctypes.POINTER(_I)]
         _lib = lib
     return _lib
 
@@ -184,29 +185,32 @@ def qsgw_cycle(S, T, V, X, c, P, eGW, eri_ao, nO, eta=0.0, srg=False, tda_w=Fals
     return {"F": F, "eGW": e}
 
 
-def ug0w0(eri_aa, eri_ab, eri_bb, eHF, nO, eta=0.0, linearize=False, tda_w=False, srg=False):
+def ug0w0(eri_aa, eri_ab, eri_bb, eHF, nO, eta=0.0, linearize=False, tda_w=False, srg=False, nC=(0, 0),
+          nR=(0, 0)):
     """GW/UG0W0.f90 through QuAcK's own unrestricted routines
-    (ref_driver.f90 qkref_ug0w0); eHF n x 2, nO = (alpha, beta)."""
+    (ref_driver.f90 qkref_ug0w0); eHF n x 2, nO, nC, nR = (alpha, beta).
+    Frozen entries of eGW are left as QuAcK leaves them (zero here)."""
     n = eHF.shape[0]
     eGW, Sig, Z = (np.zeros((n, 2), order="F") for _ in range(3))
     Ec = np.zeros(1)
     Egm = np.zeros(2)
-    load().qkref_ug0w0(int(linearize), int(tda_w), int(srg), eta, n, nO[0], nO[1],
+    load().qkref_ug0w0(int(linearize), int(tda_w), int(srg), eta, n, nO[0], nO[1], nC[0], nC[1], nR[0], nR[1],
                        _p(_eri(eri_aa)), _p(_eri(eri_ab)),
                        _p(_eri(eri_bb)), _p(_f(eHF)), _p(eGW), _p(Sig), _p(Z), _p(Ec), _p(Egm))
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "EcRPA": Ec[0], "EcGM": Egm}
 
 
 def evugw(eri_aa, eri_ab, eri_bb, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, eta=0.0, linearize=False,
-          tda_w=False, srg=False):
+          tda_w=False, srg=False, nC=(0, 0), nR=(0, 0)):
     """GW/evUGW.f90's cycle through QuAcK's own routines (ref_driver.f90
-    qkref_evugw); eHF n x 2, nO = (alpha, beta)."""
+    qkref_evugw); eHF n x 2, nO, nC, nR = (alpha, beta)."""
     n = eHF.shape[0]
     eGW, Sig, Z = (np.zeros((n, 2), order="F") for _ in range(3))
     Ec = np.zeros(1)
     Egm = np.zeros(2)
     nscf = ctypes.c_int(0)
-    load().qkref_evugw(int(maxSCF), thresh, int(max_diis), int(linearize), int(tda_w), int(srg), eta, n, nO[0], nO[1], _p(_eri(eri_aa)),
+    load().qkref_evugw(int(maxSCF), thresh, int(max_diis), int(linearize), int(tda_w), int(srg), eta, n, nO[0], nO[1],
+                       nC[0], nC[1], nR[0], nR[1], _p(_eri(eri_aa)),
                        _p(_eri(eri_ab)), _p(_eri(eri_bb)), _p(_f(eHF)), _p(eGW),
                        _p(Sig), _p(Z), _p(Ec), _p(Egm), ctypes.byref(nscf))
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "EcRPA": Ec[0], "EcGM": Egm, "nSCF": nscf.value}

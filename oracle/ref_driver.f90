@@ -296,11 +296,11 @@ end subroutine
 ! GW/UG0W0.f90:92-150 (nC = nR = 0, no printing), composed of QuAcK's own
 ! unrestricted routines in the same order.  EcGM: the non-SRG routine returns
 ! one value per spin; the SRG routine one scalar (their sum), put in EcGM(1).
-subroutine qkref_ug0w0(lin, tda_w, srg, eta, nBas, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, eGW, SigC, Z, &
-                       EcRPA, EcGM) bind(C, name='qkref_ug0w0')
+subroutine qkref_ug0w0(lin, tda_w, srg, eta, nBas, nOa, nOb, nCa, nCb, nRa, nRb, ERI_aaaa, ERI_aabb, ERI_bbbb, &
+                       eHF, eGW, SigC, Z, EcRPA, EcGM) bind(C, name='qkref_ug0w0')
   use iso_c_binding
   implicit none
-  integer(c_int), value :: lin, tda_w, srg, nBas, nOa, nOb
+  integer(c_int), value :: lin, tda_w, srg, nBas, nOa, nOb, nCa, nCb, nRa, nRb
   real(c_double), value :: eta
   real(c_double), intent(in) :: ERI_aaaa(nBas,nBas,nBas,nBas), ERI_aabb(nBas,nBas,nBas,nBas)
   real(c_double), intent(in) :: ERI_bbbb(nBas,nBas,nBas,nBas), eHF(nBas,2)
@@ -311,10 +311,10 @@ subroutine qkref_ug0w0(lin, tda_w, srg, eta, nBas, nOa, nOb, ERI_aaaa, ERI_aabb,
   logical :: TDAW, doSRG
   TDAW = tda_w /= 0
   doSRG = srg /= 0
-  nC = 0; nR = 0
+  nC(1) = nCa; nC(2) = nCb; nR(1) = nRa; nR(2) = nRb
   nO(1) = nOa; nO(2) = nOb
   nV = nBas - nO
-  nS = nO*nV
+  nS = (nO - nC)*(nV - nR)
   nSa = nS(1); nSb = nS(2); nSt = nSa + nSb
   allocate(Aph(nSt,nSt), Bph(nSt,nSt), Om(nSt), XpY(nSt,nSt), XmY(nSt,nSt), rho(nBas,nBas,nSt,2), eGWlin(nBas,2))
   Bph = 0d0
@@ -348,11 +348,12 @@ end subroutine
 
 ! evUGW's cycle (GW/evUGW.f90:103-215) composed of QuAcK's own routines, with
 ! TDA = TDA_W for the B build and the printing left out
-subroutine qkref_evugw(maxSCF, thresh, max_diis, lin, tda_w, srg, eta, nBas, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, &
-                       eHF, eGW, SigC, Z, EcRPA, EcGM, nSCF_out) bind(C, name='qkref_evugw')
+subroutine qkref_evugw(maxSCF, thresh, max_diis, lin, tda_w, srg, eta, nBas, nOa, nOb, nCa, nCb, nRa, nRb, &
+                       ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, eGW, SigC, Z, EcRPA, EcGM, nSCF_out) &
+                       bind(C, name='qkref_evugw')
   use iso_c_binding
   implicit none
-  integer(c_int), value :: maxSCF, max_diis, lin, tda_w, srg, nBas, nOa, nOb
+  integer(c_int), value :: maxSCF, max_diis, lin, tda_w, srg, nBas, nOa, nOb, nCa, nCb, nRa, nRb
   real(c_double), value :: thresh, eta
   real(c_double), intent(in) :: ERI_aaaa(nBas,nBas,nBas,nBas), ERI_aabb(nBas,nBas,nBas,nBas)
   real(c_double), intent(in) :: ERI_bbbb(nBas,nBas,nBas,nBas), eHF(nBas,2)
@@ -365,10 +366,10 @@ subroutine qkref_evugw(maxSCF, thresh, max_diis, lin, tda_w, srg, eta, nBas, nOa
   logical :: TDAW, doSRG
   TDAW = tda_w /= 0
   doSRG = srg /= 0
-  nC = 0; nR = 0
+  nC(1) = nCa; nC(2) = nCb; nR(1) = nRa; nR(2) = nRb
   nO(1) = nOa; nO(2) = nOb
   nV = nBas - nO
-  nS = nO*nV
+  nS = (nO - nC)*(nV - nR)
   nSa = nS(1); nSb = nS(2); nSt = nSa + nSb
   allocate(Aph(nSt,nSt), Bph(nSt,nSt), Om(nSt), XpY(nSt,nSt), XmY(nSt,nSt), rho(nBas,nBas,nSt,2), eOld(nBas,2))
   allocate(error_diis(nBas,max(max_diis,1),2), e_diis(nBas,max(max_diis,1),2))

@@ -548,7 +548,7 @@ void screening(Ctx* c, const double* e, const double* eh, int tda, bool vectors)
     {
         Stage st(c, 0);
         if (c->nOb >= 0) {
-            delta_u(c->s, e, c->dE, c->sd, c->nOrb, c->nO, c->nOb);
+            delta_u(c->s, e, c->dE, c->sd, c->up);
             rpa_build_c(c->s, c->K, c->dE, c->sd, c->Cm, c->scal, c->nS, tda);
         } else {
             rpa_build(c->s, e, c->K, c->dE, c->sd, c->Cm, c->scal, c->nC, c->nO, c->v, c->nS, tda);

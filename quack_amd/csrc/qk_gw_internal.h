@@ -148,10 +148,11 @@ struct qk_gw_ctx {
     long long* eig_lbuf = nullptr;
     bool have_vec = false;
     int tda_vec = 0;
-    // unrestricted pairs (the screening of qk_ug0w0 and qk_evugw): alpha nO,
-    // beta nOb, nC = nR = 0, nS = nSa + nSb, energies (e_alpha, e_beta); -1
-    // for the restricted handle
+    // unrestricted pairs (the screening of qk_ug0w0 and qk_evugw): nOb = beta
+    // occupied count, -1 for the restricted handle; up holds the pairs of both
+    // spins (nS = up.nSt), the energies are (e_alpha, e_beta)
     int nOb = -1;
+    qk::UPairs up;
     std::vector<double> e_vec;
     // dsyevd calls, refined solves, Newton steps, fallbacks; fallbacks because a
     // cluster outgrew EIG_KMAX, because merging did not settle, because the

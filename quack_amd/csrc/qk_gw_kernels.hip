@@ -56,36 +56,37 @@ __global__ void k_extract_k(const double* __restrict__ M, double* __restrict__ K
 // RPA kernels then build the unrestricted matrices unchanged from
 // K[ia + nSt jb] = (ia|jb) / 2: A = D + K_u, B = K_u give A + B = D + 2 K_u and
 // Tr A = sum D + K_u(ia, ia), the restricted forms with 2K in place of K_u.
-__device__ inline void upair(int ia, int n, int nOa, int nOb, int nSa, int& s, int& i, int& a) {
-    s = ia >= nSa;
-    const int l = ia - (s ? nSa : 0);
-    const int v = n - (s ? nOb : nOa);
-    a = (s ? nOb : nOa) + l % v;
-    i = l / v;
+__device__ inline void upair(int ia, const UPairs& u, int& s, int& i, int& a) {
+    s = ia >= u.nSa;
+    const int l = ia - (s ? u.nSa : 0);
+    const int v = u.n - u.nR[s] - u.nO[s];
+    a = u.nO[s] + l % v;
+    i = u.nC[s] + l / v;
 }
 
 // K[ia + nSt jb] = (ia|jb) / 2 from M_s[p + n q + n^2 jb] = (pq|jb), s = spin of ia
 __global__ void k_extract_ku(const double* __restrict__ M0, const double* __restrict__ M1, double* __restrict__ K,
-                             int n, int nOa, int nOb, int nSa, int nSt) {
+                             UPairs u) {
+    const int n = u.n, nSt = u.nSt;
     const long long total = (long long)nSt * nSt;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
         const int ia = (int)(t % nSt);
         const long long jb = t / nSt;
         int s, i, a;
-        upair(ia, n, nOa, nOb, nSa, s, i, a);
+        upair(ia, u, s, i, a);
         const double* M = s ? M1 : M0;
         K[t] = 0.5 * M[i + (long long)n * a + (long long)n * n * jb];
     }
 }
 
 // dE[ia] = e_s[a] - e_s[i], sd[ia] = sqrt(dE), e holding e_alpha then e_beta
-__global__ void k_delta_u(const double* __restrict__ e, double* __restrict__ dE, double* __restrict__ sd, int n,
-                          int nOa, int nOb, int nSa, int nSt) {
+__global__ void k_delta_u(const double* __restrict__ e, double* __restrict__ dE, double* __restrict__ sd, UPairs u) {
     const int ia = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ia < nSt) {
+    const int n = u.n;
+    if (ia < u.nSt) {
         int s, i, a;
-        upair(ia, n, nOa, nOb, nSa, s, i, a);
+        upair(ia, u, s, i, a);
         const double d = e[n * s + a] - e[n * s + i];
         dE[ia] = d;
         sd[ia] = sqrt(d);
@@ -916,15 +917,12 @@ void rpa_build_c(hipStream_t s, const double* K, const double* dE, const double*
     hipLaunchKernelGGL(k_trace_a, dim3(1, 1, nz), dim3(RB), 0, s, K, dE, nS, scal, zs);
     QK_HIP(hipGetLastError());
 }
-void extract_ku(hipStream_t s, const double* M0, const double* M1, double* K, int n, int nOa, int nOb) {
-    const int nSa = nOa * (n - nOa), nSt = nSa + nOb * (n - nOb);
-    hipLaunchKernelGGL(k_extract_ku, dim3(grid_for((long long)nSt * nSt)), dim3(256), 0, s, M0, M1, K, n, nOa, nOb,
-                       nSa, nSt);
+void extract_ku(hipStream_t s, const double* M0, const double* M1, double* K, const UPairs& up) {
+    hipLaunchKernelGGL(k_extract_ku, dim3(grid_for((long long)up.nSt * up.nSt)), dim3(256), 0, s, M0, M1, K, up);
     QK_HIP(hipGetLastError());
 }
-void delta_u(hipStream_t s, const double* e, double* dE, double* sd, int n, int nOa, int nOb) {
-    const int nSa = nOa * (n - nOa), nSt = nSa + nOb * (n - nOb);
-    hipLaunchKernelGGL(k_delta_u, dim3((nSt + 255) / 256), dim3(256), 0, s, e, dE, sd, n, nOa, nOb, nSa, nSt);
+void delta_u(hipStream_t s, const double* e, double* dE, double* sd, const UPairs& up) {
+    hipLaunchKernelGGL(k_delta_u, dim3((up.nSt + 255) / 256), dim3(256), 0, s, e, dE, sd, up);
     QK_HIP(hipGetLastError());
 }
 void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w,

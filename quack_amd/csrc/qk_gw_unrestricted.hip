@@ -1,5 +1,5 @@
 // Unrestricted G0W0 and evGW (qk_ug0w0, qk_evugw; GW/UG0W0.f90:92-150,
-// GW/evUGW.f90:103-215 with nC = nR = 0), and the host-array self-energy and
+// GW/evUGW.f90:103-215, frozen orbitals per spin), and the host-array self-energy and
 // graphical QP entries (qk_rgw_self_energy_diag, qk_rgw_qp_graph) that share
 // their device work buffers.
 #include "qk_gw_internal.h"
@@ -106,6 +106,15 @@ struct HostRho {
     }
 };
 
+// Orbital counts of qk_ug0w0 / qk_evugw: per spin 0 <= nC < nO < nBas - nR
+// (at least one active occupied and one active virtual orbital).
+bool upairs_ok(int nBas, const int* nC, const int* nO, const int* nR) {
+    if (nBas <= 0 || !nC || !nO || !nR) return false;
+    for (int s = 0; s < 2; ++s)
+        if (nC[s] < 0 || nR[s] < 0 || nO[s] <= nC[s] || nO[s] >= nBas - nR[s]) return false;
+    return true;
+}
+
 // Eigensolver of the unrestricted screening (qk_set_unrestricted_eigensolver),
 // process-wide, and the eigensolver counts of the calling thread's last
 // qk_ug0w0 / qk_evugw (qk_unrestricted_eig_stats).
@@ -113,9 +122,10 @@ std::atomic<int> g_u_eig_mode{0};
 std::atomic<double> g_u_eig_demax{5e-3};
 thread_local long long g_u_eig_count[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
-// Unrestricted screening and excitation densities (nC = nR = 0), shared by
-// UG0W0 and evUGW.  Pair index ia = a' + v_s i' within spin block s, the beta
-// block after the alpha one (LR/phULR_A.f90, LR/phULR_B.f90).  The integral
+// Unrestricted screening and excitation densities, shared by UG0W0 and evUGW.
+// Frozen core (nC) and frozen virtuals (nR) are counted per spin.  Pair index
+// ia = a' + v_s i' over the active i, a of spin block s, the beta block after
+// the alpha one (LR/phULR_A.f90, LR/phULR_B.f90; UPairs).  The integral
 // slabs M_s(pq, k) = (pq|k) of each spin's densities
 // (GW/UGW_excitation_density.f90:24-62) and the coupling K = (ia|jb) / 2
 // gathered from them stay on the device for the whole calculation.  The dRPA
@@ -128,7 +138,8 @@ thread_local long long g_u_eig_count[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 // restricted kernels' rhoT layout (m contiguous): the unrestricted sums weigh
 // rho^2 where the restricted kernels weigh 2 rho^2.
 struct UScreen {
-    int n, nO[2], v[2], nSa, nSb, nSt, TDA_W;
+    int n, nC[2], nO[2], nR[2], v[2], nSa, nSb, nSt, TDA_W;
+    UPairs up;
     size_t S2, n2;
     const double* G[3];
     Ctx* c = nullptr;
@@ -137,11 +148,14 @@ struct UScreen {
     double *dM[2] = {nullptr, nullptr}, *dR[2] = {nullptr, nullptr};
     QpWork* q = nullptr;
 
-    UScreen(int n_, int nOa, int nOb, const double* Gaaaa, const double* Gaabb, const double* Gbbbb, int tda)
-        : n(n_), nO{nOa, nOb}, v{n_ - nOa, n_ - nOb}, TDA_W(tda), G{Gaaaa, Gaabb, Gbbbb} {
-        nSa = nO[0] * v[0];
-        nSb = nO[1] * v[1];
-        nSt = nSa + nSb;
+    UScreen(int n_, const int* nC_, const int* nO_, const int* nR_, const double* Gaaaa, const double* Gaabb,
+            const double* Gbbbb, int tda)
+        : n(n_), nC{nC_[0], nC_[1]}, nO{nO_[0], nO_[1]}, nR{nR_[0], nR_[1]}, TDA_W(tda), up(n_, nC_, nO_, nR_),
+          G{Gaaaa, Gaabb, Gbbbb} {
+        for (int sp = 0; sp < 2; ++sp) v[sp] = up.vir(sp);
+        nSa = up.nSa;
+        nSb = up.nSt - up.nSa;
+        nSt = up.nSt;
         QK_CHECK((long long)nSt * nSt < (1LL << 40), QK_E_ARG, "nSa + nSb too large");
         S2 = (size_t)nSt * nSt;
         n2 = (size_t)n * n;
@@ -149,12 +163,13 @@ struct UScreen {
         try {
             c = new Ctx();
             c->nBas = c->nOrb = n;
-            c->nC = c->nR = 0;
-            c->nO = nOa;
-            c->nOb = nOb;
-            c->nV = n - nOa;
-            c->o = nOa;
-            c->v = n - nOa;
+            c->nC = c->nR = 0;  // the restricted pair counts are unused: screening() takes up
+            c->nO = nO[0];
+            c->nOb = nO[1];
+            c->up = up;
+            c->nV = n - nO[0];
+            c->o = nO[0];
+            c->v = n - nO[0];
             c->nS = nSt;
             QK_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
             rb_check(rocblas_create_handle(&c->rb), "rocblas_create_handle");
@@ -174,9 +189,9 @@ struct UScreen {
                 dM[sp] = dalloc<double>(n2 * nSt);
                 dR[sp] = dalloc<double>(n2 * nSt);
                 for (int sj = 0; sj < 2; ++sj)
-                    for (int j = 0; j < nO[sj]; ++j)
+                    for (int j = nC[sj]; j < nO[sj]; ++j)
                         for (int b = 0; b < v[sj]; ++b) {
-                            const size_t k = (sj ? nSa : 0) + b + (size_t)v[sj] * j;
+                            const size_t k = (sj ? nSa : 0) + b + (size_t)v[sj] * (j - nC[sj]);
                             const int Bb = nO[sj] + b;
                             for (int qq = 0; qq < n; ++qq)
                                 for (int p = 0; p < n; ++p)
@@ -187,7 +202,7 @@ struct UScreen {
                 QK_HIP(hipMemcpyAsync(dM[sp], Ms.data(), sizeof(double) * n2 * nSt, hipMemcpyHostToDevice, st));
                 QK_HIP(hipStreamSynchronize(st));  // Ms is refilled for the next spin
             }
-            extract_ku(st, dM[0], dM[1], c->K, n, nO[0], nO[1]);
+            extract_ku(st, dM[0], dM[1], c->K, up);
             const int rc = qk_gw_set_eigensolver(c, g_u_eig_mode.load(), g_u_eig_demax.load());
             QK_CHECK(rc == QK_OK, rc, "unrestricted screening: " + c->err);
             q = new QpWork(n, sigma_chunks(nSt, n));
@@ -223,8 +238,9 @@ struct UScreen {
             // must be positive definite; here it is diag(e_a - e_i)
             double dmin = HUGE_VAL;
             for (int sp = 0; sp < 2; ++sp)
-                for (int i = 0; i < nO[sp]; ++i)
-                    for (int a = nO[sp]; a < n; ++a) dmin = std::min(dmin, en[(size_t)n * sp + a] - en[(size_t)n * sp + i]);
+                for (int i = nC[sp]; i < nO[sp]; ++i)
+                    for (int a = nO[sp]; a < n - nR[sp]; ++a)
+                        dmin = std::min(dmin, en[(size_t)n * sp + a] - en[(size_t)n * sp + i]);
             QK_CHECK(dmin > 0.0, QK_E_UNSTABLE,
                      std::string(who) + ": A-B is not positive definite (smallest eigenvalue " +
                          std::to_string(dmin) + "): linear-response instability");
@@ -250,13 +266,13 @@ struct UScreen {
     // EcGM per spin, halved (the restricted sum carries 4 rho^2, the unrestricted rho^2, x2 here)
     void sigma(int sp, const double* es, double eta, int doSRG, double* sig, double* z, double* ecgm) {
         double gm = 0.0;
-        sigma_dev(st, *q, dR[sp], c->Om, doSRG, eta, 0, nO[sp], 0, nSt, es, sig, z, &gm);
+        sigma_dev(st, *q, dR[sp], c->Om, doSRG, eta, nC[sp], nO[sp], nR[sp], nSt, es, sig, z, &gm);
         if (ecgm) *ecgm = 0.5 * gm;
     }
     // GW/UGW_QP_graph.f90 for spin sp
     void qp(int sp, int doSRG, double eta, const double* eHF_s, const double* eLin, const double* eOld, double* eGW_s,
             double* z) {
-        qp_dev(st, *q, dR[sp], c->Om, doSRG, eta, SRG_FLOW, 0, nO[sp], 0, nSt, eHF_s, eLin, eOld, eGW_s, z);
+        qp_dev(st, *q, dR[sp], c->Om, doSRG, eta, SRG_FLOW, nC[sp], nO[sp], nR[sp], nSt, eHF_s, eLin, eOld, eGW_s, z);
     }
 };
 }  // namespace
@@ -292,15 +308,14 @@ int qk_rgw_qp_graph(int doSRG, double eta, double flow, int nOrb, int nC, int nO
 }
 
 
-int qk_ug0w0(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* ERI_aabb, const double* ERI_bbbb,
-             const double* eHF, double eta, int linearize, int TDA_W, int doSRG, double* eGW, double* SigC, double* Z,
-             double* Om, double* EcRPA, double* EcGM) {
-    if (nBas <= 0 || nOa <= 0 || nOb <= 0 || nOa >= nBas || nOb >= nBas || !ERI_aaaa || !ERI_aabb || !ERI_bbbb ||
-        !eHF || !eGW || !SigC || !Z)
+int qk_ug0w0(int nBas, const int* nC, const int* nO, const int* nR, const double* ERI_aaaa, const double* ERI_aabb,
+             const double* ERI_bbbb, const double* eHF, double eta, int linearize, int TDA_W, int doSRG, double* eGW,
+             double* SigC, double* Z, double* Om, double* EcRPA, double* EcGM) {
+    if (!upairs_ok(nBas, nC, nO, nR) || !ERI_aaaa || !ERI_aabb || !ERI_bbbb || !eHF || !eGW || !SigC || !Z)
         return QK_E_ARG;
     return guarded(nullptr, [&] {
         const int n = nBas;
-        UScreen u(n, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
+        UScreen u(n, nC, nO, nR, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
         u.rpa(eHF, "qk_ug0w0");
         u.densities();
         if (Om) u.om_host(Om);
@@ -321,17 +336,18 @@ int qk_ug0w0(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* E
     });
 }
 
-int qk_evugw(int nBas, int nOa, int nOb, const double* ERI_aaaa, const double* ERI_aabb, const double* ERI_bbbb,
-             const double* eHF, int maxSCF, double thresh, int max_diis, double eta, int linearize, int TDA_W,
-             int doSRG, double* eGW, double* SigC, double* Z, double* Om, double* EcRPA, double* EcGM, int* nSCF_out) {
-    if (nBas <= 0 || nOa <= 0 || nOb <= 0 || nOa >= nBas || nOb >= nBas || !ERI_aaaa || !ERI_aabb || !ERI_bbbb ||
-        !eHF || !eGW || !SigC || !Z || maxSCF < 0)
+int qk_evugw(int nBas, const int* nC, const int* nO, const int* nR, const double* ERI_aaaa, const double* ERI_aabb,
+             const double* ERI_bbbb, const double* eHF, int maxSCF, double thresh, int max_diis, double eta,
+             int linearize, int TDA_W, int doSRG, double* eGW, double* SigC, double* Z, double* Om, double* EcRPA,
+             double* EcGM, int* nSCF_out) {
+    if (!upairs_ok(nBas, nC, nO, nR) || !ERI_aaaa || !ERI_aabb || !ERI_bbbb || !eHF || !eGW || !SigC || !Z ||
+        maxSCF < 0)
         return QK_E_ARG;
     int conv_status = QK_OK;
     int rc = guarded(nullptr, [&] {
         const int n = nBas;
         const size_t n2s = (size_t)n * 2;
-        UScreen u(n, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
+        UScreen u(n, nC, nO, nR, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
         // GW/evUGW.f90:103-215: eGW = eOld = eHF, Z = 1; per cycle screening at eGW,
         // self-energy at eGW, QP energies (linearised: eHF + SigC; graphical: guess and
         // G from eOld), Conv = max |eGW - eOld| over both spins, DIIS per spin with

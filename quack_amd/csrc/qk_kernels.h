@@ -16,6 +16,26 @@ struct Zs {
     long long n = 0, S = 0, S2 = 0, R = 0, P = 0, c = 0, i = 0;
 };
 
+// Unrestricted particle-hole pairs (LR/phULR_A.f90:55-118): spin s has active
+// occupied i = nC[s] .. nO[s]-1 and virtuals a = nO[s] .. n-nR[s]-1 (0-based);
+// pair ia = (a - nO[s]) + v_s (i - nC[s]), v_s = n - nR[s] - nO[s], the beta
+// block after the nSa alpha pairs.
+struct UPairs {
+    int n = 0, nC[2] = {0, 0}, nO[2] = {0, 0}, nR[2] = {0, 0}, nSa = 0, nSt = 0;
+    UPairs() = default;
+    UPairs(int n_, const int* nC_, const int* nO_, const int* nR_) : n(n_) {
+        for (int s = 0; s < 2; ++s) {
+            nC[s] = nC_[s];
+            nO[s] = nO_[s];
+            nR[s] = nR_[s];
+        }
+        nSa = occ(0) * vir(0);
+        nSt = nSa + occ(1) * vir(1);
+    }
+    int occ(int s) const { return nO[s] - nC[s]; }
+    int vir(int s) const { return n - nR[s] - nO[s]; }
+};
+
 void permute_block(hipStream_t s, const double* X, double* M, int n, int o, int v);
 // f (N^4, QuAcK's ERI.bin order) from the 8-fold packed chemists' array
 // (N(N+1)/2 pairs, pair-of-pairs packed)
@@ -27,11 +47,11 @@ void rpa_build(hipStream_t s, const double* e, const double* K, double* dE, doub
 // rpa_build without the pair energies: dE, sd given
 void rpa_build_c(hipStream_t s, const double* K, const double* dE, const double* sd, double* Cm, double* scal, int nS,
                  int tda, const Zs& zs = Zs(), int nz = 1);
-// Unrestricted pairs (nC = nR = 0; alpha block, then beta): K = (ia|jb) / 2
-// from the spin slabs M_s[p + n q + n^2 jb] = (pq|jb), and the pair energies
-// from e = (e_alpha, e_beta)
-void extract_ku(hipStream_t s, const double* M0, const double* M1, double* K, int n, int nOa, int nOb);
-void delta_u(hipStream_t s, const double* e, double* dE, double* sd, int n, int nOa, int nOb);
+// Unrestricted pairs (alpha block, then beta): K = (ia|jb) / 2 from the spin
+// slabs M_s[p + n q + n^2 jb] = (pq|jb), and the pair energies from
+// e = (e_alpha, e_beta)
+void extract_ku(hipStream_t s, const double* M0, const double* M1, double* K, const UPairs& up);
+void delta_u(hipStream_t s, const double* e, double* dE, double* sd, const UPairs& up);
 void rpa_finalize(hipStream_t s, double* Om, double* W, const double* sd, double* scal, int nS, int tda, bool scale_w,
                   const Zs& zs = Zs(), int nz = 1);
 int sigma_chunks(int nS, int nact);

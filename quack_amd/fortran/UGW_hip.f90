@@ -6,8 +6,9 @@
 ! Screening, excitation densities, self-energies and QP solves run in
 ! libquack_amd.so (qk_ug0w0, qk_evugw); the summaries and test dumps follow
 ! src/GW/print_UG0W0.f90:55-63, print_evUGW.f90:70-81, UG0W0.f90:227-233 and
-! evUGW.f90:279-285.  Frozen core or Rydberg orbitals, BSE and ACFDT stay on
-! QuAcK's CPU routines: requesting them here stops with a nonzero code.
+! evUGW.f90:279-285.  Frozen core and Rydberg orbitals (nC, nR per spin) run
+! on the GPU; BSE and ACFDT stay on QuAcK's CPU routines: requesting them here
+! stops with a nonzero code.
 
 subroutine qk_print_utable(title, nBas, nO, eHF, SigC, Z, eGW)
   implicit none
@@ -22,11 +23,9 @@ subroutine qk_print_utable(title, nBas, nO, eHF, SigC, Z, eGW)
   end do
 end subroutine
 
-subroutine qk_ugw_checks(nC, nR, doACFDT, dophBSE)
+subroutine qk_ugw_checks(doACFDT, dophBSE)
   implicit none
-  integer, intent(in) :: nC(2), nR(2)
   logical, intent(in) :: doACFDT, dophBSE
-  if(any(nC /= 0) .or. any(nR /= 0)) call qk_unsupported('unrestricted GW with frozen core or Rydberg orbitals')
   if(dophBSE) call qk_unsupported('phBSE@UGW')
   if(doACFDT) call qk_unsupported('ACFDT@UGW')
 end subroutine
@@ -49,7 +48,7 @@ subroutine UG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,TDA_W,TDA,dBSE
   double precision :: EcRPA, EcGM(2), eHOMO, eLUMO
   integer(c_int) :: rc
 
-  call qk_ugw_checks(nC, nR, doACFDT, dophBSE)
+  call qk_ugw_checks(doACFDT, dophBSE)
   write(*,*)
   write(*,*)'*********************************'
   write(*,*)'* Unrestricted G0W0 Calculation *'
@@ -59,7 +58,7 @@ subroutine UG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,TDA_W,TDA,dBSE
   if(doSRG) write(*,*) '*** SRG regularized G0W0 scheme ***'
 
   allocate(eGW(nBas,2), SigC(nBas,2), Z(nBas,2), Om(nS(1)+nS(2)))
-  rc = qk_ug0w0(int(nBas,c_int), int(nO(1),c_int), int(nO(2),c_int), ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, eta, &
+  rc = qk_ug0w0(int(nBas,c_int), int(nC,c_int), int(nO,c_int), int(nR,c_int), ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, eta, &
                 merge(1_c_int, 0_c_int, linearize), merge(1_c_int, 0_c_int, TDA_W), merge(1_c_int, 0_c_int, doSRG), &
                 eGW, SigC, Z, Om, EcRPA, EcGM)
   call qk_check(rc, c_null_ptr, 'qk_ug0w0')
@@ -106,7 +105,7 @@ subroutine evUGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS
   double precision :: EcRPA, EcGM(2), eHOMO, eLUMO
   integer(c_int) :: rc, nSCF
 
-  call qk_ugw_checks(nC, nR, doACFDT, BSE)
+  call qk_ugw_checks(doACFDT, BSE)
   write(*,*)
   write(*,*)'*********************************'
   write(*,*)'| Unrestricted evGW Calculation *'
@@ -116,7 +115,7 @@ subroutine evUGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS
   if(doSRG) write(*,*) '*** SRG regularized evGW scheme ***'
 
   allocate(eGW(nBas,2), SigC(nBas,2), Z(nBas,2), Om(nS(1)+nS(2)))
-  rc = qk_evugw(int(nBas,c_int), int(nO(1),c_int), int(nO(2),c_int), ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, &
+  rc = qk_evugw(int(nBas,c_int), int(nC,c_int), int(nO,c_int), int(nR,c_int), ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, &
                 int(maxSCF,c_int), thresh, int(max_diis,c_int), eta, merge(1_c_int, 0_c_int, linearize), &
                 merge(1_c_int, 0_c_int, TDA_W), merge(1_c_int, 0_c_int, doSRG), eGW, SigC, Z, Om, EcRPA, EcGM, nSCF)
   if(rc == QK_E_NOCONV) then

@@ -24,7 +24,7 @@ program qk_gw_driver
   implicit none
   character(len=512) :: wd, line
   logical :: doG0W0, doevGW, doqsGW, doscGW, dotest, dotU, dotG, doRHF, doUHF
-  integer :: maxSCF, max_diis, nfreqs, nAt, nEla, nElb, nC, nR, nO, nV, nS, nOrb, ios
+  integer :: maxSCF, max_diis, nfreqs, nAt, nEla, nElb, nCore, nRyd, nC, nR, nO, nV, nS, nOrb, ios
   double precision :: thresh, eta, shift, ENuc, ERHF
   character(len=1) :: a1, a2, a3, a4, a5, a6, a7, a8
   logical :: lin, TDA_W, reg, TDA, singlet, triplet, dophBSE, dophBSE2, doppBSE, dBSE, dTDA
@@ -94,8 +94,15 @@ program qk_gw_driver
   ! molecule
   open(unit=13, file=trim(wd)//'/input/molecule', status='old', action='read')
   read(13,*)
-  read(13,*) nAt, nEla, nElb, nC, nR
+  read(13,*) nAt, nEla, nElb, nCore, nRyd
   close(13)
+  ! core and Rydberg electrons, frozen in pairs (utils/read_molecule.f90:43-53)
+  if(mod(nCore,2) /= 0 .or. mod(nRyd,2) /= 0) then
+    write(*,*) 'The number of core and Rydberg electrons must be even!'
+    error stop 2
+  end if
+  nC = nCore/2
+  nR = nRyd/2
   if(nEla /= nElb .and. .not. doUHF) then
     write(*,*) 'qk_gw_driver: open-shell molecules need the UHF branch'
     error stop 2

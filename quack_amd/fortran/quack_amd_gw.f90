@@ -82,21 +82,23 @@ module quack_amd_gw
       real(c_double), intent(out) :: EcBSE(2), OmBSE(*)
     end function
 
-    ! unrestricted G0W0 and evGW (GW/UG0W0.f90, GW/evUGW.f90), nC = nR = 0
-    integer(c_int) function qk_ug0w0(nBas, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, eta, linearize, TDA_W, doSRG, &
-                                     eGW, SigC, Z, Om, EcRPA, EcGM) bind(C, name='qk_ug0w0')
+    ! unrestricted G0W0 and evGW (GW/UG0W0.f90, GW/evUGW.f90); nC, nO, nR per spin
+    integer(c_int) function qk_ug0w0(nBas, nC, nO, nR, ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, eta, linearize, TDA_W, &
+                                     doSRG, eGW, SigC, Z, Om, EcRPA, EcGM) bind(C, name='qk_ug0w0')
       import :: c_int, c_double
-      integer(c_int), value :: nBas, nOa, nOb, linearize, TDA_W, doSRG
+      integer(c_int), value :: nBas, linearize, TDA_W, doSRG
+      integer(c_int), intent(in) :: nC(2), nO(2), nR(2)
       real(c_double), intent(in) :: ERI_aaaa(*), ERI_aabb(*), ERI_bbbb(*), eHF(*)
       real(c_double), value :: eta
       real(c_double), intent(out) :: eGW(*), SigC(*), Z(*), Om(*), EcRPA, EcGM(2)
     end function
 
-    integer(c_int) function qk_evugw(nBas, nOa, nOb, ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, maxSCF, thresh, max_diis, eta, &
-                                     linearize, TDA_W, doSRG, eGW, SigC, Z, Om, EcRPA, EcGM, nSCF) &
+    integer(c_int) function qk_evugw(nBas, nC, nO, nR, ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, maxSCF, thresh, max_diis, &
+                                     eta, linearize, TDA_W, doSRG, eGW, SigC, Z, Om, EcRPA, EcGM, nSCF) &
                                      bind(C, name='qk_evugw')
       import :: c_int, c_double
-      integer(c_int), value :: nBas, nOa, nOb, maxSCF, max_diis, linearize, TDA_W, doSRG
+      integer(c_int), value :: nBas, maxSCF, max_diis, linearize, TDA_W, doSRG
+      integer(c_int), intent(in) :: nC(2), nO(2), nR(2)
       real(c_double), intent(in) :: ERI_aaaa(*), ERI_aabb(*), ERI_bbbb(*), eHF(*)
       real(c_double), value :: thresh, eta
       real(c_double), intent(out) :: eGW(*), SigC(*), Z(*), Om(*), EcRPA, EcGM(2)

@@ -135,8 +135,9 @@ def load():
     lib.qk_noalloc_last_error.restype = ctypes.c_char_p
     lib.qk_rgw_phbse.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 5
     lib.qk_gw_sigma_full.argtypes = [_H, _P, _D, _I, _P, _P, _P, _P]
-    lib.qk_ug0w0.argtypes = [_I, _I, _I, _P, _P, _P, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
-    lib.qk_evugw.argtypes = [_I, _I, _I, _P, _P, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
+    lib.qk_ug0w0.argtypes = [_I, _IP, _IP, _IP, _P, _P, _P, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
+    lib.qk_evugw.argtypes = [_I, _IP, _IP, _IP, _P, _P, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P,
+                             _IP]
     lib.qk_set_unrestricted_eigensolver.argtypes = [_I, _D]
     lib.qk_gwb_set_graphs.argtypes = [_H, _I]
     lib.qk_set_split_aotomo.argtypes = [_I]
@@ -630,38 +631,50 @@ def unrestricted_eig_stats():
     return _eig_counts(c)
 
 
-def UG0W0(ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False):
-    """GW/UG0W0.f90 on the GPU (qk_ug0w0), nC = nR = 0.  eHF: nBas x 2 (alpha,
-    beta); nO = (nO_alpha, nO_beta).  Returns dict(eGW, SigC, Z (nBas x 2), Om,
-    EcRPA, EcGM (2))."""
+def _ucounts(n, nC, nO, nR):
+    """Per-spin int32 arrays nC, nO, nR and the pair count nSa + nSb."""
+    c, o, r = (np.ascontiguousarray(x, dtype=np.int32).reshape(2) for x in (nC, nO, nR))
+    nSt = int(sum((o[s] - c[s]) * (n - r[s] - o[s]) for s in range(2)))
+    return c, o, r, nSt
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+def UG0W0(ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False, nC=(0, 0),
+          nR=(0, 0)):
+    """GW/UG0W0.f90 on the GPU (qk_ug0w0).  eHF: nBas x 2 (alpha, beta); nO,
+    nC, nR = (alpha, beta) occupied, frozen core and frozen virtual counts.
+    Returns dict(eGW, SigC, Z (nBas x 2), Om, EcRPA, EcGM (2))."""
     eHF = np.asfortranarray(eHF, dtype=np.float64)
     n = eHF.shape[0]
-    nSt = nO[0] * (n - nO[0]) + nO[1] * (n - nO[1])
+    c, o, r, nSt = _ucounts(n, nC, nO, nR)
     eGW, Sig, Z = (np.zeros((n, 2), order="F") for _ in range(3))
     Om = np.zeros(nSt)
     Ec = _D(0.0)
     Egm = np.zeros(2)
-    _check(load().qk_ug0w0(n, int(nO[0]), int(nO[1]), _p(_f64(ERI_aaaa)), _p(_f64(ERI_aabb)), _p(_f64(ERI_bbbb)),
+    _check(load().qk_ug0w0(n, _ip(c), _ip(o), _ip(r), _p(_f64(ERI_aaaa)), _p(_f64(ERI_aabb)), _p(_f64(ERI_bbbb)),
                            _p(eHF), eta, int(linearize), int(TDA_W), int(doSRG), _p(eGW), _p(Sig), _p(Z), _p(Om),
                            ctypes.byref(Ec), _p(Egm)))
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": Ec.value, "EcGM": Egm}
 
 
 def evUGW(ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, eta=0.0, linearize=False,
-          TDA_W=False, doSRG=False):
-    """GW/evUGW.f90 on the GPU (qk_evugw), nC = nR = 0, QuAcK's defaults for
-    maxSCF, thresh and max_diis.  Returns dict(eGW, SigC, Z, Om, EcRPA, EcGM,
-    nSCF, converged); non-convergence is reported in "converged" with the last
-    cycle's outputs (QuAcK stops there)."""
+          TDA_W=False, doSRG=False, nC=(0, 0), nR=(0, 0)):
+    """GW/evUGW.f90 on the GPU (qk_evugw), QuAcK's defaults for maxSCF, thresh
+    and max_diis; orbital counts as UG0W0.  Returns dict(eGW, SigC, Z, Om,
+    EcRPA, EcGM, nSCF, converged); non-convergence is reported in "converged"
+    with the last cycle's outputs (QuAcK stops there)."""
     eHF = np.asfortranarray(eHF, dtype=np.float64)
     n = eHF.shape[0]
-    nSt = nO[0] * (n - nO[0]) + nO[1] * (n - nO[1])
+    c, o, r, nSt = _ucounts(n, nC, nO, nR)
     eGW, Sig, Z = (np.zeros((n, 2), order="F") for _ in range(3))
     Om = np.zeros(nSt)
     Ec = _D(0.0)
     Egm = np.zeros(2)
     nscf = _I(0)
-    rc = load().qk_evugw(n, int(nO[0]), int(nO[1]), _p(_f64(ERI_aaaa)), _p(_f64(ERI_aabb)), _p(_f64(ERI_bbbb)),
+    rc = load().qk_evugw(n, _ip(c), _ip(o), _ip(r), _p(_f64(ERI_aaaa)), _p(_f64(ERI_aabb)), _p(_f64(ERI_bbbb)),
                          _p(eHF), int(maxSCF), thresh, int(max_diis), eta, int(linearize), int(TDA_W), int(doSRG),
                          _p(eGW), _p(Sig), _p(Z), _p(Om), ctypes.byref(Ec), _p(Egm), ctypes.byref(nscf))
     if rc != QK_E_NOCONV:

@@ -71,13 +71,16 @@ def tf(b):
 
 
 def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=1e-5, max_diis=5, lin=False,
-                 eta=0.0, TDA_W=False, reg=False, eri_mo=None, bse=False, uhf=False, nel=None, use_gpu=True):
+                 eta=0.0, TDA_W=False, reg=False, eri_mo=None, bse=False, uhf=False, nel=None, use_gpu=True,
+                 ncore=0, nryd=0):
     """sysd: a dict from quack_amd.intgen.system(); eri_mo: MO integrals (nOrb^4, flat);
     bse: the phBSE flag of the "# BSE:" options line (static kernel); uhf: the
     methods file selects UHF instead of RHF (the unrestricted GW branch);
     nel: (nEla, nElb) for input/molecule (default: the closed shell); use_gpu:
     the GPU switch of input/hpc_flags (its fourth line, read_hpc_flags.f90:31-33),
-    None to leave the file out (QuAcK then takes both switches as F)."""
+    None to leave the file out (QuAcK then takes both switches as F); ncore,
+    nryd: core and Rydberg electrons of input/molecule (even; QuAcK freezes
+    ncore/2 and nryd/2 orbitals per spin, utils/read_molecule.f90:43-53)."""
     for d in ("input", "int", "test"):
         os.makedirs(os.path.join(wd, d), exist_ok=True)
     mol = sysd["mol"]
@@ -95,7 +98,8 @@ def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=
             f.write("# if True (T), switch to HPC mode\n  F\n# if True (T), use GPU\n  %s\n" % tf(use_gpu))
     with open(os.path.join(wd, "input", "molecule"), "w") as f:
         nea, neb = nel if nel is not None else (mol.nO, mol.nO)
-        f.write("# nAt nEla nElb nCore nRyd\n%d %d %d 0 0\n# Znuc x  y  z\n" % (len(mol.symbols), nea, neb))
+        f.write("# nAt nEla nElb nCore nRyd\n%d %d %d %d %d\n# Znuc x  y  z\n"
+                % (len(mol.symbols), nea, neb, ncore, nryd))
         for s, r in zip(mol.symbols, mol.xyz):
             f.write("%s %.17g %.17g %.17g\n" % (s, r[0], r[1], r[2]))
     with open(os.path.join(wd, "int", "ENuc.dat"), "w") as f:

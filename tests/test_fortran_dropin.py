@@ -130,6 +130,54 @@ def test_driver_unrestricted(tmp_path, mo_systems):
             assert abs(d[k0 + k][1] - ref[k]) < 1e-8, (k0, k, d[k0 + k], ref[k])
 
 
+@pytest.mark.gpu
+def test_driver_frozen_core(tmp_path, mo_systems):
+    """Core and Rydberg electrons in input/molecule (2 and 4: one frozen core
+    and two frozen virtual orbitals per spin, utils/read_molecule.f90:43-53)
+    reach RG0W0_hip / evRGW_hip and UG0W0_hip / evUGW_hip as QuAcK's nC, nR;
+    the dumped values match the CPU restatement with the same counts."""
+    import numpy as np
+
+    from oracle import gw_oracle as go
+    s = mo_systems["H2O"]
+    nO, mo, e = s["nO"], s["eri_mo"], s["eHF"]
+    print("fortran driver frozen core RHF H2O", flush=True)
+    wd = tmp_path / "rhf"
+    quackio.write_inputs(str(wd), s, g0w0=True, evgw=True, maxSCF=64, thresh=1e-6, max_diis=5, eta=0.1, eri_mo=mo,
+                         ncore=2, nryd=4)
+    r = subprocess.run([DRIVER, str(wd)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    dump = quackio.read_test_dump(str(wd / "test" / "Rtest.dat"))
+    for tag, o in (("G0W0", go.rg0w0(mo, e, nO, eta=0.1, nC=1, nR=2)),
+                   ("evGW", go.evrgw(mo, e, nO, maxSCF=64, thresh=1e-6, max_diis=5, eta=0.1, nC=1, nR=2))):
+        assert abs(dump[tag + " correlation energy"] - o["EcRPA"]) < 1e-8, tag
+        assert abs(dump[tag + " HOMO energy"] - o["eGW"][nO - 1]) < 1e-8, tag
+        assert abs(dump[tag + " LUMO energy"] - o["eGW"][nO]) < 1e-8, tag
+    print("fortran driver frozen core UHF H2O open shell", flush=True)
+    wd = tmp_path / "uhf"
+    occ = (nO + 1, nO)
+    quackio.write_inputs(str(wd), s, g0w0=True, evgw=True, maxSCF=64, thresh=1e-5, max_diis=5, eri_mo=mo, uhf=True,
+                         nel=occ, ncore=2, nryd=4)
+    r = subprocess.run([DRIVER, str(wd)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    d = quackio.read_test_dump_pairs(str(wd / "test" / "Utest.dat"))
+    e2 = np.stack([e, e], axis=1)
+    fz = {"nC": (1, 1), "nR": (2, 2)}
+    for k0, o in ((0, go.ug0w0(mo, mo, mo, e2, occ, **fz)), (5, go.evugw(mo, mo, mo, e2, occ, maxSCF=64, **fz))):
+        ref = [o["EcRPA"], o["eGW"][occ[0] - 1, 0], o["eGW"][occ[0], 0], o["eGW"][occ[1] - 1, 1], o["eGW"][occ[1], 1]]
+        for k in range(5):
+            assert abs(d[k0 + k][1] - ref[k]) < 1e-8, (k0, k, d[k0 + k], ref[k])
+
+
+def test_driver_odd_core_electrons_rejected(tmp_path, systems):
+    """An odd number of core electrons stops the driver before any device
+    call, as QuAcK's read_molecule does."""
+    s = systems["He"]
+    quackio.write_inputs(str(tmp_path), s, ncore=1)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "must be even" in r.stdout
+
+
 @pytest.mark.parametrize("use_gpu", [False, None])
 def test_driver_gpu_switch_off(tmp_path, systems, use_gpu):
     """input/hpc_flags with the GPU switch F (or no file, which QuAcK's

@@ -471,3 +471,55 @@ def test_unrestricted_distinct_spins_vs_compiled_reference(uhf_h2o):
     o = go.evugw(u["aa"], u["ab"], u["bb"], u["e2"], occ, doSRG=True)
     assert o["converged"] and o["nSCF"] == r["nSCF"]
     assert np.abs(o["eGW"] - r["eGW"]).max() < 1e-10
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+def test_unrestricted_frozen_vs_compiled_reference(uhf_h2o):
+    """UG0W0 and evUGW with frozen core / virtuals counted per spin (nC(2),
+    nR(2) of GW/UG0W0.f90), different in the two spins, with distinct alpha
+    and beta orbitals: the restatement against QuAcK's compiled unrestricted
+    routines (active entries for G0W0, whose frozen entries QuAcK leaves
+    undefined; every entry and the cycle count for evGW)."""
+    u = uhf_h2o
+    n, nO = u["n"], u["nO"]
+    for occ, nC, nR in (((nO, nO), (1, 1), (0, 0)), ((nO + 1, nO), (1, 0), (2, 1)), ((nO + 1, nO), (0, 1), (3, 0))):
+        act = np.zeros((n, 2), bool)
+        for s in range(2):
+            act[nC[s]:n - nR[s], s] = True
+        for kw in ({"doSRG": True}, {"eta": 0.05}, {"eta": 0.1, "linearize": True}):
+            r = qkref.ug0w0(u["aa"], u["ab"], u["bb"], u["e2"], occ, eta=kw.get("eta", 0.0),
+                            linearize=kw.get("linearize", False), srg=kw.get("doSRG", False), nC=nC, nR=nR)
+            o = go.ug0w0(u["aa"], u["ab"], u["bb"], u["e2"], occ, nC=nC, nR=nR, **kw)
+            assert np.abs(o["eGW"] - r["eGW"])[act].max() < 1e-12, (occ, nC, nR, kw)
+            assert abs(o["EcRPA"] - r["EcRPA"]) < 1e-11
+        r = qkref.evugw(u["aa"], u["ab"], u["bb"], u["e2"], occ, eta=0.1, nC=nC, nR=nR)
+        o = go.evugw(u["aa"], u["ab"], u["bb"], u["e2"], occ, eta=0.1, nC=nC, nR=nR)
+        assert o["converged"] and o["nSCF"] == r["nSCF"], (occ, nC, nR)
+        assert np.abs(o["eGW"] - r["eGW"]).max() < 1e-12
+
+
+def test_unrestricted_frozen_reductions(mo_systems, uhf_h2o):
+    """Two reductions that fix the unrestricted frozen-orbital sums without
+    QuAcK: with alpha = beta it is the restricted frozen-core calculation;
+    with the same frozen sets in both spins it is the unrestricted
+    calculation on the active orbitals alone."""
+    s = mo_systems["H2O"]
+    mo, nO, e = s["eri_mo"], s["nO"], s["eHF"]
+    n = len(e)
+    e2 = np.stack([e, e], axis=1)
+    u = go.ug0w0(mo, mo, mo, e2, (nO, nO), eta=0.1, nC=(1, 1), nR=(2, 2))
+    rr = go.rg0w0(mo, e, nO, eta=0.1, nC=1, nR=2)
+    assert np.abs(u["eGW"] - rr["eGW"][:, None]).max() < 1e-12 and abs(u["EcRPA"] - rr["EcRPA"]) < 1e-11
+    uv = go.evugw(mo, mo, mo, e2, (nO, nO), eta=0.1, thresh=1e-6, nC=(1, 1), nR=(2, 2))
+    rv = go.evrgw(mo, e, nO, eta=0.1, thresh=1e-6, maxSCF=64, nC=1, nR=2)
+    assert uv["nSCF"] == rv["nSCF"] and np.abs(uv["eGW"] - rv["eGW"][:, None]).max() < 1e-11
+    h = uhf_h2o
+    nC, nR = 1, 2
+    act = slice(nC, n - nR)
+    cut = lambda x: np.asfortranarray(go.G4(x, n)[act, act, act, act]).reshape(-1, order="F")
+    occ = (nO + 1, nO)
+    full = go.ug0w0(h["aa"], h["ab"], h["bb"], h["e2"], occ, eta=0.05, nC=(nC, nC), nR=(nR, nR))
+    part = go.ug0w0(cut(h["aa"]), cut(h["ab"]), cut(h["bb"]), h["e2"][act].copy(), (occ[0] - nC, occ[1] - nC),
+                    eta=0.05)
+    assert np.abs(full["eGW"][act] - part["eGW"]).max() < 1e-12
+    assert np.abs(full["Om"] - part["Om"]).max() < 1e-12 and abs(full["EcRPA"] - part["EcRPA"]) < 1e-11
