@@ -393,6 +393,34 @@ const char *qk_mlp_last_error(qk_mlp_handle h);
  * order (ERI(mu,la,nu,si) = <mu la|nu si> at mu + N la + N^2 nu + N^3 si).
  * One handle per basis size; it owns the device copies of P, J, K and the
  * partial sums.  Not re-entrant per handle; synchronous. */
+/* ---- quasiparticle self-consistent GW (GW/qsRGW.f90) -------------------- */
+
+/* A qsGW calculation (GW/qsRGW.f90:1-431 without linDM): sizes as qk_gw_create
+ * (nOrb <= nBas <= 512), DIIS on the Fock matrix with max_diis vectors, and
+ * QuAcK's inputs S, T, V, Hc (nBas x nBas), X (nBas x nOrb), ERI_AO (nBas^4,
+ * QuAcK's ERI.bin order), PHF (nBas x nBas), cHF (nBas x nOrb), eHF (nOrb).
+ * The handle keeps the AO integrals in device memory. */
+typedef struct qk_qsgw_ctx *qk_qsgw_handle;
+int qk_qsgw_create(qk_qsgw_handle *h, int nBas, int nOrb, int nC, int nO, int nV, int nR, int nS, int max_diis,
+                   const double *S, const double *T, const double *V, const double *Hc, const double *X,
+                   const double *ERI_AO, const double *PHF, const double *cHF, const double *eHF);
+/* One pass of the body of qsRGW's loop (GW/qsRGW.f90:199-341): J and K of the
+ * current density, the MO integrals of the current orbitals, dRPA screening
+ * at the current quasiparticle energies, the full self-energy
+ * (GW/RGW_self_energy.f90, or GW/RGW_SRG_self_energy.f90 with doSRG),
+ * symmetrised and taken to the AO basis, F = Hc + J + K/2 + SigC_AO
+ * (projected on the kept orbitals when nOrb < nBas), the commutator
+ * FPS - SPF, the energies, DIIS on F, the new orbitals and energies and the
+ * density.  Outputs: eGW (nOrb), c (nBas x nOrb), SigC (nOrb x nOrb, in the
+ * new orbitals), Z (nOrb), E[5] = ET, EV, EJ, EK, EqsGW, EcRPA, EcGM, and
+ * err_max = max |FPS - SPF|, which QuAcK takes as Conv once its cycle counter
+ * (starting at -1) exceeds 1.  The caller keeps the loop and its stopping rule
+ * (Conv > thresh and nSCF < maxSCF). */
+int qk_qsgw_cycle(qk_qsgw_handle h, double eta, int TDA_W, int doSRG, double *eGW, double *c, double *SigC, double *Z,
+                  double *E, double *EcRPA, double *EcGM, double *err_max);
+int qk_qsgw_destroy(qk_qsgw_handle h);
+const char *qk_qsgw_last_error(qk_qsgw_handle h);
+
 typedef struct qk_scf_ctx *qk_scf_handle;
 int qk_rhf_create(qk_scf_handle *h, int nBas); /* nBas <= 512 */
 int qk_rhf_destroy(qk_scf_handle h);

@@ -420,7 +420,7 @@ def rgw_srg_self_energy(flow, e, Om, rho, nO, nC=0, nR=0):
 
 
 def qsrgw(S, T, V, eri_ao, X, cHF, PHF, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, eta=0.0, doSRG=False,
-          TDA_W=False):
+          TDA_W=False, nC=0, nR=0):
     """GW/qsRGW.f90:193-352 (no linDM, no BSE): per cycle J and
     K from the AO integrals, the MO integrals of the current orbitals, dRPA
     screening at eGW, rho, the full self-energy (symmetrised, to the AO basis),
@@ -428,7 +428,9 @@ def qsrgw(S, T, V, eri_ao, X, cHF, PHF, eHF, nO, maxSCF=64, thresh=1e-5, max_dii
     starting at -1), DIIS on F, diagonalisation of X^T F X (nBas == nOrb) or
     of C^T F C (nBas > nOrb, where F is first projected on the kept orbitals,
     :281-284, :323-333).  X: nBas x nOrb.  Returns a dict with eGW, c, EqsGW
-    (electronic), EcRPA, EcGM, Z, nSCF, Conv, converged."""
+    (electronic), EcRPA, EcGM, Z, nSCF, Conv, converged, and c_last_cycle: the
+    orbitals whose MO integrals the last cycle used, which qsRGW leaves in
+    ERI_MO for the BSE after it (:210, :375)."""
     from .scf_oracle import exchange_matrix, hartree_matrix
     nB, n = X.shape
     Hc = T + V
@@ -437,19 +439,21 @@ def qsrgw(S, T, V, eri_ao, X, cHF, PHF, eHF, nO, maxSCF=64, thresh=1e-5, max_dii
     diis = DIIS(nB * nB, max(1, max_diis))
     EqsGW = EcRPA = EcGM = 0.0
     Z = np.ones(n)
+    c_prev = c
     while Conv > thresh and nSCF < maxSCF:
         nSCF += 1
+        c_prev = c
         J = hartree_matrix(P, eri_ao, nB)
         K = exchange_matrix(P, eri_ao, nB)
         mo = aotomo_eri_rhf(c, eri_ao)
-        A = phRLR_A(eGW, mo, nO, dRPA=True)
-        B = np.zeros_like(A) if TDA_W else phRLR_B(mo, n, nO, dRPA=True)
+        A = phRLR_A(eGW, mo, nO, dRPA=True, nC=nC, nR=nR)
+        B = np.zeros_like(A) if TDA_W else phRLR_B(mo, n, nO, dRPA=True, nC=nC, nR=nR)
         EcRPA, Om, XpY, _ = phRLR(TDA_W, A, B)
-        rho = excitation_density(mo, XpY, n, nO)
+        rho = excitation_density(mo, XpY, n, nO, nC, nR)
         if doSRG:
-            EcGM, SigC, Z = rgw_srg_self_energy(FLOW, eGW, Om, rho, nO)
+            EcGM, SigC, Z = rgw_srg_self_energy(FLOW, eGW, Om, rho, nO, nC, nR)
         else:
-            EcGM, SigC, Z = rgw_self_energy(eta, eGW, Om, rho, nO)
+            EcGM, SigC, Z = rgw_self_energy(eta, eGW, Om, rho, nO, nC, nR)
         SigC = 0.5 * (SigC + SigC.T)
         SigCp = S @ c @ SigC @ c.T @ S                       # AOtoMO/MOtoAO.f90
         F = Hc + J + 0.5 * K + SigCp
@@ -469,8 +473,8 @@ def qsrgw(S, T, V, eri_ao, X, cHF, PHF, eHF, nO, maxSCF=64, thresh=1e-5, max_dii
             eGW, cp = diagonalize_matrix(c.T @ F @ c)
             c = c @ cp
         P = 2.0 * c[:, :nO] @ c[:, :nO].T
-    return {"eGW": eGW, "c": c, "EqsGW": EqsGW, "EcRPA": EcRPA, "EcGM": EcGM, "Z": Z, "nSCF": nSCF, "Conv": Conv,
-            "converged": nSCF < maxSCF}
+    return {"eGW": eGW, "c": c, "c_last_cycle": c_prev, "EqsGW": EqsGW, "EcRPA": EcRPA, "EcGM": EcGM, "Z": Z,
+            "nSCF": nSCF, "Conv": Conv, "converged": nSCF < maxSCF}
 
 
 # ---- unrestricted G0W0 (GW/UG0W0.f90) ----------------------------------------

@@ -172,11 +172,11 @@ def self_energy_full(e, Om, rho, nO, eta=0.0, srg=False, nC=0, nR=0):
     return {"EcGM": Ec[0], "SigC": Sig, "Z": Z}
 
 
-def qsgw_cycle(S, T, V, X, c, P, eGW, eri_ao, nO, eta=0.0, srg=False, tda_w=False):
+def qsgw_cycle(S, T, V, X, c, P, eGW, eri_ao, nO, eta=0.0, srg=False, tda_w=False, nC=0, nR=0):
     """One cycle of GW/qsRGW.f90 through QuAcK's own routines: F and the new
     quasiparticle energies."""
     n = S.shape[0]
-    nC, nO, nV, nR, nS = _dims(n, nO)
+    nC, nO, nV, nR, nS = _dims(n, nO, nC, nR)
     F = np.zeros((n, n), order="F")
     e = np.zeros(n)
     load().qkref_qsgw_cycle(int(srg), int(tda_w), eta, n, nC, nO, nV, nR, nS, _p(_f(S)), _p(_f(T)), _p(_f(V)),

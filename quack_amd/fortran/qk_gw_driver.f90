@@ -17,7 +17,14 @@
 !                    in-process before RGW (src/QuAcK/RQuAcK.f90:252)
 !   int/ERI_MO.bin   ERI_MO(nOrb,nOrb,nOrb,nOrb), raw FP64 stream, the array
 !                    AOtoMO_ERI_RHF hands to RGW (src/QuAcK/RQuAcK.f90:459,684)
-! and writes test/Rtest.dat with QuAcK's dump format when Rtest is set.
+! and for qsGW also:
+!   int/nBas.dat, int/ERI.bin (nBas^4, utils/read_2e_integrals.f90:55-63) and
+!   int/Ov.dat, Kin.dat, Nuc.dat (utils/read_1e_integrals.f90:38-94)
+!   int/X.bin, int/cHF.bin  X and cHF (nBas x nOrb, raw FP64), which QuAcK
+!                    computes in-process before RGW
+! and writes test/Rtest.dat with QuAcK's dump format when Rtest is set.  The
+! driver holds no dipole integrals: they are zero, and so is the dipole that
+! qsGW prints.
 
 program qk_gw_driver
 
@@ -30,13 +37,16 @@ program qk_gw_driver
   logical :: lin, TDA_W, reg, TDA, singlet, triplet, dophBSE, dophBSE2, doppBSE, dBSE, dTDA
   logical :: switch_hpc, use_gpu
   double precision, allocatable :: eHF(:), ERI(:,:,:,:), dip(:,:,:), eGW(:), eU(:,:), SU(:,:), cU(:,:,:)
+  double precision, allocatable :: S(:,:), T(:,:), V(:,:), Hc(:,:), X(:,:), cHF(:,:), PHF(:,:), ERI_AO(:,:,:,:)
+  double precision, allocatable :: dipAO(:,:,:), ZNuc(:), rNuc(:,:)
+  integer :: nBas
   integer :: nOU(2), nCU(2), nVU(2), nRU(2), nSU(2)
 
   call get_command_argument(1, wd)
   if(len_trim(wd) == 0) wd = '.'
 
   ! methods
-  doG0W0 = .false.; doevGW = .false.; dotest = .false.; doRHF = .true.; doUHF = .false.
+  doG0W0 = .false.; doevGW = .false.; doqsGW = .false.; dotest = .false.; doRHF = .true.; doUHF = .false.
   open(unit=11, file=trim(wd)//'/input/methods', status='old', action='read')
   do
     read(11,'(A)',iostat=ios) line
@@ -81,7 +91,7 @@ program qk_gw_driver
   ! GPU switch and the dispatch of the patched RGW/UGW: the options the GPU
   ! drop-ins do not carry (phBSE2, ppBSE, dynamical BSE) stay on the CPU routines
   call qk_read_hpc_flags(wd, switch_hpc, use_gpu)
-  if((doG0W0 .or. doevGW) .and. .not.(use_gpu .and. .not.(dophBSE2 .or. doppBSE .or. dBSE))) then
+  if((doG0W0 .or. doevGW .or. doqsGW) .and. .not.(use_gpu .and. .not.(dophBSE2 .or. doppBSE .or. dBSE))) then
     if(.not. use_gpu) then
       write(*,*) 'qk_gw_driver: use_gpu is F in input/hpc_flags: G0W0/evGW belong to QuAcK''s CPU routines'
     else
@@ -146,7 +156,7 @@ program qk_gw_driver
     deallocate(eU, SU, cU)
   end if
   if(.not. doRHF) then
-    doG0W0 = .false.; doevGW = .false.
+    doG0W0 = .false.; doevGW = .false.; doqsGW = .false.
   end if
 
   if(doG0W0) call RG0W0_hip(dotest, .false., .false., .false., dophBSE, dophBSE2, TDA_W, TDA, dBSE, dTDA, &
@@ -156,10 +166,58 @@ program qk_gw_driver
                             TDA, dBSE, dTDA, doppBSE, singlet, triplet, lin, eta, reg, nOrb, nOrb, nC, nO, &
                             nV, nR, nS, ENuc, ERHF, ERI, dip, eHF)
 
+  if(doqsGW) then
+    ! qsGW (src/GW/RGW.f90, after G0W0 and evGW): the AO-basis inputs of qsRGW
+    open(unit=18, file=trim(wd)//'/int/nBas.dat', status='old', action='read')
+    read(18,*) nBas
+    close(18)
+    allocate(S(nBas,nBas), T(nBas,nBas), V(nBas,nBas), Hc(nBas,nBas), X(nBas,nOrb), cHF(nBas,nOrb), &
+             PHF(nBas,nBas), ERI_AO(nBas,nBas,nBas,nBas), dipAO(nBas,nBas,3), ZNuc(1), rNuc(1,3))
+    call qk_read_1e(trim(wd)//'/int/Ov.dat', nBas, S)
+    call qk_read_1e(trim(wd)//'/int/Kin.dat', nBas, T)
+    call qk_read_1e(trim(wd)//'/int/Nuc.dat', nBas, V)
+    Hc = T + V
+    open(unit=19, file=trim(wd)//'/int/ERI.bin', status='old', action='read', form='unformatted', access='stream')
+    read(19) ERI_AO
+    close(19)
+    open(unit=19, file=trim(wd)//'/int/X.bin', status='old', action='read', form='unformatted', access='stream')
+    read(19) X
+    close(19)
+    open(unit=19, file=trim(wd)//'/int/cHF.bin', status='old', action='read', form='unformatted', access='stream')
+    read(19) cHF
+    close(19)
+    PHF = 2d0*matmul(cHF(:,1:nO), transpose(cHF(:,1:nO)))
+    dipAO = 0d0; ZNuc = 0d0; rNuc = 0d0
+    call qsRGW_hip(dotest, maxSCF, thresh, max_diis, .false., .false., .false., dophBSE, dophBSE2, TDA_W, TDA, &
+                   dBSE, dTDA, doppBSE, singlet, triplet, eta, reg, .false., 0, ZNuc, rNuc, ENuc, nBas, nOrb, nC, &
+                   nO, nV, nR, nS, ERHF, S, X, T, V, Hc, ERI_AO, ERI, dipAO, dip, PHF, cHF, eHF)
+    deallocate(S, T, V, Hc, X, cHF, PHF, ERI_AO, dipAO, ZNuc, rNuc)
+  end if
+
   if(dotest) close(1231597)
   if(dotest .and. doUHF) close(1232584)
 
 end program
+
+! One symmetric one-electron matrix from PyDuck's "mu nu value" lines
+! (utils/read_1e_integrals.f90:43-48).
+subroutine qk_read_1e(path, n, M)
+  implicit none
+  character(len=*), intent(in) :: path
+  integer, intent(in) :: n
+  double precision, intent(out) :: M(n,n)
+  integer :: mu, nu, ios
+  double precision :: val
+  M = 0d0
+  open(unit=20, file=path, status='old', action='read')
+  do
+    read(20,*,iostat=ios) mu, nu, val
+    if(ios /= 0) exit
+    M(mu,nu) = val
+    M(nu,mu) = val
+  end do
+  close(20)
+end subroutine
 
 ! The reader of src/QuAcK/read_hpc_flags.f90:1-39: a comment line, the HPC
 ! switch, a comment line, the GPU switch; both F when the file is absent.

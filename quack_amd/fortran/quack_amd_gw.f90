@@ -131,6 +131,38 @@ module quack_amd_gw
       real(c_double), intent(out) :: J(*), K(*)
     end function
 
+    ! full AO->MO transformation of the integrals (AOtoMO/AOtoMO_ERI_RHF.f90)
+    integer(c_int) function qk_aotomo_eri_rhf(nBas, nOrb, c, ERI_AO, ERI_MO) bind(C, name='qk_aotomo_eri_rhf')
+      import :: c_int, c_double
+      integer(c_int), value :: nBas, nOrb
+      real(c_double), intent(in) :: c(*), ERI_AO(*)
+      real(c_double), intent(out) :: ERI_MO(*)
+    end function
+
+    ! quasiparticle self-consistent GW (GW/qsRGW.f90): one handle per calculation,
+    ! one call per pass of the loop body
+    integer(c_int) function qk_qsgw_create(h, nBas, nOrb, nC, nO, nV, nR, nS, max_diis, S, T, V, Hc, X, ERI_AO, &
+                                           PHF, cHF, eHF) bind(C, name='qk_qsgw_create')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), intent(out) :: h
+      integer(c_int), value :: nBas, nOrb, nC, nO, nV, nR, nS, max_diis
+      real(c_double), intent(in) :: S(*), T(*), V(*), Hc(*), X(*), ERI_AO(*), PHF(*), cHF(*), eHF(*)
+    end function
+
+    integer(c_int) function qk_qsgw_cycle(h, eta, TDA_W, doSRG, eGW, c, SigC, Z, E, EcRPA, EcGM, err_max) &
+                                          bind(C, name='qk_qsgw_cycle')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: eta
+      integer(c_int), value :: TDA_W, doSRG
+      real(c_double), intent(out) :: eGW(*), c(*), SigC(*), Z(*), E(5), EcRPA, EcGM, err_max
+    end function
+
+    integer(c_int) function qk_qsgw_destroy(h) bind(C, name='qk_qsgw_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
   end interface
 
 contains

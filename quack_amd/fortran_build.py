@@ -14,7 +14,7 @@ def build():
         return None
     obj = os.path.join(HERE, "_build")
     os.makedirs(obj, exist_ok=True)
-    srcs = ["quack_amd_gw.f90", "RGW_hip.f90", "UGW_hip.f90", "qk_gw_driver.f90"]
+    srcs = ["quack_amd_gw.f90", "RGW_hip.f90", "UGW_hip.f90", "qsRGW_hip.f90", "qk_gw_driver.f90"]
     objs = []
     for s in srcs:
         o = os.path.join(obj, s.replace(".f90", ".o"))

@@ -33,7 +33,8 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
            "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms",
            "qk_rgw_qp_graph_noalloc", "qk_noalloc_last_error", "qk_gw_qp_iters",
-           "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats", "qk_gwb_set_graphs", "qk_gwb_graph_stats", "qk_set_split_aotomo"]
+           "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats", "qk_gwb_set_graphs", "qk_gwb_graph_stats", "qk_set_split_aotomo",
+           "qk_qsgw_create", "qk_qsgw_cycle", "qk_qsgw_destroy", "qk_qsgw_last_error"]
 
 
 EIG_MODES = {"syevd": 0, "refine": 1, "sytrd": 2, "refine+sytrd": 3}
@@ -150,6 +151,11 @@ def load():
     lib.qk_rhf_set_eri_ao.argtypes = [_H, _P]
     lib.qk_rhf_jk.argtypes = [_H, _H, _LL, _P, _P, _P]
     lib.qk_rhf_jk_ms.argtypes = [_H, _P]
+    lib.qk_qsgw_create.argtypes = [ctypes.POINTER(_H)] + [_I] * 8 + [_P] * 9
+    lib.qk_qsgw_cycle.argtypes = [_H, _D, _I, _I] + [_P] * 8
+    lib.qk_qsgw_destroy.argtypes = [_H]
+    lib.qk_qsgw_last_error.argtypes = [_H]
+    lib.qk_qsgw_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
 

@@ -4,6 +4,11 @@ Writes the files QuAcK reads (input/methods, input/options, input/molecule,
 input/hpc_flags, int/ENuc.dat, int/nBas.dat, int/ERI.bin with the layouts of
 PyDuck.py:137-153,329-345 and input/*.default) plus the two arrays QuAcK
 computes in-process before RGW (int/MO.dat: nOrb, ERHF, eHF; int/ERI_MO.bin).
+For qsGW also the one-electron integrals int/Ov.dat, Kin.dat, Nuc.dat in
+PyDuck's "mu nu value" lines (PyDuck.py:277-284, read by
+utils/read_1e_integrals.f90), and two more in-process arrays: int/X.bin (the
+orthogonalisation matrix, nBas x nOrb) and int/cHF.bin (the RHF orbitals,
+nBas x nOrb), raw FP64 column-major.
 """
 import os
 
@@ -26,7 +31,7 @@ METHODS = """# RHF UHF GHF ROHF HFB cRHF cUHF eRHF scGHF
 # G0F2 evGF2 qsGF2 G0F3 evGF3 psdG0F3 scGF2
   F    F     F     F    F     F     F
 # G0W0 evGW qsGW scGW
-  {g0w0}    {evgw}    F    F
+  {g0w0}    {evgw}    {qsgw}    F
 # G0T0pp evGTpp qsGTpp ufG0T0pp
   F      F        F       F
 # G0T0eh evGTeh qsGTeh
@@ -72,7 +77,7 @@ def tf(b):
 
 def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=1e-5, max_diis=5, lin=False,
                  eta=0.0, TDA_W=False, reg=False, eri_mo=None, bse=False, uhf=False, nel=None, use_gpu=True,
-                 ncore=0, nryd=0):
+                 ncore=0, nryd=0, qsgw=False):
     """sysd: a dict from quack_amd.intgen.system(); eri_mo: MO integrals (nOrb^4, flat);
     bse: the phBSE flag of the "# BSE:" options line (static kernel); uhf: the
     methods file selects UHF instead of RHF (the unrestricted GW branch);
@@ -85,7 +90,8 @@ def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=
         os.makedirs(os.path.join(wd, d), exist_ok=True)
     mol = sysd["mol"]
     with open(os.path.join(wd, "input", "methods"), "w") as f:
-        f.write(METHODS.format(g0w0=tf(g0w0), evgw=tf(evgw), test=tf(test), rhf=tf(not uhf), uhf=tf(uhf)))
+        f.write(METHODS.format(g0w0=tf(g0w0), evgw=tf(evgw), qsgw=tf(qsgw), test=tf(test), rhf=tf(not uhf),
+                               uhf=tf(uhf)))
     with open(os.path.join(wd, "input", "options"), "w") as f:
         f.write(OPTIONS.format(maxscf=maxSCF, thresh="%.10g" % thresh, diis=max_diis, lin=tf(lin), eta="%.10g" % eta,
                                tda=tf(TDA_W), reg=tf(reg), bse=tf(bse)))
@@ -113,6 +119,23 @@ def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=
             f.write("%.17g\n" % e)
     if eri_mo is not None:
         np.ascontiguousarray(eri_mo, dtype=np.float64).tofile(os.path.join(wd, "int", "ERI_MO.bin"))
+    if qsgw:
+        from .scf import orthogonalization_matrix
+        S, T, V = mol.one_electron()
+        for name, M in (("Ov", S), ("Kin", T), ("Nuc", V)):
+            write_matrix(os.path.join(wd, "int", name + ".dat"), M)
+        np.asfortranarray(orthogonalization_matrix(S)).ravel(order="F").tofile(os.path.join(wd, "int", "X.bin"))
+        np.asfortranarray(sysd["c"], dtype=np.float64).ravel(order="F").tofile(os.path.join(wd, "int", "cHF.bin"))
+
+
+def write_matrix(path, M):
+    """A symmetric matrix as PyDuck's "mu nu value" lines, 1-based, mu <= nu
+    (utils/read_1e_integrals.f90:43-48 fills both triangles)."""
+    n = M.shape[0]
+    with open(path, "w") as f:
+        for nu in range(n):
+            for mu in range(nu + 1):
+                f.write("%d %d %.17g\n" % (mu + 1, nu + 1, M[mu, nu]))
 
 
 def s8_pairs(nBas):

@@ -169,6 +169,68 @@ def test_driver_frozen_core(tmp_path, mo_systems):
             assert abs(d[k0 + k][1] - ref[k]) < 1e-8, (k0, k, d[k0 + k], ref[k])
 
 
+@pytest.mark.gpu
+def test_driver_qsgw(tmp_path, mo_systems):
+    """qsRGW_hip through the driver (qsGW switched on in input/methods, SRG,
+    with static phBSE after it): the dumped qsGW values and the Tr@BSE@qsGW
+    lines match the CPU restatement of qsRGW and RGW_phBSE (the BSE on the MO
+    integrals of the last cycle's orbitals, as qsRGW leaves them)."""
+    import numpy as np
+
+    from oracle import gw_oracle as go
+    from quack_amd import scf
+    print("fortran driver qsGW H2O", flush=True)
+    s = mo_systems["H2O"]
+    nO, mo = s["nO"], s["eri_mo"]
+    quackio.write_inputs(str(tmp_path), s, g0w0=False, evgw=False, qsgw=True, maxSCF=64, thresh=1e-5, max_diis=5,
+                         reg=True, eri_mo=mo, bse=True)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "qsGW@RHF HOMO      energy" in r.stdout and "qsRGW        energy" in r.stdout
+    S, T, V = s["mol"].one_electron()
+    X = scf.orthogonalization_matrix(S)
+    c = np.asarray(s["c"])
+    o = go.qsrgw(S, T, V, s["eri"], X, c, 2.0 * c[:, :nO] @ c[:, :nO].T, s["eHF"], nO, doSRG=True)
+    assert o["converged"]
+    dump = quackio.read_test_dump(str(tmp_path / "test" / "Rtest.dat"))
+    assert abs(dump["qsGW correlation energy"] - o["EcRPA"]) < 1e-8
+    assert abs(dump["qsGW HOMO energy"] - o["eGW"][nO - 1]) < 1e-8
+    assert abs(dump["qsGW LUMO energy"] - o["eGW"][nO]) < 1e-8
+    vals = {}
+    for ln in r.stdout.splitlines():
+        for man in ("singlet", "triplet"):
+            key = "Tr@BSE@qsGW@RHF correlation energy (%s) =" % man
+            if key in ln:
+                vals[man] = float(ln.split("=")[1].split()[0])
+    mo_last = go.aotomo_eri_rhf(o["c_last_cycle"], s["eri"])
+    Ec, _, _ = go.rgw_phbse(mo_last, o["eGW"], o["eGW"], nO)
+    assert abs(vals["singlet"] - Ec[0]) < 1e-8 and abs(vals["triplet"] - Ec[1]) < 1e-8
+
+
+def test_input_files_qsgw(tmp_path, systems):
+    """The qsGW inputs: the methods switch, PyDuck's "mu nu value" lines of the
+    one-electron integrals (read back as QuAcK's read_1e_integrals does) and
+    the X and cHF arrays."""
+    import numpy as np
+    s = systems["H2O"]
+    quackio.write_inputs(str(tmp_path), s, g0w0=False, qsgw=True)
+    m = open(tmp_path / "input" / "methods").read().splitlines()
+    k = [i for i, ln in enumerate(m) if ln.startswith("# G0W0")][0]
+    assert m[k + 1].split()[:3] == ["F", "F", "T"]
+    S, T, V = s["mol"].one_electron()
+    n = S.shape[0]
+    for name, M in (("Ov", S), ("Kin", T), ("Nuc", V)):
+        R = np.zeros((n, n))
+        for ln in open(tmp_path / "int" / (name + ".dat")):
+            mu, nu, v = ln.split()
+            R[int(mu) - 1, int(nu) - 1] = R[int(nu) - 1, int(mu) - 1] = float(v)
+        U = np.triu(M)
+        assert np.array_equal(R, U + np.triu(M, 1).T), name  # the upper triangle, exactly
+        assert np.abs(R - M).max() < 1e-14, name
+    c = np.fromfile(tmp_path / "int" / "cHF.bin").reshape(s["c"].shape, order="F")
+    assert np.array_equal(c, np.asarray(s["c"]))
+
+
 def test_driver_odd_core_electrons_rejected(tmp_path, systems):
     """An odd number of core electrons stops the driver before any device
     call, as QuAcK's read_molecule does."""

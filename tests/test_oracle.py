@@ -375,7 +375,7 @@ def test_full_self_energy_vs_compiled_reference(mo_systems, name):
 def test_qsgw_cycle_vs_compiled_reference(systems, name):
     """The first qsGW cycle of the restatement (J, K, MO integrals, screening,
     full self-energy, MOtoAO, F, diagonalisation) against the same cycle
-    composed of QuAcK's own routines."""
+    composed of QuAcK's own routines, with and without frozen orbitals."""
     s = systems[name]
     mol = s["mol"]
     S, T, V = mol.one_electron()
@@ -383,11 +383,15 @@ def test_qsgw_cycle_vs_compiled_reference(systems, name):
     X = U / np.sqrt(s_)
     nO, c = s["nO"], np.asarray(s["c"])
     P = 2.0 * c[:, :nO] @ c[:, :nO].T
-    for srg, eta, tda_w in ((False, 0.0, False), (True, 0.0, False), (False, 0.05, True)):
-        ref = qkref.qsgw_cycle(S, T, V, X, c, P, s["eHF"], s["eri"], nO, eta=eta, srg=srg, tda_w=tda_w)
-        r = go.qsrgw(S, T, V, s["eri"], X, c, P, s["eHF"], nO, maxSCF=0, eta=eta, doSRG=srg, TDA_W=tda_w)
-        assert r["nSCF"] == 0 and not r["converged"]
-        assert np.abs(r["eGW"] - ref["eGW"]).max() < 1e-10, (name, srg, eta, tda_w)
+    fz = [(0, 0)] if name == "He" else [(0, 0), (1, 2)]
+    for (nC, nR) in fz:
+        for srg, eta, tda_w in ((False, 0.0, False), (True, 0.0, False), (False, 0.05, True)):
+            ref = qkref.qsgw_cycle(S, T, V, X, c, P, s["eHF"], s["eri"], nO, eta=eta, srg=srg, tda_w=tda_w, nC=nC,
+                                   nR=nR)
+            r = go.qsrgw(S, T, V, s["eri"], X, c, P, s["eHF"], nO, maxSCF=0, eta=eta, doSRG=srg, TDA_W=tda_w,
+                         nC=nC, nR=nR)
+            assert r["nSCF"] == 0 and not r["converged"]
+            assert np.abs(r["eGW"] - ref["eGW"]).max() < 1e-10, (name, nC, nR, srg, eta, tda_w)
 
 
 @pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
