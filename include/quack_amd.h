@@ -285,13 +285,32 @@ int qk_unrestricted_eig_stats(long long *counts, int n);
  * plus the kernels, each solved by phRLR (TDA for the BSE).  ERI: nOrb^4 MO
  * integrals as RG0W0 receives them.  Outputs: EcBSE[2] (triplet x 3 as at
  * :193, both halved with exchange_kernel), OmBSE[2 nS] (singlet, then
- * triplet; zeros for a manifold not asked).  dophBSE2 or dBSE:
- * QK_E_UNSUPPORTED (the second-order kernel and the dynamical correction are
- * not built); dTDA is read only by the latter.  Oscillator strengths
- * (phLR_transition_vectors) are not computed. */
+ * triplet; zeros for a manifold not asked).  dophBSE2: the second-order
+ * static kernels (GW/RGW_phBSE_static_kernel.f90 for W, FP64 matrix-core
+ * product; GW/RGW_phBSE2_static_kernel_A.f90, _B.f90) are added to the
+ * first-order ones before the singlet BSE, and the triplet BSE after it uses
+ * the kernels so changed (QuAcK's BSE2 routines accumulate into the array
+ * they receive); W costs 2 nOrb^4 doubles on the device and the host.  dBSE
+ * needs qk_rgw_phbse_dyn, which returns the corrections (QK_E_ARG here).
+ * dophBSE2 needs nC = nR = 0 (QuAcK's BSE2 and dynamical kernels index pairs
+ * as (a-nO) + (i-1)(nBas-nO)): QK_E_UNSUPPORTED otherwise.  Oscillator
+ * strengths (phLR_transition_vectors) are not computed. */
 int qk_rgw_phbse(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE, int dTDA, int singlet, int triplet,
                  double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double *ERI, const double *eW,
                  const double *eGW, double *EcBSE, double *OmBSE);
+/* qk_rgw_phbse with the first-order dynamical correction when dBSE != 0
+ * (GW/RGW_phBSE_dynamic_perturbation.f90:73-122, nC = nR = 0): for the first
+ * min(nS, 10) excitations of each manifold asked, the dynamic kernels at the
+ * excitation energy (GW/RGW_phBSE_dynamic_kernel_A.f90, _B.f90; with
+ * dophBSE2 also GW/RGW_phBSE2_dynamic_kernel_A.f90, _B.f90) contracted with
+ * the BSE vectors on the device; dTDA keeps the resonant part only.
+ * OmDyn[20], ZDyn[20]: the renormalised corrections Z (X^T (K(Om) - K_sta) X
+ * ...) and the factors Z = 1/(1 - X^T ZA X ...), singlet at 0..9, triplet at
+ * 10..19 (zeros beyond nS and for a manifold not asked); QuAcK prints
+ * OmBSE + OmDyn and Z. */
+int qk_rgw_phbse_dyn(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE, int dTDA, int singlet,
+                     int triplet, double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double *ERI,
+                     const double *eW, const double *eGW, double *EcBSE, double *OmBSE, double *OmDyn, double *ZDyn);
 
 const char *qk_last_error(void);
 

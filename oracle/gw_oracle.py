@@ -345,35 +345,219 @@ def phbse_static_kernel_B(eta, Om, rho, nO, lam=1.0, nC=0, nR=0):
     return 4.0 * lam ** 2 * K.reshape(nS, nS)
 
 
+def _fw(x, eta):
+    """x/(x^2 + eta^2), the regularised 1/x of the BSE kernels."""
+    return x / (x ** 2 + eta ** 2)
+
+
+def _gw(x, eta):
+    """(x^2 - eta^2)/(x^2 + eta^2)^2, minus its derivative."""
+    return (x ** 2 - eta ** 2) / (x ** 2 + eta ** 2) ** 2
+
+
+def phbse_static_screening(eta, eri, Om, rho, lam=1.0):
+    """GW/RGW_phBSE_static_kernel.f90:40-60: the static screened interaction
+    W(p,s,q,r) = -lam ERI(p,s,q,r) + 4 lam sum_m rho(p,q,m) rho(r,s,m) Om_m/(Om_m^2+eta^2),
+    returned as W[p, s, q, r] (the 4-index array QuAcK holds)."""
+    n = rho.shape[0]
+    chi = np.einsum("pqm,rsm,m->pqrs", rho, rho, Om / (Om ** 2 + eta ** 2), optimize=True)
+    return -lam * G4(eri, n) + 4.0 * lam * chi.transpose(0, 3, 1, 2)
+
+
+def _bse2_blocks(W, nO):
+    n = W.shape[0]
+    o, v = slice(0, nO), slice(nO, n)
+    return o, v, n - nO
+
+
+def phbse2_static_kernel_A(eta, eW, W, nO):
+    """GW/RGW_phBSE2_static_kernel_A.f90:36-90 (nC = nR = 0): the second-order
+    static kernel of the resonant block, from W and the energies eW."""
+    o, v, nv = _bse2_blocks(W, nO)
+    eo, ev = eW[o], eW[v]
+    F1 = _fw(-(ev[None, :] - eo[:, None]), eta)            # [k, c]
+    F2 = _fw(+(ev[None, :] - eo[:, None]), eta)
+    Fvv = _fw(-(ev[:, None] + ev[None, :]), eta)            # [c, d]
+    Foo = _fw(-(eo[:, None] + eo[None, :]), eta)            # [k, l]
+    K = -2.0 * np.einsum("jkic,acbk,kc->iajb", W[o, o, o, v], W[v, v, v, o], F1, optimize=True)
+    K += 2.0 * np.einsum("jcik,akbc,kc->iajb", W[o, v, o, o], W[v, o, v, v], F2, optimize=True)
+    K += 2.0 * np.einsum("ajcd,cdib,cd->iajb", W[v, o, v, v], W[v, v, o, v], Fvv, optimize=True)
+    K -= 2.0 * np.einsum("ajkl,klib,kl->iajb", W[v, o, o, o], W[o, o, o, v], Foo, optimize=True)
+    nS = nO * nv
+    return K.reshape(nS, nS)
+
+
+def phbse2_static_kernel_B(eta, eW, W, nO):
+    """GW/RGW_phBSE2_static_kernel_B.f90:36-90 (nC = nR = 0): the
+    second-order static kernel of the coupling block."""
+    o, v, nv = _bse2_blocks(W, nO)
+    eo, ev = eW[o], eW[v]
+    F1 = _fw(-(ev[None, :] - eo[:, None]), eta)
+    F2 = _fw(+(ev[None, :] - eo[:, None]), eta)
+    Fvv = _fw(-(ev[:, None] + ev[None, :]), eta)
+    Foo = _fw(-(eo[:, None] + eo[None, :]), eta)
+    K = -2.0 * np.einsum("bkic,acjk,kc->iajb", W[v, o, o, v], W[v, v, o, o], F1, optimize=True)
+    K += 2.0 * np.einsum("bcik,akjc,kc->iajb", W[v, v, o, o], W[v, o, o, v], F2, optimize=True)
+    K += 2.0 * np.einsum("abcd,cdij,cd->iajb", W[v, v, v, v], W[v, v, o, o], Fvv, optimize=True)
+    K -= 2.0 * np.einsum("abkl,klij,kl->iajb", W[v, v, o, o], W[o, o, o, o], Foo, optimize=True)
+    nS = nO * nv
+    return K.reshape(nS, nS)
+
+
+def phbse_dynamic_kernel_A(eta, eGW, OmRPA, rho, nO, Om):
+    """GW/RGW_phBSE_dynamic_kernel_A.f90:38-82 (nC = nR = 0, lambda 1) at the
+    excitation energy Om: KA(ia,jb) = -2 sum_m rho_ij rho_ab [f(Om - Om_m -
+    (e_a - e_j)) + f(Om - Om_m - (e_b - e_i))], ZA with g in place of f and +2."""
+    n = rho.shape[0]
+    o, v = slice(0, nO), slice(nO, n)
+    roo, rvv = rho[o, o, :], rho[v, v, :]
+    D1 = Om - OmRPA[None, None, :] - (eGW[v][:, None, None] - eGW[o][None, :, None])   # [a, j, m]
+    D2 = Om - OmRPA[None, None, :] - (eGW[v][:, None, None] - eGW[o][None, :, None])   # [b, i, m]
+    KA = -2.0 * (np.einsum("ijm,abm,ajm->iajb", roo, rvv, _fw(D1, eta), optimize=True)
+                 + np.einsum("ijm,abm,bim->iajb", roo, rvv, _fw(D2, eta), optimize=True))
+    ZA = 2.0 * (np.einsum("ijm,abm,ajm->iajb", roo, rvv, _gw(D1, eta), optimize=True)
+                + np.einsum("ijm,abm,bim->iajb", roo, rvv, _gw(D2, eta), optimize=True))
+    nS = nO * (n - nO)
+    return KA.reshape(nS, nS), ZA.reshape(nS, nS)
+
+
+def phbse_dynamic_kernel_B(eta, eGW, OmRPA, rho, nO):
+    """GW/RGW_phBSE_dynamic_kernel_B.f90:34-62 (nC = nR = 0, lambda 1):
+    KB(ia,jb) = -2 sum_m rho_ib rho_ja [f(-Om_m - (e_b - e_i)) + f(-Om_m - (e_a - e_j))]."""
+    n = rho.shape[0]
+    o, v = slice(0, nO), slice(nO, n)
+    rov = rho[o, v, :]
+    E = -OmRPA[None, None, :] - (eGW[v][:, None, None] - eGW[o][None, :, None])   # [b, i, m] / [a, j, m]
+    KB = -2.0 * (np.einsum("ibm,jam,bim->iajb", rov, rov, _fw(E, eta), optimize=True)
+                 + np.einsum("ibm,jam,ajm->iajb", rov, rov, _fw(E, eta), optimize=True))
+    nS = nO * (n - nO)
+    return KB.reshape(nS, nS)
+
+
+def phbse2_dynamic_kernel_A(eta, eGW, W, nO, Om):
+    """GW/RGW_phBSE2_dynamic_kernel_A.f90:36-90 (nC = nR = 0) at Om: the
+    second-order dynamic kernel and its derivative term."""
+    o, v, nv = _bse2_blocks(W, nO)
+    eo, ev = eGW[o], eGW[v]
+    # [a, c, k, j]: Om - e_a - e_c + e_k + e_j
+    d1 = Om - ev[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] + eo[None, None, None, :]
+    # [i, c, k, b]: Om + e_i - e_c + e_k - e_b
+    d2 = Om + eo[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] - ev[None, None, None, :]
+    # [i, j, c, d]: Om + e_i + e_j - e_c - e_d
+    d3 = Om + eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] - ev[None, None, None, :]
+    # [a, b, k, l]: Om - e_a - e_b + e_k + e_l
+    d4 = Om - ev[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] + eo[None, None, None, :]
+    KA = np.zeros((nO, nv, nO, nv))
+    ZA = np.zeros((nO, nv, nO, nv))
+    t1 = ("jkic,acbk,ackj->iajb", W[o, o, o, v], W[v, v, v, o], d1)
+    t2 = ("jcik,akbc,ickb->iajb", W[o, v, o, o], W[v, o, v, v], d2)
+    t3 = ("ajcd,cdib,ijcd->iajb", W[v, o, v, v], W[v, v, o, v], d3)
+    t4 = ("ajkl,klib,abkl->iajb", W[v, o, o, o], W[o, o, o, v], d4)
+    for (eq, A1, A2, d), sg in ((t1, -1.0), (t2, -1.0), (t3, 1.0), (t4, 1.0)):
+        KA += sg * 2.0 * np.einsum(eq, A1, A2, _fw(d, eta), optimize=True)
+        ZA -= sg * 2.0 * np.einsum(eq, A1, A2, _gw(d, eta), optimize=True)
+    nS = nO * nv
+    return KA.reshape(nS, nS), ZA.reshape(nS, nS)
+
+
+def phbse2_dynamic_kernel_B(eta, eGW, W, nO):
+    """GW/RGW_phBSE2_dynamic_kernel_B.f90:36-84 (nC = nR = 0)."""
+    o, v, nv = _bse2_blocks(W, nO)
+    eo, ev = eGW[o], eGW[v]
+    d1 = -ev[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] + eo[None, None, None, :]  # [a,c,k,j]
+    d2 = eo[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] - ev[None, None, None, :]   # [i,c,k,b]
+    d3 = eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] - ev[None, None, None, :]   # [i,j,c,d]
+    d4 = -ev[:, None, None, None] - ev[None, :, None, None] + eo[None, None, :, None] + eo[None, None, None, :]  # [a,b,k,l]
+    KB = -2.0 * np.einsum("bkic,acjk,ackj->iajb", W[v, o, o, v], W[v, v, o, o], _fw(d1, eta), optimize=True)
+    KB -= 2.0 * np.einsum("bcik,akjc,ickb->iajb", W[v, v, o, o], W[v, o, o, v], _fw(d2, eta), optimize=True)
+    KB += 2.0 * np.einsum("abcd,cdij,ijcd->iajb", W[v, v, v, v], W[v, v, o, o], _fw(d3, eta), optimize=True)
+    KB += 2.0 * np.einsum("abkl,klij,abkl->iajb", W[v, v, o, o], W[o, o, o, o], _fw(d4, eta), optimize=True)
+    nS = nO * nv
+    return KB.reshape(nS, nS)
+
+
+DYN_MAXS = 10  # GW/RGW_phBSE_dynamic_perturbation.f90:36
+
+
+def phbse_dynamic_perturbation(dophBSE2, dTDA, eta, eGW, OmRPA, rho, nO, OmBSE, XpY, XmY, KA_sta, KB_sta, W=None):
+    """GW/RGW_phBSE_dynamic_perturbation.f90:73-122: first-order dynamical
+    correction of the first min(nS, 10) BSE excitations.  Returns (Om_dyn,
+    Z_dyn): the renormalised corrections Z (X^T (K(Om) - K_sta) X ...) and Z."""
+    nS = len(OmBSE)
+    m = min(nS, DYN_MAXS)
+    Om_dyn, Z_dyn = np.zeros(m), np.zeros(m)
+    for ia in range(m):
+        X = 0.5 * (XpY[ia, :] + XmY[ia, :])
+        Y = 0.5 * (XpY[ia, :] - XmY[ia, :])
+        KAp, ZAp = phbse_dynamic_kernel_A(eta, eGW, OmRPA, rho, nO, +OmBSE[ia])
+        if dophBSE2:
+            K2, Z2 = phbse2_dynamic_kernel_A(eta, eGW, W, nO, OmBSE[ia])
+            KAp, ZAp = KAp + K2, ZAp + Z2
+        if dTDA:
+            z = X @ ZAp @ X
+            om = X @ (KAp - KA_sta) @ X
+        else:
+            KAm, ZAm = phbse_dynamic_kernel_A(eta, eGW, OmRPA, rho, nO, -OmBSE[ia])
+            KB = phbse_dynamic_kernel_B(eta, eGW, OmRPA, rho, nO)
+            if dophBSE2:
+                KB = KB + phbse2_dynamic_kernel_B(eta, eGW, W, nO)
+            z = X @ ZAp @ X + Y @ ZAm @ Y
+            om = (X @ (KAp - KA_sta) @ X - Y @ (KAm - KA_sta) @ Y + X @ (KB - KB_sta) @ Y
+                  - Y @ (KB - KB_sta) @ X)
+        Z_dyn[ia] = 1.0 / (1.0 - z)
+        Om_dyn[ia] = Z_dyn[ia] * om
+    return Om_dyn, Z_dyn
+
+
 def rgw_phbse(eri, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, singlet=True, triplet=True,
-              exchange_kernel=False, nC=0, nR=0):
-    """GW/RGW_phBSE.f90:100-195 with the static kernel (no phBSE2, no dynamical
-    correction): dRPA screening at eW, rho, KA/KB, then the singlet and
-    triplet BSE matrices at eGW (phRLR_A/B with dRPA = false) plus the
-    kernels, each solved by phRLR.  Returns EcBSE(2) (triplet x 3, halved with
+              exchange_kernel=False, nC=0, nR=0, dophBSE2=False, dBSE=False, dTDA=True, dyn=None):
+    """GW/RGW_phBSE.f90:100-195: dRPA screening at eW, rho, the static kernels
+    KA/KB, then the singlet and triplet BSE matrices at eGW (phRLR_A/B with
+    dRPA = false) plus the kernels, each solved by phRLR.  dophBSE2 (singlet
+    block, :123-138): the second-order static kernels from W are added to
+    KA/KB (QuAcK's BSE2 routines accumulate into their output array, which
+    holds the first-order kernel), and the triplet block after it uses the
+    kernels so changed.  dBSE (:150-152, :181-183): the first-order dynamical
+    correction of each manifold (dTDA: its resonant part only), returned in
+    dyn["singlet"/"triplet"] = (Om_dyn, Z_dyn) when a dict is passed.
+    BSE2 and dBSE follow QuAcK's pair index (a-nO) + (i-1)(nBas-nO), valid for
+    nC = nR = 0 only.  Returns EcBSE(2) (triplet x 3, halved with
     exchange_kernel) and the singlet and triplet excitation energies."""
     n = len(eW)
     nS = ph_dims(n, nO, nC, nR)[4]
+    if (dophBSE2 or dBSE) and (nC or nR):
+        raise NotImplementedError("phBSE2 and dBSE index pairs as nC = nR = 0")
     A = phRLR_A(eW, eri, nO, dRPA=True, nC=nC, nR=nR)
     B = np.zeros((nS, nS)) if TDA_W else phRLR_B(eri, n, nO, dRPA=True, nC=nC, nR=nR)
     _, OmRPA, XpY, _ = phRLR(TDA_W, A, B)
     rho = excitation_density(eri, XpY, n, nO, nC, nR)
     KA = phbse_static_kernel_A(eta, OmRPA, rho, nO, nC=nC, nR=nR)
     KB = np.zeros((nS, nS)) if TDA else phbse_static_kernel_B(eta, OmRPA, rho, nO, nC=nC, nR=nR)
+    W = phbse_static_screening(eta, eri, OmRPA, rho) if dophBSE2 else None
     EcBSE = np.zeros(2)
     Om = {1: np.zeros(nS), 2: np.zeros(nS)}
     for ispin, on in ((1, singlet), (2, triplet)):
         if not on:
             continue
-        A = phRLR_A(eGW, eri, nO, dRPA=False, ispin=ispin, nC=nC, nR=nR) + KA
+        A = phRLR_A(eGW, eri, nO, dRPA=False, ispin=ispin, nC=nC, nR=nR)
         B = (np.zeros((nS, nS)) if TDA
-             else phRLR_B(eri, n, nO, dRPA=False, ispin=ispin, nC=nC, nR=nR) + KB)
-        EcBSE[ispin - 1], Om[ispin], _, _ = phRLR(TDA, A, B)
+             else phRLR_B(eri, n, nO, dRPA=False, ispin=ispin, nC=nC, nR=nR))
+        if ispin == 1 and dophBSE2:
+            KA = KA + phbse2_static_kernel_A(eta, eW, W, nO)
+            if not TDA:
+                KB = KB + phbse2_static_kernel_B(eta, eW, W, nO)
+        A = A + KA
+        if not TDA:
+            B = B + KB
+        EcBSE[ispin - 1], Om[ispin], Xp, Xm = phRLR(TDA, A, B)
+        if dBSE:
+            r = phbse_dynamic_perturbation(dophBSE2, dTDA, eta, eGW, OmRPA, rho, nO, Om[ispin], Xp, Xm, KA, KB, W)
+            if dyn is not None:
+                dyn["singlet" if ispin == 1 else "triplet"] = r
     EcBSE[1] *= 3.0
     if exchange_kernel:
         EcBSE *= 0.5
     return EcBSE, Om[1], Om[2]
-
 
 
 def rgw_self_energy(eta, e, Om, rho, nO, nC=0, nR=0):

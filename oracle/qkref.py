@@ -43,6 +43,7 @@ def load():
         lib.qkref_evrgw.argtypes = [_I, _I, _I, _I, _D, _I, _D] + [_I] * 6 + [_D, _D] + [_P] * 7 + [_IP]
         lib.qkref_evrgw.restype = _I
         lib.qkref_phbse.argtypes = [_I] * 4 + [_D] + [_I] * 6 + [_P] * 6
+        lib.qkref_phbse_full.argtypes = [_I] * 7 + [_D] + [_I] * 6 + [_P] * 8
         lib.qkref_self_energy_full.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
         lib.qkref_qsgw_cycle.argtypes = [_I, _I, _D] + [_I] * 6 + [_P] * 10
         lib.qkref_ug0w0.argtypes = [_I, _I, _I, _D, _I, _I, _I] + [_I] * 4 + [_P] * 9
@@ -158,6 +159,25 @@ def phbse(eri_mo, eW, eGW, nO, eta=0.0, tda_w=False, tda=False, singlet=True, tr
     load().qkref_phbse(int(tda_w), int(tda), int(singlet), int(triplet), eta, nOrb, nC, nO, nV, nR, nS,
                        _p(_eri(eri_mo)), _p(_f(eW)), _p(_f(eGW)), _p(Ec), _p(OmS), _p(OmT))
     return {"EcBSE": Ec, "OmS": OmS, "OmT": OmT}
+
+
+def phbse_full(eri_mo, eW, eGW, nO, eta=0.0, tda_w=False, tda=False, singlet=True, triplet=True, bse2=False,
+               dbse=False, dtda=True):
+    """phBSE@GW with QuAcK's second-order kernel and first-order dynamical
+    correction (ref_driver.f90 qkref_phbse_full).  Returns EcBSE, OmS, OmT
+    and the corrections OmDyn/ZDyn of the first min(nS, 10) states of each
+    manifold (singlet, triplet)."""
+    nOrb = len(eW)
+    nC, nO, nV, nR, nS = _dims(nOrb, nO)
+    Ec = np.zeros(2)
+    OmS, OmT = np.zeros(nS), np.zeros(nS)
+    Od, Zd = np.zeros(20), np.zeros(20)
+    load().qkref_phbse_full(int(tda_w), int(tda), int(singlet), int(triplet), int(bse2), int(dbse), int(dtda), eta,
+                            nOrb, nC, nO, nV, nR, nS, _p(_eri(eri_mo)), _p(_f(eW)), _p(_f(eGW)), _p(Ec), _p(OmS),
+                            _p(OmT), _p(Od), _p(Zd))
+    m = min(nS, 10)
+    return {"EcBSE": Ec, "OmS": OmS, "OmT": OmT, "dyn": {"singlet": (Od[:m], Zd[:m]), "triplet": (Od[10:10 + m],
+                                                                                                 Zd[10:10 + m])}}
 
 
 def self_energy_full(e, Om, rho, nO, eta=0.0, srg=False, nC=0, nR=0):

@@ -191,6 +191,100 @@ end function
 ! correction, no transition vectors), composed of QuAcK's own routines in the
 ! same order: dRPA screening at eW, excitation densities, static kernels, then
 ! the singlet (ispin 1) and triplet (ispin 2) BSE at eGW with dRPA = false.
+! GW/RGW_phBSE.f90:100-195 with the second-order kernel (dophBSE2) and the
+! first-order dynamical correction (dBSE, dTDA), composed of QuAcK's own
+! routines in RGW_phBSE's order: the BSE2 kernels are called on KA/KB, which
+! hold the static GW kernels (as RGW_phBSE passes KA_sta/KB_sta), and the
+! correction repeats the arithmetic of GW/RGW_phBSE_dynamic_perturbation.f90:73-122
+! (which only prints) on QuAcK's dynamic kernels.  OmDyn, ZDyn: the first
+! min(nS, 10) corrections of the singlet (1:10) and triplet (11:20) manifolds.
+subroutine qkref_phbse_full(tda_w, tda, dosing, dotrip, bse2, dbse, dtda, eta, nOrb, nC, nO, nV, nR, nS, ERI, eW, &
+                            eGW, EcBSE, OmS, OmT, OmDyn, ZDyn) bind(C, name='qkref_phbse_full')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: tda_w, tda, dosing, dotrip, bse2, dbse, dtda, nOrb, nC, nO, nV, nR, nS
+  real(c_double), value :: eta
+  real(c_double), intent(in) :: ERI(nOrb, nOrb, nOrb, nOrb), eW(nOrb), eGW(nOrb)
+  real(c_double), intent(out) :: EcBSE(2), OmS(nS), OmT(nS), OmDyn(20), ZDyn(20)
+  double precision, allocatable :: Aph(:,:), Bph(:,:), KA(:,:), KB(:,:), OmRPA(:), XpY(:,:), XmY(:,:), rho(:,:,:)
+  double precision, allocatable :: W(:,:,:,:)
+  double precision :: EcRPA
+  logical :: TDAW, TDAB
+  integer :: ispin
+  TDAW = tda_w /= 0
+  TDAB = tda /= 0
+  allocate(Aph(nS, nS), Bph(nS, nS), KA(nS, nS), KB(nS, nS), OmRPA(nS), XpY(nS, nS), XmY(nS, nS), &
+           rho(nOrb, nOrb, nS), W(nOrb, nOrb, nOrb, nOrb))
+  Aph = 0d0; Bph = 0d0; KA = 0d0; KB = 0d0
+  EcBSE = 0d0; OmS = 0d0; OmT = 0d0; OmDyn = 0d0; ZDyn = 0d0
+  call phRLR_A(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, eW, ERI, Aph)
+  if(.not. TDAW) call phRLR_B(1, .true., nOrb, nC, nO, nV, nR, nS, 1d0, ERI, Bph)
+  call phRLR(TDAW, nS, Aph, Bph, EcRPA, OmRPA, XpY, XmY)
+  call RGW_excitation_density(nOrb, nC, nO, nR, nS, ERI, XpY, rho)
+  call RGW_phBSE_static_kernel_A(eta, nOrb, nC, nO, nV, nR, nS, 1d0, ERI, OmRPA, rho, KA)
+  if(.not. TDAB) call RGW_phBSE_static_kernel_B(eta, nOrb, nC, nO, nV, nR, nS, 1d0, ERI, OmRPA, rho, KB)
+  if(bse2 /= 0) call RGW_phBSE_static_kernel(eta, nOrb, nC, nO, nV, nR, nS, 1d0, ERI, OmRPA, rho, W)
+  do ispin = 1, 2
+    if(ispin == 1 .and. dosing == 0) cycle
+    if(ispin == 2 .and. dotrip == 0) cycle
+    Bph = 0d0
+    call phRLR_A(ispin, .false., nOrb, nC, nO, nV, nR, nS, 1d0, eGW, ERI, Aph)
+    if(.not. TDAB) call phRLR_B(ispin, .false., nOrb, nC, nO, nV, nR, nS, 1d0, ERI, Bph)
+    if(ispin == 1 .and. bse2 /= 0) then
+      call RGW_phBSE2_static_kernel_A(eta, nOrb, nC, nO, nV, nR, nS, 1d0, eW, W, KA)
+      if(.not. TDAB) call RGW_phBSE2_static_kernel_B(eta, nOrb, nC, nO, nV, nR, nS, 1d0, eW, W, KB)
+    end if
+    Aph = Aph + KA
+    if(.not. TDAB) Bph = Bph + KB
+    if(ispin == 1) then
+      call phRLR(TDAB, nS, Aph, Bph, EcBSE(1), OmS, XpY, XmY)
+      if(dbse /= 0) call qkref_dyn(bse2 /= 0, dtda /= 0, eta, nOrb, nC, nO, nV, nR, nS, eGW, OmRPA, rho, W, OmS, &
+                                   XpY, XmY, KA, KB, OmDyn(1:10), ZDyn(1:10))
+    else
+      call phRLR(TDAB, nS, Aph, Bph, EcBSE(2), OmT, XpY, XmY)
+      if(dbse /= 0) call qkref_dyn(bse2 /= 0, dtda /= 0, eta, nOrb, nC, nO, nV, nR, nS, eGW, OmRPA, rho, W, OmT, &
+                                   XpY, XmY, KA, KB, OmDyn(11:20), ZDyn(11:20))
+    end if
+  end do
+  EcBSE(2) = 3d0 * EcBSE(2)
+  flush(6)
+  deallocate(Aph, Bph, KA, KB, OmRPA, XpY, XmY, rho, W)
+end subroutine
+
+subroutine qkref_dyn(bse2, dtda, eta, nOrb, nC, nO, nV, nR, nS, eGW, OmRPA, rho, W, OmBSE, XpY, XmY, KA_sta, &
+                     KB_sta, OmDyn, ZDyn)
+  implicit none
+  logical, intent(in) :: bse2, dtda
+  integer, intent(in) :: nOrb, nC, nO, nV, nR, nS
+  double precision, intent(in) :: eta, eGW(nOrb), OmRPA(nS), rho(nOrb,nOrb,nS), W(nOrb,nOrb,nOrb,nOrb)
+  double precision, intent(in) :: OmBSE(nS), XpY(nS,nS), XmY(nS,nS), KA_sta(nS,nS), KB_sta(nS,nS)
+  double precision, intent(out) :: OmDyn(10), ZDyn(10)
+  double precision, allocatable :: X(:), Y(:), KAp(:,:), ZAp(:,:), KAm(:,:), ZAm(:,:), KBd(:,:)
+  integer :: ia
+  allocate(X(nS), Y(nS), KAp(nS,nS), ZAp(nS,nS), KAm(nS,nS), ZAm(nS,nS), KBd(nS,nS))
+  OmDyn = 0d0; ZDyn = 0d0
+  do ia = 1, min(nS, 10)
+    X(:) = 0.5d0*(XpY(ia,:) + XmY(ia,:))
+    Y(:) = 0.5d0*(XpY(ia,:) - XmY(ia,:))
+    call RGW_phBSE_dynamic_kernel_A(eta, nOrb, nC, nO, nV, nR, nS, 1d0, eGW, OmRPA, rho, +OmBSE(ia), KAp, ZAp)
+    if(bse2) call RGW_phBSE2_dynamic_kernel_A(eta, nOrb, nC, nO, nV, nR, nS, eGW, W, OmBSE(ia), KAp, ZAp)
+    if(dtda) then
+      ZDyn(ia)  = dot_product(X, matmul(ZAp, X))
+      OmDyn(ia) = dot_product(X, matmul(KAp - KA_sta, X))
+    else
+      call RGW_phBSE_dynamic_kernel_A(eta, nOrb, nC, nO, nV, nR, nS, 1d0, eGW, OmRPA, rho, -OmBSE(ia), KAm, ZAm)
+      call RGW_phBSE_dynamic_kernel_B(eta, nOrb, nC, nO, nV, nR, nS, 1d0, eGW, OmRPA, rho, KBd)
+      if(bse2) call RGW_phBSE2_dynamic_kernel_B(eta, nOrb, nC, nO, nV, nR, nS, eGW, W, KBd)
+      ZDyn(ia)  = dot_product(X, matmul(ZAp, X)) + dot_product(Y, matmul(ZAm, Y))
+      OmDyn(ia) = dot_product(X, matmul(KAp - KA_sta, X)) - dot_product(Y, matmul(KAm - KA_sta, Y)) &
+                + dot_product(X, matmul(KBd - KB_sta, Y)) - dot_product(Y, matmul(KBd - KB_sta, X))
+    end if
+    ZDyn(ia)  = 1d0/(1d0 - ZDyn(ia))
+    OmDyn(ia) = ZDyn(ia)*OmDyn(ia)
+  end do
+  deallocate(X, Y, KAp, ZAp, KAm, ZAm, KBd)
+end subroutine
+
 subroutine qkref_phbse(tda_w, tda, dosing, dotrip, eta, nOrb, nC, nO, nV, nR, nS, ERI, eW, eGW, EcBSE, OmS, OmT) &
     bind(C, name='qkref_phbse')
   use iso_c_binding

@@ -17,7 +17,8 @@ PKG = os.path.join(ROOT, "quack_amd")
 CSRC = os.path.join(PKG, "csrc")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIP_SOURCES = ["qk_gemm.hip", "qk_gemm_bf16x3.hip", "qk_gw_kernels.hip", "qk_gw_api.hip", "qk_mlp.hip", "qk_sytrd.hip",
-               "qk_int.hip", "qk_scf.hip", "qk_noalloc.hip", "qk_gw_unrestricted.hip", "qk_gw_batch.hip", "qk_gw_qsgw.hip"]
+               "qk_int.hip", "qk_scf.hip", "qk_noalloc.hip", "qk_gw_unrestricted.hip", "qk_gw_batch.hip", "qk_gw_qsgw.hip",
+               "qk_gw_bse.hip"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

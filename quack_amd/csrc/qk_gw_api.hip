@@ -1549,12 +1549,23 @@ int qk_rgw_excitation_density(int nOrb, int nC, int nO, int nR, int nS, const do
 int qk_rgw_phbse(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE, int dTDA, int singlet, int triplet,
                  double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double* ERI, const double* eW,
                  const double* eGW, double* EcBSE, double* OmBSE) {
-    (void)dTDA;  // only read by the dynamical correction, which is not built
-    if (nOrb <= 0 || nC < 0 || nO <= nC || nR < 0 || nV != nOrb - nO || nS != (nO - nC) * (nV - nR) || nS <= 0 ||
-        !ERI || !eW || !eGW || !EcBSE || !OmBSE)
+    if (dBSE) {
+        g_last_error = "qk_rgw_phbse: the dynamical correction has its outputs in qk_rgw_phbse_dyn";
         return QK_E_ARG;
-    if (dophBSE2 || dBSE) {
-        g_last_error = "qk_rgw_phbse: the second-order kernel and the dynamical correction are not built";
+    }
+    return qk_rgw_phbse_dyn(dophBSE2, exchange_kernel, TDA_W, TDA, 0, dTDA, singlet, triplet, eta, nOrb, nC, nO, nV,
+                            nR, nS, ERI, eW, eGW, EcBSE, OmBSE, nullptr, nullptr);
+}
+
+int qk_rgw_phbse_dyn(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE, int dTDA, int singlet,
+                     int triplet, double eta, int nOrb, int nC, int nO, int nV, int nR, int nS, const double* ERI,
+                     const double* eW, const double* eGW, double* EcBSE, double* OmBSE, double* OmDyn, double* ZDyn) {
+    if (nOrb <= 0 || nC < 0 || nO <= nC || nR < 0 || nV != nOrb - nO || nS != (nO - nC) * (nV - nR) || nS <= 0 ||
+        !ERI || !eW || !eGW || !EcBSE || !OmBSE || (dBSE && (!OmDyn || !ZDyn)))
+        return QK_E_ARG;
+    if ((dophBSE2 || dBSE) && (nC || nR)) {
+        // QuAcK's BSE2 and dynamical kernels index pairs as (a-nO) + (i-1)(nBas-nO)
+        g_last_error = "qk_rgw_phbse_dyn: the second-order kernel and the dynamical correction need nC = nR = 0";
         return QK_E_UNSUPPORTED;
     }
     qk_gw_handle c = nullptr;
@@ -1615,17 +1626,80 @@ int qk_rgw_phbse(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE
             }
             dfree(X); dfree(Y); dfree(G);
         }
-        // singlet (ispin 1) and triplet (ispin 2): phRLR_A/B with dRPA = false at
-        // eGW (LR/phRLR_A.f90:43-115, LR/phRLR_B.f90:34-95) plus the kernels,
-        // then phRLR (GW/RGW_phBSE.f90:133-190)
         auto G = [&](int p, int q, int r, int t) {  // ERI(p,q,r,s), 0-based
             return ERI[p + (size_t)n * (q + (size_t)n * (r + (size_t)n * t))];
         };
+        // device buffers of the second-order kernel and the dynamical correction
+        double *dW = nullptr, *dE = nullptr, *dX = nullptr, *dY = nullptr, *dPart = nullptr, *dK = nullptr;
+        auto free_extra = [&] {
+            (void)hipStreamSynchronize(c->s);
+            dfree(dW); dfree(dE); dfree(dX); dfree(dY); dfree(dPart); dfree(dK);
+        };
+        try {
+        if (dophBSE2 || dBSE) {
+            dE = dalloc<double>(2 * (size_t)n);  // eW, then eGW
+            QK_HIP(hipMemcpyAsync(dE, eW, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+            QK_HIP(hipMemcpyAsync(dE + n, eGW, sizeof(double) * n, hipMemcpyHostToDevice, c->s));
+        }
+        if (dophBSE2) {
+            // GW/RGW_phBSE_static_kernel.f90:40-60: W(p,s,q,r) = -ERI(p,s,q,r) + 4 chi(pq, rs),
+            // chi = sum_m rho(p,q,m) w_m rho(r,s,m) as one FP64 matrix-core product
+            const size_t n2 = (size_t)n * n, n4 = n2 * n2;
+            double* Xw = dalloc<double>(n2 * nS);
+            double* chi = nullptr;
+            try {
+                chi = dalloc<double>(n4);
+                rho_block(c->s, c->rhoT, c->Om, eta, 1, n, nS, 0, n, 0, n, Xw);
+                dgemm(c->s, true, false, (int)n2, (int)n2, nS, 1.0, Xw, nS, c->rhoT, nS, 0.0, chi, n2);
+                std::vector<double> hchi(n4), hW(n4);
+                QK_HIP(hipMemcpyAsync(hchi.data(), chi, sizeof(double) * n4, hipMemcpyDeviceToHost, c->s));
+                QK_HIP(hipStreamSynchronize(c->s));
+                for (int r = 0; r < n; ++r)
+                    for (int q = 0; q < n; ++q)
+                        for (int sI = 0; sI < n; ++sI)
+                            for (int p = 0; p < n; ++p)
+                                hW[p + n * (sI + n * (q + (size_t)n * r))] =
+                                    -G(p, sI, q, r) + 4.0 * hchi[(p + (size_t)n * q) + n2 * (r + (size_t)n * sI)];
+                dW = dalloc<double>(n4);
+                QK_HIP(hipMemcpyAsync(dW, hW.data(), sizeof(double) * n4, hipMemcpyHostToDevice, c->s));
+                QK_HIP(hipStreamSynchronize(c->s));
+            } catch (...) {
+                (void)hipStreamSynchronize(c->s);
+                dfree(Xw); dfree(chi);
+                throw;
+            }
+            dfree(Xw); dfree(chi);
+        }
+        if (dBSE) {
+            dX = dalloc<double>(nS);
+            dY = dalloc<double>(nS);
+            dPart = dalloc<double>(6 * (size_t)bse_dyn_blocks(nS));
+            for (int k = 0; k < 20; ++k) OmDyn[k] = ZDyn[k] = 0.0;
+        }
+        // singlet (ispin 1) and triplet (ispin 2): phRLR_A/B with dRPA = false at
+        // eGW (LR/phRLR_A.f90:43-115, LR/phRLR_B.f90:34-95) plus the kernels,
+        // then phRLR (GW/RGW_phBSE.f90:133-190)
         std::vector<double> A(S2), B(S2), XpY(S2), XmY(S2);
         for (int k = 0; k < 2 * nS; ++k) OmBSE[k] = 0.0;
         EcBSE[0] = EcBSE[1] = 0.0;
         for (int ispin = 1; ispin <= 2; ++ispin) {
             if ((ispin == 1 && !singlet) || (ispin == 2 && !triplet)) continue;
+            if (ispin == 1 && dophBSE2) {
+                // GW/RGW_phBSE.f90:123-138: the second-order kernels are added onto
+                // KA/KB (the BSE2 routines accumulate into the array RGW_phBSE
+                // passes, which holds the first-order kernel); the triplet
+                // block after this one uses the kernels so changed
+                dK = dalloc<double>(2 * S2);
+                bse2_static(c->s, dW, dE, eta, n, nO, nS, dK, TDA ? nullptr : dK + S2);
+                std::vector<double> h(2 * S2);
+                QK_HIP(hipMemcpyAsync(h.data(), dK, sizeof(double) * (TDA ? S2 : 2 * S2), hipMemcpyDeviceToHost,
+                                      c->s));
+                QK_HIP(hipStreamSynchronize(c->s));
+                for (size_t k = 0; k < S2; ++k) KA[k] += h[k];
+                if (!TDA)
+                    for (size_t k = 0; k < S2; ++k) KB[k] += h[S2 + k];
+                dfree(dK);
+            }
             const double ct1 = ispin == 1 ? 2.0 : 0.0, ct2 = -1.0;
             for (int i = 0; i < o; ++i)
                 for (int a = 0; a < v; ++a) {
@@ -1643,16 +1717,63 @@ int qk_rgw_phbse(int dophBSE2, int exchange_kernel, int TDA_W, int TDA, int dBSE
                         }
                 }
             double ec = 0.0;
-            const int r = qk_phrlr(TDA, nS, A.data(), B.data(), &ec, OmBSE + (size_t)(ispin - 1) * nS, XpY.data(),
-                                   XmY.data());
+            double* Om = OmBSE + (size_t)(ispin - 1) * nS;
+            const int r = qk_phrlr(TDA, nS, A.data(), B.data(), &ec, Om, XpY.data(), XmY.data());
             QK_CHECK(r == QK_OK, r, "qk_rgw_phbse: phRLR failed: " + g_last_error);
             EcBSE[ispin - 1] = ec;
+            if (!dBSE) continue;
+            // GW/RGW_phBSE_dynamic_perturbation.f90:73-122 for the first
+            // min(nS, 10) excitations of this manifold: the dynamic kernels'
+            // quadratic forms on the device, the static ones here
+            const int nb = bse_dyn_blocks(nS);
+            std::vector<double> X(nS), Y(nS), part(6 * (size_t)nb);
+            for (int ia = 0; ia < std::min(nS, 10); ++ia) {
+                for (int k = 0; k < nS; ++k) {
+                    X[k] = 0.5 * (XpY[ia + (size_t)nS * k] + XmY[ia + (size_t)nS * k]);
+                    Y[k] = 0.5 * (XpY[ia + (size_t)nS * k] - XmY[ia + (size_t)nS * k]);
+                }
+                QK_HIP(hipMemcpyAsync(dX, X.data(), sizeof(double) * nS, hipMemcpyHostToDevice, c->s));
+                QK_HIP(hipMemcpyAsync(dY, Y.data(), sizeof(double) * nS, hipMemcpyHostToDevice, c->s));
+                bse_dyn(c->s, c->rhoT, c->Om, dE + n, dophBSE2 ? dW : nullptr, dX, dY, Om[ia], eta, n, nO, nS,
+                        dTDA ? 1 : 0, dPart);
+                QK_HIP(hipMemcpyAsync(part.data(), dPart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->s));
+                QK_HIP(hipStreamSynchronize(c->s));
+                double qd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                for (int bI = 0; bI < nb; ++bI)
+                    for (int k = 0; k < 6; ++k) qd[k] += part[6 * (size_t)bI + k];
+                auto quad = [&](const std::vector<double>& u, const std::vector<double>& M, const std::vector<double>& w) {
+                    double t = 0.0;
+                    for (int jb = 0; jb < nS; ++jb) {
+                        double row = 0.0;
+                        for (int kk = 0; kk < nS; ++kk) row += M[kk + (size_t)nS * jb] * u[kk];
+                        t += row * w[jb];
+                    }
+                    return t;
+                };
+                double z, om;
+                if (dTDA) {
+                    z = qd[1];
+                    om = qd[0] - quad(X, KA, X);
+                } else {
+                    z = qd[1] + qd[3];
+                    om = (qd[0] - quad(X, KA, X)) - (qd[2] - quad(Y, KA, Y)) + (qd[4] - quad(X, KB, Y)) -
+                         (qd[5] - quad(Y, KB, X));
+                }
+                z = 1.0 / (1.0 - z);
+                ZDyn[10 * (ispin - 1) + ia] = z;
+                OmDyn[10 * (ispin - 1) + ia] = z * om;
+            }
         }
         EcBSE[1] *= 3.0;  // GW/RGW_phBSE.f90:193
         if (exchange_kernel) {
             EcBSE[0] *= 0.5;
             EcBSE[1] *= 0.5;
         }
+        } catch (...) {
+            free_extra();
+            throw;
+        }
+        free_extra();
     });
     if (rc != QK_OK && c) g_last_error = c->err;
     qk_gw_destroy(c);

@@ -69,6 +69,16 @@ void sigma_full(hipStream_t s, const double* rhoT, const double* e, const double
                 int n, int nC, int nO, int nR, int nS, double* Sig, double* work);
 void rho_block(hipStream_t s, const double* rhoT, const double* Om, double eta, int weighted, int n, int nS, int p0,
                int P, int q0, int Q, double* out);
+// Second-order static BSE kernels (GW/RGW_phBSE2_static_kernel_A.f90, _B.f90)
+// from the 4-index W of GW/RGW_phBSE_static_kernel.f90 (KB2 skipped when null).
+void bse2_static(hipStream_t s, const double* W, const double* e, double eta, int n, int nO, int nS, double* KA2,
+                 double* KB2);
+// The quadratic forms of the first-order dynamical BSE correction for one
+// excitation (GW/RGW_phBSE_dynamic_perturbation.f90): 6 partial sums per
+// block of bse_dyn_blocks(nS) blocks in part.
+int bse_dyn_blocks(int nS);
+void bse_dyn(hipStream_t s, const double* rhoT, const double* OmR, const double* e, const double* W, const double* X,
+             const double* Y, double Om, double eta, int n, int nO, int nS, int dtda, double* part);
 void qp_init(hipStream_t s, const double* eGWlin, double* w, double* f, int* nit, int* done, int n, int nC, int nR,
              const Zs& zs = Zs(), int nz = 1);
 void qp_update(hipStream_t s, const double* part, int nchunk, const double* eHF, double* w, double* f, double* df,

@@ -8,8 +8,10 @@
 ! libquack_amd.so through the C-ABI; the tables and test dumps are written
 ! in QuAcK's formats (src/GW/print_RG0W0.f90:33-69, print_evRGW.f90:107-152,
 ! src/GW/RG0W0.f90:299-305, src/GW/evRGW.f90:264-270).
-! The linDM, phBSE, ppBSE and ACFDT branches stay on QuAcK's CPU routines:
-! requesting them here stops with a nonzero code.
+! phBSE (static kernel, second-order kernel, first-order dynamical
+! correction) runs on the GPU after the GW step; the linDM, ppBSE and ACFDT
+! branches stay on QuAcK's CPU routines: requesting them here stops with a
+! nonzero code.
 
 subroutine qk_unsupported(what)
   implicit none
@@ -59,8 +61,6 @@ subroutine RG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,dophBSE2,TDA_W
   type(c_ptr) :: h
   integer(c_int) :: rc
 
-  if(dophBSE2) call qk_unsupported('phBSE2@G0W0')
-  if(dophBSE .and. dBSE) call qk_unsupported('dynamical phBSE@G0W0')
   if(doppBSE) call qk_unsupported('ppBSE@G0W0')
   if(doACFDT) call qk_unsupported('ACFDT@G0W0')
   if(do_linDM_GW) call qk_unsupported('linDM@G0W0')
@@ -111,8 +111,8 @@ subroutine RG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,dophBSE2,TDA_W
   write(*,*)'-------------------------------------------------------------------------------'
   write(*,*)
 
-  ! static phBSE at the G0W0 energies (GW/RG0W0.f90:239-255): W from eHF
-  if(dophBSE) call qk_phbse_print('G0W0', exchange_kernel, TDA_W, TDA, dTDA, singlet, triplet, eta, &
+  ! phBSE at the G0W0 energies (GW/RG0W0.f90:239-255): W from eHF
+  if(dophBSE) call qk_phbse_print('G0W0', dophBSE2, exchange_kernel, TDA_W, TDA, dBSE, dTDA, singlet, triplet, eta, &
                                   nOrb, nC, nO, nV, nR, nS, ENuc, ERHF, ERI, eHF, eGW_out)
 
   if(dotest) then
@@ -124,39 +124,85 @@ subroutine RG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,dophBSE2,TDA_W
   deallocate(SigC, Z, Om)
 end subroutine
 
-! Static phBSE (qk_rgw_phbse) after a GW calculation, with QuAcK's output:
-! the excitation tables that RGW_phBSE prints (GW/RGW_phBSE.f90:145,180) and
-! the Tr@BSE energies of GW/RG0W0.f90:248-255 / GW/evRGW.f90:219-226.
-subroutine qk_phbse_print(tag, exchange_kernel, TDA_W, TDA, dTDA, singlet, triplet, eta, nOrb, nC, nO, nV, nR, nS, &
-                          ENuc, ERHF, ERI, eW, eGW)
+! phBSE (qk_rgw_phbse_dyn) after a GW calculation, with QuAcK's output: the
+! excitation tables that RGW_phBSE prints (GW/RGW_phBSE.f90:145,180), the
+! second-order kernel's notice (:117-121), the first-order dynamical
+! correction's table (GW/RGW_phBSE_dynamic_perturbation.f90:62-122) after
+! each manifold, and the Tr@BSE energies of GW/RG0W0.f90:248-255 /
+! GW/evRGW.f90:219-226 (Eref: ERHF there, EqsGW after qsGW).
+subroutine qk_phbse_print(tag, dophBSE2, exchange_kernel, TDA_W, TDA, dBSE, dTDA, singlet, triplet, eta, nOrb, nC, &
+                          nO, nV, nR, nS, ENuc, Eref, ERI, eW, eGW)
   use quack_amd_gw
   implicit none
   character(len=*), intent(in) :: tag
-  logical, intent(in) :: exchange_kernel, TDA_W, TDA, dTDA, singlet, triplet
-  double precision, intent(in) :: eta, ENuc, ERHF
+  logical, intent(in) :: dophBSE2, exchange_kernel, TDA_W, TDA, dBSE, dTDA, singlet, triplet
+  double precision, intent(in) :: eta, ENuc, Eref
   integer, intent(in) :: nOrb, nC, nO, nV, nR, nS
   double precision, intent(in) :: ERI(nOrb,nOrb,nOrb,nOrb), eW(nOrb), eGW(nOrb)
   double precision, allocatable :: OmBSE(:)
-  double precision :: EcBSE(2)
+  double precision :: EcBSE(2), OmDyn(20), ZDyn(20)
   integer(c_int) :: rc
   allocate(OmBSE(2*nS))
-  rc = qk_rgw_phbse(0_c_int, merge(1_c_int, 0_c_int, exchange_kernel), merge(1_c_int, 0_c_int, TDA_W), &
-                    merge(1_c_int, 0_c_int, TDA), 0_c_int, merge(1_c_int, 0_c_int, dTDA),                 &
-                    merge(1_c_int, 0_c_int, singlet), merge(1_c_int, 0_c_int, triplet), eta,             &
-                    int(nOrb,c_int), int(nC,c_int), int(nO,c_int), int(nV,c_int), int(nR,c_int),        &
-                    int(nS,c_int), ERI, eW, eGW, EcBSE, OmBSE)
-  call qk_check(rc, c_null_ptr, 'qk_rgw_phbse')
-  if(singlet) call qk_print_excitations('phBSE@GW@RHF', 'singlet', nS, OmBSE(1:nS))
-  if(triplet) call qk_print_excitations('phBSE@GW@RHF', 'triplet', nS, OmBSE(nS+1:2*nS))
+  rc = qk_rgw_phbse_dyn(merge(1_c_int, 0_c_int, dophBSE2), merge(1_c_int, 0_c_int, exchange_kernel),       &
+                        merge(1_c_int, 0_c_int, TDA_W), merge(1_c_int, 0_c_int, TDA),                        &
+                        merge(1_c_int, 0_c_int, dBSE), merge(1_c_int, 0_c_int, dTDA),                        &
+                        merge(1_c_int, 0_c_int, singlet), merge(1_c_int, 0_c_int, triplet), eta,           &
+                        int(nOrb,c_int), int(nC,c_int), int(nO,c_int), int(nV,c_int), int(nR,c_int),      &
+                        int(nS,c_int), ERI, eW, eGW, EcBSE, OmBSE, OmDyn, ZDyn)
+  call qk_check(rc, c_null_ptr, 'qk_rgw_phbse_dyn')
+  if(singlet) then
+    if(dophBSE2) then
+      write(*,*)
+      write(*,*) '*** Second-order BSE static kernel activated! ***'
+      write(*,*)
+    end if
+    call qk_print_excitations('phBSE@GW@RHF', 'singlet', nS, OmBSE(1:nS))
+    if(dBSE) call qk_print_dynamic(dophBSE2, dTDA, nS, OmBSE(1:nS), OmDyn(1:10), ZDyn(1:10))
+  end if
+  if(triplet) then
+    call qk_print_excitations('phBSE@GW@RHF', 'triplet', nS, OmBSE(nS+1:2*nS))
+    if(dBSE) call qk_print_dynamic(dophBSE2, dTDA, nS, OmBSE(nS+1:2*nS), OmDyn(11:20), ZDyn(11:20))
+  end if
   write(*,*)
   write(*,*)'-------------------------------------------------------------------------------'
   write(*,'(2X,A50,F20.10,A3)') 'Tr@BSE@'//tag//'@RHF correlation energy (singlet) = ',EcBSE(1),' au'
   write(*,'(2X,A50,F20.10,A3)') 'Tr@BSE@'//tag//'@RHF correlation energy (triplet) = ',EcBSE(2),' au'
   write(*,'(2X,A50,F20.10,A3)') 'Tr@BSE@'//tag//'@RHF correlation energy           = ',sum(EcBSE),' au'
-  write(*,'(2X,A50,F20.10,A3)') 'Tr@BSE@'//tag//'@RHF total       energy           = ',ENuc + ERHF + sum(EcBSE),' au'
+  write(*,'(2X,A50,F20.10,A3)') 'Tr@BSE@'//tag//'@RHF total       energy           = ',ENuc + Eref + sum(EcBSE),' au'
   write(*,*)'-------------------------------------------------------------------------------'
   write(*,*)
   deallocate(OmBSE)
+end subroutine
+
+! The table of GW/RGW_phBSE_dynamic_perturbation.f90:56-122 for one manifold.
+subroutine qk_print_dynamic(dophBSE2, dTDA, nS, OmBSE, OmDyn, ZDyn)
+  implicit none
+  logical, intent(in) :: dophBSE2, dTDA
+  integer, intent(in) :: nS
+  double precision, intent(in) :: OmBSE(nS), OmDyn(10), ZDyn(10)
+  double precision, parameter :: HaToeV = 27.211386245988d0
+  integer :: ia
+  if(dTDA) then
+    write(*,*)
+    write(*,*) '*** dynamical TDA activated ***'
+    write(*,*)
+  end if
+  if(dophBSE2) then
+    write(*,*)
+    write(*,*) '*** Second-order BSE dynamic kernel activated! ***'
+    write(*,*)
+  end if
+  write(*,*) '---------------------------------------------------------------------------------------------------'
+  write(*,*) ' First-order dynamical correction to static Bethe-Salpeter excitation energies                     '
+  write(*,*) '---------------------------------------------------------------------------------------------------'
+  write(*,'(2X,A5,1X,A20,1X,A20,1X,A20,1X,A20)') '#','Static (eV)','Dynamic (eV)','Correction (eV)','Renorm. (eV)'
+  write(*,*) '---------------------------------------------------------------------------------------------------'
+  do ia = 1, min(nS, 10)
+    write(*,'(2X,I5,5X,F15.6,5X,F15.6,5X,F15.6,5X,F15.6)') &
+      ia, OmBSE(ia)*HaToeV, (OmBSE(ia) + OmDyn(ia))*HaToeV, OmDyn(ia)*HaToeV, ZDyn(ia)
+  end do
+  write(*,*) '---------------------------------------------------------------------------------------------------'
+  write(*,*)
 end subroutine
 
 ! Excitation energies of one spin manifold, in the table of QuAcK's
@@ -200,8 +246,6 @@ subroutine evRGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS
   integer(c_int) :: rc, nSCF
   type(c_ptr) :: h
 
-  if(dophBSE2) call qk_unsupported('phBSE2@evGW')
-  if(dophBSE .and. dBSE) call qk_unsupported('dynamical phBSE@evGW')
   if(doppBSE) call qk_unsupported('ppBSE@evGW')
   if(doACFDT) call qk_unsupported('ACFDT@evGW')
 
@@ -255,8 +299,8 @@ subroutine evRGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS
   write(*,*)'-------------------------------------------------------------------------------'
   write(*,*)
 
-  ! static phBSE at the converged energies (GW/evRGW.f90:214-226): W from eGW too
-  if(dophBSE) call qk_phbse_print('evGW', exchange_kernel, TDA_W, TDA, dTDA, singlet, triplet, eta, &
+  ! phBSE at the converged energies (GW/evRGW.f90:214-226): W from eGW too
+  if(dophBSE) call qk_phbse_print('evGW', dophBSE2, exchange_kernel, TDA_W, TDA, dBSE, dTDA, singlet, triplet, eta, &
                                   nOrb, nC, nO, nV, nR, nS, ENuc, ERHF, ERI, eGW, eGW)
 
   if(dotest) then

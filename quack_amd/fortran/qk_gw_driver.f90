@@ -88,14 +88,14 @@ program qk_gw_driver
   end do
   close(12)
 
-  ! GPU switch and the dispatch of the patched RGW/UGW: the options the GPU
-  ! drop-ins do not carry (phBSE2, ppBSE, dynamical BSE) stay on the CPU routines
+  ! GPU switch and the dispatch of the patched RGW/UGW: the option the GPU
+  ! drop-ins do not carry (ppBSE) stays on the CPU routines
   call qk_read_hpc_flags(wd, switch_hpc, use_gpu)
-  if((doG0W0 .or. doevGW .or. doqsGW) .and. .not.(use_gpu .and. .not.(dophBSE2 .or. doppBSE .or. dBSE))) then
+  if((doG0W0 .or. doevGW .or. doqsGW) .and. .not.(use_gpu .and. .not. doppBSE)) then
     if(.not. use_gpu) then
       write(*,*) 'qk_gw_driver: use_gpu is F in input/hpc_flags: G0W0/evGW belong to QuAcK''s CPU routines'
     else
-      write(*,*) 'qk_gw_driver: phBSE2, ppBSE and dBSE belong to QuAcK''s CPU routines'
+      write(*,*) 'qk_gw_driver: ppBSE belongs to QuAcK''s CPU routines'
     end if
     write(*,*) '              (RG0W0/evRGW, UG0W0/evUGW), which this driver does not contain'
     error stop 3

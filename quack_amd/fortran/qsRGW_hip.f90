@@ -7,9 +7,9 @@
 ! J and K, the MO integrals, screening, the full self-energy, F, DIIS, the new
 ! orbitals, all on the device).  X is read as nBas x nOrb, the shape QuAcK
 ! allocates for it.  On return ERI_MO and dipole_int_MO hold the integrals of
-! the last cycle's orbitals, as qsRGW leaves them.  linDM, phBSE2, dynamical
-! BSE, ppBSE and ACFDT stay on QuAcK's CPU routines: requesting them here
-! stops with a nonzero code; static phBSE runs on the GPU.
+! the last cycle's orbitals, as qsRGW leaves them.  linDM, ppBSE and ACFDT
+! stay on QuAcK's CPU routines: requesting them here stops with a nonzero
+! code; phBSE (static, second-order, dynamical correction) runs on the GPU.
 
 ! GW/print_qsRGW.f90:1-150: one cycle's table and energies, and the summary
 ! once the commutator is below thresh.
@@ -110,8 +110,6 @@ subroutine qsRGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS
   type(c_ptr) :: h
 
   if(do_linDM_GW) call qk_unsupported('linDM@qsGW')
-  if(dophBSE2) call qk_unsupported('phBSE2@qsGW')
-  if(dophBSE .and. dBSE) call qk_unsupported('dynamical phBSE@qsGW')
   if(doppBSE) call qk_unsupported('ppBSE@qsGW')
   if(doACFDT) call qk_unsupported('ACFDT@qsGW')
 
@@ -183,9 +181,9 @@ subroutine qsRGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS
     dipole_int_MO(:,:,ixyz) = matmul(transpose(cOld), matmul(dipole_int_AO(:,:,ixyz), cOld))
   end do
 
-  ! static phBSE@qsGW (:373-384), with the total energy on EqsGW
-  if(dophBSE) call qk_phbse_print('qsGW', exchange_kernel, TDA_W, TDA, dTDA, singlet, triplet, eta, nOrb, nC, nO, &
-                                  nV, nR, nS, ENuc, E(5), ERI_MO, eGW, eGW)
+  ! phBSE@qsGW (:373-384), with the total energy on EqsGW
+  if(dophBSE) call qk_phbse_print('qsGW', dophBSE2, exchange_kernel, TDA_W, TDA, dBSE, dTDA, singlet, triplet, eta, &
+                                  nOrb, nC, nO, nV, nR, nS, ENuc, E(5), ERI_MO, eGW, eGW)
 
   if(dotest) then
     call dump_test_value('R','qsGW correlation energy',EcRPA)

@@ -82,6 +82,17 @@ module quack_amd_gw
       real(c_double), intent(out) :: EcBSE(2), OmBSE(*)
     end function
 
+    integer(c_int) function qk_rgw_phbse_dyn(dophBSE2, exchange_kernel, TDA_W, TDA, dBSE, dTDA, singlet, triplet, &
+                                             eta, nOrb, nC, nO, nV, nR, nS, ERI, eW, eGW, EcBSE, OmBSE, OmDyn, ZDyn) &
+                                             bind(C, name='qk_rgw_phbse_dyn')
+      import :: c_int, c_double
+      integer(c_int), value :: dophBSE2, exchange_kernel, TDA_W, TDA, dBSE, dTDA, singlet, triplet
+      real(c_double), value :: eta
+      integer(c_int), value :: nOrb, nC, nO, nV, nR, nS
+      real(c_double), intent(in) :: ERI(*), eW(*), eGW(*)
+      real(c_double), intent(out) :: EcBSE(2), OmBSE(*), OmDyn(20), ZDyn(20)
+    end function
+
     ! unrestricted G0W0 and evGW (GW/UG0W0.f90, GW/evUGW.f90); nC, nO, nR per spin
     integer(c_int) function qk_ug0w0(nBas, nC, nO, nR, ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, eta, linearize, TDA_W, &
                                      doSRG, eGW, SigC, Z, Om, EcRPA, EcGM) bind(C, name='qk_ug0w0')

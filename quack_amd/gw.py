@@ -34,7 +34,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms",
            "qk_rgw_qp_graph_noalloc", "qk_noalloc_last_error", "qk_gw_qp_iters",
            "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats", "qk_gwb_set_graphs", "qk_gwb_graph_stats", "qk_set_split_aotomo",
-           "qk_qsgw_create", "qk_qsgw_cycle", "qk_qsgw_destroy", "qk_qsgw_last_error"]
+           "qk_qsgw_create", "qk_qsgw_cycle", "qk_qsgw_destroy", "qk_qsgw_last_error", "qk_rgw_phbse_dyn"]
 
 
 EIG_MODES = {"syevd": 0, "refine": 1, "sytrd": 2, "refine+sytrd": 3}
@@ -135,6 +135,7 @@ def load():
     lib.qk_rgw_qp_graph_noalloc.argtypes = [_I, _D, _D] + [_I] * 6 + [_H] * 8
     lib.qk_noalloc_last_error.restype = ctypes.c_char_p
     lib.qk_rgw_phbse.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 5
+    lib.qk_rgw_phbse_dyn.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 7
     lib.qk_gw_sigma_full.argtypes = [_H, _P, _D, _I, _P, _P, _P, _P]
     lib.qk_ug0w0.argtypes = [_I, _IP, _IP, _IP, _P, _P, _P, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
     lib.qk_evugw.argtypes = [_I, _IP, _IP, _IP, _P, _P, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P,
@@ -603,9 +604,11 @@ class MLP:
 
 
 def RGW_phBSE(ERI, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, singlet=True, triplet=True, exchange_kernel=False,
-              nC=0, nR=0):
-    """GW/RGW_phBSE.f90 with the static kernel (qk_rgw_phbse).  Returns
-    dict(EcBSE (2), OmS, OmT)."""
+              nC=0, nR=0, dophBSE2=False, dBSE=False, dTDA=True):
+    """GW/RGW_phBSE.f90 (qk_rgw_phbse_dyn): the static kernel, with
+    dophBSE2 the second-order one added, with dBSE the first-order dynamical
+    correction of the first min(nS, 10) states of each manifold.  Returns
+    dict(EcBSE (2), OmS, OmT, dyn: {"singlet"/"triplet": (Om_dyn, Z_dyn)})."""
     eW, eGW = _f64(eW), _f64(eGW)
     n = len(eW)
     nV = n - nO
@@ -613,9 +616,18 @@ def RGW_phBSE(ERI, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, singlet=True, t
     ERI = _f64(ERI)  # QuAcK's column-major nOrb^4 array (a flat array as it is)
     Ec = np.zeros(2)
     Om = np.zeros(2 * nS)
-    _check(load().qk_rgw_phbse(0, int(exchange_kernel), int(TDA_W), int(TDA), 0, 0, int(singlet), int(triplet), eta,
-                               n, nC, nO, nV, nR, nS, _p(ERI), _p(eW), _p(eGW), _p(Ec), _p(Om)))
-    return {"EcBSE": Ec, "OmS": Om[:nS].copy(), "OmT": Om[nS:].copy()}
+    Od, Zd = np.zeros(20), np.zeros(20)
+    _check(load().qk_rgw_phbse_dyn(int(dophBSE2), int(exchange_kernel), int(TDA_W), int(TDA), int(dBSE), int(dTDA),
+                                   int(singlet), int(triplet), eta, n, nC, nO, nV, nR, nS, _p(ERI), _p(eW), _p(eGW),
+                                   _p(Ec), _p(Om), _p(Od), _p(Zd)))
+    m = min(nS, 10)
+    dyn = {}
+    if dBSE:
+        if singlet:
+            dyn["singlet"] = (Od[:m].copy(), Zd[:m].copy())
+        if triplet:
+            dyn["triplet"] = (Od[10:10 + m].copy(), Zd[10:10 + m].copy())
+    return {"EcBSE": Ec, "OmS": Om[:nS].copy(), "OmT": Om[nS:].copy(), "dyn": dyn}
 
 
 def set_split_aotomo(on):

@@ -67,7 +67,7 @@ OPTIONS = """# HF: maxSCF thresh    DIIS guess mix shift stab search aordm readF
 # ACFDT: AC Kx  XBS
          F  F   F
 # BSE: phBSE phBSE2 ppBSE dBSE dTDA
-       {bse}     F      F     F    F
+       {bse}     {bse2}      F     {dbse}    {dtda}
 """
 
 
@@ -77,9 +77,10 @@ def tf(b):
 
 def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=1e-5, max_diis=5, lin=False,
                  eta=0.0, TDA_W=False, reg=False, eri_mo=None, bse=False, uhf=False, nel=None, use_gpu=True,
-                 ncore=0, nryd=0, qsgw=False):
+                 ncore=0, nryd=0, qsgw=False, bse2=False, dbse=False, dtda=False):
     """sysd: a dict from quack_amd.intgen.system(); eri_mo: MO integrals (nOrb^4, flat);
-    bse: the phBSE flag of the "# BSE:" options line (static kernel); uhf: the
+    bse, bse2, dbse, dtda: the phBSE, phBSE2, dBSE and dTDA flags of the
+    "# BSE:" options line (read_options.f90:300-315); uhf: the
     methods file selects UHF instead of RHF (the unrestricted GW branch);
     nel: (nEla, nElb) for input/molecule (default: the closed shell); use_gpu:
     the GPU switch of input/hpc_flags (its fourth line, read_hpc_flags.f90:31-33),
@@ -94,7 +95,8 @@ def write_inputs(wd, sysd, g0w0=True, evgw=False, test=True, maxSCF=256, thresh=
                                uhf=tf(uhf)))
     with open(os.path.join(wd, "input", "options"), "w") as f:
         f.write(OPTIONS.format(maxscf=maxSCF, thresh="%.10g" % thresh, diis=max_diis, lin=tf(lin), eta="%.10g" % eta,
-                               tda=tf(TDA_W), reg=tf(reg), bse=tf(bse)))
+                               tda=tf(TDA_W), reg=tf(reg), bse=tf(bse), bse2=tf(bse2), dbse=tf(dbse),
+                               dtda=tf(dtda)))
     hpc = os.path.join(wd, "input", "hpc_flags")
     if use_gpu is None:
         if os.path.exists(hpc):

@@ -79,6 +79,49 @@ def test_driver_phbse_matches_oracle(tmp_path, mo_systems):
     assert np.isfinite(vals[("evGW", "singlet")])
 
 
+@pytest.mark.gpu
+def test_driver_phbse2_dynamic(tmp_path, mo_systems):
+    """phBSE with the second-order kernel and the dynamical correction
+    switched on in input/options (phBSE2, dBSE, dTDA F): the drop-in prints
+    the Tr@BSE energies and the dynamical-correction tables, which match the
+    CPU restatement at the printed precision."""
+    import numpy as np
+
+    from oracle import gw_oracle as go
+    print("fortran driver phBSE2 + dBSE H2O", flush=True)
+    s = mo_systems["H2O"]
+    nO, mo, e = s["nO"], s["eri_mo"], s["eHF"]
+    quackio.write_inputs(str(tmp_path), s, g0w0=True, evgw=False, eta=0.1, eri_mo=mo, bse=True, bse2=True, dbse=True,
+                         dtda=False)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Second-order BSE static kernel activated" in r.stdout
+    eGW = go.rg0w0(mo, e, nO, eta=0.1)["eGW"]
+    dyn = {}
+    Ec, OmS, OmT = go.rgw_phbse(mo, e, eGW, nO, eta=0.1, dophBSE2=True, dBSE=True, dTDA=False, dyn=dyn)
+    vals = {}
+    for ln in r.stdout.splitlines():
+        for man in ("singlet", "triplet"):
+            key = "Tr@BSE@G0W0@RHF correlation energy (%s) =" % man
+            if key in ln:
+                vals[man] = float(ln.split("=")[1].split()[0])
+    assert abs(vals["singlet"] - Ec[0]) < 2e-10 and abs(vals["triplet"] - Ec[1]) < 2e-10
+    # the two dynamical-correction tables: state, static, dynamic, correction (eV), Z
+    lines = r.stdout.splitlines()
+    tables = [k for k, ln in enumerate(lines) if "First-order dynamical correction" in ln]
+    assert len(tables) == 2
+    HA = 27.211386245988
+    for t0, (man, Om) in zip(tables, (("singlet", OmS), ("triplet", OmT))):
+        # the rows: the first ten lines after the title that start with a state number
+        # (the runtime may wrap the 99-character rules of list-directed output)
+        rows = [ln.split() for ln in lines[t0 + 1:] if ln.split() and ln.split()[0].isdigit()][:10]
+        od, zd = dyn[man]
+        for k, row in enumerate(rows):
+            assert int(row[0]) == k + 1
+            assert abs(float(row[1]) - Om[k] * HA) < 2e-6 and abs(float(row[3]) - od[k] * HA) < 2e-6
+            assert abs(float(row[4]) - zd[k]) < 2e-6
+
+
 def test_input_files_uhf_flag(tmp_path, systems):
     """The unrestricted branch is selected by the HF line of input/methods and
     the electron counts of input/molecule, as QuAcK reads them."""

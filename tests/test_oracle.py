@@ -347,6 +347,40 @@ def test_phbse_vs_compiled_reference(mo_systems, name):
 
 
 @pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+@pytest.mark.parametrize("name", ["He", "H2O"])
+def test_phbse2_dynamic_vs_compiled_reference(mo_systems, name):
+    """phBSE@G0W0 with QuAcK's second-order static kernel (dophBSE2:
+    RGW_phBSE_static_kernel's W and RGW_phBSE2_static_kernel_A/B, which add
+    onto the first-order kernels, the triplet block reusing the kernels the
+    singlet block changed) and the first-order dynamical correction (dBSE:
+    RGW_phBSE_dynamic_kernel_A/B and the BSE2 dynamic kernels, with the
+    arithmetic of RGW_phBSE_dynamic_perturbation), full and dynamical TDA,
+    against QuAcK's routines composed in RGW_phBSE's order."""
+    s = mo_systems[name]
+    nO, mo, e = s["nO"], s["eri_mo"], s["eHF"]
+    eGW = go.rg0w0(mo, e, nO, eta=0.1)["eGW"]
+    cases = [{"bse2": True}, {"dbse": True, "dtda": False}, {"bse2": True, "dbse": True, "dtda": False},
+             {"bse2": True, "dbse": True, "tda": True}, {"bse2": True, "singlet": False}]
+    for kw in cases:
+        r = qkref.phbse_full(mo, e, eGW, nO, eta=0.1, **kw)
+        dyn = {}
+        Ec, OmS, OmT = go.rgw_phbse(mo, e, eGW, nO, eta=0.1, TDA=kw.get("tda", False),
+                                    singlet=kw.get("singlet", True), dophBSE2=kw.get("bse2", False),
+                                    dBSE=kw.get("dbse", False), dTDA=kw.get("dtda", True), dyn=dyn)
+        assert np.abs(OmS - r["OmS"]).max() < 1e-10 and np.abs(OmT - r["OmT"]).max() < 1e-10, kw
+        assert np.abs(Ec - r["EcBSE"]).max() < 1e-10, kw
+        for man, (od, zd) in dyn.items():
+            assert np.abs(od - r["dyn"][man][0]).max() < 1e-12 and np.abs(zd - r["dyn"][man][1]).max() < 1e-12
+    # the second-order kernel and the dynamical correction move the energies
+    _, OmS1, _ = go.rgw_phbse(mo, e, eGW, nO, eta=0.1)
+    _, OmS2, _ = go.rgw_phbse(mo, e, eGW, nO, eta=0.1, dophBSE2=True)
+    assert np.abs(OmS2 - OmS1).max() > 1e-4
+    dyn = {}
+    go.rgw_phbse(mo, e, eGW, nO, eta=0.1, dBSE=True, dyn=dyn)
+    assert np.abs(dyn["singlet"][0]).max() > 1e-4 and np.all(dyn["singlet"][1] > 0.5)
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
 @pytest.mark.parametrize("name", ["H2O", "H10"])
 def test_full_self_energy_vs_compiled_reference(mo_systems, name):
     """GW/RGW_self_energy.f90 and RGW_SRG_self_energy.f90 (the full matrix qsGW
