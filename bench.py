@@ -34,7 +34,7 @@ run through the library's HIP kernels on the device.
 
     python bench.py [--steps K] [--warmup W] [--form fp64|mixed|mixed2] [--eig syevd|refine|sytrd|refine+sytrd]
                     [--concurrent C] [--method evgw|g0w0] [--reg eta|srg|none] [--eta ETA] [--max-diis D]
-                    [--learned on|off] [--cpu-baseline auto|off]
+                    [--learned on|off] [--cpu-baseline auto|off] [--lib-gemm on|off]
                     [--workload h50|df600|batch|learned|ugw|cpu250]
 
 Prints one JSON line on stdout; progress goes to stderr.
@@ -324,6 +324,10 @@ def main():
     ap.add_argument("--learned", choices=["on", "off"], default="on",
                     help="on: the timed form starts evGW from the MLP's prediction (quack_amd/learned.py)")
     ap.add_argument("--baseline-reps", type=int, default=5, help="runs of the plain FP64 form (median reported)")
+    ap.add_argument("--lib-gemm", choices=["on", "off"], default="off",
+                    help="on: rocBLAS's dgemm joins the autotuner's candidates for large plain FP64 products, in "
+                         "every form (the plain FP64 baseline included); off keeps every product on the own "
+                         "kernels, whose tile choices give bit-identical results")
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--ugw-atoms", type=int, default=10, help="ugw workload: H chain length (one more alpha electron)")
     ap.add_argument("--workload", choices=["h50", "df600", "batch", "learned", "ugw", "cpu250"], default="h50",
@@ -384,6 +388,7 @@ def run_h50(args):
     from quack_amd import gw, intgen, learned, workloads
     torch.cuda.init()  # torch's HIP runtime first, then the library binds to it
     gw.load()
+    gw.set_library_gemm(args.lib_gemm == "on")
     mk = method_kw(args)
     evkw = {"maxSCF": 256, "thresh": 1e-6, "max_diis": args.max_diis, **mk}
     mols, inputs = [], []
@@ -585,6 +590,7 @@ def run_h50(args):
                    "plain_cycles_per_molecule": [x.get("nSCF", 1) for x in refs], "form": args.form,
                    "eigensolver": args.eig, "refine_de_max": args.de_max, "learned": use_learned,
                    "molecules_in_flight": K, "hip_hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+                   "library_gemm": args.lib_gemm == "on",
                    "full_rpa_solves_per_molecule_timed": full_timed / (K * args.steps),
                    "refined_rpa_solves_per_molecule_timed": refined_timed / (K * args.steps),
                    "step": "%d molecules" % K},

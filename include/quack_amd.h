@@ -321,6 +321,13 @@ const char *qk_last_error(void);
  * qk_autotune_report writes one line per tuned shape (QK_E_ARG if len is too small). */
 int qk_set_autotune(int on);
 int qk_autotune_report(char *buf, int len);
+/* rocBLAS's dgemm as one more autotuner candidate (process-wide; off by
+ * default) for plain FP64 products: unbatched, all tiles, every dimension at
+ * least 512, 32-bit indexable, outside graph capture.  It sums in its own
+ * order, so with it on, results can differ from the default's in the last
+ * bits; off, every candidate gives identical results.  No counterpart in
+ * QuAcK (its DGEMMs are the host BLAS's). */
+int qk_set_library_gemm(int on);
 
 /* hipGraph replay of the own full eigensolve's launch sequences (process-wide;
  * off by default, since the replay measured slower than stream launches): the

@@ -47,6 +47,8 @@ inline bool stream_capturing(hipStream_t s) {
 
 // Process-wide GEMM tile autotuner (on by default): see qk_gemm.hip.
 void set_autotune(bool on);
+// The library FP64 GEMM as an autotuner candidate for large plain products (off by default).
+void set_library_gemm(bool on);
 bool autotune_on();
 std::string autotune_report();
 std::string autotune_report_bf16();

@@ -13,6 +13,8 @@
 // that lane l of an accumulator holds row l%16 of C (in 4 columns): a
 // wavefront's store then writes 16 consecutive doubles of each column, i.e.
 // 128-byte segments of the column-major output.
+#include <rocblas/rocblas.h>
+
 #include <atomic>
 #include <map>
 #include <mutex>
@@ -171,13 +173,57 @@ void launch(hipStream_t s, int M, int N, int K, double alpha, const double* A, l
     QK_HIP(hipGetLastError());
 }
 
-// Tile configurations: 0: 256 x 32, 1: 64 x 64, 2: 128 x 64, 3: 128 x 128.
-constexpr int NCFG = 4;
-const char* const CFG_NAME[NCFG] = {"256x32", "64x64", "128x64", "128x128"};
+// Tile configurations: 0: 256 x 32, 1: 64 x 64, 2: 128 x 64, 3: 128 x 128;
+// 4: the library's rocblas_dgemm (only where lib_ok allows it).
+constexpr int NOWN = 4, CFG_LIB = 4, NCFG = 5;
+const char* const CFG_NAME[NCFG] = {"256x32", "64x64", "128x64", "128x128", "rocblas"};
+
+// Library FP64 GEMM as a tuning candidate (qk_set_library_gemm, off by
+// default): plain products only (one matrix, all tiles), 32-bit indexable,
+// at least 512 in every dimension, and never inside a graph capture.  It sums
+// in its own order, so with it on, results can differ from the own kernels'
+// in the last bits.
+std::atomic<int> g_libgemm{0};
+struct ThreadBlas {
+    rocblas_handle h = nullptr;
+    ~ThreadBlas() {
+        if (h) (void)rocblas_destroy_handle(h);
+    }
+};
+thread_local ThreadBlas t_blas;
+
+bool lib_ok(hipStream_t s, int M, int N, int K, long long lda, long long ldb, long long ldc, const Batch& bt) {
+    const long long lim = (1LL << 31) - 1;
+    return g_libgemm.load() && bt.count == 1 && !bt.upper && M >= 512 && N >= 512 && K >= 512 &&
+           (long long)M * N < lim && (long long)M * K < lim && (long long)K * N < lim && lda < lim && ldb < lim &&
+           ldc < lim && !stream_capturing(s);
+}
+
+template <bool TA, bool TB>
+void launch_lib(hipStream_t s, int M, int N, int K, double alpha, const double* A, long long lda, const double* B,
+                long long ldb, double beta, double* C, long long ldc) {
+    if (!t_blas.h) {
+        rocblas_status st = rocblas_create_handle(&t_blas.h);
+        QK_CHECK(st == rocblas_status_success, QK_E_HIP, "rocblas_create_handle: status " + std::to_string((int)st));
+    }
+    rocblas_status st = rocblas_set_stream(t_blas.h, s);
+    QK_CHECK(st == rocblas_status_success, QK_E_HIP, "rocblas_set_stream: status " + std::to_string((int)st));
+    st = rocblas_dgemm(t_blas.h, TA ? rocblas_operation_transpose : rocblas_operation_none,
+                       TB ? rocblas_operation_transpose : rocblas_operation_none, M, N, K, &alpha, A, (int)lda, B,
+                       (int)ldb, &beta, C, (int)ldc);
+    QK_CHECK(st == rocblas_status_success, QK_E_HIP, "rocblas_dgemm: status " + std::to_string((int)st));
+}
 
 template <bool TA, bool TB>
 void launch_cfg(int cfg, hipStream_t s, int M, int N, int K, double alpha, const double* A, long long lda,
                 const double* B, long long ldb, double beta, double* C, long long ldc, const Batch& bt) {
+    if (cfg == CFG_LIB) {
+        if (lib_ok(s, M, N, K, lda, ldb, ldc, bt)) {
+            launch_lib<TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+            return;
+        }
+        cfg = 3;  // not allowed here (switched off meanwhile, or capturing): the largest own tile
+    }
     switch (cfg) {
         case 0: launch<4, 1, 4, 2, TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt); break;
         case 1: launch<2, 2, 2, 2, TA, TB>(s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt); break;
@@ -199,9 +245,10 @@ int default_cfg(int M, int N) {
 // configurations add the k-terms of an element in the same order, so the
 // tuning runs leave the same C and later calls give identical results.
 struct TuneKey {
-    int ta, tb, M, N, K, count, upper;
+    int ta, tb, M, N, K, count, upper, lib;
     bool operator<(const TuneKey& o) const {
-        return std::tie(ta, tb, M, N, K, count, upper) < std::tie(o.ta, o.tb, o.M, o.N, o.K, o.count, o.upper);
+        return std::tie(ta, tb, M, N, K, count, upper, lib) <
+               std::tie(o.ta, o.tb, o.M, o.N, o.K, o.count, o.upper, o.lib);
     }
 };
 struct TuneEntry {
@@ -220,7 +267,8 @@ int dispatch(hipStream_t s, int M, int N, int K, double alpha, const double* A, 
         launch_cfg<TA, TB>(default_cfg(M, N), s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
         return 1;
     }
-    TuneKey key{TA, TB, M, N, K, bt.count, bt.upper};
+    const bool lib = lib_ok(s, M, N, K, lda, ldb, ldc, bt);
+    TuneKey key{TA, TB, M, N, K, bt.count, bt.upper, lib ? 1 : 0};
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
@@ -231,12 +279,14 @@ int dispatch(hipStream_t s, int M, int N, int K, double alpha, const double* A, 
         launch_cfg<TA, TB>(default_cfg(M, N), s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
         return 1;
     }
-    TuneEntry e{default_cfg(M, N), {0.f, 0.f, 0.f, 0.f}};
+    TuneEntry e{default_cfg(M, N), {0.f, 0.f, 0.f, 0.f, 0.f}};
     hipEvent_t ev[2] = {nullptr, nullptr};
     QK_HIP(hipEventCreate(&ev[0]));
     QK_HIP(hipEventCreate(&ev[1]));
     float best = 1e30f;
-    for (int c = 0; c < NCFG; ++c) {
+    const int ncand = lib ? NCFG : NOWN;
+    for (int c = 0; c < NCFG; ++c) e.ms[c] = 0.f;
+    for (int c = 0; c < ncand; ++c) {
         (void)hipEventRecord(ev[0], s);
         launch_cfg<TA, TB>(c, s, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, bt);
         (void)hipEventRecord(ev[1], s);
@@ -252,12 +302,13 @@ int dispatch(hipStream_t s, int M, int N, int K, double alpha, const double* A, 
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     g_tune.emplace(key, e);
-    return NCFG;
+    return ncand;
 }
 
 }  // namespace
 
 void set_autotune(bool on) { g_autotune.store(on ? 1 : 0); }
+void set_library_gemm(bool on) { g_libgemm.store(on ? 1 : 0); }
 bool autotune_on() { return g_autotune.load() != 0; }
 
 std::string autotune_report() {
@@ -269,7 +320,7 @@ std::string autotune_report() {
         std::snprintf(buf, sizeof buf, "dgemm %c%c M=%d N=%d K=%d batch=%d%s -> %s (ms:", k.ta ? 'T' : 'N',
                       k.tb ? 'T' : 'N', k.M, k.N, k.K, k.count, k.upper ? " upper" : "", CFG_NAME[kv.second.cfg]);
         r += buf;
-        for (int c = 0; c < NCFG; ++c) {
+        for (int c = 0; c < (k.lib ? NCFG : NOWN); ++c) {
             std::snprintf(buf, sizeof buf, " %s %.3f", CFG_NAME[c], kv.second.ms[c]);
             r += buf;
         }

@@ -1795,6 +1795,11 @@ int qk_set_autotune(int on) {
     return QK_OK;
 }
 
+int qk_set_library_gemm(int on) {
+    qk::set_library_gemm(on != 0);
+    return QK_OK;
+}
+
 int qk_autotune_report(char* buf, int len) {
     if (!buf || len <= 0) return QK_E_ARG;
     std::string r = qk::autotune_report() + qk::autotune_report_bf16();

@@ -34,7 +34,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms",
            "qk_rgw_qp_graph_noalloc", "qk_noalloc_last_error", "qk_gw_qp_iters",
            "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats", "qk_gwb_set_graphs", "qk_gwb_graph_stats", "qk_set_split_aotomo",
-           "qk_qsgw_create", "qk_qsgw_cycle", "qk_qsgw_destroy", "qk_qsgw_last_error", "qk_rgw_phbse_dyn"]
+           "qk_qsgw_create", "qk_qsgw_cycle", "qk_qsgw_destroy", "qk_qsgw_last_error", "qk_rgw_phbse_dyn", "qk_set_library_gemm"]
 
 
 EIG_MODES = {"syevd": 0, "refine": 1, "sytrd": 2, "refine+sytrd": 3}
@@ -135,6 +135,7 @@ def load():
     lib.qk_rgw_qp_graph_noalloc.argtypes = [_I, _D, _D] + [_I] * 6 + [_H] * 8
     lib.qk_noalloc_last_error.restype = ctypes.c_char_p
     lib.qk_rgw_phbse.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 5
+    lib.qk_set_library_gemm.argtypes = [_I]
     lib.qk_rgw_phbse_dyn.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 7
     lib.qk_gw_sigma_full.argtypes = [_H, _P, _D, _I, _P, _P, _P, _P]
     lib.qk_ug0w0.argtypes = [_I, _IP, _IP, _IP, _P, _P, _P, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
@@ -628,6 +629,12 @@ def RGW_phBSE(ERI, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, singlet=True, t
         if triplet:
             dyn["triplet"] = (Od[10:10 + m].copy(), Zd[10:10 + m].copy())
     return {"EcBSE": Ec, "OmS": Om[:nS].copy(), "OmT": Om[nS:].copy(), "dyn": dyn}
+
+
+def set_library_gemm(on):
+    """rocBLAS's dgemm as an autotuner candidate for large plain FP64 products
+    (process-wide, off by default; results can then differ in the last bits)."""
+    _check(load().qk_set_library_gemm(int(bool(on))))
 
 
 def set_split_aotomo(on):

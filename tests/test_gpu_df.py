@@ -157,3 +157,31 @@ def test_autotuner_identical_results():
     rep = gw.autotune_report()
     say(*rep)
     assert any(l.startswith("dgemm") for l in rep) and any(l.startswith("gemm_bf16x3") for l in rep)
+
+
+def test_library_gemm_candidate():
+    """rocBLAS's dgemm as an autotuner candidate (qk_set_library_gemm): the
+    large plain FP64 products of the DF path (nS = 5184) time it beside the
+    own tiles, and the results stay within rounding of the default (the
+    library sums in its own order)."""
+    import torch
+
+    from quack_amd import dfgen, gw
+    say("library gemm candidate")
+    n, nO, naux = 240, 24, 720
+    e, Bt, _ = dfgen.synthetic_df(n, nO, naux, seed=5, device="cuda")
+    out = {}
+    try:
+        with gw.GWEngine(n, n, nO) as g:
+            g.set_precision("fp64")
+            for lib in (False, True):
+                gw.set_library_gemm(lib)
+                g.set_eri_df_device(Bt.data_ptr(), naux)
+                out[lib] = g.g0w0(e, doSRG=True)
+    finally:
+        gw.set_library_gemm(False)
+    torch.cuda.synchronize()
+    assert np.abs(out[True]["eGW"] - out[False]["eGW"]).max() < 1e-10
+    assert np.abs(out[True]["Om"] - out[False]["Om"]).max() < 1e-10
+    rep = gw.autotune_report()
+    assert any("rocblas" in l for l in rep if l.startswith("dgemm")), rep
