@@ -226,6 +226,12 @@ constexpr int EIG_MAXIT = 8;       // Newton steps before falling back to dsyevd
 // later cycles converge in 3-5 steps.
 constexpr double EIG_BIG = 0.1;
 constexpr double EIG_TOL = 1e-8;   // last step: the next correction would be O(1e-16)
+// Split-precision forms, whose vectors enter only an FP32-level contraction:
+// the last step is the one with max|E| below this; its update X (I + E) leaves
+// the vectors within O(1e-10), and its Rayleigh quotients S_ii/G_ii are
+// already exact to O(1e-10) relative (second order in E), one Newton step
+// (three of its four n^3 products) earlier than EIG_TOL.
+constexpr double EIG_TOL_SPLIT = 1e-5;
 constexpr double EIG_DEG = 1e-9;   // eigenvalues closer than this (relative) form a cluster
 constexpr int EIG_KMAX = 32;       // largest cluster handled
 // QK_EIG_TRACE=1 prints the largest correction of every refinement step to stderr
@@ -498,12 +504,11 @@ bool refine_eig(Ctx* c) {
         if (eig_trace())
             std::fprintf(stderr, "refine: n %d step %d max|E| %.3e clusters %d largest %d\n", n, it, emax, cl.ncl,
                          cl.biggest);
-        ok = emax < EIG_TOL;
-        // The last correction is below 1e-8.  The eigenvalues S_ii/G_ii of this
-        // step are already exact to O(1e-16); in the split-precision form the
-        // vectors are used only through the FP32-level contraction, so the
-        // final update X (I + E) is skipped there.
-        if (ok && c->precision != 0) break;
+        // FP64 form: the last correction is below 1e-8, the eigenvalues S_ii/G_ii
+        // of this step are exact to O(1e-16), and the final X (I + E) is applied.
+        // Split-precision forms: the last correction is below EIG_TOL_SPLIT and
+        // the final X (I + E) is applied too.
+        ok = emax < (c->precision != 0 ? EIG_TOL_SPLIT : EIG_TOL);
         eig_update(s, c->Sm, c->Gm, c->lam, d_lab, n);
         dgemm(s, false, false, n, n, n, 1.0, X, n, c->Sm, n, 0.0, Y, n);
         std::swap(X, Y);
