@@ -265,6 +265,22 @@ int qk_evugw(int nBas, const int *nC, const int *nO, const int *nR, const double
              int linearize, int TDA_W, int doSRG, double *eGW, double *SigC, double *Z, double *Om, double *EcRPA,
              double *EcGM, int *nSCF_out);
 
+/* Unrestricted static phBSE@GW (GW/UGW_phBSE.f90:60-196 with the static
+ * kernels GW/UGW_phBSE_static_kernel_A.f90 and _B.f90): spin-conserved dRPA
+ * screening at eW, its excitation densities, then the spin-conserved
+ * (spin_conserved != 0; LR/phULR_A.f90, _B.f90 ispin 1 with dRPA = false)
+ * and spin-flip (spin_flip != 0; ispin 2) BSE matrices at eGW plus the
+ * kernels (FP64 matrix-core products over the RPA modes), each solved by
+ * phULR (TDA for the BSE as asked).  Orbital counts and integrals as
+ * qk_ug0w0; eW, eGW: nBas x 2.  Outputs: EcBSE[2] (spin-conserved,
+ * spin-flip; halved with exchange_kernel, :189-193), OmSC[nS_aa + nS_bb] and
+ * OmSF[nS_ab + nS_ba] (nS_ab = (nO_a - nC_a)(nBas - nR_b - nO_b), nS_ba the
+ * same with the spins exchanged), each ignored for a manifold not asked.  The
+ * dynamical correction (dBSE) and oscillator strengths are not computed. */
+int qk_ugw_phbse(int nBas, const int *nC, const int *nO, const int *nR, const double *ERI_aaaa, const double *ERI_aabb,
+                 const double *ERI_bbbb, const double *eW, const double *eGW, double eta, int TDA_W, int TDA,
+                 int spin_conserved, int spin_flip, int exchange_kernel, double *EcBSE, double *OmSC, double *OmSF);
+
 /* Eigensolver of the unrestricted screening of qk_ug0w0 and qk_evugw,
  * process-wide: mode and de_max as qk_gw_set_eigensolver (0: rocSOLVER dsyevd
  * for every RPA, QuAcK's sequence and the default; 1: Newton refinement of the

@@ -675,13 +675,33 @@ def _ublocks(n, nO, nC=(0, 0), nR=(0, 0)):
     return (oa, va, ob, vb, (nO[0] - nC[0]) * (n - nR[0] - nO[0]), (nO[1] - nC[1]) * (n - nR[1] - nO[1]))
 
 
-def phULR_A(e, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True, nC=(0, 0), nR=(0, 0)):
-    """LR/phULR_A.f90:26-80 (spin-conserving, ispin 1); pairs ia with i, a
-    active in their spin, a fastest."""
+def _sf_blocks(n, nO, nC=(0, 0), nR=(0, 0)):
+    """Spin-flip pairs (LR/phULR_A.f90:120-168): alpha-occupied to
+    beta-virtual (nS_ab of them), then beta-occupied to alpha-virtual."""
+    oa, ob = slice(nC[0], nO[0]), slice(nC[1], nO[1])
+    va, vb = slice(nO[0], n - nR[0]), slice(nO[1], n - nR[1])
+    return (oa, vb, ob, va, (nO[0] - nC[0]) * (n - nR[1] - nO[1]), (nO[1] - nC[1]) * (n - nR[0] - nO[0]))
+
+
+def phULR_A(e, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True, nC=(0, 0), nR=(0, 0), ispin=1):
+    """LR/phULR_A.f90:26-170: ispin 1 spin-conserving (alpha pairs, then
+    beta), ispin 2 spin-flip (alpha-to-beta pairs, then beta-to-alpha); pairs
+    ia with i, a active in their spin, a fastest."""
     n = e.shape[0]
     Ga, Gm, Gb = G4(eri_aa, n), G4(eri_ab, n), G4(eri_bb, n)
-    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO, nC, nR)
     dr = 1.0 if dRPA else 0.0
+    if ispin == 2:
+        oa, vb, ob, va, nab, nba = _sf_blocks(n, nO, nC, nR)
+        A = np.zeros((nab + nba, nab + nba))
+        # alpha i -> beta a: delta (e_a(b) - e_i(a)) - (1-dRPA) lam <ib|ja>_aabb
+        A[:nab, :nab] = -(1 - dr) * lam * np.einsum("ibja->iajb", Gm[oa, vb, oa, vb]).reshape(nab, nab)
+        # beta i -> alpha a: - (1-dRPA) lam ERI_aabb(b,i,a,j)
+        A[nab:, nab:] = -(1 - dr) * lam * np.einsum("biaj->iajb", Gm[va, ob, va, ob]).reshape(nba, nba)
+        d = np.concatenate([(e[vb, 1][None, :] - e[oa, 0][:, None]).reshape(-1),
+                            (e[va, 0][None, :] - e[ob, 1][:, None]).reshape(-1)])
+        A[np.diag_indices(nab + nba)] += d
+        return A
+    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO, nC, nR)
     A = np.zeros((nSa + nSb, nSa + nSb))
     A[:nSa, :nSa] = lam * Ga[oa, va, va, oa].transpose(0, 2, 3, 1).reshape(nSa, nSa) \
         - (1 - dr) * lam * Ga[oa, va, oa, va].transpose(0, 3, 2, 1).reshape(nSa, nSa)
@@ -695,11 +715,20 @@ def phULR_A(e, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True, nC=(0, 0), nR=(0,
     return A
 
 
-def phULR_B(n, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True, nC=(0, 0), nR=(0, 0)):
-    """LR/phULR_B.f90:26-80 (spin-conserving, ispin 1)."""
+def phULR_B(n, eri_aa, eri_ab, eri_bb, nO, lam=1.0, dRPA=True, nC=(0, 0), nR=(0, 0), ispin=1):
+    """LR/phULR_B.f90:26-168: ispin 1 spin-conserving, ispin 2 spin-flip
+    (only the alpha-to-beta / beta-to-alpha coupling blocks)."""
     Ga, Gm, Gb = G4(eri_aa, n), G4(eri_ab, n), G4(eri_bb, n)
-    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO, nC, nR)
     dr = 1.0 if dRPA else 0.0
+    if ispin == 2:
+        oa, vb, ob, va, nab, nba = _sf_blocks(n, nO, nC, nR)
+        B = np.zeros((nab + nba, nab + nba))
+        # (alpha i -> beta a, beta j -> alpha b): - (1-dRPA) lam ERI_aabb(i,j,b,a)
+        B[:nab, nab:] = -(1 - dr) * lam * np.einsum("ijba->iajb", Gm[oa, ob, va, vb]).reshape(nab, nba)
+        # (beta i -> alpha a, alpha j -> beta b): - (1-dRPA) lam ERI_aabb(j,i,a,b)
+        B[nab:, :nab] = -(1 - dr) * lam * np.einsum("jiab->iajb", Gm[oa, ob, va, vb]).reshape(nba, nab)
+        return B
+    oa, va, ob, vb, nSa, nSb = _ublocks(n, nO, nC, nR)
     B = np.zeros((nSa + nSb, nSa + nSb))
     B[:nSa, :nSa] = lam * Ga[oa, oa, va, va].transpose(0, 2, 1, 3).reshape(nSa, nSa) \
         - (1 - dr) * lam * Ga[oa, oa, va, va].transpose(0, 3, 1, 2).reshape(nSa, nSa)
@@ -726,6 +755,80 @@ def ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO, nC=(0, 0), nR=(0,
     rho[:, :, :, 1] = np.einsum("jpbq,mjb->pqm", Gm[oa, :, va, :], Xa, optimize=True) \
         + np.einsum("pjqb,mjb->pqm", Gb[:, ob, :, vb], Xb, optimize=True)
     return rho
+
+
+def ugw_phbse_static_kernel_A(ispin, eta, Om, rho, nO, nC=(0, 0), nR=(0, 0), lam=1.0):
+    """GW/UGW_phBSE_static_kernel_A.f90:36-151: 2 lam^2 sum_m rho_s(i,j,m)
+    rho_s'(a,b,m) Om_m/(Om_m^2+eta^2), on the aaaa and bbbb blocks (ispin 1;
+    no coupling block) or the abab and baba blocks (ispin 2)."""
+    n = rho.shape[0]
+    w = Om / (Om ** 2 + eta ** 2)
+    if ispin == 1:
+        oa, va, ob, vb, na, nb = _ublocks(n, nO, nC, nR)
+        blocks = ((oa, va, 0, 0), (ob, vb, 1, 1))
+    else:
+        oa, vb, ob, va, na, nb = _sf_blocks(n, nO, nC, nR)
+        blocks = ((oa, vb, 0, 1), (ob, va, 1, 0))
+    K = np.zeros((na + nb, na + nb))
+    for (o, v, so, sv), off, m in zip(blocks, (0, na), (na, nb)):
+        k = np.einsum("ijm,abm,m->iajb", rho[o, o, :, so], rho[v, v, :, sv], w, optimize=True).reshape(m, m)
+        K[off:off + m, off:off + m] = 2.0 * lam ** 2 * k
+    return K
+
+
+def ugw_phbse_static_kernel_B(ispin, eta, Om, rho, nO, nC=(0, 0), nR=(0, 0), lam=1.0):
+    """GW/UGW_phBSE_static_kernel_B.f90:36-153: 2 lam^2 sum_m rho(i,b,m)
+    rho(a,j,m) Om_m/(Om_m^2+eta^2) on the aaaa and bbbb blocks (ispin 1) or
+    the two coupling blocks of the spin-flip pairs (ispin 2)."""
+    n = rho.shape[0]
+    w = Om / (Om ** 2 + eta ** 2)
+    if ispin == 1:
+        oa, va, ob, vb, na, nb = _ublocks(n, nO, nC, nR)
+        K = np.zeros((na + nb, na + nb))
+        for o, v, sp, off, m in ((oa, va, 0, 0, na), (ob, vb, 1, na, nb)):
+            k = np.einsum("ibm,ajm,m->iajb", rho[o, v, :, sp], rho[v, o, :, sp], w, optimize=True).reshape(m, m)
+            K[off:off + m, off:off + m] = 2.0 * lam ** 2 * k
+        return K
+    oa, vb, ob, va, na, nb = _sf_blocks(n, nO, nC, nR)
+    K = np.zeros((na + nb, na + nb))
+    # (alpha i -> beta a, beta j -> alpha b): rho_a(i,b) rho_b(a,j)
+    K[:na, na:] = 2.0 * lam ** 2 * np.einsum("ibm,ajm,m->iajb", rho[oa, va, :, 0], rho[vb, ob, :, 1], w,
+                                              optimize=True).reshape(na, nb)
+    # (beta i -> alpha a, alpha j -> beta b): rho_b(i,b) rho_a(a,j)
+    K[na:, :na] = 2.0 * lam ** 2 * np.einsum("ibm,ajm,m->iajb", rho[ob, vb, :, 1], rho[va, oa, :, 0], w,
+                                              optimize=True).reshape(nb, na)
+    return K
+
+
+def ugw_phbse(eri_aa, eri_ab, eri_bb, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, spin_conserved=True,
+              spin_flip=True, exchange_kernel=False, nC=(0, 0), nR=(0, 0)):
+    """GW/UGW_phBSE.f90:60-196 with the static kernel (no dynamical
+    correction): spin-conserved dRPA screening at eW, its excitation
+    densities, then the spin-conserved (ispin 1) and spin-flip (ispin 2) BSE
+    at eGW (phULR_A/B with dRPA = false) plus the static kernels, each solved
+    by phULR.  eW, eGW: n x 2.  Returns EcBSE(2) (halved with
+    exchange_kernel), the spin-conserved and the spin-flip excitation energies."""
+    n = eW.shape[0]
+    A = phULR_A(eW, eri_aa, eri_ab, eri_bb, nO, nC=nC, nR=nR)
+    B = np.zeros_like(A) if TDA_W else phULR_B(n, eri_aa, eri_ab, eri_bb, nO, nC=nC, nR=nR)
+    _, OmRPA, XpY, _ = phRLR(TDA_W, A, B)
+    rho = ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO, nC, nR)
+    EcBSE = np.zeros(2)
+    Om = {1: None, 2: None}
+    for ispin, on in ((1, spin_conserved), (2, spin_flip)):
+        if not on:
+            continue
+        A = phULR_A(eGW, eri_aa, eri_ab, eri_bb, nO, dRPA=False, nC=nC, nR=nR, ispin=ispin)
+        A = A + ugw_phbse_static_kernel_A(ispin, eta, OmRPA, rho, nO, nC, nR)
+        if TDA:
+            B = np.zeros_like(A)
+        else:
+            B = phULR_B(n, eri_aa, eri_ab, eri_bb, nO, dRPA=False, nC=nC, nR=nR, ispin=ispin)
+            B = B + ugw_phbse_static_kernel_B(ispin, eta, OmRPA, rho, nO, nC, nR)
+        EcBSE[ispin - 1], Om[ispin], _, _ = phRLR(TDA, A, B)
+    if exchange_kernel:
+        EcBSE *= 0.5
+    return EcBSE, Om[1], Om[2]
 
 
 def ug0w0(eri_aa, eri_ab, eri_bb, eHF, nO, eta=0.0, linearize=False, TDA_W=False, doSRG=False, nC=(0, 0),

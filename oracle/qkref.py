@@ -43,6 +43,7 @@ def load():
         lib.qkref_evrgw.argtypes = [_I, _I, _I, _I, _D, _I, _D] + [_I] * 6 + [_D, _D] + [_P] * 7 + [_IP]
         lib.qkref_evrgw.restype = _I
         lib.qkref_phbse.argtypes = [_I] * 4 + [_D] + [_I] * 6 + [_P] * 6
+        lib.qkref_uphbse.argtypes = [_I] * 4 + [_D] + [_I] * 7 + [_P] * 8
         lib.qkref_phbse_full.argtypes = [_I] * 7 + [_D] + [_I] * 6 + [_P] * 8
         lib.qkref_self_energy_full.argtypes = [_I, _D] + [_I] * 6 + [_P] * 6
         lib.qkref_qsgw_cycle.argtypes = [_I, _I, _D] + [_I] * 6 + [_P] * 10
@@ -178,6 +179,21 @@ def phbse_full(eri_mo, eW, eGW, nO, eta=0.0, tda_w=False, tda=False, singlet=Tru
     m = min(nS, 10)
     return {"EcBSE": Ec, "OmS": OmS, "OmT": OmT, "dyn": {"singlet": (Od[:m], Zd[:m]), "triplet": (Od[10:10 + m],
                                                                                                  Zd[10:10 + m])}}
+
+
+def uphbse(eri_aa, eri_ab, eri_bb, eW, eGW, nO, eta=0.0, tda_w=False, tda=False, spin_conserved=True,
+           spin_flip=True, nC=(0, 0), nR=(0, 0)):
+    """Unrestricted static phBSE@GW through QuAcK's own routines (ref_driver.f90
+    qkref_uphbse, the order of GW/UGW_phBSE.f90:60-196); eW, eGW n x 2."""
+    n = eW.shape[0]
+    nSsc = sum((nO[s] - nC[s]) * (n - nR[s] - nO[s]) for s in range(2))
+    nSsf = (nO[0] - nC[0]) * (n - nR[1] - nO[1]) + (nO[1] - nC[1]) * (n - nR[0] - nO[0])
+    Ec = np.zeros(2)
+    OmSC, OmSF = np.zeros(nSsc), np.zeros(nSsf)
+    load().qkref_uphbse(int(tda_w), int(tda), int(spin_conserved), int(spin_flip), eta, n, nO[0], nO[1], nC[0], nC[1],
+                        nR[0], nR[1], _p(_eri(eri_aa)), _p(_eri(eri_ab)), _p(_eri(eri_bb)), _p(_f(eW)),
+                        _p(_f(eGW)), _p(Ec), _p(OmSC), _p(OmSF))
+    return {"EcBSE": Ec, "OmSC": OmSC, "OmSF": OmSF}
 
 
 def self_energy_full(e, Om, rho, nO, eta=0.0, srg=False, nC=0, nR=0):

@@ -440,6 +440,81 @@ subroutine qkref_ug0w0(lin, tda_w, srg, eta, nBas, nOa, nOb, nCa, nCb, nRa, nRb,
   deallocate(Aph, Bph, Om, XpY, XmY, rho, eGWlin)
 end subroutine
 
+! GW/UGW_phBSE.f90:60-196 with the static kernel (no dynamical correction, no
+! transition vectors), composed of QuAcK's own routines in the same order:
+! spin-conserved dRPA screening at eW, its excitation densities, then the
+! spin-conserved (ispin 1) and spin-flip (ispin 2) BSE at eGW with dRPA =
+! false plus the static kernels, each solved by phULR.
+subroutine qkref_uphbse(tda_w, tda, dosc, dosf, eta, nBas, nOa, nOb, nCa, nCb, nRa, nRb, ERI_aaaa, ERI_aabb, &
+                        ERI_bbbb, eW, eGW, EcBSE, OmSC, OmSF) bind(C, name='qkref_uphbse')
+  use iso_c_binding
+  implicit none
+  integer(c_int), value :: tda_w, tda, dosc, dosf, nBas, nOa, nOb, nCa, nCb, nRa, nRb
+  real(c_double), value :: eta
+  real(c_double), intent(in) :: ERI_aaaa(nBas,nBas,nBas,nBas), ERI_aabb(nBas,nBas,nBas,nBas)
+  real(c_double), intent(in) :: ERI_bbbb(nBas,nBas,nBas,nBas), eW(nBas,2), eGW(nBas,2)
+  real(c_double), intent(out) :: EcBSE(2), OmSC(*), OmSF(*)
+  integer :: nC(2), nO(2), nV(2), nR(2), nS(2), nS_aa, nS_bb, nS_sc, nS_ab, nS_ba, nS_sf
+  double precision, allocatable :: Aph(:,:), Bph(:,:), KA(:,:), KB(:,:), OmRPA(:), XpY(:,:), XmY(:,:), rho(:,:,:,:)
+  double precision, allocatable :: Om(:)
+  double precision :: EcRPA
+  logical :: TDAW, TDAB
+  TDAW = tda_w /= 0
+  TDAB = tda /= 0
+  nC(1) = nCa; nC(2) = nCb; nR(1) = nRa; nR(2) = nRb
+  nO(1) = nOa; nO(2) = nOb
+  nV = nBas - nO
+  nS = (nO - nC)*(nV - nR)
+  nS_aa = nS(1); nS_bb = nS(2); nS_sc = nS_aa + nS_bb
+  nS_ab = (nO(1) - nC(1))*(nV(2) - nR(2))
+  nS_ba = (nO(2) - nC(2))*(nV(1) - nR(1))
+  nS_sf = nS_ab + nS_ba
+  EcBSE = 0d0
+  allocate(Aph(nS_sc,nS_sc), Bph(nS_sc,nS_sc), OmRPA(nS_sc), XpY(nS_sc,nS_sc), XmY(nS_sc,nS_sc), &
+           rho(nBas,nBas,nS_sc,2))
+  Bph = 0d0
+  call phULR_A(1, .true., nBas, nC, nO, nV, nR, nS_aa, nS_bb, nS_sc, 1d0, eW, ERI_aaaa, ERI_aabb, ERI_bbbb, Aph)
+  if(.not. TDAW) call phULR_B(1, .true., nBas, nC, nO, nV, nR, nS_aa, nS_bb, nS_sc, 1d0, ERI_aaaa, ERI_aabb, &
+                              ERI_bbbb, Bph)
+  call phULR(TDAW, nS_aa, nS_bb, nS_sc, Aph, Bph, EcRPA, OmRPA, XpY, XmY)
+  call UGW_excitation_density(nBas, nC, nO, nR, nS_aa, nS_bb, nS_sc, ERI_aaaa, ERI_aabb, ERI_bbbb, XpY, rho)
+  deallocate(Aph, Bph, XpY, XmY)
+  if(dosc /= 0) then
+    allocate(Aph(nS_sc,nS_sc), Bph(nS_sc,nS_sc), KA(nS_sc,nS_sc), KB(nS_sc,nS_sc), Om(nS_sc), XpY(nS_sc,nS_sc), &
+             XmY(nS_sc,nS_sc))
+    Bph = 0d0; KB = 0d0
+    call phULR_A(1, .false., nBas, nC, nO, nV, nR, nS_aa, nS_bb, nS_sc, 1d0, eGW, ERI_aaaa, ERI_aabb, ERI_bbbb, Aph)
+    if(.not. TDAB) call phULR_B(1, .false., nBas, nC, nO, nV, nR, nS_aa, nS_bb, nS_sc, 1d0, ERI_aaaa, ERI_aabb, &
+                                ERI_bbbb, Bph)
+    call UGW_phBSE_static_kernel_A(1, eta, nBas, nC, nO, nV, nR, nS_aa, nS_bb, nS_sc, nS_sc, 1d0, OmRPA, rho, KA)
+    if(.not. TDAB) call UGW_phBSE_static_kernel_B(1, eta, nBas, nC, nO, nV, nR, nS_aa, nS_bb, nS_sc, nS_sc, 1d0, &
+                                                  OmRPA, rho, KB)
+    Aph = Aph + KA
+    if(.not. TDAB) Bph = Bph + KB
+    call phULR(TDAB, nS_aa, nS_bb, nS_sc, Aph, Bph, EcBSE(1), Om, XpY, XmY)
+    OmSC(1:nS_sc) = Om
+    deallocate(Aph, Bph, KA, KB, Om, XpY, XmY)
+  end if
+  if(dosf /= 0) then
+    allocate(Aph(nS_sf,nS_sf), Bph(nS_sf,nS_sf), KA(nS_sf,nS_sf), KB(nS_sf,nS_sf), Om(nS_sf), XpY(nS_sf,nS_sf), &
+             XmY(nS_sf,nS_sf))
+    Bph = 0d0; KB = 0d0
+    call phULR_A(2, .false., nBas, nC, nO, nV, nR, nS_ab, nS_ba, nS_sf, 1d0, eGW, ERI_aaaa, ERI_aabb, ERI_bbbb, Aph)
+    if(.not. TDAB) call phULR_B(2, .false., nBas, nC, nO, nV, nR, nS_ab, nS_ba, nS_sf, 1d0, ERI_aaaa, ERI_aabb, &
+                                ERI_bbbb, Bph)
+    call UGW_phBSE_static_kernel_A(2, eta, nBas, nC, nO, nV, nR, nS_ab, nS_ba, nS_sf, nS_sc, 1d0, OmRPA, rho, KA)
+    if(.not. TDAB) call UGW_phBSE_static_kernel_B(2, eta, nBas, nC, nO, nV, nR, nS_ab, nS_ba, nS_sf, nS_sc, 1d0, &
+                                                  OmRPA, rho, KB)
+    Aph = Aph + KA
+    if(.not. TDAB) Bph = Bph + KB
+    call phULR(TDAB, nS_ab, nS_ab, nS_sf, Aph, Bph, EcBSE(2), Om, XpY, XmY)
+    OmSF(1:nS_sf) = Om
+    deallocate(Aph, Bph, KA, KB, Om, XpY, XmY)
+  end if
+  flush(6)
+  deallocate(OmRPA, rho)
+end subroutine
+
 ! evUGW's cycle (GW/evUGW.f90:103-215) composed of QuAcK's own routines, with
 ! TDA = TDA_W for the B build and the printing left out
 subroutine qkref_evugw(maxSCF, thresh, max_diis, lin, tda_w, srg, eta, nBas, nOa, nOb, nCa, nCb, nRa, nRb, &

@@ -274,6 +274,33 @@ struct UScreen {
             double* z) {
         qp_dev(st, *q, dR[sp], c->Om, doSRG, eta, SRG_FLOW, nC[sp], nO[sp], nR[sp], nSt, eHF_s, eLin, eOld, eGW_s, z);
     }
+    // One block of the static BSE kernels (GW/UGW_phBSE_static_kernel_A.f90,
+    // _B.f90) from the last densities():
+    //   G[(x + X1 x2) + X1 X2 (y + Y1 y2)] = 4 sum_m w_m rhoX(px0 + x, qx0 + x2, m) rhoY(py0 + y, qy0 + y2, m)
+    // with w_m = Om_m/(Om_m^2 + eta^2) and rho_s = sqrt(2) dR[s] (the unrestricted
+    // kernels carry 2 rho rho), as one FP64 matrix-core product over the modes.
+    std::vector<double> kernel_block(double eta, int sx, int px0, int X1, int qx0, int X2, int sy, int py0, int Y1,
+                                     int qy0, int Y2) {
+        const size_t nx = (size_t)X1 * X2, ny = (size_t)Y1 * Y2;
+        std::vector<double> h(nx * ny);
+        double *X = nullptr, *Y = nullptr, *G = nullptr;
+        try {
+            X = dalloc<double>((size_t)nSt * nx);
+            Y = dalloc<double>((size_t)nSt * ny);
+            G = dalloc<double>(nx * ny);
+            rho_block(st, dR[sx], c->Om, eta, 1, n, nSt, px0, X1, qx0, X2, X);
+            rho_block(st, dR[sy], c->Om, eta, 0, n, nSt, py0, Y1, qy0, Y2, Y);
+            dgemm(st, true, false, (int)nx, (int)ny, nSt, 4.0, X, nSt, Y, nSt, 0.0, G, (long long)nx);
+            QK_HIP(hipMemcpyAsync(h.data(), G, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+            QK_HIP(hipStreamSynchronize(st));
+        } catch (...) {
+            (void)hipStreamSynchronize(st);
+            dfree(X); dfree(Y); dfree(G);
+            throw;
+        }
+        dfree(X); dfree(Y); dfree(G);
+        return h;
+    }
 };
 }  // namespace
 }  // namespace qk
@@ -395,6 +422,137 @@ int qk_evugw(int nBas, const int* nC, const int* nO, const int* nR, const double
         }
     });
     return rc != QK_OK ? rc : conv_status;
+}
+
+int qk_ugw_phbse(int nBas, const int* nC, const int* nO, const int* nR, const double* ERI_aaaa, const double* ERI_aabb,
+                 const double* ERI_bbbb, const double* eW, const double* eGW, double eta, int TDA_W, int TDA,
+                 int spin_conserved, int spin_flip, int exchange_kernel, double* EcBSE, double* OmSC, double* OmSF) {
+    if (!upairs_ok(nBas, nC, nO, nR) || !ERI_aaaa || !ERI_aabb || !ERI_bbbb || !eW || !eGW || !EcBSE ||
+        (spin_conserved && !OmSC) || (spin_flip && !OmSF))
+        return QK_E_ARG;
+    return guarded(nullptr, [&] {
+        const int n = nBas;
+        UScreen u(n, nC, nO, nR, ERI_aaaa, ERI_aabb, ERI_bbbb, TDA_W);
+        // GW/UGW_phBSE.f90:90-100: spin-conserved dRPA screening at eW and its densities
+        u.rpa(eW, "qk_ugw_phbse");
+        u.densities();
+        const double* Gk[3] = {ERI_aaaa, ERI_aabb, ERI_bbbb};
+        auto G = [&](int k, int p, int q, int r, int t) {  // ERI_k(p, q, r, s), 0-based
+            return Gk[k][p + (size_t)n * (q + (size_t)n * (r + (size_t)n * t))];
+        };
+        const double* eg[2] = {eGW, eGW + n};
+        const int o[2] = {nO[0] - nC[0], nO[1] - nC[1]};
+        const int v[2] = {n - nR[0] - nO[0], n - nR[1] - nO[1]};
+        EcBSE[0] = EcBSE[1] = 0.0;
+        // GW/UGW_phBSE.f90:108-140 (ispin 1) and :150-182 (ispin 2): the BSE at eGW
+        // (phULR_A/B with dRPA = false, LR/phULR_A.f90, LR/phULR_B.f90) plus the
+        // static kernels, each solved by phULR (the general solver of qk_phrlr)
+        for (int ispin = 1; ispin <= 2; ++ispin) {
+            if ((ispin == 1 && !spin_conserved) || (ispin == 2 && !spin_flip)) continue;
+            // pair blocks: (occupied spin, virtual spin) of the first and second block
+            const int so[2] = {0, 1};
+            const int sv[2] = {ispin == 1 ? 0 : 1, ispin == 1 ? 1 : 0};
+            const int nb0 = o[so[0]] * v[sv[0]], nb1 = o[so[1]] * v[sv[1]], N = nb0 + nb1;
+            const int off[2] = {0, nb0};
+            std::vector<double> A((size_t)N * N, 0.0), B((size_t)N * N, 0.0), XpY((size_t)N * N), XmY((size_t)N * N);
+            for (int bI = 0; bI < 2; ++bI) {
+                const int si = so[bI], sa = sv[bI], oi = o[si], va = v[sa];
+                // KA on the diagonal blocks: rho_si(i, j) rho_sa(a, b)
+                std::vector<double> ka = u.kernel_block(eta, si, nC[si], oi, nC[si], oi, sa, nO[sa], va, nO[sa], va);
+                for (int i = 0; i < oi; ++i)
+                    for (int a = 0; a < va; ++a)
+                        for (int j = 0; j < oi; ++j)
+                            for (int b = 0; b < va; ++b) {
+                                const size_t ia = off[bI] + a + (size_t)va * i, jb = off[bI] + b + (size_t)va * j;
+                                const int I = nC[si] + i, J = nC[si] + j, Aa = nO[sa] + a, Bb = nO[sa] + b;
+                                double x = ka[(i + (size_t)oi * j) + (size_t)oi * oi * (a + (size_t)va * b)];
+                                if (ispin == 1) {
+                                    const int k = si == 0 ? 0 : 2;  // aaaa or bbbb
+                                    x += G(k, I, Bb, Aa, J) - G(k, I, Bb, J, Aa);
+                                } else {
+                                    x -= si == 0 ? G(1, I, Bb, J, Aa) : G(1, Bb, I, Aa, J);
+                                }
+                                if (i == j && a == b) x += eg[sa][Aa] - eg[si][I];
+                                A[ia + (size_t)N * jb] = x;
+                            }
+            }
+            if (ispin == 1) {
+                // the coupling blocks of A: lam ERI_aabb(i,b,a,j) and ERI_aabb(b,i,j,a)
+                for (int i = 0; i < o[0]; ++i)
+                    for (int a = 0; a < v[0]; ++a)
+                        for (int j = 0; j < o[1]; ++j)
+                            for (int b = 0; b < v[1]; ++b) {
+                                const size_t ia = a + (size_t)v[0] * i, jb = nb0 + b + (size_t)v[1] * j;
+                                const int I = nC[0] + i, Aa = nO[0] + a, J = nC[1] + j, Bb = nO[1] + b;
+                                A[ia + (size_t)N * jb] = G(1, I, Bb, Aa, J);
+                                A[jb + (size_t)N * ia] = G(1, Aa, J, I, Bb);
+                            }
+            }
+            if (!TDA) {
+                if (ispin == 1) {
+                    for (int bI = 0; bI < 2; ++bI) {
+                        const int sp = bI, oi = o[sp], va = v[sp];
+                        std::vector<double> kb =
+                            u.kernel_block(eta, sp, nC[sp], oi, nO[sp], va, sp, nO[sp], va, nC[sp], oi);
+                        const int k = sp == 0 ? 0 : 2;
+                        for (int i = 0; i < oi; ++i)
+                            for (int a = 0; a < va; ++a)
+                                for (int j = 0; j < oi; ++j)
+                                    for (int b = 0; b < va; ++b) {
+                                        const size_t ia = off[bI] + a + (size_t)va * i, jb = off[bI] + b + (size_t)va * j;
+                                        const int I = nC[sp] + i, J = nC[sp] + j, Aa = nO[sp] + a, Bb = nO[sp] + b;
+                                        B[ia + (size_t)N * jb] =
+                                            kb[(i + (size_t)oi * b) + (size_t)oi * va * (a + (size_t)va * j)] +
+                                            G(k, I, J, Aa, Bb) - G(k, I, J, Bb, Aa);
+                                    }
+                    }
+                    for (int i = 0; i < o[0]; ++i)
+                        for (int a = 0; a < v[0]; ++a)
+                            for (int j = 0; j < o[1]; ++j)
+                                for (int b = 0; b < v[1]; ++b) {
+                                    const size_t ia = a + (size_t)v[0] * i, jb = nb0 + b + (size_t)v[1] * j;
+                                    const int I = nC[0] + i, Aa = nO[0] + a, J = nC[1] + j, Bb = nO[1] + b;
+                                    B[ia + (size_t)N * jb] = G(1, I, J, Aa, Bb);
+                                    B[jb + (size_t)N * ia] = G(1, I, J, Aa, Bb);
+                                }
+                } else {
+                    // (alpha i -> beta a, beta j -> alpha b) and (beta i -> alpha a, alpha j -> beta b)
+                    std::vector<double> k01 =
+                        u.kernel_block(eta, 0, nC[0], o[0], nO[0], v[0], 1, nO[1], v[1], nC[1], o[1]);
+                    std::vector<double> k10 =
+                        u.kernel_block(eta, 1, nC[1], o[1], nO[1], v[1], 0, nO[0], v[0], nC[0], o[0]);
+                    for (int i = 0; i < o[0]; ++i)
+                        for (int a = 0; a < v[1]; ++a)
+                            for (int j = 0; j < o[1]; ++j)
+                                for (int b = 0; b < v[0]; ++b) {
+                                    const size_t ia = a + (size_t)v[1] * i, jb = nb0 + b + (size_t)v[0] * j;
+                                    const int I = nC[0] + i, Aa = nO[1] + a, J = nC[1] + j, Bb = nO[0] + b;
+                                    B[ia + (size_t)N * jb] =
+                                        k01[(i + (size_t)o[0] * b) + (size_t)o[0] * v[0] * (a + (size_t)v[1] * j)] -
+                                        G(1, I, J, Bb, Aa);
+                                }
+                    for (int i = 0; i < o[1]; ++i)
+                        for (int a = 0; a < v[0]; ++a)
+                            for (int j = 0; j < o[0]; ++j)
+                                for (int b = 0; b < v[1]; ++b) {
+                                    const size_t ia = nb0 + a + (size_t)v[0] * i, jb = b + (size_t)v[1] * j;
+                                    const int I = nC[1] + i, Aa = nO[0] + a, J = nC[0] + j, Bb = nO[1] + b;
+                                    B[ia + (size_t)N * jb] =
+                                        k10[(i + (size_t)o[1] * b) + (size_t)o[1] * v[1] * (a + (size_t)v[0] * j)] -
+                                        G(1, J, I, Aa, Bb);
+                                }
+                }
+            }
+            double ec = 0.0;
+            const int r = qk_phrlr(TDA, N, A.data(), B.data(), &ec, ispin == 1 ? OmSC : OmSF, XpY.data(), XmY.data());
+            QK_CHECK(r == QK_OK, r, "qk_ugw_phbse: phULR failed: " + g_last_error);
+            EcBSE[ispin - 1] = ec;
+        }
+        if (exchange_kernel) {  // GW/UGW_phBSE.f90:189-193
+            EcBSE[0] *= 0.5;
+            EcBSE[1] *= 0.5;
+        }
+    });
 }
 
 int qk_set_unrestricted_eigensolver(int mode, double de_max) {

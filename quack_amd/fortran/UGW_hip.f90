@@ -6,8 +6,9 @@
 ! Screening, excitation densities, self-energies and QP solves run in
 ! libquack_amd.so (qk_ug0w0, qk_evugw); the summaries and test dumps follow
 ! src/GW/print_UG0W0.f90:55-63, print_evUGW.f90:70-81, UG0W0.f90:227-233 and
-! evUGW.f90:279-285.  Frozen core and Rydberg orbitals (nC, nR per spin) run
-! on the GPU; BSE and ACFDT stay on QuAcK's CPU routines: requesting them here
+! evUGW.f90:279-285.  Frozen core and Rydberg orbitals (nC, nR per spin) and
+! the static phBSE after the GW step run on the GPU; the dynamical BSE
+! correction and ACFDT stay on QuAcK's CPU routines: requesting them here
 ! stops with a nonzero code.
 
 subroutine qk_print_utable(title, nBas, nO, eHF, SigC, Z, eGW)
@@ -23,11 +24,63 @@ subroutine qk_print_utable(title, nBas, nO, eHF, SigC, Z, eGW)
   end do
 end subroutine
 
-subroutine qk_ugw_checks(doACFDT, dophBSE)
+subroutine qk_ugw_checks(doACFDT, dophBSE, dBSE)
   implicit none
-  logical, intent(in) :: doACFDT, dophBSE
-  if(dophBSE) call qk_unsupported('phBSE@UGW')
+  logical, intent(in) :: doACFDT, dophBSE, dBSE
+  if(dophBSE .and. dBSE) call qk_unsupported('dynamical phBSE@UGW')
   if(doACFDT) call qk_unsupported('ACFDT@UGW')
+end subroutine
+
+! Static phBSE after an unrestricted GW calculation (qk_ugw_phbse,
+! GW/UGW_phBSE.f90:60-196) with QuAcK's output: the excitation tables of the
+! spin-conserved and spin-flip manifolds (:128,:170) and the Tr@BSE lines of
+! GW/UG0W0.f90:195-202 (label width 60) / GW/evUGW.f90:248-255 (width 50).
+subroutine qk_uphbse_print(tag, wid, exchange_kernel, TDA_W, TDA, spin_conserved, spin_flip, eta, nBas, nC, nO, nR, &
+                           ENuc, EUHF, ERI_aaaa, ERI_aabb, ERI_bbbb, eW, eGW)
+  use quack_amd_gw
+  implicit none
+  character(len=*), intent(in) :: tag
+  integer, intent(in) :: wid, nBas, nC(2), nO(2), nR(2)
+  logical, intent(in) :: exchange_kernel, TDA_W, TDA, spin_conserved, spin_flip
+  double precision, intent(in) :: eta, ENuc, EUHF, eW(nBas,2), eGW(nBas,2)
+  double precision, intent(in) :: ERI_aaaa(nBas,nBas,nBas,nBas), ERI_aabb(nBas,nBas,nBas,nBas)
+  double precision, intent(in) :: ERI_bbbb(nBas,nBas,nBas,nBas)
+  double precision, allocatable :: OmSC(:), OmSF(:)
+  double precision :: EcBSE(2)
+  integer :: nSC, nSF
+  integer(c_int) :: rc
+  character(len=80) :: fmt
+  nSC = (nO(1) - nC(1))*(nBas - nR(1) - nO(1)) + (nO(2) - nC(2))*(nBas - nR(2) - nO(2))
+  nSF = (nO(1) - nC(1))*(nBas - nR(2) - nO(2)) + (nO(2) - nC(2))*(nBas - nR(1) - nO(1))
+  allocate(OmSC(nSC), OmSF(nSF))
+  if(TDA) then
+    write(*,*) 'Tamm-Dancoff approximation activated in phBSE!'
+    write(*,*)
+  end if
+  rc = qk_ugw_phbse(int(nBas,c_int), int(nC,c_int), int(nO,c_int), int(nR,c_int), ERI_aaaa, ERI_aabb, ERI_bbbb, &
+                    eW, eGW, eta, merge(1_c_int, 0_c_int, TDA_W), merge(1_c_int, 0_c_int, TDA),                  &
+                    merge(1_c_int, 0_c_int, spin_conserved), merge(1_c_int, 0_c_int, spin_flip),                &
+                    merge(1_c_int, 0_c_int, exchange_kernel), EcBSE, OmSC, OmSF)
+  call qk_check(rc, c_null_ptr, 'qk_ugw_phbse')
+  if(spin_conserved) call qk_print_excitations('phBSE@GW@UHF', 'spin-conserved', nSC, OmSC)
+  if(spin_flip) call qk_print_excitations('phBSE@GW@UHF', 'spin-flip', nSF, OmSF)
+  write(fmt,'(A,I0,A)') '(2X,A', wid, ',F20.10,A3)'
+  write(*,*)
+  write(*,*)'-------------------------------------------------------------------------------'
+  if(wid == 60) then
+    write(*,fmt) 'Tr@BSE@'//tag//'@UHF correlation energy (spin-conserved) = ',EcBSE(1),' au'
+    write(*,fmt) 'Tr@BSE@'//tag//'@UHF correlation energy (spin-flip)      = ',EcBSE(2),' au'
+    write(*,fmt) 'Tr@BSE@'//tag//'@UHF correlation energy                  = ',sum(EcBSE),' au'
+    write(*,fmt) 'Tr@BSE@'//tag//'@UHF total       energy                  = ',ENuc + EUHF + sum(EcBSE),' au'
+  else
+    write(*,fmt) 'Tr@BSE@'//tag//'@UHF correlation energy (spin-conserved) =',EcBSE(1),' au'
+    write(*,fmt) 'Tr@BSE@'//tag//'@UHF correlation energy (spin-flip)      =',EcBSE(2),' au'
+    write(*,fmt) 'Tr@BSE@'//tag//'@UHF correlation energy                  =',sum(EcBSE),' au'
+    write(*,fmt) 'Tr@BSE@'//tag//'@UHF total energy                        =',ENuc + EUHF + sum(EcBSE),' au'
+  end if
+  write(*,*)'-------------------------------------------------------------------------------'
+  write(*,*)
+  deallocate(OmSC, OmSF)
 end subroutine
 
 subroutine UG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,TDA_W,TDA,dBSE,dTDA,spin_conserved,spin_flip, &
@@ -48,7 +101,7 @@ subroutine UG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,TDA_W,TDA,dBSE
   double precision :: EcRPA, EcGM(2), eHOMO, eLUMO
   integer(c_int) :: rc
 
-  call qk_ugw_checks(doACFDT, dophBSE)
+  call qk_ugw_checks(doACFDT, dophBSE, dBSE)
   write(*,*)
   write(*,*)'*********************************'
   write(*,*)'* Unrestricted G0W0 Calculation *'
@@ -75,6 +128,10 @@ subroutine UG0W0_hip(dotest,doACFDT,exchange_kernel,doXBS,dophBSE,TDA_W,TDA,dBSE
   write(*,'(2X,A60,F15.6,A3)') '   GM@G0W0@UHF total       energy = ',ENuc + EUHF + sum(EcGM),' au'
   write(*,'(2X,A60,F15.6,A3)') '   GM@G0W0@UHF correlation energy = ',sum(EcGM),' au'
   write(*,*)'-------------------------------------------------------------------------------'
+
+  ! static phBSE at the G0W0 energies, W from eHF (GW/UG0W0.f90:190-202)
+  if(dophBSE) call qk_uphbse_print('G0W0', 60, exchange_kernel, TDA_W, TDA, spin_conserved, spin_flip, eta, nBas, &
+                                   nC, nO, nR, ENuc, EUHF, ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, eGW)
 
   if(dotest) then
     call dump_test_value('U','G0W0 correlation energy',EcRPA)
@@ -105,7 +162,7 @@ subroutine evUGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS
   double precision :: EcRPA, EcGM(2), eHOMO, eLUMO
   integer(c_int) :: rc, nSCF
 
-  call qk_ugw_checks(doACFDT, BSE)
+  call qk_ugw_checks(doACFDT, BSE, dBSE)
   write(*,*)
   write(*,*)'*********************************'
   write(*,*)'| Unrestricted evGW Calculation *'
@@ -142,6 +199,10 @@ subroutine evUGW_hip(dotest,maxSCF,thresh,max_diis,doACFDT,exchange_kernel,doXBS
   write(*,'(2X,A60,F15.6,A3)') ' GM@evG@UHFW total       energy = ',ENuc + EUHF + sum(EcGM),' au'
   write(*,'(2X,A60,F15.6,A3)') ' GM@evG@UHFW correlation energy = ',sum(EcGM),' au'
   write(*,*)'-------------------------------------------------------------------------------'
+
+  ! static phBSE at the converged energies, W from eGW too (GW/evUGW.f90:243-255)
+  if(BSE) call qk_uphbse_print('evGW', 50, exchange_kernel, TDA_W, TDA, spin_conserved, spin_flip, eta, nBas, nC, &
+                               nO, nR, ENuc, EUHF, ERI_aaaa, ERI_aabb, ERI_bbbb, eGW, eGW)
 
   if(dotest) then
     call dump_test_value('U','evGW correlation energy',EcRPA)

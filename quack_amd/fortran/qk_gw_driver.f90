@@ -147,11 +147,13 @@ program qk_gw_driver
     nOU = [nEla, nElb]; nCU = nC; nRU = nR; nVU = nOrb - nOU; nSU = (nOU - nCU)*(nVU - nRU)
     allocate(eU(nOrb,2), SU(nOrb,nOrb), cU(nOrb,nOrb,2))
     eU(:,1) = eHF; eU(:,2) = eHF; SU = 0d0; cU = 0d0
-    if(doG0W0) call UG0W0_hip(dotest, .false., .false., .false., dophBSE, TDA_W, TDA, dBSE, dTDA, .true., .false., &
+    ! the "# spin:" line's second and third answers are spin_conserved and spin_flip for
+    ! the unrestricted branch (src/QuAcK/read_options.f90:189-197)
+    if(doG0W0) call UG0W0_hip(dotest, .false., .false., .false., dophBSE, TDA_W, TDA, dBSE, dTDA, singlet, triplet, &
                               lin, eta, reg, nOrb, nCU, nOU, nVU, nRU, nSU, ENuc, ERHF, SU, ERI, ERI, ERI, dip, dip, &
                               cU, eU)
     if(doevGW) call evUGW_hip(dotest, maxSCF, thresh, max_diis, .false., .false., .false., dophBSE, TDA_W, TDA, &
-                              dBSE, dTDA, .true., .false., lin, eta, reg, nOrb, nCU, nOU, nVU, nRU, nSU, ENuc, ERHF, &
+                              dBSE, dTDA, singlet, triplet, lin, eta, reg, nOrb, nCU, nOU, nVU, nRU, nSU, ENuc, ERHF, &
                               SU, ERI, ERI, ERI, dip, dip, cU, eU)
     deallocate(eU, SU, cU)
   end if

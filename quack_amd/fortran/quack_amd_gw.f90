@@ -116,6 +116,18 @@ module quack_amd_gw
       integer(c_int), intent(out) :: nSCF
     end function
 
+    ! unrestricted static phBSE (GW/UGW_phBSE.f90)
+    integer(c_int) function qk_ugw_phbse(nBas, nC, nO, nR, ERI_aaaa, ERI_aabb, ERI_bbbb, eW, eGW, eta, TDA_W, TDA, &
+                                         spin_conserved, spin_flip, exchange_kernel, EcBSE, OmSC, OmSF) &
+                                         bind(C, name='qk_ugw_phbse')
+      import :: c_int, c_double
+      integer(c_int), value :: nBas, TDA_W, TDA, spin_conserved, spin_flip, exchange_kernel
+      integer(c_int), intent(in) :: nC(2), nO(2), nR(2)
+      real(c_double), intent(in) :: ERI_aaaa(*), ERI_aabb(*), ERI_bbbb(*), eW(*), eGW(*)
+      real(c_double), value :: eta
+      real(c_double), intent(out) :: EcBSE(2), OmSC(*), OmSF(*)
+    end function
+
     ! RHF Fock builds (HF/RHF.f90:178-179) on the device
     integer(c_int) function qk_rhf_create(h, nBas) bind(C, name='qk_rhf_create')
       import :: c_int, c_ptr

@@ -34,7 +34,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms",
            "qk_rgw_qp_graph_noalloc", "qk_noalloc_last_error", "qk_gw_qp_iters",
            "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats", "qk_gwb_set_graphs", "qk_gwb_graph_stats", "qk_set_split_aotomo",
-           "qk_qsgw_create", "qk_qsgw_cycle", "qk_qsgw_destroy", "qk_qsgw_last_error", "qk_rgw_phbse_dyn", "qk_set_library_gemm"]
+           "qk_qsgw_create", "qk_qsgw_cycle", "qk_qsgw_destroy", "qk_qsgw_last_error", "qk_rgw_phbse_dyn", "qk_set_library_gemm", "qk_ugw_phbse"]
 
 
 EIG_MODES = {"syevd": 0, "refine": 1, "sytrd": 2, "refine+sytrd": 3}
@@ -136,6 +136,7 @@ def load():
     lib.qk_noalloc_last_error.restype = ctypes.c_char_p
     lib.qk_rgw_phbse.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 5
     lib.qk_set_library_gemm.argtypes = [_I]
+    lib.qk_ugw_phbse.argtypes = [_I, _IP, _IP, _IP] + [_P] * 5 + [_D] + [_I] * 5 + [_P] * 3
     lib.qk_rgw_phbse_dyn.argtypes = [_I] * 8 + [_D] + [_I] * 6 + [_P] * 7
     lib.qk_gw_sigma_full.argtypes = [_H, _P, _D, _I, _P, _P, _P, _P]
     lib.qk_ug0w0.argtypes = [_I, _IP, _IP, _IP, _P, _P, _P, _P, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P]
@@ -683,6 +684,22 @@ def UG0W0(ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, nO, eta=0.0, linearize=False, TDA_W
                            _p(eHF), eta, int(linearize), int(TDA_W), int(doSRG), _p(eGW), _p(Sig), _p(Z), _p(Om),
                            ctypes.byref(Ec), _p(Egm)))
     return {"eGW": eGW, "SigC": Sig, "Z": Z, "Om": Om, "EcRPA": Ec.value, "EcGM": Egm}
+
+
+def UGW_phBSE(ERI_aaaa, ERI_aabb, ERI_bbbb, eW, eGW, nO, eta=0.0, TDA_W=False, TDA=False, spin_conserved=True,
+              spin_flip=True, exchange_kernel=False, nC=(0, 0), nR=(0, 0)):
+    """GW/UGW_phBSE.f90 with the static kernel (qk_ugw_phbse).  eW, eGW:
+    nBas x 2.  Returns dict(EcBSE (2), OmSC, OmSF)."""
+    eW = np.asfortranarray(eW, dtype=np.float64)
+    eGW = np.asfortranarray(eGW, dtype=np.float64)
+    n = eW.shape[0]
+    c, o, r, nSc = _ucounts(n, nC, nO, nR)
+    nSf = int((o[0] - c[0]) * (n - r[1] - o[1]) + (o[1] - c[1]) * (n - r[0] - o[0]))
+    Ec, OmSC, OmSF = np.zeros(2), np.zeros(nSc), np.zeros(nSf)
+    _check(load().qk_ugw_phbse(n, _ip(c), _ip(o), _ip(r), _p(_f64(ERI_aaaa)), _p(_f64(ERI_aabb)),
+                               _p(_f64(ERI_bbbb)), _p(eW), _p(eGW), eta, int(TDA_W), int(TDA), int(spin_conserved),
+                               int(spin_flip), int(exchange_kernel), _p(Ec), _p(OmSC), _p(OmSF)))
+    return {"EcBSE": Ec, "OmSC": OmSC, "OmSF": OmSF}
 
 
 def evUGW(ERI_aaaa, ERI_aabb, ERI_bbbb, eHF, nO, maxSCF=64, thresh=1e-5, max_diis=5, eta=0.0, linearize=False,

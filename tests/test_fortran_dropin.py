@@ -122,6 +122,39 @@ def test_driver_phbse2_dynamic(tmp_path, mo_systems):
             assert abs(float(row[4]) - zd[k]) < 2e-6
 
 
+@pytest.mark.gpu
+def test_driver_unrestricted_phbse(tmp_path, mo_systems):
+    """phBSE switched on in the UHF branch: UG0W0_hip and evUGW_hip run the
+    static unrestricted BSE (spin-conserved and spin-flip, the "# spin:" line's
+    answers) and print QuAcK's Tr@BSE lines, which match the CPU restatement
+    at the printed precision."""
+    import numpy as np
+
+    from oracle import gw_oracle as go
+    print("fortran driver UHF phBSE H2O", flush=True)
+    s = mo_systems["H2O"]
+    nO, mo, e = s["nO"], s["eri_mo"], s["eHF"]
+    quackio.write_inputs(str(tmp_path), s, g0w0=True, evgw=True, maxSCF=64, thresh=1e-5, max_diis=5, eri_mo=mo,
+                         uhf=True, bse=True)
+    r = subprocess.run([DRIVER, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "phBSE@GW@UHF" in r.stdout and "spin-flip" in r.stdout
+    e2 = np.stack([e, e], axis=1)
+    occ = (nO, nO)
+    g0 = go.ug0w0(mo, mo, mo, e2, occ)["eGW"]
+    ev = go.evugw(mo, mo, mo, e2, occ, maxSCF=64)["eGW"]
+    for tag, eW, eG in (("G0W0", e2, g0), ("evGW", ev, ev)):
+        Ec, _, _ = go.ugw_phbse(mo, mo, mo, eW, eG, occ)
+        vals = {}
+        for ln in r.stdout.splitlines():
+            for man in ("spin-conserved", "spin-flip"):
+                # evUGW's A50 label field cuts the label before ") =" (GW/evUGW.f90:250-251)
+                key = "Tr@BSE@%s@UHF correlation energy (%s" % (tag, man)
+                if key in ln:
+                    vals[man] = float(ln.split()[-2])
+        assert abs(vals["spin-conserved"] - Ec[0]) < 2e-9 and abs(vals["spin-flip"] - Ec[1]) < 2e-9, (tag, vals, Ec)
+
+
 def test_input_files_uhf_flag(tmp_path, systems):
     """The unrestricted branch is selected by the HF line of input/methods and
     the electron counts of input/molecule, as QuAcK reads them."""

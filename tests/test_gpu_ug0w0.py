@@ -137,6 +137,42 @@ def test_unrestricted_frozen_match_oracle(uhf_h2o, occ_d, nC, nR):
         assert np.abs(r["eGW"] - o["eGW"]).max() < 1e-8, kw
 
 
+@pytest.mark.parametrize("nC,nR", [((0, 0), (0, 0)), ((1, 1), (1, 2)), ((1, 0), (0, 1))])
+def test_unrestricted_phbse_matches_oracle(uhf_h2o, nC, nR):
+    """Unrestricted static phBSE (qk_ugw_phbse, GW/UGW_phBSE.f90) with
+    distinct alpha and beta orbitals: spin-conserved and spin-flip
+    excitation energies and EcBSE within 1e-8 of the oracle (pinned to
+    QuAcK's compiled routines), for full and TDA BSE and TDA screening, with
+    frozen orbitals counted per spin."""
+    from quack_amd import gw
+    u = uhf_h2o
+    occ = (u["nO"], u["nO"])
+    eGW = go.ug0w0(u["aa"], u["ab"], u["bb"], u["e2"], occ, eta=0.05, nC=nC, nR=nR)["eGW"]
+    for kw in ({}, {"TDA": True}, {"TDA_W": True}, {"spin_flip": False}):
+        print("unrestricted phbse", nC, nR, kw, flush=True)
+        r = gw.UGW_phBSE(u["aa"], u["ab"], u["bb"], u["e2"], eGW, occ, eta=0.05, nC=nC, nR=nR, **kw)
+        Ec, OmSC, OmSF = go.ugw_phbse(u["aa"], u["ab"], u["bb"], u["e2"], eGW, occ, eta=0.05, nC=nC, nR=nR, **kw)
+        assert np.abs(r["OmSC"] - OmSC).max() < 1e-8, kw
+        if OmSF is not None:
+            assert np.abs(r["OmSF"] - OmSF).max() < 1e-8, kw
+        assert np.abs(r["EcBSE"] - Ec).max() < 1e-8, kw
+
+
+def test_unrestricted_phbse_closed_shell_is_restricted(mo_systems):
+    """alpha = beta on the device: the spin-conserved manifold is the
+    restricted engine's singlets and triplets, the spin-flip one its triplets
+    twice."""
+    from quack_amd import gw
+    s = mo_systems["H2O"]
+    mo, nO, e = s["eri_mo"], s["nO"], s["eHF"]
+    eGW = gw.RG0W0(mo, e, nO, eta=0.05)["eGW"]
+    r = gw.RGW_phBSE(mo, e, eGW, nO, eta=0.05)
+    u = gw.UGW_phBSE(mo, mo, mo, np.stack([e, e], axis=1), np.stack([eGW, eGW], axis=1), (nO, nO), eta=0.05)
+    assert np.abs(np.sort(u["OmSC"]) - np.sort(np.concatenate([r["OmS"], r["OmT"]]))).max() < 1e-8
+    assert np.abs(np.sort(u["OmSF"]) - np.sort(np.concatenate([r["OmT"], r["OmT"]]))).max() < 1e-8
+    assert abs(u["EcBSE"].sum() - r["EcBSE"].sum()) < 1e-8
+
+
 def test_unrestricted_frozen_argument_errors():
     """Orbital counts outside 0 <= nC < nO < nBas - nR in either spin are
     rejected before any device work."""

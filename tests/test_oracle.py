@@ -561,3 +561,40 @@ def test_unrestricted_frozen_reductions(mo_systems, uhf_h2o):
                     eta=0.05)
     assert np.abs(full["eGW"][act] - part["eGW"]).max() < 1e-12
     assert np.abs(full["Om"] - part["Om"]).max() < 1e-12 and abs(full["EcRPA"] - part["EcRPA"]) < 1e-11
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+def test_unrestricted_phbse_vs_compiled_reference(uhf_h2o):
+    """Unrestricted static phBSE@G0W0 (GW/UGW_phBSE.f90): spin-conserved and
+    spin-flip manifolds, full and TDA BSE, TDA screening, frozen orbitals
+    counted per spin, with distinct alpha and beta orbitals, against QuAcK's
+    phULR_A/B (ispin 1 and 2), phULR, UGW_excitation_density and
+    UGW_phBSE_static_kernel_A/B composed in UGW_phBSE's order."""
+    u = uhf_h2o
+    n, nO = u["n"], u["nO"]
+    occ = (nO, nO)
+    for nC, nR in (((0, 0), (0, 0)), ((1, 1), (1, 2)), ((1, 0), (0, 1))):
+        eGW = go.ug0w0(u["aa"], u["ab"], u["bb"], u["e2"], occ, eta=0.05, nC=nC, nR=nR)["eGW"]
+        for kw in ({}, {"tda": True}, {"tda_w": True}):
+            r = qkref.uphbse(u["aa"], u["ab"], u["bb"], u["e2"], eGW, occ, eta=0.05, nC=nC, nR=nR, **kw)
+            Ec, OmSC, OmSF = go.ugw_phbse(u["aa"], u["ab"], u["bb"], u["e2"], eGW, occ, eta=0.05,
+                                         TDA=kw.get("tda", False), TDA_W=kw.get("tda_w", False), nC=nC, nR=nR)
+            assert np.abs(OmSC - r["OmSC"]).max() < 1e-10 and np.abs(OmSF - r["OmSF"]).max() < 1e-10, (nC, nR, kw)
+            assert np.abs(Ec - r["EcBSE"]).max() < 1e-10, (nC, nR, kw)
+
+
+def test_unrestricted_phbse_closed_shell_reduction(mo_systems):
+    """With alpha = beta the unrestricted BSE splits into the restricted
+    manifolds: the spin-conserved energies are the restricted singlets and
+    triplets together, the spin-flip energies the triplets twice (the M_s = +1
+    and -1 components), and EcBSE(1) + EcBSE(2) equals the restricted singlet
+    plus triplet sum."""
+    s = mo_systems["H2O"]
+    mo, nO, e = s["eri_mo"], s["nO"], s["eHF"]
+    eGW = go.rg0w0(mo, e, nO, eta=0.05)["eGW"]
+    e2, g2 = np.stack([e, e], axis=1), np.stack([eGW, eGW], axis=1)
+    Ec, OmSC, OmSF = go.ugw_phbse(mo, mo, mo, e2, g2, (nO, nO), eta=0.05)
+    EcR, OmS, OmT = go.rgw_phbse(mo, e, eGW, nO, eta=0.05)
+    assert np.abs(np.sort(OmSC) - np.sort(np.concatenate([OmS, OmT]))).max() < 1e-10
+    assert np.abs(np.sort(OmSF) - np.sort(np.concatenate([OmT, OmT]))).max() < 1e-10
+    assert abs(Ec.sum() - EcR.sum()) < 1e-10
