@@ -146,7 +146,9 @@ int qk_gw_set_precision(qk_gw_handle h, int mode);
  *           handle's last eigenvectors were computed (same TDA_W), refine those
  *           eigenvectors by Newton steps for X^T X = I, X^T C X diagonal (FP64
  *           matrix-core GEMMs, quadratic convergence, stopped when the largest
- *           correction is below 1e-8; close eigenvalues handled as clusters);
+ *           correction is below 1e-8, or below 1e-5 in the split-precision
+ *           modes of qk_gw_set_precision, with that step's update applied;
+ *           close eigenvalues handled as clusters);
  *           a full solve otherwise, or when a cluster outgrows 32 vectors,
  *           merging clusters does not settle, or eight steps do not converge.
  * Both modes return FP64 eigenpairs of the same matrix.  No counterpart in
