@@ -35,7 +35,7 @@ run through the library's HIP kernels on the device.
     python bench.py [--steps K] [--warmup W] [--form fp64|mixed|mixed2] [--eig syevd|refine|sytrd|refine+sytrd]
                     [--concurrent C] [--method evgw|g0w0] [--reg eta|srg|none] [--eta ETA] [--max-diis D]
                     [--learned on|off] [--cpu-baseline auto|off] [--lib-gemm on|off]
-                    [--workload h50|df600|batch|learned|ugw|cpu250]
+                    [--workload h50|df600|batch|learned|ugw|cpu250|bse|qsgw]
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -330,12 +330,15 @@ def main():
                          "kernels, whose tile choices give bit-identical results")
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--ugw-atoms", type=int, default=10, help="ugw workload: H chain length (one more alpha electron)")
-    ap.add_argument("--workload", choices=["h50", "df600", "batch", "learned", "ugw", "cpu250"], default="h50",
+    ap.add_argument("--wide-atoms", type=int, default=20, help="bse and qsgw workloads: H chain length")
+    ap.add_argument("--workload", choices=["h50", "df600", "batch", "learned", "ugw", "cpu250", "bse", "qsgw"],
+                    default="h50",
                     help="h50: the headline N=250 lattices; df600: the precision-form measurement on the synthetic "
                          "density-fitted system (N=600, o=60, naux=1800), FP64 against split precision; batch: "
                          "batched G0W0 over seeded H chains/lattices of uneven size; learned: the MLP warm start "
                          "against plain evGW on held-out systems; cpu250: QuAcK's compiled path on the host cores "
-                         "at N=250 (AO->MO + one evGW cycle)")
+                         "at N=250 (AO->MO + one evGW cycle); bse: static phBSE@G0W0 and qsgw: one qsGW cycle, "
+                         "each against QuAcK's compiled routines on an H chain (--wide-atoms)")
     ap.add_argument("--nmol", type=int, default=64, help="batch and learned workloads: number of molecules")
     ap.add_argument("--kmax", type=int, default=12, help="batch workload: largest H_2k (N = 10k)")
     ap.add_argument("--chunk", type=int, default=256, help="batch workload: most molecules resident at once")
@@ -358,6 +361,8 @@ def main():
         return run_learned(args)
     if args.workload == "cpu250":
         return run_cpu250(args)
+    if args.workload in ("bse", "qsgw"):
+        return run_widened(args)
     return run_h50(args)
 
 
@@ -962,6 +967,121 @@ def run_ugw(args):
         "cpu_baseline": {"value": 1.0 / float(d["t"]), "unit": "mol/s", "cores": sample_threads(), "host": host_cpu(),
                          "kind": "reference", "sample": "the same molecule, QuAcK's compiled unrestricted routines, "
                                                         "one run"},
+    }), flush=True)
+
+
+# QuAcK's compiled routines (oracle/_ref) on an H chain, for the bse and qsgw
+# workloads: the static phBSE@G0W0 of GW/RGW_phBSE.f90:113-190 at eta = 0.05
+# (W and G from G0W0 at eta = 0.05), or the first qsGW cycle of
+# GW/qsRGW.f90:199-341 with SRG from the RHF orbitals; each timed as the median
+# of the runs within 10 s (at least 3); the inputs and results go to a file.
+WIDE_SAMPLE = r"""
+import sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from quack_amd import intgen as ig
+from oracle import qkref
+work, natoms, out = sys.argv[2], int(sys.argv[3]), sys.argv[4]
+# 1.8 bohr spacing: no near-dependent functions (nOrb = nBas), the case of
+# qsRGW's first branch that the composed reference cycle covers
+s = ig.system(ig.h_chain(natoms, spacing=1.8, seed=0))
+nO = s["nO"]
+mol = s["mol"]
+def timed(f):
+    r = f()
+    ts = []
+    t0 = time.time()
+    while time.time() - t0 < 10.0 or len(ts) < 3:
+        t = time.time(); r = f(); ts.append(time.time() - t)
+    ts.sort()
+    return r, ts[len(ts) // 2], len(ts)
+if work == "bse":
+    mo = qkref.aotomo(s["c"], s["eri"])
+    eGW = qkref.rg0w0(mo, s["eHF"], nO, eta=0.05)["eGW"]
+    r, t, reps = timed(lambda: qkref.phbse(mo, s["eHF"], eGW, nO, eta=0.05))
+    np.savez(out, mo=mo, eHF=s["eHF"], eGW=eGW, nO=nO, Ec=r["EcBSE"], OmS=r["OmS"], OmT=r["OmT"], t=t, reps=reps)
+else:
+    S, T, V = mol.one_electron()
+    w, U = np.linalg.eigh(S)
+    X = U / np.sqrt(w)
+    c = np.asarray(s["c"])
+    P = 2.0 * c[:, :nO] @ c[:, :nO].T
+    r, t, reps = timed(lambda: qkref.qsgw_cycle(S, T, V, X, c, P, s["eHF"], s["eri"], nO, srg=True))
+    np.savez(out, S=S, T=T, V=V, X=X, c=c, eHF=s["eHF"], eri=s["eri"], nO=nO, eGW=r["eGW"], t=t, reps=reps)
+"""
+
+
+def run_widened(args):
+    """The rows beside the path, each on an H chain of --wide-atoms atoms
+    (1.8 bohr spacing, cc-pVDZ), against QuAcK's own routines compiled from source (oracle/_ref,
+    flang + MKL on the host cores, in a child process without torch) on the
+    same inputs:
+      bse:  static phBSE@G0W0 (qk_rgw_phbse: screening, kernels, singlet and
+            triplet BSE) from the MO integrals in host memory, as the Fortran
+            drop-in passes them;
+      qsgw: the first qsGW cycle (qk_qsgw_cycle with SRG) from the RHF
+            orbitals, the AO integrals uploaded beforehand (qk_qsgw_create,
+            outside the timed call)."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from quack_amd import gw, qsgw
+    path = os.path.join(tempfile.mkdtemp(), "wide.npz")
+    p = subprocess.run([sys.executable, "-c", WIDE_SAMPLE, ROOT, args.workload, str(args.wide_atoms), path],
+                       capture_output=True, text=True, timeout=900)
+    if p.returncode != 0:
+        raise RuntimeError("reference run failed: " + p.stderr[-2000:])
+    d = np.load(path)
+    nO = int(d["nO"])
+    torch.cuda.init()
+    reps = max(5, args.steps)
+    ts = []
+    if args.workload == "bse":
+        mo, eHF, eGW = d["mo"], d["eHF"], d["eGW"]
+        n = len(eHF)
+        for k in range(max(args.warmup, 1) + reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = gw.RGW_phBSE(mo, eHF, eGW, nO, eta=0.05)
+            torch.cuda.synchronize()
+            if k >= max(args.warmup, 1):
+                ts.append(time.perf_counter() - t0)
+        err = max(float(np.abs(r["OmS"] - d["OmS"]).max()), float(np.abs(r["OmT"] - d["OmT"]).max()),
+                  float(np.abs(r["EcBSE"] - d["Ec"]).max()))
+        metric, what = "static phBSE@G0W0 calculations per second", "phBSE"
+        extra = {"max_abs_dOm_dEc_vs_reference_Ha": err}
+    else:
+        S, T, V, X, c, eHF, eri = (d[k] for k in ("S", "T", "V", "X", "c", "eHF", "eri"))
+        n = len(eHF)
+        for k in range(max(args.warmup, 1) + reps):
+            with qsgw.QsGW(S, T, V, X, eri, c, eHF, nO) as q:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r = q.cycle(doSRG=True)
+                torch.cuda.synchronize()
+                if k >= max(args.warmup, 1):
+                    ts.append(time.perf_counter() - t0)
+        err = float(np.abs(r["eGW"] - d["eGW"]).max())
+        metric, what = "qsGW cycles per second", "one qsGW cycle"
+        extra = {"max_abs_dE_vs_reference_Ha": err}
+    t = float(np.median(ts))
+    tr = float(d["t"])
+    log("%s: GPU %.4f s, reference %.3f s (%.1fx), max |d| %.1e Ha" % (what, t, tr, tr / t, err))
+    print(json.dumps({
+        "metric": metric, "value": 1.0 / t, "unit": "1/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t, "statistic": "median of %d runs" % reps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%s: H%d chain (1.8 bohr spacing, seed 0) cc-pVDZ, nBas %d, nO %d, nS %d%s" % (
+            args.workload, args.wide_atoms, n, nO, nO * (n - nO),
+            ", eta 0.05, W and G from G0W0 at eta 0.05" if args.workload == "bse" else ", SRG, from RHF")},
+        **extra,
+        "cpu_baseline": {"value": 1.0 / tr, "unit": "1/s", "cores": sample_threads(), "host": host_cpu(),
+                         "kind": "reference", "sample": "the same calculation, QuAcK's compiled routines, median of "
+                                                        "%d runs" % int(d["reps"])},
+        "gpu_over_reference": tr / t,
     }), flush=True)
 
 
