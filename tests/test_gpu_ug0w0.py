@@ -148,11 +148,13 @@ def test_unrestricted_phbse_matches_oracle(uhf_h2o, nC, nR):
     u = uhf_h2o
     occ = (u["nO"], u["nO"])
     eGW = go.ug0w0(u["aa"], u["ab"], u["bb"], u["e2"], occ, eta=0.05, nC=nC, nR=nR)["eGW"]
-    for kw in ({}, {"TDA": True}, {"TDA_W": True}, {"spin_flip": False}):
+    for kw in ({}, {"TDA": True}, {"TDA_W": True}, {"spin_flip": False}, {"spin_conserved": False},
+               {"exchange_kernel": True}):
         print("unrestricted phbse", nC, nR, kw, flush=True)
         r = gw.UGW_phBSE(u["aa"], u["ab"], u["bb"], u["e2"], eGW, occ, eta=0.05, nC=nC, nR=nR, **kw)
         Ec, OmSC, OmSF = go.ugw_phbse(u["aa"], u["ab"], u["bb"], u["e2"], eGW, occ, eta=0.05, nC=nC, nR=nR, **kw)
-        assert np.abs(r["OmSC"] - OmSC).max() < 1e-8, kw
+        if OmSC is not None:
+            assert np.abs(r["OmSC"] - OmSC).max() < 1e-8, kw
         if OmSF is not None:
             assert np.abs(r["OmSF"] - OmSF).max() < 1e-8, kw
         assert np.abs(r["EcBSE"] - Ec).max() < 1e-8, kw
@@ -183,6 +185,8 @@ def test_unrestricted_frozen_argument_errors():
     for nC, nO, nR in (((3, 0), (3, 3), (0, 0)), ((0, 0), (3, 3), (0, 3)), ((-1, 0), (3, 3), (0, 0))):
         with pytest.raises(gw.QKError):
             gw.UG0W0(z, z, z, e2, nO, nC=nC, nR=nR)
+        with pytest.raises(gw.QKError):
+            gw.UGW_phBSE(z, z, z, e2, e2, nO, nC=nC, nR=nR)
 
 
 def test_evugw_own_eigensolver(mo_systems, uhf_h2o):
