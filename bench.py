@@ -624,6 +624,9 @@ def run_h50(args):
                                                            if "start" in r1 else None)},
         "roofline": dom,
         "roofline_excitation_gemm": gemm_roof,
+        # the split-precision GEMM shapes the autotuner timed, with the tile group
+        # and wave layout it kept and every candidate's time
+        "split_gemm_autotune": [l for l in gw.autotune_report() if l.startswith("gemm_bf16x3")],
         "roofline_tridiagonalisation": trd_roof,
         "cpu_baseline": cpu,
         "result": {"HOMO": float(rs[0]["eGW"][nO - 1]), "LUMO": float(rs[0]["eGW"][nO]), "EcRPA": rs[0]["EcRPA"],

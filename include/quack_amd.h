@@ -334,11 +334,15 @@ const char *qk_last_error(void);
 
 /* Tile autotuner of the library's GEMMs (process-wide; on by default): the first
  * call of each large GEMM shape times every tile configuration (FP64) or tile
- * grouping (split precision) once on the caller's stream and keeps the fastest.
- * All candidates give identical results.  No counterpart in QuAcK.
- * qk_autotune_report writes one line per tuned shape (QK_E_ARG if len is too small). */
+ * grouping and wave layout (split precision) once on the caller's stream and
+ * keeps the fastest; the tuning call's output is the last candidate's.  All
+ * candidates give identical results.  No counterpart in QuAcK.
+ * qk_autotune_report writes one line per tuned shape (QK_E_ARG if len is too
+ * small); qk_autotune_reset forgets every tuned shape, so the next large call
+ * of each shape tunes again (calls that overlap it on other threads race). */
 int qk_set_autotune(int on);
 int qk_autotune_report(char *buf, int len);
+int qk_autotune_reset(void);
 /* rocBLAS's dgemm as one more autotuner candidate (process-wide; off by
  * default) for plain FP64 products: unbatched, all tiles, every dimension at
  * least 512, 32-bit indexable, outside graph capture.  It sums in its own

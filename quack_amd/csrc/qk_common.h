@@ -52,6 +52,9 @@ void set_library_gemm(bool on);
 bool autotune_on();
 std::string autotune_report();
 std::string autotune_report_bf16();
+// forget every tuned shape (the next large call of each shape tunes again)
+void autotune_reset();
+void autotune_reset_bf16();
 
 // C(MxN) = alpha * op(A)(MxK) * op(B)(KxN) + beta * C, column-major, FP64 MFMA.
 // Returns the number of kernel launches it issued (more than one while tuning).

@@ -308,6 +308,10 @@ int dispatch(hipStream_t s, int M, int N, int K, double alpha, const double* A, 
 }  // namespace
 
 void set_autotune(bool on) { g_autotune.store(on ? 1 : 0); }
+void autotune_reset() {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune.clear();
+}
 void set_library_gemm(bool on) { g_libgemm.store(on ? 1 : 0); }
 bool autotune_on() { return g_autotune.load() != 0; }
 

@@ -110,9 +110,13 @@ __global__ void k_split3_blk(const double* __restrict__ src, long long ld, long 
     }
 }
 
-// Workgroup tile BM x BN = 128 x 128, 4 wavefronts as 2 x 2, each 64 x 64 =
-// 4 x 4 MFMA tiles; a 32-wide k-slab of the three slices of its A rows and B
-// rows is 48 KB of LDS.
+// Workgroup tile BM x BN = 128 x 128 in one of two wave layouts (tuned):
+// 4 wavefronts as 2 x 2, each 64 x 64 = 4 x 4 MFMA tiles (WR = 4), or 8
+// wavefronts as 4 x 2, each 32 x 64 = 2 x 4 MFMA tiles (WR = 2: half the
+// accumulators per wave, so two waves fit on a SIMD).  Every output element
+// sees the same sequence of matrix-core operations and FP64 chunk additions in
+// both layouts, so they give identical results.  A 32-wide k-slab of the three
+// slices of the A rows and B rows is 48 KB of LDS.
 //
 // Tile order.  Workgroups are dispatched round-robin over the 8 XCDs, each
 // with its own L2; workgroup p runs logical tile L = x q + min(x, r) + p / 8
@@ -121,8 +125,10 @@ __global__ void k_split3_blk(const double* __restrict__ src, long long ld, long 
 // n-tiles, m fastest, so the ~32 workgroups an XCD holds at once share about
 // 8 A row-blocks and 4 B row-blocks: each slab fetched from HBM feeds several
 // workgroups from L2.
-constexpr int BM = 128, BN = 128, NTH = 256, NXCD = 8;
-static_assert(BM == 128 && BN == 128 && NTH == 256, "slab moves assume a 128 x 128 tile and 256 threads");
+constexpr int BM = 128, BN = 128, NXCD = 8;
+// threads of a workgroup in wave layout WR (wave rows = 16 WR)
+__host__ __device__ constexpr int threads_of(int WR) { return 64 * (BM / (16 * WR)) * (BN / 64); }
+static_assert(BM == 128 && BN == 128, "slab moves assume a 128 x 128 tile");
 inline int grid_for(long long total) { return (int)std::min<long long>((total + 255) / 256, 65536); }
 
 __device__ __forceinline__ void tile_of(int pid, int tm, int tn, int gm, int& mb, int& nb) {
@@ -150,15 +156,19 @@ constexpr int slab_bytes() { return 2 * NSL * 128 * 64; }
 
 // OutT: double, or float where the consumer needs only the FP32-level accuracy
 // the slicing gives (the FP64 recombination is then rounded once on the store).
-template <typename OutT, int NSL>
-__global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, const __bf16* __restrict__ A0,
+template <typename OutT, int NSL, int WR>
+__global__ __launch_bounds__(threads_of(WR)) void k_gemm_bf16x3(int M, int N, int kpad, const __bf16* __restrict__ A0,
                                                          const __bf16* __restrict__ A1, const __bf16* __restrict__ A2,
                                                          const __bf16* __restrict__ B0, const __bf16* __restrict__ B1,
                                                          const __bf16* __restrict__ B2, OutT* __restrict__ C,
                                                          long long ldc, int tiles_m, int tiles_n, int gm) {
     static_assert(NSL == 2 || NSL == 3, "two or three slices");
+    static_assert(WR == 4 || WR == 2, "64 or 32 wave rows");
+    constexpr int NW = threads_of(WR) / 64;  // waves
+    constexpr int WM = BM / (16 * WR);       // waves along m
     constexpr int SLAB_B = slab_bytes<NSL>();
-    constexpr int LPW = 4 * NSL;  // 1 KB slab loads per wave per slab
+    constexpr int LPW = 16 * NSL / NW;  // 1 KB slab loads per wave per slab
+    static_assert(LPW * NW == 16 * NSL, "slab blocks split evenly over the waves");
     __shared__ __attribute__((aligned(1024))) char st0[SLAB_B];
     __shared__ __attribute__((aligned(1024))) char st1[SLAB_B];
     __shared__ __attribute__((aligned(1024))) char st2[SLAB_B];
@@ -167,7 +177,7 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
     const int m0 = mb * BM, n0 = nb * BN;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const int wm = (wave % WM) * 16 * WR, wn = (wave / WM) * 64;
     const int l16 = lane & 15;
     // operand reads: lane reads row (.. + l16), k-chunk lane/16, stored in slot
     const int rslot = ((lane >> 4) ^ ((l16 >> 1) & 3)) * 16;
@@ -195,10 +205,10 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
                                              (__attribute__((address_space(3))) void*)((stage) + ldo[j]), 16, 0, 0); \
     } while (0)
 
-    d4 acc64[4][4];
-    f4 acc[4][4];
+    d4 acc64[WR][4];
+    f4 acc[WR][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < WR; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             acc64[a][b] = d4{0.0, 0.0, 0.0, 0.0};
@@ -212,7 +222,7 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
         _Pragma("unroll") for (int b = 0; b < 4; ++b)                                                        \
             _Pragma("unroll") for (int sl = 0; sl < NSL; ++sl)                                               \
                 fb[b][sl] = QK_FRAG(stage, NSL + sl, wn + b * 16 + l16);                                     \
-        _Pragma("unroll") for (int a = 0; a < 4; ++a) {                                                      \
+        _Pragma("unroll") for (int a = 0; a < WR; ++a) {                                                     \
             bf16x8 fa[NSL];                                                                                  \
             _Pragma("unroll") for (int sl = 0; sl < NSL; ++sl) fa[sl] = QK_FRAG(stage, sl, wm + a * 16 + l16); \
             if constexpr (NSL == 3) {                                                                        \
@@ -231,7 +241,7 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[b][0], fa[0], acc[a][b], 0, 0, 0);   \
         }                                                                                                    \
         if (((k0) + KS) % KC == 0 || (k0) + KS >= kpad) {                                                    \
-            _Pragma("unroll") for (int a = 0; a < 4; ++a)                                                    \
+            _Pragma("unroll") for (int a = 0; a < WR; ++a)                                                   \
                 _Pragma("unroll") for (int b = 0; b < 4; ++b) {                                              \
                     _Pragma("unroll") for (int r = 0; r < 4; ++r) acc64[a][b][r] += (double)acc[a][b][r];    \
                     acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};                                                      \
@@ -270,7 +280,7 @@ __global__ __launch_bounds__(NTH) void k_gemm_bf16x3(int M, int N, int kpad, con
 
     // acc64[a][b][r] = C(row m0+wm+a*16+l16, col n0+wn+b*16+4*(lane/16)+r)
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int a = 0; a < WR; ++a) {
         int i = m0 + wm + a * 16 + l16;
         if (i >= M) continue;
 #pragma unroll
@@ -315,23 +325,40 @@ void split3_blocks(hipStream_t s, const double* src, long long ld, long long bs,
 }
 
 namespace {
-// Autotuned group size of the tile order, per (M, N, kpad, output type, slices): the first call of a
-// shape with at least 1e10 flop times each candidate once (HIP events on the
-// caller's stream); every candidate writes the same C.
-constexpr int NGM = 4;
+// Autotuned tile order and wave layout, per (M, N, kpad, output type, slices):
+// candidate c has group size GM_CAND[c % NGM] and wave layout WR 4 (c < NGM)
+// or 2.  The first call of a shape with at least 1e10 flop times each
+// candidate once (HIP events on the caller's stream); every candidate writes
+// the same C.
+constexpr int NGM = 4, NCAND = 2 * NGM, CAND_DEFAULT = 2;  // group 8, 64-row waves
 const int GM_CAND[NGM] = {2, 4, 8, 16};
 std::mutex g_mu;
-std::map<std::tuple<int, int, int, int, int>, std::pair<int, std::array<float, NGM>>> g_gm;
+std::map<std::tuple<int, int, int, int, int>, std::pair<int, std::array<float, NCAND>>> g_gm;
+
+template <typename OutT, int NSL, int WR>
+void launch_k(hipStream_t s, int gm, int M, int N, const Split3& A, const Split3& B, OutT* C, long long ldc, int tm,
+              int tn) {
+    hipLaunchKernelGGL((k_gemm_bf16x3<OutT, NSL, WR>), dim3(tm * tn), dim3(threads_of(WR)), 0, s, M, N, A.kpad,
+                       A.s[0], A.s[1], NSL == 3 ? A.s[2] : (const __bf16*)nullptr, B.s[0], B.s[1],
+                       NSL == 3 ? B.s[2] : (const __bf16*)nullptr, C, ldc, tm, tn, gm);
+}
 
 template <typename OutT>
-void launch_gm(hipStream_t s, int gm, int M, int N, const Split3& A, const Split3& B, OutT* C, long long ldc, int tm,
+void launch_gm(hipStream_t s, int cand, int M, int N, const Split3& A, const Split3& B, OutT* C, long long ldc, int tm,
                int tn) {
-    if (A.nsl == 3)
-        hipLaunchKernelGGL((k_gemm_bf16x3<OutT, 3>), dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1],
-                           A.s[2], B.s[0], B.s[1], B.s[2], C, ldc, tm, tn, gm);
-    else
-        hipLaunchKernelGGL((k_gemm_bf16x3<OutT, 2>), dim3(tm * tn), dim3(NTH), 0, s, M, N, A.kpad, A.s[0], A.s[1],
-                           (const __bf16*)nullptr, B.s[0], B.s[1], (const __bf16*)nullptr, C, ldc, tm, tn, gm);
+    const int gm = GM_CAND[cand % NGM];
+    const bool wide = cand < NGM;
+    if (A.nsl == 3) {
+        if (wide)
+            launch_k<OutT, 3, 4>(s, gm, M, N, A, B, C, ldc, tm, tn);
+        else
+            launch_k<OutT, 3, 2>(s, gm, M, N, A, B, C, ldc, tm, tn);
+    } else {
+        if (wide)
+            launch_k<OutT, 2, 4>(s, gm, M, N, A, B, C, ldc, tm, tn);
+        else
+            launch_k<OutT, 2, 2>(s, gm, M, N, A, B, C, ldc, tm, tn);
+    }
     QK_HIP(hipGetLastError());
 }
 
@@ -343,7 +370,7 @@ int gemm_bf16x3_t(hipStream_t s, int M, int N, const Split3& A, const Split3& B,
     QK_CHECK((long long)tm * tn < (1LL << 31), QK_E_ARG, "gemm_bf16x3: grid too large");
     const double flop = 2.0 * (A.nsl == 3 ? 6 : 3) * M * (double)N * A.kpad;  // slice products
     if (!autotune_on() || flop < 1e10) {
-        launch_gm(s, 8, M, N, A, B, C, ldc, tm, tn);
+        launch_gm(s, CAND_DEFAULT, M, N, A, B, C, ldc, tm, tn);
         return 1;
     }
     std::lock_guard<std::mutex> lock(g_mu);
@@ -354,31 +381,31 @@ int gemm_bf16x3_t(hipStream_t s, int M, int N, const Split3& A, const Split3& B,
         return 1;
     }
     if (stream_capturing(s)) {  // no timing inside a graph capture
-        launch_gm(s, 8, M, N, A, B, C, ldc, tm, tn);
+        launch_gm(s, CAND_DEFAULT, M, N, A, B, C, ldc, tm, tn);
         return 1;
     }
     hipEvent_t ev[2] = {nullptr, nullptr};
     QK_HIP(hipEventCreate(&ev[0]));
     QK_HIP(hipEventCreate(&ev[1]));
-    std::array<float, NGM> ms{};
-    int best = 8;
+    std::array<float, NCAND> ms{};
+    int best = CAND_DEFAULT;
     float tbest = 1e30f;
-    for (int c = 0; c < NGM; ++c) {
+    for (int c = 0; c < NCAND; ++c) {
         (void)hipEventRecord(ev[0], s);
-        launch_gm(s, GM_CAND[c], M, N, A, B, C, ldc, tm, tn);
+        launch_gm(s, c, M, N, A, B, C, ldc, tm, tn);
         (void)hipEventRecord(ev[1], s);
         float t = 1e30f;
         if (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&t, ev[0], ev[1]) != hipSuccess) t = 1e30f;
         ms[c] = t;
         if (t < tbest) {
             tbest = t;
-            best = GM_CAND[c];
+            best = c;
         }
     }
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     g_gm.emplace(key, std::make_pair(best, ms));
-    return NGM;
+    return NCAND;
 }
 }  // namespace
 
@@ -390,17 +417,24 @@ int gemm_bf16x3(hipStream_t s, int M, int N, const Split3& A, const Split3& B, f
     return gemm_bf16x3_t(s, M, N, A, B, C, ldc);
 }
 
+void autotune_reset_bf16() {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_gm.clear();
+}
+
 std::string autotune_report_bf16() {
     std::lock_guard<std::mutex> lock(g_mu);
     std::string r;
     for (const auto& kv : g_gm) {
         char buf[256];
-        std::snprintf(buf, sizeof buf, "gemm_bf16x3 M=%d N=%d kpad=%d out=f%d slices=%d -> group %d (ms:",
+        const int best = kv.second.first;
+        std::snprintf(buf, sizeof buf,
+                      "gemm_bf16x3 M=%d N=%d kpad=%d out=f%d slices=%d -> group %d, %d-row waves (ms:",
                       std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), 8 * std::get<3>(kv.first),
-                      std::get<4>(kv.first), kv.second.first);
+                      std::get<4>(kv.first), GM_CAND[best % NGM], best < NGM ? 64 : 32);
         r += buf;
-        for (int c = 0; c < NGM; ++c) {
-            std::snprintf(buf, sizeof buf, " g%d %.3f", GM_CAND[c], kv.second.second[c]);
+        for (int c = 0; c < NCAND; ++c) {
+            std::snprintf(buf, sizeof buf, " g%d/w%d %.3f", GM_CAND[c % NGM], c < NGM ? 64 : 32, kv.second.second[c]);
             r += buf;
         }
         r += ")\n";

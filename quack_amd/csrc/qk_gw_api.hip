@@ -1800,6 +1800,12 @@ int qk_set_autotune(int on) {
     return QK_OK;
 }
 
+int qk_autotune_reset(void) {
+    qk::autotune_reset();
+    qk::autotune_reset_bf16();
+    return QK_OK;
+}
+
 int qk_set_library_gemm(int on) {
     qk::set_library_gemm(on != 0);
     return QK_OK;

@@ -30,7 +30,7 @@ EXPORTS = ["qk_gw_create", "qk_gw_destroy", "qk_gw_last_error", "qk_gw_set_eri_a
            "qk_mlp_last_error", "qk_gw_screen", "qk_gw_sigma_at", "qk_gw_set_eigensolver", "qk_gw_eig_stats",
            "qk_gw_set_eri_df", "qk_gw_set_eri_df_device", "qk_gwb_create", "qk_gwb_destroy", "qk_gwb_last_error",
            "qk_gwb_set_precision", "qk_gwb_set_eri_ao_device", "qk_gwb_set_eri_mo", "qk_gwb_g0w0",
-           "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
+           "qk_gwb_stage_times", "qk_gwb_evgw", "qk_set_autotune", "qk_autotune_report", "qk_autotune_reset", "qk_set_graphs", "qk_gw_set_eri_ao_packed", "qk_int_eri_device",
            "qk_rgw_phbse", "qk_gw_sigma_full", "qk_ug0w0", "qk_evugw", "qk_rhf_create", "qk_rhf_destroy", "qk_rhf_last_error", "qk_rhf_set_eri_ao", "qk_rhf_jk", "qk_rhf_jk_ms",
            "qk_rgw_qp_graph_noalloc", "qk_noalloc_last_error", "qk_gw_qp_iters",
            "qk_set_unrestricted_eigensolver", "qk_unrestricted_eig_stats", "qk_gwb_set_graphs", "qk_gwb_graph_stats", "qk_set_split_aotomo",
@@ -109,6 +109,7 @@ def load():
     lib.qk_gwb_stage_times.argtypes = [_H, _P, _I]
     lib.qk_gwb_evgw.argtypes = [_H, _P, _P, _I, _D, _I, _D, _I, _I, _I, _P, _P, _P, _P, _P, _P, _IP]
     lib.qk_set_autotune.argtypes = [_I]
+    lib.qk_autotune_reset.argtypes = []
     lib.qk_set_graphs.argtypes = [_I]
     lib.qk_autotune_report.argtypes = [ctypes.c_char_p, _I]
     lib.qk_gw_set_eri_df_device.argtypes = [_H, _H, _I]
@@ -344,6 +345,12 @@ def set_graphs(on):
     """Process-wide hipGraph replay of the own full eigensolve (off by default;
     identical results either way)."""
     load().qk_set_graphs(int(bool(on)))
+
+
+def autotune_reset():
+    """Forget every tuned GEMM shape (qk_autotune_reset): the next large call
+    of each shape tunes again."""
+    _check(load().qk_autotune_reset())
 
 
 def autotune_report():

@@ -159,6 +159,41 @@ def test_autotuner_identical_results():
     assert any(l.startswith("dgemm") for l in rep) and any(l.startswith("gemm_bf16x3") for l in rep)
 
 
+def test_split_gemm_wave_layouts_identical():
+    """The split-precision GEMM's two wave layouts (64- and 32-row waves) give
+    bit-identical results.  After qk_autotune_reset the first call of each
+    large shape tunes, and its output is the last candidate's (32-row waves,
+    group 16); the run with the tuner off uses the default (64-row waves,
+    group 8) throughout.  Both split forms (three and two slices), FP32 and FP64
+    stores (the excitation densities and the DF (ia|jb) product)."""
+    import torch
+
+    from quack_amd import dfgen, gw
+    say("split gemm wave layouts")
+    n, nO, naux = 240, 24, 720
+    e, Bt, _ = dfgen.synthetic_df(n, nO, naux, seed=6, device="cuda")
+    out = {}
+    try:
+        with gw.GWEngine(n, n, nO) as g:
+            for form in ("mixed", "mixed2"):
+                g.set_precision(form)
+                for tuned in (True, False):
+                    gw.set_autotune(tuned)
+                    if tuned:
+                        gw.autotune_reset()
+                    g.set_eri_df_device(Bt.data_ptr(), naux)
+                    out[(form, tuned)] = g.g0w0(e, doSRG=True)
+    finally:
+        gw.set_autotune(True)
+    torch.cuda.synchronize()
+    for form in ("mixed", "mixed2"):
+        for k in ("eGW", "Om", "SigC"):
+            assert np.array_equal(out[(form, False)][k], out[(form, True)][k]), (form, k)
+    rep = [l for l in gw.autotune_report() if l.startswith("gemm_bf16x3")]
+    say(*rep)
+    assert rep and all("w32" in l for l in rep)
+
+
 def test_library_gemm_candidate():
     """rocBLAS's dgemm as an autotuner candidate (qk_set_library_gemm): the
     large plain FP64 products of the DF path (nS = 5184) time it beside the
