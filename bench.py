@@ -882,17 +882,19 @@ def read_trd_traffic():
 
 def read_traffic(form):
     """HBM bytes per launch of the excitation GEMM from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor):
+    the form's last row, which is the kernel layout the bench now runs."""
     f = os.path.join(ROOT, "profiles", "pmc_excitation_gemm_h50_lattice.md")
     if not os.path.exists(f):
         return None
+    val = None
     for line in open(f):
         if line.startswith("| %s |" % form):
             try:
-                return float(line.split("|")[-2])
+                val = float(line.split("|")[-2])
             except ValueError:
-                return None
-    return None
+                pass
+    return val
 
 
 UGW_SAMPLE = r"""
