@@ -680,24 +680,54 @@ def run_df600(args):
         acc[form] = {"max_abs_dE_meV_all": float(d.max() / mev), "orbitals_over_1meV": int((d > mev).sum()),
                      "max_abs_dOm_Ha": float(dom.max()), "HOMO_dE_meV": float(d[nO - 1] / mev),
                      "LUMO_dE_meV": float(d[nO] / mev)}
+    report = gw.autotune_report()
     for form, peak, mult in (("fp64", PEAK_FP64, 1.0), ("mixed", PEAK_BF16, 6.0), ("mixed2", PEAK_BF16, 3.0)):
         r, dt, st, t_k = res[form]
-        ms = st["excitation_gemm"] / max(1.0, st["excitation_gemm_launches"])
+        # the rho product is the largest GEMM of the DF path; its first call tuned
+        # it, so the kept candidate's time is the smallest of the tuner's HIP-event
+        # timings of that shape (the call's stage time also holds every trial)
+        ms = tuned_gemm_ms(report, form)
+        if ms is None:
+            ms = st["excitation_gemm"] / max(1.0, st["excitation_gemm_launches"])
         ach = mult * flops_rho / (ms * 1e-3) / 1e12
         forms[form] = {"g0w0_s": dt, "ia_jb_s": t_k, "stages_ms": st, "rho_gemm_ms": ms,
                        "rho_gemm_roofline": {"achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
                                              "flops_per_launch": mult * flops_rho}}
     out = {"metric": "precision form: split-precision vs FP64 contractions on the synthetic DF system",
-           "value": res["fp64"][2]["excitation_gemm"] / max(1e-9, res["mixed"][2]["excitation_gemm"]),
+           "value": forms["fp64"]["rho_gemm_ms"] / forms["mixed"]["rho_gemm_ms"],
            "unit": "x (rho GEMM speed-up, mixed over fp64)", "n_gpus": 1, "higher_is_better": True,
            "dtype": "f64 vs bf16x3-slices/f32-acc/f64", "data": "synthetic: seeded DF factors (dfgen)",
-           "mixed2_rho_gemm_speedup_vs_fp64": res["fp64"][2]["excitation_gemm"] / max(1e-9, res["mixed2"][2]["excitation_gemm"]),
+           "mixed2_rho_gemm_speedup_vs_fp64": forms["fp64"]["rho_gemm_ms"] / forms["mixed2"]["rho_gemm_ms"],
+           "rho_gemm_timing": "the kept autotuner candidate's HIP-event time for the largest GEMM shape of each form",
            "config": {"workload": "synthetic DF N=600 o=60 naux=1800 SRG-G0W0", "nOrb": n, "nO": nO, "nS": nS,
                       "naux": naux, "decay": 40.0, "kmax_Ha": 0.1, "scale": scale, "seed": args.seed},
            "forms": forms,
            "accuracy_mixed_vs_fp64": acc["mixed"], "accuracy_mixed2_vs_fp64": acc["mixed2"]}
     print(json.dumps(out), flush=True)
     g.close()
+
+
+def tuned_gemm_ms(report, form):
+    """Time (ms) of the kept candidate for the largest GEMM shape of a form in
+    the autotuner report: split forms ("mixed": three slices, "mixed2": two)
+    from the gemm_bf16x3 lines, "fp64" from the dgemm lines (M N K x batch)."""
+    import re
+    best = None
+    for line in report:
+        if form == "fp64" and line.startswith("dgemm"):
+            m = re.search(r"M=(\d+) N=(\d+) K=(\d+) batch=(\d+)", line)
+            size = int(m.group(1)) * int(m.group(2)) * int(m.group(3)) * int(m.group(4)) if m else 0
+        elif form != "fp64" and line.startswith("gemm_bf16x3"):
+            m = re.search(r"M=(\d+) N=(\d+) kpad=(\d+) out=f\d+ slices=(\d)", line)
+            if not m or int(m.group(4)) != (3 if form == "mixed" else 2):
+                continue
+            size = int(m.group(1)) * int(m.group(2)) * int(m.group(3))
+        else:
+            continue
+        times = [float(x) for x in re.findall(r" ([0-9.]+)(?= |\)$)", line.split("(ms:", 1)[1])]
+        if times and (best is None or size > best[0]):
+            best = (size, min(times))
+    return best[1] if best else None
 
 
 def run_batch(args):
