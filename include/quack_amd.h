@@ -244,7 +244,9 @@ const char *qk_noalloc_last_error(void);
  * (physicist, column-major; in ERI_aabb the first and third indices are
  * alpha).  nC[2], nO[2], nR[2]: frozen core, occupied and frozen virtual
  * counts per spin (alpha, beta), as QuAcK's nC(nspin), nO(nspin), nR(nspin);
- * per spin 0 <= nC < nO < nBas - nR, nS_s = (nO - nC)(nBas - nR - nO).
+ * per spin 0 <= nC <= nO <= nBas - nR and nC + nR < nBas, nS_s = (nO - nC)(nBas -
+ * nR - nO); one spin may have no pair (an empty beta spin, say), but
+ * nS_a + nS_b >= 1.
  * eHF, eGW, SigC, Z: nBas x 2 (alpha column, then beta); frozen entries of
  * eGW are eHF (QuAcK leaves them undefined), SigC 0 and Z 1 there.  Om: nSa +
  * nSb.  EcGM[2]: per spin (QuAcK's SRG routine returns their sum). */

@@ -745,8 +745,9 @@ def ugw_excitation_density(n, eri_aa, eri_ab, eri_bb, XpY, nO, nC=(0, 0), nR=(0,
     :86-87), the others are never read."""
     Ga, Gm, Gb = G4(eri_aa, n), G4(eri_ab, n), G4(eri_bb, n)
     oa, va, ob, vb, nSa, nSb = _ublocks(n, nO, nC, nR)
-    Xa = XpY[:, :nSa].reshape(-1, oa.stop - oa.start, va.stop - va.start)
-    Xb = XpY[:, nSa:].reshape(-1, ob.stop - ob.start, vb.stop - vb.start)
+    nm = XpY.shape[0]  # explicit: a spin may have no active pairs
+    Xa = XpY[:, :nSa].reshape(nm, oa.stop - oa.start, va.stop - va.start)
+    Xb = XpY[:, nSa:].reshape(nm, ob.stop - ob.start, vb.stop - vb.start)
     rho = np.zeros((n, n, nSa + nSb, 2))
     # alpha: sum_jb(a) <pj|qb>_aaaa X(m, jb) + sum_jb(b) <pj|qb>_aabb X(m, nSa + jb)
     rho[:, :, :, 0] = np.einsum("pjqb,mjb->pqm", Ga[:, oa, :, va], Xa, optimize=True) \

@@ -108,11 +108,18 @@ struct HostRho {
 
 // Orbital counts of qk_ug0w0 / qk_evugw: per spin 0 <= nC < nO < nBas - nR
 // (at least one active occupied and one active virtual orbital).
+// per spin 0 <= nC <= nO <= nBas - nR with at least one active orbital
+// (nC + nR < nBas); a spin may have no active pair (no active occupied or no
+// active virtual orbital, e.g. the empty beta spin of a one-electron system),
+// as QuAcK's loops allow, but both spins together need one
 bool upairs_ok(int nBas, const int* nC, const int* nO, const int* nR) {
     if (nBas <= 0 || !nC || !nO || !nR) return false;
-    for (int s = 0; s < 2; ++s)
-        if (nC[s] < 0 || nR[s] < 0 || nO[s] <= nC[s] || nO[s] >= nBas - nR[s]) return false;
-    return true;
+    long long pairs = 0;
+    for (int s = 0; s < 2; ++s) {
+        if (nC[s] < 0 || nR[s] < 0 || nO[s] < nC[s] || nO[s] > nBas - nR[s] || nC[s] + nR[s] >= nBas) return false;
+        pairs += (long long)(nO[s] - nC[s]) * (nBas - nR[s] - nO[s]);
+    }
+    return pairs > 0;
 }
 
 // Eigensolver of the unrestricted screening (qk_set_unrestricted_eigensolver),
@@ -283,6 +290,7 @@ struct UScreen {
                                      int qy0, int Y2) {
         const size_t nx = (size_t)X1 * X2, ny = (size_t)Y1 * Y2;
         std::vector<double> h(nx * ny);
+        if (nx == 0 || ny == 0) return h;  // a spin block without pairs
         double *X = nullptr, *Y = nullptr, *G = nullptr;
         try {
             X = dalloc<double>((size_t)nSt * nx);
@@ -454,6 +462,7 @@ int qk_ugw_phbse(int nBas, const int* nC, const int* nO, const int* nR, const do
             const int sv[2] = {ispin == 1 ? 0 : 1, ispin == 1 ? 1 : 0};
             const int nb0 = o[so[0]] * v[sv[0]], nb1 = o[so[1]] * v[sv[1]], N = nb0 + nb1;
             const int off[2] = {0, nb0};
+            if (N == 0) continue;  // no pair in this manifold: EcBSE stays 0
             std::vector<double> A((size_t)N * N, 0.0), B((size_t)N * N, 0.0), XpY((size_t)N * N), XmY((size_t)N * N);
             for (int bI = 0; bI < 2; ++bI) {
                 const int si = so[bI], sa = sv[bI], oi = o[si], va = v[sa];

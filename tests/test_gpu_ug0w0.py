@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import gw_oracle as go
+from tests.cases import EMPTY_SPIN, empty_spin_case
 
 pytestmark = pytest.mark.gpu
 
@@ -175,14 +176,49 @@ def test_unrestricted_phbse_closed_shell_is_restricted(mo_systems):
     assert abs(u["EcBSE"].sum() - r["EcBSE"].sum()) < 1e-8
 
 
+@pytest.mark.parametrize("case", EMPTY_SPIN)
+def test_unrestricted_empty_spin_matches_oracle(mo_systems, case):
+    """A spin without active pairs (the empty beta spin of an open shell, or
+    every occupied orbital of one spin frozen): G0W0, evGW and the static BSE
+    within 1e-8 Ha of the oracle, itself pinned to QuAcK's compiled routines
+    for these cases (tests/test_oracle.py::test_unrestricted_empty_spin_vs_compiled_reference).
+    The open-shell occupations take the TDA BSE (their full A - B is
+    indefinite on closed-shell orbitals)."""
+    from quack_amd import gw
+    s = mo_systems["H2O"]
+    mo, nO = s["eri_mo"], s["nO"]
+    n = len(s["eHF"])
+    occ, nC, nR = empty_spin_case(nO, case)
+    e2 = np.stack([s["eHF"], s["eHF"] + 0.01 * np.sin(np.arange(n))], axis=1)
+    for kw in ({"doSRG": True}, {"eta": 0.05}):
+        print("unrestricted empty spin", case, kw, flush=True)
+        r = gw.UG0W0(mo, mo, mo, e2, occ, nC=nC, nR=nR, **kw)
+        o = go.ug0w0(mo, mo, mo, e2, occ, nC=nC, nR=nR, **kw)
+        for k in range(2):
+            frozen = np.r_[np.arange(nC[k]), np.arange(n - nR[k], n)]
+            o["eGW"][frozen, k] = e2[frozen, k]
+        assert np.abs(r["eGW"] - o["eGW"]).max() < 1e-8 and abs(r["EcRPA"] - o["EcRPA"]) < 1e-8, kw
+    r = gw.evUGW(mo, mo, mo, e2, occ, nC=nC, nR=nR, doSRG=True)
+    o = go.evugw(mo, mo, mo, e2, occ, nC=nC, nR=nR, doSRG=True)
+    assert r["converged"] and r["nSCF"] == o["nSCF"]
+    assert np.abs(r["eGW"] - o["eGW"]).max() < 1e-8
+    eG = go.ug0w0(mo, mo, mo, e2, occ, nC=nC, nR=nR, eta=0.05)["eGW"]
+    for kw in (({}, {"TDA": True}) if occ[0] == occ[1] else ({"TDA": True}, {"TDA": True, "TDA_W": True})):
+        g = gw.UGW_phBSE(mo, mo, mo, e2, eG, occ, eta=0.05, nC=nC, nR=nR, **kw)
+        Ec, OmSC, OmSF = go.ugw_phbse(mo, mo, mo, e2, eG, occ, eta=0.05, nC=nC, nR=nR, **kw)
+        assert np.abs(g["EcBSE"] - Ec).max() < 1e-8, kw
+        assert np.abs(g["OmSC"] - OmSC).max() < 1e-8 and np.abs(g["OmSF"] - OmSF).max() < 1e-8, kw
+
+
 def test_unrestricted_frozen_argument_errors():
-    """Orbital counts outside 0 <= nC < nO < nBas - nR in either spin are
-    rejected before any device work."""
+    """Orbital counts outside 0 <= nC <= nO <= nBas - nR in either spin, or
+    with no active pair in either spin, are rejected before any device work."""
     from quack_amd import gw
     n = 6
     z = np.zeros(n ** 4)
     e2 = np.zeros((n, 2))
-    for nC, nO, nR in (((3, 0), (3, 3), (0, 0)), ((0, 0), (3, 3), (0, 3)), ((-1, 0), (3, 3), (0, 0))):
+    for nC, nO, nR in (((4, 0), (3, 3), (0, 0)), ((0, 0), (3, 3), (0, 4)), ((-1, 0), (3, 3), (0, 0)),
+                       ((3, 3), (3, 3), (0, 0))):
         with pytest.raises(gw.QKError):
             gw.UG0W0(z, z, z, e2, nO, nC=nC, nR=nR)
         with pytest.raises(gw.QKError):

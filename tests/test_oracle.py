@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from oracle import gw_oracle as go
+from tests.cases import EMPTY_SPIN, empty_spin_case
 from oracle import qkref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -581,6 +582,40 @@ def test_unrestricted_phbse_vs_compiled_reference(uhf_h2o):
                                          TDA=kw.get("tda", False), TDA_W=kw.get("tda_w", False), nC=nC, nR=nR)
             assert np.abs(OmSC - r["OmSC"]).max() < 1e-10 and np.abs(OmSF - r["OmSF"]).max() < 1e-10, (nC, nR, kw)
             assert np.abs(Ec - r["EcBSE"]).max() < 1e-10, (nC, nR, kw)
+
+
+@pytest.mark.skipif(not qkref.available(), reason="oracle/_ref not built (needs the QuAcK sources)")
+@pytest.mark.parametrize("case", EMPTY_SPIN)
+def test_unrestricted_empty_spin_vs_compiled_reference(mo_systems, case):
+    """A spin with no active pair (QuAcK's loops run empty there): UG0W0,
+    evUGW and the static unrestricted BSE of the restatement against QuAcK's
+    compiled routines on H2O integrals.  Open-shell occupations on closed-shell
+    orbitals leave the full BSE's A - B indefinite, so those take TDA."""
+    s = mo_systems["H2O"]
+    mo, nO = s["eri_mo"], s["nO"]
+    n = len(s["eHF"])
+    occ, nC, nR = empty_spin_case(nO, case)
+    e2 = np.stack([s["eHF"], s["eHF"] + 0.01 * np.sin(np.arange(n))], axis=1)
+    active = [np.r_[nC[k]:n - nR[k]] for k in range(2)]
+    for kw, qkw in (({"doSRG": True}, {"srg": True}), ({"eta": 0.05}, {"eta": 0.05})):
+        o = go.ug0w0(mo, mo, mo, e2, occ, nC=nC, nR=nR, **kw)
+        r = qkref.ug0w0(mo, mo, mo, e2, occ, nC=nC, nR=nR, **qkw)
+        for k in range(2):
+            assert np.abs(o["eGW"][active[k], k] - r["eGW"][active[k], k]).max() < 1e-10, (case, kw)
+        assert abs(o["EcRPA"] - r["EcRPA"]) < 1e-10
+    o = go.evugw(mo, mo, mo, e2, occ, nC=nC, nR=nR, doSRG=True)
+    r = qkref.evugw(mo, mo, mo, e2, occ, nC=nC, nR=nR, srg=True)
+    assert o["nSCF"] == r["nSCF"]
+    for k in range(2):
+        assert np.abs(o["eGW"][active[k], k] - r["eGW"][active[k], k]).max() < 1e-10
+    eG = go.ug0w0(mo, mo, mo, e2, occ, nC=nC, nR=nR, eta=0.05)["eGW"]
+    closed = occ[0] == occ[1]
+    for kw in (({}, {"TDA": True}) if closed else ({"TDA": True}, {"TDA": True, "TDA_W": True})):
+        Ec, OmSC, OmSF = go.ugw_phbse(mo, mo, mo, e2, eG, occ, eta=0.05, nC=nC, nR=nR, **kw)
+        r = qkref.uphbse(mo, mo, mo, e2, eG, occ, eta=0.05, nC=nC, nR=nR, tda=kw.get("TDA", False),
+                         tda_w=kw.get("TDA_W", False))
+        assert np.abs(Ec - r["EcBSE"]).max() < 1e-10, (case, kw)
+        assert np.abs(OmSC - r["OmSC"]).max() < 1e-10 and np.abs(OmSF - r["OmSF"]).max() < 1e-10
 
 
 def test_unrestricted_phbse_closed_shell_reduction(mo_systems):
