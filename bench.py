@@ -35,7 +35,7 @@ run through the library's HIP kernels on the device.
     python bench.py [--steps K] [--warmup W] [--form fp64|mixed|mixed2] [--eig syevd|refine|sytrd|refine+sytrd]
                     [--concurrent C] [--method evgw|g0w0] [--reg eta|srg|none] [--eta ETA] [--max-diis D]
                     [--learned on|off] [--cpu-baseline auto|off] [--lib-gemm on|off]
-                    [--workload h50|df600|batch|learned|ugw|cpu250|bse|qsgw]
+                    [--workload h50|df600|batch|learned|ugw|cpu250|bse|qsgw|ubse]
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -331,14 +331,15 @@ def main():
     ap.add_argument("--cpu-baseline", choices=["auto", "off"], default="auto")
     ap.add_argument("--ugw-atoms", type=int, default=10, help="ugw workload: H chain length (one more alpha electron)")
     ap.add_argument("--wide-atoms", type=int, default=20, help="bse and qsgw workloads: H chain length")
-    ap.add_argument("--workload", choices=["h50", "df600", "batch", "learned", "ugw", "cpu250", "bse", "qsgw"],
+    ap.add_argument("--workload", choices=["h50", "df600", "batch", "learned", "ugw", "cpu250", "bse", "qsgw", "ubse"],
                     default="h50",
                     help="h50: the headline N=250 lattices; df600: the precision-form measurement on the synthetic "
                          "density-fitted system (N=600, o=60, naux=1800), FP64 against split precision; batch: "
                          "batched G0W0 over seeded H chains/lattices of uneven size; learned: the MLP warm start "
                          "against plain evGW on held-out systems; cpu250: QuAcK's compiled path on the host cores "
-                         "at N=250 (AO->MO + one evGW cycle); bse: static phBSE@G0W0 and qsgw: one qsGW cycle, "
-                         "each against QuAcK's compiled routines on an H chain (--wide-atoms)")
+                         "at N=250 (AO->MO + one evGW cycle); bse: static phBSE@G0W0, ubse: its unrestricted form "
+                         "(spin-conserved and spin-flip, closed-shell occupations) and qsgw: one qsGW cycle, each "
+                         "against QuAcK's compiled routines on an H chain (--wide-atoms)")
     ap.add_argument("--nmol", type=int, default=64, help="batch and learned workloads: number of molecules")
     ap.add_argument("--kmax", type=int, default=12, help="batch workload: largest H_2k (N = 10k)")
     ap.add_argument("--chunk", type=int, default=256, help="batch workload: most molecules resident at once")
@@ -361,7 +362,7 @@ def main():
         return run_learned(args)
     if args.workload == "cpu250":
         return run_cpu250(args)
-    if args.workload in ("bse", "qsgw"):
+    if args.workload in ("bse", "qsgw", "ubse"):
         return run_widened(args)
     return run_h50(args)
 
@@ -1035,6 +1036,12 @@ if work == "bse":
     eGW = qkref.rg0w0(mo, s["eHF"], nO, eta=0.05)["eGW"]
     r, t, reps = timed(lambda: qkref.phbse(mo, s["eHF"], eGW, nO, eta=0.05))
     np.savez(out, mo=mo, eHF=s["eHF"], eGW=eGW, nO=nO, Ec=r["EcBSE"], OmS=r["OmS"], OmT=r["OmT"], t=t, reps=reps)
+elif work == "ubse":
+    mo = qkref.aotomo(s["c"], s["eri"])
+    e2 = np.stack([s["eHF"], s["eHF"]], axis=1)
+    eGW = qkref.ug0w0(mo, mo, mo, e2, (nO, nO), eta=0.05)["eGW"]
+    r, t, reps = timed(lambda: qkref.uphbse(mo, mo, mo, e2, eGW, (nO, nO), eta=0.05))
+    np.savez(out, mo=mo, e2=e2, eGW=eGW, nO=nO, Ec=r["EcBSE"], OmS=r["OmSC"], OmT=r["OmSF"], t=t, reps=reps)
 else:
     S, T, V = mol.one_electron()
     w, U = np.linalg.eigh(S)
@@ -1088,6 +1095,20 @@ def run_widened(args):
                   float(np.abs(r["EcBSE"] - d["Ec"]).max()))
         metric, what = "static phBSE@G0W0 calculations per second", "phBSE"
         extra = {"max_abs_dOm_dEc_vs_reference_Ha": err}
+    elif args.workload == "ubse":
+        mo, e2, eGW = d["mo"], d["e2"], d["eGW"]
+        n = e2.shape[0]
+        for k in range(max(args.warmup, 1) + reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = gw.UGW_phBSE(mo, mo, mo, e2, eGW, (nO, nO), eta=0.05)
+            torch.cuda.synchronize()
+            if k >= max(args.warmup, 1):
+                ts.append(time.perf_counter() - t0)
+        err = max(float(np.abs(r["OmSC"] - d["OmS"]).max()), float(np.abs(r["OmSF"] - d["OmT"]).max()),
+                  float(np.abs(r["EcBSE"] - d["Ec"]).max()))
+        metric, what = "unrestricted static phBSE@G0W0 calculations per second", "UGW phBSE"
+        extra = {"max_abs_dOm_dEc_vs_reference_Ha": err}
     else:
         S, T, V, X, c, eHF, eri = (d[k] for k in ("S", "T", "V", "X", "c", "eHF", "eri"))
         n = len(eHF)
@@ -1111,7 +1132,8 @@ def run_widened(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "%s: H%d chain (1.8 bohr spacing, seed 0) cc-pVDZ, nBas %d, nO %d, nS %d%s" % (
             args.workload, args.wide_atoms, n, nO, nO * (n - nO),
-            ", eta 0.05, W and G from G0W0 at eta 0.05" if args.workload == "bse" else ", SRG, from RHF")},
+            ", SRG, from RHF" if args.workload == "qsgw" else ", eta 0.05, W and G from G0W0 at eta 0.05" +
+            (", both spins from the RHF orbitals" if args.workload == "ubse" else ""))},
         **extra,
         "cpu_baseline": {"value": 1.0 / tr, "unit": "1/s", "cores": sample_threads(), "host": host_cpu(),
                          "kind": "reference", "sample": "the same calculation, QuAcK's compiled routines, median of "
