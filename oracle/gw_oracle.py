@@ -1,4 +1,5 @@
-"""TEST INFRASTRUCTURE ONLY: CPU restatement of QuAcK's restricted G0W0/evGW.
+"""TEST INFRASTRUCTURE ONLY: CPU restatement of QuAcK's G0W0/evGW (restricted
+and unrestricted), phBSE@GW and qsGW.
 
 This module is the checker for the HIP path.  Only tests/, the smoke check
 and the CPU-baseline leg of bench.py import it; the product library never
@@ -16,6 +17,9 @@ conventions:
 dsyev('V','U') of utils/wrap_lapack.f90:99-136 is called through SciPy's
 LAPACK binding with the same driver and triangle; dsysvx of
 wrap_lapack.f90:560-598 likewise.
+
+Unrestricted functions take (alpha, beta) pairs of counts and n x 2 energies
+and index spin-blocked pairs as LR/phULR_A.f90 (alpha pairs first).
 
 Pinning: tests/test_oracle.py checks each function against the compiled
 QuAcK routines (oracle/_ref), the FeatherBench / Rtest_ref.dat goldens

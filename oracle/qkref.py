@@ -1,8 +1,9 @@
 """TEST INFRASTRUCTURE ONLY: ctypes view of oracle/_ref/libqkref.so.
 
-libqkref.so is QuAcK's own restricted G0W0/evGW Fortran (GW/, LR/, utils/,
-AOtoMO/ sources compiled unchanged by oracle/Makefile) plus the bind(C)
-entry points of oracle/ref_driver.f90.  Only tests, the smoke check and the
+libqkref.so is QuAcK's own Fortran for the restricted and unrestricted
+G0W0/evGW paths, phBSE and the qsGW cycle (GW/, LR/, utils/, AOtoMO/ sources
+compiled unchanged by oracle/Makefile) plus the bind(C) entry points of
+oracle/ref_driver.f90, which compose them in QuAcK's order.  Only tests, the smoke check and the
 CPU-baseline leg of bench.py use it, and only as the checker.
 """
 import ctypes
