@@ -1,5 +1,7 @@
 /*
- * quack_amd: C-ABI of the MI355X (gfx950) restricted G0W0/evGW engine.
+ * quack_amd: C-ABI of the MI355X (gfx950) G0W0/evGW engine: QuAcK's restricted
+ * path, and beside it phBSE@GW, the qsGW cycle, the unrestricted G0W0/evGW
+ * and static phBSE, the RHF Fock builds and the device integral producer.
  *
  * Conventions (identical to QuAcK's Fortran, so a bind(C) interface passes
  * its arrays straight through):
@@ -10,7 +12,10 @@
  *   - particle-hole index ia = (a-nO) + (i-nC-1)*(nBas-nR-nO) with 1-based
  *     i, a (LR/phRLR_A.f90:151), a fastest;
  *   - XpY(m, ia): row = RPA mode m, column = ph pair (LR/phRLR.f90:77-78);
- *   - rho(p, q, m) (GW/RGW_excitation_density.f90:24).
+ *   - rho(p, q, m) (GW/RGW_excitation_density.f90:24);
+ *   - unrestricted entries take per-spin counts nC[2], nO[2], nR[2] (alpha,
+ *     beta) and nBas x 2 energies; their pairs are the alpha pairs, then the
+ *     beta pairs, each indexed as above (LR/phULR_A.f90).
  * Every entry point returns QK_OK (0) or one of the QK_E_* codes below; a
  * message for the last failure is kept per handle (qk_gw_last_error) and per
  * thread for the handle-less entry points (qk_last_error).
